@@ -1,0 +1,257 @@
+"""h2weak_amd -- MI355X-native H2* / (H2+H)* fast matrix-vector product and
+GMRES driver, a drop-in for RiteshKhan/H2weak's operator facade.
+
+The compute path is libh2w.so (hand-written HIP kernels for gfx950 behind the
+C ABI in include/h2w.h).  This module is a thin ctypes mirror of the
+reference's host interface:
+
+  * ``Initialisation`` mirrors ``class initialisation``
+    ({1,2,3}D/codes/matvec.hpp): constructor, ``tree_initialisation()``,
+    ``mat_vec_prod(charge, output)``;
+  * ``IterativeSolver`` mirrors ``class iterative_solver``
+    (1D/codes/gmres.hpp:14-201): ``GMRES(A, x, b)`` with ``resid`` and
+    ``max_iter`` (the iteration count) set on return.
+
+There is no CPU fallback: importing works without a GPU, but constructing an
+operator raises ``H2WError`` when libh2w.so is missing or no GPU is visible.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libh2w.so")
+
+H2W_OK, H2W_EINVAL, H2W_ESTATE, H2W_EHIP, H2W_ENOMEM, H2W_ENOCONV, H2W_ENOLIB = range(7)
+NESTED, SEMI_NESTED = 0, 1
+CHOICE_IE, CHOICE_RBF = 0, 1
+HOST, DEVICE = 0, 1
+FP64, FP32 = 0, 1
+ORDER_INPUT, ORDER_TREE = 0, 1
+GMRES_FP64, GMRES_FP32_IR = 0, 1
+
+# every symbol include/h2w.h declares
+EXPORTS = ("h2w_default_opts", "h2w_create", "h2w_assemble", "h2w_matvec", "h2w_matvec_ex", "h2w_gmres",
+           "h2w_dense_matvec", "h2w_get_stats", "h2w_tree_order", "h2w_box_skeleton", "h2w_interaction_list",
+           "h2w_near_list", "h2w_destroy", "h2w_last_error", "h2w_bench_matvec")
+
+
+class H2WError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"h2w error {code}: {msg}")
+        self.code = code
+
+
+class Opts(ctypes.Structure):
+    _fields_ = [("compat", ctypes.c_int), ("nlevels", ctypes.c_int), ("fp32_shadow", ctypes.c_int),
+                ("device", ctypes.c_int), ("rank_cap", ctypes.c_int), ("workspace_bytes", ctypes.c_int64),
+                ("plan_only", ctypes.c_int)]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [("dim", ctypes.c_int), ("variant", ctypes.c_int), ("choice", ctypes.c_int), ("nlevels", ctypes.c_int),
+                ("N", ctypes.c_int64), ("memory", ctypes.c_double), ("diag", ctypes.c_double),
+                ("assembly_seconds", ctypes.c_double), ("max_rank_far", ctypes.c_int),
+                ("max_rank_vertex", ctypes.c_int), ("mean_rank_far", ctypes.c_double),
+                ("mean_rank_vertex", ctypes.c_double), ("stored_bytes_fp64", ctypes.c_int64),
+                ("stored_bytes_fp32", ctypes.c_int64), ("n_launches_matvec", ctypes.c_int),
+                ("aca_reruns", ctypes.c_int)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class GmresStats(ctypes.Structure):
+    _fields_ = [("iterations", ctypes.c_int), ("outer_iterations", ctypes.c_int), ("matvecs_fp64", ctypes.c_int),
+                ("matvecs_fp32", ctypes.c_int), ("resid", ctypes.c_double), ("seconds", ctypes.c_double),
+                ("converged", ctypes.c_int)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class BenchResult(ctypes.Structure):
+    _fields_ = [("ms_per_matvec", ctypes.c_double), ("launch_ms", ctypes.c_double * 64),
+                ("n_launch", ctypes.c_int), ("launch_bytes", ctypes.c_int64 * 64)]
+
+
+_lib = None
+
+
+def lib():
+    """Load libh2w.so (in-tree).  Raises H2WError if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise H2WError(H2W_ENOLIB, f"{LIB_PATH} is not built (run __graft_entry__.build())")
+    L = ctypes.CDLL(LIB_PATH)
+    p, i, i64, d = ctypes.c_void_p, ctypes.c_int, ctypes.c_int64, ctypes.c_double
+    L.h2w_default_opts.argtypes = [ctypes.POINTER(Opts)]
+    L.h2w_create.argtypes = [i, i, i64, p, i, d, i, i, ctypes.POINTER(Opts), ctypes.POINTER(p)]
+    L.h2w_assemble.argtypes = [p]
+    L.h2w_matvec.argtypes = [p, p, p, i]
+    L.h2w_matvec_ex.argtypes = [p, p, p, i, i64, i, i, i]
+    L.h2w_gmres.argtypes = [p, p, p, i, d, i, i, ctypes.POINTER(GmresStats)]
+    L.h2w_dense_matvec.argtypes = [p, p, p, p, i64, i]
+    L.h2w_get_stats.argtypes = [p, ctypes.POINTER(Stats)]
+    L.h2w_tree_order.argtypes = [p, p]
+    L.h2w_box_skeleton.argtypes = [p, i, i, i, p, p, i]
+    L.h2w_interaction_list.argtypes = [p, i, i, p, p, i]
+    L.h2w_near_list.argtypes = [p, i, i, p, i]
+    L.h2w_destroy.argtypes = [p]
+    L.h2w_last_error.restype = ctypes.c_char_p
+    L.h2w_bench_matvec.argtypes = [p, i, i, ctypes.POINTER(BenchResult)]
+    _lib = L
+    return L
+
+
+def _check(rc):
+    if rc != H2W_OK:
+        raise H2WError(rc, lib().h2w_last_error().decode())
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+class Initialisation:
+    """Mirror of the reference's ``initialisation`` facade ({1,2,3}D/codes/matvec.hpp).
+
+    dim, N, n_max, L, tol_pow (ACA/NCA tolerance 10^-tol_pow) and choice
+    (0 = integral equation, 1 = RBF) have the reference's meaning; ``variant``
+    picks H2* (NESTED) or (H2+H)* (SEMI_NESTED), which the reference selects
+    at compile time.  ``points`` (dim x N) defaults to the reference's grid.
+    """
+
+    def __init__(self, dim, N, n_max, L=1.0, tol_pow=12, choice=CHOICE_RBF, variant=NESTED, points=None,
+                 nlevels=0, compat=1, fp32_shadow=1, rank_cap=128, device=0, workspace_bytes=0,
+                 plan_only=0):
+        self.dim, self.N = int(dim), int(N)
+        o = Opts()
+        lib().h2w_default_opts(ctypes.byref(o))
+        o.compat, o.nlevels, o.fp32_shadow, o.device = compat, nlevels, fp32_shadow, device
+        o.rank_cap, o.workspace_bytes, o.plan_only = rank_cap, workspace_bytes, plan_only
+        self._pts = None
+        pp = None
+        if points is not None:
+            self._pts = _f64(np.asarray(points).reshape(dim, N))
+            pp = _ptr(self._pts)
+        h = ctypes.c_void_p()
+        _check(lib().h2w_create(dim, variant, int(N), pp, int(n_max), float(L), int(tol_pow), int(choice),
+                                ctypes.byref(o), ctypes.byref(h)))
+        self._h = h
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h and _lib is not None:
+            _lib.h2w_destroy(h)
+            self._h = None
+
+    def tree_initialisation(self):
+        _check(lib().h2w_assemble(self._h))
+
+    def mat_vec_prod(self, charge, output=None, precision=FP64):
+        """output = A charge (input order).  numpy arrays (host) or objects with
+        ``data_ptr()`` (device pointers on the handle's GPU)."""
+        if hasattr(charge, "data_ptr"):
+            _check(lib().h2w_matvec_ex(self._h, ctypes.c_void_p(charge.data_ptr()),
+                                       ctypes.c_void_p(output.data_ptr()), 1, self.N, precision, ORDER_INPUT, DEVICE))
+            return output
+        x = _f64(charge)
+        y = np.empty(self.N) if output is None else output
+        _check(lib().h2w_matvec_ex(self._h, _ptr(x), _ptr(y), 1, self.N, precision, ORDER_INPUT, HOST))
+        return y
+
+    def matvec_multi(self, X, precision=FP64):
+        X = _f64(X)
+        nrhs = X.shape[0]
+        Y = np.empty_like(X)
+        _check(lib().h2w_matvec_ex(self._h, _ptr(X), _ptr(Y), nrhs, self.N, precision, ORDER_INPUT, HOST))
+        return Y
+
+    def dense_rows(self, rows, x):
+        x = _f64(x)
+        rows = None if rows is None else np.ascontiguousarray(rows, dtype=np.int64)
+        n = self.N if rows is None else len(rows)
+        y = np.empty(n)
+        _check(lib().h2w_dense_matvec(self._h, _ptr(x), _ptr(y), None if rows is None else _ptr(rows), n, HOST))
+        return y
+
+    def gmres(self, b, x0=None, max_iter=500, tol=1e-12, mode=GMRES_FP64):
+        b = _f64(b)
+        x = np.ones(self.N) if x0 is None else np.array(x0, dtype=np.float64)
+        st = GmresStats()
+        rc = lib().h2w_gmres(self._h, _ptr(b), _ptr(x), int(max_iter), float(tol), mode, HOST, ctypes.byref(st))
+        if rc not in (H2W_OK, H2W_ENOCONV):
+            _check(rc)
+        return x, st.as_dict()
+
+    def stats(self):
+        st = Stats()
+        _check(lib().h2w_get_stats(self._h, ctypes.byref(st)))
+        return st.as_dict()
+
+    @property
+    def memory(self):
+        return self.stats()["memory"]
+
+    def tree_order(self):
+        p = np.empty(self.N, dtype=np.int32)
+        _check(lib().h2w_tree_order(self._h, _ptr(p)))
+        return p
+
+    def skeleton(self, family, level, box, cap=4096):
+        ir = np.empty(cap, dtype=np.int32)
+        ic = np.empty(cap, dtype=np.int32)
+        r = lib().h2w_box_skeleton(self._h, family, level, box, _ptr(ir), _ptr(ic), cap)
+        if r < 0:
+            _check(-r)
+        return ir[:r].tolist(), ic[:r].tolist()
+
+    def interaction_list(self, level, box):
+        il = np.empty(512, dtype=np.int32)
+        v = np.empty(512, dtype=np.int32)
+        n = lib().h2w_interaction_list(self._h, level, box, _ptr(il), _ptr(v), 512)
+        if n < 0:
+            _check(-n)
+        return il[:n].tolist(), v[:n].tolist()
+
+    def near_list(self, level, box):
+        nb = np.empty(64, dtype=np.int32)
+        n = lib().h2w_near_list(self._h, level, box, _ptr(nb), 64)
+        if n < 0:
+            _check(-n)
+        return nb[:n].tolist()
+
+    def bench_matvec(self, nrep=10, precision=FP64):
+        r = BenchResult()
+        _check(lib().h2w_bench_matvec(self._h, int(nrep), precision, ctypes.byref(r)))
+        n = r.n_launch
+        return {"ms_per_matvec": r.ms_per_matvec, "launch_ms": list(r.launch_ms[:n]),
+                "launch_bytes": list(r.launch_bytes[:n])}
+
+
+class IterativeSolver:
+    """Mirror of ``iterative_solver`` (1D/codes/gmres.hpp:14-201)."""
+
+    def __init__(self, max_iter, gmres_tol_pow, mode=GMRES_FP64):
+        self.restart = self.max_iter = int(max_iter)
+        self.gmres_tol = 10.0 ** (-gmres_tol_pow)
+        self.resid = 0.0
+        self.mode = mode
+        self.stats = None
+
+    def GMRES(self, A, x, b):
+        """Solves A x = b in place (x holds x0 on entry), as the reference's GMRES."""
+        xs, st = A.gmres(b, x, self.max_iter, self.gmres_tol, self.mode)
+        x[:] = xs
+        self.resid = st["resid"]
+        self.max_iter = st["iterations"]
+        self.stats = st
+        return x

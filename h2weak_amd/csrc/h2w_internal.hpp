@@ -1,0 +1,149 @@
+// h2w_internal.hpp -- shared declarations of libh2w (host plan, device work
+// descriptors, launch wrappers).  Not part of the public C ABI (include/h2w.h).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/h2w.h"
+
+namespace h2w {
+
+// ------------------------------------------------------------------ errors
+void set_error(const std::string &msg);
+struct Error {
+    int code;
+    std::string msg;
+};
+#define H2W_HIP(call)                                                                        \
+    do {                                                                                     \
+        hipError_t e_ = (call);                                                              \
+        if (e_ != hipSuccess)                                                                \
+            throw ::h2w::Error{H2W_EHIP, std::string(#call) + ": " + hipGetErrorString(e_)}; \
+    } while (0)
+
+// ------------------------------------------------------------------ kernel parameters
+// Matrix entry A(i,j) of the reference's userkernel (*/codes/kernel.hpp); i, j are
+// tree-order point indices, i == j takes the diagonal rule.
+struct KParams {
+    int dim;
+    int choice;
+    double diag;
+    const double *px, *py, *pz;  // tree-order coordinates (device)
+};
+
+// ------------------------------------------------------------------ host plan
+// Tree, tree-order permutation and interaction lists, built from integer box
+// coordinates (the reference builds them from hand-written child tables,
+// 3D/codes/HODLR3Dtree.hpp:155-1680; the lists and their order are the same).
+struct Plan {
+    int dim = 3, nchild = 8, nL = 1;
+    int64_t N = 0;
+    double L = 1.0;
+    std::vector<int32_t> perm;                          // tree index -> input index
+    std::vector<double> px, py, pz;                     // tree-order coordinates
+    std::vector<int> nboxes;                            // per level
+    std::vector<std::vector<int64_t>> bstart, bcount;   // [level][box] point range (tree order)
+    std::vector<std::vector<std::vector<int32_t>>> il;  // [level][box] interaction list
+    std::vector<std::vector<std::vector<uint8_t>>> ilv; // vertex flags of il entries
+    std::vector<std::vector<int32_t>> near;             // [leaf box] near neighbours (slot order)
+
+    void build(int dim, int64_t N, const double *pts_soa, double L, int nlevels);
+    std::vector<int32_t> vert_of(int j, int b) const;
+    std::vector<int32_t> far_of(int j, int b) const;
+};
+
+// ------------------------------------------------------------------ device descriptors
+struct AcaJob {
+    int64_t rows;  // offset into the index array (int32 entries)
+    int64_t cols;
+    int32_t m, n;
+    int32_t rcap;
+    int32_t pad;
+    int64_t wsU;   // offset (doubles) of U, m x rcap column-major
+    int64_t wsV;   // offset (doubles) of V, n x rcap column-major
+    int64_t wsF;   // offset (bytes) of the used-row/used-col flags (m + n bytes)
+    int64_t piv;   // offset into rowpiv/colpiv (rcap entries each)
+};
+
+struct LrJob {     // copy L (unit lower) and R (upper) out of an ACA workspace
+    int64_t wsU, wsV, piv;
+    int32_t m, n, rank, pad;
+    int64_t out;   // offset (doubles): L at out, R at out + rank*rank, both column-major
+};
+
+struct EvalJob {   // A = K(rows, cols), m x n column-major at a
+    int64_t rows, cols, a;
+    int32_t m, n;
+};
+
+struct XferJob {   // E = K(S, Ic) (L R)^{-1}, m x r column-major at e; E^T (r x m) at et
+    int64_t s, ic, lr, e, et;
+    int32_t m, r;
+};
+
+struct CurJob {    // A = (L R)^{-1} K(Ir, V), r x n column-major at a
+    int64_t ir, v, lr, a;
+    int32_t r, n;
+};
+
+// Batched skinny GEMV work: y[row0 .. row0+rows) (+)= sum_t A_t x_t
+struct GTerm {
+    int64_t a;     // element offset of A (column-major, leading dimension = item.lda) at row 0
+    int64_t x;     // element offset of x in the vector pool
+    int32_t cols;
+    int32_t pad;
+};
+struct GItem {
+    int64_t y;     // element offset of the output segment (first row of this tile)
+    int32_t rows;  // <= 64
+    int32_t lda;
+    int32_t row0;  // first row of this tile inside the terms' matrices
+    int32_t t0, t1;
+    int32_t flags; // 0 = exclusive add, 1 = atomic add
+};
+
+// ------------------------------------------------------------------ launch wrappers (kernels.hip)
+void launch_aca(const AcaJob *jobs, int njobs, const int32_t *idx, double *ws, uint8_t *flags, int32_t *rank,
+                int32_t *status, int32_t *rowpiv, int32_t *colpiv, KParams kp, double tol, int big,
+                hipStream_t s);
+void launch_lr_extract(const LrJob *jobs, int njobs, const double *ws, const int32_t *rowpiv,
+                       const int32_t *colpiv, double *out, hipStream_t s);
+void launch_eval(const EvalJob *jobs, int njobs, const int32_t *idx, double *A, KParams kp, hipStream_t s);
+void launch_xfer(const XferJob *jobs, int njobs, const int32_t *idx, const double *lr, double *A, KParams kp,
+                 hipStream_t s);
+void launch_cur(const CurJob *jobs, int njobs, const int32_t *idx, const double *lr, double *A, KParams kp,
+                hipStream_t s);
+void launch_bgemv64(const GItem *items, int nitems, const GTerm *terms, const double *A, double *vec,
+                    hipStream_t s);
+void launch_bgemv32(const GItem *items, int nitems, const GTerm *terms, const float *A, float *vec,
+                    hipStream_t s);
+void launch_to_fp32(const double *a, float *b, int64_t n, hipStream_t s);
+void launch_to_fp64(const float *a, double *b, int64_t n, hipStream_t s);
+void launch_fill(double *a, int64_t n, double v, hipStream_t s);
+void launch_fill32(float *a, int64_t n, float v, hipStream_t s);
+// tree-order gather/scatter with optional precision change, nrhs vectors of leading dimension ld
+void launch_permute_in(const double *x, int64_t ldx, double *xt, int64_t ldt, const int32_t *perm, int64_t N,
+                       int nrhs, hipStream_t s);
+void launch_permute_in32(const double *x, int64_t ldx, float *xt, int64_t ldt, const int32_t *perm, int64_t N,
+                         int nrhs, hipStream_t s);
+void launch_permute_out(const double *yt, int64_t ldt, double *y, int64_t ldy, const int32_t *perm, int64_t N,
+                        int nrhs, hipStream_t s);
+void launch_permute_out32(const float *yt, int64_t ldt, double *y, int64_t ldy, const int32_t *perm, int64_t N,
+                          int nrhs, hipStream_t s);
+void launch_dense_rows(const int32_t *rows, int64_t nrows, const double *xt, double *y, int64_t N, KParams kp,
+                       hipStream_t s);
+// BLAS-1 for the Arnoldi process (all FP64)
+void launch_mdot(const double *V, int64_t ldv, int k, const double *w, int64_t n, double *partial, int nblk,
+                 hipStream_t s);
+void launch_mdot_reduce(const double *partial, int nblk, int k, double *out, hipStream_t s);
+void launch_mgemv_sub(const double *V, int64_t ldv, int k, const double *h, double *w, int64_t n, hipStream_t s);
+void launch_axpby(double a, const double *x, double b, double *y, int64_t n, hipStream_t s);
+void launch_vec_update(const double *V, int64_t ldv, int k, const double *c, double *x, int64_t n, hipStream_t s);
+
+int mdot_blocks(int64_t n);
+
+}  // namespace h2w

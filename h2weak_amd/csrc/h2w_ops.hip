@@ -1,0 +1,1573 @@
+// h2w_ops.hip -- host orchestration of libh2w: assembly, matvec, GMRES and
+// the C ABI declared in include/h2w.h.
+//
+// Device layout (one handle):
+//   points  px/py/pz    tree-order coordinates (FP64)
+//   mat64 / mat32       one pool with every stored operator, column-major,
+//                       32-element aligned; mat32 is the FP32 shadow at the
+//                       same element offsets
+//   vec64 / vec32       one pool with x and y in tree order, then the outgoing
+//                       (o) and incoming (u) skeleton vectors of every family,
+//                       level and box, then the semi-nested pair vectors (t)
+// A matvec is a fixed sequence of batched-GEMV launches over work lists built
+// once at assembly (up pass per level, one M2L + near-field launch, down pass
+// per level, leaf down, semi-nested CUR down).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "h2w_internal.hpp"
+
+namespace h2w {
+
+static thread_local std::string g_err;
+void set_error(const std::string &m) { g_err = m; }
+
+namespace {
+
+constexpr double kPI = 3.141592653589793238462643383279502884;
+
+double now_s() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+template <typename T>
+struct DevBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    DevBuf() {}
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    ~DevBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    void alloc(size_t cnt) {
+        release();
+        if (cnt == 0) return;
+        hipError_t e = hipMalloc(&p, cnt * sizeof(T));
+        if (e != hipSuccess) {
+            p = nullptr;
+            throw Error{H2W_ENOMEM, "hipMalloc of " + std::to_string(cnt * sizeof(T)) + " bytes failed"};
+        }
+        n = cnt;
+    }
+    void ensure(size_t cnt) {
+        if (cnt > n) alloc(cnt);
+    }
+    void upload(const std::vector<T> &v, hipStream_t s) {
+        ensure(v.size());
+        if (!v.empty()) H2W_HIP(hipMemcpyAsync(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s));
+    }
+};
+
+int64_t align_up(int64_t x, int64_t a) { return (x + a - 1) / a * a; }
+
+// one ACA problem: rows/cols are tree-order point indices
+struct AcaBlock {
+    std::vector<int32_t> rows, cols;
+    int rank = 0;
+    std::vector<int32_t> rb, cb;  // local pivot indices
+    std::vector<double> lr;       // L then R, rank x rank column-major each
+};
+
+// skeleton data of one family at one level
+struct FamLevel {
+    int nb = 0;
+    std::vector<int> rank;
+    std::vector<std::vector<int32_t>> Ir, Ic;  // tree-order indices
+    std::vector<std::vector<double>> lr;
+    std::vector<std::vector<int32_t>> S;       // transfer row set (leaf: box points; else children's Ir)
+    std::vector<int64_t> o_off, u_off;         // vec pool
+    std::vector<int64_t> e_off, et_off;        // mat pool
+};
+
+struct Pair {                                  // semi-nested vertex pair (B <- V) at a level
+    int level, b, v;
+    int rank;
+    std::vector<int32_t> Ir, Ic;
+    std::vector<double> lr;
+    int64_t ac_off, ar_off, t_off;
+};
+
+struct Launch {
+    int64_t item0, nitems, term0, nterms;
+};
+
+struct WorkList {
+    std::vector<GItem> items;
+    std::vector<GTerm> terms;
+};
+
+struct TermSpec {
+    int64_t a, x;
+    int32_t cols;
+};
+
+}  // namespace
+
+}  // namespace h2w
+
+struct h2w_handle {
+    int dim = 3, variant = 0, choice = 0, tol_pow = 10, n_max = 64;
+    int64_t N = 0;
+    double L = 1.0;
+    h2w_opts opts;
+    h2w::Plan plan;
+    bool assembled = false;
+    double diag = 0.0;
+    double memory = 0.0;
+    double assembly_seconds = 0.0;
+    int aca_reruns = 0;
+    hipStream_t stream = nullptr;
+    h2w::KParams kp;
+    h2w::DevBuf<double> d_px, d_py, d_pz;
+    h2w::DevBuf<int32_t> d_perm;
+    h2w::DevBuf<double> mat64;
+    h2w::DevBuf<float> mat32;
+    int64_t mat_len = 0;
+    h2w::DevBuf<double> vec64;
+    h2w::DevBuf<float> vec32;
+    int64_t vec_len = 0, off_x = 0, off_y = 0;
+    std::vector<h2w::FamLevel> fam[2];
+    std::vector<h2w::Pair> pairs;
+    std::vector<h2w::Launch> launches;
+    std::vector<int64_t> launch_elems;  // operator elements each launch streams
+    h2w::DevBuf<h2w::GItem> d_items;
+    h2w::DevBuf<h2w::GTerm> d_terms;
+    int64_t stored_elems = 0;
+    // staging for host-pointer calls and GMRES
+    h2w::DevBuf<double> d_stage_in, d_stage_out, d_dense;
+    h2w::DevBuf<double> g_V, g_w, g_r, g_b, g_x, g_h, g_part, g_d;
+    int64_t g_Vcols = 0;
+};
+
+namespace h2w {
+namespace {
+
+using Handle = h2w_handle;
+
+int ref_nlevels(int dim, int64_t N, int n_max, bool grid) {
+    if (grid && dim > 1) {
+        int n = (int)std::llround(dim == 2 ? std::sqrt((double)N) : std::cbrt((double)N));
+        int p = (int)std::llround(dim == 2 ? std::sqrt((double)n_max) : std::cbrt((double)n_max));
+        if (dim == 2) return (int)std::ceil(2.0 * std::log(double(n) / p) / std::log(4));
+        return (int)std::ceil(3 * std::log(double(n) / p) / std::log(8));
+    }
+    return (int)std::ceil(std::log(double(N) / n_max) / std::log((double)(1 << dim)));
+}
+
+// GL-16 nodes/weights by Newton on P_16, for the IE diagonal's Kii
+// (*/codes/integral{1,2,3}d.hpp; Gauss_Legendre_Nodes_and_Weights.hpp:471-523)
+void gl16(double *nodes, double *weights) {
+    const int n = 16;
+    double pos[8], wts[8];
+    for (int i = 0; i < 8; ++i) {
+        double x = std::cos(kPI * (i + 0.75) / (n + 0.5));
+        double dp = 1.0;
+        for (int it = 0; it < 100; ++it) {
+            double p0 = 1.0, p1 = x;
+            for (int k = 2; k <= n; ++k) {
+                double p2 = ((2.0 * k - 1.0) * x * p1 - (k - 1.0) * p0) / k;
+                p0 = p1;
+                p1 = p2;
+            }
+            dp = n * (x * p1 - p0) / (x * x - 1.0);
+            double dx = p1 / dp;
+            x -= dx;
+            if (std::fabs(dx) < 1e-17) break;
+        }
+        double p0 = 1.0, p1 = x;
+        for (int k = 2; k <= n; ++k) {
+            double p2 = ((2.0 * k - 1.0) * x * p1 - (k - 1.0) * p0) / k;
+            p0 = p1;
+            p1 = p2;
+        }
+        dp = n * (x * p1 - p0) / (x * x - 1.0);
+        pos[i] = x;
+        wts[i] = 2.0 / ((1.0 - x * x) * dp * dp);
+    }
+    for (int i = 0; i < 8; ++i) {
+        int s = 7 - i;
+        nodes[2 * i] = -pos[s];
+        nodes[2 * i + 1] = pos[s];
+        weights[2 * i] = weights[2 * i + 1] = wts[s];
+    }
+}
+
+// diagonal value of A (the i == j branch of getMatrixEntry):
+// IE: (1 + Kii) / h^dim with Kii = 2^dim * GL16^dim of the integrand on [0, h/2]^dim,
+// h = N^(-1/dim) (1D/codes/kernel.hpp:95-114, 2D/codes/kernel.hpp:101-122, 3D/codes/kernel.hpp:101-129);
+// RBF: 1D sqrt(2N) (reference_compat) / 2D 1 / 3D sqrt(N).
+double diag_value(int dim, int choice, int64_t N, int compat) {
+    if (choice == 1) {
+        if (dim == 1) return std::sqrt(double(compat ? 2 * N : N));
+        if (dim == 2) return 1.0;
+        return std::sqrt(double(N));
+    }
+    double nd[16], w[16];
+    gl16(nd, w);
+    double h = dim == 1 ? 1.0 / double(N) : (dim == 2 ? 1.0 / std::sqrt(double(N)) : 1.0 / std::cbrt(double(N)));
+    double cx = 0.5 * (0.5 * h), Lx = cx;
+    double res = 0.0;
+    if (dim == 1) {
+        for (int i = 0; i < 16; ++i) res += w[i] * std::log(std::fabs(1.0 * (cx * nd[i] + Lx)));
+        res *= cx;
+        res *= 2;
+        return (1.0 + res) / h;
+    }
+    if (dim == 2) {
+        double c = -1.0 / (2 * kPI);
+        for (int i = 0; i < 16; ++i)
+            for (int j = 0; j < 16; ++j) {
+                double tx = cx * nd[i] + Lx, ty = cx * nd[j] + Lx;
+                res += w[i] * w[j] * (c * 0.5 * std::log(tx * tx + ty * ty));
+            }
+        res *= (cx * cx);
+        res *= 4;
+        return (1.0 + res) / std::pow(h, 2.0);
+    }
+    for (int i = 0; i < 16; ++i)
+        for (int j = 0; j < 16; ++j)
+            for (int k = 0; k < 16; ++k) {
+                double tx = cx * nd[i] + Lx, ty = cx * nd[j] + Lx, tz = cx * nd[k] + Lx;
+                res += (w[i] * w[j] * w[k] * (1.0 / ((4 * kPI) * std::sqrt(tx * tx + ty * ty + tz * tz))));
+            }
+    res *= (cx * cx * cx);
+    res *= 8;
+    return (1.0 + res) / std::pow(h, 3.0);
+}
+
+// ------------------------------------------------------------------ batched ACA driver
+// Runs every block through the device ACA in workspace-bounded batches; blocks
+// that reach their rank cap before converging are re-run with a 4x larger cap.
+void run_aca(Handle &h, std::vector<AcaBlock> &blocks) {
+    if (blocks.empty()) return;
+    const double tol = std::pow(10.0, -1.0 * h.tol_pow);
+    hipStream_t s = h.stream;
+    int64_t budget = h.opts.workspace_bytes;
+    if (budget <= 0) {
+        size_t fr = 0, tot = 0;
+        H2W_HIP(hipMemGetInfo(&fr, &tot));
+        budget = std::min<int64_t>((int64_t)8 << 30, (int64_t)(fr / 4));
+        budget = std::max<int64_t>(budget, (int64_t)64 << 20);
+    }
+    std::vector<int> cap(blocks.size());
+    for (size_t b = 0; b < blocks.size(); ++b) {
+        int mn = (int)std::min(blocks[b].rows.size(), blocks[b].cols.size());
+        cap[b] = std::min(mn, std::max(1, h.opts.rank_cap));
+    }
+    std::vector<size_t> todo(blocks.size());
+    std::iota(todo.begin(), todo.end(), 0);
+    DevBuf<double> ws;
+    DevBuf<uint8_t> flags;
+    DevBuf<int32_t> didx, drank, dstat, drp, dcp;
+    DevBuf<AcaJob> djobs;
+    DevBuf<LrJob> dlr;
+    DevBuf<double> dlrout;
+    while (!todo.empty()) {
+        std::vector<size_t> rerun;
+        size_t pos = 0;
+        while (pos < todo.size()) {
+            // form a batch (big blocks get 1024-thread workgroups in their own launch)
+            std::vector<size_t> batch;
+            int64_t wsz = 0, isz = 0;
+            bool big = false;
+            {
+                size_t b0 = todo[pos];
+                big = blocks[b0].rows.size() + blocks[b0].cols.size() > 65536;
+            }
+            while (pos < todo.size()) {
+                size_t b = todo[pos];
+                const AcaBlock &B = blocks[b];
+                bool bb = B.rows.size() + B.cols.size() > 65536;
+                if (bb != big) break;
+                int64_t need = align_up((int64_t)(B.rows.size() + B.cols.size()) * cap[b], 32);
+                if (!batch.empty() && (wsz + need) * 8 > budget) break;
+                batch.push_back(b);
+                wsz += need;
+                isz += (int64_t)(B.rows.size() + B.cols.size());
+                ++pos;
+                if (batch.size() >= 65535) break;
+            }
+            std::vector<AcaJob> jobs(batch.size());
+            std::vector<int32_t> idx;
+            idx.reserve(isz);
+            int64_t wo = 0, fo = 0, po = 0;
+            for (size_t q = 0; q < batch.size(); ++q) {
+                const AcaBlock &B = blocks[batch[q]];
+                AcaJob &J = jobs[q];
+                J.m = (int32_t)B.rows.size();
+                J.n = (int32_t)B.cols.size();
+                J.rcap = cap[batch[q]];
+                J.pad = 0;
+                J.rows = (int64_t)idx.size();
+                idx.insert(idx.end(), B.rows.begin(), B.rows.end());
+                J.cols = (int64_t)idx.size();
+                idx.insert(idx.end(), B.cols.begin(), B.cols.end());
+                J.wsU = wo;
+                J.wsV = wo + (int64_t)J.m * J.rcap;
+                wo += align_up((int64_t)(J.m + J.n) * J.rcap, 32);
+                J.wsF = fo;
+                fo += align_up((int64_t)(J.m + J.n), 64);
+                J.piv = po;
+                po += J.rcap;
+            }
+            ws.ensure(std::max<int64_t>(wo, 1));
+            flags.ensure(std::max<int64_t>(fo, 1));
+            didx.upload(idx, s);
+            djobs.upload(jobs, s);
+            drank.ensure(batch.size());
+            dstat.ensure(batch.size());
+            drp.ensure(std::max<int64_t>(po, 1));
+            dcp.ensure(std::max<int64_t>(po, 1));
+            launch_aca(djobs.p, (int)batch.size(), didx.p, ws.p, flags.p, drank.p, dstat.p, drp.p, dcp.p, h.kp, tol,
+                       big ? 1 : 0, s);
+            H2W_HIP(hipGetLastError());
+            std::vector<int32_t> rk(batch.size()), st(batch.size()), rp(po), cp(po);
+            H2W_HIP(hipMemcpyAsync(rk.data(), drank.p, rk.size() * 4, hipMemcpyDeviceToHost, s));
+            H2W_HIP(hipMemcpyAsync(st.data(), dstat.p, st.size() * 4, hipMemcpyDeviceToHost, s));
+            H2W_HIP(hipMemcpyAsync(rp.data(), drp.p, po * 4, hipMemcpyDeviceToHost, s));
+            H2W_HIP(hipMemcpyAsync(cp.data(), dcp.p, po * 4, hipMemcpyDeviceToHost, s));
+            H2W_HIP(hipStreamSynchronize(s));
+            // L and R of the converged blocks
+            std::vector<LrJob> lj;
+            std::vector<size_t> lb;
+            int64_t lo = 0;
+            for (size_t q = 0; q < batch.size(); ++q) {
+                size_t b = batch[q];
+                if (st[q] != 0) {
+                    int mn = (int)std::min(blocks[b].rows.size(), blocks[b].cols.size());
+                    cap[b] = std::min(mn, cap[b] * 4);
+                    rerun.push_back(b);
+                    h.aca_reruns++;
+                    continue;
+                }
+                AcaBlock &B = blocks[b];
+                B.rank = rk[q];
+                B.rb.assign(rp.begin() + jobs[q].piv, rp.begin() + jobs[q].piv + rk[q]);
+                B.cb.assign(cp.begin() + jobs[q].piv, cp.begin() + jobs[q].piv + rk[q]);
+                if (rk[q] > 0) {
+                    LrJob J;
+                    J.wsU = jobs[q].wsU;
+                    J.wsV = jobs[q].wsV;
+                    J.piv = jobs[q].piv;
+                    J.m = jobs[q].m;
+                    J.n = jobs[q].n;
+                    J.rank = rk[q];
+                    J.pad = 0;
+                    J.out = lo;
+                    lo += 2LL * rk[q] * rk[q];
+                    lj.push_back(J);
+                    lb.push_back(b);
+                }
+            }
+            if (!lj.empty()) {
+                dlr.upload(lj, s);
+                dlrout.ensure(lo);
+                launch_lr_extract(dlr.p, (int)lj.size(), ws.p, drp.p, dcp.p, dlrout.p, s);
+                H2W_HIP(hipGetLastError());
+                std::vector<double> hl(lo);
+                H2W_HIP(hipMemcpyAsync(hl.data(), dlrout.p, lo * 8, hipMemcpyDeviceToHost, s));
+                H2W_HIP(hipStreamSynchronize(s));
+                for (size_t q = 0; q < lj.size(); ++q) {
+                    AcaBlock &B = blocks[lb[q]];
+                    B.lr.assign(hl.begin() + lj[q].out, hl.begin() + lj[q].out + 2LL * lj[q].rank * lj[q].rank);
+                }
+            }
+        }
+        todo.swap(rerun);
+    }
+}
+
+std::vector<int32_t> range_list(int64_t start, int64_t count) {
+    std::vector<int32_t> v(count);
+    for (int64_t i = 0; i < count; ++i) v[i] = (int32_t)(start + i);
+    return v;
+}
+
+// far field, bottom-up NCA (3D/codes/HODLR3Dtree.hpp:1872-1939)
+void far_nca(Handle &h) {
+    const Plan &P = h.plan;
+    auto &F = h.fam[0];
+    F.assign(P.nL + 1, FamLevel());
+    if (P.dim == 1) return;
+    for (int j = P.nL; j >= 2; --j) {
+        FamLevel &fl = F[j];
+        fl.nb = P.nboxes[j];
+        fl.S.resize(fl.nb);
+        for (int b = 0; b < fl.nb; ++b) {
+            if (j == P.nL) fl.S[b] = range_list(P.bstart[j][b], P.bcount[j][b]);
+            else
+                for (int c = 0; c < P.nchild; ++c) {
+                    auto &ir = F[j + 1].Ir[b * P.nchild + c];
+                    fl.S[b].insert(fl.S[b].end(), ir.begin(), ir.end());
+                }
+        }
+        std::vector<AcaBlock> blocks(fl.nb);
+        for (int b = 0; b < fl.nb; ++b) {
+            blocks[b].rows = fl.S[b];
+            for (int f : P.far_of(j, b)) blocks[b].cols.insert(blocks[b].cols.end(), fl.S[f].begin(), fl.S[f].end());
+        }
+        run_aca(h, blocks);
+        fl.rank.resize(fl.nb);
+        fl.Ir.resize(fl.nb);
+        fl.Ic.resize(fl.nb);
+        fl.lr.resize(fl.nb);
+        for (int b = 0; b < fl.nb; ++b) {
+            AcaBlock &B = blocks[b];
+            fl.rank[b] = B.rank;
+            for (int q = 0; q < B.rank; ++q) {
+                fl.Ir[b].push_back(B.rows[B.rb[q]]);
+                fl.Ic[b].push_back(B.cols[B.cb[q]]);
+            }
+            fl.lr[b] = std::move(B.lr);
+        }
+    }
+}
+
+// vertex-sharing field, top-down NCA with the parent's column pivots appended
+// (3D/codes/HODLR3Dtree.hpp:2062-2107; 1D/codes/HODLR1Dtree.hpp:231-276)
+void vertex_nca(Handle &h) {
+    const Plan &P = h.plan;
+    auto &F = h.fam[1];
+    F.assign(P.nL + 1, FamLevel());
+    for (int j = 1; j <= P.nL; ++j) {
+        FamLevel &fl = F[j];
+        fl.nb = P.nboxes[j];
+        std::vector<AcaBlock> blocks(fl.nb);
+        for (int b = 0; b < fl.nb; ++b) {
+            blocks[b].rows = range_list(P.bstart[j][b], P.bcount[j][b]);
+            for (int v : P.vert_of(j, b)) {
+                auto r = range_list(P.bstart[j][v], P.bcount[j][v]);
+                blocks[b].cols.insert(blocks[b].cols.end(), r.begin(), r.end());
+            }
+            if (j > 1) {
+                auto &pic = F[j - 1].Ic[b / P.nchild];
+                blocks[b].cols.insert(blocks[b].cols.end(), pic.begin(), pic.end());
+            }
+        }
+        run_aca(h, blocks);
+        fl.rank.resize(fl.nb);
+        fl.Ir.resize(fl.nb);
+        fl.Ic.resize(fl.nb);
+        fl.lr.resize(fl.nb);
+        for (int b = 0; b < fl.nb; ++b) {
+            AcaBlock &B = blocks[b];
+            fl.rank[b] = B.rank;
+            for (int q = 0; q < B.rank; ++q) {
+                fl.Ir[b].push_back(B.rows[B.rb[q]]);
+                fl.Ic[b].push_back(B.cols[B.cb[q]]);
+            }
+            fl.lr[b] = std::move(B.lr);
+        }
+    }
+    for (int j = 1; j <= P.nL; ++j) {
+        FamLevel &fl = F[j];
+        fl.S.assign(fl.nb, {});
+        for (int b = 0; b < fl.nb; ++b) {
+            if (j == P.nL) fl.S[b] = range_list(P.bstart[j][b], P.bcount[j][b]);
+            else
+                for (int c = 0; c < P.nchild; ++c) {
+                    auto &ir = F[j + 1].Ir[b * P.nchild + c];
+                    fl.S[b].insert(fl.S[b].end(), ir.begin(), ir.end());
+                }
+        }
+    }
+}
+
+// semi-nested vertex blocks: one cross approximation per directed pair and level
+// (3D/codes/HODLR3Dtree.hpp:2250-2321; 1D/codes/HODLR1Dtree.hpp:418-443)
+void cur_pairs(Handle &h) {
+    const Plan &P = h.plan;
+    h.pairs.clear();
+    std::vector<AcaBlock> blocks;
+    for (int j = 1; j <= P.nL; ++j)
+        for (int b = 0; b < P.nboxes[j]; ++b)
+            for (int v : P.vert_of(j, b)) {
+                Pair p;
+                p.level = j;
+                p.b = b;
+                p.v = v;
+                p.rank = 0;
+                h.pairs.push_back(p);
+                AcaBlock B;
+                B.rows = range_list(P.bstart[j][b], P.bcount[j][b]);
+                B.cols = range_list(P.bstart[j][v], P.bcount[j][v]);
+                blocks.push_back(std::move(B));
+            }
+    run_aca(h, blocks);
+    for (size_t q = 0; q < blocks.size(); ++q) {
+        Pair &p = h.pairs[q];
+        AcaBlock &B = blocks[q];
+        p.rank = B.rank;
+        for (int k = 0; k < B.rank; ++k) {
+            p.Ir.push_back(B.rows[B.rb[k]]);
+            p.Ic.push_back(B.cols[B.cb[k]]);
+        }
+        p.lr = std::move(B.lr);
+    }
+}
+
+// ------------------------------------------------------------------ work lists
+constexpr int64_t kItemWork = 64 * 4096;  // elements of A per GEMV item before column chunking
+
+void add_item(WorkList &W, int64_t y, int rows, const std::vector<TermSpec> &terms, bool atomic) {
+    if (rows <= 0 || terms.empty()) return;
+    int64_t C = 0;
+    for (auto &t : terms) C += t.cols;
+    if (C == 0) return;
+    for (int r0 = 0; r0 < rows; r0 += 64) {
+        int tr = std::min(64, rows - r0);
+        if ((int64_t)tr * C <= kItemWork) {
+            GItem it;
+            it.y = y + r0;
+            it.rows = tr;
+            it.lda = rows;
+            it.row0 = r0;
+            it.t0 = (int32_t)W.terms.size();
+            for (auto &t : terms)
+                if (t.cols > 0) W.terms.push_back(GTerm{t.a, t.x, t.cols, 0});
+            it.t1 = (int32_t)W.terms.size();
+            it.flags = atomic ? 1 : 0;
+            W.items.push_back(it);
+        } else {
+            // column chunks accumulate atomically
+            const int64_t maxc = std::max<int64_t>(1, kItemWork / 64);
+            GItem it;
+            it.rows = tr;
+            it.lda = rows;
+            it.row0 = r0;
+            it.y = y + r0;
+            it.flags = 1;
+            int64_t used = 0;
+            it.t0 = (int32_t)W.terms.size();
+            for (auto &t : terms) {
+                for (int64_t c0 = 0; c0 < t.cols;) {
+                    int64_t take = std::min<int64_t>(t.cols - c0, maxc - used);
+                    W.terms.push_back(GTerm{t.a + c0 * rows, t.x + c0, (int32_t)take, 0});
+                    used += take;
+                    c0 += take;
+                    if (used == maxc) {
+                        it.t1 = (int32_t)W.terms.size();
+                        W.items.push_back(it);
+                        it.t0 = it.t1;
+                        used = 0;
+                    }
+                }
+            }
+            it.t1 = (int32_t)W.terms.size();
+            if (it.t1 > it.t0) W.items.push_back(it);
+        }
+    }
+}
+
+// ------------------------------------------------------------------ assembly
+void assemble(Handle &h) {
+    double t0 = now_s();
+    const Plan &P = h.plan;
+    hipStream_t s = h.stream;
+    h.aca_reruns = 0;
+    if (P.dim >= 2) far_nca(h);
+    else h.fam[0].assign(P.nL + 1, FamLevel());
+    if (h.variant == H2W_NESTED) vertex_nca(h);
+    else {
+        h.fam[1].assign(P.nL + 1, FamLevel());
+        cur_pairs(h);
+    }
+    const int nfam_lo[2] = {2, 1};  // first level of each family
+    const bool fam_on[2] = {P.dim >= 2, h.variant == H2W_NESTED};
+
+    // ---- vector pool
+    int64_t vo = 0;
+    h.off_x = vo;
+    vo += align_up(P.N, 32);
+    h.off_y = vo;
+    vo += align_up(P.N, 32);
+    for (int f = 0; f < 2; ++f) {
+        if (!fam_on[f]) continue;
+        for (int j = nfam_lo[f]; j <= P.nL; ++j) {
+            FamLevel &fl = h.fam[f][j];
+            fl.o_off.resize(fl.nb);
+            fl.u_off.resize(fl.nb);
+            // o segments of one parent's children are contiguous (the parent's up input)
+            for (int b = 0; b < fl.nb; ++b) {
+                fl.o_off[b] = vo;
+                vo += fl.rank[b];
+            }
+            vo = align_up(vo, 32);
+            for (int b = 0; b < fl.nb; ++b) {
+                fl.u_off[b] = vo;
+                vo += fl.rank[b];
+            }
+            vo = align_up(vo, 32);
+        }
+    }
+    for (auto &p : h.pairs) {
+        p.t_off = vo;
+        vo += p.rank;
+    }
+    h.vec_len = align_up(std::max<int64_t>(vo, 1), 32);
+
+    // ---- matrix pool offsets
+    int64_t mo = 0;
+    auto take = [&](int64_t elems) {
+        int64_t o = mo;
+        mo += align_up(elems, 32);
+        return o;
+    };
+    double memory = 0.0;
+    for (int f = 0; f < 2; ++f) {
+        if (!fam_on[f]) continue;
+        for (int j = nfam_lo[f]; j <= P.nL; ++j) {
+            FamLevel &fl = h.fam[f][j];
+            fl.e_off.assign(fl.nb, -1);
+            fl.et_off.assign(fl.nb, -1);
+            for (int b = 0; b < fl.nb; ++b) {
+                int64_t m = (int64_t)fl.S[b].size(), r = fl.rank[b];
+                memory += 2.0 * (double)(m * r);
+                if (m * r == 0) continue;
+                fl.e_off[b] = take(m * r);
+                fl.et_off[b] = take(m * r);
+            }
+        }
+    }
+    struct M2LBlk {
+        int f, j, b, p;
+        int64_t off;
+    };
+    std::vector<M2LBlk> m2l;
+    for (int f = 0; f < 2; ++f) {
+        if (!fam_on[f]) continue;
+        for (int j = nfam_lo[f]; j <= P.nL; ++j) {
+            FamLevel &fl = h.fam[f][j];
+            for (int b = 0; b < fl.nb; ++b) {
+                auto partners = f == 0 ? P.far_of(j, b) : P.vert_of(j, b);
+                for (int p : partners) {
+                    int64_t e = (int64_t)fl.rank[b] * fl.rank[p];
+                    memory += (double)e;
+                    if (e == 0) continue;
+                    m2l.push_back(M2LBlk{f, j, b, p, take(e)});
+                }
+            }
+        }
+    }
+    struct NearBlk {
+        int b, n;
+        int64_t off;
+    };
+    std::vector<NearBlk> nearb;
+    const int nleaf = P.nboxes[P.nL];
+    const bool count_near = !(P.dim == 1 && h.variant == H2W_SEMI_NESTED);  // 1D findMemory_sn skips it
+    for (int b = 0; b < nleaf; ++b) {
+        std::vector<int32_t> lst = P.near[b];
+        lst.push_back(b);  // self last (3D/codes/HODLR3Dtree.hpp:2388)
+        for (int n : lst) {
+            int64_t e = P.bcount[P.nL][b] * P.bcount[P.nL][n];
+            if (count_near) memory += (double)e;
+            if (e == 0) continue;
+            nearb.push_back(NearBlk{b, n, take(e)});
+        }
+    }
+    for (auto &p : h.pairs) {
+        int64_t mb = P.bcount[p.level][p.b], nv = P.bcount[p.level][p.v];
+        if (P.dim == 1) memory += (double)(mb * p.rank + (int64_t)p.rank * nv);  // U + V
+        else memory += 2.0 * p.rank * p.rank + (double)(mb * p.rank + (int64_t)p.rank * nv);  // L + R + Ac + Ar
+        p.ac_off = p.ar_off = -1;
+        if (p.rank == 0) continue;
+        p.ac_off = take(mb * p.rank);
+        p.ar_off = take((int64_t)p.rank * nv);
+    }
+    h.mat_len = std::max<int64_t>(mo, 32);
+    h.memory = memory;
+    h.mat64.alloc(h.mat_len);
+    H2W_HIP(hipMemsetAsync(h.mat64.p, 0, h.mat_len * 8, s));
+    h.vec64.alloc(h.vec_len);
+    h.vec32.alloc(h.vec_len);
+
+    // ---- evaluate operators on the device
+    {
+        // transfers
+        std::vector<int32_t> idx;
+        std::vector<XferJob> xj;
+        std::vector<double> lr;
+        for (int f = 0; f < 2; ++f) {
+            if (!fam_on[f]) continue;
+            for (int j = nfam_lo[f]; j <= P.nL; ++j) {
+                FamLevel &fl = h.fam[f][j];
+                for (int b = 0; b < fl.nb; ++b) {
+                    if (fl.e_off[b] < 0) continue;
+                    XferJob J;
+                    J.m = (int32_t)fl.S[b].size();
+                    J.r = fl.rank[b];
+                    J.s = (int64_t)idx.size();
+                    idx.insert(idx.end(), fl.S[b].begin(), fl.S[b].end());
+                    J.ic = (int64_t)idx.size();
+                    idx.insert(idx.end(), fl.Ic[b].begin(), fl.Ic[b].end());
+                    J.lr = (int64_t)lr.size();
+                    lr.insert(lr.end(), fl.lr[b].begin(), fl.lr[b].end());
+                    J.e = fl.e_off[b];
+                    J.et = fl.et_off[b];
+                    xj.push_back(J);
+                }
+            }
+        }
+        // semi-nested row factors
+        std::vector<CurJob> cj;
+        for (auto &p : h.pairs) {
+            if (p.rank == 0) continue;
+            CurJob J;
+            J.r = p.rank;
+            J.n = (int32_t)P.bcount[p.level][p.v];
+            J.ir = (int64_t)idx.size();
+            idx.insert(idx.end(), p.Ir.begin(), p.Ir.end());
+            J.v = (int64_t)idx.size();
+            auto vr = range_list(P.bstart[p.level][p.v], J.n);
+            idx.insert(idx.end(), vr.begin(), vr.end());
+            J.lr = (int64_t)lr.size();
+            lr.insert(lr.end(), p.lr.begin(), p.lr.end());
+            J.a = p.ar_off;
+            cj.push_back(J);
+        }
+        DevBuf<int32_t> didx;
+        DevBuf<double> dlr;
+        DevBuf<XferJob> dxj;
+        DevBuf<CurJob> dcj;
+        didx.upload(idx, s);
+        dlr.upload(lr, s);
+        dxj.upload(xj, s);
+        dcj.upload(cj, s);
+        launch_xfer(dxj.p, (int)xj.size(), didx.p, dlr.p, h.mat64.p, h.kp, s);
+        H2W_HIP(hipGetLastError());
+        launch_cur(dcj.p, (int)cj.size(), didx.p, dlr.p, h.mat64.p, h.kp, s);
+        H2W_HIP(hipGetLastError());
+        H2W_HIP(hipStreamSynchronize(s));
+    }
+    {
+        // plain entry blocks: M2L, near field, semi-nested column factors
+        std::vector<int32_t> idx;
+        std::vector<EvalJob> ej;
+        auto add = [&](const std::vector<int32_t> &r, const std::vector<int32_t> &c, int64_t off) {
+            EvalJob J;
+            J.m = (int32_t)r.size();
+            J.n = (int32_t)c.size();
+            J.rows = (int64_t)idx.size();
+            idx.insert(idx.end(), r.begin(), r.end());
+            J.cols = (int64_t)idx.size();
+            idx.insert(idx.end(), c.begin(), c.end());
+            J.a = off;
+            ej.push_back(J);
+        };
+        DevBuf<int32_t> didx;
+        DevBuf<EvalJob> dej;
+        auto flush = [&]() {
+            if (ej.empty()) return;
+            didx.upload(idx, s);
+            dej.upload(ej, s);
+            launch_eval(dej.p, (int)ej.size(), didx.p, h.mat64.p, h.kp, s);
+            H2W_HIP(hipGetLastError());
+            H2W_HIP(hipStreamSynchronize(s));
+            idx.clear();
+            ej.clear();
+        };
+        for (auto &m : m2l) {
+            add(h.fam[m.f][m.j].Ir[m.b], h.fam[m.f][m.j].Ir[m.p], m.off);
+            if (idx.size() > ((size_t)64 << 20)) flush();
+        }
+        for (auto &nb : nearb) {
+            add(range_list(P.bstart[P.nL][nb.b], P.bcount[P.nL][nb.b]),
+                range_list(P.bstart[P.nL][nb.n], P.bcount[P.nL][nb.n]), nb.off);
+            if (idx.size() > ((size_t)64 << 20)) flush();
+        }
+        for (auto &p : h.pairs) {
+            if (p.rank == 0) continue;
+            add(range_list(P.bstart[p.level][p.b], P.bcount[p.level][p.b]), p.Ic, p.ac_off);
+            if (idx.size() > ((size_t)64 << 20)) flush();
+        }
+        flush();
+    }
+    if (h.opts.fp32_shadow) {
+        h.mat32.alloc(h.mat_len);
+        launch_to_fp32(h.mat64.p, h.mat32.p, h.mat_len, s);
+        H2W_HIP(hipGetLastError());
+    } else {
+        h.mat32.release();
+    }
+
+    // ---- GEMV work lists, one per launch of the matvec
+    std::vector<WorkList> WL;
+    auto fam_up_item = [&](WorkList &W, int f, int j, int b, int64_t src) {
+        FamLevel &fl = h.fam[f][j];
+        if (fl.et_off[b] < 0) return;
+        add_item(W, fl.o_off[b], fl.rank[b], {TermSpec{fl.et_off[b], src, (int32_t)fl.S[b].size()}}, false);
+    };
+    // up, leaf level (and semi-nested row factors, which read x directly)
+    {
+        WorkList W;
+        for (int f = 0; f < 2; ++f) {
+            if (!fam_on[f] || P.nL < nfam_lo[f]) continue;
+            for (int b = 0; b < P.nboxes[P.nL]; ++b) fam_up_item(W, f, P.nL, b, h.off_x + P.bstart[P.nL][b]);
+        }
+        for (auto &p : h.pairs)
+            if (p.rank > 0)
+                add_item(W, p.t_off, p.rank,
+                         {TermSpec{p.ar_off, h.off_x + P.bstart[p.level][p.v], (int32_t)P.bcount[p.level][p.v]}},
+                         false);
+        WL.push_back(std::move(W));
+    }
+    // up, levels nL-1 .. 1
+    for (int j = P.nL - 1; j >= 1; --j) {
+        WorkList W;
+        for (int f = 0; f < 2; ++f) {
+            if (!fam_on[f] || j < nfam_lo[f]) continue;
+            for (int b = 0; b < P.nboxes[j]; ++b)
+                fam_up_item(W, f, j, b, h.fam[f][j + 1].o_off[b * P.nchild]);
+        }
+        WL.push_back(std::move(W));
+    }
+    // M2L (every level, both families) + near field
+    {
+        WorkList W;
+        for (int f = 0; f < 2; ++f) {
+            if (!fam_on[f]) continue;
+            for (int j = nfam_lo[f]; j <= P.nL; ++j) {
+                FamLevel &fl = h.fam[f][j];
+                std::vector<std::vector<TermSpec>> per(fl.nb);
+                for (auto &m : m2l)
+                    if (m.f == f && m.j == j)
+                        per[m.b].push_back(TermSpec{m.off, fl.o_off[m.p], (int32_t)fl.rank[m.p]});
+                for (int b = 0; b < fl.nb; ++b) add_item(W, fl.u_off[b], fl.rank[b], per[b], false);
+            }
+        }
+        std::vector<std::vector<TermSpec>> per(nleaf);
+        for (auto &nb : nearb)
+            per[nb.b].push_back(TermSpec{nb.off, h.off_x + P.bstart[P.nL][nb.n], (int32_t)P.bcount[P.nL][nb.n]});
+        for (int b = 0; b < nleaf; ++b) add_item(W, h.off_y + P.bstart[P.nL][b], (int)P.bcount[P.nL][b], per[b], false);
+        WL.push_back(std::move(W));
+    }
+    // down, levels 1 .. nL-1
+    for (int j = 1; j <= P.nL - 1; ++j) {
+        WorkList W;
+        for (int f = 0; f < 2; ++f) {
+            if (!fam_on[f] || j < nfam_lo[f]) continue;
+            FamLevel &fl = h.fam[f][j];
+            for (int b = 0; b < fl.nb; ++b) {
+                if (fl.e_off[b] < 0) continue;
+                int64_t m = (int64_t)fl.S[b].size();
+                add_item(W, h.fam[f][j + 1].u_off[b * P.nchild], (int)m,
+                         {TermSpec{fl.e_off[b], fl.u_off[b], (int32_t)fl.rank[b]}}, false);
+            }
+        }
+        WL.push_back(std::move(W));
+    }
+    // down, leaf level: far + vertex into y
+    {
+        WorkList W;
+        for (int b = 0; b < nleaf; ++b) {
+            std::vector<TermSpec> t;
+            for (int f = 0; f < 2; ++f) {
+                if (!fam_on[f] || P.nL < nfam_lo[f]) continue;
+                FamLevel &fl = h.fam[f][P.nL];
+                if (fl.e_off[b] >= 0) t.push_back(TermSpec{fl.e_off[b], fl.u_off[b], (int32_t)fl.rank[b]});
+            }
+            add_item(W, h.off_y + P.bstart[P.nL][b], (int)P.bcount[P.nL][b], t, false);
+        }
+        WL.push_back(std::move(W));
+    }
+    // semi-nested column factors into y (boxes of all levels overlap: atomic)
+    if (!h.pairs.empty()) {
+        WorkList W;
+        std::vector<std::vector<std::vector<TermSpec>>> per(P.nL + 1);
+        for (int j = 1; j <= P.nL; ++j) per[j].assign(P.nboxes[j], {});
+        for (auto &p : h.pairs)
+            if (p.rank > 0) per[p.level][p.b].push_back(TermSpec{p.ac_off, p.t_off, (int32_t)p.rank});
+        for (int j = 1; j <= P.nL; ++j)
+            for (int b = 0; b < P.nboxes[j]; ++b)
+                add_item(W, h.off_y + P.bstart[j][b], (int)P.bcount[j][b], per[j][b], true);
+        WL.push_back(std::move(W));
+    }
+    std::vector<GItem> items;
+    std::vector<GTerm> terms;
+    h.launches.clear();
+    for (auto &W : WL) {
+        if (W.items.empty()) continue;
+        Launch Lh;
+        Lh.item0 = (int64_t)items.size();
+        Lh.nitems = (int64_t)W.items.size();
+        Lh.term0 = (int64_t)terms.size();
+        Lh.nterms = (int64_t)W.terms.size();
+        for (auto it : W.items) {
+            it.t0 += (int32_t)Lh.term0;
+            it.t1 += (int32_t)Lh.term0;
+            items.push_back(it);
+        }
+        terms.insert(terms.end(), W.terms.begin(), W.terms.end());
+        h.launches.push_back(Lh);
+    }
+    h.d_items.upload(items, s);
+    h.d_terms.upload(terms, s);
+    int64_t se = 0;
+    h.launch_elems.assign(h.launches.size(), 0);
+    for (size_t q = 0; q < h.launches.size(); ++q) {
+        const Launch &Lh = h.launches[q];
+        for (int64_t i = Lh.item0; i < Lh.item0 + Lh.nitems; ++i)
+            for (int t = items[i].t0; t < items[i].t1; ++t) h.launch_elems[q] += (int64_t)items[i].rows * terms[t].cols;
+        se += h.launch_elems[q];
+    }
+    h.stored_elems = se;
+    H2W_HIP(hipStreamSynchronize(s));
+    h.assembled = true;
+    h.assembly_seconds = now_s() - t0;
+}
+
+// ------------------------------------------------------------------ matvec
+// y_tree = A x_tree inside the vector pool (x already at off_x)
+void run_launches(Handle &h, int precision) {
+    hipStream_t s = h.stream;
+    if (precision == H2W_FP32) {
+        launch_fill32(h.vec32.p + h.off_y, h.vec_len - h.off_y, 0.0f, s);
+        for (auto &Lh : h.launches)
+            launch_bgemv32(h.d_items.p + Lh.item0, (int)Lh.nitems, h.d_terms.p, h.mat32.p, h.vec32.p, s);
+    } else {
+        launch_fill(h.vec64.p + h.off_y, h.vec_len - h.off_y, 0.0, s);
+        for (auto &Lh : h.launches)
+            launch_bgemv64(h.d_items.p + Lh.item0, (int)Lh.nitems, h.d_terms.p, h.mat64.p, h.vec64.p, s);
+    }
+    H2W_HIP(hipGetLastError());
+}
+
+// device-pointer matvec on one vector; tree-ordered in/out when tree == true
+void matvec_dev(Handle &h, const double *x, double *y, int precision, bool tree) {
+    hipStream_t s = h.stream;
+    const int64_t N = h.N;
+    if (precision == H2W_FP32) {
+        if (!h.mat32.p) throw Error{H2W_ESTATE, "FP32 matvec requested but the FP32 shadow was not built"};
+        if (tree) launch_to_fp32(x, h.vec32.p + h.off_x, N, s);
+        else launch_permute_in32(x, N, h.vec32.p + h.off_x, N, h.d_perm.p, N, 1, s);
+        run_launches(h, precision);
+        if (tree) launch_to_fp64(h.vec32.p + h.off_y, y, N, s);
+        else launch_permute_out32(h.vec32.p + h.off_y, N, y, N, h.d_perm.p, N, 1, s);
+    } else {
+        if (tree) H2W_HIP(hipMemcpyAsync(h.vec64.p + h.off_x, x, N * 8, hipMemcpyDeviceToDevice, s));
+        else launch_permute_in(x, N, h.vec64.p + h.off_x, N, h.d_perm.p, N, 1, s);
+        run_launches(h, precision);
+        if (tree) H2W_HIP(hipMemcpyAsync(y, h.vec64.p + h.off_y, N * 8, hipMemcpyDeviceToDevice, s));
+        else launch_permute_out(h.vec64.p + h.off_y, N, y, N, h.d_perm.p, N, 1, s);
+    }
+    H2W_HIP(hipGetLastError());
+}
+
+}  // namespace
+}  // namespace h2w
+
+namespace h2w {
+namespace {
+
+// ------------------------------------------------------------------ GMRES (tree order, device vectors)
+// Unrestarted GMRES as the reference's iterative_solver::GMRES (1D/codes/gmres.hpp:91-200):
+// restart = max_iter, Givens rotations (:30-75), resid = |s_{i+1}| / ||b|| (:162),
+// back-substitution update (:77-90).  The Arnoldi step orthogonalises with
+// classical Gram-Schmidt applied twice (two fused passes over the basis instead
+// of the reference's i+1 dependent modified Gram-Schmidt steps); all of it FP64.
+struct GmresOut {
+    int iters = 0;
+    double resid = 0.0;
+    bool conv = false;
+    int matvecs = 0;
+};
+
+void ensure_V(Handle &h, int64_t cols) {
+    if (cols <= h.g_Vcols) return;
+    int64_t nc = std::max<int64_t>(cols, std::max<int64_t>(16, h.g_Vcols * 2));
+    double *np = nullptr;
+    if (hipMalloc(&np, (size_t)nc * h.N * 8) != hipSuccess) throw Error{H2W_ENOMEM, "GMRES basis allocation failed"};
+    if (h.g_Vcols > 0)
+        H2W_HIP(hipMemcpyAsync(np, h.g_V.p, (size_t)h.g_Vcols * h.N * 8, hipMemcpyDeviceToDevice, h.stream));
+    H2W_HIP(hipStreamSynchronize(h.stream));
+    h.g_V.release();
+    h.g_V.p = np;
+    h.g_V.n = (size_t)nc * h.N;
+    h.g_Vcols = nc;
+}
+
+double dev_norm(Handle &h, const double *v) {
+    int nb = mdot_blocks(h.N);
+    h.g_part.ensure((size_t)nb * 4);
+    h.g_h.ensure(8);
+    launch_mdot(v, h.N, 1, v, h.N, h.g_part.p, nb, h.stream);
+    launch_mdot_reduce(h.g_part.p, nb, 1, h.g_h.p, h.stream);
+    double r = 0.0;
+    H2W_HIP(hipMemcpyAsync(&r, h.g_h.p, 8, hipMemcpyDeviceToHost, h.stream));
+    H2W_HIP(hipStreamSynchronize(h.stream));
+    return std::sqrt(r);
+}
+
+void gen_rot(double dx, double dy, double &cs, double &sn) {
+    if (dy == 0.0) {
+        cs = 1.0;
+        sn = 0.0;
+    } else if (std::fabs(dy) > std::fabs(dx)) {
+        double t = dx / dy;
+        sn = 1.0 / std::sqrt(1.0 + t * t);
+        cs = t * sn;
+    } else {
+        double t = dy / dx;
+        cs = 1.0 / std::sqrt(1.0 + t * t);
+        sn = t * cs;
+    }
+}
+void app_rot(double &dx, double &dy, double cs, double sn) {
+    double t = cs * dx + sn * dy;
+    dy = -sn * dx + cs * dy;
+    dx = t;
+}
+
+// Solves A x = b (tree order, device).  x_zero: x0 = 0 and no initial matvec.
+// normb_in > 0 overrides ||b|| in the stopping test.
+GmresOut gmres_core(Handle &h, const double *b, double *x, int max_iter, double tol, int prec, bool x_zero,
+                    double normb_in) {
+    const int64_t N = h.N;
+    hipStream_t s = h.stream;
+    GmresOut out;
+    h.g_w.ensure(N);
+    h.g_b.ensure(N);  // residual of this solve
+    double *w = h.g_w.p, *r = h.g_b.p;
+    double normb = normb_in > 0 ? normb_in : dev_norm(h, b);
+    if (x_zero) {
+        H2W_HIP(hipMemcpyAsync(r, b, N * 8, hipMemcpyDeviceToDevice, s));
+    } else {
+        matvec_dev(h, x, w, prec, true);
+        out.matvecs++;
+        H2W_HIP(hipMemcpyAsync(r, b, N * 8, hipMemcpyDeviceToDevice, s));
+        launch_axpby(-1.0, w, 1.0, r, N, s);
+    }
+    double beta = dev_norm(h, r);
+    if (normb == 0.0) normb = 1;
+    out.resid = beta / normb;
+    if (out.resid <= tol) {
+        out.conv = true;
+        return out;
+    }
+    const int m = max_iter;
+    std::vector<double> H((size_t)(m + 1) * std::max(m, 1), 0.0), sv(m + 1, 0.0), cs(m + 1, 0.0), sn(m + 1, 0.0);
+    auto Hh = [&](int i, int j) -> double & { return H[(size_t)j * (m + 1) + i]; };
+    ensure_V(h, std::min<int64_t>(m + 1, 32));
+    launch_axpby(1.0 / beta, r, 0.0, h.g_V.p, N, s);
+    sv[0] = beta;
+    const int nb = mdot_blocks(N);
+    h.g_part.ensure((size_t)nb * (m + 2));
+    h.g_h.ensure((size_t)2 * (m + 2) + 2);
+    std::vector<double> hh(2 * (m + 2) + 2);
+    for (int i = 0; i < m; ++i) {
+        ensure_V(h, i + 2);
+        double *V = h.g_V.p;
+        matvec_dev(h, V + (int64_t)i * N, w, prec, true);
+        out.matvecs++;
+        const int k = i + 1;
+        launch_mdot(V, N, k, w, N, h.g_part.p, nb, s);
+        launch_mdot_reduce(h.g_part.p, nb, k, h.g_h.p, s);
+        launch_mgemv_sub(V, N, k, h.g_h.p, w, N, s);
+        launch_mdot(V, N, k, w, N, h.g_part.p, nb, s);
+        launch_mdot_reduce(h.g_part.p, nb, k, h.g_h.p + k, s);
+        launch_mgemv_sub(V, N, k, h.g_h.p + k, w, N, s);
+        launch_mdot(w, N, 1, w, N, h.g_part.p, nb, s);
+        launch_mdot_reduce(h.g_part.p, nb, 1, h.g_h.p + 2 * k, s);
+        H2W_HIP(hipMemcpyAsync(hh.data(), h.g_h.p, (2 * k + 1) * 8, hipMemcpyDeviceToHost, s));
+        H2W_HIP(hipStreamSynchronize(s));
+        for (int q = 0; q < k; ++q) Hh(q, i) = hh[q] + hh[k + q];
+        Hh(i + 1, i) = std::sqrt(hh[2 * k]);
+        launch_axpby(1.0 / Hh(i + 1, i), w, 0.0, V + (int64_t)(i + 1) * N, N, s);
+        for (int q = 0; q < i; ++q) app_rot(Hh(q, i), Hh(q + 1, i), cs[q], sn[q]);
+        gen_rot(Hh(i, i), Hh(i + 1, i), cs[i], sn[i]);
+        app_rot(Hh(i, i), Hh(i + 1, i), cs[i], sn[i]);
+        app_rot(sv[i], sv[i + 1], cs[i], sn[i]);
+        out.resid = std::fabs(sv[i + 1]) / normb;
+        out.iters = i + 1;
+        bool done = out.resid < tol || i == m - 1;
+        if (done) {
+            std::vector<double> y(sv.begin(), sv.begin() + i + 1);
+            for (int q = i; q >= 0; q--) {
+                y[q] /= Hh(q, q);
+                for (int jj = q - 1; jj >= 0; jj--) y[jj] -= Hh(jj, q) * y[q];
+            }
+            H2W_HIP(hipMemcpyAsync(h.g_h.p, y.data(), (i + 1) * 8, hipMemcpyHostToDevice, s));
+            launch_vec_update(V, N, i + 1, h.g_h.p, x, N, s);
+            H2W_HIP(hipStreamSynchronize(s));
+            out.conv = out.resid < tol;
+            return out;
+        }
+    }
+    return out;
+}
+
+}  // namespace
+}  // namespace h2w
+
+// ------------------------------------------------------------------ C ABI
+using h2w::Error;
+
+#define H2W_TRY try {
+#define H2W_CATCH                                         \
+    }                                                     \
+    catch (const Error &e) {                              \
+        h2w::set_error(e.msg);                            \
+        return e.code;                                    \
+    }                                                     \
+    catch (const std::bad_alloc &) {                      \
+        h2w::set_error("host allocation failed");         \
+        return H2W_ENOMEM;                                \
+    }                                                     \
+    catch (const std::exception &e) {                     \
+        h2w::set_error(e.what());                         \
+        return H2W_EINVAL;                                \
+    }
+
+extern "C" {
+
+void h2w_default_opts(h2w_opts *o) {
+    if (!o) return;
+    std::memset(o, 0, sizeof(*o));
+    o->compat = 1;
+    o->nlevels = 0;
+    o->fp32_shadow = 1;
+    o->device = 0;
+    o->rank_cap = 128;
+    o->workspace_bytes = 0;
+    o->plan_only = 0;
+}
+
+const char *h2w_last_error(void) { return h2w::g_err.c_str(); }
+
+int h2w_create(int dim, int variant, int64_t N, const double *points, int n_max, double L, int tol_pow, int choice,
+               const h2w_opts *opts, h2w_handle **out) {
+    H2W_TRY
+    if (!out) throw Error{H2W_EINVAL, "out is NULL"};
+    *out = nullptr;
+    if (dim < 1 || dim > 3) throw Error{H2W_EINVAL, "dim must be 1, 2 or 3"};
+    if (variant != H2W_NESTED && variant != H2W_SEMI_NESTED) throw Error{H2W_EINVAL, "unknown variant"};
+    if (choice != 0 && choice != 1) throw Error{H2W_EINVAL, "Wrong choice! choice must be 0 (IE) or 1 (RBF)"};
+    if (N < 2 || N > INT32_MAX) throw Error{H2W_EINVAL, "N out of range"};
+    if (n_max < 1) throw Error{H2W_EINVAL, "n_max must be positive"};
+    if (!(L > 0.0)) throw Error{H2W_EINVAL, "L must be positive"};
+    if (tol_pow < 0 || tol_pow > 20) throw Error{H2W_EINVAL, "tol_pow out of range"};
+    h2w_opts o;
+    h2w_default_opts(&o);
+    if (opts) o = *opts;
+    int ndev = 0;
+    if (!o.plan_only) {
+        if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+            throw Error{H2W_ENOLIB, "no HIP device visible; libh2w has no CPU fallback"};
+        if (o.device < 0 || o.device >= ndev) throw Error{H2W_EINVAL, "device ordinal out of range"};
+    }
+    std::vector<double> grid;
+    const bool is_grid = points == nullptr;
+    if (is_grid) {
+        // the reference's uniform grids (1D/codes/HODLR1Dtree.hpp:59-65, 2D/codes/HODLR2Dtree.hpp:75-91,
+        // 3D/codes/HODLR3Dtree.hpp:82-102)
+        grid.assign((size_t)dim * N, 0.0);
+        if (dim == 1) {
+            for (int64_t k = 0; k < N; ++k) grid[k] = -1.0 + 2.0 * (k + 1.0) / (N + 1.0);
+        } else {
+            int64_t n = std::llround(dim == 2 ? std::sqrt((double)N) : std::cbrt((double)N));
+            if ((dim == 2 && n * n != N) || (dim == 3 && n * n * n != N))
+                throw Error{H2W_EINVAL, "the uniform grid needs N to be a square (2D) or a cube (3D)"};
+            std::vector<double> nd(n);
+            for (int64_t k = 0; k < n; ++k) nd[k] = -L + 2.0 * L * (k + 1.0) / (n + 1.0);
+            if (dim == 2) {
+                for (int64_t j = 0; j < n; ++j)
+                    for (int64_t k = 0; k < n; ++k) {
+                        grid[j * n + k] = nd[k];
+                        grid[N + j * n + k] = nd[j];
+                    }
+            } else {
+                for (int64_t j = 0; j < n; ++j)
+                    for (int64_t k = 0; k < n; ++k)
+                        for (int64_t i = 0; i < n; ++i) {
+                            int64_t p = (j * n + k) * n + i;
+                            grid[p] = nd[k];
+                            grid[N + p] = nd[j];
+                            grid[2 * N + p] = nd[i];
+                        }
+            }
+        }
+        points = grid.data();
+    }
+    int nl = o.nlevels > 0 ? o.nlevels : h2w::ref_nlevels(dim, N, n_max, is_grid);
+    if (nl < 1) throw Error{H2W_EINVAL, "nLevels < 1: N must exceed n_max"};
+    if (nl > (dim == 1 ? 24 : (dim == 2 ? 12 : 8))) throw Error{H2W_EINVAL, "tree too deep"};
+    std::unique_ptr<h2w_handle> h(new h2w_handle());
+    h->dim = dim;
+    h->variant = variant;
+    h->choice = choice;
+    h->tol_pow = tol_pow;
+    h->n_max = n_max;
+    h->N = N;
+    h->L = L;
+    h->opts = o;
+    h->plan.build(dim, N, points, L, nl);
+    h->diag = h2w::diag_value(dim, choice, N, o.compat);
+    if (o.plan_only) {
+        *out = h.release();
+        return H2W_OK;
+    }
+    H2W_HIP(hipSetDevice(o.device));
+    H2W_HIP(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+    h->d_px.upload(h->plan.px, h->stream);
+    h->d_py.upload(h->plan.py, h->stream);
+    h->d_pz.upload(h->plan.pz, h->stream);
+    h->d_perm.upload(h->plan.perm, h->stream);
+    H2W_HIP(hipStreamSynchronize(h->stream));
+    h->kp.dim = dim;
+    h->kp.choice = choice;
+    h->kp.diag = h->diag;
+    h->kp.px = h->d_px.p;
+    h->kp.py = h->d_py.p;
+    h->kp.pz = h->d_pz.p;
+    *out = h.release();
+    return H2W_OK;
+    H2W_CATCH
+}
+
+int h2w_assemble(h2w_handle *h) {
+    H2W_TRY
+    if (!h) throw Error{H2W_EINVAL, "NULL handle"};
+    if (h->opts.plan_only) throw Error{H2W_ESTATE, "handle was created with plan_only"};
+    H2W_HIP(hipSetDevice(h->opts.device));
+    h2w::assemble(*h);
+    return H2W_OK;
+    H2W_CATCH
+}
+
+int h2w_matvec_ex(h2w_handle *h, const double *x, double *y, int nrhs, int64_t ld, int precision, int order,
+                  int where) {
+    H2W_TRY
+    if (!h || !x || !y) throw Error{H2W_EINVAL, "NULL argument"};
+    if (!h->assembled) throw Error{H2W_ESTATE, "h2w_matvec before h2w_assemble"};
+    if (nrhs < 1 || ld < h->N) throw Error{H2W_EINVAL, "bad nrhs / ld"};
+    if (precision != H2W_FP64 && precision != H2W_FP32) throw Error{H2W_EINVAL, "bad precision"};
+    H2W_HIP(hipSetDevice(h->opts.device));
+    const int64_t N = h->N;
+    for (int r = 0; r < nrhs; ++r) {
+        const double *xr = x + r * ld;
+        double *yr = y + r * ld;
+        if (where == H2W_HOST) {
+            h->d_stage_in.ensure(N);
+            h->d_stage_out.ensure(N);
+            H2W_HIP(hipMemcpyAsync(h->d_stage_in.p, xr, N * 8, hipMemcpyHostToDevice, h->stream));
+            h2w::matvec_dev(*h, h->d_stage_in.p, h->d_stage_out.p, precision, order == H2W_ORDER_TREE);
+            H2W_HIP(hipMemcpyAsync(yr, h->d_stage_out.p, N * 8, hipMemcpyDeviceToHost, h->stream));
+        } else {
+            h2w::matvec_dev(*h, xr, yr, precision, order == H2W_ORDER_TREE);
+        }
+    }
+    H2W_HIP(hipStreamSynchronize(h->stream));
+    return H2W_OK;
+    H2W_CATCH
+}
+
+int h2w_matvec(h2w_handle *h, const double *x, double *y, int where) {
+    return h2w_matvec_ex(h, x, y, 1, h ? h->N : 0, H2W_FP64, H2W_ORDER_INPUT, where);
+}
+
+int h2w_gmres(h2w_handle *h, const double *b, double *x, int max_iter, double tol, int mode, int where,
+              h2w_gmres_stats *st) {
+    H2W_TRY
+    if (!h || !b || !x) throw Error{H2W_EINVAL, "NULL argument"};
+    if (!h->assembled) throw Error{H2W_ESTATE, "h2w_gmres before h2w_assemble"};
+    if (max_iter < 1) throw Error{H2W_EINVAL, "max_iter must be positive"};
+    if (mode != H2W_GMRES_FP64 && mode != H2W_GMRES_FP32_IR) throw Error{H2W_EINVAL, "bad mode"};
+    if (mode == H2W_GMRES_FP32_IR && !h->mat32.p) throw Error{H2W_ESTATE, "FP32 shadow not built"};
+    H2W_HIP(hipSetDevice(h->opts.device));
+    const int64_t N = h->N;
+    hipStream_t s = h->stream;
+    h->g_x.ensure(N);
+    h->g_r.ensure(N);
+    h->g_d.ensure(N);
+    h->d_stage_in.ensure(N);
+    h->d_stage_out.ensure(N);
+    // b and x0 to tree order
+    const double *bsrc = b, *xsrc = x;
+    if (where == H2W_HOST) {
+        H2W_HIP(hipMemcpyAsync(h->d_stage_in.p, b, N * 8, hipMemcpyHostToDevice, s));
+        H2W_HIP(hipMemcpyAsync(h->d_stage_out.p, x, N * 8, hipMemcpyHostToDevice, s));
+        bsrc = h->d_stage_in.p;
+        xsrc = h->d_stage_out.p;
+    }
+    h2w::DevBuf<double> bt;
+    bt.alloc(N);
+    h2w::launch_permute_in(bsrc, N, bt.p, N, h->d_perm.p, N, 1, s);
+    h2w::launch_permute_in(xsrc, N, h->g_x.p, N, h->d_perm.p, N, 1, s);
+    H2W_HIP(hipStreamSynchronize(s));
+    double t0 = h2w::now_s();
+    h2w_gmres_stats S;
+    std::memset(&S, 0, sizeof(S));
+    if (mode == H2W_GMRES_FP64) {
+        auto o = h2w::gmres_core(*h, bt.p, h->g_x.p, max_iter, tol, H2W_FP64, false, -1.0);
+        S.iterations = o.iters;
+        S.resid = o.resid;
+        S.matvecs_fp64 = o.matvecs;
+        S.converged = o.conv;
+    } else {
+        // FP64 iterative refinement: r = b - A x in FP64, correction from GMRES on the FP32 operator
+        double normb = h2w::dev_norm(*h, bt.p);
+        if (normb == 0.0) normb = 1.0;
+        int total = 0, outer = 0;
+        double rn = 0.0;
+        while (true) {
+            h2w::matvec_dev(*h, h->g_x.p, h->g_d.p, H2W_FP64, true);
+            S.matvecs_fp64++;
+            H2W_HIP(hipMemcpyAsync(h->g_r.p, bt.p, N * 8, hipMemcpyDeviceToDevice, s));
+            h2w::launch_axpby(-1.0, h->g_d.p, 1.0, h->g_r.p, N, s);
+            double nr = h2w::dev_norm(*h, h->g_r.p);
+            rn = nr / normb;
+            if (rn <= tol || total >= max_iter || outer >= 50) break;
+            double eta = std::min(1e-2, std::max(1e-6, 0.1 * tol / rn));
+            h2w::launch_fill(h->g_d.p, N, 0.0, s);
+            auto o = h2w::gmres_core(*h, h->g_r.p, h->g_d.p, max_iter - total, eta, H2W_FP32, true, nr);
+            total += o.iters;
+            S.matvecs_fp32 += o.matvecs;
+            h2w::launch_axpby(1.0, h->g_d.p, 1.0, h->g_x.p, N, s);
+            ++outer;
+            if (o.iters == 0) break;
+        }
+        S.iterations = total;
+        S.outer_iterations = outer;
+        S.resid = rn;
+        S.converged = rn <= tol;
+    }
+    H2W_HIP(hipStreamSynchronize(s));
+    S.seconds = h2w::now_s() - t0;
+    double *xdst = where == H2W_HOST ? h->d_stage_out.p : x;
+    h2w::launch_permute_out(h->g_x.p, N, xdst, N, h->d_perm.p, N, 1, s);
+    if (where == H2W_HOST) H2W_HIP(hipMemcpyAsync(x, h->d_stage_out.p, N * 8, hipMemcpyDeviceToHost, s));
+    H2W_HIP(hipStreamSynchronize(s));
+    if (st) *st = S;
+    return S.converged ? H2W_OK : H2W_ENOCONV;
+    H2W_CATCH
+}
+
+int h2w_dense_matvec(h2w_handle *h, const double *x, double *y, const int64_t *rows, int64_t nrows, int where) {
+    H2W_TRY
+    if (!h || !x || !y) throw Error{H2W_EINVAL, "NULL argument"};
+    if (h->opts.plan_only) throw Error{H2W_ESTATE, "handle was created with plan_only"};
+    H2W_HIP(hipSetDevice(h->opts.device));
+    const int64_t N = h->N;
+    hipStream_t s = h->stream;
+    if (!rows) nrows = N;
+    if (nrows < 0) throw Error{H2W_EINVAL, "nrows < 0"};
+    // inverse permutation: input index -> tree index
+    std::vector<int32_t> tr(nrows);
+    {
+        std::vector<int32_t> inv(N);
+        for (int64_t t = 0; t < N; ++t) inv[h->plan.perm[t]] = (int32_t)t;
+        for (int64_t q = 0; q < nrows; ++q) {
+            int64_t i = rows ? rows[q] : q;
+            if (i < 0 || i >= N) throw Error{H2W_EINVAL, "row index out of range"};
+            tr[q] = inv[i];
+        }
+    }
+    h2w::DevBuf<int32_t> drows;
+    drows.upload(tr, s);
+    h2w::DevBuf<double> xin, xt, yo;
+    xt.alloc(N);
+    yo.alloc(std::max<int64_t>(nrows, 1));
+    const double *xs = x;
+    if (where == H2W_HOST) {
+        xin.alloc(N);
+        H2W_HIP(hipMemcpyAsync(xin.p, x, N * 8, hipMemcpyHostToDevice, s));
+        xs = xin.p;
+    }
+    h2w::launch_permute_in(xs, N, xt.p, N, h->d_perm.p, N, 1, s);
+    h2w::launch_dense_rows(drows.p, nrows, xt.p, yo.p, N, h->kp, s);
+    H2W_HIP(hipGetLastError());
+    H2W_HIP(hipMemcpyAsync(y, yo.p, nrows * 8, where == H2W_HOST ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice, s));
+    H2W_HIP(hipStreamSynchronize(s));
+    return H2W_OK;
+    H2W_CATCH
+}
+
+int h2w_get_stats(const h2w_handle *h, h2w_stats *st) {
+    if (!h || !st) {
+        h2w::set_error("NULL argument");
+        return H2W_EINVAL;
+    }
+    std::memset(st, 0, sizeof(*st));
+    st->dim = h->dim;
+    st->variant = h->variant;
+    st->choice = h->choice;
+    st->nlevels = h->plan.nL;
+    st->N = h->N;
+    st->memory = h->memory;
+    st->diag = h->diag;
+    st->assembly_seconds = h->assembly_seconds;
+    st->aca_reruns = h->aca_reruns;
+    for (int f = 0; f < 2; ++f) {
+        int mx = 0;
+        double sum = 0.0;
+        int64_t cnt = 0;
+        for (auto &fl : h->fam[f])
+            for (int r : fl.rank) {
+                mx = std::max(mx, r);
+                sum += r;
+                ++cnt;
+            }
+        if (f == 1)
+            for (auto &p : h->pairs) {
+                mx = std::max(mx, p.rank);
+                sum += p.rank;
+                ++cnt;
+            }
+        if (f == 0) {
+            st->max_rank_far = mx;
+            st->mean_rank_far = cnt ? sum / cnt : 0.0;
+        } else {
+            st->max_rank_vertex = mx;
+            st->mean_rank_vertex = cnt ? sum / cnt : 0.0;
+        }
+    }
+    st->stored_bytes_fp64 = h->stored_elems * 8;
+    st->stored_bytes_fp32 = h->mat32.p ? h->stored_elems * 4 : 0;
+    st->n_launches_matvec = (int)h->launches.size() + 3;
+    return H2W_OK;
+}
+
+int h2w_tree_order(const h2w_handle *h, int32_t *perm) {
+    if (!h || !perm) {
+        h2w::set_error("NULL argument");
+        return H2W_EINVAL;
+    }
+    std::copy(h->plan.perm.begin(), h->plan.perm.end(), perm);
+    return H2W_OK;
+}
+
+int h2w_box_skeleton(const h2w_handle *h, int family, int level, int box, int32_t *ir, int32_t *ic, int cap) {
+    if (!h || family < 0 || family > 1 || level < 1 || level > h->plan.nL || box < 0 ||
+        box >= h->plan.nboxes[level]) {
+        h2w::set_error("bad skeleton query");
+        return -H2W_EINVAL;
+    }
+    if (!h->assembled) {
+        h2w::set_error("not assembled");
+        return -H2W_ESTATE;
+    }
+    const auto &F = h->fam[family];
+    if ((int)F.size() <= level || F[level].rank.empty()) return 0;
+    const auto &fl = F[level];
+    int r = fl.rank[box];
+    for (int q = 0; q < r && q < cap; ++q) {
+        if (ir) ir[q] = h->plan.perm[fl.Ir[box][q]];
+        if (ic) ic[q] = h->plan.perm[fl.Ic[box][q]];
+    }
+    return r;
+}
+
+int h2w_interaction_list(const h2w_handle *h, int level, int box, int32_t *il, int32_t *isv, int cap) {
+    if (!h || level < 1 || level > h->plan.nL || box < 0 || box >= h->plan.nboxes[level]) {
+        h2w::set_error("bad list query");
+        return -H2W_EINVAL;
+    }
+    const auto &l = h->plan.il[level][box];
+    const auto &v = h->plan.ilv[level][box];
+    for (size_t q = 0; q < l.size() && (int)q < cap; ++q) {
+        if (il) il[q] = l[q];
+        if (isv) isv[q] = v[q];
+    }
+    return (int)l.size();
+}
+
+int h2w_near_list(const h2w_handle *h, int level, int box, int32_t *nb, int cap) {
+    if (!h || level != h->plan.nL || box < 0 || box >= h->plan.nboxes[level]) {
+        h2w::set_error("near lists exist at the leaf level only");
+        return -H2W_EINVAL;
+    }
+    const auto &l = h->plan.near[box];
+    for (size_t q = 0; q < l.size() && (int)q < cap; ++q)
+        if (nb) nb[q] = l[q];
+    return (int)l.size();
+}
+
+int h2w_bench_matvec(h2w_handle *h, int nrep, int precision, h2w_bench *out) {
+    H2W_TRY
+    if (!h || !out || nrep < 1) throw Error{H2W_EINVAL, "bad argument"};
+    if (!h->assembled) throw Error{H2W_ESTATE, "not assembled"};
+    if (precision == H2W_FP32 && !h->mat32.p) throw Error{H2W_ESTATE, "FP32 shadow not built"};
+    H2W_HIP(hipSetDevice(h->opts.device));
+    std::memset(out, 0, sizeof(*out));
+    hipStream_t s = h->stream;
+    const int64_t N = h->N;
+    // deterministic input in the pool's x segment
+    h2w::launch_fill(h->vec64.p + h->off_x, N, 1.0, s);
+    h2w::launch_fill32(h->vec32.p + h->off_x, N, 1.0f, s);
+    const int nl = std::min<int>((int)h->launches.size(), H2W_MAX_LAUNCHES - 1);
+    std::vector<hipEvent_t> ev(nl + 2);
+    for (auto &e : ev) H2W_HIP(hipEventCreate(&e));
+    std::vector<double> acc(nl + 1, 0.0);
+    // warm-up
+    h2w::run_launches(*h, precision);
+    H2W_HIP(hipStreamSynchronize(s));
+    hipEvent_t t0, t1;
+    H2W_HIP(hipEventCreate(&t0));
+    H2W_HIP(hipEventCreate(&t1));
+    H2W_HIP(hipEventRecord(t0, s));
+    for (int r = 0; r < nrep; ++r) h2w::run_launches(*h, precision);
+    H2W_HIP(hipEventRecord(t1, s));
+    H2W_HIP(hipEventSynchronize(t1));
+    float ms = 0.f;
+    H2W_HIP(hipEventElapsedTime(&ms, t0, t1));
+    out->ms_per_matvec = ms / nrep;
+    // per-launch durations: fill + each bgemv launch
+    for (int r = 0; r < nrep; ++r) {
+        H2W_HIP(hipEventRecord(ev[0], s));
+        if (precision == H2W_FP32) h2w::launch_fill32(h->vec32.p + h->off_y, h->vec_len - h->off_y, 0.0f, s);
+        else h2w::launch_fill(h->vec64.p + h->off_y, h->vec_len - h->off_y, 0.0, s);
+        H2W_HIP(hipEventRecord(ev[1], s));
+        for (int q = 0; q < nl; ++q) {
+            auto &Lh = h->launches[q];
+            if (precision == H2W_FP32)
+                h2w::launch_bgemv32(h->d_items.p + Lh.item0, (int)Lh.nitems, h->d_terms.p, h->mat32.p, h->vec32.p, s);
+            else
+                h2w::launch_bgemv64(h->d_items.p + Lh.item0, (int)Lh.nitems, h->d_terms.p, h->mat64.p, h->vec64.p, s);
+            H2W_HIP(hipEventRecord(ev[q + 2], s));
+        }
+        H2W_HIP(hipEventSynchronize(ev[nl + 1]));
+        for (int q = 0; q <= nl; ++q) {
+            float m2 = 0.f;
+            H2W_HIP(hipEventElapsedTime(&m2, ev[q], ev[q + 1]));
+            acc[q] += m2;
+        }
+    }
+    out->n_launch = nl + 1;
+    const int es = precision == H2W_FP32 ? 4 : 8;
+    out->launch_ms[0] = acc[0] / nrep;
+    out->launch_bytes[0] = (h->vec_len - h->off_y) * (int64_t)es;
+    for (int q = 0; q < nl; ++q) {
+        out->launch_ms[q + 1] = acc[q + 1] / nrep;
+        auto &Lh = h->launches[q];
+        (void)Lh;
+        out->launch_bytes[q + 1] = h->launch_elems[q] * es;
+    }
+    for (auto &e : ev) (void)hipEventDestroy(e);
+    (void)hipEventDestroy(t0);
+    (void)hipEventDestroy(t1);
+    return H2W_OK;
+    H2W_CATCH
+}
+
+void h2w_destroy(h2w_handle *h) {
+    if (!h) return;
+    if (!h->opts.plan_only) (void)hipSetDevice(h->opts.device);
+    if (h->stream) (void)hipStreamSynchronize(h->stream);
+    if (h->stream) (void)hipStreamDestroy(h->stream);
+    delete h;
+}
+
+}  // extern "C"

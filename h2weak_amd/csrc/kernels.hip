@@ -1,0 +1,696 @@
+// kernels.hip -- gfx950 kernels of libh2w.
+//
+// This file is compiled with -ffp-contract=off so that the kernel entries
+// and the ACA residual updates round like the reference's scalar code
+// (no fused multiply-add); the streaming GEMV and BLAS-1 loops ask for fused
+// multiply-adds explicitly with fma().
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+#include <climits>
+
+#include "h2w_internal.hpp"
+
+namespace h2w {
+
+namespace {
+
+constexpr double kPI = 3.141592653589793238462643383279502884;
+
+// ------------------------------------------------------------------ kernel entry
+// A(i,j) of the reference's userkernel::getMatrixEntry for choice 0 (IE) and
+// 1 (RBF): 1D/codes/kernel.hpp:108-185, 2D/codes/kernel.hpp:118-228,299-318,
+// 3D/codes/kernel.hpp:123-192.  Off-diagonal entries depend on the pair's
+// distance; i == j returns the diagonal rule (kp.diag).
+// Flop weights (add/mul = 1, sqrt/log/div = 1 each): 3D IE 9, 3D RBF 10.
+__device__ __forceinline__ double koff(int dim, int choice, double dx, double dy, double dz) {
+    const double a = 0.0001;
+    if (dim == 1) {
+        double R = fabs(dx);
+        if (choice == 0) return log(R);
+        return R < a ? R / a : a / R;
+    } else if (dim == 2) {
+        double R2 = dx * dx + dy * dy;
+        if (choice == 0) return (-1.0 / (2 * kPI)) * 0.5 * log(R2);
+        double R = sqrt(R2);
+        return R < a ? R / a : a / R;
+    } else {
+        double R2 = dx * dx + dy * dy + dz * dz;
+        double R = sqrt(R2);
+        if (choice == 0) return 1.0 / (4.0 * kPI * R);
+        if (R < a) return (R * log(R) - 1.0) / (a * log(a) - 1.0);
+        return (1.0 * log(R)) / log(a);
+    }
+}
+
+struct PointI {
+    double x, y, z;
+    int i;
+};
+__device__ __forceinline__ PointI load_point(const KParams &kp, int i) {
+    PointI p;
+    p.i = i;
+    p.x = kp.px[i];
+    p.y = kp.dim > 1 ? kp.py[i] : 0.0;
+    p.z = kp.dim > 2 ? kp.pz[i] : 0.0;
+    return p;
+}
+__device__ __forceinline__ double kentry(const KParams &kp, const PointI &p, int j) {
+    if (p.i == j) return kp.diag;
+    double dx = p.x - kp.px[j];
+    double dy = kp.dim > 1 ? p.y - kp.py[j] : 0.0;
+    double dz = kp.dim > 2 ? p.z - kp.pz[j] : 0.0;
+    return koff(kp.dim, kp.choice, dx, dy, dz);
+}
+__device__ __forceinline__ double kentry2(const KParams &kp, int i, int j) {
+    return kentry(kp, load_point(kp, i), j);
+}
+
+// ------------------------------------------------------------------ reductions (wave64)
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+// argmax of |v| with first-index tie break; (a = |v|, idx, sv = signed v)
+struct AMax {
+    double a;
+    int idx;
+    double sv;
+};
+__device__ __forceinline__ void amax_merge(AMax &x, const AMax &y) {
+    if (y.a > x.a || (y.a == x.a && y.idx < x.idx)) x = y;
+}
+__device__ __forceinline__ AMax wave_amax(AMax v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        AMax w;
+        w.a = __shfl_xor(v.a, o, 64);
+        w.idx = __shfl_xor(v.idx, o, 64);
+        w.sv = __shfl_xor(v.sv, o, 64);
+        amax_merge(v, w);
+    }
+    return v;
+}
+__device__ __forceinline__ void amax_update(AMax &m, double v, int idx) {
+    double a = fabs(v);
+    if (a > m.a || (a == m.a && idx < m.idx)) {
+        m.a = a;
+        m.idx = idx;
+        m.sv = v;
+    }
+}
+
+template <int NT>
+struct BlockRed {
+    static constexpr int NW = NT / 64;
+    double s0[NW], s1[NW];
+    double a[NW], sv[NW];
+    int idx[NW];
+};
+
+// block-wide (sum, sum, argmax); all threads receive the result
+template <int NT>
+__device__ __forceinline__ void block_reduce(BlockRed<NT> &sh, double &s0, double &s1, AMax &m) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    s0 = wave_sum(s0);
+    s1 = wave_sum(s1);
+    m = wave_amax(m);
+    __syncthreads();
+    if (lane == 0) {
+        sh.s0[w] = s0;
+        sh.s1[w] = s1;
+        sh.a[w] = m.a;
+        sh.idx[w] = m.idx;
+        sh.sv[w] = m.sv;
+    }
+    __syncthreads();
+    s0 = 0.0;
+    s1 = 0.0;
+    m.a = -1.0;
+    m.idx = INT_MAX;
+    m.sv = 0.0;
+    for (int q = 0; q < BlockRed<NT>::NW; ++q) {
+        s0 += sh.s0[q];
+        s1 += sh.s1[q];
+        AMax o{sh.a[q], sh.idx[q], sh.sv[q]};
+        amax_merge(m, o);
+    }
+}
+
+// ------------------------------------------------------------------ batched ACA
+// One workgroup per block.  Restates LowRank::ACA_only_nodes / ACA_only_nodesCUR
+// (2D/codes/aca.hpp:158-446) and ACA_only_nodesUV (1D/codes/aca.hpp:40-156),
+// which share the pivot loop:
+//   - first row (in order) whose max |entry| exceeds 1e-36 starts the cross,
+//   - pivot column = first argmax of |residual row| over unused columns,
+//   - u = residual column / pivot, next row = first argmax of |residual column|
+//     over unused rows,
+//   - stop when |u||v| <= tol * normS or rank = min(m, n), or when |u| or |v|
+//     falls below 1e-36 (that step is dropped),
+//   - normS^2 += (|u||v|)^2 + 2 sum_{l<=new} |<U_l,u><V_l,v>| (the sum includes
+//     the newest term, as the reference's loop does, 2D/codes/aca.hpp:404-409).
+// Residual rows/columns subtract the previous terms one by one in rank order
+// with separate multiply and subtract, as the reference does.
+// U (m x rcap) and V (n x rcap) live in global workspace; pivots go to
+// rowpiv/colpiv.  status = 1 when the block reached rcap before converging
+// (the host re-runs it with a larger cap).
+constexpr int kChunk = 512;
+
+template <int NT>
+__global__ __launch_bounds__(NT) void k_aca(const AcaJob *__restrict__ jobs, const int32_t *__restrict__ idx,
+                                            double *__restrict__ ws, uint8_t *__restrict__ flags,
+                                            int32_t *__restrict__ rank_out, int32_t *__restrict__ status_out,
+                                            int32_t *__restrict__ rowpiv, int32_t *__restrict__ colpiv, KParams kp,
+                                            double tol) {
+    const AcaJob jb = jobs[blockIdx.x];
+    const int m = jb.m, n = jb.n, rcap = jb.rcap;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    constexpr int NW = NT / 64;
+    const int32_t *rows = idx + jb.rows;
+    const int32_t *cols = idx + jb.cols;
+    double *U = ws + jb.wsU;
+    double *V = ws + jb.wsV;
+    uint8_t *ru = flags + jb.wsF;
+    uint8_t *cu = ru + m;
+    int32_t *rp_out = rowpiv + jb.piv;
+    int32_t *cp_out = colpiv + jb.piv;
+    __shared__ double coef[kChunk];
+    __shared__ BlockRed<NT> red;
+    __shared__ double dacc[NW];
+
+    if (m == 0 || n == 0 || rcap == 0) {
+        if (tid == 0) {
+            rank_out[blockIdx.x] = 0;
+            status_out[blockIdx.x] = 0;
+        }
+        return;
+    }
+    for (int i = tid; i < m; i += NT) ru[i] = 0;
+    for (int s = tid; s < n; s += NT) cu[s] = 0;
+    const int mn = min(m, n);
+
+    // ---- first row with a nonzero entry
+    int l = 0, cp = -1;
+    double piv = 0.0;
+    for (l = 0; l < m; ++l) {
+        PointI pr = load_point(kp, rows[l]);
+        AMax am{-1.0, INT_MAX, 0.0};
+        for (int s = tid; s < n; s += NT) {
+            double v = kentry(kp, pr, cols[s]);
+            V[s] = v;
+            amax_update(am, v, s);
+        }
+        double d0 = 0.0, d1 = 0.0;
+        block_reduce<NT>(red, d0, d1, am);
+        if (am.idx != INT_MAX && am.a > 1e-36) {
+            cp = am.idx;
+            piv = am.sv;
+            break;
+        }
+    }
+    if (l == m) {
+        if (tid == 0) {
+            rank_out[blockIdx.x] = 0;
+            status_out[blockIdx.x] = 0;
+        }
+        return;
+    }
+    int rp = l;
+    // ---- first column
+    double un2 = 0.0, vn2 = 0.0;
+    AMax an{-1.0, INT_MAX, 0.0};
+    {
+        PointI pc = load_point(kp, cols[cp]);
+        for (int t = tid; t < m; t += NT) {
+            double c = kentry(kp, pc, rows[t]);  // A(rows[t], cols[cp]) (symmetric entry)
+            double u = c / piv;
+            U[t] = u;
+            un2 += u * u;
+            if (t != rp) amax_update(an, c, t);
+        }
+        for (int s = tid; s < n; s += NT) vn2 += V[s] * V[s];
+    }
+    block_reduce<NT>(red, un2, vn2, an);
+    if (tid == 0) {
+        rp_out[0] = rp;
+        cp_out[0] = cp;
+        ru[rp] = 1;
+        cu[cp] = 1;
+    }
+    int rank = 1;
+    int status = 0;
+    double normS = 0.0;
+    double un = sqrt(un2), vn = sqrt(vn2);
+    int next_rp = an.idx;
+    __syncthreads();
+
+    while (un * vn > tol * normS && rank < mn) {
+        if (rank >= rcap) {
+            status = 1;
+            break;
+        }
+        rp = next_rp;
+        if (rp == INT_MAX) break;
+        const int nch = (rank + kChunk - 1) / kChunk;
+        // ---- residual row -> V[:, rank]
+        double *Vr = V + (int64_t)rank * n;
+        PointI pr = load_point(kp, rows[rp]);
+        AMax am{-1.0, INT_MAX, 0.0};
+        vn2 = 0.0;
+        for (int ch = 0; ch < nch; ++ch) {
+            const int l0 = ch * kChunk, l1 = min(rank, l0 + kChunk);
+            for (int q = l0 + tid; q < l1; q += NT) coef[q - l0] = U[(int64_t)q * m + rp];
+            __syncthreads();
+            for (int s = tid; s < n; s += NT) {
+                double r = ch == 0 ? kentry(kp, pr, cols[s]) : Vr[s];
+                for (int q = l0; q < l1; ++q) r -= coef[q - l0] * V[(int64_t)q * n + s];
+                Vr[s] = r;
+                if (ch == nch - 1) {
+                    vn2 += r * r;
+                    if (!cu[s]) amax_update(am, r, s);
+                }
+            }
+            __syncthreads();
+        }
+        double dummy = 0.0;
+        block_reduce<NT>(red, vn2, dummy, am);
+        if (am.idx == INT_MAX) break;
+        cp = am.idx;
+        piv = am.sv;
+        // ---- residual column -> U[:, rank]
+        double *Ur = U + (int64_t)rank * m;
+        PointI pc = load_point(kp, cols[cp]);
+        an = AMax{-1.0, INT_MAX, 0.0};
+        un2 = 0.0;
+        for (int ch = 0; ch < nch; ++ch) {
+            const int l0 = ch * kChunk, l1 = min(rank, l0 + kChunk);
+            for (int q = l0 + tid; q < l1; q += NT) coef[q - l0] = V[(int64_t)q * n + cp];
+            __syncthreads();
+            for (int t = tid; t < m; t += NT) {
+                double c = ch == 0 ? kentry(kp, pc, rows[t]) : Ur[t];
+                for (int q = l0; q < l1; ++q) c -= coef[q - l0] * U[(int64_t)q * m + t];
+                if (ch == nch - 1) {
+                    double u = c / piv;
+                    Ur[t] = u;
+                    un2 += u * u;
+                    if (t != rp && !ru[t]) amax_update(an, c, t);
+                } else {
+                    Ur[t] = c;
+                }
+            }
+            __syncthreads();
+        }
+        block_reduce<NT>(red, un2, dummy, an);
+        un = sqrt(un2);
+        vn = sqrt(vn2);
+        if (un < 1e-36 || vn < 1e-36) break;
+        if (tid == 0) {
+            rp_out[rank] = rp;
+            cp_out[rank] = cp;
+            ru[rp] = 1;
+            cu[cp] = 1;
+        }
+        ++rank;
+        next_rp = an.idx;
+        // ---- normS update: dots of the new term with every term
+        const double *un_ = U + (int64_t)(rank - 1) * m;
+        const double *vn_ = V + (int64_t)(rank - 1) * n;
+        double acc = 0.0;
+        for (int q = wv; q < rank; q += NW) {
+            const double *Uq = U + (int64_t)q * m;
+            const double *Vq = V + (int64_t)q * n;
+            double du = 0.0, dv = 0.0;
+            for (int t = lane; t < m; t += 64) du += Uq[t] * un_[t];
+            for (int s = lane; s < n; s += 64) dv += Vq[s] * vn_[s];
+            du = wave_sum(du);
+            dv = wave_sum(dv);
+            acc += fabs(du * dv);
+        }
+        if (lane == 0) dacc[wv] = acc;
+        __syncthreads();
+        double S = 0.0;
+        for (int q = 0; q < NW; ++q) S += dacc[q];
+        double uv = un * vn;
+        normS = sqrt(normS * normS + uv * uv + 2.0 * S);
+        __syncthreads();
+    }
+    if (tid == 0) {
+        rank_out[blockIdx.x] = rank;
+        status_out[blockIdx.x] = status;
+    }
+}
+
+// L (unit lower) and R (upper) of the cross: L(i,j) = U_j[rb_i], R(j,i) = V_j[cb_i]
+// (2D/codes/aca.hpp:419-445)
+__global__ void k_lr_extract(const LrJob *__restrict__ jobs, const double *__restrict__ ws,
+                             const int32_t *__restrict__ rowpiv, const int32_t *__restrict__ colpiv,
+                             double *__restrict__ out) {
+    const LrJob jb = jobs[blockIdx.x];
+    const int r = jb.rank;
+    double *L = out + jb.out;
+    double *R = L + (int64_t)r * r;
+    const double *U = ws + jb.wsU;
+    const double *V = ws + jb.wsV;
+    for (int e = threadIdx.x; e < r * r; e += blockDim.x) {
+        int i = e % r, j = e / r;  // row i, column j
+        double lv = 0.0, rv = 0.0;
+        if (i == j) lv = 1.0;
+        else if (j < i) lv = U[(int64_t)j * jb.m + rowpiv[jb.piv + i]];
+        if (i <= j) rv = V[(int64_t)i * jb.n + colpiv[jb.piv + j]];
+        L[e] = lv;
+        R[e] = rv;
+    }
+}
+
+// A = K(rows, cols), column-major (getMatrix, 3D/codes/kernel.hpp:58-71)
+__global__ void k_eval(const EvalJob *__restrict__ jobs, const int32_t *__restrict__ idx, double *__restrict__ A,
+                       KParams kp) {
+    const EvalJob jb = jobs[blockIdx.x];
+    const int64_t tot = (int64_t)jb.m * jb.n;
+    const int32_t *rows = idx + jb.rows;
+    const int32_t *cols = idx + jb.cols;
+    for (int64_t e = threadIdx.x; e < tot; e += blockDim.x) {
+        int i = (int)(e % jb.m), j = (int)(e / jb.m);
+        A[jb.a + e] = kentry2(kp, rows[i], cols[j]);
+    }
+}
+
+// E = K(S, Ic) R^{-1} L^{-1} (m x r) and its transpose.  For the far field this
+// is L2P (3D/codes/HODLR3Dtree.hpp:1926-1927); for the vertex-sharing field it
+// is the transpose of M2M_ver = (L R)^{-T} K(Ic, S) (:2110-2133), the same
+// matrix since the kernel is symmetric.  One thread per row of S.
+__global__ void k_xfer(const XferJob *__restrict__ jobs, const int32_t *__restrict__ idx,
+                       const double *__restrict__ lr, double *__restrict__ A, KParams kp) {
+    const XferJob jb = jobs[blockIdx.x];
+    const int m = jb.m, r = jb.r;
+    const double *L = lr + jb.lr;
+    const double *R = L + (int64_t)r * r;
+    double *E = A + jb.e;
+    double *ET = A + jb.et;
+    const int32_t *S = idx + jb.s;
+    const int32_t *Ic = idx + jb.ic;
+    for (int t = threadIdx.x; t < m; t += blockDim.x) {
+        PointI p = load_point(kp, S[t]);
+        // X R = K(S_t, Ic)
+        for (int k = 0; k < r; ++k) {
+            double s = kentry(kp, p, Ic[k]);
+            for (int q = 0; q < k; ++q) s -= E[(int64_t)q * m + t] * R[(int64_t)k * r + q];
+            E[(int64_t)k * m + t] = s / R[(int64_t)k * r + k];
+        }
+        // Y L = X
+        for (int k = r - 1; k >= 0; --k) {
+            double s = E[(int64_t)k * m + t];
+            for (int q = k + 1; q < r; ++q) s -= E[(int64_t)q * m + t] * L[(int64_t)k * r + q];
+            E[(int64_t)k * m + t] = s / L[(int64_t)k * r + k];
+        }
+        for (int k = 0; k < r; ++k) ET[(int64_t)t * r + k] = E[(int64_t)k * m + t];
+    }
+}
+
+// A = R^{-1} L^{-1} K(Ir, V) (r x n), one thread per column: the semi-nested
+// vertex block's row factor with the two triangular solves of the reference's
+// apply (3D/codes/HODLR3Dtree.hpp:2341-2344) folded in at assembly.
+__global__ void k_cur(const CurJob *__restrict__ jobs, const int32_t *__restrict__ idx,
+                      const double *__restrict__ lr, double *__restrict__ A, KParams kp) {
+    const CurJob jb = jobs[blockIdx.x];
+    const int r = jb.r, n = jb.n;
+    const double *L = lr + jb.lr;
+    const double *R = L + (int64_t)r * r;
+    const int32_t *Ir = idx + jb.ir;
+    const int32_t *Vv = idx + jb.v;
+    for (int t = threadIdx.x; t < n; t += blockDim.x) {
+        double *a = A + jb.a + (int64_t)t * r;
+        PointI p = load_point(kp, Vv[t]);
+        for (int k = 0; k < r; ++k) a[k] = kentry(kp, p, Ir[k]);
+        for (int k = 0; k < r; ++k) {
+            double s = a[k];
+            for (int q = 0; q < k; ++q) s -= L[(int64_t)q * r + k] * a[q];
+            a[k] = s / L[(int64_t)k * r + k];
+        }
+        for (int k = r - 1; k >= 0; --k) {
+            double s = a[k];
+            for (int q = k + 1; q < r; ++q) s -= R[(int64_t)q * r + k] * a[q];
+            a[k] = s / R[(int64_t)k * r + k];
+        }
+    }
+}
+
+// ------------------------------------------------------------------ batched skinny GEMV
+// One workgroup (4 waves) per item; lanes run over the item's rows (<= 64),
+// the waves split every term's columns.  A is column-major, so each wave load
+// reads `rows` contiguous elements of one column.  Partial sums of the waves
+// are combined in LDS and added to y (exclusively or atomically).
+template <typename T>
+__global__ __launch_bounds__(256) void k_bgemv(const GItem *__restrict__ items, const GTerm *__restrict__ terms,
+                                               const T *__restrict__ A, T *vec) {
+    const GItem it = items[blockIdx.x];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    __shared__ T part[4][64];
+    T acc0 = 0, acc1 = 0;
+    if (lane < it.rows) {
+        for (int tt = it.t0; tt < it.t1; ++tt) {
+            const GTerm tm = terms[tt];
+            const T *a = A + tm.a + it.row0 + lane;
+            const T *x = vec + tm.x;
+            const int64_t lda = it.lda;
+            int c = wv;
+            for (; c + 12 < tm.cols; c += 16) {
+                T a0 = a[(int64_t)c * lda], a1 = a[(int64_t)(c + 4) * lda];
+                T a2 = a[(int64_t)(c + 8) * lda], a3 = a[(int64_t)(c + 12) * lda];
+                acc0 = fma(a0, x[c], acc0);
+                acc1 = fma(a1, x[c + 4], acc1);
+                acc0 = fma(a2, x[c + 8], acc0);
+                acc1 = fma(a3, x[c + 12], acc1);
+            }
+            for (; c < tm.cols; c += 4) acc0 = fma(a[(int64_t)c * lda], x[c], acc0);
+        }
+    }
+    part[wv][lane] = acc0 + acc1;
+    __syncthreads();
+    if (wv == 0 && lane < it.rows) {
+        T s = part[0][lane] + part[1][lane] + part[2][lane] + part[3][lane];
+        T *y = vec + it.y + lane;
+        if (it.flags & 1) atomicAdd(y, s);
+        else *y += s;
+    }
+}
+
+// ------------------------------------------------------------------ small vector kernels
+__global__ void k_to_fp32(const double *a, float *b, int64_t n) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        b[i] = (float)a[i];
+}
+__global__ void k_to_fp64(const float *a, double *b, int64_t n) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        b[i] = (double)a[i];
+}
+template <typename T>
+__global__ void k_fill(T *a, int64_t n, T v) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        a[i] = v;
+}
+template <typename T>
+__global__ void k_permute_in(const double *x, int64_t ldx, T *xt, int64_t ldt, const int32_t *perm, int64_t N) {
+    const int r = blockIdx.y;
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < N; t += (int64_t)gridDim.x * blockDim.x)
+        xt[r * ldt + t] = (T)x[r * ldx + perm[t]];
+}
+template <typename T>
+__global__ void k_permute_out(const T *yt, int64_t ldt, double *y, int64_t ldy, const int32_t *perm, int64_t N) {
+    const int r = blockIdx.y;
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < N; t += (int64_t)gridDim.x * blockDim.x)
+        y[r * ldy + perm[t]] = (double)yt[r * ldt + t];
+}
+
+// Dense rows of A x with entries evaluated on the fly (getTruePoten,
+// 3D/codes/HODLR3Dtree.hpp:2630-2643).  64 rows per workgroup, 4 column
+// groups; source coordinates and x staged through LDS.
+__global__ __launch_bounds__(256) void k_dense_rows(const int32_t *__restrict__ rows, int64_t nrows,
+                                                    const double *__restrict__ xt, double *__restrict__ y, int64_t N,
+                                                    KParams kp) {
+    __shared__ double sx[256], sy[256], sz[256], sq[256];
+    __shared__ double part[4][64];
+    const int lane = threadIdx.x & 63, g = threadIdx.x >> 6;
+    const int64_t r = blockIdx.x * 64LL + lane;
+    const bool active = r < nrows;
+    const int i = active ? (rows ? rows[r] : (int)r) : 0;
+    PointI p = load_point(kp, i);
+    double acc = 0.0;
+    for (int64_t j0 = 0; j0 < N; j0 += 256) {
+        int64_t j = j0 + threadIdx.x;
+        __syncthreads();
+        if (j < N) {
+            sx[threadIdx.x] = kp.px[j];
+            sy[threadIdx.x] = kp.dim > 1 ? kp.py[j] : 0.0;
+            sz[threadIdx.x] = kp.dim > 2 ? kp.pz[j] : 0.0;
+            sq[threadIdx.x] = xt[j];
+        }
+        __syncthreads();
+        const int nj = (int)min<int64_t>(256, N - j0);
+        for (int q = g; q < nj; q += 4) {
+            int jj = (int)(j0 + q);
+            double v = (jj == i) ? kp.diag : koff(kp.dim, kp.choice, p.x - sx[q], p.y - sy[q], p.z - sz[q]);
+            acc = fma(v, sq[q], acc);
+        }
+    }
+    part[g][lane] = acc;
+    __syncthreads();
+    if (g == 0 && active) y[r] = part[0][lane] + part[1][lane] + part[2][lane] + part[3][lane];
+}
+
+// ------------------------------------------------------------------ BLAS-1 for Arnoldi (FP64)
+constexpr int kDotThreads = 256;
+
+// partial[l * nblk + b] = sum over block b's rows of V_l . w
+__global__ __launch_bounds__(kDotThreads) void k_mdot(const double *__restrict__ V, int64_t ldv,
+                                                      const double *__restrict__ w, int64_t n,
+                                                      double *__restrict__ partial) {
+    const int b = blockIdx.x, l = blockIdx.y, nblk = gridDim.x;
+    const double *v = V + (int64_t)l * ldv;
+    double acc = 0.0;
+    for (int64_t i = (int64_t)b * kDotThreads + threadIdx.x; i < n; i += (int64_t)nblk * kDotThreads)
+        acc = fma(v[i], w[i], acc);
+    acc = wave_sum(acc);
+    __shared__ double sh[kDotThreads / 64];
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s = 0.0;
+        for (int q = 0; q < kDotThreads / 64; ++q) s += sh[q];
+        partial[(int64_t)l * nblk + b] = s;
+    }
+}
+__global__ void k_mdot_reduce(const double *__restrict__ partial, int nblk, double *__restrict__ out) {
+    const int l = blockIdx.x;
+    double acc = 0.0;
+    for (int b = threadIdx.x; b < nblk; b += 64) acc += partial[(int64_t)l * nblk + b];
+    acc = wave_sum(acc);
+    if (threadIdx.x == 0) out[l] = acc;
+}
+// w -= V h  (sign = -1) or x += V c (sign = +1)
+__global__ void k_mgemv(const double *__restrict__ V, int64_t ldv, int k, const double *__restrict__ h,
+                        double *__restrict__ w, int64_t n, double sign) {
+    __shared__ double hs[1024];
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i - threadIdx.x < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        double acc = 0.0;
+        for (int l0 = 0; l0 < k; l0 += 1024) {
+            int l1 = min(k, l0 + 1024);
+            __syncthreads();
+            for (int l = l0 + threadIdx.x; l < l1; l += blockDim.x) hs[l - l0] = h[l];
+            __syncthreads();
+            if (i < n)
+                for (int l = l0; l < l1; ++l) acc = fma(V[(int64_t)l * ldv + i], hs[l - l0], acc);
+        }
+        if (i < n) w[i] += sign * acc;
+    }
+}
+__global__ void k_axpby(double a, const double *__restrict__ x, double b, double *__restrict__ y, int64_t n) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        y[i] = b == 0.0 ? a * x[i] : a * x[i] + b * y[i];
+}
+
+inline int grid_for(int64_t n, int nt = 256, int cap = 8192) {
+    int64_t g = (n + nt - 1) / nt;
+    if (g < 1) g = 1;
+    if (g > cap) g = cap;
+    return (int)g;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ launch wrappers
+void launch_aca(const AcaJob *jobs, int njobs, const int32_t *idx, double *ws, uint8_t *flags, int32_t *rank,
+                int32_t *status, int32_t *rowpiv, int32_t *colpiv, KParams kp, double tol, int big, hipStream_t s) {
+    if (njobs <= 0) return;
+    if (big) hipLaunchKernelGGL(k_aca<1024>, dim3(njobs), dim3(1024), 0, s, jobs, idx, ws, flags, rank, status,
+                                rowpiv, colpiv, kp, tol);
+    else hipLaunchKernelGGL(k_aca<256>, dim3(njobs), dim3(256), 0, s, jobs, idx, ws, flags, rank, status, rowpiv,
+                            colpiv, kp, tol);
+}
+void launch_lr_extract(const LrJob *jobs, int njobs, const double *ws, const int32_t *rowpiv, const int32_t *colpiv,
+                       double *out, hipStream_t s) {
+    if (njobs > 0) hipLaunchKernelGGL(k_lr_extract, dim3(njobs), dim3(256), 0, s, jobs, ws, rowpiv, colpiv, out);
+}
+void launch_eval(const EvalJob *jobs, int njobs, const int32_t *idx, double *A, KParams kp, hipStream_t s) {
+    if (njobs > 0) hipLaunchKernelGGL(k_eval, dim3(njobs), dim3(256), 0, s, jobs, idx, A, kp);
+}
+void launch_xfer(const XferJob *jobs, int njobs, const int32_t *idx, const double *lr, double *A, KParams kp,
+                 hipStream_t s) {
+    if (njobs > 0) hipLaunchKernelGGL(k_xfer, dim3(njobs), dim3(64), 0, s, jobs, idx, lr, A, kp);
+}
+void launch_cur(const CurJob *jobs, int njobs, const int32_t *idx, const double *lr, double *A, KParams kp,
+                hipStream_t s) {
+    if (njobs > 0) hipLaunchKernelGGL(k_cur, dim3(njobs), dim3(256), 0, s, jobs, idx, lr, A, kp);
+}
+void launch_bgemv64(const GItem *items, int nitems, const GTerm *terms, const double *A, double *vec,
+                    hipStream_t s) {
+    if (nitems > 0) hipLaunchKernelGGL(k_bgemv<double>, dim3(nitems), dim3(256), 0, s, items, terms, A, vec);
+}
+void launch_bgemv32(const GItem *items, int nitems, const GTerm *terms, const float *A, float *vec,
+                    hipStream_t s) {
+    if (nitems > 0) hipLaunchKernelGGL(k_bgemv<float>, dim3(nitems), dim3(256), 0, s, items, terms, A, vec);
+}
+void launch_to_fp32(const double *a, float *b, int64_t n, hipStream_t s) {
+    if (n > 0) hipLaunchKernelGGL(k_to_fp32, dim3(grid_for(n)), dim3(256), 0, s, a, b, n);
+}
+void launch_to_fp64(const float *a, double *b, int64_t n, hipStream_t s) {
+    if (n > 0) hipLaunchKernelGGL(k_to_fp64, dim3(grid_for(n)), dim3(256), 0, s, a, b, n);
+}
+void launch_fill(double *a, int64_t n, double v, hipStream_t s) {
+    if (n > 0) hipLaunchKernelGGL(k_fill<double>, dim3(grid_for(n)), dim3(256), 0, s, a, n, v);
+}
+void launch_fill32(float *a, int64_t n, float v, hipStream_t s) {
+    if (n > 0) hipLaunchKernelGGL(k_fill<float>, dim3(grid_for(n)), dim3(256), 0, s, a, n, v);
+}
+void launch_permute_in(const double *x, int64_t ldx, double *xt, int64_t ldt, const int32_t *perm, int64_t N,
+                       int nrhs, hipStream_t s) {
+    hipLaunchKernelGGL(k_permute_in<double>, dim3(grid_for(N, 256, 2048), nrhs), dim3(256), 0, s, x, ldx, xt, ldt,
+                       perm, N);
+}
+void launch_permute_in32(const double *x, int64_t ldx, float *xt, int64_t ldt, const int32_t *perm, int64_t N,
+                         int nrhs, hipStream_t s) {
+    hipLaunchKernelGGL(k_permute_in<float>, dim3(grid_for(N, 256, 2048), nrhs), dim3(256), 0, s, x, ldx, xt, ldt,
+                       perm, N);
+}
+void launch_permute_out(const double *yt, int64_t ldt, double *y, int64_t ldy, const int32_t *perm, int64_t N,
+                        int nrhs, hipStream_t s) {
+    hipLaunchKernelGGL(k_permute_out<double>, dim3(grid_for(N, 256, 2048), nrhs), dim3(256), 0, s, yt, ldt, y, ldy,
+                       perm, N);
+}
+void launch_permute_out32(const float *yt, int64_t ldt, double *y, int64_t ldy, const int32_t *perm, int64_t N,
+                          int nrhs, hipStream_t s) {
+    hipLaunchKernelGGL(k_permute_out<float>, dim3(grid_for(N, 256, 2048), nrhs), dim3(256), 0, s, yt, ldt, y, ldy,
+                       perm, N);
+}
+void launch_dense_rows(const int32_t *rows, int64_t nrows, const double *xt, double *y, int64_t N, KParams kp,
+                       hipStream_t s) {
+    if (nrows > 0)
+        hipLaunchKernelGGL(k_dense_rows, dim3((unsigned)((nrows + 63) / 64)), dim3(256), 0, s, rows, nrows, xt, y, N,
+                           kp);
+}
+int mdot_blocks(int64_t n) {
+    int64_t b = (n + kDotThreads * 8 - 1) / (kDotThreads * 8);
+    if (b < 1) b = 1;
+    if (b > 512) b = 512;
+    return (int)b;
+}
+void launch_mdot(const double *V, int64_t ldv, int k, const double *w, int64_t n, double *partial, int nblk,
+                 hipStream_t s) {
+    if (k > 0) hipLaunchKernelGGL(k_mdot, dim3(nblk, k), dim3(kDotThreads), 0, s, V, ldv, w, n, partial);
+}
+void launch_mdot_reduce(const double *partial, int nblk, int k, double *out, hipStream_t s) {
+    if (k > 0) hipLaunchKernelGGL(k_mdot_reduce, dim3(k), dim3(64), 0, s, partial, nblk, out);
+}
+void launch_mgemv_sub(const double *V, int64_t ldv, int k, const double *h, double *w, int64_t n, hipStream_t s) {
+    if (k > 0) hipLaunchKernelGGL(k_mgemv, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, V, ldv, k, h, w, n, -1.0);
+}
+void launch_vec_update(const double *V, int64_t ldv, int k, const double *c, double *x, int64_t n, hipStream_t s) {
+    if (k > 0) hipLaunchKernelGGL(k_mgemv, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, V, ldv, k, c, x, n, 1.0);
+}
+void launch_axpby(double a, const double *x, double b, double *y, int64_t n, hipStream_t s) {
+    hipLaunchKernelGGL(k_axpby, dim3(grid_for(n)), dim3(256), 0, s, a, x, b, y, n);
+}
+
+}  // namespace h2w

@@ -1,0 +1,179 @@
+/*
+ * h2w.h -- C ABI of the MI355X-native H2* / (H2+H)* fast matrix-vector
+ * product (libh2w.so).  Plain pointers and sizes only.
+ *
+ * It replaces the reference's operator facade, the class `initialisation`
+ * (1D/codes/matvec.hpp:6-113, 2D/codes/matvec.hpp:6-172,
+ * 3D/codes/matvec.hpp:6-178), and the GMRES driver that calls it
+ * ({1,2,3}D/codes/gmres.hpp:91-200, {1,2,3}D/codes/main.cpp).
+ *
+ * Conventions
+ *   - All functions return an H2W_* status; they never exit the process
+ *     (the reference exits on a bad `choice`, 3D/codes/kernel.hpp:175-179,
+ *     and on nLevels < 1, 3D/codes/main.cpp:12-16).  h2w_last_error() gives
+ *     a message for the last failure on the calling thread.
+ *   - Vectors are FP64 and in INPUT order unless the call says otherwise.
+ *   - `where` = H2W_HOST (host pointers, copied through pinned staging) or
+ *     H2W_DEVICE (device pointers on the handle's HIP device).
+ *   - Calls on one handle are serialized by the caller (the reference's
+ *     matvec is not re-entrant either); distinct handles are independent.
+ */
+#ifndef H2W_H
+#define H2W_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define H2W_OK 0
+#define H2W_EINVAL 1   /* invalid argument (dim, choice, N, n_max, nLevels < 1, ...) */
+#define H2W_ESTATE 2   /* wrong call order, e.g. matvec before assemble            */
+#define H2W_EHIP 3     /* a HIP runtime call failed                                 */
+#define H2W_ENOMEM 4   /* device or host allocation failed                          */
+#define H2W_ENOCONV 5  /* GMRES stopped at max_iter without reaching tol            */
+#define H2W_ENOLIB 6   /* no usable GPU (the library has no CPU fallback)           */
+
+/* hierarchical format (compile-time -DUSE_nHODLRdD / -DUSE_snHODLRdD in the reference) */
+#define H2W_NESTED 0      /* H2*    : nHODLRdD, nested far + nested vertex-sharing blocks */
+#define H2W_SEMI_NESTED 1 /* (H2+H)*: s-nHODLRdD, nested far + per-pair CUR vertex blocks */
+
+/* kernel choice, as argv[5] of the reference's main */
+#define H2W_CHOICE_IE 0   /* integral equation kernel with its diagonal rule */
+#define H2W_CHOICE_RBF 1  /* the repo's RBF kernel                            */
+
+#define H2W_HOST 0
+#define H2W_DEVICE 1
+
+#define H2W_FP64 0
+#define H2W_FP32 1
+
+#define H2W_ORDER_INPUT 0
+#define H2W_ORDER_TREE 1
+
+typedef struct h2w_handle h2w_handle;
+
+typedef struct {
+    int compat;              /* 1 (default): reproduce the reference's quirks, i.e. the 1D RBF
+                                diagonal sqrt(2N) of the doubled particle vector
+                                (1D/codes/matvec.hpp:35,38-39; 1D/codes/kernel.hpp:180)          */
+    int nlevels;             /* <= 0 (default): the reference main's formula
+                                ceil(log_{2^d}(N / n_max)) ({1,2,3}D/codes/main.cpp:11-13)             */
+    int fp32_shadow;         /* 1 (default): also keep an FP32 copy of every stored operator     */
+    int device;              /* HIP device ordinal (default 0)                                   */
+    int rank_cap;            /* first-pass ACA rank cap per block (default 128); blocks that
+                                reach it are re-run with a larger cap, so results do not depend
+                                on it                                                            */
+    int64_t workspace_bytes; /* ACA workspace budget in bytes (0 = automatic)                    */
+    int plan_only;           /* 1: build the tree and lists only, touch no device (assembly and
+                                products then return H2W_ESTATE); for host-side inspection        */
+} h2w_opts;
+
+typedef struct {
+    int dim, variant, choice, nlevels;
+    int64_t N;
+    double memory;           /* stored-element count as the reference's findMemory_n /
+                                findMemory_sn counts it (3D/codes/HODLR3Dtree.hpp:2505-2628)     */
+    double diag;             /* value of the diagonal entries                                    */
+    double assembly_seconds; /* wall time of h2w_assemble                                        */
+    int max_rank_far, max_rank_vertex;
+    double mean_rank_far, mean_rank_vertex;
+    int64_t stored_bytes_fp64;   /* bytes one FP64 matvec streams from the operator pools     */
+    int64_t stored_bytes_fp32;   /* same for the FP32 shadow (0 if not built)                 */
+    int n_launches_matvec;   /* kernel launches per matvec                                       */
+    int aca_reruns;          /* blocks re-run because they reached the rank cap                  */
+} h2w_stats;
+
+typedef struct {
+    int iterations;          /* Arnoldi steps (FP64 mode) / total inner steps (FP32-IR mode),
+                                counted like the reference's `max_iter` output                   */
+    int outer_iterations;    /* refinement steps (FP32-IR mode), 0 for FP64 mode                 */
+    int matvecs_fp64, matvecs_fp32;
+    double resid;            /* final |s_{i+1}|/||b|| (FP64) or true ||b - A x||/||b|| (IR)      */
+    double seconds;          /* wall time, first residual matvec to convergence                  */
+    int converged;
+} h2w_gmres_stats;
+
+#define H2W_GMRES_FP64 0     /* unrestarted GMRES, FP64 matvecs                                  */
+#define H2W_GMRES_FP32_IR 1  /* FP64 iterative refinement around GMRES with FP32 matvecs        */
+
+void h2w_default_opts(h2w_opts *opts);
+
+/* Constructor.  Replaces `initialisation(N, nLevels, n_max, L, TOL_POW, choice)`
+ * (1D/codes/matvec.hpp:18; 2D/codes/matvec.hpp:20 and 3D/codes/matvec.hpp:20
+ * take sqrt(N)/cbrt(N) and the per-axis leaf size instead).
+ *   points: dim x N, SoA (x[0..N), y[0..N), z[0..N)), host memory; NULL selects
+ *           the reference's uniform grid (1D/codes/HODLR1Dtree.hpp:59-65,
+ *           2D/codes/HODLR2Dtree.hpp:75-91, 3D/codes/HODLR3Dtree.hpp:82-102),
+ *           which needs N to be a square (2D) or cube (3D).
+ *   L: half-width of the root box [-L, L]^dim; tol_pow: ACA/NCA tolerance 10^-tol_pow. */
+int h2w_create(int dim, int variant, int64_t N, const double *points, int n_max, double L, int tol_pow,
+               int choice, const h2w_opts *opts, h2w_handle **out);
+
+/* Assembly.  Replaces `initialisation::tree_initialisation()`
+ * (3D/codes/matvec.hpp:33-61 nested, :86-113 semi-nested; 1D/codes/matvec.hpp:28-49,67-87). */
+int h2w_assemble(h2w_handle *h);
+
+/* y = A x (overwrites y).  Replaces `initialisation::mat_vec_prod(Vec& charge, Vec& output)`
+ * (3D/codes/matvec.hpp:63-83, :115-136; 1D/codes/matvec.hpp:51-65,89-103). */
+int h2w_matvec(h2w_handle *h, const double *x, double *y, int where);
+
+/* General form: precision H2W_FP64 or H2W_FP32 (FP32 operators, FP32 accumulation; x and y
+ * are FP64 either way), order H2W_ORDER_INPUT or H2W_ORDER_TREE, nrhs right-hand sides
+ * stored one after another with leading dimension ld (>= N). */
+int h2w_matvec_ex(h2w_handle *h, const double *x, double *y, int nrhs, int64_t ld, int precision, int order,
+                  int where);
+
+/* GMRES driver.  Replaces `iterative_solver(max_iter, tol_pow).GMRES(A, x, b)`
+ * (1D/codes/gmres.hpp:22-28,91-200): unrestarted (restart = max_iter); x holds x0 on entry
+ * (the reference's main passes ones, 1D/codes/main.cpp:34) and the solution on return.
+ * Returns H2W_ENOCONV (with x updated) when max_iter is reached first. */
+int h2w_gmres(h2w_handle *h, const double *b, double *x, int max_iter, double tol, int mode, int where,
+              h2w_gmres_stats *stats);
+
+/* Dense FP64 product of selected rows of A with x, entries evaluated on the fly on the GPU.
+ * rows == NULL means all N rows.  Replaces `getTruePoten` (3D/codes/HODLR3Dtree.hpp:2630-2643). */
+int h2w_dense_matvec(h2w_handle *h, const double *x, double *y, const int64_t *rows, int64_t nrows, int where);
+
+int h2w_get_stats(const h2w_handle *h, h2w_stats *stats);
+
+/* Timing of the matvec on device-resident data (no counterpart in the reference, whose
+ * transcripts time whole GMRES solves with omp_get_wtime, 1D/codes/main.cpp:37-40).
+ * Runs nrep tree-order matvecs back to back on the handle's stream, bracketed by HIP
+ * events; also records each launch of the sequence (average over nrep) and the operator
+ * bytes that launch streams. */
+#define H2W_MAX_LAUNCHES 64
+typedef struct {
+    double ms_per_matvec;
+    double launch_ms[H2W_MAX_LAUNCHES];
+    int n_launch;
+    int64_t launch_bytes[H2W_MAX_LAUNCHES];
+} h2w_bench;
+int h2w_bench_matvec(h2w_handle *h, int nrep, int precision, h2w_bench *out);
+
+/* Tree-order permutation: perm[t] = input index of the t-th point in tree order
+ * (leaf boxes' chargeLocations concatenated, 3D/codes/HODLR3Dtree.hpp:2489-2503). */
+int h2w_tree_order(const h2w_handle *h, int32_t *perm);
+
+/* Skeleton of one box after assembly.  family 0 = far field (incoming_checkPoints /
+ * incoming_chargePoints, 3D/codes/HODLR3Dtree.hpp:1915-1924), 1 = vertex-sharing nested
+ * (…_ver, :2083-2092).  ir/ic receive input-order point indices (capacity cap each).
+ * Returns the rank (>= 0) or -H2W_* on error. */
+int h2w_box_skeleton(const h2w_handle *h, int family, int level, int box, int32_t *ir, int32_t *ic, int cap);
+
+/* Interaction list of a box in the reference's order (3D/codes/HODLR3Dtree.hpp:155-1680) with
+ * the vertex flag of each entry (:1744-1767); near neighbours (leaf level) in slot order.
+ * Returns the list length or -H2W_* on error. */
+int h2w_interaction_list(const h2w_handle *h, int level, int box, int32_t *il, int32_t *is_vertex, int cap);
+int h2w_near_list(const h2w_handle *h, int level, int box, int32_t *nb, int cap);
+
+void h2w_destroy(h2w_handle *h);
+
+const char *h2w_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* H2W_H */
