@@ -17,6 +17,8 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <numeric>
@@ -574,16 +576,25 @@ void add_item(WorkList &W, int64_t y, int rows, const std::vector<TermSpec> &ter
 // ------------------------------------------------------------------ assembly
 void assemble(Handle &h) {
     double t0 = now_s();
+    const bool verbose = std::getenv("H2W_VERBOSE") != nullptr;
+    auto phase = [&](const char *what) {
+        if (verbose) {
+            (void)hipStreamSynchronize(h.stream);
+            std::fprintf(stderr, "[h2w] %-28s %8.3f s\n", what, now_s() - t0);
+        }
+    };
     const Plan &P = h.plan;
     hipStream_t s = h.stream;
     h.aca_reruns = 0;
     if (P.dim >= 2) far_nca(h);
     else h.fam[0].assign(P.nL + 1, FamLevel());
+    phase("far NCA");
     if (h.variant == H2W_NESTED) vertex_nca(h);
     else {
         h.fam[1].assign(P.nL + 1, FamLevel());
         cur_pairs(h);
     }
+    phase("vertex NCA / CUR");
     const int nfam_lo[2] = {2, 1};  // first level of each family
     const bool fam_on[2] = {P.dim >= 2, h.variant == H2W_NESTED};
 
@@ -752,6 +763,7 @@ void assemble(Handle &h) {
         H2W_HIP(hipGetLastError());
         H2W_HIP(hipStreamSynchronize(s));
     }
+    phase("transfer operators");
     {
         // plain entry blocks: M2L, near field, semi-nested column factors
         std::vector<int32_t> idx;
@@ -795,6 +807,7 @@ void assemble(Handle &h) {
         }
         flush();
     }
+    phase("M2L / near / CUR blocks");
     if (h.opts.fp32_shadow) {
         h.mat32.alloc(h.mat_len);
         launch_to_fp32(h.mat64.p, h.mat32.p, h.mat_len, s);
@@ -925,6 +938,10 @@ void assemble(Handle &h) {
     }
     h.stored_elems = se;
     H2W_HIP(hipStreamSynchronize(s));
+    phase("work lists");
+    if (verbose)
+        std::fprintf(stderr, "[h2w] memory %.0f elements (%.3f GB fp64), pool %.3f GB, %zu launches, %zu items\n",
+                     h.memory, h.memory * 8e-9, h.mat_len * 8e-9, h.launches.size(), items.size());
     h.assembled = true;
     h.assembly_seconds = now_s() - t0;
 }
