@@ -1,0 +1,207 @@
+#!/usr/bin/env python3
+"""Benchmark of the H2* fast matvec driving GMRES on one MI355X.
+
+Workload (the configuration the metric is quoted on): 3D H2* with the repo's
+RBF kernel (choice=1) on the reference's uniform grid 64^3 (N = 262144) in
+[-1,1]^3, n_max = 64 (4 levels), ACA/NCA tolerance 1e-10, GMRES tolerance
+1e-10, x_true ~ U(-1,1) (seed 2), b = A x_true from the GPU's dense FP64
+product, x0 = 1 (as the reference's main).
+
+A step is one GMRES solve to 1e-10 with FP32 operators inside FP64 iterative
+refinement, b and x resident in HBM.  The FP64 solve on the stored FP64
+operators is timed beside it.  Matvec bandwidth is the reference's stored
+element count M (findMemory_n) times the element size plus the two N-vectors,
+per HIP-event-timed matvec.  The CPU baseline is the oracle (a C++/OpenMP
+restatement of the reference) on a bounded sample.
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "GMRES time-to-1e-10 (s) & matvec GB/s: FP32+refine batched vs FP64 GPU, 1 GPU"
+HBM_PEAK_GBPS = 8000.0
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+class Hip:
+    """Device buffers through the HIP runtime (plumbing for device-resident inputs)."""
+
+    def __init__(self):
+        self.lib = ctypes.CDLL("libamdhip64.so.7")
+
+    def malloc(self, nbytes):
+        p = ctypes.c_void_p()
+        assert self.lib.hipMalloc(ctypes.byref(p), ctypes.c_size_t(nbytes)) == 0
+        return p
+
+    def h2d(self, dst, arr):
+        assert self.lib.hipMemcpy(dst, arr.ctypes.data_as(ctypes.c_void_p), ctypes.c_size_t(arr.nbytes), 1) == 0
+
+    def d2d(self, dst, src, nbytes):
+        assert self.lib.hipMemcpy(dst, src, ctypes.c_size_t(nbytes), 3) == 0
+
+    def d2h(self, arr, src):
+        assert self.lib.hipMemcpy(arr.ctypes.data_as(ctypes.c_void_p), src, ctypes.c_size_t(arr.nbytes), 2) == 0
+
+    def sync(self):
+        assert self.lib.hipDeviceSynchronize() == 0
+
+    def free(self, p):
+        self.lib.hipFree(p)
+
+
+def cpu_baseline(n_side, gmres_matvecs_full, memory_full, tol_pow, gtol):
+    """Oracle (port of the reference's CPU algorithm, OpenMP on all host cores) on a
+    bounded sample: the same workload at n_side^3 points.  Its GMRES time-to-tolerance
+    is scaled to the full workload by the ratio of streamed work (M x matvecs)."""
+    import oracle
+    N = n_side ** 3
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    t0 = time.time()
+    o = oracle.Oracle(3, N, 64, 1.0, tol_pow, 1, 0)
+    o.tree_initialisation()
+    t_asm = time.time() - t0
+    xt = np.random.default_rng(2).uniform(-1, 1, N)
+    b = o.dense_matvec(xt)
+    t0 = time.time()
+    x, it, resid, nmv = o.gmres(b, None, 500, gtol)
+    t_gm = time.time() - t0
+    scale = (memory_full * gmres_matvecs_full) / (o.memory * nmv)
+    err = float(np.linalg.norm(x - xt) / np.linalg.norm(xt))
+    return {"value": t_gm * scale, "unit": "s", "cores": cores, "kind": "port",
+            "sample": (f"oracle on 3D H2* RBF grid {n_side}^3 (N={N}), n_max=64, tol 1e-{tol_pow}: assembly "
+                       f"{t_asm:.1f} s, GMRES {it} its / {nmv} matvecs in {t_gm:.3f} s (rel. error {err:.1e}); "
+                       f"scaled x{scale:.1f} by M*matvecs to N=262144"),
+            "sample_gmres_s": t_gm, "sample_assembly_s": t_asm, "sample_iterations": it}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--side", type=int, default=64, help="grid points per axis (N = side^3)")
+    ap.add_argument("--tol-pow", type=int, default=10)
+    ap.add_argument("--cpu-side", type=int, default=24, help="grid side of the CPU-baseline sample")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); the sharded "
+              "design is described in DESIGN.md and not launched", file=sys.stderr)
+        sys.exit(2)
+    import h2weak_amd as h2w
+
+    side, tp = args.side, args.tol_pow
+    N = side ** 3
+    gtol = 10.0 ** (-tp)
+    log(f"workload: 3D H2* RBF grid {side}^3 N={N} n_max=64 tol 1e-{tp}")
+    t0 = time.time()
+    A = h2w.Initialisation(3, N, 64, 1.0, tp, h2w.CHOICE_RBF, h2w.NESTED)
+    A.tree_initialisation()
+    st = A.stats()
+    log(f"assembly {time.time() - t0:.1f} s, M = {st['memory']:.4e} elements ({st['memory'] * 8e-9:.2f} GB fp64)")
+    xt = np.random.default_rng(2).uniform(-1.0, 1.0, N)
+    b = A.dense_rows(None, xt)
+    log("b = A x_true from the GPU dense FP64 product")
+
+    hip = Hip()
+    db, dx = hip.malloc(8 * N), hip.malloc(8 * N)
+    hip.h2d(db, b)
+    ones = np.ones(N)
+    dones = hip.malloc(8 * N)
+    hip.h2d(dones, ones)
+    L = h2w.lib()
+
+    def solve(mode):
+        hip.d2d(dx, dones, 8 * N)
+        s = h2w.GmresStats()
+        rc = L.h2w_gmres(A._h, db, dx, 500, gtol, mode, h2w.DEVICE, ctypes.byref(s))
+        assert rc == h2w.H2W_OK, (rc, L.h2w_last_error())
+        return s.as_dict()
+
+    for _ in range(args.warmup):
+        solve(h2w.GMRES_FP32_IR)
+        solve(h2w.GMRES_FP64)
+    hip.sync()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        s32 = solve(h2w.GMRES_FP32_IR)
+    hip.sync()
+    t_step = (time.perf_counter() - t0) / args.steps
+    x32 = np.empty(N)
+    hip.d2h(x32, dx)
+    e32 = float(np.linalg.norm(x32 - xt) / np.linalg.norm(xt))
+    log(f"FP32+IR solve {t_step * 1e3:.2f} ms: {s32}")
+    nfp64 = max(1, min(args.steps, 3))
+    hip.sync()
+    t0 = time.perf_counter()
+    for _ in range(nfp64):
+        s64 = solve(h2w.GMRES_FP64)
+    hip.sync()
+    t64 = (time.perf_counter() - t0) / nfp64
+    x64 = np.empty(N)
+    hip.d2h(x64, dx)
+    e64 = float(np.linalg.norm(x64 - xt) / np.linalg.norm(xt))
+    log(f"FP64 solve {t64 * 1e3:.2f} ms: {s64}")
+
+    M = st["memory"]
+    mv = {}
+    roof = None
+    for prec, name, s_op in ((h2w.FP64, "fp64", 8), (h2w.FP32, "fp32", 4)):
+        r = A.bench_matvec(20, prec)
+        alg = s_op * M + 2 * 8 * N
+        mv[name] = {"ms": r["ms_per_matvec"], "alg_GBps": alg / (r["ms_per_matvec"] * 1e-3) * 1e-9,
+                    "launch_ms": [round(v, 4) for v in r["launch_ms"]],
+                    "launch_GB": [round(v * 1e-9, 4) for v in r["launch_bytes"]]}
+        log(f"matvec {name}: {r['ms_per_matvec']:.3f} ms, {mv[name]['alg_GBps']:.0f} GB/s algorithmic")
+        if prec == h2w.FP32:
+            # dominant kernel of the measured path: the FP32 M2L + near-field launch
+            q = r["launch_role"].index(1)
+            nbytes = r["launch_bytes"][q] + 4 * N * 2  # stored elements + x read + y written
+            ach = nbytes / (r["launch_ms"][q] * 1e-3) * 1e-9
+            roof = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                    "frac": round(ach / HBM_PEAK_GBPS, 4), "traffic": None,
+                    "kernel": "k_bgemv<float, 1> (M2L + near field)", "launch_ms": round(r["launch_ms"][q], 4),
+                    "algorithmic_bytes": int(nbytes)}
+            tr_path = os.path.join(ROOT, "profiles", "pmc_traffic.json")
+            if os.path.exists(tr_path):
+                tr = json.load(open(tr_path))
+                key = f"3d_rbf_{side}_{tp}_fp32_m2l_near"
+                if key in tr:
+                    roof["traffic"] = tr[key]["hbm_bytes_per_launch"]
+
+    out = {"metric": METRIC, "value": round(t_step, 6), "unit": "s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": round(t_step * 1e3, 3), "higher_is_better": False,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f32 operators, f64 refinement",
+           "data": "synthetic (reference uniform grid, x_true ~ U(-1,1) seed 2)",
+           "config": {"workload": f"3D H2* RBF grid {side}^3 N={N} n_max=64 ACA tol 1e-{tp} GMRES tol 1e-{tp}",
+                      "global_batch": 1, "seq_len": N, "parallelism": "single GPU"},
+           "fp64_gmres_s": round(t64, 6), "speedup_fp32ir_vs_fp64": round(t64 / t_step, 3),
+           "gmres_fp32ir": {k: s32[k] for k in ("iterations", "outer_iterations", "matvecs_fp64", "matvecs_fp32",
+                                                "resid")},
+           "gmres_fp64": {k: s64[k] for k in ("iterations", "matvecs_fp64", "resid")},
+           "rel_solution_error": {"fp32ir": e32, "fp64": e64},
+           "matvec": {k: {kk: (round(vv, 4) if isinstance(vv, float) else vv) for kk, vv in v.items()}
+                      for k, v in mv.items()},
+           "assembly_s": round(st["assembly_seconds"], 2), "memory_elements": M,
+           "roofline": roof, "cpu_baseline": None}
+    if not args.no_cpu:
+        log(f"CPU baseline (oracle) on grid {args.cpu_side}^3")
+        out["cpu_baseline"] = cpu_baseline(args.cpu_side, s64["matvecs_fp64"], M, tp, gtol)
+        out["speedup_vs_cpu"] = round(out["cpu_baseline"]["value"] / t_step, 2)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -73,7 +73,7 @@ class GmresStats(ctypes.Structure):
 
 class BenchResult(ctypes.Structure):
     _fields_ = [("ms_per_matvec", ctypes.c_double), ("launch_ms", ctypes.c_double * 64),
-                ("n_launch", ctypes.c_int), ("launch_bytes", ctypes.c_int64 * 64)]
+                ("n_launch", ctypes.c_int), ("launch_bytes", ctypes.c_int64 * 64), ("launch_role", ctypes.c_int * 64)]
 
 
 _lib = None
@@ -130,7 +130,7 @@ class Initialisation:
     """
 
     def __init__(self, dim, N, n_max, L=1.0, tol_pow=12, choice=CHOICE_RBF, variant=NESTED, points=None,
-                 nlevels=0, compat=1, fp32_shadow=1, rank_cap=128, device=0, workspace_bytes=0,
+                 nlevels=0, compat=1, fp32_shadow=1, rank_cap=256, device=0, workspace_bytes=0,
                  plan_only=0):
         self.dim, self.N = int(dim), int(N)
         o = Opts()
@@ -234,7 +234,7 @@ class Initialisation:
         _check(lib().h2w_bench_matvec(self._h, int(nrep), precision, ctypes.byref(r)))
         n = r.n_launch
         return {"ms_per_matvec": r.ms_per_matvec, "launch_ms": list(r.launch_ms[:n]),
-                "launch_bytes": list(r.launch_bytes[:n])}
+                "launch_bytes": list(r.launch_bytes[:n]), "launch_role": list(r.launch_role[:n])}
 
 
 class IterativeSolver:

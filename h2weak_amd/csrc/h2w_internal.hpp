@@ -57,20 +57,22 @@ struct Plan {
 };
 
 // ------------------------------------------------------------------ device descriptors
+// One ACA problem.  Its U (m) and V (n) columns are carved out of a shared
+// workspace pool by a device-side bump allocator, one column pair per step;
+// the pool offsets of column pair l are kept at tab[2l] (U) and tab[2l+1] (V).
 struct AcaJob {
     int64_t rows;  // offset into the index array (int32 entries)
     int64_t cols;
     int32_t m, n;
-    int32_t rcap;
+    int32_t tcap;  // capacity of the column table (max rank)
     int32_t pad;
-    int64_t wsU;   // offset (doubles) of U, m x rcap column-major
-    int64_t wsV;   // offset (doubles) of V, n x rcap column-major
+    int64_t tab;   // offset (int64 entries) of the column table, 2*tcap entries
     int64_t wsF;   // offset (bytes) of the used-row/used-col flags (m + n bytes)
-    int64_t piv;   // offset into rowpiv/colpiv (rcap entries each)
+    int64_t piv;   // offset into rowpiv/colpiv (tcap entries each)
 };
 
 struct LrJob {     // copy L (unit lower) and R (upper) out of an ACA workspace
-    int64_t wsU, wsV, piv;
+    int64_t tab, piv;
     int32_t m, n, rank, pad;
     int64_t out;   // offset (doubles): L at out, R at out + rank*rank, both column-major
 };
@@ -107,19 +109,19 @@ struct GItem {
 };
 
 // ------------------------------------------------------------------ launch wrappers (kernels.hip)
-void launch_aca(const AcaJob *jobs, int njobs, const int32_t *idx, double *ws, uint8_t *flags, int32_t *rank,
-                int32_t *status, int32_t *rowpiv, int32_t *colpiv, KParams kp, double tol, int big,
-                hipStream_t s);
-void launch_lr_extract(const LrJob *jobs, int njobs, const double *ws, const int32_t *rowpiv,
-                       const int32_t *colpiv, double *out, hipStream_t s);
+void launch_aca(const AcaJob *jobs, int njobs, const int32_t *idx, double *ws, int64_t ws_cap,
+                unsigned long long *ws_top, int64_t *tab, uint8_t *flags, int32_t *rank, int32_t *status,
+                int32_t *rowpiv, int32_t *colpiv, KParams kp, double tol, int big, hipStream_t s);
+void launch_lr_extract(const LrJob *jobs, int njobs, const double *ws, const int64_t *tab,
+                       const int32_t *rowpiv, const int32_t *colpiv, double *out, hipStream_t s);
 void launch_eval(const EvalJob *jobs, int njobs, const int32_t *idx, double *A, KParams kp, hipStream_t s);
 void launch_xfer(const XferJob *jobs, int njobs, const int32_t *idx, const double *lr, double *A, KParams kp,
                  hipStream_t s);
 void launch_cur(const CurJob *jobs, int njobs, const int32_t *idx, const double *lr, double *A, KParams kp,
                 hipStream_t s);
-void launch_bgemv64(const GItem *items, int nitems, const GTerm *terms, const double *A, double *vec,
+void launch_bgemv64(const GItem *items, int nitems, const GTerm *terms, const double *A, double *vec, int role,
                     hipStream_t s);
-void launch_bgemv32(const GItem *items, int nitems, const GTerm *terms, const float *A, float *vec,
+void launch_bgemv32(const GItem *items, int nitems, const GTerm *terms, const float *A, float *vec, int role,
                     hipStream_t s);
 void launch_to_fp32(const double *a, float *b, int64_t n, hipStream_t s);
 void launch_to_fp64(const float *a, double *b, int64_t n, hipStream_t s);

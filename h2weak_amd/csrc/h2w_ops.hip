@@ -103,11 +103,13 @@ struct Pair {                                  // semi-nested vertex pair (B <- 
 
 struct Launch {
     int64_t item0, nitems, term0, nterms;
+    int role;  // 0 transfer pass, 1 M2L + near field, 2 semi-nested column factors
 };
 
 struct WorkList {
     std::vector<GItem> items;
     std::vector<GTerm> terms;
+    int role = 0;
 };
 
 struct TermSpec {
@@ -250,90 +252,112 @@ double diag_value(int dim, int choice, int64_t N, int compat) {
 }
 
 // ------------------------------------------------------------------ batched ACA driver
-// Runs every block through the device ACA in workspace-bounded batches; blocks
-// that reach their rank cap before converging are re-run with a 4x larger cap.
-void run_aca(Handle &h, std::vector<AcaBlock> &blocks) {
-    if (blocks.empty()) return;
-    const double tol = std::pow(10.0, -1.0 * h.tol_pow);
-    hipStream_t s = h.stream;
+// Runs every block through the device ACA.  A batch shares one workspace pool
+// from which each block takes one (U, V) column pair per step; batches are
+// admitted by an estimated rank (rank_hint), and a block that runs out of pool
+// is re-run in a later batch with a larger estimate (alone if needed).
+struct AcaPool {
+    DevBuf<double> ws;
+    DevBuf<unsigned long long> top;
+    int64_t cap = 0;  // doubles
+};
+
+void ensure_pool(Handle &h, AcaPool &P) {
+    if (P.cap > 0) return;
     int64_t budget = h.opts.workspace_bytes;
     if (budget <= 0) {
         size_t fr = 0, tot = 0;
         H2W_HIP(hipMemGetInfo(&fr, &tot));
-        budget = std::min<int64_t>((int64_t)8 << 30, (int64_t)(fr / 4));
+        budget = std::min<int64_t>((int64_t)96 << 30, (int64_t)(fr / 2));
         budget = std::max<int64_t>(budget, (int64_t)64 << 20);
     }
-    std::vector<int> cap(blocks.size());
+    P.cap = budget / 8;
+    P.ws.alloc(P.cap);
+    P.top.alloc(1);
+}
+
+void run_aca(Handle &h, std::vector<AcaBlock> &blocks, int rank_hint, AcaPool &pool) {
+    if (blocks.empty()) return;
+    const double tol = std::pow(10.0, -1.0 * h.tol_pow);
+    hipStream_t s = h.stream;
+    const bool verbose = std::getenv("H2W_VERBOSE") != nullptr;
+    ensure_pool(h, pool);
+    std::vector<int> est(blocks.size());
     for (size_t b = 0; b < blocks.size(); ++b) {
         int mn = (int)std::min(blocks[b].rows.size(), blocks[b].cols.size());
-        cap[b] = std::min(mn, std::max(1, h.opts.rank_cap));
+        est[b] = std::min(mn, std::max(1, rank_hint));
     }
     std::vector<size_t> todo(blocks.size());
     std::iota(todo.begin(), todo.end(), 0);
-    DevBuf<double> ws;
     DevBuf<uint8_t> flags;
     DevBuf<int32_t> didx, drank, dstat, drp, dcp;
+    DevBuf<int64_t> dtab;
     DevBuf<AcaJob> djobs;
     DevBuf<LrJob> dlr;
     DevBuf<double> dlrout;
+    const size_t kBig = 16384;  // m + n above which a block gets a 1024-thread workgroup
     while (!todo.empty()) {
         std::vector<size_t> rerun;
         size_t pos = 0;
         while (pos < todo.size()) {
-            // form a batch (big blocks get 1024-thread workgroups in their own launch)
             std::vector<size_t> batch;
-            int64_t wsz = 0, isz = 0;
-            bool big = false;
-            {
-                size_t b0 = todo[pos];
-                big = blocks[b0].rows.size() + blocks[b0].cols.size() > 65536;
-            }
+            int64_t need = 0, isz = 0;
             while (pos < todo.size()) {
                 size_t b = todo[pos];
                 const AcaBlock &B = blocks[b];
-                bool bb = B.rows.size() + B.cols.size() > 65536;
-                if (bb != big) break;
-                int64_t need = align_up((int64_t)(B.rows.size() + B.cols.size()) * cap[b], 32);
-                if (!batch.empty() && (wsz + need) * 8 > budget) break;
+                int64_t nb = (int64_t)(B.rows.size() + B.cols.size()) * est[b];
+                if (!batch.empty() && need + nb > pool.cap) break;
                 batch.push_back(b);
-                wsz += need;
+                need += nb;
                 isz += (int64_t)(B.rows.size() + B.cols.size());
                 ++pos;
                 if (batch.size() >= 65535) break;
             }
+            // big blocks first, each group contiguous
+            std::stable_partition(batch.begin(), batch.end(), [&](size_t b) {
+                return blocks[b].rows.size() + blocks[b].cols.size() > kBig;
+            });
+            size_t nbig = 0;
+            while (nbig < batch.size() && blocks[batch[nbig]].rows.size() + blocks[batch[nbig]].cols.size() > kBig)
+                ++nbig;
             std::vector<AcaJob> jobs(batch.size());
             std::vector<int32_t> idx;
             idx.reserve(isz);
-            int64_t wo = 0, fo = 0, po = 0;
+            int64_t fo = 0, po = 0, to = 0;
             for (size_t q = 0; q < batch.size(); ++q) {
                 const AcaBlock &B = blocks[batch[q]];
                 AcaJob &J = jobs[q];
                 J.m = (int32_t)B.rows.size();
                 J.n = (int32_t)B.cols.size();
-                J.rcap = cap[batch[q]];
+                J.tcap = std::min(std::min(J.m, J.n), 8192);
                 J.pad = 0;
                 J.rows = (int64_t)idx.size();
                 idx.insert(idx.end(), B.rows.begin(), B.rows.end());
                 J.cols = (int64_t)idx.size();
                 idx.insert(idx.end(), B.cols.begin(), B.cols.end());
-                J.wsU = wo;
-                J.wsV = wo + (int64_t)J.m * J.rcap;
-                wo += align_up((int64_t)(J.m + J.n) * J.rcap, 32);
+                J.tab = to;
+                to += 2LL * J.tcap;
                 J.wsF = fo;
                 fo += align_up((int64_t)(J.m + J.n), 64);
                 J.piv = po;
-                po += J.rcap;
+                po += J.tcap;
             }
-            ws.ensure(std::max<int64_t>(wo, 1));
             flags.ensure(std::max<int64_t>(fo, 1));
+            dtab.ensure(std::max<int64_t>(to, 1));
             didx.upload(idx, s);
             djobs.upload(jobs, s);
             drank.ensure(batch.size());
             dstat.ensure(batch.size());
             drp.ensure(std::max<int64_t>(po, 1));
             dcp.ensure(std::max<int64_t>(po, 1));
-            launch_aca(djobs.p, (int)batch.size(), didx.p, ws.p, flags.p, drank.p, dstat.p, drp.p, dcp.p, h.kp, tol,
-                       big ? 1 : 0, s);
+            H2W_HIP(hipMemsetAsync(pool.top.p, 0, sizeof(unsigned long long), s));
+            double tb = now_s();
+            if (nbig > 0)
+                launch_aca(djobs.p, (int)nbig, didx.p, pool.ws.p, pool.cap, pool.top.p, dtab.p, flags.p, drank.p,
+                           dstat.p, drp.p, dcp.p, h.kp, tol, 1, s);
+            if (batch.size() > nbig)
+                launch_aca(djobs.p + nbig, (int)(batch.size() - nbig), didx.p, pool.ws.p, pool.cap, pool.top.p,
+                           dtab.p, flags.p, drank.p + nbig, dstat.p + nbig, drp.p, dcp.p, h.kp, tol, 0, s);
             H2W_HIP(hipGetLastError());
             std::vector<int32_t> rk(batch.size()), st(batch.size()), rp(po), cp(po);
             H2W_HIP(hipMemcpyAsync(rk.data(), drank.p, rk.size() * 4, hipMemcpyDeviceToHost, s));
@@ -341,7 +365,15 @@ void run_aca(Handle &h, std::vector<AcaBlock> &blocks) {
             H2W_HIP(hipMemcpyAsync(rp.data(), drp.p, po * 4, hipMemcpyDeviceToHost, s));
             H2W_HIP(hipMemcpyAsync(cp.data(), dcp.p, po * 4, hipMemcpyDeviceToHost, s));
             H2W_HIP(hipStreamSynchronize(s));
-            // L and R of the converged blocks
+            if (verbose) {
+                int mr = 0, nov = 0;
+                for (size_t q = 0; q < batch.size(); ++q) {
+                    mr = std::max(mr, rk[q]);
+                    nov += st[q] != 0;
+                }
+                std::fprintf(stderr, "[h2w]   aca batch: %zu blocks (%zu big), max rank %d, %d re-run, %.3f s\n",
+                             batch.size(), nbig, mr, nov, now_s() - tb);
+            }
             std::vector<LrJob> lj;
             std::vector<size_t> lb;
             int64_t lo = 0;
@@ -349,7 +381,9 @@ void run_aca(Handle &h, std::vector<AcaBlock> &blocks) {
                 size_t b = batch[q];
                 if (st[q] != 0) {
                     int mn = (int)std::min(blocks[b].rows.size(), blocks[b].cols.size());
-                    cap[b] = std::min(mn, cap[b] * 4);
+                    if (batch.size() == 1 && est[b] >= mn)
+                        throw Error{H2W_ENOMEM, "ACA workspace pool too small for one block"};
+                    est[b] = std::min(mn, std::max(est[b] * 4, rk[q] * 2));
                     rerun.push_back(b);
                     h.aca_reruns++;
                     continue;
@@ -360,8 +394,7 @@ void run_aca(Handle &h, std::vector<AcaBlock> &blocks) {
                 B.cb.assign(cp.begin() + jobs[q].piv, cp.begin() + jobs[q].piv + rk[q]);
                 if (rk[q] > 0) {
                     LrJob J;
-                    J.wsU = jobs[q].wsU;
-                    J.wsV = jobs[q].wsV;
+                    J.tab = jobs[q].tab;
                     J.piv = jobs[q].piv;
                     J.m = jobs[q].m;
                     J.n = jobs[q].n;
@@ -376,7 +409,7 @@ void run_aca(Handle &h, std::vector<AcaBlock> &blocks) {
             if (!lj.empty()) {
                 dlr.upload(lj, s);
                 dlrout.ensure(lo);
-                launch_lr_extract(dlr.p, (int)lj.size(), ws.p, drp.p, dcp.p, dlrout.p, s);
+                launch_lr_extract(dlr.p, (int)lj.size(), pool.ws.p, dtab.p, drp.p, dcp.p, dlrout.p, s);
                 H2W_HIP(hipGetLastError());
                 std::vector<double> hl(lo);
                 H2W_HIP(hipMemcpyAsync(hl.data(), dlrout.p, lo * 8, hipMemcpyDeviceToHost, s));
@@ -398,7 +431,7 @@ std::vector<int32_t> range_list(int64_t start, int64_t count) {
 }
 
 // far field, bottom-up NCA (3D/codes/HODLR3Dtree.hpp:1872-1939)
-void far_nca(Handle &h) {
+void far_nca(Handle &h, AcaPool &pool) {
     const Plan &P = h.plan;
     auto &F = h.fam[0];
     F.assign(P.nL + 1, FamLevel());
@@ -420,7 +453,10 @@ void far_nca(Handle &h) {
             blocks[b].rows = fl.S[b];
             for (int f : P.far_of(j, b)) blocks[b].cols.insert(blocks[b].cols.end(), fl.S[f].begin(), fl.S[f].end());
         }
-        run_aca(h, blocks);
+        // the far rank is bounded by the row-set size (children's skeletons)
+        int hint = 0;
+        for (auto &B : blocks) hint = std::max<int>(hint, (int)B.rows.size());
+        run_aca(h, blocks, hint, pool);
         fl.rank.resize(fl.nb);
         fl.Ir.resize(fl.nb);
         fl.Ic.resize(fl.nb);
@@ -434,12 +470,13 @@ void far_nca(Handle &h) {
             }
             fl.lr[b] = std::move(B.lr);
         }
+        if (std::getenv("H2W_VERBOSE")) std::fprintf(stderr, "[h2w]  far level %d done\n", j);
     }
 }
 
 // vertex-sharing field, top-down NCA with the parent's column pivots appended
 // (3D/codes/HODLR3Dtree.hpp:2062-2107; 1D/codes/HODLR1Dtree.hpp:231-276)
-void vertex_nca(Handle &h) {
+void vertex_nca(Handle &h, AcaPool &pool) {
     const Plan &P = h.plan;
     auto &F = h.fam[1];
     F.assign(P.nL + 1, FamLevel());
@@ -458,7 +495,14 @@ void vertex_nca(Handle &h) {
                 blocks[b].cols.insert(blocks[b].cols.end(), pic.begin(), pic.end());
             }
         }
-        run_aca(h, blocks);
+        // ranks shrink with the box size: start from twice the parent level's largest rank
+        int hint = std::max(16, h.opts.rank_cap);
+        if (j > 1) {
+            hint = 0;
+            for (int r : F[j - 1].rank) hint = std::max(hint, r);
+            hint = std::max(16, 2 * hint);
+        }
+        run_aca(h, blocks, hint, pool);
         fl.rank.resize(fl.nb);
         fl.Ir.resize(fl.nb);
         fl.Ic.resize(fl.nb);
@@ -472,6 +516,7 @@ void vertex_nca(Handle &h) {
             }
             fl.lr[b] = std::move(B.lr);
         }
+        if (std::getenv("H2W_VERBOSE")) std::fprintf(stderr, "[h2w]  vertex level %d done\n", j);
     }
     for (int j = 1; j <= P.nL; ++j) {
         FamLevel &fl = F[j];
@@ -489,7 +534,7 @@ void vertex_nca(Handle &h) {
 
 // semi-nested vertex blocks: one cross approximation per directed pair and level
 // (3D/codes/HODLR3Dtree.hpp:2250-2321; 1D/codes/HODLR1Dtree.hpp:418-443)
-void cur_pairs(Handle &h) {
+void cur_pairs(Handle &h, AcaPool &pool) {
     const Plan &P = h.plan;
     h.pairs.clear();
     std::vector<AcaBlock> blocks;
@@ -507,7 +552,7 @@ void cur_pairs(Handle &h) {
                 B.cols = range_list(P.bstart[j][v], P.bcount[j][v]);
                 blocks.push_back(std::move(B));
             }
-    run_aca(h, blocks);
+    run_aca(h, blocks, std::max(16, h.opts.rank_cap), pool);
     for (size_t q = 0; q < blocks.size(); ++q) {
         Pair &p = h.pairs[q];
         AcaBlock &B = blocks[q];
@@ -586,15 +631,18 @@ void assemble(Handle &h) {
     const Plan &P = h.plan;
     hipStream_t s = h.stream;
     h.aca_reruns = 0;
-    if (P.dim >= 2) far_nca(h);
-    else h.fam[0].assign(P.nL + 1, FamLevel());
-    phase("far NCA");
-    if (h.variant == H2W_NESTED) vertex_nca(h);
-    else {
-        h.fam[1].assign(P.nL + 1, FamLevel());
-        cur_pairs(h);
+    {
+        AcaPool pool;
+        if (P.dim >= 2) far_nca(h, pool);
+        else h.fam[0].assign(P.nL + 1, FamLevel());
+        phase("far NCA");
+        if (h.variant == H2W_NESTED) vertex_nca(h, pool);
+        else {
+            h.fam[1].assign(P.nL + 1, FamLevel());
+            cur_pairs(h, pool);
+        }
+        phase("vertex NCA / CUR");
     }
-    phase("vertex NCA / CUR");
     const int nfam_lo[2] = {2, 1};  // first level of each family
     const bool fam_on[2] = {P.dim >= 2, h.variant == H2W_NESTED};
 
@@ -850,6 +898,7 @@ void assemble(Handle &h) {
     // M2L (every level, both families) + near field
     {
         WorkList W;
+        W.role = 1;
         for (int f = 0; f < 2; ++f) {
             if (!fam_on[f]) continue;
             for (int j = nfam_lo[f]; j <= P.nL; ++j) {
@@ -899,6 +948,7 @@ void assemble(Handle &h) {
     // semi-nested column factors into y (boxes of all levels overlap: atomic)
     if (!h.pairs.empty()) {
         WorkList W;
+        W.role = 2;
         std::vector<std::vector<std::vector<TermSpec>>> per(P.nL + 1);
         for (int j = 1; j <= P.nL; ++j) per[j].assign(P.nboxes[j], {});
         for (auto &p : h.pairs)
@@ -918,6 +968,7 @@ void assemble(Handle &h) {
         Lh.nitems = (int64_t)W.items.size();
         Lh.term0 = (int64_t)terms.size();
         Lh.nterms = (int64_t)W.terms.size();
+        Lh.role = W.role;
         for (auto it : W.items) {
             it.t0 += (int32_t)Lh.term0;
             it.t1 += (int32_t)Lh.term0;
@@ -953,11 +1004,11 @@ void run_launches(Handle &h, int precision) {
     if (precision == H2W_FP32) {
         launch_fill32(h.vec32.p + h.off_y, h.vec_len - h.off_y, 0.0f, s);
         for (auto &Lh : h.launches)
-            launch_bgemv32(h.d_items.p + Lh.item0, (int)Lh.nitems, h.d_terms.p, h.mat32.p, h.vec32.p, s);
+            launch_bgemv32(h.d_items.p + Lh.item0, (int)Lh.nitems, h.d_terms.p, h.mat32.p, h.vec32.p, Lh.role, s);
     } else {
         launch_fill(h.vec64.p + h.off_y, h.vec_len - h.off_y, 0.0, s);
         for (auto &Lh : h.launches)
-            launch_bgemv64(h.d_items.p + Lh.item0, (int)Lh.nitems, h.d_terms.p, h.mat64.p, h.vec64.p, s);
+            launch_bgemv64(h.d_items.p + Lh.item0, (int)Lh.nitems, h.d_terms.p, h.mat64.p, h.vec64.p, Lh.role, s);
     }
     H2W_HIP(hipGetLastError());
 }
@@ -1157,7 +1208,7 @@ void h2w_default_opts(h2w_opts *o) {
     o->nlevels = 0;
     o->fp32_shadow = 1;
     o->device = 0;
-    o->rank_cap = 128;
+    o->rank_cap = 256;
     o->workspace_bytes = 0;
     o->plan_only = 0;
 }
@@ -1550,9 +1601,9 @@ int h2w_bench_matvec(h2w_handle *h, int nrep, int precision, h2w_bench *out) {
         for (int q = 0; q < nl; ++q) {
             auto &Lh = h->launches[q];
             if (precision == H2W_FP32)
-                h2w::launch_bgemv32(h->d_items.p + Lh.item0, (int)Lh.nitems, h->d_terms.p, h->mat32.p, h->vec32.p, s);
+                h2w::launch_bgemv32(h->d_items.p + Lh.item0, (int)Lh.nitems, h->d_terms.p, h->mat32.p, h->vec32.p, Lh.role, s);
             else
-                h2w::launch_bgemv64(h->d_items.p + Lh.item0, (int)Lh.nitems, h->d_terms.p, h->mat64.p, h->vec64.p, s);
+                h2w::launch_bgemv64(h->d_items.p + Lh.item0, (int)Lh.nitems, h->d_terms.p, h->mat64.p, h->vec64.p, Lh.role, s);
             H2W_HIP(hipEventRecord(ev[q + 2], s));
         }
         H2W_HIP(hipEventSynchronize(ev[nl + 1]));
@@ -1566,10 +1617,11 @@ int h2w_bench_matvec(h2w_handle *h, int nrep, int precision, h2w_bench *out) {
     const int es = precision == H2W_FP32 ? 4 : 8;
     out->launch_ms[0] = acc[0] / nrep;
     out->launch_bytes[0] = (h->vec_len - h->off_y) * (int64_t)es;
+    out->launch_role[0] = -1;
     for (int q = 0; q < nl; ++q) {
         out->launch_ms[q + 1] = acc[q + 1] / nrep;
         auto &Lh = h->launches[q];
-        (void)Lh;
+        out->launch_role[q + 1] = Lh.role;
         out->launch_bytes[q + 1] = h->launch_elems[q] * es;
     }
     for (auto &e : ev) (void)hipEventDestroy(e);

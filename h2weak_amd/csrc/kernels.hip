@@ -152,34 +152,56 @@ __device__ __forceinline__ void block_reduce(BlockRed<NT> &sh, double &s0, doubl
 //     the newest term, as the reference's loop does, 2D/codes/aca.hpp:404-409).
 // Residual rows/columns subtract the previous terms one by one in rank order
 // with separate multiply and subtract, as the reference does.
-// U (m x rcap) and V (n x rcap) live in global workspace; pivots go to
-// rowpiv/colpiv.  status = 1 when the block reached rcap before converging
-// (the host re-runs it with a larger cap).
+// U and V columns live in a workspace pool shared by the batch: before each
+// step the workgroup's leader reserves m + n doubles with an atomic bump; the
+// column offsets go to the job's table.  status = 1 when the table or the pool
+// is full before the cross converged (the host re-runs the block alone).
 constexpr int kChunk = 512;
 
 template <int NT>
+__device__ __forceinline__ bool aca_reserve(int64_t *tab, int k, int m, int n, int tcap,
+                                            unsigned long long *ws_top, int64_t ws_cap, int64_t *sh_off) {
+    if (threadIdx.x == 0) {
+        int64_t off = -1;
+        if (k < tcap) {
+            unsigned long long o = atomicAdd(ws_top, (unsigned long long)(m + n));
+            if ((int64_t)o + m + n <= ws_cap) off = (int64_t)o;
+        }
+        *sh_off = off;
+        if (off >= 0) {
+            tab[2 * k] = off;
+            tab[2 * k + 1] = off + m;
+        }
+    }
+    __syncthreads();
+    return *sh_off >= 0;
+}
+
+template <int NT>
 __global__ __launch_bounds__(NT) void k_aca(const AcaJob *__restrict__ jobs, const int32_t *__restrict__ idx,
-                                            double *__restrict__ ws, uint8_t *__restrict__ flags,
+                                            double *__restrict__ ws, int64_t ws_cap, unsigned long long *ws_top,
+                                            int64_t *__restrict__ tab_all, uint8_t *__restrict__ flags,
                                             int32_t *__restrict__ rank_out, int32_t *__restrict__ status_out,
                                             int32_t *__restrict__ rowpiv, int32_t *__restrict__ colpiv, KParams kp,
                                             double tol) {
     const AcaJob jb = jobs[blockIdx.x];
-    const int m = jb.m, n = jb.n, rcap = jb.rcap;
+    const int m = jb.m, n = jb.n, tcap = jb.tcap;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     constexpr int NW = NT / 64;
     const int32_t *rows = idx + jb.rows;
     const int32_t *cols = idx + jb.cols;
-    double *U = ws + jb.wsU;
-    double *V = ws + jb.wsV;
+    int64_t *tab = tab_all + jb.tab;
     uint8_t *ru = flags + jb.wsF;
     uint8_t *cu = ru + m;
     int32_t *rp_out = rowpiv + jb.piv;
     int32_t *cp_out = colpiv + jb.piv;
     __shared__ double coef[kChunk];
+    __shared__ int64_t cofs[kChunk];
     __shared__ BlockRed<NT> red;
     __shared__ double dacc[NW];
+    __shared__ int64_t sh_off;
 
-    if (m == 0 || n == 0 || rcap == 0) {
+    if (m == 0 || n == 0 || tcap == 0) {
         if (tid == 0) {
             rank_out[blockIdx.x] = 0;
             status_out[blockIdx.x] = 0;
@@ -189,6 +211,15 @@ __global__ __launch_bounds__(NT) void k_aca(const AcaJob *__restrict__ jobs, con
     for (int i = tid; i < m; i += NT) ru[i] = 0;
     for (int s = tid; s < n; s += NT) cu[s] = 0;
     const int mn = min(m, n);
+    if (!aca_reserve<NT>(tab, 0, m, n, tcap, ws_top, ws_cap, &sh_off)) {
+        if (tid == 0) {
+            rank_out[blockIdx.x] = 0;
+            status_out[blockIdx.x] = 1;
+        }
+        return;
+    }
+    double *U0 = ws + sh_off;
+    double *V0 = U0 + m;
 
     // ---- first row with a nonzero entry
     int l = 0, cp = -1;
@@ -198,7 +229,7 @@ __global__ __launch_bounds__(NT) void k_aca(const AcaJob *__restrict__ jobs, con
         AMax am{-1.0, INT_MAX, 0.0};
         for (int s = tid; s < n; s += NT) {
             double v = kentry(kp, pr, cols[s]);
-            V[s] = v;
+            V0[s] = v;
             amax_update(am, v, s);
         }
         double d0 = 0.0, d1 = 0.0;
@@ -225,11 +256,11 @@ __global__ __launch_bounds__(NT) void k_aca(const AcaJob *__restrict__ jobs, con
         for (int t = tid; t < m; t += NT) {
             double c = kentry(kp, pc, rows[t]);  // A(rows[t], cols[cp]) (symmetric entry)
             double u = c / piv;
-            U[t] = u;
+            U0[t] = u;
             un2 += u * u;
             if (t != rp) amax_update(an, c, t);
         }
-        for (int s = tid; s < n; s += NT) vn2 += V[s] * V[s];
+        for (int s = tid; s < n; s += NT) vn2 += V0[s] * V0[s];
     }
     block_reduce<NT>(red, un2, vn2, an);
     if (tid == 0) {
@@ -246,25 +277,29 @@ __global__ __launch_bounds__(NT) void k_aca(const AcaJob *__restrict__ jobs, con
     __syncthreads();
 
     while (un * vn > tol * normS && rank < mn) {
-        if (rank >= rcap) {
+        rp = next_rp;
+        if (rp == INT_MAX) break;
+        if (!aca_reserve<NT>(tab, rank, m, n, tcap, ws_top, ws_cap, &sh_off)) {
             status = 1;
             break;
         }
-        rp = next_rp;
-        if (rp == INT_MAX) break;
+        double *Ur = ws + sh_off;
+        double *Vr = Ur + m;
         const int nch = (rank + kChunk - 1) / kChunk;
-        // ---- residual row -> V[:, rank]
-        double *Vr = V + (int64_t)rank * n;
+        // ---- residual row -> V column `rank`
         PointI pr = load_point(kp, rows[rp]);
         AMax am{-1.0, INT_MAX, 0.0};
         vn2 = 0.0;
         for (int ch = 0; ch < nch; ++ch) {
             const int l0 = ch * kChunk, l1 = min(rank, l0 + kChunk);
-            for (int q = l0 + tid; q < l1; q += NT) coef[q - l0] = U[(int64_t)q * m + rp];
+            for (int q = l0 + tid; q < l1; q += NT) {
+                coef[q - l0] = ws[tab[2 * q] + rp];
+                cofs[q - l0] = tab[2 * q + 1];
+            }
             __syncthreads();
             for (int s = tid; s < n; s += NT) {
                 double r = ch == 0 ? kentry(kp, pr, cols[s]) : Vr[s];
-                for (int q = l0; q < l1; ++q) r -= coef[q - l0] * V[(int64_t)q * n + s];
+                for (int q = l0; q < l1; ++q) r -= coef[q - l0] * ws[cofs[q - l0] + s];
                 Vr[s] = r;
                 if (ch == nch - 1) {
                     vn2 += r * r;
@@ -278,18 +313,20 @@ __global__ __launch_bounds__(NT) void k_aca(const AcaJob *__restrict__ jobs, con
         if (am.idx == INT_MAX) break;
         cp = am.idx;
         piv = am.sv;
-        // ---- residual column -> U[:, rank]
-        double *Ur = U + (int64_t)rank * m;
+        // ---- residual column -> U column `rank`
         PointI pc = load_point(kp, cols[cp]);
         an = AMax{-1.0, INT_MAX, 0.0};
         un2 = 0.0;
         for (int ch = 0; ch < nch; ++ch) {
             const int l0 = ch * kChunk, l1 = min(rank, l0 + kChunk);
-            for (int q = l0 + tid; q < l1; q += NT) coef[q - l0] = V[(int64_t)q * n + cp];
+            for (int q = l0 + tid; q < l1; q += NT) {
+                coef[q - l0] = ws[tab[2 * q + 1] + cp];
+                cofs[q - l0] = tab[2 * q];
+            }
             __syncthreads();
             for (int t = tid; t < m; t += NT) {
                 double c = ch == 0 ? kentry(kp, pc, rows[t]) : Ur[t];
-                for (int q = l0; q < l1; ++q) c -= coef[q - l0] * U[(int64_t)q * m + t];
+                for (int q = l0; q < l1; ++q) c -= coef[q - l0] * ws[cofs[q - l0] + t];
                 if (ch == nch - 1) {
                     double u = c / piv;
                     Ur[t] = u;
@@ -314,15 +351,13 @@ __global__ __launch_bounds__(NT) void k_aca(const AcaJob *__restrict__ jobs, con
         ++rank;
         next_rp = an.idx;
         // ---- normS update: dots of the new term with every term
-        const double *un_ = U + (int64_t)(rank - 1) * m;
-        const double *vn_ = V + (int64_t)(rank - 1) * n;
         double acc = 0.0;
         for (int q = wv; q < rank; q += NW) {
-            const double *Uq = U + (int64_t)q * m;
-            const double *Vq = V + (int64_t)q * n;
+            const double *Uq = ws + tab[2 * q];
+            const double *Vq = ws + tab[2 * q + 1];
             double du = 0.0, dv = 0.0;
-            for (int t = lane; t < m; t += 64) du += Uq[t] * un_[t];
-            for (int s = lane; s < n; s += 64) dv += Vq[s] * vn_[s];
+            for (int t = lane; t < m; t += 64) du += Uq[t] * Ur[t];
+            for (int s = lane; s < n; s += 64) dv += Vq[s] * Vr[s];
             du = wave_sum(du);
             dv = wave_sum(dv);
             acc += fabs(du * dv);
@@ -344,20 +379,19 @@ __global__ __launch_bounds__(NT) void k_aca(const AcaJob *__restrict__ jobs, con
 // L (unit lower) and R (upper) of the cross: L(i,j) = U_j[rb_i], R(j,i) = V_j[cb_i]
 // (2D/codes/aca.hpp:419-445)
 __global__ void k_lr_extract(const LrJob *__restrict__ jobs, const double *__restrict__ ws,
-                             const int32_t *__restrict__ rowpiv, const int32_t *__restrict__ colpiv,
-                             double *__restrict__ out) {
+                             const int64_t *__restrict__ tab_all, const int32_t *__restrict__ rowpiv,
+                             const int32_t *__restrict__ colpiv, double *__restrict__ out) {
     const LrJob jb = jobs[blockIdx.x];
     const int r = jb.rank;
     double *L = out + jb.out;
     double *R = L + (int64_t)r * r;
-    const double *U = ws + jb.wsU;
-    const double *V = ws + jb.wsV;
+    const int64_t *tab = tab_all + jb.tab;
     for (int e = threadIdx.x; e < r * r; e += blockDim.x) {
         int i = e % r, j = e / r;  // row i, column j
         double lv = 0.0, rv = 0.0;
         if (i == j) lv = 1.0;
-        else if (j < i) lv = U[(int64_t)j * jb.m + rowpiv[jb.piv + i]];
-        if (i <= j) rv = V[(int64_t)i * jb.n + colpiv[jb.piv + j]];
+        else if (j < i) lv = ws[tab[2 * j] + rowpiv[jb.piv + i]];
+        if (i <= j) rv = ws[tab[2 * i + 1] + colpiv[jb.piv + j]];
         L[e] = lv;
         R[e] = rv;
     }
@@ -441,7 +475,9 @@ __global__ void k_cur(const CurJob *__restrict__ jobs, const int32_t *__restrict
 // the waves split every term's columns.  A is column-major, so each wave load
 // reads `rows` contiguous elements of one column.  Partial sums of the waves
 // are combined in LDS and added to y (exclusively or atomically).
-template <typename T>
+// ROLE only names the instance (0: transfer passes, 1: M2L + near field,
+// 2: semi-nested column factors) so that profiles separate the launches.
+template <typename T, int ROLE>
 __global__ __launch_bounds__(256) void k_bgemv(const GItem *__restrict__ items, const GTerm *__restrict__ terms,
                                                const T *__restrict__ A, T *vec) {
     const GItem it = items[blockIdx.x];
@@ -601,17 +637,19 @@ inline int grid_for(int64_t n, int nt = 256, int cap = 8192) {
 }  // namespace
 
 // ------------------------------------------------------------------ launch wrappers
-void launch_aca(const AcaJob *jobs, int njobs, const int32_t *idx, double *ws, uint8_t *flags, int32_t *rank,
-                int32_t *status, int32_t *rowpiv, int32_t *colpiv, KParams kp, double tol, int big, hipStream_t s) {
+void launch_aca(const AcaJob *jobs, int njobs, const int32_t *idx, double *ws, int64_t ws_cap,
+                unsigned long long *ws_top, int64_t *tab, uint8_t *flags, int32_t *rank, int32_t *status,
+                int32_t *rowpiv, int32_t *colpiv, KParams kp, double tol, int big, hipStream_t s) {
     if (njobs <= 0) return;
-    if (big) hipLaunchKernelGGL(k_aca<1024>, dim3(njobs), dim3(1024), 0, s, jobs, idx, ws, flags, rank, status,
-                                rowpiv, colpiv, kp, tol);
-    else hipLaunchKernelGGL(k_aca<256>, dim3(njobs), dim3(256), 0, s, jobs, idx, ws, flags, rank, status, rowpiv,
-                            colpiv, kp, tol);
+    if (big) hipLaunchKernelGGL(k_aca<1024>, dim3(njobs), dim3(1024), 0, s, jobs, idx, ws, ws_cap, ws_top, tab, flags,
+                                rank, status, rowpiv, colpiv, kp, tol);
+    else hipLaunchKernelGGL(k_aca<256>, dim3(njobs), dim3(256), 0, s, jobs, idx, ws, ws_cap, ws_top, tab, flags, rank,
+                            status, rowpiv, colpiv, kp, tol);
 }
-void launch_lr_extract(const LrJob *jobs, int njobs, const double *ws, const int32_t *rowpiv, const int32_t *colpiv,
-                       double *out, hipStream_t s) {
-    if (njobs > 0) hipLaunchKernelGGL(k_lr_extract, dim3(njobs), dim3(256), 0, s, jobs, ws, rowpiv, colpiv, out);
+void launch_lr_extract(const LrJob *jobs, int njobs, const double *ws, const int64_t *tab, const int32_t *rowpiv,
+                       const int32_t *colpiv, double *out, hipStream_t s) {
+    if (njobs > 0)
+        hipLaunchKernelGGL(k_lr_extract, dim3(njobs), dim3(256), 0, s, jobs, ws, tab, rowpiv, colpiv, out);
 }
 void launch_eval(const EvalJob *jobs, int njobs, const int32_t *idx, double *A, KParams kp, hipStream_t s) {
     if (njobs > 0) hipLaunchKernelGGL(k_eval, dim3(njobs), dim3(256), 0, s, jobs, idx, A, kp);
@@ -624,13 +662,21 @@ void launch_cur(const CurJob *jobs, int njobs, const int32_t *idx, const double 
                 hipStream_t s) {
     if (njobs > 0) hipLaunchKernelGGL(k_cur, dim3(njobs), dim3(256), 0, s, jobs, idx, lr, A, kp);
 }
-void launch_bgemv64(const GItem *items, int nitems, const GTerm *terms, const double *A, double *vec,
-                    hipStream_t s) {
-    if (nitems > 0) hipLaunchKernelGGL(k_bgemv<double>, dim3(nitems), dim3(256), 0, s, items, terms, A, vec);
+template <typename T>
+static void launch_bgemv_t(const GItem *items, int nitems, const GTerm *terms, const T *A, T *vec, int role,
+                           hipStream_t s) {
+    if (nitems <= 0) return;
+    if (role == 1) hipLaunchKernelGGL((k_bgemv<T, 1>), dim3(nitems), dim3(256), 0, s, items, terms, A, vec);
+    else if (role == 2) hipLaunchKernelGGL((k_bgemv<T, 2>), dim3(nitems), dim3(256), 0, s, items, terms, A, vec);
+    else hipLaunchKernelGGL((k_bgemv<T, 0>), dim3(nitems), dim3(256), 0, s, items, terms, A, vec);
 }
-void launch_bgemv32(const GItem *items, int nitems, const GTerm *terms, const float *A, float *vec,
+void launch_bgemv64(const GItem *items, int nitems, const GTerm *terms, const double *A, double *vec, int role,
                     hipStream_t s) {
-    if (nitems > 0) hipLaunchKernelGGL(k_bgemv<float>, dim3(nitems), dim3(256), 0, s, items, terms, A, vec);
+    launch_bgemv_t<double>(items, nitems, terms, A, vec, role, s);
+}
+void launch_bgemv32(const GItem *items, int nitems, const GTerm *terms, const float *A, float *vec, int role,
+                    hipStream_t s) {
+    launch_bgemv_t<float>(items, nitems, terms, A, vec, role, s);
 }
 void launch_to_fp32(const double *a, float *b, int64_t n, hipStream_t s) {
     if (n > 0) hipLaunchKernelGGL(k_to_fp32, dim3(grid_for(n)), dim3(256), 0, s, a, b, n);

@@ -63,9 +63,10 @@ typedef struct {
                                 ceil(log_{2^d}(N / n_max)) ({1,2,3}D/codes/main.cpp:11-13)             */
     int fp32_shadow;         /* 1 (default): also keep an FP32 copy of every stored operator     */
     int device;              /* HIP device ordinal (default 0)                                   */
-    int rank_cap;            /* first-pass ACA rank cap per block (default 128); blocks that
-                                reach it are re-run with a larger cap, so results do not depend
-                                on it                                                            */
+    int rank_cap;            /* first-pass rank estimate (default 256) used to admit ACA blocks
+                                of the top vertex level and of the semi-nested pairs to a shared
+                                workspace batch; a block that outgrows the batch's pool is re-run,
+                                so results do not depend on it                                   */
     int64_t workspace_bytes; /* ACA workspace budget in bytes (0 = automatic)                    */
     int plan_only;           /* 1: build the tree and lists only, touch no device (assembly and
                                 products then return H2W_ESTATE); for host-side inspection        */
@@ -150,6 +151,8 @@ typedef struct {
     double launch_ms[H2W_MAX_LAUNCHES];
     int n_launch;
     int64_t launch_bytes[H2W_MAX_LAUNCHES];
+    int launch_role[H2W_MAX_LAUNCHES]; /* -1 zero fill, 0 transfer pass, 1 M2L + near field,
+                                          2 semi-nested column factors                       */
 } h2w_bench;
 int h2w_bench_matvec(h2w_handle *h, int nrep, int precision, h2w_bench *out);
 

@@ -16,7 +16,7 @@ def pytest_configure(config):
 def gpu_available():
     try:
         import ctypes
-        hip = ctypes.CDLL("libamdhip64.so")
+        hip = ctypes.CDLL("libamdhip64.so.7")
         n = ctypes.c_int(0)
         return hip.hipGetDeviceCount(ctypes.byref(n)) == 0 and n.value > 0
     except OSError:
