@@ -7,9 +7,12 @@ RBF kernel (choice=1) on the reference's uniform grid 64^3 (N = 262144) in
 1e-10, x_true ~ U(-1,1) (seed 2), b = A x_true from the GPU's dense FP64
 product, x0 = 1 (as the reference's main).
 
-A step is one GMRES solve to 1e-10 with FP32 operators inside FP64 iterative
-refinement, b and x resident in HBM.  The FP64 solve on the stored FP64
-operators is timed beside it.  Matvec bandwidth is the reference's stored
+A step is one GMRES solve to 1e-10 inside FP64 iterative refinement: residuals
+on the stored FP64 operators, corrections from GMRES on the FP32 inner
+operator (every box truncated to the leading pivots at which its cross met
+1e-6; P2P/M2L in FP32 with FP32 accumulation), b and x resident in HBM.  The
+plain FP64 solve on the stored FP64 operators, the refinement around the full
+FP32 operators and around on-the-fly FP32 P2P/M2L are timed beside it.  Matvec bandwidth is the reference's stored
 element count M (findMemory_n) times the element size plus the two N-vectors,
 per HIP-event-timed matvec.  The CPU baseline is the oracle (a C++/OpenMP
 restatement of the reference) on a bounded sample.
@@ -28,6 +31,11 @@ sys.path.insert(0, ROOT)
 
 METRIC = "GMRES time-to-1e-10 (s) & matvec GB/s: FP32+refine batched vs FP64 GPU, 1 GPU"
 HBM_PEAK_GBPS = 8000.0
+FP32_PEAK_TFLOPS = 157.3
+# flops of one 3D RBF entry evaluated on the fly (add/mul = 1, fma = 2, sqrt/log = 1):
+# 3 differences, R^2 (mul + 2 fma), sqrt, log, scale, accumulate (fma)
+FLOP_PER_ENTRY = 3 + 5 + 1 + 1 + 1 + 2
+HEADLINE = "fp32_inner_ir"
 
 
 def log(*a):
@@ -130,56 +138,75 @@ def main():
         assert rc == h2w.H2W_OK, (rc, L.h2w_last_error())
         return s.as_dict()
 
+    modes = {"fp32_inner_ir": h2w.GMRES_FP32_INNER_IR, "fp32_ir": h2w.GMRES_FP32_IR,
+             "fp32_otf_ir": h2w.GMRES_FP32_OTF_IR, "fp64": h2w.GMRES_FP64}
     for _ in range(args.warmup):
-        solve(h2w.GMRES_FP32_IR)
-        solve(h2w.GMRES_FP64)
-    hip.sync()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        s32 = solve(h2w.GMRES_FP32_IR)
-    hip.sync()
-    t_step = (time.perf_counter() - t0) / args.steps
-    x32 = np.empty(N)
-    hip.d2h(x32, dx)
-    e32 = float(np.linalg.norm(x32 - xt) / np.linalg.norm(xt))
-    log(f"FP32+IR solve {t_step * 1e3:.2f} ms: {s32}")
-    nfp64 = max(1, min(args.steps, 3))
-    hip.sync()
-    t0 = time.perf_counter()
-    for _ in range(nfp64):
-        s64 = solve(h2w.GMRES_FP64)
-    hip.sync()
-    t64 = (time.perf_counter() - t0) / nfp64
-    x64 = np.empty(N)
-    hip.d2h(x64, dx)
-    e64 = float(np.linalg.norm(x64 - xt) / np.linalg.norm(xt))
-    log(f"FP64 solve {t64 * 1e3:.2f} ms: {s64}")
+        for m in modes.values():
+            solve(m)
+    res = {}
+    for name, m in modes.items():
+        k = args.steps if name == HEADLINE else max(1, min(args.steps, 3))
+        hip.sync()
+        t0 = time.perf_counter()
+        for _ in range(k):
+            st_ = solve(m)
+        hip.sync()
+        t = (time.perf_counter() - t0) / k
+        xs = np.empty(N)
+        hip.d2h(xs, dx)
+        res[name] = {"s": t, "stats": st_, "err": float(np.linalg.norm(xs - xt) / np.linalg.norm(xt)), "runs": k}
+        log(f"{name}: {t * 1e3:.2f} ms/solve, error {res[name]['err']:.2e}, {st_}")
+    t_step = res[HEADLINE]["s"]
+    t64 = res["fp64"]["s"]
 
     M = st["memory"]
     mv = {}
-    roof = None
-    for prec, name, s_op in ((h2w.FP64, "fp64", 8), (h2w.FP32, "fp32", 4)):
-        r = A.bench_matvec(20, prec)
+    kinfo = {}
+    for prec, name, s_op in ((h2w.FP64, "fp64", 8), (h2w.FP32, "fp32", 4), (h2w.FP32_INNER, "fp32_inner", 4),
+                             (h2w.FP32_OTF, "fp32_otf", 4), (h2w.FP64_OTF, "fp64_otf", 8)):
+        r = A.bench_matvec(10, prec)
         alg = s_op * M + 2 * 8 * N
         mv[name] = {"ms": r["ms_per_matvec"], "alg_GBps": alg / (r["ms_per_matvec"] * 1e-3) * 1e-9,
                     "launch_ms": [round(v, 4) for v in r["launch_ms"]],
                     "launch_GB": [round(v * 1e-9, 4) for v in r["launch_bytes"]]}
-        log(f"matvec {name}: {r['ms_per_matvec']:.3f} ms, {mv[name]['alg_GBps']:.0f} GB/s algorithmic")
-        if prec == h2w.FP32:
-            # dominant kernel of the measured path: the FP32 M2L + near-field launch
-            q = r["launch_role"].index(1)
-            nbytes = r["launch_bytes"][q] + 4 * N * 2  # stored elements + x read + y written
-            ach = nbytes / (r["launch_ms"][q] * 1e-3) * 1e-9
-            roof = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-                    "frac": round(ach / HBM_PEAK_GBPS, 4), "traffic": None,
-                    "kernel": "k_bgemv<float, 1> (M2L + near field)", "launch_ms": round(r["launch_ms"][q], 4),
-                    "algorithmic_bytes": int(nbytes)}
-            tr_path = os.path.join(ROOT, "profiles", "pmc_traffic.json")
-            if os.path.exists(tr_path):
-                tr = json.load(open(tr_path))
-                key = f"3d_rbf_{side}_{tp}_fp32_m2l_near"
-                if key in tr:
-                    roof["traffic"] = tr[key]["hbm_bytes_per_launch"]
+        q = r["launch_role"].index(1)
+        kinfo[name] = (r["launch_ms"][q], r["launch_bytes"][q] // s_op)  # (ms, entries) of the M2L + near launch
+        log(f"matvec {name}: {r['ms_per_matvec']:.3f} ms ({mv[name]['alg_GBps']:.0f} GB/s algorithmic); "
+            f"M2L+near launch {r['launch_ms'][q]:.3f} ms")
+    # dominant kernel of the measured solve: the inner FP32 M2L + near launches (truncated operator)
+    # against the outer FP64 stored M2L + near launches
+    so = res[HEADLINE]["stats"]
+    t_in = so["matvecs_fp32"] * kinfo["fp32_inner"][0]
+    t_st = so["matvecs_fp64"] * kinfo["fp64"][0]
+    ms_i, ent_i = kinfo["fp32_inner"]
+    nb_i = 4 * ent_i + 2 * 4 * N
+    roof_in = {"bound": "hbm", "achieved": round(nb_i / (ms_i * 1e-3) * 1e-9, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+               "frac": round(nb_i / (ms_i * 1e-3) * 1e-9 / HBM_PEAK_GBPS, 4), "traffic": None,
+               "kernel": "k_bgemv<float, 1> (M2L + near field of the truncated FP32 inner operator)",
+               "launch_ms": round(ms_i, 4), "algorithmic_bytes": int(nb_i)}
+    ms_s, ent_s = kinfo["fp64"]
+    nbytes = 8 * ent_s + 2 * 8 * N
+    ach = nbytes / (ms_s * 1e-3) * 1e-9
+    roof_st = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+               "frac": round(ach / HBM_PEAK_GBPS, 4), "traffic": None,
+               "kernel": "k_bgemv<double, 1> (M2L + near field, stored FP64)", "launch_ms": round(ms_s, 4),
+               "algorithmic_bytes": int(nbytes)}
+    ms_o, ent = kinfo["fp32_otf"]
+    flop = FLOP_PER_ENTRY * ent
+    roof_otf = {"bound": "valu", "achieved": round(flop / (ms_o * 1e-3) * 1e-12, 2), "peak": FP32_PEAK_TFLOPS,
+                "unit": "TFLOP/s", "frac": round(flop / (ms_o * 1e-3) * 1e-12 / FP32_PEAK_TFLOPS, 4),
+                "kernel": "k_otf<float> (M2L + near field, entries evaluated in FP32)",
+                "launch_ms": round(ms_o, 4), "entries": int(ent), "flop_per_entry": FLOP_PER_ENTRY}
+    tr_path = os.path.join(ROOT, "profiles", "pmc_traffic.json")
+    if os.path.exists(tr_path):
+        tr = json.load(open(tr_path))
+        key = f"3d_rbf_{side}_{tp}_fp64_m2l_near"
+        if key in tr:
+            roof_st["traffic"] = tr[key]["hbm_bytes_per_launch"]
+        key = f"3d_rbf_{side}_{tp}_fp32inner_m2l_near"
+        if key in tr:
+            roof_in["traffic"] = tr[key]["hbm_bytes_per_launch"]
+    roof, roof2 = (roof_in, roof_st) if t_in >= t_st else (roof_st, roof_in)
 
     out = {"metric": METRIC, "value": round(t_step, 6), "unit": "s", "n_gpus": 1, "steps": args.steps,
            "warmup": args.warmup, "ms_per_step": round(t_step * 1e3, 3), "higher_is_better": False,
@@ -187,18 +214,19 @@ def main():
            "data": "synthetic (reference uniform grid, x_true ~ U(-1,1) seed 2)",
            "config": {"workload": f"3D H2* RBF grid {side}^3 N={N} n_max=64 ACA tol 1e-{tp} GMRES tol 1e-{tp}",
                       "global_batch": 1, "seq_len": N, "parallelism": "single GPU"},
-           "fp64_gmres_s": round(t64, 6), "speedup_fp32ir_vs_fp64": round(t64 / t_step, 3),
-           "gmres_fp32ir": {k: s32[k] for k in ("iterations", "outer_iterations", "matvecs_fp64", "matvecs_fp32",
-                                                "resid")},
-           "gmres_fp64": {k: s64[k] for k in ("iterations", "matvecs_fp64", "resid")},
-           "rel_solution_error": {"fp32ir": e32, "fp64": e64},
+           "fp64_gmres_s": round(t64, 6), "fp32_stored_ir_gmres_s": round(res["fp32_ir"]["s"], 6),
+           "speedup_vs_fp64": round(t64 / t_step, 3),
+           "gmres": {k: {"s": round(v["s"], 6), "rel_solution_error": v["err"],
+                         **{kk: v["stats"][kk] for kk in ("iterations", "outer_iterations", "matvecs_fp64",
+                                                          "matvecs_fp32", "resid")}} for k, v in res.items()},
            "matvec": {k: {kk: (round(vv, 4) if isinstance(vv, float) else vv) for kk, vv in v.items()}
                       for k, v in mv.items()},
            "assembly_s": round(st["assembly_seconds"], 2), "memory_elements": M,
-           "roofline": roof, "cpu_baseline": None}
+           "roofline": roof, "roofline_other": roof2, "roofline_otf": roof_otf,
+           "inner_operator_bytes_fp32": st["stored_bytes_inner_fp32"], "cpu_baseline": None}
     if not args.no_cpu:
         log(f"CPU baseline (oracle) on grid {args.cpu_side}^3")
-        out["cpu_baseline"] = cpu_baseline(args.cpu_side, s64["matvecs_fp64"], M, tp, gtol)
+        out["cpu_baseline"] = cpu_baseline(args.cpu_side, res["fp64"]["stats"]["matvecs_fp64"], M, tp, gtol)
         out["speedup_vs_cpu"] = round(out["cpu_baseline"]["value"] / t_step, 2)
     print(json.dumps(out), flush=True)
 

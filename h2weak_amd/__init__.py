@@ -27,9 +27,9 @@ H2W_OK, H2W_EINVAL, H2W_ESTATE, H2W_EHIP, H2W_ENOMEM, H2W_ENOCONV, H2W_ENOLIB = 
 NESTED, SEMI_NESTED = 0, 1
 CHOICE_IE, CHOICE_RBF = 0, 1
 HOST, DEVICE = 0, 1
-FP64, FP32 = 0, 1
+FP64, FP32, FP32_OTF, FP64_OTF, FP32_INNER = 0, 1, 2, 3, 4
 ORDER_INPUT, ORDER_TREE = 0, 1
-GMRES_FP64, GMRES_FP32_IR = 0, 1
+GMRES_FP64, GMRES_FP32_IR, GMRES_FP32_OTF_IR, GMRES_FP32_INNER_IR = 0, 1, 2, 3
 
 # every symbol include/h2w.h declares
 EXPORTS = ("h2w_default_opts", "h2w_create", "h2w_assemble", "h2w_matvec", "h2w_matvec_ex", "h2w_gmres",
@@ -46,7 +46,7 @@ class H2WError(RuntimeError):
 class Opts(ctypes.Structure):
     _fields_ = [("compat", ctypes.c_int), ("nlevels", ctypes.c_int), ("fp32_shadow", ctypes.c_int),
                 ("device", ctypes.c_int), ("rank_cap", ctypes.c_int), ("workspace_bytes", ctypes.c_int64),
-                ("plan_only", ctypes.c_int)]
+                ("plan_only", ctypes.c_int), ("ir_tol_pow", ctypes.c_int)]
 
 
 class Stats(ctypes.Structure):
@@ -56,7 +56,7 @@ class Stats(ctypes.Structure):
                 ("max_rank_vertex", ctypes.c_int), ("mean_rank_far", ctypes.c_double),
                 ("mean_rank_vertex", ctypes.c_double), ("stored_bytes_fp64", ctypes.c_int64),
                 ("stored_bytes_fp32", ctypes.c_int64), ("n_launches_matvec", ctypes.c_int),
-                ("aca_reruns", ctypes.c_int)]
+                ("aca_reruns", ctypes.c_int), ("stored_bytes_inner_fp32", ctypes.c_int64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -131,12 +131,13 @@ class Initialisation:
 
     def __init__(self, dim, N, n_max, L=1.0, tol_pow=12, choice=CHOICE_RBF, variant=NESTED, points=None,
                  nlevels=0, compat=1, fp32_shadow=1, rank_cap=256, device=0, workspace_bytes=0,
-                 plan_only=0):
+                 plan_only=0, ir_tol_pow=6):
         self.dim, self.N = int(dim), int(N)
         o = Opts()
         lib().h2w_default_opts(ctypes.byref(o))
         o.compat, o.nlevels, o.fp32_shadow, o.device = compat, nlevels, fp32_shadow, device
         o.rank_cap, o.workspace_bytes, o.plan_only = rank_cap, workspace_bytes, plan_only
+        o.ir_tol_pow = ir_tol_pow
         self._pts = None
         pp = None
         if points is not None:

@@ -35,6 +35,18 @@ struct KParams {
     const double *px, *py, *pz;  // tree-order coordinates (device)
 };
 
+// Coordinates and point id of the point each vector-pool slot belongs to
+// (x/y slots: the tree-order points; o/u slots: the skeleton points), so that
+// near-field and M2L blocks can be evaluated on the fly from the work lists.
+struct SlotPoints {
+    const float *x, *y, *z;          // FP32 coordinates (rounded)
+    const float *xl, *yl, *zl;       // FP32 remainders: x = x + xl to ~2^-48 relative
+    const double *xd, *yd, *zd;      // FP64 coordinates
+    const int32_t *pid;              // tree-order point index, -1 for slots without a point
+    int dim, choice;
+    double diag;
+};
+
 // ------------------------------------------------------------------ host plan
 // Tree, tree-order permutation and interaction lists, built from integer box
 // coordinates (the reference builds them from hand-written child tables,
@@ -111,7 +123,7 @@ struct GItem {
 // ------------------------------------------------------------------ launch wrappers (kernels.hip)
 void launch_aca(const AcaJob *jobs, int njobs, const int32_t *idx, double *ws, int64_t ws_cap,
                 unsigned long long *ws_top, int64_t *tab, uint8_t *flags, int32_t *rank, int32_t *status,
-                int32_t *rowpiv, int32_t *colpiv, KParams kp, double tol, int big, hipStream_t s);
+                int32_t *rowpiv, int32_t *colpiv, KParams kp, double tol, double *hist, int big, hipStream_t s);
 void launch_lr_extract(const LrJob *jobs, int njobs, const double *ws, const int64_t *tab,
                        const int32_t *rowpiv, const int32_t *colpiv, double *out, hipStream_t s);
 void launch_eval(const EvalJob *jobs, int njobs, const int32_t *idx, double *A, KParams kp, hipStream_t s);
@@ -123,6 +135,8 @@ void launch_bgemv64(const GItem *items, int nitems, const GTerm *terms, const do
                     hipStream_t s);
 void launch_bgemv32(const GItem *items, int nitems, const GTerm *terms, const float *A, float *vec, int role,
                     hipStream_t s);
+void launch_otf32(const GItem *items, int nitems, const GTerm *terms, SlotPoints sp, float *vec, hipStream_t s);
+void launch_otf64(const GItem *items, int nitems, const GTerm *terms, SlotPoints sp, double *vec, hipStream_t s);
 void launch_to_fp32(const double *a, float *b, int64_t n, hipStream_t s);
 void launch_to_fp64(const float *a, double *b, int64_t n, hipStream_t s);
 void launch_fill(double *a, int64_t n, double v, hipStream_t s);

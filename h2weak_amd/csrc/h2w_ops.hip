@@ -80,6 +80,7 @@ struct AcaBlock {
     int rank = 0;
     std::vector<int32_t> rb, cb;  // local pivot indices
     std::vector<double> lr;       // L then R, rank x rank column-major each
+    std::vector<double> hist;     // per step: |u||v| and normS after the step
 };
 
 // skeleton data of one family at one level
@@ -88,6 +89,9 @@ struct FamLevel {
     std::vector<int> rank;
     std::vector<std::vector<int32_t>> Ir, Ic;  // tree-order indices
     std::vector<std::vector<double>> lr;
+    std::vector<std::vector<double>> hist;     // ACA history (|u||v|, normS) per step
+    std::vector<int> kin;                      // rank of the inner (truncated) operator
+    std::vector<int64_t> ein_off, eint_off;    // inner transfers in the FP32 pool
     std::vector<std::vector<int32_t>> S;       // transfer row set (leaf: box points; else children's Ir)
     std::vector<int64_t> o_off, u_off;         // vec pool
     std::vector<int64_t> e_off, et_off;        // mat pool
@@ -110,6 +114,14 @@ struct WorkList {
     std::vector<GItem> items;
     std::vector<GTerm> terms;
     int role = 0;
+};
+
+// Launch lists as uploaded to the device
+struct LaunchSet {
+    std::vector<Launch> launches;
+    std::vector<int64_t> elems;
+    DevBuf<GItem> d_items;
+    DevBuf<GTerm> d_terms;
 };
 
 struct TermSpec {
@@ -141,11 +153,18 @@ struct h2w_handle {
     int64_t mat_len = 0;
     h2w::DevBuf<double> vec64;
     h2w::DevBuf<float> vec32;
+    h2w::DevBuf<float> sx32, sy32, sz32;   // slot coordinates (FP32)
+    h2w::DevBuf<float> sxl, syl, szl;      // FP32 remainders of the slot coordinates
+    h2w::DevBuf<double> sx64, sy64, sz64;  // slot coordinates (FP64)
+    h2w::DevBuf<int32_t> spid;             // slot point ids
+    h2w::SlotPoints sp;
     int64_t vec_len = 0, off_x = 0, off_y = 0;
     std::vector<h2w::FamLevel> fam[2];
     std::vector<h2w::Pair> pairs;
     std::vector<h2w::Launch> launches;
     std::vector<int64_t> launch_elems;  // operator elements each launch streams
+    h2w::LaunchSet inner;               // the refinement's inner operator (ranks truncated)
+    int64_t inner_elems = 0;
     h2w::DevBuf<h2w::GItem> d_items;
     h2w::DevBuf<h2w::GTerm> d_terms;
     int64_t stored_elems = 0;
@@ -291,6 +310,7 @@ void run_aca(Handle &h, std::vector<AcaBlock> &blocks, int rank_hint, AcaPool &p
     std::iota(todo.begin(), todo.end(), 0);
     DevBuf<uint8_t> flags;
     DevBuf<int32_t> didx, drank, dstat, drp, dcp;
+    DevBuf<double> dhist;
     DevBuf<int64_t> dtab;
     DevBuf<AcaJob> djobs;
     DevBuf<LrJob> dlr;
@@ -350,16 +370,19 @@ void run_aca(Handle &h, std::vector<AcaBlock> &blocks, int rank_hint, AcaPool &p
             dstat.ensure(batch.size());
             drp.ensure(std::max<int64_t>(po, 1));
             dcp.ensure(std::max<int64_t>(po, 1));
+            dhist.ensure(std::max<int64_t>(2 * po, 2));
             H2W_HIP(hipMemsetAsync(pool.top.p, 0, sizeof(unsigned long long), s));
             double tb = now_s();
             if (nbig > 0)
                 launch_aca(djobs.p, (int)nbig, didx.p, pool.ws.p, pool.cap, pool.top.p, dtab.p, flags.p, drank.p,
-                           dstat.p, drp.p, dcp.p, h.kp, tol, 1, s);
+                           dstat.p, drp.p, dcp.p, h.kp, tol, dhist.p, 1, s);
             if (batch.size() > nbig)
                 launch_aca(djobs.p + nbig, (int)(batch.size() - nbig), didx.p, pool.ws.p, pool.cap, pool.top.p,
-                           dtab.p, flags.p, drank.p + nbig, dstat.p + nbig, drp.p, dcp.p, h.kp, tol, 0, s);
+                           dtab.p, flags.p, drank.p + nbig, dstat.p + nbig, drp.p, dcp.p, h.kp, tol, dhist.p, 0, s);
             H2W_HIP(hipGetLastError());
             std::vector<int32_t> rk(batch.size()), st(batch.size()), rp(po), cp(po);
+            std::vector<double> hs(2 * po);
+            H2W_HIP(hipMemcpyAsync(hs.data(), dhist.p, 2 * po * 8, hipMemcpyDeviceToHost, s));
             H2W_HIP(hipMemcpyAsync(rk.data(), drank.p, rk.size() * 4, hipMemcpyDeviceToHost, s));
             H2W_HIP(hipMemcpyAsync(st.data(), dstat.p, st.size() * 4, hipMemcpyDeviceToHost, s));
             H2W_HIP(hipMemcpyAsync(rp.data(), drp.p, po * 4, hipMemcpyDeviceToHost, s));
@@ -392,6 +415,7 @@ void run_aca(Handle &h, std::vector<AcaBlock> &blocks, int rank_hint, AcaPool &p
                 B.rank = rk[q];
                 B.rb.assign(rp.begin() + jobs[q].piv, rp.begin() + jobs[q].piv + rk[q]);
                 B.cb.assign(cp.begin() + jobs[q].piv, cp.begin() + jobs[q].piv + rk[q]);
+                B.hist.assign(hs.begin() + 2 * jobs[q].piv, hs.begin() + 2 * (jobs[q].piv + rk[q]));
                 if (rk[q] > 0) {
                     LrJob J;
                     J.tab = jobs[q].tab;
@@ -461,9 +485,11 @@ void far_nca(Handle &h, AcaPool &pool) {
         fl.Ir.resize(fl.nb);
         fl.Ic.resize(fl.nb);
         fl.lr.resize(fl.nb);
+        fl.hist.resize(fl.nb);
         for (int b = 0; b < fl.nb; ++b) {
             AcaBlock &B = blocks[b];
             fl.rank[b] = B.rank;
+            fl.hist[b] = std::move(B.hist);
             for (int q = 0; q < B.rank; ++q) {
                 fl.Ir[b].push_back(B.rows[B.rb[q]]);
                 fl.Ic[b].push_back(B.cols[B.cb[q]]);
@@ -507,9 +533,11 @@ void vertex_nca(Handle &h, AcaPool &pool) {
         fl.Ir.resize(fl.nb);
         fl.Ic.resize(fl.nb);
         fl.lr.resize(fl.nb);
+        fl.hist.resize(fl.nb);
         for (int b = 0; b < fl.nb; ++b) {
             AcaBlock &B = blocks[b];
             fl.rank[b] = B.rank;
+            fl.hist[b] = std::move(B.hist);
             for (int q = 0; q < B.rank; ++q) {
                 fl.Ir[b].push_back(B.rows[B.rb[q]]);
                 fl.Ic[b].push_back(B.cols[B.cb[q]]);
@@ -568,8 +596,10 @@ void cur_pairs(Handle &h, AcaPool &pool) {
 // ------------------------------------------------------------------ work lists
 constexpr int64_t kItemWork = 64 * 4096;  // elements of A per GEMV item before column chunking
 
-void add_item(WorkList &W, int64_t y, int rows, const std::vector<TermSpec> &terms, bool atomic) {
+void add_item(WorkList &W, int64_t y, int rows, const std::vector<TermSpec> &terms, bool atomic, int lda = -1,
+              int row_base = 0) {
     if (rows <= 0 || terms.empty()) return;
+    if (lda < 0) lda = rows;
     int64_t C = 0;
     for (auto &t : terms) C += t.cols;
     if (C == 0) return;
@@ -579,8 +609,8 @@ void add_item(WorkList &W, int64_t y, int rows, const std::vector<TermSpec> &ter
             GItem it;
             it.y = y + r0;
             it.rows = tr;
-            it.lda = rows;
-            it.row0 = r0;
+            it.lda = lda;
+            it.row0 = row_base + r0;
             it.t0 = (int32_t)W.terms.size();
             for (auto &t : terms)
                 if (t.cols > 0) W.terms.push_back(GTerm{t.a, t.x, t.cols, 0});
@@ -592,8 +622,8 @@ void add_item(WorkList &W, int64_t y, int rows, const std::vector<TermSpec> &ter
             const int64_t maxc = std::max<int64_t>(1, kItemWork / 64);
             GItem it;
             it.rows = tr;
-            it.lda = rows;
-            it.row0 = r0;
+            it.lda = lda;
+            it.row0 = row_base + r0;
             it.y = y + r0;
             it.flags = 1;
             int64_t used = 0;
@@ -601,7 +631,7 @@ void add_item(WorkList &W, int64_t y, int rows, const std::vector<TermSpec> &ter
             for (auto &t : terms) {
                 for (int64_t c0 = 0; c0 < t.cols;) {
                     int64_t take = std::min<int64_t>(t.cols - c0, maxc - used);
-                    W.terms.push_back(GTerm{t.a + c0 * rows, t.x + c0, (int32_t)take, 0});
+                    W.terms.push_back(GTerm{t.a + c0 * lda, t.x + c0, (int32_t)take, 0});
                     used += take;
                     c0 += take;
                     if (used == maxc) {
@@ -616,6 +646,42 @@ void add_item(WorkList &W, int64_t y, int rows, const std::vector<TermSpec> &ter
             if (it.t1 > it.t0) W.items.push_back(it);
         }
     }
+}
+
+// rank at which the ACA would have stopped for tolerance tau: the first accepted
+// step k >= 1 with |u_k||v_k| <= tau * normS_k ends the cross (2D/codes/aca.hpp:363)
+int inner_rank(const std::vector<double> &hist, int rank, double tau) {
+    for (int k = 1; k < rank && 2 * k + 1 < (int)hist.size(); ++k)
+        if (hist[2 * k] <= tau * hist[2 * k + 1]) return k + 1;
+    return rank;
+}
+
+void upload_lists(std::vector<WorkList> &WL, LaunchSet &LS, hipStream_t s) {
+    std::vector<GItem> items;
+    std::vector<GTerm> terms;
+    LS.launches.clear();
+    LS.elems.clear();
+    for (auto &W : WL) {
+        if (W.items.empty()) continue;
+        Launch Lh;
+        Lh.item0 = (int64_t)items.size();
+        Lh.nitems = (int64_t)W.items.size();
+        Lh.term0 = (int64_t)terms.size();
+        Lh.nterms = (int64_t)W.terms.size();
+        Lh.role = W.role;
+        int64_t e = 0;
+        for (auto it : W.items) {
+            for (int q = it.t0; q < it.t1; ++q) e += (int64_t)it.rows * W.terms[q].cols;
+            it.t0 += (int32_t)Lh.term0;
+            it.t1 += (int32_t)Lh.term0;
+            items.push_back(it);
+        }
+        terms.insert(terms.end(), W.terms.begin(), W.terms.end());
+        LS.launches.push_back(Lh);
+        LS.elems.push_back(e);
+    }
+    LS.d_items.upload(items, s);
+    LS.d_terms.upload(terms, s);
 }
 
 // ------------------------------------------------------------------ assembly
@@ -752,6 +818,58 @@ void assemble(Handle &h) {
     H2W_HIP(hipMemsetAsync(h.mat64.p, 0, h.mat_len * 8, s));
     h.vec64.alloc(h.vec_len);
     h.vec32.alloc(h.vec_len);
+    {
+        // the point of every vector slot, for on-the-fly near-field / M2L evaluation
+        std::vector<int32_t> pid(h.vec_len, -1);
+        for (int64_t t = 0; t < P.N; ++t) pid[h.off_x + t] = pid[h.off_y + t] = (int32_t)t;
+        for (int f = 0; f < 2; ++f) {
+            if (!fam_on[f]) continue;
+            for (int j = nfam_lo[f]; j <= P.nL; ++j) {
+                FamLevel &fl = h.fam[f][j];
+                for (int b = 0; b < fl.nb; ++b)
+                    for (int k = 0; k < fl.rank[b]; ++k) pid[fl.o_off[b] + k] = pid[fl.u_off[b] + k] = fl.Ir[b][k];
+            }
+        }
+        std::vector<float> fx(h.vec_len, 0.f), fy(h.vec_len, 0.f), fz(h.vec_len, 0.f);
+        std::vector<float> lx(h.vec_len, 0.f), ly(h.vec_len, 0.f), lz(h.vec_len, 0.f);
+        std::vector<double> dx(h.vec_len, 0.0), dy(h.vec_len, 0.0), dz(h.vec_len, 0.0);
+        for (int64_t v = 0; v < h.vec_len; ++v) {
+            if (pid[v] < 0) continue;
+            dx[v] = P.px[pid[v]];
+            dy[v] = P.py[pid[v]];
+            dz[v] = P.pz[pid[v]];
+            fx[v] = (float)dx[v];
+            fy[v] = (float)dy[v];
+            fz[v] = (float)dz[v];
+            lx[v] = (float)(dx[v] - (double)fx[v]);
+            ly[v] = (float)(dy[v] - (double)fy[v]);
+            lz[v] = (float)(dz[v] - (double)fz[v]);
+        }
+        h.spid.upload(pid, s);
+        h.sx32.upload(fx, s);
+        h.sy32.upload(fy, s);
+        h.sz32.upload(fz, s);
+        h.sxl.upload(lx, s);
+        h.syl.upload(ly, s);
+        h.szl.upload(lz, s);
+        h.sx64.upload(dx, s);
+        h.sy64.upload(dy, s);
+        h.sz64.upload(dz, s);
+        H2W_HIP(hipStreamSynchronize(s));
+        h.sp.x = h.sx32.p;
+        h.sp.y = h.sy32.p;
+        h.sp.z = h.sz32.p;
+        h.sp.xl = h.sxl.p;
+        h.sp.yl = h.syl.p;
+        h.sp.zl = h.szl.p;
+        h.sp.xd = h.sx64.p;
+        h.sp.yd = h.sy64.p;
+        h.sp.zd = h.sz64.p;
+        h.sp.pid = h.spid.p;
+        h.sp.dim = P.dim;
+        h.sp.choice = h.choice;
+        h.sp.diag = h.diag;
+    }
 
     // ---- evaluate operators on the device
     {
@@ -856,12 +974,96 @@ void assemble(Handle &h) {
         flush();
     }
     phase("M2L / near / CUR blocks");
+    // ---- inner operator of the refinement: each box keeps the leading pivots at which its
+    // cross would have stopped for tolerance 10^-ir_tol_pow; its M2L blocks are the leading
+    // sub-blocks of the stored ones, its transfers are recomputed from the leading L, R blocks
+    const bool inner_on = h.opts.fp32_shadow && h.opts.ir_tol_pow > 0 && h.opts.ir_tol_pow < h.tol_pow;
+    int64_t in_len = 0;
+    if (inner_on) {
+        const double tau = std::pow(10.0, -1.0 * h.opts.ir_tol_pow);
+        for (int f = 0; f < 2; ++f) {
+            if (!fam_on[f]) continue;
+            for (int j = nfam_lo[f]; j <= P.nL; ++j) {
+                FamLevel &fl = h.fam[f][j];
+                fl.kin.assign(fl.nb, 0);
+                for (int b = 0; b < fl.nb; ++b)
+                    fl.kin[b] = fl.rank[b] > 0 ? inner_rank(fl.hist[b], fl.rank[b], tau) : 0;
+            }
+            for (int j = nfam_lo[f]; j <= P.nL; ++j) {
+                FamLevel &fl = h.fam[f][j];
+                fl.ein_off.assign(fl.nb, -1);
+                fl.eint_off.assign(fl.nb, -1);
+                for (int b = 0; b < fl.nb; ++b) {
+                    int64_t m = 0;
+                    if (j == P.nL) m = P.bcount[j][b];
+                    else
+                        for (int c = 0; c < P.nchild; ++c) m += h.fam[f][j + 1].kin[b * P.nchild + c];
+                    int64_t e = m * fl.kin[b];
+                    if (e == 0) continue;
+                    fl.ein_off[b] = in_len;
+                    in_len += align_up(e, 32);
+                    fl.eint_off[b] = in_len;
+                    in_len += align_up(e, 32);
+                }
+            }
+        }
+    }
     if (h.opts.fp32_shadow) {
-        h.mat32.alloc(h.mat_len);
+        h.mat32.alloc(h.mat_len + in_len);
         launch_to_fp32(h.mat64.p, h.mat32.p, h.mat_len, s);
         H2W_HIP(hipGetLastError());
     } else {
         h.mat32.release();
+    }
+    if (inner_on && in_len > 0) {
+        std::vector<int32_t> idx;
+        std::vector<XferJob> xj;
+        std::vector<double> lr;
+        for (int f = 0; f < 2; ++f) {
+            if (!fam_on[f]) continue;
+            for (int j = nfam_lo[f]; j <= P.nL; ++j) {
+                FamLevel &fl = h.fam[f][j];
+                for (int b = 0; b < fl.nb; ++b) {
+                    if (fl.ein_off[b] < 0) continue;
+                    const int k = fl.kin[b], r = fl.rank[b];
+                    XferJob J;
+                    J.r = k;
+                    J.s = (int64_t)idx.size();
+                    if (j == P.nL) {
+                        auto rl = range_list(P.bstart[j][b], P.bcount[j][b]);
+                        idx.insert(idx.end(), rl.begin(), rl.end());
+                    } else {
+                        for (int c = 0; c < P.nchild; ++c) {
+                            const int cc = b * P.nchild + c;
+                            auto &ir = h.fam[f][j + 1].Ir[cc];
+                            idx.insert(idx.end(), ir.begin(), ir.begin() + h.fam[f][j + 1].kin[cc]);
+                        }
+                    }
+                    J.m = (int32_t)((int64_t)idx.size() - J.s);
+                    J.ic = (int64_t)idx.size();
+                    idx.insert(idx.end(), fl.Ic[b].begin(), fl.Ic[b].begin() + k);
+                    J.lr = (int64_t)lr.size();
+                    for (int part = 0; part < 2; ++part)  // leading k x k blocks of L and R
+                        for (int cc = 0; cc < k; ++cc)
+                            for (int rr = 0; rr < k; ++rr) lr.push_back(fl.lr[b][(size_t)part * r * r + (size_t)cc * r + rr]);
+                    J.e = fl.ein_off[b];
+                    J.et = fl.eint_off[b];
+                    xj.push_back(J);
+                }
+            }
+        }
+        DevBuf<int32_t> didx;
+        DevBuf<double> dlr, tmp;
+        DevBuf<XferJob> dxj;
+        didx.upload(idx, s);
+        dlr.upload(lr, s);
+        dxj.upload(xj, s);
+        tmp.alloc(in_len);
+        launch_xfer(dxj.p, (int)xj.size(), didx.p, dlr.p, tmp.p, h.kp, s);
+        H2W_HIP(hipGetLastError());
+        launch_to_fp32(tmp.p, h.mat32.p + h.mat_len, in_len, s);
+        H2W_HIP(hipGetLastError());
+        H2W_HIP(hipStreamSynchronize(s));
     }
 
     // ---- GEMV work lists, one per launch of the matvec
@@ -988,6 +1190,124 @@ void assemble(Handle &h) {
         se += h.launch_elems[q];
     }
     h.stored_elems = se;
+    if (inner_on) {
+        const int64_t base = h.mat_len;  // inner transfers follow the main operators in the FP32 pool
+        std::vector<WorkList> IL;
+        {
+            WorkList W;
+            for (int f = 0; f < 2; ++f) {
+                if (!fam_on[f] || P.nL < nfam_lo[f]) continue;
+                FamLevel &fl = h.fam[f][P.nL];
+                for (int b = 0; b < fl.nb; ++b)
+                    if (fl.eint_off[b] >= 0)
+                        add_item(W, fl.o_off[b], fl.kin[b],
+                                 {TermSpec{base + fl.eint_off[b], h.off_x + P.bstart[P.nL][b], (int32_t)P.bcount[P.nL][b]}},
+                                 false);
+            }
+            for (auto &p : h.pairs)
+                if (p.rank > 0)
+                    add_item(W, p.t_off, p.rank,
+                             {TermSpec{p.ar_off, h.off_x + P.bstart[p.level][p.v], (int32_t)P.bcount[p.level][p.v]}},
+                             false);
+            IL.push_back(std::move(W));
+        }
+        for (int j = P.nL - 1; j >= 1; --j) {
+            WorkList W;
+            for (int f = 0; f < 2; ++f) {
+                if (!fam_on[f] || j < nfam_lo[f]) continue;
+                FamLevel &fl = h.fam[f][j];
+                FamLevel &ch = h.fam[f][j + 1];
+                for (int b = 0; b < fl.nb; ++b) {
+                    if (fl.eint_off[b] < 0) continue;
+                    std::vector<TermSpec> t;
+                    int64_t col = 0;
+                    for (int c = 0; c < P.nchild; ++c) {
+                        const int cc = b * P.nchild + c;
+                        if (ch.kin[cc] > 0)
+                            t.push_back(TermSpec{base + fl.eint_off[b] + col * fl.kin[b], ch.o_off[cc], (int32_t)ch.kin[cc]});
+                        col += ch.kin[cc];
+                    }
+                    add_item(W, fl.o_off[b], fl.kin[b], t, false);
+                }
+            }
+            IL.push_back(std::move(W));
+        }
+        {
+            WorkList W;
+            W.role = 1;
+            for (int f = 0; f < 2; ++f) {
+                if (!fam_on[f]) continue;
+                for (int j = nfam_lo[f]; j <= P.nL; ++j) {
+                    FamLevel &fl = h.fam[f][j];
+                    std::vector<std::vector<TermSpec>> per(fl.nb);
+                    for (auto &m : m2l)
+                        if (m.f == f && m.j == j && fl.kin[m.p] > 0)
+                            per[m.b].push_back(TermSpec{m.off, fl.o_off[m.p], (int32_t)fl.kin[m.p]});
+                    for (int b = 0; b < fl.nb; ++b)
+                        if (fl.kin[b] > 0) add_item(W, fl.u_off[b], fl.kin[b], per[b], false, fl.rank[b]);
+                }
+            }
+            std::vector<std::vector<TermSpec>> per(nleaf);
+            for (auto &nb : nearb)
+                per[nb.b].push_back(TermSpec{nb.off, h.off_x + P.bstart[P.nL][nb.n], (int32_t)P.bcount[P.nL][nb.n]});
+            for (int b = 0; b < nleaf; ++b)
+                add_item(W, h.off_y + P.bstart[P.nL][b], (int)P.bcount[P.nL][b], per[b], false);
+            IL.push_back(std::move(W));
+        }
+        for (int j = 1; j <= P.nL - 1; ++j) {
+            WorkList W;
+            for (int f = 0; f < 2; ++f) {
+                if (!fam_on[f] || j < nfam_lo[f]) continue;
+                FamLevel &fl = h.fam[f][j];
+                FamLevel &ch = h.fam[f][j + 1];
+                for (int b = 0; b < fl.nb; ++b) {
+                    if (fl.ein_off[b] < 0) continue;
+                    int m = 0;
+                    for (int c = 0; c < P.nchild; ++c) m += ch.kin[b * P.nchild + c];
+                    int row = 0;
+                    for (int c = 0; c < P.nchild; ++c) {
+                        const int cc = b * P.nchild + c;
+                        if (ch.kin[cc] > 0)
+                            add_item(W, ch.u_off[cc], ch.kin[cc],
+                                     {TermSpec{base + fl.ein_off[b], fl.u_off[b], (int32_t)fl.kin[b]}}, false, m, row);
+                        row += ch.kin[cc];
+                    }
+                }
+            }
+            IL.push_back(std::move(W));
+        }
+        {
+            WorkList W;
+            for (int b = 0; b < nleaf; ++b) {
+                std::vector<TermSpec> t;
+                for (int f = 0; f < 2; ++f) {
+                    if (!fam_on[f] || P.nL < nfam_lo[f]) continue;
+                    FamLevel &fl = h.fam[f][P.nL];
+                    if (fl.ein_off[b] >= 0) t.push_back(TermSpec{base + fl.ein_off[b], fl.u_off[b], (int32_t)fl.kin[b]});
+                }
+                add_item(W, h.off_y + P.bstart[P.nL][b], (int)P.bcount[P.nL][b], t, false);
+            }
+            IL.push_back(std::move(W));
+        }
+        if (!h.pairs.empty()) {
+            WorkList W;
+            W.role = 2;
+            std::vector<std::vector<std::vector<TermSpec>>> per(P.nL + 1);
+            for (int j = 1; j <= P.nL; ++j) per[j].assign(P.nboxes[j], {});
+            for (auto &p : h.pairs)
+                if (p.rank > 0) per[p.level][p.b].push_back(TermSpec{p.ac_off, p.t_off, (int32_t)p.rank});
+            for (int j = 1; j <= P.nL; ++j)
+                for (int b = 0; b < P.nboxes[j]; ++b)
+                    add_item(W, h.off_y + P.bstart[j][b], (int)P.bcount[j][b], per[j][b], true);
+            IL.push_back(std::move(W));
+        }
+        upload_lists(IL, h.inner, s);
+        h.inner_elems = 0;
+        for (auto e : h.inner.elems) h.inner_elems += e;
+    } else {
+        h.inner.launches.clear();
+        h.inner_elems = 0;
+    }
     H2W_HIP(hipStreamSynchronize(s));
     phase("work lists");
     if (verbose)
@@ -999,17 +1319,42 @@ void assemble(Handle &h) {
 
 // ------------------------------------------------------------------ matvec
 // y_tree = A x_tree inside the vector pool (x already at off_x)
-void run_launches(Handle &h, int precision) {
+bool is32(int precision) { return precision == H2W_FP32 || precision == H2W_FP32_OTF || precision == H2W_FP32_INNER; }
+
+void run_launch(Handle &h, const Launch &Lh, int precision) {
     hipStream_t s = h.stream;
-    if (precision == H2W_FP32) {
-        launch_fill32(h.vec32.p + h.off_y, h.vec_len - h.off_y, 0.0f, s);
-        for (auto &Lh : h.launches)
-            launch_bgemv32(h.d_items.p + Lh.item0, (int)Lh.nitems, h.d_terms.p, h.mat32.p, h.vec32.p, Lh.role, s);
-    } else {
-        launch_fill(h.vec64.p + h.off_y, h.vec_len - h.off_y, 0.0, s);
-        for (auto &Lh : h.launches)
-            launch_bgemv64(h.d_items.p + Lh.item0, (int)Lh.nitems, h.d_terms.p, h.mat64.p, h.vec64.p, Lh.role, s);
+    const bool otf = (precision == H2W_FP32_OTF || precision == H2W_FP64_OTF) && Lh.role == 1;
+    if (precision == H2W_FP32_INNER) {
+        launch_bgemv32(h.inner.d_items.p + Lh.item0, (int)Lh.nitems, h.inner.d_terms.p, h.mat32.p, h.vec32.p, Lh.role,
+                       s);
+        return;
     }
+    if (is32(precision)) {
+        if (otf) launch_otf32(h.d_items.p + Lh.item0, (int)Lh.nitems, h.d_terms.p, h.sp, h.vec32.p, s);
+        else launch_bgemv32(h.d_items.p + Lh.item0, (int)Lh.nitems, h.d_terms.p, h.mat32.p, h.vec32.p, Lh.role, s);
+    } else {
+        if (otf) launch_otf64(h.d_items.p + Lh.item0, (int)Lh.nitems, h.d_terms.p, h.sp, h.vec64.p, s);
+        else launch_bgemv64(h.d_items.p + Lh.item0, (int)Lh.nitems, h.d_terms.p, h.mat64.p, h.vec64.p, Lh.role, s);
+    }
+}
+
+void zero_outputs(Handle &h, int precision) {
+    if (is32(precision)) launch_fill32(h.vec32.p + h.off_y, h.vec_len - h.off_y, 0.0f, h.stream);
+    else launch_fill(h.vec64.p + h.off_y, h.vec_len - h.off_y, 0.0, h.stream);
+}
+
+const std::vector<Launch> &launches_for(Handle &h, int precision) {
+    if (precision == H2W_FP32_INNER) {
+        if (h.inner.launches.empty()) throw Error{H2W_ESTATE, "the inner operator was not built (opts.ir_tol_pow)"};
+        return h.inner.launches;
+    }
+    return h.launches;
+}
+
+void run_launches(Handle &h, int precision) {
+    const auto &L = launches_for(h, precision);
+    zero_outputs(h, precision);
+    for (auto &Lh : L) run_launch(h, Lh, precision);
     H2W_HIP(hipGetLastError());
 }
 
@@ -1017,7 +1362,7 @@ void run_launches(Handle &h, int precision) {
 void matvec_dev(Handle &h, const double *x, double *y, int precision, bool tree) {
     hipStream_t s = h.stream;
     const int64_t N = h.N;
-    if (precision == H2W_FP32) {
+    if (is32(precision)) {
         if (!h.mat32.p) throw Error{H2W_ESTATE, "FP32 matvec requested but the FP32 shadow was not built"};
         if (tree) launch_to_fp32(x, h.vec32.p + h.off_x, N, s);
         else launch_permute_in32(x, N, h.vec32.p + h.off_x, N, h.d_perm.p, N, 1, s);
@@ -1211,6 +1556,7 @@ void h2w_default_opts(h2w_opts *o) {
     o->rank_cap = 256;
     o->workspace_bytes = 0;
     o->plan_only = 0;
+    o->ir_tol_pow = 6;
 }
 
 const char *h2w_last_error(void) { return h2w::g_err.c_str(); }
@@ -1321,7 +1667,7 @@ int h2w_matvec_ex(h2w_handle *h, const double *x, double *y, int nrhs, int64_t l
     if (!h || !x || !y) throw Error{H2W_EINVAL, "NULL argument"};
     if (!h->assembled) throw Error{H2W_ESTATE, "h2w_matvec before h2w_assemble"};
     if (nrhs < 1 || ld < h->N) throw Error{H2W_EINVAL, "bad nrhs / ld"};
-    if (precision != H2W_FP64 && precision != H2W_FP32) throw Error{H2W_EINVAL, "bad precision"};
+    if (precision < H2W_FP64 || precision > H2W_FP32_INNER) throw Error{H2W_EINVAL, "bad precision"};
     H2W_HIP(hipSetDevice(h->opts.device));
     const int64_t N = h->N;
     for (int r = 0; r < nrhs; ++r) {
@@ -1352,8 +1698,13 @@ int h2w_gmres(h2w_handle *h, const double *b, double *x, int max_iter, double to
     if (!h || !b || !x) throw Error{H2W_EINVAL, "NULL argument"};
     if (!h->assembled) throw Error{H2W_ESTATE, "h2w_gmres before h2w_assemble"};
     if (max_iter < 1) throw Error{H2W_EINVAL, "max_iter must be positive"};
-    if (mode != H2W_GMRES_FP64 && mode != H2W_GMRES_FP32_IR) throw Error{H2W_EINVAL, "bad mode"};
-    if (mode == H2W_GMRES_FP32_IR && !h->mat32.p) throw Error{H2W_ESTATE, "FP32 shadow not built"};
+    if (mode < H2W_GMRES_FP64 || mode > H2W_GMRES_FP32_INNER_IR) throw Error{H2W_EINVAL, "bad mode"};
+    if (mode != H2W_GMRES_FP64 && !h->mat32.p) throw Error{H2W_ESTATE, "FP32 shadow not built"};
+    if (mode == H2W_GMRES_FP32_INNER_IR && h->inner.launches.empty())
+        throw Error{H2W_ESTATE, "the inner operator was not built (opts.ir_tol_pow)"};
+    const int inner_prec = mode == H2W_GMRES_FP32_OTF_IR ? H2W_FP32_OTF
+                           : mode == H2W_GMRES_FP32_INNER_IR ? H2W_FP32_INNER
+                                                              : H2W_FP32;
     H2W_HIP(hipSetDevice(h->opts.device));
     const int64_t N = h->N;
     hipStream_t s = h->stream;
@@ -1398,9 +1749,10 @@ int h2w_gmres(h2w_handle *h, const double *b, double *x, int max_iter, double to
             double nr = h2w::dev_norm(*h, h->g_r.p);
             rn = nr / normb;
             if (rn <= tol || total >= max_iter || outer >= 50) break;
-            double eta = std::min(1e-2, std::max(1e-6, 0.1 * tol / rn));
+            const double floor_eta = mode == H2W_GMRES_FP32_INNER_IR ? std::pow(10.0, -1.0 * h->opts.ir_tol_pow) : 1e-6;
+            double eta = std::min(1e-2, std::max(floor_eta, 0.1 * tol / rn));
             h2w::launch_fill(h->g_d.p, N, 0.0, s);
-            auto o = h2w::gmres_core(*h, h->g_r.p, h->g_d.p, max_iter - total, eta, H2W_FP32, true, nr);
+            auto o = h2w::gmres_core(*h, h->g_r.p, h->g_d.p, max_iter - total, eta, inner_prec, true, nr);
             total += o.iters;
             S.matvecs_fp32 += o.matvecs;
             h2w::launch_axpby(1.0, h->g_d.p, 1.0, h->g_x.p, N, s);
@@ -1505,6 +1857,7 @@ int h2w_get_stats(const h2w_handle *h, h2w_stats *st) {
     st->stored_bytes_fp64 = h->stored_elems * 8;
     st->stored_bytes_fp32 = h->mat32.p ? h->stored_elems * 4 : 0;
     st->n_launches_matvec = (int)h->launches.size() + 3;
+    st->stored_bytes_inner_fp32 = h->inner_elems * 4;
     return H2W_OK;
 }
 
@@ -1567,7 +1920,9 @@ int h2w_bench_matvec(h2w_handle *h, int nrep, int precision, h2w_bench *out) {
     H2W_TRY
     if (!h || !out || nrep < 1) throw Error{H2W_EINVAL, "bad argument"};
     if (!h->assembled) throw Error{H2W_ESTATE, "not assembled"};
-    if (precision == H2W_FP32 && !h->mat32.p) throw Error{H2W_ESTATE, "FP32 shadow not built"};
+    if (precision < H2W_FP64 || precision > H2W_FP32_INNER) throw Error{H2W_EINVAL, "bad precision"};
+    if (h2w::is32(precision) && !h->mat32.p) throw Error{H2W_ESTATE, "FP32 shadow not built"};
+    const auto &LL = h2w::launches_for(*h, precision);
     H2W_HIP(hipSetDevice(h->opts.device));
     std::memset(out, 0, sizeof(*out));
     hipStream_t s = h->stream;
@@ -1575,7 +1930,7 @@ int h2w_bench_matvec(h2w_handle *h, int nrep, int precision, h2w_bench *out) {
     // deterministic input in the pool's x segment
     h2w::launch_fill(h->vec64.p + h->off_x, N, 1.0, s);
     h2w::launch_fill32(h->vec32.p + h->off_x, N, 1.0f, s);
-    const int nl = std::min<int>((int)h->launches.size(), H2W_MAX_LAUNCHES - 1);
+    const int nl = std::min<int>((int)h2w::launches_for(*h, precision).size(), H2W_MAX_LAUNCHES - 1);
     std::vector<hipEvent_t> ev(nl + 2);
     for (auto &e : ev) H2W_HIP(hipEventCreate(&e));
     std::vector<double> acc(nl + 1, 0.0);
@@ -1595,15 +1950,10 @@ int h2w_bench_matvec(h2w_handle *h, int nrep, int precision, h2w_bench *out) {
     // per-launch durations: fill + each bgemv launch
     for (int r = 0; r < nrep; ++r) {
         H2W_HIP(hipEventRecord(ev[0], s));
-        if (precision == H2W_FP32) h2w::launch_fill32(h->vec32.p + h->off_y, h->vec_len - h->off_y, 0.0f, s);
-        else h2w::launch_fill(h->vec64.p + h->off_y, h->vec_len - h->off_y, 0.0, s);
+        h2w::zero_outputs(*h, precision);
         H2W_HIP(hipEventRecord(ev[1], s));
         for (int q = 0; q < nl; ++q) {
-            auto &Lh = h->launches[q];
-            if (precision == H2W_FP32)
-                h2w::launch_bgemv32(h->d_items.p + Lh.item0, (int)Lh.nitems, h->d_terms.p, h->mat32.p, h->vec32.p, Lh.role, s);
-            else
-                h2w::launch_bgemv64(h->d_items.p + Lh.item0, (int)Lh.nitems, h->d_terms.p, h->mat64.p, h->vec64.p, Lh.role, s);
+            h2w::run_launch(*h, LL[q], precision);
             H2W_HIP(hipEventRecord(ev[q + 2], s));
         }
         H2W_HIP(hipEventSynchronize(ev[nl + 1]));
@@ -1614,15 +1964,16 @@ int h2w_bench_matvec(h2w_handle *h, int nrep, int precision, h2w_bench *out) {
         }
     }
     out->n_launch = nl + 1;
-    const int es = precision == H2W_FP32 ? 4 : 8;
+    const int es = h2w::is32(precision) ? 4 : 8;
     out->launch_ms[0] = acc[0] / nrep;
     out->launch_bytes[0] = (h->vec_len - h->off_y) * (int64_t)es;
     out->launch_role[0] = -1;
     for (int q = 0; q < nl; ++q) {
         out->launch_ms[q + 1] = acc[q + 1] / nrep;
-        auto &Lh = h->launches[q];
+        auto &Lh = LL[q];
         out->launch_role[q + 1] = Lh.role;
-        out->launch_bytes[q + 1] = h->launch_elems[q] * es;
+        out->launch_bytes[q + 1] =
+            (precision == H2W_FP32_INNER ? h->inner.elems[q] : h->launch_elems[q]) * es;
     }
     for (auto &e : ev) (void)hipEventDestroy(e);
     (void)hipEventDestroy(t0);

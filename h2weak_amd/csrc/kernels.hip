@@ -183,9 +183,10 @@ __global__ __launch_bounds__(NT) void k_aca(const AcaJob *__restrict__ jobs, con
                                             int64_t *__restrict__ tab_all, uint8_t *__restrict__ flags,
                                             int32_t *__restrict__ rank_out, int32_t *__restrict__ status_out,
                                             int32_t *__restrict__ rowpiv, int32_t *__restrict__ colpiv, KParams kp,
-                                            double tol) {
+                                            double tol, double *__restrict__ hist_all) {
     const AcaJob jb = jobs[blockIdx.x];
     const int m = jb.m, n = jb.n, tcap = jb.tcap;
+    double *hist = hist_all + 2 * jb.piv;  // per accepted step: |u||v|, normS after the step
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     constexpr int NW = NT / 64;
     const int32_t *rows = idx + jb.rows;
@@ -274,6 +275,10 @@ __global__ __launch_bounds__(NT) void k_aca(const AcaJob *__restrict__ jobs, con
     double normS = 0.0;
     double un = sqrt(un2), vn = sqrt(vn2);
     int next_rp = an.idx;
+    if (tid == 0) {
+        hist[0] = un * vn;
+        hist[1] = 0.0;
+    }
     __syncthreads();
 
     while (un * vn > tol * normS && rank < mn) {
@@ -368,6 +373,10 @@ __global__ __launch_bounds__(NT) void k_aca(const AcaJob *__restrict__ jobs, con
         for (int q = 0; q < NW; ++q) S += dacc[q];
         double uv = un * vn;
         normS = sqrt(normS * normS + uv * uv + 2.0 * S);
+        if (tid == 0) {
+            hist[2 * (rank - 1)] = uv;
+            hist[2 * (rank - 1) + 1] = normS;
+        }
         __syncthreads();
     }
     if (tid == 0) {
@@ -512,6 +521,120 @@ __global__ __launch_bounds__(256) void k_bgemv(const GItem *__restrict__ items, 
     }
 }
 
+// ------------------------------------------------------------------ on-the-fly P2P / M2L
+// The near-field and M2L terms of a work list are blocks of plain kernel entries
+// K(target points, source points); this kernel evaluates them instead of reading
+// them.  Target row r of an item is vector slot it.y + r, source column c of a
+// term is slot tm.x + c; the slot tables give each slot's point.  VALU-bound:
+// per entry 3 sub + 3 mul/fma + sqrt + log (3D RBF) and the diagonal test.
+__device__ __forceinline__ float koff_f(int dim, int choice, float dx, float dy, float dz) {
+    const float a = 1.0e-4f;
+    if (dim == 1) {
+        float R = fabsf(dx);
+        if (choice == 0) return __logf(R);
+        return R < a ? R * 1.0e4f : a / R;
+    } else if (dim == 2) {
+        float R2 = dx * dx + dy * dy;
+        if (choice == 0) return -0.0795774715459476679f * __logf(R2);  // -(1/2pi) * 0.5 * log R^2
+        float R = sqrtf(R2);
+        return R < a ? R * 1.0e4f : a / R;
+    } else {
+        float R2 = dx * dx + dy * dy + dz * dz;
+        if (choice == 0) return 0.0795774715459476679f * rsqrtf(R2);  // 1 / (4 pi R)
+        float R = sqrtf(R2);
+        float lr = __logf(R);
+        // (R log R - 1) / (a log a - 1) below a, log R / log a above
+        return R < a ? (R * lr - 1.0f) * -0.9990798135f : lr * -0.1085736205f;
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ T koff_t(int dim, int choice, T dx, T dy, T dz);
+template <>
+__device__ __forceinline__ float koff_t<float>(int dim, int choice, float dx, float dy, float dz) {
+    return koff_f(dim, choice, dx, dy, dz);
+}
+template <>
+__device__ __forceinline__ double koff_t<double>(int dim, int choice, double dx, double dy, double dz) {
+    return koff(dim, choice, dx, dy, dz);
+}
+
+// Coordinate differences: FP64 directly; FP32 from (hi, lo) pairs, where the
+// hi difference of two nearby points is exact or nearly so and the lo
+// difference restores the digits plain FP32 coordinates would lose.
+template <typename T>
+struct Coords {
+    const T *x, *y, *z;
+    const float *xl, *yl, *zl;
+};
+template <typename T>
+__device__ __forceinline__ T cdiff(const T *h, const float *l, T th, float tl, int64_t s);
+template <>
+__device__ __forceinline__ float cdiff<float>(const float *h, const float *l, float th, float tl, int64_t s) {
+    return (th - h[s]) + (tl - l[s]);
+}
+template <>
+__device__ __forceinline__ double cdiff<double>(const double *h, const float *, double th, float, int64_t s) {
+    return th - h[s];
+}
+
+template <typename T>
+__device__ __forceinline__ T otf_entry(const Coords<T> &C, const int32_t *pid, int dim, int choice, T diag, T tx,
+                                       T ty, T tz, float txl, float tyl, float tzl, int tp, int64_t s) {
+    if (pid[s] == tp) return diag;
+    T dx = cdiff<T>(C.x, C.xl, tx, txl, s);
+    T dy = dim > 1 ? cdiff<T>(C.y, C.yl, ty, tyl, s) : T(0);
+    T dz = dim > 2 ? cdiff<T>(C.z, C.zl, tz, tzl, s) : T(0);
+    return koff_t<T>(dim, choice, dx, dy, dz);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_otf(const GItem *__restrict__ items, const GTerm *__restrict__ terms,
+                                             Coords<T> C, const int32_t *__restrict__ pid, int dim, int choice,
+                                             T diag, T *vec) {
+    const GItem it = items[blockIdx.x];
+    const int lane = threadIdx.x & 63;
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    __shared__ T part[4][64];
+    T acc0 = 0, acc1 = 0;
+    if (lane < it.rows) {
+        const int64_t ts = it.y + lane;
+        const T tx = C.x[ts], ty = dim > 1 ? C.y[ts] : T(0), tz = dim > 2 ? C.z[ts] : T(0);
+        float txl = 0.f, tyl = 0.f, tzl = 0.f;
+        if (C.xl) {
+            txl = C.xl[ts];
+            tyl = dim > 1 ? C.yl[ts] : 0.f;
+            tzl = dim > 2 ? C.zl[ts] : 0.f;
+        }
+        const int tp = pid[ts];
+        for (int tt = it.t0; tt < it.t1; ++tt) {
+            const GTerm tm = terms[tt];
+            const int64_t x0 = tm.x;
+            int c = wv;
+            for (; c + 4 < tm.cols; c += 8) {
+                const int64_t s0 = x0 + c, s1 = s0 + 4;
+                T e0 = otf_entry<T>(C, pid, dim, choice, diag, tx, ty, tz, txl, tyl, tzl, tp, s0);
+                T e1 = otf_entry<T>(C, pid, dim, choice, diag, tx, ty, tz, txl, tyl, tzl, tp, s1);
+                acc0 = fma(e0, vec[s0], acc0);
+                acc1 = fma(e1, vec[s1], acc1);
+            }
+            for (; c < tm.cols; c += 4) {
+                const int64_t s0 = x0 + c;
+                T e0 = otf_entry<T>(C, pid, dim, choice, diag, tx, ty, tz, txl, tyl, tzl, tp, s0);
+                acc0 = fma(e0, vec[s0], acc0);
+            }
+        }
+    }
+    part[threadIdx.x >> 6][lane] = acc0 + acc1;
+    __syncthreads();
+    if (threadIdx.x < 64 && lane < it.rows) {
+        T s = part[0][lane] + part[1][lane] + part[2][lane] + part[3][lane];
+        T *y = vec + it.y + lane;
+        if (it.flags & 1) atomicAdd(y, s);
+        else *y += s;
+    }
+}
+
 // ------------------------------------------------------------------ small vector kernels
 __global__ void k_to_fp32(const double *a, float *b, int64_t n) {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
@@ -639,12 +762,12 @@ inline int grid_for(int64_t n, int nt = 256, int cap = 8192) {
 // ------------------------------------------------------------------ launch wrappers
 void launch_aca(const AcaJob *jobs, int njobs, const int32_t *idx, double *ws, int64_t ws_cap,
                 unsigned long long *ws_top, int64_t *tab, uint8_t *flags, int32_t *rank, int32_t *status,
-                int32_t *rowpiv, int32_t *colpiv, KParams kp, double tol, int big, hipStream_t s) {
+                int32_t *rowpiv, int32_t *colpiv, KParams kp, double tol, double *hist, int big, hipStream_t s) {
     if (njobs <= 0) return;
     if (big) hipLaunchKernelGGL(k_aca<1024>, dim3(njobs), dim3(1024), 0, s, jobs, idx, ws, ws_cap, ws_top, tab, flags,
-                                rank, status, rowpiv, colpiv, kp, tol);
+                                rank, status, rowpiv, colpiv, kp, tol, hist);
     else hipLaunchKernelGGL(k_aca<256>, dim3(njobs), dim3(256), 0, s, jobs, idx, ws, ws_cap, ws_top, tab, flags, rank,
-                            status, rowpiv, colpiv, kp, tol);
+                            status, rowpiv, colpiv, kp, tol, hist);
 }
 void launch_lr_extract(const LrJob *jobs, int njobs, const double *ws, const int64_t *tab, const int32_t *rowpiv,
                        const int32_t *colpiv, double *out, hipStream_t s) {
@@ -677,6 +800,18 @@ void launch_bgemv64(const GItem *items, int nitems, const GTerm *terms, const do
 void launch_bgemv32(const GItem *items, int nitems, const GTerm *terms, const float *A, float *vec, int role,
                     hipStream_t s) {
     launch_bgemv_t<float>(items, nitems, terms, A, vec, role, s);
+}
+void launch_otf32(const GItem *items, int nitems, const GTerm *terms, SlotPoints sp, float *vec, hipStream_t s) {
+    if (nitems > 0)
+        hipLaunchKernelGGL(k_otf<float>, dim3(nitems), dim3(256), 0, s, items, terms,
+                           Coords<float>{sp.x, sp.y, sp.z, sp.xl, sp.yl, sp.zl}, sp.pid, sp.dim, sp.choice,
+                           (float)sp.diag, vec);
+}
+void launch_otf64(const GItem *items, int nitems, const GTerm *terms, SlotPoints sp, double *vec, hipStream_t s) {
+    if (nitems > 0)
+        hipLaunchKernelGGL(k_otf<double>, dim3(nitems), dim3(256), 0, s, items, terms,
+                           Coords<double>{sp.xd, sp.yd, sp.zd, nullptr, nullptr, nullptr}, sp.pid, sp.dim, sp.choice,
+                           sp.diag, vec);
 }
 void launch_to_fp32(const double *a, float *b, int64_t n, hipStream_t s) {
     if (n > 0) hipLaunchKernelGGL(k_to_fp32, dim3(grid_for(n)), dim3(256), 0, s, a, b, n);

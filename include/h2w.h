@@ -47,8 +47,12 @@ extern "C" {
 #define H2W_HOST 0
 #define H2W_DEVICE 1
 
-#define H2W_FP64 0
-#define H2W_FP32 1
+#define H2W_FP64 0      /* stored FP64 operators                                                  */
+#define H2W_FP32 1      /* stored FP32 copies of every operator, FP32 accumulation                */
+#define H2W_FP32_OTF 2  /* FP32: near-field and M2L entries evaluated on the fly, transfers stored */
+#define H2W_FP64_OTF 3  /* FP64: near-field and M2L entries evaluated on the fly, transfers stored */
+#define H2W_FP32_INNER 4 /* FP32 inner operator of the refinement: every box keeps the leading
+                            pivots at which its cross would have stopped at 10^-ir_tol_pow       */
 
 #define H2W_ORDER_INPUT 0
 #define H2W_ORDER_TREE 1
@@ -70,6 +74,8 @@ typedef struct {
     int64_t workspace_bytes; /* ACA workspace budget in bytes (0 = automatic)                    */
     int plan_only;           /* 1: build the tree and lists only, touch no device (assembly and
                                 products then return H2W_ESTATE); for host-side inspection        */
+    int ir_tol_pow;          /* tolerance exponent of the refinement's truncated inner operator
+                                (default 6; 0 or >= tol_pow: not built)                            */
 } h2w_opts;
 
 typedef struct {
@@ -84,7 +90,8 @@ typedef struct {
     int64_t stored_bytes_fp64;   /* bytes one FP64 matvec streams from the operator pools     */
     int64_t stored_bytes_fp32;   /* same for the FP32 shadow (0 if not built)                 */
     int n_launches_matvec;   /* kernel launches per matvec                                       */
-    int aca_reruns;          /* blocks re-run because they reached the rank cap                  */
+    int aca_reruns;          /* blocks re-run because they outgrew their workspace batch         */
+    int64_t stored_bytes_inner_fp32; /* bytes one H2W_FP32_INNER matvec streams                      */
 } h2w_stats;
 
 typedef struct {
@@ -97,8 +104,12 @@ typedef struct {
     int converged;
 } h2w_gmres_stats;
 
-#define H2W_GMRES_FP64 0     /* unrestarted GMRES, FP64 matvecs                                  */
-#define H2W_GMRES_FP32_IR 1  /* FP64 iterative refinement around GMRES with FP32 matvecs        */
+#define H2W_GMRES_FP64 0         /* unrestarted GMRES, stored FP64 operators                          */
+#define H2W_GMRES_FP32_IR 1      /* FP64 iterative refinement (stored FP64 residuals) around GMRES on
+                                    the stored FP32 operators                                         */
+#define H2W_GMRES_FP32_OTF_IR 2  /* as H2W_GMRES_FP32_IR with the inner operator's near-field and M2L
+                                    blocks evaluated on the fly in FP32                               */
+#define H2W_GMRES_FP32_INNER_IR 3 /* as H2W_GMRES_FP32_IR with the truncated FP32 inner operator     */
 
 void h2w_default_opts(h2w_opts *opts);
 
@@ -121,8 +132,8 @@ int h2w_assemble(h2w_handle *h);
  * (3D/codes/matvec.hpp:63-83, :115-136; 1D/codes/matvec.hpp:51-65,89-103). */
 int h2w_matvec(h2w_handle *h, const double *x, double *y, int where);
 
-/* General form: precision H2W_FP64 or H2W_FP32 (FP32 operators, FP32 accumulation; x and y
- * are FP64 either way), order H2W_ORDER_INPUT or H2W_ORDER_TREE, nrhs right-hand sides
+/* General form: precision H2W_FP64, H2W_FP32 (FP32 operators, FP32 accumulation), H2W_FP32_OTF
+ * or H2W_FP64_OTF (x and y are FP64 either way), order H2W_ORDER_INPUT or H2W_ORDER_TREE, nrhs right-hand sides
  * stored one after another with leading dimension ld (>= N). */
 int h2w_matvec_ex(h2w_handle *h, const double *x, double *y, int nrhs, int64_t ld, int precision, int order,
                   int where);

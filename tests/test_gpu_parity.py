@@ -101,6 +101,17 @@ def test_matvec_parity(built, case):
     # FP32 shadow: FP32 operator and accumulation
     y32 = g.mat_vec_prod(x, precision=h2w.FP32)
     assert np.linalg.norm(y32 - yg) <= 2e-5 * np.linalg.norm(yg)
+    # near field and M2L evaluated on the fly: FP64 equals the stored product to rounding,
+    # FP32 stays within FP32 accuracy of it
+    y64o = g.mat_vec_prod(x, precision=h2w.FP64_OTF)
+    assert np.linalg.norm(y64o - yg) <= 1e-12 * np.linalg.norm(yg)
+    y32o = g.mat_vec_prod(x, precision=h2w.FP32_OTF)
+    assert np.linalg.norm(y32o - yg) <= 2e-5 * np.linalg.norm(yg)
+    # truncated inner operator of the refinement (ranks cut where the cross meets 1e-6)
+    if case[3] > 6:
+        yin = g.mat_vec_prod(x, precision=h2w.FP32_INNER)
+        assert np.linalg.norm(yin - yg) <= 1e-4 * np.linalg.norm(yg)
+        assert g.stats()["stored_bytes_inner_fp32"] <= g.stats()["stored_bytes_fp32"]
     # several right-hand sides at once
     X = np.stack([x, 2 * x, rng.normal(size=N)])
     Y = g.matvec_multi(X)
@@ -142,14 +153,15 @@ def test_fp32_refinement_reaches_fp64_accuracy(built, case):
     xt = np.random.default_rng(3).normal(size=N)
     b = o.dense_matvec(xt)
     x64, s64 = g.gmres(b, None, 300, gtol, h2w.GMRES_FP64)
-    x32, s32 = g.gmres(b, None, 300, gtol, h2w.GMRES_FP32_IR)
-    assert s32["converged"] and s32["resid"] <= gtol
-    # the true FP64 residual of the refined solution
-    r = np.linalg.norm(b - g.mat_vec_prod(x32)) / np.linalg.norm(b)
-    assert r <= 2 * gtol
     e64 = np.linalg.norm(x64 - xt) / np.linalg.norm(xt)
-    e32 = np.linalg.norm(x32 - xt) / np.linalg.norm(xt)
-    assert e32 <= 10 * e64 + 1e-12
+    for mode in (h2w.GMRES_FP32_IR, h2w.GMRES_FP32_OTF_IR, h2w.GMRES_FP32_INNER_IR):
+        x32, s32 = g.gmres(b, None, 300, gtol, mode)
+        assert s32["converged"] and s32["resid"] <= gtol
+        # the true FP64 residual of the refined solution
+        r = np.linalg.norm(b - g.mat_vec_prod(x32)) / np.linalg.norm(b)
+        assert r <= 2 * gtol
+        e32 = np.linalg.norm(x32 - xt) / np.linalg.norm(xt)
+        assert e32 <= 10 * e64 + 1e-12
 
 
 def test_single_level_and_errors():
