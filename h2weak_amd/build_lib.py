@@ -44,6 +44,14 @@ def build(verbose=False):
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
+    # the command-line driver over include/h2w_initialisation.hpp (links libh2w in-tree)
+    drv = os.path.join(HERE, "h2w_main")
+    cmd = ["g++", "-O2", "-std=c++17", "-Wall", os.path.join(CSRC, "h2w_main.cpp"), "-I",
+           os.path.join(HERE, "..", "include"), "-L", HERE, "-lh2w", "-Wl,-rpath,$ORIGIN", "-o", drv + ".tmp"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"driver build failed:\n{r.stdout}\n{r.stderr}")
+    os.replace(drv + ".tmp", drv)
     return LIB
 
 
