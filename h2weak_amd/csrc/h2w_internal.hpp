@@ -89,19 +89,29 @@ struct LrJob {     // copy L (unit lower) and R (upper) out of an ACA workspace
     int64_t out;   // offset (doubles): L at out, R at out + rank*rank, both column-major
 };
 
-struct EvalJob {   // A = K(rows, cols), m x n column-major at a
+// Stored blocks are column-major with the leading dimension padded to a multiple
+// of 4 elements (zero rows), so that every column starts 16-byte aligned in both
+// the FP64 pool and its FP32 shadow.
+inline int64_t pad_ld(int64_t rows) { return (rows + 3) / 4 * 4; }
+
+struct EvalJob {   // A = K(rows, cols), m x n column-major at a, leading dimension lda
     int64_t rows, cols, a;
-    int32_t m, n;
+    int32_t m, n, lda, pad;
 };
 
-struct XferJob {   // E = K(S, Ic) (L R)^{-1}, m x r column-major at e; E^T (r x m) at et
+struct XferJob {   // E = K(S, Ic) (L R)^{-1}, m x r at e (ld lde); E^T (r x m) at et (ld ldet)
     int64_t s, ic, lr, e, et;
-    int32_t m, r;
+    int32_t m, r, lde, ldet;
 };
 
-struct CurJob {    // A = (L R)^{-1} K(Ir, V), r x n column-major at a
+struct CurJob {    // A = (L R)^{-1} K(Ir, V), r x n column-major at a, leading dimension lda
     int64_t ir, v, lr, a;
-    int32_t r, n;
+    int32_t r, n, lda, pad;
+};
+
+struct CopyJob {   // B(0:m, 0:n) = A(0:m, 0:n): leading sub-block copy between pool regions
+    int64_t src, dst;
+    int32_t m, n, lds, ldd;
 };
 
 // Batched skinny GEMV work: y[row0 .. row0+rows) (+)= sum_t A_t x_t
@@ -113,12 +123,14 @@ struct GTerm {
 };
 struct GItem {
     int64_t y;     // element offset of the output segment (first row of this tile)
-    int32_t rows;  // <= 64
+    int32_t rows;  // <= kRowTile
     int32_t lda;
     int32_t row0;  // first row of this tile inside the terms' matrices
     int32_t t0, t1;
     int32_t flags; // 0 = exclusive add, 1 = atomic add
 };
+
+constexpr int kRowTile = 128;  // rows of one GEMV item
 
 // ------------------------------------------------------------------ launch wrappers (kernels.hip)
 void launch_aca(const AcaJob *jobs, int njobs, const int32_t *idx, double *ws, int64_t ws_cap,
@@ -131,6 +143,7 @@ void launch_xfer(const XferJob *jobs, int njobs, const int32_t *idx, const doubl
                  hipStream_t s);
 void launch_cur(const CurJob *jobs, int njobs, const int32_t *idx, const double *lr, double *A, KParams kp,
                 hipStream_t s);
+void launch_copy32(const CopyJob *jobs, int njobs, float *pool, hipStream_t s);
 void launch_bgemv64(const GItem *items, int nitems, const GTerm *terms, const double *A, double *vec, int role,
                     hipStream_t s);
 void launch_bgemv32(const GItem *items, int nitems, const GTerm *terms, const float *A, float *vec, int role,

@@ -164,6 +164,8 @@ struct h2w_handle {
     std::vector<h2w::Launch> launches;
     std::vector<int64_t> launch_elems;  // operator elements each launch streams
     h2w::LaunchSet inner;               // the refinement's inner operator (ranks truncated)
+    hipGraphExec_t graph[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};  // one per precision
+    bool use_graph = true;
     int64_t inner_elems = 0;
     h2w::DevBuf<h2w::GItem> d_items;
     h2w::DevBuf<h2w::GTerm> d_terms;
@@ -594,7 +596,7 @@ void cur_pairs(Handle &h, AcaPool &pool) {
 }
 
 // ------------------------------------------------------------------ work lists
-constexpr int64_t kItemWork = 64 * 4096;  // elements of A per GEMV item before column chunking
+constexpr int64_t kItemWork = 128 * 4096;  // elements of A per GEMV item before column chunking
 
 void add_item(WorkList &W, int64_t y, int rows, const std::vector<TermSpec> &terms, bool atomic, int lda = -1,
               int row_base = 0) {
@@ -603,8 +605,8 @@ void add_item(WorkList &W, int64_t y, int rows, const std::vector<TermSpec> &ter
     int64_t C = 0;
     for (auto &t : terms) C += t.cols;
     if (C == 0) return;
-    for (int r0 = 0; r0 < rows; r0 += 64) {
-        int tr = std::min(64, rows - r0);
+    for (int r0 = 0; r0 < rows; r0 += kRowTile) {
+        int tr = std::min(kRowTile, rows - r0);
         if ((int64_t)tr * C <= kItemWork) {
             GItem it;
             it.y = y + r0;
@@ -619,7 +621,7 @@ void add_item(WorkList &W, int64_t y, int rows, const std::vector<TermSpec> &ter
             W.items.push_back(it);
         } else {
             // column chunks accumulate atomically
-            const int64_t maxc = std::max<int64_t>(1, kItemWork / 64);
+            const int64_t maxc = std::max<int64_t>(1, kItemWork / kRowTile);
             GItem it;
             it.rows = tr;
             it.lda = lda;
@@ -685,7 +687,10 @@ void upload_lists(std::vector<WorkList> &WL, LaunchSet &LS, hipStream_t s) {
 }
 
 // ------------------------------------------------------------------ assembly
+void drop_graphs(Handle &h);
+
 void assemble(Handle &h) {
+    drop_graphs(h);
     double t0 = now_s();
     const bool verbose = std::getenv("H2W_VERBOSE") != nullptr;
     auto phase = [&](const char *what) {
@@ -761,8 +766,8 @@ void assemble(Handle &h) {
                 int64_t m = (int64_t)fl.S[b].size(), r = fl.rank[b];
                 memory += 2.0 * (double)(m * r);
                 if (m * r == 0) continue;
-                fl.e_off[b] = take(m * r);
-                fl.et_off[b] = take(m * r);
+                fl.e_off[b] = take(pad_ld(m) * r);
+                fl.et_off[b] = take(pad_ld(r) * m);
             }
         }
     }
@@ -781,7 +786,7 @@ void assemble(Handle &h) {
                     int64_t e = (int64_t)fl.rank[b] * fl.rank[p];
                     memory += (double)e;
                     if (e == 0) continue;
-                    m2l.push_back(M2LBlk{f, j, b, p, take(e)});
+                    m2l.push_back(M2LBlk{f, j, b, p, take(pad_ld(fl.rank[b]) * fl.rank[p])});
                 }
             }
         }
@@ -800,7 +805,7 @@ void assemble(Handle &h) {
             int64_t e = P.bcount[P.nL][b] * P.bcount[P.nL][n];
             if (count_near) memory += (double)e;
             if (e == 0) continue;
-            nearb.push_back(NearBlk{b, n, take(e)});
+            nearb.push_back(NearBlk{b, n, take(pad_ld(P.bcount[P.nL][b]) * P.bcount[P.nL][n])});
         }
     }
     for (auto &p : h.pairs) {
@@ -809,8 +814,8 @@ void assemble(Handle &h) {
         else memory += 2.0 * p.rank * p.rank + (double)(mb * p.rank + (int64_t)p.rank * nv);  // L + R + Ac + Ar
         p.ac_off = p.ar_off = -1;
         if (p.rank == 0) continue;
-        p.ac_off = take(mb * p.rank);
-        p.ar_off = take((int64_t)p.rank * nv);
+        p.ac_off = take(pad_ld(mb) * p.rank);
+        p.ar_off = take(pad_ld(p.rank) * nv);
     }
     h.mat_len = std::max<int64_t>(mo, 32);
     h.memory = memory;
@@ -894,6 +899,8 @@ void assemble(Handle &h) {
                     lr.insert(lr.end(), fl.lr[b].begin(), fl.lr[b].end());
                     J.e = fl.e_off[b];
                     J.et = fl.et_off[b];
+                    J.lde = (int32_t)pad_ld(J.m);
+                    J.ldet = (int32_t)pad_ld(J.r);
                     xj.push_back(J);
                 }
             }
@@ -913,6 +920,8 @@ void assemble(Handle &h) {
             J.lr = (int64_t)lr.size();
             lr.insert(lr.end(), p.lr.begin(), p.lr.end());
             J.a = p.ar_off;
+            J.lda = (int32_t)pad_ld(J.r);
+            J.pad = 0;
             cj.push_back(J);
         }
         DevBuf<int32_t> didx;
@@ -938,6 +947,8 @@ void assemble(Handle &h) {
             EvalJob J;
             J.m = (int32_t)r.size();
             J.n = (int32_t)c.size();
+            J.lda = (int32_t)pad_ld(J.m);
+            J.pad = 0;
             J.rows = (int64_t)idx.size();
             idx.insert(idx.end(), r.begin(), r.end());
             J.cols = (int64_t)idx.size();
@@ -979,6 +990,13 @@ void assemble(Handle &h) {
     // sub-blocks of the stored ones, its transfers are recomputed from the leading L, R blocks
     const bool inner_on = h.opts.fp32_shadow && h.opts.ir_tol_pow > 0 && h.opts.ir_tol_pow < h.tol_pow;
     int64_t in_len = 0;
+    auto take_in = [&](int64_t elems) {
+        int64_t o = in_len;
+        in_len += align_up(elems, 32);
+        return o;
+    };
+    // compact copies of the inner M2L sub-blocks: (index into m2l, offset in the inner region)
+    std::vector<std::pair<size_t, int64_t>> m2l_in;
     if (inner_on) {
         const double tau = std::pow(10.0, -1.0 * h.opts.ir_tol_pow);
         for (int f = 0; f < 2; ++f) {
@@ -989,6 +1007,8 @@ void assemble(Handle &h) {
                 for (int b = 0; b < fl.nb; ++b)
                     fl.kin[b] = fl.rank[b] > 0 ? inner_rank(fl.hist[b], fl.rank[b], tau) : 0;
             }
+            // inner transfers: rows are the children's leading pivots, each child's segment
+            // padded to a multiple of 4 rows so every row tile stays aligned
             for (int j = nfam_lo[f]; j <= P.nL; ++j) {
                 FamLevel &fl = h.fam[f][j];
                 fl.ein_off.assign(fl.nb, -1);
@@ -997,19 +1017,23 @@ void assemble(Handle &h) {
                     int64_t m = 0;
                     if (j == P.nL) m = P.bcount[j][b];
                     else
-                        for (int c = 0; c < P.nchild; ++c) m += h.fam[f][j + 1].kin[b * P.nchild + c];
-                    int64_t e = m * fl.kin[b];
-                    if (e == 0) continue;
-                    fl.ein_off[b] = in_len;
-                    in_len += align_up(e, 32);
-                    fl.eint_off[b] = in_len;
-                    in_len += align_up(e, 32);
+                        for (int c = 0; c < P.nchild; ++c) m += pad_ld(h.fam[f][j + 1].kin[b * P.nchild + c]);
+                    if (m * fl.kin[b] == 0) continue;
+                    fl.ein_off[b] = take_in(pad_ld(m) * fl.kin[b]);
+                    fl.eint_off[b] = take_in(pad_ld(fl.kin[b]) * m);
                 }
             }
+        }
+        for (size_t q = 0; q < m2l.size(); ++q) {
+            const M2LBlk &m = m2l[q];
+            const FamLevel &fl = h.fam[m.f][m.j];
+            if (fl.kin[m.b] > 0 && fl.kin[m.p] > 0)
+                m2l_in.push_back({q, take_in(pad_ld(fl.kin[m.b]) * fl.kin[m.p])});
         }
     }
     if (h.opts.fp32_shadow) {
         h.mat32.alloc(h.mat_len + in_len);
+        if (in_len > 0) H2W_HIP(hipMemsetAsync(h.mat32.p + h.mat_len, 0, in_len * 4, s));
         launch_to_fp32(h.mat64.p, h.mat32.p, h.mat_len, s);
         H2W_HIP(hipGetLastError());
     } else {
@@ -1026,29 +1050,46 @@ void assemble(Handle &h) {
                 for (int b = 0; b < fl.nb; ++b) {
                     if (fl.ein_off[b] < 0) continue;
                     const int k = fl.kin[b], r = fl.rank[b];
-                    XferJob J;
-                    J.r = k;
-                    J.s = (int64_t)idx.size();
+                    // leading k x k blocks of L and R
+                    const int64_t lro = (int64_t)lr.size();
+                    for (int part = 0; part < 2; ++part)
+                        for (int cc = 0; cc < k; ++cc)
+                            for (int rr = 0; rr < k; ++rr)
+                                lr.push_back(fl.lr[b][(size_t)part * r * r + (size_t)cc * r + rr]);
+                    const int64_t icq = (int64_t)idx.size();
+                    idx.insert(idx.end(), fl.Ic[b].begin(), fl.Ic[b].begin() + k);
+                    int64_t m = 0;
+                    if (j == P.nL) m = P.bcount[j][b];
+                    else
+                        for (int c = 0; c < P.nchild; ++c) m += pad_ld(h.fam[f][j + 1].kin[b * P.nchild + c]);
+                    const int lde = (int)pad_ld(m), ldet = (int)pad_ld(k);
+                    auto add_job = [&](const int32_t *S, int ms, int64_t row) {
+                        if (ms == 0) return;
+                        XferJob J;
+                        J.r = k;
+                        J.m = ms;
+                        J.s = (int64_t)idx.size();
+                        idx.insert(idx.end(), S, S + ms);
+                        J.ic = icq;
+                        J.lr = lro;
+                        J.e = fl.ein_off[b] + row;
+                        J.et = fl.eint_off[b] + row * ldet;
+                        J.lde = lde;
+                        J.ldet = ldet;
+                        xj.push_back(J);
+                    };
                     if (j == P.nL) {
                         auto rl = range_list(P.bstart[j][b], P.bcount[j][b]);
-                        idx.insert(idx.end(), rl.begin(), rl.end());
+                        add_job(rl.data(), (int)rl.size(), 0);
                     } else {
+                        int64_t row = 0;
                         for (int c = 0; c < P.nchild; ++c) {
                             const int cc = b * P.nchild + c;
-                            auto &ir = h.fam[f][j + 1].Ir[cc];
-                            idx.insert(idx.end(), ir.begin(), ir.begin() + h.fam[f][j + 1].kin[cc]);
+                            const int kc = h.fam[f][j + 1].kin[cc];
+                            add_job(h.fam[f][j + 1].Ir[cc].data(), kc, row);
+                            row += pad_ld(kc);
                         }
                     }
-                    J.m = (int32_t)((int64_t)idx.size() - J.s);
-                    J.ic = (int64_t)idx.size();
-                    idx.insert(idx.end(), fl.Ic[b].begin(), fl.Ic[b].begin() + k);
-                    J.lr = (int64_t)lr.size();
-                    for (int part = 0; part < 2; ++part)  // leading k x k blocks of L and R
-                        for (int cc = 0; cc < k; ++cc)
-                            for (int rr = 0; rr < k; ++rr) lr.push_back(fl.lr[b][(size_t)part * r * r + (size_t)cc * r + rr]);
-                    J.e = fl.ein_off[b];
-                    J.et = fl.eint_off[b];
-                    xj.push_back(J);
                 }
             }
         }
@@ -1059,9 +1100,28 @@ void assemble(Handle &h) {
         dlr.upload(lr, s);
         dxj.upload(xj, s);
         tmp.alloc(in_len);
+        H2W_HIP(hipMemsetAsync(tmp.p, 0, in_len * 8, s));
         launch_xfer(dxj.p, (int)xj.size(), didx.p, dlr.p, tmp.p, h.kp, s);
         H2W_HIP(hipGetLastError());
         launch_to_fp32(tmp.p, h.mat32.p + h.mat_len, in_len, s);
+        H2W_HIP(hipGetLastError());
+        // compact copies of the leading M2L sub-blocks
+        std::vector<CopyJob> cj;
+        for (auto &pr : m2l_in) {
+            const M2LBlk &m = m2l[pr.first];
+            const FamLevel &fl = h.fam[m.f][m.j];
+            CopyJob J;
+            J.src = m.off;
+            J.dst = h.mat_len + pr.second;
+            J.m = fl.kin[m.b];
+            J.n = fl.kin[m.p];
+            J.lds = (int32_t)pad_ld(fl.rank[m.b]);
+            J.ldd = (int32_t)pad_ld(fl.kin[m.b]);
+            cj.push_back(J);
+        }
+        DevBuf<CopyJob> dcj;
+        dcj.upload(cj, s);
+        launch_copy32(dcj.p, (int)cj.size(), h.mat32.p, s);
         H2W_HIP(hipGetLastError());
         H2W_HIP(hipStreamSynchronize(s));
     }
@@ -1071,7 +1131,8 @@ void assemble(Handle &h) {
     auto fam_up_item = [&](WorkList &W, int f, int j, int b, int64_t src) {
         FamLevel &fl = h.fam[f][j];
         if (fl.et_off[b] < 0) return;
-        add_item(W, fl.o_off[b], fl.rank[b], {TermSpec{fl.et_off[b], src, (int32_t)fl.S[b].size()}}, false);
+        add_item(W, fl.o_off[b], fl.rank[b], {TermSpec{fl.et_off[b], src, (int32_t)fl.S[b].size()}}, false,
+                 (int)pad_ld(fl.rank[b]));
     };
     // up, leaf level (and semi-nested row factors, which read x directly)
     {
@@ -1084,7 +1145,7 @@ void assemble(Handle &h) {
             if (p.rank > 0)
                 add_item(W, p.t_off, p.rank,
                          {TermSpec{p.ar_off, h.off_x + P.bstart[p.level][p.v], (int32_t)P.bcount[p.level][p.v]}},
-                         false);
+                         false, (int)pad_ld(p.rank));
         WL.push_back(std::move(W));
     }
     // up, levels nL-1 .. 1
@@ -1109,13 +1170,16 @@ void assemble(Handle &h) {
                 for (auto &m : m2l)
                     if (m.f == f && m.j == j)
                         per[m.b].push_back(TermSpec{m.off, fl.o_off[m.p], (int32_t)fl.rank[m.p]});
-                for (int b = 0; b < fl.nb; ++b) add_item(W, fl.u_off[b], fl.rank[b], per[b], false);
+                for (int b = 0; b < fl.nb; ++b)
+                    add_item(W, fl.u_off[b], fl.rank[b], per[b], false, (int)pad_ld(fl.rank[b]));
             }
         }
         std::vector<std::vector<TermSpec>> per(nleaf);
         for (auto &nb : nearb)
             per[nb.b].push_back(TermSpec{nb.off, h.off_x + P.bstart[P.nL][nb.n], (int32_t)P.bcount[P.nL][nb.n]});
-        for (int b = 0; b < nleaf; ++b) add_item(W, h.off_y + P.bstart[P.nL][b], (int)P.bcount[P.nL][b], per[b], false);
+        for (int b = 0; b < nleaf; ++b)
+            add_item(W, h.off_y + P.bstart[P.nL][b], (int)P.bcount[P.nL][b], per[b], false,
+                     (int)pad_ld(P.bcount[P.nL][b]));
         WL.push_back(std::move(W));
     }
     // down, levels 1 .. nL-1
@@ -1128,7 +1192,7 @@ void assemble(Handle &h) {
                 if (fl.e_off[b] < 0) continue;
                 int64_t m = (int64_t)fl.S[b].size();
                 add_item(W, h.fam[f][j + 1].u_off[b * P.nchild], (int)m,
-                         {TermSpec{fl.e_off[b], fl.u_off[b], (int32_t)fl.rank[b]}}, false);
+                         {TermSpec{fl.e_off[b], fl.u_off[b], (int32_t)fl.rank[b]}}, false, (int)pad_ld(m));
             }
         }
         WL.push_back(std::move(W));
@@ -1143,7 +1207,8 @@ void assemble(Handle &h) {
                 FamLevel &fl = h.fam[f][P.nL];
                 if (fl.e_off[b] >= 0) t.push_back(TermSpec{fl.e_off[b], fl.u_off[b], (int32_t)fl.rank[b]});
             }
-            add_item(W, h.off_y + P.bstart[P.nL][b], (int)P.bcount[P.nL][b], t, false);
+            add_item(W, h.off_y + P.bstart[P.nL][b], (int)P.bcount[P.nL][b], t, false,
+                     (int)pad_ld(P.bcount[P.nL][b]));
         }
         WL.push_back(std::move(W));
     }
@@ -1157,7 +1222,8 @@ void assemble(Handle &h) {
             if (p.rank > 0) per[p.level][p.b].push_back(TermSpec{p.ac_off, p.t_off, (int32_t)p.rank});
         for (int j = 1; j <= P.nL; ++j)
             for (int b = 0; b < P.nboxes[j]; ++b)
-                add_item(W, h.off_y + P.bstart[j][b], (int)P.bcount[j][b], per[j][b], true);
+                add_item(W, h.off_y + P.bstart[j][b], (int)P.bcount[j][b], per[j][b], true,
+                         (int)pad_ld(P.bcount[j][b]));
         WL.push_back(std::move(W));
     }
     std::vector<GItem> items;
@@ -1191,7 +1257,7 @@ void assemble(Handle &h) {
     }
     h.stored_elems = se;
     if (inner_on) {
-        const int64_t base = h.mat_len;  // inner transfers follow the main operators in the FP32 pool
+        const int64_t base = h.mat_len;  // the inner operator follows the main operators in the FP32 pool
         std::vector<WorkList> IL;
         {
             WorkList W;
@@ -1202,13 +1268,13 @@ void assemble(Handle &h) {
                     if (fl.eint_off[b] >= 0)
                         add_item(W, fl.o_off[b], fl.kin[b],
                                  {TermSpec{base + fl.eint_off[b], h.off_x + P.bstart[P.nL][b], (int32_t)P.bcount[P.nL][b]}},
-                                 false);
+                                 false, (int)pad_ld(fl.kin[b]));
             }
             for (auto &p : h.pairs)
                 if (p.rank > 0)
                     add_item(W, p.t_off, p.rank,
                              {TermSpec{p.ar_off, h.off_x + P.bstart[p.level][p.v], (int32_t)P.bcount[p.level][p.v]}},
-                             false);
+                             false, (int)pad_ld(p.rank));
             IL.push_back(std::move(W));
         }
         for (int j = P.nL - 1; j >= 1; --j) {
@@ -1219,15 +1285,16 @@ void assemble(Handle &h) {
                 FamLevel &ch = h.fam[f][j + 1];
                 for (int b = 0; b < fl.nb; ++b) {
                     if (fl.eint_off[b] < 0) continue;
+                    const int64_t ldet = pad_ld(fl.kin[b]);
                     std::vector<TermSpec> t;
                     int64_t col = 0;
                     for (int c = 0; c < P.nchild; ++c) {
                         const int cc = b * P.nchild + c;
                         if (ch.kin[cc] > 0)
-                            t.push_back(TermSpec{base + fl.eint_off[b] + col * fl.kin[b], ch.o_off[cc], (int32_t)ch.kin[cc]});
-                        col += ch.kin[cc];
+                            t.push_back(TermSpec{base + fl.eint_off[b] + col * ldet, ch.o_off[cc], (int32_t)ch.kin[cc]});
+                        col += pad_ld(ch.kin[cc]);
                     }
-                    add_item(W, fl.o_off[b], fl.kin[b], t, false);
+                    add_item(W, fl.o_off[b], fl.kin[b], t, false, (int)ldet);
                 }
             }
             IL.push_back(std::move(W));
@@ -1235,23 +1302,32 @@ void assemble(Handle &h) {
         {
             WorkList W;
             W.role = 1;
+            std::vector<std::vector<std::vector<std::vector<TermSpec>>>> per(2);
+            for (int f = 0; f < 2; ++f) {
+                if (!fam_on[f]) continue;
+                per[f].assign(P.nL + 1, {});
+                for (int j = nfam_lo[f]; j <= P.nL; ++j) per[f][j].assign(h.fam[f][j].nb, {});
+            }
+            for (auto &pr : m2l_in) {
+                const M2LBlk &m = m2l[pr.first];
+                const FamLevel &fl = h.fam[m.f][m.j];
+                per[m.f][m.j][m.b].push_back(TermSpec{base + pr.second, fl.o_off[m.p], (int32_t)fl.kin[m.p]});
+            }
             for (int f = 0; f < 2; ++f) {
                 if (!fam_on[f]) continue;
                 for (int j = nfam_lo[f]; j <= P.nL; ++j) {
                     FamLevel &fl = h.fam[f][j];
-                    std::vector<std::vector<TermSpec>> per(fl.nb);
-                    for (auto &m : m2l)
-                        if (m.f == f && m.j == j && fl.kin[m.p] > 0)
-                            per[m.b].push_back(TermSpec{m.off, fl.o_off[m.p], (int32_t)fl.kin[m.p]});
                     for (int b = 0; b < fl.nb; ++b)
-                        if (fl.kin[b] > 0) add_item(W, fl.u_off[b], fl.kin[b], per[b], false, fl.rank[b]);
+                        if (fl.kin[b] > 0)
+                            add_item(W, fl.u_off[b], fl.kin[b], per[f][j][b], false, (int)pad_ld(fl.kin[b]));
                 }
             }
-            std::vector<std::vector<TermSpec>> per(nleaf);
+            std::vector<std::vector<TermSpec>> pn(nleaf);
             for (auto &nb : nearb)
-                per[nb.b].push_back(TermSpec{nb.off, h.off_x + P.bstart[P.nL][nb.n], (int32_t)P.bcount[P.nL][nb.n]});
+                pn[nb.b].push_back(TermSpec{nb.off, h.off_x + P.bstart[P.nL][nb.n], (int32_t)P.bcount[P.nL][nb.n]});
             for (int b = 0; b < nleaf; ++b)
-                add_item(W, h.off_y + P.bstart[P.nL][b], (int)P.bcount[P.nL][b], per[b], false);
+                add_item(W, h.off_y + P.bstart[P.nL][b], (int)P.bcount[P.nL][b], pn[b], false,
+                         (int)pad_ld(P.bcount[P.nL][b]));
             IL.push_back(std::move(W));
         }
         for (int j = 1; j <= P.nL - 1; ++j) {
@@ -1262,15 +1338,16 @@ void assemble(Handle &h) {
                 FamLevel &ch = h.fam[f][j + 1];
                 for (int b = 0; b < fl.nb; ++b) {
                     if (fl.ein_off[b] < 0) continue;
-                    int m = 0;
-                    for (int c = 0; c < P.nchild; ++c) m += ch.kin[b * P.nchild + c];
-                    int row = 0;
+                    int64_t m = 0;
+                    for (int c = 0; c < P.nchild; ++c) m += pad_ld(ch.kin[b * P.nchild + c]);
+                    int64_t row = 0;
                     for (int c = 0; c < P.nchild; ++c) {
                         const int cc = b * P.nchild + c;
                         if (ch.kin[cc] > 0)
                             add_item(W, ch.u_off[cc], ch.kin[cc],
-                                     {TermSpec{base + fl.ein_off[b], fl.u_off[b], (int32_t)fl.kin[b]}}, false, m, row);
-                        row += ch.kin[cc];
+                                     {TermSpec{base + fl.ein_off[b], fl.u_off[b], (int32_t)fl.kin[b]}}, false,
+                                     (int)pad_ld(m), (int)row);
+                        row += pad_ld(ch.kin[cc]);
                     }
                 }
             }
@@ -1285,7 +1362,8 @@ void assemble(Handle &h) {
                     FamLevel &fl = h.fam[f][P.nL];
                     if (fl.ein_off[b] >= 0) t.push_back(TermSpec{base + fl.ein_off[b], fl.u_off[b], (int32_t)fl.kin[b]});
                 }
-                add_item(W, h.off_y + P.bstart[P.nL][b], (int)P.bcount[P.nL][b], t, false);
+                add_item(W, h.off_y + P.bstart[P.nL][b], (int)P.bcount[P.nL][b], t, false,
+                         (int)pad_ld(P.bcount[P.nL][b]));
             }
             IL.push_back(std::move(W));
         }
@@ -1298,7 +1376,8 @@ void assemble(Handle &h) {
                 if (p.rank > 0) per[p.level][p.b].push_back(TermSpec{p.ac_off, p.t_off, (int32_t)p.rank});
             for (int j = 1; j <= P.nL; ++j)
                 for (int b = 0; b < P.nboxes[j]; ++b)
-                    add_item(W, h.off_y + P.bstart[j][b], (int)P.bcount[j][b], per[j][b], true);
+                    add_item(W, h.off_y + P.bstart[j][b], (int)P.bcount[j][b], per[j][b], true,
+                             (int)pad_ld(P.bcount[j][b]));
             IL.push_back(std::move(W));
         }
         upload_lists(IL, h.inner, s);
@@ -1351,11 +1430,47 @@ const std::vector<Launch> &launches_for(Handle &h, int precision) {
     return h.launches;
 }
 
-void run_launches(Handle &h, int precision) {
+void run_launches_eager(Handle &h, int precision) {
     const auto &L = launches_for(h, precision);
     zero_outputs(h, precision);
     for (auto &Lh : L) run_launch(h, Lh, precision);
     H2W_HIP(hipGetLastError());
+}
+
+// The launch sequence of a matvec is fixed per handle and precision (pool pointers and work
+// lists do not change after assembly), so it is captured once into a HIP graph and replayed.
+// Every node is a kernel (the output zeroing included).
+void run_launches(Handle &h, int precision) {
+    if (!h.use_graph) {
+        run_launches_eager(h, precision);
+        return;
+    }
+    if (!h.graph[precision]) {
+        hipGraph_t g = nullptr;
+        H2W_HIP(hipStreamBeginCapture(h.stream, hipStreamCaptureModeThreadLocal));
+        try {
+            run_launches_eager(h, precision);
+        } catch (...) {
+            (void)hipStreamEndCapture(h.stream, &g);
+            if (g) (void)hipGraphDestroy(g);
+            throw;
+        }
+        H2W_HIP(hipStreamEndCapture(h.stream, &g));
+        hipGraphExec_t ex = nullptr;
+        hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (e != hipSuccess) throw Error{H2W_EHIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e)};
+        h.graph[precision] = ex;
+    }
+    H2W_HIP(hipGraphLaunch(h.graph[precision], h.stream));
+}
+
+void drop_graphs(Handle &h) {
+    for (auto &g : h.graph)
+        if (g) {
+            (void)hipGraphExecDestroy(g);
+            g = nullptr;
+        }
 }
 
 // device-pointer matvec on one vector; tree-ordered in/out when tree == true
@@ -1627,6 +1742,7 @@ int h2w_create(int dim, int variant, int64_t N, const double *points, int n_max,
     h->N = N;
     h->L = L;
     h->opts = o;
+    h->use_graph = std::getenv("H2W_NO_GRAPH") == nullptr;
     h->plan.build(dim, N, points, L, nl);
     h->diag = h2w::diag_value(dim, choice, N, o.compat);
     if (o.plan_only) {
@@ -1986,6 +2102,7 @@ void h2w_destroy(h2w_handle *h) {
     if (!h) return;
     if (!h->opts.plan_only) (void)hipSetDevice(h->opts.device);
     if (h->stream) (void)hipStreamSynchronize(h->stream);
+    h2w::drop_graphs(*h);
     if (h->stream) (void)hipStreamDestroy(h->stream);
     delete h;
 }

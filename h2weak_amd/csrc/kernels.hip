@@ -415,7 +415,7 @@ __global__ void k_eval(const EvalJob *__restrict__ jobs, const int32_t *__restri
     const int32_t *cols = idx + jb.cols;
     for (int64_t e = threadIdx.x; e < tot; e += blockDim.x) {
         int i = (int)(e % jb.m), j = (int)(e / jb.m);
-        A[jb.a + e] = kentry2(kp, rows[i], cols[j]);
+        A[jb.a + (int64_t)j * jb.lda + i] = kentry2(kp, rows[i], cols[j]);
     }
 }
 
@@ -427,6 +427,7 @@ __global__ void k_xfer(const XferJob *__restrict__ jobs, const int32_t *__restri
                        const double *__restrict__ lr, double *__restrict__ A, KParams kp) {
     const XferJob jb = jobs[blockIdx.x];
     const int m = jb.m, r = jb.r;
+    const int64_t lde = jb.lde, ldet = jb.ldet;
     const double *L = lr + jb.lr;
     const double *R = L + (int64_t)r * r;
     double *E = A + jb.e;
@@ -438,16 +439,16 @@ __global__ void k_xfer(const XferJob *__restrict__ jobs, const int32_t *__restri
         // X R = K(S_t, Ic)
         for (int k = 0; k < r; ++k) {
             double s = kentry(kp, p, Ic[k]);
-            for (int q = 0; q < k; ++q) s -= E[(int64_t)q * m + t] * R[(int64_t)k * r + q];
-            E[(int64_t)k * m + t] = s / R[(int64_t)k * r + k];
+            for (int q = 0; q < k; ++q) s -= E[q * lde + t] * R[(int64_t)k * r + q];
+            E[k * lde + t] = s / R[(int64_t)k * r + k];
         }
         // Y L = X
         for (int k = r - 1; k >= 0; --k) {
-            double s = E[(int64_t)k * m + t];
-            for (int q = k + 1; q < r; ++q) s -= E[(int64_t)q * m + t] * L[(int64_t)k * r + q];
-            E[(int64_t)k * m + t] = s / L[(int64_t)k * r + k];
+            double s = E[k * lde + t];
+            for (int q = k + 1; q < r; ++q) s -= E[q * lde + t] * L[(int64_t)k * r + q];
+            E[k * lde + t] = s / L[(int64_t)k * r + k];
         }
-        for (int k = 0; k < r; ++k) ET[(int64_t)t * r + k] = E[(int64_t)k * m + t];
+        for (int k = 0; k < r; ++k) ET[t * ldet + k] = E[k * lde + t];
     }
 }
 
@@ -463,7 +464,7 @@ __global__ void k_cur(const CurJob *__restrict__ jobs, const int32_t *__restrict
     const int32_t *Ir = idx + jb.ir;
     const int32_t *Vv = idx + jb.v;
     for (int t = threadIdx.x; t < n; t += blockDim.x) {
-        double *a = A + jb.a + (int64_t)t * r;
+        double *a = A + jb.a + (int64_t)t * jb.lda;
         PointI p = load_point(kp, Vv[t]);
         for (int k = 0; k < r; ++k) a[k] = kentry(kp, p, Ir[k]);
         for (int k = 0; k < r; ++k) {
@@ -480,44 +481,83 @@ __global__ void k_cur(const CurJob *__restrict__ jobs, const int32_t *__restrict
 }
 
 // ------------------------------------------------------------------ batched skinny GEMV
-// One workgroup (4 waves) per item; lanes run over the item's rows (<= 64),
-// the waves split every term's columns.  A is column-major, so each wave load
-// reads `rows` contiguous elements of one column.  Partial sums of the waves
-// are combined in LDS and added to y (exclusively or atomically).
+// One workgroup (4 waves) per item of <= 128 rows.  Each lane loads VEC
+// consecutive rows of a column with one 16-byte load (VEC = 2 for FP64, 4 for
+// FP32); ceil(rows / VEC) lanes, rounded up to a power of two, cover a column,
+// so a wave load reads 64/lpc whole columns.  The waves split the columns of
+// every term.  Partial sums are combined across the wave's column groups with
+// cross-lane adds, then across the waves in LDS, and added to y exclusively
+// or atomically.  Blocks are column-major with leading dimensions padded to a
+// multiple of 4, so every VEC group is aligned and stays inside the column.
 // ROLE only names the instance (0: transfer passes, 1: M2L + near field,
 // 2: semi-nested column factors) so that profiles separate the launches.
-template <typename T, int ROLE>
+template <typename T, int VEC, int ROLE>
 __global__ __launch_bounds__(256) void k_bgemv(const GItem *__restrict__ items, const GTerm *__restrict__ terms,
                                                const T *__restrict__ A, T *vec) {
+    typedef T V __attribute__((ext_vector_type(VEC)));
+    constexpr int UNR = 4;
     const GItem it = items[blockIdx.x];
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    __shared__ T part[4][64];
-    T acc0 = 0, acc1 = 0;
-    if (lane < it.rows) {
-        for (int tt = it.t0; tt < it.t1; ++tt) {
-            const GTerm tm = terms[tt];
-            const T *a = A + tm.a + it.row0 + lane;
-            const T *x = vec + tm.x;
-            const int64_t lda = it.lda;
-            int c = wv;
-            for (; c + 12 < tm.cols; c += 16) {
-                T a0 = a[(int64_t)c * lda], a1 = a[(int64_t)(c + 4) * lda];
-                T a2 = a[(int64_t)(c + 8) * lda], a3 = a[(int64_t)(c + 12) * lda];
-                acc0 = fma(a0, x[c], acc0);
-                acc1 = fma(a1, x[c + 4], acc1);
-                acc0 = fma(a2, x[c + 8], acc0);
-                acc1 = fma(a3, x[c + 12], acc1);
+    const int lane = threadIdx.x & 63;
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int rows = it.rows;
+    int lpc = 1;
+    while (lpc * VEC < rows) lpc <<= 1;
+    const int cpw = 64 / lpc;
+    const int q = lane / lpc, r0 = (lane % lpc) * VEC;
+    const bool act = r0 < rows;
+    __shared__ T part[4][kRowTile];
+    V acc = (V)0;
+    const int64_t lda = it.lda;
+    const int step = 4 * cpw;
+    for (int tt = it.t0; tt < it.t1; ++tt) {
+        const GTerm tm = terms[tt];
+        const T *a = A + tm.a + it.row0 + r0;
+        const T *x = vec + tm.x;
+        const int cols = tm.cols;
+        for (int c = wv * cpw + q; c < cols + 0; c += step * UNR) {
+            V av[UNR];
+            T xv[UNR];
+#pragma unroll
+            for (int u = 0; u < UNR; ++u) {
+                const int cc = c + u * step;
+                if (act && cc < cols) {
+                    av[u] = __builtin_nontemporal_load((const V *)(a + (int64_t)cc * lda));
+                    xv[u] = x[cc];
+                } else {
+                    av[u] = (V)0;
+                    xv[u] = T(0);
+                }
             }
-            for (; c < tm.cols; c += 4) acc0 = fma(a[(int64_t)c * lda], x[c], acc0);
+#pragma unroll
+            for (int u = 0; u < UNR; ++u)
+#pragma unroll
+                for (int v = 0; v < VEC; ++v) acc[v] = fma(av[u][v], xv[u], acc[v]);
         }
     }
-    part[wv][lane] = acc0 + acc1;
+#pragma unroll
+    for (int v = 0; v < VEC; ++v)
+        for (int o = lpc; o < 64; o <<= 1) acc[v] += __shfl_xor(acc[v], o, 64);
+    if (q == 0 && act)
+#pragma unroll
+        for (int v = 0; v < VEC; ++v)
+            if (r0 + v < kRowTile) part[threadIdx.x >> 6][r0 + v] = acc[v];
     __syncthreads();
-    if (wv == 0 && lane < it.rows) {
-        T s = part[0][lane] + part[1][lane] + part[2][lane] + part[3][lane];
-        T *y = vec + it.y + lane;
+    if (threadIdx.x < rows) {
+        const int t = threadIdx.x;
+        T s = part[0][t] + part[1][t] + part[2][t] + part[3][t];
+        T *y = vec + it.y + t;
         if (it.flags & 1) atomicAdd(y, s);
         else *y += s;
+    }
+}
+
+// leading sub-block copies inside the FP32 pool (compact inner-operator blocks)
+__global__ void k_copy32(const CopyJob *__restrict__ jobs, float *pool) {
+    const CopyJob jb = jobs[blockIdx.x];
+    const int64_t tot = (int64_t)jb.m * jb.n;
+    for (int64_t e = threadIdx.x; e < tot; e += blockDim.x) {
+        int i = (int)(e % jb.m), j = (int)(e / jb.m);
+        pool[jb.dst + (int64_t)j * jb.ldd + i] = pool[jb.src + (int64_t)j * jb.lds + i];
     }
 }
 
@@ -595,41 +635,45 @@ __global__ __launch_bounds__(256) void k_otf(const GItem *__restrict__ items, co
     const GItem it = items[blockIdx.x];
     const int lane = threadIdx.x & 63;
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    __shared__ T part[4][64];
-    T acc0 = 0, acc1 = 0;
-    if (lane < it.rows) {
-        const int64_t ts = it.y + lane;
-        const T tx = C.x[ts], ty = dim > 1 ? C.y[ts] : T(0), tz = dim > 2 ? C.z[ts] : T(0);
-        float txl = 0.f, tyl = 0.f, tzl = 0.f;
-        if (C.xl) {
-            txl = C.xl[ts];
-            tyl = dim > 1 ? C.yl[ts] : 0.f;
-            tzl = dim > 2 ? C.zl[ts] : 0.f;
-        }
-        const int tp = pid[ts];
-        for (int tt = it.t0; tt < it.t1; ++tt) {
-            const GTerm tm = terms[tt];
-            const int64_t x0 = tm.x;
-            int c = wv;
-            for (; c + 4 < tm.cols; c += 8) {
-                const int64_t s0 = x0 + c, s1 = s0 + 4;
-                T e0 = otf_entry<T>(C, pid, dim, choice, diag, tx, ty, tz, txl, tyl, tzl, tp, s0);
-                T e1 = otf_entry<T>(C, pid, dim, choice, diag, tx, ty, tz, txl, tyl, tzl, tp, s1);
-                acc0 = fma(e0, vec[s0], acc0);
-                acc1 = fma(e1, vec[s1], acc1);
+    __shared__ T part[4][kRowTile];
+    for (int half = 0; half * 64 < it.rows; ++half) {
+        const int row = half * 64 + lane;
+        T acc0 = 0, acc1 = 0;
+        if (row < it.rows) {
+            const int64_t ts = it.y + row;
+            const T tx = C.x[ts], ty = dim > 1 ? C.y[ts] : T(0), tz = dim > 2 ? C.z[ts] : T(0);
+            float txl = 0.f, tyl = 0.f, tzl = 0.f;
+            if (C.xl) {
+                txl = C.xl[ts];
+                tyl = dim > 1 ? C.yl[ts] : 0.f;
+                tzl = dim > 2 ? C.zl[ts] : 0.f;
             }
-            for (; c < tm.cols; c += 4) {
-                const int64_t s0 = x0 + c;
-                T e0 = otf_entry<T>(C, pid, dim, choice, diag, tx, ty, tz, txl, tyl, tzl, tp, s0);
-                acc0 = fma(e0, vec[s0], acc0);
+            const int tp = pid[ts];
+            for (int tt = it.t0; tt < it.t1; ++tt) {
+                const GTerm tm = terms[tt];
+                const int64_t x0 = tm.x;
+                int c = wv;
+                for (; c + 4 < tm.cols; c += 8) {
+                    const int64_t s0 = x0 + c, s1 = s0 + 4;
+                    T e0 = otf_entry<T>(C, pid, dim, choice, diag, tx, ty, tz, txl, tyl, tzl, tp, s0);
+                    T e1 = otf_entry<T>(C, pid, dim, choice, diag, tx, ty, tz, txl, tyl, tzl, tp, s1);
+                    acc0 = fma(e0, vec[s0], acc0);
+                    acc1 = fma(e1, vec[s1], acc1);
+                }
+                for (; c < tm.cols; c += 4) {
+                    const int64_t s0 = x0 + c;
+                    T e0 = otf_entry<T>(C, pid, dim, choice, diag, tx, ty, tz, txl, tyl, tzl, tp, s0);
+                    acc0 = fma(e0, vec[s0], acc0);
+                }
             }
         }
+        part[threadIdx.x >> 6][half * 64 + lane] = acc0 + acc1;
     }
-    part[threadIdx.x >> 6][lane] = acc0 + acc1;
     __syncthreads();
-    if (threadIdx.x < 64 && lane < it.rows) {
-        T s = part[0][lane] + part[1][lane] + part[2][lane] + part[3][lane];
-        T *y = vec + it.y + lane;
+    if (threadIdx.x < it.rows) {
+        const int t = threadIdx.x;
+        T s = part[0][t] + part[1][t] + part[2][t] + part[3][t];
+        T *y = vec + it.y + t;
         if (it.flags & 1) atomicAdd(y, s);
         else *y += s;
     }
@@ -789,9 +833,14 @@ template <typename T>
 static void launch_bgemv_t(const GItem *items, int nitems, const GTerm *terms, const T *A, T *vec, int role,
                            hipStream_t s) {
     if (nitems <= 0) return;
-    if (role == 1) hipLaunchKernelGGL((k_bgemv<T, 1>), dim3(nitems), dim3(256), 0, s, items, terms, A, vec);
-    else if (role == 2) hipLaunchKernelGGL((k_bgemv<T, 2>), dim3(nitems), dim3(256), 0, s, items, terms, A, vec);
-    else hipLaunchKernelGGL((k_bgemv<T, 0>), dim3(nitems), dim3(256), 0, s, items, terms, A, vec);
+    constexpr int VEC = sizeof(T) == 8 ? 2 : 4;
+    if (role == 1) hipLaunchKernelGGL((k_bgemv<T, VEC, 1>), dim3(nitems), dim3(256), 0, s, items, terms, A, vec);
+    else if (role == 2)
+        hipLaunchKernelGGL((k_bgemv<T, VEC, 2>), dim3(nitems), dim3(256), 0, s, items, terms, A, vec);
+    else hipLaunchKernelGGL((k_bgemv<T, VEC, 0>), dim3(nitems), dim3(256), 0, s, items, terms, A, vec);
+}
+void launch_copy32(const CopyJob *jobs, int njobs, float *pool, hipStream_t s) {
+    if (njobs > 0) hipLaunchKernelGGL(k_copy32, dim3(njobs), dim3(256), 0, s, jobs, pool);
 }
 void launch_bgemv64(const GItem *items, int nitems, const GTerm *terms, const double *A, double *vec, int role,
                     hipStream_t s) {

@@ -177,3 +177,18 @@ def test_single_level_and_errors():
     g.tree_initialisation()
     x = np.random.default_rng(2).normal(size=600)
     np.testing.assert_allclose(g.mat_vec_prod(x), o.mat_vec_prod(x), rtol=1e-10, atol=1e-12)
+
+
+def test_graph_replays_match(built):
+    """The matvec is a captured HIP graph replayed on every call: replays with
+    different inputs, interleaved with other work, each match the oracle."""
+    o, g, N = built(CASES[6])
+    rng = np.random.default_rng(21)
+    xs = [rng.normal(size=N) for _ in range(3)]
+    ref = [o.mat_vec_prod(x) for x in xs]
+    for rep in range(3):
+        for x, r in zip(xs, ref):
+            y = g.mat_vec_prod(x)
+            assert np.linalg.norm(y - r) <= 1e-12 * np.linalg.norm(r)
+            g.mat_vec_prod(x, precision=h2w.FP32)
+        g.dense_rows([0, 1, 2], xs[0])
