@@ -103,6 +103,9 @@ def main():
     ap.add_argument("--tol-pow", type=int, default=10)
     ap.add_argument("--cpu-side", type=int, default=24, help="grid side of the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--pmc-probe", action="store_true",
+                    help="assemble, then run 3 FP64 and 3 FP32-inner matvecs and a 1 GiB device copy "
+                         "(the calibration kernel) for rocprofv3 --pmc; prints what it ran")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); the sharded "
@@ -119,6 +122,18 @@ def main():
     A.tree_initialisation()
     st = A.stats()
     log(f"assembly {time.time() - t0:.1f} s, M = {st['memory']:.4e} elements ({st['memory'] * 8e-9:.2f} GB fp64)")
+    if args.pmc_probe:
+        hip = Hip()
+        A.bench_matvec(3, h2w.FP64)
+        A.bench_matvec(3, h2w.FP32_INNER)
+        nb = 1 << 30
+        src, dst = hip.malloc(nb), hip.malloc(nb)
+        hip.d2d(dst, src, nb)
+        hip.sync()
+        print(json.dumps({"pmc_probe": True, "copy_bytes": nb, "workload": f"3D RBF grid {side}^3 tol 1e-{tp}",
+                          "launch_elems_fp64": A.bench_matvec(1, h2w.FP64)["launch_bytes"],
+                          "launch_elems_fp32_inner": A.bench_matvec(1, h2w.FP32_INNER)["launch_bytes"]}))
+        return
     xt = np.random.default_rng(2).uniform(-1.0, 1.0, N)
     b = A.dense_rows(None, xt)
     log("b = A x_true from the GPU dense FP64 product")

@@ -24,7 +24,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libh2w.so")
 
 H2W_OK, H2W_EINVAL, H2W_ESTATE, H2W_EHIP, H2W_ENOMEM, H2W_ENOCONV, H2W_ENOLIB = range(7)
-NESTED, SEMI_NESTED = 0, 1
+NESTED, SEMI_NESTED, NON_NESTED = 0, 1, 2
 CHOICE_IE, CHOICE_RBF = 0, 1
 HOST, DEVICE = 0, 1
 FP64, FP32, FP32_OTF, FP64_OTF, FP32_INNER = 0, 1, 2, 3, 4

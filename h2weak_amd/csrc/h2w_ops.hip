@@ -570,7 +570,7 @@ void cur_pairs(Handle &h, AcaPool &pool) {
     std::vector<AcaBlock> blocks;
     for (int j = 1; j <= P.nL; ++j)
         for (int b = 0; b < P.nboxes[j]; ++b)
-            for (int v : P.vert_of(j, b)) {
+            for (int v : (h.variant == H2W_NON_NESTED ? P.il[j][b] : P.vert_of(j, b))) {
                 Pair p;
                 p.level = j;
                 p.b = b;
@@ -704,7 +704,7 @@ void assemble(Handle &h) {
     h.aca_reruns = 0;
     {
         AcaPool pool;
-        if (P.dim >= 2) far_nca(h, pool);
+        if (P.dim >= 2 && h.variant != H2W_NON_NESTED) far_nca(h, pool);
         else h.fam[0].assign(P.nL + 1, FamLevel());
         phase("far NCA");
         if (h.variant == H2W_NESTED) vertex_nca(h, pool);
@@ -715,7 +715,7 @@ void assemble(Handle &h) {
         phase("vertex NCA / CUR");
     }
     const int nfam_lo[2] = {2, 1};  // first level of each family
-    const bool fam_on[2] = {P.dim >= 2, h.variant == H2W_NESTED};
+    const bool fam_on[2] = {P.dim >= 2 && h.variant != H2W_NON_NESTED, h.variant == H2W_NESTED};
 
     // ---- vector pool
     int64_t vo = 0;
@@ -1682,7 +1682,9 @@ int h2w_create(int dim, int variant, int64_t N, const double *points, int n_max,
     if (!out) throw Error{H2W_EINVAL, "out is NULL"};
     *out = nullptr;
     if (dim < 1 || dim > 3) throw Error{H2W_EINVAL, "dim must be 1, 2 or 3"};
-    if (variant != H2W_NESTED && variant != H2W_SEMI_NESTED) throw Error{H2W_EINVAL, "unknown variant"};
+    if (variant != H2W_NESTED && variant != H2W_SEMI_NESTED && variant != H2W_NON_NESTED)
+        throw Error{H2W_EINVAL, "unknown variant"};
+    if (variant == H2W_NON_NESTED && dim == 1) throw Error{H2W_EINVAL, "H* is defined for 2D and 3D"};
     if (choice != 0 && choice != 1) throw Error{H2W_EINVAL, "Wrong choice! choice must be 0 (IE) or 1 (RBF)"};
     if (N < 2 || N > INT32_MAX) throw Error{H2W_EINVAL, "N out of range"};
     if (n_max < 1) throw Error{H2W_EINVAL, "n_max must be positive"};

@@ -39,6 +39,9 @@ extern "C" {
 /* hierarchical format (compile-time -DUSE_nHODLRdD / -DUSE_snHODLRdD in the reference) */
 #define H2W_NESTED 0      /* H2*    : nHODLRdD, nested far + nested vertex-sharing blocks */
 #define H2W_SEMI_NESTED 1 /* (H2+H)*: s-nHODLRdD, nested far + per-pair CUR vertex blocks */
+#define H2W_NON_NESTED 2  /* H*     : HODLRdD (-DUSE_HODLRdD), per-pair CUR on every interaction-list
+                                       pair, 2D/3D only (2D/codes/HODLR2Dtree.hpp:971-1073,
+                                       3D/codes/HODLR3Dtree.hpp:2239-2373)                        */
 
 /* kernel choice, as argv[5] of the reference's main */
 #define H2W_CHOICE_IE 0   /* integral equation kernel with its diagonal rule */

@@ -760,7 +760,7 @@ struct Oracle {
             for (int k = 0; k < nBoxes[j]; ++k) {
                 Box &B = tree[j][k];
                 for (size_t i = 0; i < B.il.size(); ++i) {
-                    if (!B.vert[i]) continue;
+                    if (!B.vert[i] && variant != 2) continue;
                     int ki = B.il[i];
                     Box &V = tree[j][ki];
                     AcaResult a;
@@ -808,8 +808,12 @@ struct Oracle {
         if (variant == 0) {
             assemble_far();
             assemble_vertex_nested();
-        } else {
+        } else if (variant == 1) {
             assemble_far();
+            assemble_vertex_semi();
+        } else {
+            // non-nested H* (USE_HODLRdD): CUR on every interaction-list pair, no far NCA
+            // (2D/codes/HODLR2Dtree.hpp:971-980,1007-1021; 3D/codes/HODLR3Dtree.hpp:2239-2248,2282-2296)
             assemble_vertex_semi();
         }
         assemble_near();
@@ -835,6 +839,25 @@ struct Oracle {
                     for (int k = 0; k < nBoxes[j]; ++k)
                         for (auto &kv : tree[j][k].U)
                             memory += (double)kv.second.size() + (double)tree[j][k].Vt[kv.first].size();
+            }
+            return;
+        }
+        if (variant == 2) {
+            // findMemory_nn (3D/codes/HODLR3Dtree.hpp:2561-2583): L + R + Ac + Ar of every pair, near field
+            for (int j = 1; j <= nL; ++j)
+                for (int k = 0; k < nBoxes[j]; ++k) {
+                    Box &B = tree[j][k];
+                    for (size_t i = 0; i < B.il.size(); ++i) {
+                        int ki = B.il[i];
+                        memory += (double)B.Lc[(int)i].size() + (double)B.Rc[(int)i].size();
+                        if (B.Ac.count(ki)) memory += (double)B.Ac[ki].size() + (double)B.Ar[ki].size();
+                    }
+                }
+            for (int k = 0; k < nBoxes[nL]; ++k) {
+                Box &B = tree[nL][k];
+                memory += (double)B.pts.size() * B.pts.size();
+                for (int n = 0; n < nslot; ++n)
+                    if (B.nbr[n] != -1) memory += (double)B.pts.size() * tree[nL][B.nbr[n]].pts.size();
             }
             return;
         }
@@ -883,7 +906,7 @@ struct Oracle {
                 B.potential_ver.clear();
             }
         // far up / M2L / down (3D/codes/HODLR3Dtree.hpp:1941-2055)
-        if (dim > 1) {
+        if (dim > 1 && variant != 2) {
             for (int j = nL; j >= 2; --j) {
 #pragma omp parallel for
                 for (int k = 0; k < nBoxes[j]; ++k) {
@@ -979,7 +1002,7 @@ struct Oracle {
                     Box &B = tree[j][k];
                     B.potential_ver.assign(B.pts.size(), 0.0);
                     for (size_t i = 0; i < B.il.size(); ++i) {
-                        if (!B.vert[i]) continue;
+                        if (!B.vert[i] && variant != 2) continue;
                         int ki = B.il[i];
                         Box &V = tree[j][ki];
                         if (dim == 1) {
@@ -1154,7 +1177,8 @@ extern "C" {
 
 void *orc_create(int dim, int variant, int64_t N, const double *pts, int n_max, double L, int tol_pow, int choice,
                  int nlevels, int compat) {
-    if (dim < 1 || dim > 3 || (choice != 0 && choice != 1) || N < 2) return nullptr;
+    if (dim < 1 || dim > 3 || (choice != 0 && choice != 1) || N < 2 || variant < 0 || variant > 2) return nullptr;
+    if (variant == 2 && dim == 1) return nullptr;  // the reference has no 1D H* build
     Oracle *o = new Oracle();
     o->dim = dim;
     o->variant = variant;

@@ -1,4 +1,5 @@
-"""Re-run the oracle at the reference's eight published transcript configs.
+"""Re-run the oracle at the reference's eight published transcript configs
+(H2*, (H2+H)* and non-nested H* in 1D/2D/3D as published).
 
 The reference publishes stdout transcripts only (1D/results/*.txt,
 2D/results/*.txt, 3D/results/*.txt; README.md:53-118).  This script runs the
@@ -34,6 +35,12 @@ PUBLISHED = {
     "2d_semi": dict(dim=2, variant=1, N=90000, n_max=400, tol_pow=12, choice=1, gtol=1e-12, max_iter=500,
                     iters=28, resid=2.94831e-13, err=1.50418e-12, storage_gb=3.03009,
                     src="2D/results/sn_hodlr.txt:6-7,25,29"),
+    "2d_hstar": dict(dim=2, variant=2, N=90000, n_max=400, tol_pow=12, choice=1, gtol=1e-12, max_iter=500,
+                     iters=28, resid=2.99153e-13, err=7.38439e-13, storage_gb=4.17922,
+                     src="2D/results/hodlr.txt:6-7,25,29"),
+    "3d_hstar": dict(dim=3, variant=2, N=42875, n_max=512, tol_pow=7, choice=0, gtol=1e-10, max_iter=1000,
+                     iters=5, resid=2.9799e-12, err=9.34489e-11, storage_gb=4.39326,
+                     src="3D/results/hodlr.txt:6-7,25,29"),
     "3d_nested": dict(dim=3, variant=0, N=42875, n_max=512, tol_pow=7, choice=0, gtol=1e-10, max_iter=1000,
                       iters=5, resid=2.9799e-12, err=3.60016e-10, storage_gb=2.87549,
                       src="3D/results/nhodlr.txt:6-7,25,29"),
