@@ -68,6 +68,27 @@ class Hip:
     def free(self, p):
         self.lib.hipFree(p)
 
+    def copy_gbps(self, nbytes=4 << 30, reps=5):
+        """HBM bandwidth of a plain device-to-device copy (read + write bytes / time), the
+        achievable ceiling a streaming kernel is compared with."""
+        src, dst = self.malloc(nbytes), self.malloc(nbytes)
+        e0, e1 = ctypes.c_void_p(), ctypes.c_void_p()
+        assert self.lib.hipEventCreate(ctypes.byref(e0)) == 0 and self.lib.hipEventCreate(ctypes.byref(e1)) == 0
+        n = ctypes.c_size_t(nbytes)
+        assert self.lib.hipMemcpyAsync(dst, src, n, 3, None) == 0
+        assert self.lib.hipEventRecord(e0, None) == 0
+        for _ in range(reps):
+            assert self.lib.hipMemcpyAsync(dst, src, n, 3, None) == 0
+        assert self.lib.hipEventRecord(e1, None) == 0
+        assert self.lib.hipEventSynchronize(e1) == 0
+        ms = ctypes.c_float()
+        assert self.lib.hipEventElapsedTime(ctypes.byref(ms), e0, e1) == 0
+        self.lib.hipEventDestroy(e0)
+        self.lib.hipEventDestroy(e1)
+        self.free(src)
+        self.free(dst)
+        return 2.0 * nbytes * reps / (ms.value * 1e-3) * 1e-9
+
 
 def cpu_baseline(n_side, gmres_matvecs_full, memory_full, tol_pow, gtol):
     """Oracle (port of the reference's CPU algorithm, OpenMP on all host cores) on a
@@ -94,6 +115,39 @@ def cpu_baseline(n_side, gmres_matvecs_full, memory_full, tol_pow, gtol):
             "sample_gmres_s": t_gm, "sample_assembly_s": t_asm, "sample_iterations": it}
 
 
+def pmc_reduce(d, side, tp):
+    """HBM bytes per dispatch of the M2L + near-field launches from FETCH_SIZE / WRITE_SIZE (KiB per
+    dispatch), each scaled so that the 1 GiB device copy of the probe reads and writes exactly 1 GiB."""
+    import csv
+    import re
+    per = {}
+    for tag in ("fetch", "write"):
+        vals = {}
+        for r in csv.DictReader(open(os.path.join(d, f"{tag}_counter_collection.csv"))):
+            k = int(r["Dispatch_Id"])
+            name, v = vals.get(k, (r["Kernel_Name"], 0.0))
+            vals[k] = (name, v + float(r["Counter_Value"]) * 1024.0)
+        per[tag] = vals
+    copy = [k for k, (n, v) in per["write"].items() if "copyBuffer" in n]
+    kc = max(copy, key=lambda k: per["write"][k][1])
+    nb = float(1 << 30)
+    f_fetch, f_write = nb / per["fetch"][kc][1], nb / per["write"][kc][1]
+    out = {"calibration": {"copy_bytes": int(nb), "fetch_factor": round(f_fetch, 5), "write_factor": round(f_write, 5),
+                           "note": "bytes = counter KiB * 1024 * factor; factors from the 1 GiB device copy"}}
+    for prec, key in (("double", "fp64_m2l_near"), ("float", "fp32inner_m2l_near")):
+        pat = re.compile(r"k_bgemv<%s, \d+, 1[,>]" % prec)
+        ks = sorted(k for k, (n, v) in per["fetch"].items() if pat.search(n) and k in per["write"])
+        rd = [per["fetch"][k][1] * f_fetch for k in ks]
+        wr = [per["write"][k][1] * f_write for k in ks]
+        tot = sorted(a + b for a, b in zip(rd, wr))
+        out[f"3d_rbf_{side}_{tp}_{key}"] = {"dispatches": len(ks), "fetch_bytes": int(sorted(rd)[len(rd) // 2]),
+                                           "write_bytes": int(sorted(wr)[len(wr) // 2]),
+                                           "hbm_bytes_per_launch": int(tot[len(tot) // 2])}
+    path = os.path.join(ROOT, "profiles", "pmc_traffic.json")
+    json.dump(out, open(path, "w"), indent=1)
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -103,10 +157,17 @@ def main():
     ap.add_argument("--tol-pow", type=int, default=10)
     ap.add_argument("--cpu-side", type=int, default=24, help="grid side of the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (default launch shapes, inner 1e-6)")
     ap.add_argument("--pmc-probe", action="store_true",
                     help="assemble, then run 3 FP64 and 3 FP32-inner matvecs and a 1 GiB device copy "
                          "(the calibration kernel) for rocprofv3 --pmc; prints what it ran")
+    ap.add_argument("--pmc-reduce", metavar="DIR",
+                    help="read DIR/fetch_counter_collection.csv and DIR/write_counter_collection.csv written by "
+                         "two rocprofv3 --pmc runs of --pmc-probe and write profiles/pmc_traffic.json (no GPU)")
     args = ap.parse_args()
+    if args.pmc_reduce:
+        pmc_reduce(args.pmc_reduce, args.side, args.tol_pow)
+        return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); the sharded "
               "design is described in DESIGN.md and not launched", file=sys.stderr)
@@ -137,6 +198,15 @@ def main():
     xt = np.random.default_rng(2).uniform(-1.0, 1.0, N)
     b = A.dense_rows(None, xt)
     log("b = A x_true from the GPU dense FP64 product")
+    tune = None
+    if not args.no_tune:
+        t0 = time.time()
+        tune = A.autotune(b, tol=gtol, max_iter=500)
+        log(f"autotune {time.time() - t0:.1f} s: inner tolerance 1e-{tune['ir_tol_pow']}; "
+            + ", ".join(f"1e-{c['ir_tol_pow']}: {c['seconds'] * 1e3:.1f} ms ({c['iterations']}+{c['outer']})"
+                        for c in tune["candidates"])
+            + f"; matvec ms FP64/FP32/inner {['%.3f' % v for v in tune['matvec_ms_before']]} -> "
+            f"{['%.3f' % v for v in tune['matvec_ms_after']]}")
 
     hip = Hip()
     db, dx = hip.malloc(8 * N), hip.malloc(8 * N)
@@ -222,6 +292,11 @@ def main():
         if key in tr:
             roof_in["traffic"] = tr[key]["hbm_bytes_per_launch"]
     roof, roof2 = (roof_in, roof_st) if t_in >= t_st else (roof_st, roof_in)
+    copy_bw = hip.copy_gbps()
+    log(f"device-to-device copy: {copy_bw:.0f} GB/s (read + write)")
+    for r in (roof_in, roof_st):
+        r["copy_GBps"] = round(copy_bw, 1)
+        r["frac_of_copy"] = round(r["achieved"] / copy_bw, 4)
 
     out = {"metric": METRIC, "value": round(t_step, 6), "unit": "s", "n_gpus": 1, "steps": args.steps,
            "warmup": args.warmup, "ms_per_step": round(t_step * 1e3, 3), "higher_is_better": False,
@@ -237,6 +312,7 @@ def main():
            "matvec": {k: {kk: (round(vv, 4) if isinstance(vv, float) else vv) for kk, vv in v.items()}
                       for k, v in mv.items()},
            "assembly_s": round(st["assembly_seconds"], 2), "memory_elements": M,
+           "autotune": tune,
            "roofline": roof, "roofline_other": roof2, "roofline_otf": roof_otf,
            "inner_operator_bytes_fp32": st["stored_bytes_inner_fp32"], "cpu_baseline": None}
     if not args.no_cpu:

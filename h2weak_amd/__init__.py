@@ -34,7 +34,9 @@ GMRES_FP64, GMRES_FP32_IR, GMRES_FP32_OTF_IR, GMRES_FP32_INNER_IR = 0, 1, 2, 3
 # every symbol include/h2w.h declares
 EXPORTS = ("h2w_default_opts", "h2w_create", "h2w_assemble", "h2w_matvec", "h2w_matvec_ex", "h2w_gmres",
            "h2w_dense_matvec", "h2w_get_stats", "h2w_tree_order", "h2w_box_skeleton", "h2w_interaction_list",
-           "h2w_near_list", "h2w_destroy", "h2w_last_error", "h2w_bench_matvec")
+           "h2w_near_list", "h2w_destroy", "h2w_last_error", "h2w_bench_matvec", "h2w_set_inner_tolerance",
+           "h2w_autotune")
+TUNE_UNROLL, TUNE_INNER = 1, 2
 
 
 class H2WError(RuntimeError):
@@ -76,6 +78,14 @@ class BenchResult(ctypes.Structure):
                 ("n_launch", ctypes.c_int), ("launch_bytes", ctypes.c_int64 * 64), ("launch_role", ctypes.c_int * 64)]
 
 
+class Tune(ctypes.Structure):
+    _fields_ = [("ir_tol_pow", ctypes.c_int), ("n_cand", ctypes.c_int), ("cand_ir_tol_pow", ctypes.c_int * 8),
+                ("cand_seconds", ctypes.c_double * 8), ("cand_iterations", ctypes.c_int * 8),
+                ("cand_outer", ctypes.c_int * 8), ("cand_converged", ctypes.c_int * 8),
+                ("cand_inner_bytes", ctypes.c_int64 * 8), ("matvec_ms_before", ctypes.c_double * 3),
+                ("matvec_ms_after", ctypes.c_double * 3)]
+
+
 _lib = None
 
 
@@ -103,6 +113,8 @@ def lib():
     L.h2w_destroy.argtypes = [p]
     L.h2w_last_error.restype = ctypes.c_char_p
     L.h2w_bench_matvec.argtypes = [p, i, i, ctypes.POINTER(BenchResult)]
+    L.h2w_set_inner_tolerance.argtypes = [p, i]
+    L.h2w_autotune.argtypes = [p, p, i, d, i, i, ctypes.POINTER(Tune)]
     _lib = L
     return L
 
@@ -236,6 +248,29 @@ class Initialisation:
         n = r.n_launch
         return {"ms_per_matvec": r.ms_per_matvec, "launch_ms": list(r.launch_ms[:n]),
                 "launch_bytes": list(r.launch_bytes[:n]), "launch_role": list(r.launch_role[:n])}
+
+
+    def set_inner_tolerance(self, ir_tol_pow):
+        """Rebuild the truncated FP32 inner operator for tolerance 10^-ir_tol_pow (0 removes it)."""
+        _check(lib().h2w_set_inner_tolerance(self._h, int(ir_tol_pow)))
+
+    def autotune(self, b=None, tol=1e-10, max_iter=1000, flags=TUNE_UNROLL | TUNE_INNER):
+        """GEMV unroll depth per launch and inner tolerance of the refinement (b in input order)."""
+        t = Tune()
+        bb = None
+        if b is not None:
+            bb = np.ascontiguousarray(b, dtype=np.float64)
+            if bb.shape != (self.N,):
+                raise ValueError("b must have N entries")
+        _check(lib().h2w_autotune(self._h, _ptr(bb) if bb is not None else None, HOST, float(tol),
+                                  int(max_iter), int(flags), ctypes.byref(t)))
+        n = t.n_cand
+        return {"ir_tol_pow": t.ir_tol_pow,
+                "candidates": [{"ir_tol_pow": t.cand_ir_tol_pow[q], "seconds": t.cand_seconds[q],
+                                "iterations": t.cand_iterations[q], "outer": t.cand_outer[q],
+                                "converged": bool(t.cand_converged[q]), "inner_bytes": t.cand_inner_bytes[q]}
+                               for q in range(n)],
+                "matvec_ms_before": list(t.matvec_ms_before), "matvec_ms_after": list(t.matvec_ms_after)}
 
 
 class IterativeSolver:

@@ -127,7 +127,7 @@ struct GItem {
     int32_t lda;
     int32_t row0;  // first row of this tile inside the terms' matrices
     int32_t t0, t1;
-    int32_t flags; // 0 = exclusive add, 1 = atomic add
+    int32_t flags; // bit 0: atomic add (else exclusive); bit 1: operator in the second pool
 };
 
 constexpr int kRowTile = 128;  // rows of one GEMV item
@@ -143,11 +143,12 @@ void launch_xfer(const XferJob *jobs, int njobs, const int32_t *idx, const doubl
                  hipStream_t s);
 void launch_cur(const CurJob *jobs, int njobs, const int32_t *idx, const double *lr, double *A, KParams kp,
                 hipStream_t s);
-void launch_copy32(const CopyJob *jobs, int njobs, float *pool, hipStream_t s);
-void launch_bgemv64(const GItem *items, int nitems, const GTerm *terms, const double *A, double *vec, int role,
-                    hipStream_t s);
-void launch_bgemv32(const GItem *items, int nitems, const GTerm *terms, const float *A, float *vec, int role,
-                    hipStream_t s);
+void launch_copy32(const CopyJob *jobs, int njobs, const float *src, float *dst, hipStream_t s);
+// operator pool A0, or A1 for items whose flags have bit 1 set; unr in {2, 4, 8}
+void launch_bgemv64(const GItem *items, int nitems, const GTerm *terms, const double *A0, const double *A1,
+                    double *vec, int role, int unr, hipStream_t s);
+void launch_bgemv32(const GItem *items, int nitems, const GTerm *terms, const float *A0, const float *A1, float *vec,
+                    int role, int unr, hipStream_t s);
 void launch_otf32(const GItem *items, int nitems, const GTerm *terms, SlotPoints sp, float *vec, hipStream_t s);
 void launch_otf64(const GItem *items, int nitems, const GTerm *terms, SlotPoints sp, double *vec, hipStream_t s);
 void launch_to_fp32(const double *a, float *b, int64_t n, hipStream_t s);
@@ -171,6 +172,7 @@ void launch_mdot(const double *V, int64_t ldv, int k, const double *w, int64_t n
 void launch_mdot_reduce(const double *partial, int nblk, int k, double *out, hipStream_t s);
 void launch_mgemv_sub(const double *V, int64_t ldv, int k, const double *h, double *w, int64_t n, hipStream_t s);
 void launch_axpby(double a, const double *x, double b, double *y, int64_t n, hipStream_t s);
+void launch_scal(double a, double *y, int64_t n, hipStream_t s);  // y *= a
 void launch_vec_update(const double *V, int64_t ldv, int k, const double *c, double *x, int64_t n, hipStream_t s);
 
 int mdot_blocks(int64_t n);

@@ -107,7 +107,9 @@ struct Pair {                                  // semi-nested vertex pair (B <- 
 
 struct Launch {
     int64_t item0, nitems, term0, nterms;
-    int role;  // 0 transfer pass, 1 M2L + near field, 2 semi-nested column factors
+    int role;      // 0 transfer pass, 1 M2L + near field, 2 semi-nested column factors
+    int unr = 4;   // columns in flight per wave (autotuned), FP64 / inner FP32 lists
+    int unr32 = 4; // the same for the stored FP32 copy
 };
 
 struct WorkList {
@@ -127,6 +129,15 @@ struct LaunchSet {
 struct TermSpec {
     int64_t a, x;
     int32_t cols;
+};
+
+struct M2LBlk {  // one stored M2L block K(Ir_B, Ir_F): family, level, box, partner, pool offset
+    int f, j, b, p;
+    int64_t off;
+};
+struct NearBlk {  // one stored near-field block K(P(B), P(N))
+    int b, n;
+    int64_t off;
 };
 
 }  // namespace
@@ -163,6 +174,9 @@ struct h2w_handle {
     std::vector<h2w::Pair> pairs;
     std::vector<h2w::Launch> launches;
     std::vector<int64_t> launch_elems;  // operator elements each launch streams
+    std::vector<h2w::M2LBlk> m2l;
+    std::vector<h2w::NearBlk> nearb;
+    h2w::DevBuf<float> mat32in;         // the inner operator's transfers and compact M2L blocks
     h2w::LaunchSet inner;               // the refinement's inner operator (ranks truncated)
     hipGraphExec_t graph[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};  // one per precision
     bool use_graph = true;
@@ -599,7 +613,7 @@ void cur_pairs(Handle &h, AcaPool &pool) {
 constexpr int64_t kItemWork = 128 * 4096;  // elements of A per GEMV item before column chunking
 
 void add_item(WorkList &W, int64_t y, int rows, const std::vector<TermSpec> &terms, bool atomic, int lda = -1,
-              int row_base = 0) {
+              int row_base = 0, int pool = 0) {
     if (rows <= 0 || terms.empty()) return;
     if (lda < 0) lda = rows;
     int64_t C = 0;
@@ -617,7 +631,7 @@ void add_item(WorkList &W, int64_t y, int rows, const std::vector<TermSpec> &ter
             for (auto &t : terms)
                 if (t.cols > 0) W.terms.push_back(GTerm{t.a, t.x, t.cols, 0});
             it.t1 = (int32_t)W.terms.size();
-            it.flags = atomic ? 1 : 0;
+            it.flags = (atomic ? 1 : 0) | (pool << 1);
             W.items.push_back(it);
         } else {
             // column chunks accumulate atomically
@@ -627,7 +641,7 @@ void add_item(WorkList &W, int64_t y, int rows, const std::vector<TermSpec> &ter
             it.lda = lda;
             it.row0 = row_base + r0;
             it.y = y + r0;
-            it.flags = 1;
+            it.flags = 1 | (pool << 1);
             int64_t used = 0;
             it.t0 = (int32_t)W.terms.size();
             for (auto &t : terms) {
@@ -684,6 +698,297 @@ void upload_lists(std::vector<WorkList> &WL, LaunchSet &LS, hipStream_t s) {
     }
     LS.d_items.upload(items, s);
     LS.d_terms.upload(terms, s);
+}
+
+// ------------------------------------------------------------------ the refinement's inner operator
+// Every box keeps the leading pivots at which its cross would have stopped for
+// tolerance 10^-irp (from the ACA history); its M2L blocks are compact copies of
+// the leading sub-blocks of the stored FP32 M2L blocks, its transfers are
+// recomputed from the leading blocks of L and R.  Lives in its own FP32 buffer
+// (work items flagged with pool 1); rebuilt by h2w_set_inner_tolerance and the
+// autotuner.
+void drop_graph(Handle &h, int precision);
+
+void build_inner(Handle &h, int irp) {
+    drop_graph(h, H2W_FP32_INNER);
+    const Plan &P = h.plan;
+    hipStream_t s = h.stream;
+    const int nfam_lo[2] = {2, 1};
+    const bool fam_on[2] = {P.dim >= 2 && h.variant != H2W_NON_NESTED, h.variant == H2W_NESTED};
+    const int nleaf = P.nboxes[P.nL];
+    auto &m2l = h.m2l;
+    auto &nearb = h.nearb;
+    h.inner.launches.clear();
+    h.inner_elems = 0;
+    h.mat32in.release();
+    // ---- inner operator of the refinement: each box keeps the leading pivots at which its
+    // cross would have stopped for tolerance 10^-ir_tol_pow; its M2L blocks are the leading
+    // sub-blocks of the stored ones, its transfers are recomputed from the leading L, R blocks
+    const bool inner_on = h.opts.fp32_shadow && irp > 0 && irp < h.tol_pow;
+    int64_t in_len = 0;
+    auto take_in = [&](int64_t elems) {
+        int64_t o = in_len;
+        in_len += align_up(elems, 32);
+        return o;
+    };
+    // compact copies of the inner M2L sub-blocks: (index into m2l, offset in the inner region)
+    std::vector<std::pair<size_t, int64_t>> m2l_in;
+    if (inner_on) {
+        const double tau = std::pow(10.0, -1.0 * irp);
+        for (int f = 0; f < 2; ++f) {
+            if (!fam_on[f]) continue;
+            for (int j = nfam_lo[f]; j <= P.nL; ++j) {
+                FamLevel &fl = h.fam[f][j];
+                fl.kin.assign(fl.nb, 0);
+                for (int b = 0; b < fl.nb; ++b)
+                    fl.kin[b] = fl.rank[b] > 0 ? inner_rank(fl.hist[b], fl.rank[b], tau) : 0;
+            }
+            // inner transfers: rows are the children's leading pivots, each child's segment
+            // padded to a multiple of 4 rows so every row tile stays aligned
+            for (int j = nfam_lo[f]; j <= P.nL; ++j) {
+                FamLevel &fl = h.fam[f][j];
+                fl.ein_off.assign(fl.nb, -1);
+                fl.eint_off.assign(fl.nb, -1);
+                for (int b = 0; b < fl.nb; ++b) {
+                    int64_t m = 0;
+                    if (j == P.nL) m = P.bcount[j][b];
+                    else
+                        for (int c = 0; c < P.nchild; ++c) m += pad_ld(h.fam[f][j + 1].kin[b * P.nchild + c]);
+                    if (m * fl.kin[b] == 0) continue;
+                    fl.ein_off[b] = take_in(pad_ld(m) * fl.kin[b]);
+                    fl.eint_off[b] = take_in(pad_ld(fl.kin[b]) * m);
+                }
+            }
+        }
+        for (size_t q = 0; q < m2l.size(); ++q) {
+            const M2LBlk &m = m2l[q];
+            const FamLevel &fl = h.fam[m.f][m.j];
+            if (fl.kin[m.b] > 0 && fl.kin[m.p] > 0)
+                m2l_in.push_back({q, take_in(pad_ld(fl.kin[m.b]) * fl.kin[m.p])});
+        }
+    }
+    if (inner_on) h.mat32in.alloc(std::max<int64_t>(in_len, 32));
+    if (inner_on && in_len > 0) H2W_HIP(hipMemsetAsync(h.mat32in.p, 0, in_len * 4, s));
+    if (inner_on && in_len > 0) {
+        std::vector<int32_t> idx;
+        std::vector<XferJob> xj;
+        std::vector<double> lr;
+        for (int f = 0; f < 2; ++f) {
+            if (!fam_on[f]) continue;
+            for (int j = nfam_lo[f]; j <= P.nL; ++j) {
+                FamLevel &fl = h.fam[f][j];
+                for (int b = 0; b < fl.nb; ++b) {
+                    if (fl.ein_off[b] < 0) continue;
+                    const int k = fl.kin[b], r = fl.rank[b];
+                    // leading k x k blocks of L and R
+                    const int64_t lro = (int64_t)lr.size();
+                    for (int part = 0; part < 2; ++part)
+                        for (int cc = 0; cc < k; ++cc)
+                            for (int rr = 0; rr < k; ++rr)
+                                lr.push_back(fl.lr[b][(size_t)part * r * r + (size_t)cc * r + rr]);
+                    const int64_t icq = (int64_t)idx.size();
+                    idx.insert(idx.end(), fl.Ic[b].begin(), fl.Ic[b].begin() + k);
+                    int64_t m = 0;
+                    if (j == P.nL) m = P.bcount[j][b];
+                    else
+                        for (int c = 0; c < P.nchild; ++c) m += pad_ld(h.fam[f][j + 1].kin[b * P.nchild + c]);
+                    const int lde = (int)pad_ld(m), ldet = (int)pad_ld(k);
+                    auto add_job = [&](const int32_t *S, int ms, int64_t row) {
+                        if (ms == 0) return;
+                        XferJob J;
+                        J.r = k;
+                        J.m = ms;
+                        J.s = (int64_t)idx.size();
+                        idx.insert(idx.end(), S, S + ms);
+                        J.ic = icq;
+                        J.lr = lro;
+                        J.e = fl.ein_off[b] + row;
+                        J.et = fl.eint_off[b] + row * ldet;
+                        J.lde = lde;
+                        J.ldet = ldet;
+                        xj.push_back(J);
+                    };
+                    if (j == P.nL) {
+                        auto rl = range_list(P.bstart[j][b], P.bcount[j][b]);
+                        add_job(rl.data(), (int)rl.size(), 0);
+                    } else {
+                        int64_t row = 0;
+                        for (int c = 0; c < P.nchild; ++c) {
+                            const int cc = b * P.nchild + c;
+                            const int kc = h.fam[f][j + 1].kin[cc];
+                            add_job(h.fam[f][j + 1].Ir[cc].data(), kc, row);
+                            row += pad_ld(kc);
+                        }
+                    }
+                }
+            }
+        }
+        DevBuf<int32_t> didx;
+        DevBuf<double> dlr, tmp;
+        DevBuf<XferJob> dxj;
+        didx.upload(idx, s);
+        dlr.upload(lr, s);
+        dxj.upload(xj, s);
+        tmp.alloc(in_len);
+        H2W_HIP(hipMemsetAsync(tmp.p, 0, in_len * 8, s));
+        launch_xfer(dxj.p, (int)xj.size(), didx.p, dlr.p, tmp.p, h.kp, s);
+        H2W_HIP(hipGetLastError());
+        launch_to_fp32(tmp.p, h.mat32in.p, in_len, s);
+        H2W_HIP(hipGetLastError());
+        // compact copies of the leading M2L sub-blocks
+        std::vector<CopyJob> cj;
+        for (auto &pr : m2l_in) {
+            const M2LBlk &m = m2l[pr.first];
+            const FamLevel &fl = h.fam[m.f][m.j];
+            CopyJob J;
+            J.src = m.off;
+            J.dst = pr.second;
+            J.m = fl.kin[m.b];
+            J.n = fl.kin[m.p];
+            J.lds = (int32_t)pad_ld(fl.rank[m.b]);
+            J.ldd = (int32_t)pad_ld(fl.kin[m.b]);
+            cj.push_back(J);
+        }
+        DevBuf<CopyJob> dcj;
+        dcj.upload(cj, s);
+        launch_copy32(dcj.p, (int)cj.size(), h.mat32.p, h.mat32in.p, s);
+        H2W_HIP(hipGetLastError());
+        H2W_HIP(hipStreamSynchronize(s));
+    }
+
+    if (inner_on) {
+        const int64_t base = 0;  // offsets into the inner buffer (items flagged with pool 1)
+        std::vector<WorkList> IL;
+        {
+            WorkList W;
+            for (int f = 0; f < 2; ++f) {
+                if (!fam_on[f] || P.nL < nfam_lo[f]) continue;
+                FamLevel &fl = h.fam[f][P.nL];
+                for (int b = 0; b < fl.nb; ++b)
+                    if (fl.eint_off[b] >= 0)
+                        add_item(W, fl.o_off[b], fl.kin[b],
+                                 {TermSpec{base + fl.eint_off[b], h.off_x + P.bstart[P.nL][b], (int32_t)P.bcount[P.nL][b]}},
+                                 false, (int)pad_ld(fl.kin[b]), 0, 1);
+            }
+            for (auto &p : h.pairs)
+                if (p.rank > 0)
+                    add_item(W, p.t_off, p.rank,
+                             {TermSpec{p.ar_off, h.off_x + P.bstart[p.level][p.v], (int32_t)P.bcount[p.level][p.v]}},
+                             false, (int)pad_ld(p.rank));
+            IL.push_back(std::move(W));
+        }
+        for (int j = P.nL - 1; j >= 1; --j) {
+            WorkList W;
+            for (int f = 0; f < 2; ++f) {
+                if (!fam_on[f] || j < nfam_lo[f]) continue;
+                FamLevel &fl = h.fam[f][j];
+                FamLevel &ch = h.fam[f][j + 1];
+                for (int b = 0; b < fl.nb; ++b) {
+                    if (fl.eint_off[b] < 0) continue;
+                    const int64_t ldet = pad_ld(fl.kin[b]);
+                    std::vector<TermSpec> t;
+                    int64_t col = 0;
+                    for (int c = 0; c < P.nchild; ++c) {
+                        const int cc = b * P.nchild + c;
+                        if (ch.kin[cc] > 0)
+                            t.push_back(TermSpec{base + fl.eint_off[b] + col * ldet, ch.o_off[cc], (int32_t)ch.kin[cc]});
+                        col += pad_ld(ch.kin[cc]);
+                    }
+                    add_item(W, fl.o_off[b], fl.kin[b], t, false, (int)ldet, 0, 1);
+                }
+            }
+            IL.push_back(std::move(W));
+        }
+        {
+            WorkList W;
+            W.role = 1;
+            std::vector<std::vector<std::vector<std::vector<TermSpec>>>> per(2);
+            for (int f = 0; f < 2; ++f) {
+                if (!fam_on[f]) continue;
+                per[f].assign(P.nL + 1, {});
+                for (int j = nfam_lo[f]; j <= P.nL; ++j) per[f][j].assign(h.fam[f][j].nb, {});
+            }
+            for (auto &pr : m2l_in) {
+                const M2LBlk &m = m2l[pr.first];
+                const FamLevel &fl = h.fam[m.f][m.j];
+                per[m.f][m.j][m.b].push_back(TermSpec{base + pr.second, fl.o_off[m.p], (int32_t)fl.kin[m.p]});
+            }
+            for (int f = 0; f < 2; ++f) {
+                if (!fam_on[f]) continue;
+                for (int j = nfam_lo[f]; j <= P.nL; ++j) {
+                    FamLevel &fl = h.fam[f][j];
+                    for (int b = 0; b < fl.nb; ++b)
+                        if (fl.kin[b] > 0)
+                            add_item(W, fl.u_off[b], fl.kin[b], per[f][j][b], false, (int)pad_ld(fl.kin[b]), 0, 1);
+                }
+            }
+            std::vector<std::vector<TermSpec>> pn(nleaf);
+            for (auto &nb : nearb)
+                pn[nb.b].push_back(TermSpec{nb.off, h.off_x + P.bstart[P.nL][nb.n], (int32_t)P.bcount[P.nL][nb.n]});
+            for (int b = 0; b < nleaf; ++b)
+                add_item(W, h.off_y + P.bstart[P.nL][b], (int)P.bcount[P.nL][b], pn[b], false,
+                         (int)pad_ld(P.bcount[P.nL][b]));
+            IL.push_back(std::move(W));
+        }
+        for (int j = 1; j <= P.nL - 1; ++j) {
+            WorkList W;
+            for (int f = 0; f < 2; ++f) {
+                if (!fam_on[f] || j < nfam_lo[f]) continue;
+                FamLevel &fl = h.fam[f][j];
+                FamLevel &ch = h.fam[f][j + 1];
+                for (int b = 0; b < fl.nb; ++b) {
+                    if (fl.ein_off[b] < 0) continue;
+                    int64_t m = 0;
+                    for (int c = 0; c < P.nchild; ++c) m += pad_ld(ch.kin[b * P.nchild + c]);
+                    int64_t row = 0;
+                    for (int c = 0; c < P.nchild; ++c) {
+                        const int cc = b * P.nchild + c;
+                        if (ch.kin[cc] > 0)
+                            add_item(W, ch.u_off[cc], ch.kin[cc],
+                                     {TermSpec{base + fl.ein_off[b], fl.u_off[b], (int32_t)fl.kin[b]}}, false,
+                                     (int)pad_ld(m), (int)row, 1);
+                        row += pad_ld(ch.kin[cc]);
+                    }
+                }
+            }
+            IL.push_back(std::move(W));
+        }
+        {
+            WorkList W;
+            for (int b = 0; b < nleaf; ++b) {
+                std::vector<TermSpec> t;
+                for (int f = 0; f < 2; ++f) {
+                    if (!fam_on[f] || P.nL < nfam_lo[f]) continue;
+                    FamLevel &fl = h.fam[f][P.nL];
+                    if (fl.ein_off[b] >= 0) t.push_back(TermSpec{base + fl.ein_off[b], fl.u_off[b], (int32_t)fl.kin[b]});
+                }
+                add_item(W, h.off_y + P.bstart[P.nL][b], (int)P.bcount[P.nL][b], t, false,
+                         (int)pad_ld(P.bcount[P.nL][b]), 0, 1);
+            }
+            IL.push_back(std::move(W));
+        }
+        if (!h.pairs.empty()) {
+            WorkList W;
+            W.role = 2;
+            std::vector<std::vector<std::vector<TermSpec>>> per(P.nL + 1);
+            for (int j = 1; j <= P.nL; ++j) per[j].assign(P.nboxes[j], {});
+            for (auto &p : h.pairs)
+                if (p.rank > 0) per[p.level][p.b].push_back(TermSpec{p.ac_off, p.t_off, (int32_t)p.rank});
+            for (int j = 1; j <= P.nL; ++j)
+                for (int b = 0; b < P.nboxes[j]; ++b)
+                    add_item(W, h.off_y + P.bstart[j][b], (int)P.bcount[j][b], per[j][b], true,
+                             (int)pad_ld(P.bcount[j][b]));
+            IL.push_back(std::move(W));
+        }
+        upload_lists(IL, h.inner, s);
+        h.inner_elems = 0;
+        for (auto e : h.inner.elems) h.inner_elems += e;
+    } else {
+        h.inner.launches.clear();
+        h.inner_elems = 0;
+    }
+    H2W_HIP(hipStreamSynchronize(s));
+    h.opts.ir_tol_pow = inner_on ? irp : 0;
 }
 
 // ------------------------------------------------------------------ assembly
@@ -771,11 +1076,8 @@ void assemble(Handle &h) {
             }
         }
     }
-    struct M2LBlk {
-        int f, j, b, p;
-        int64_t off;
-    };
-    std::vector<M2LBlk> m2l;
+    auto &m2l = h.m2l;
+    m2l.clear();
     for (int f = 0; f < 2; ++f) {
         if (!fam_on[f]) continue;
         for (int j = nfam_lo[f]; j <= P.nL; ++j) {
@@ -791,11 +1093,8 @@ void assemble(Handle &h) {
             }
         }
     }
-    struct NearBlk {
-        int b, n;
-        int64_t off;
-    };
-    std::vector<NearBlk> nearb;
+    auto &nearb = h.nearb;
+    nearb.clear();
     const int nleaf = P.nboxes[P.nL];
     const bool count_near = !(P.dim == 1 && h.variant == H2W_SEMI_NESTED);  // 1D findMemory_sn skips it
     for (int b = 0; b < nleaf; ++b) {
@@ -985,147 +1284,13 @@ void assemble(Handle &h) {
         flush();
     }
     phase("M2L / near / CUR blocks");
-    // ---- inner operator of the refinement: each box keeps the leading pivots at which its
-    // cross would have stopped for tolerance 10^-ir_tol_pow; its M2L blocks are the leading
-    // sub-blocks of the stored ones, its transfers are recomputed from the leading L, R blocks
-    const bool inner_on = h.opts.fp32_shadow && h.opts.ir_tol_pow > 0 && h.opts.ir_tol_pow < h.tol_pow;
-    int64_t in_len = 0;
-    auto take_in = [&](int64_t elems) {
-        int64_t o = in_len;
-        in_len += align_up(elems, 32);
-        return o;
-    };
-    // compact copies of the inner M2L sub-blocks: (index into m2l, offset in the inner region)
-    std::vector<std::pair<size_t, int64_t>> m2l_in;
-    if (inner_on) {
-        const double tau = std::pow(10.0, -1.0 * h.opts.ir_tol_pow);
-        for (int f = 0; f < 2; ++f) {
-            if (!fam_on[f]) continue;
-            for (int j = nfam_lo[f]; j <= P.nL; ++j) {
-                FamLevel &fl = h.fam[f][j];
-                fl.kin.assign(fl.nb, 0);
-                for (int b = 0; b < fl.nb; ++b)
-                    fl.kin[b] = fl.rank[b] > 0 ? inner_rank(fl.hist[b], fl.rank[b], tau) : 0;
-            }
-            // inner transfers: rows are the children's leading pivots, each child's segment
-            // padded to a multiple of 4 rows so every row tile stays aligned
-            for (int j = nfam_lo[f]; j <= P.nL; ++j) {
-                FamLevel &fl = h.fam[f][j];
-                fl.ein_off.assign(fl.nb, -1);
-                fl.eint_off.assign(fl.nb, -1);
-                for (int b = 0; b < fl.nb; ++b) {
-                    int64_t m = 0;
-                    if (j == P.nL) m = P.bcount[j][b];
-                    else
-                        for (int c = 0; c < P.nchild; ++c) m += pad_ld(h.fam[f][j + 1].kin[b * P.nchild + c]);
-                    if (m * fl.kin[b] == 0) continue;
-                    fl.ein_off[b] = take_in(pad_ld(m) * fl.kin[b]);
-                    fl.eint_off[b] = take_in(pad_ld(fl.kin[b]) * m);
-                }
-            }
-        }
-        for (size_t q = 0; q < m2l.size(); ++q) {
-            const M2LBlk &m = m2l[q];
-            const FamLevel &fl = h.fam[m.f][m.j];
-            if (fl.kin[m.b] > 0 && fl.kin[m.p] > 0)
-                m2l_in.push_back({q, take_in(pad_ld(fl.kin[m.b]) * fl.kin[m.p])});
-        }
-    }
     if (h.opts.fp32_shadow) {
-        h.mat32.alloc(h.mat_len + in_len);
-        if (in_len > 0) H2W_HIP(hipMemsetAsync(h.mat32.p + h.mat_len, 0, in_len * 4, s));
+        h.mat32.alloc(h.mat_len);
         launch_to_fp32(h.mat64.p, h.mat32.p, h.mat_len, s);
         H2W_HIP(hipGetLastError());
     } else {
         h.mat32.release();
     }
-    if (inner_on && in_len > 0) {
-        std::vector<int32_t> idx;
-        std::vector<XferJob> xj;
-        std::vector<double> lr;
-        for (int f = 0; f < 2; ++f) {
-            if (!fam_on[f]) continue;
-            for (int j = nfam_lo[f]; j <= P.nL; ++j) {
-                FamLevel &fl = h.fam[f][j];
-                for (int b = 0; b < fl.nb; ++b) {
-                    if (fl.ein_off[b] < 0) continue;
-                    const int k = fl.kin[b], r = fl.rank[b];
-                    // leading k x k blocks of L and R
-                    const int64_t lro = (int64_t)lr.size();
-                    for (int part = 0; part < 2; ++part)
-                        for (int cc = 0; cc < k; ++cc)
-                            for (int rr = 0; rr < k; ++rr)
-                                lr.push_back(fl.lr[b][(size_t)part * r * r + (size_t)cc * r + rr]);
-                    const int64_t icq = (int64_t)idx.size();
-                    idx.insert(idx.end(), fl.Ic[b].begin(), fl.Ic[b].begin() + k);
-                    int64_t m = 0;
-                    if (j == P.nL) m = P.bcount[j][b];
-                    else
-                        for (int c = 0; c < P.nchild; ++c) m += pad_ld(h.fam[f][j + 1].kin[b * P.nchild + c]);
-                    const int lde = (int)pad_ld(m), ldet = (int)pad_ld(k);
-                    auto add_job = [&](const int32_t *S, int ms, int64_t row) {
-                        if (ms == 0) return;
-                        XferJob J;
-                        J.r = k;
-                        J.m = ms;
-                        J.s = (int64_t)idx.size();
-                        idx.insert(idx.end(), S, S + ms);
-                        J.ic = icq;
-                        J.lr = lro;
-                        J.e = fl.ein_off[b] + row;
-                        J.et = fl.eint_off[b] + row * ldet;
-                        J.lde = lde;
-                        J.ldet = ldet;
-                        xj.push_back(J);
-                    };
-                    if (j == P.nL) {
-                        auto rl = range_list(P.bstart[j][b], P.bcount[j][b]);
-                        add_job(rl.data(), (int)rl.size(), 0);
-                    } else {
-                        int64_t row = 0;
-                        for (int c = 0; c < P.nchild; ++c) {
-                            const int cc = b * P.nchild + c;
-                            const int kc = h.fam[f][j + 1].kin[cc];
-                            add_job(h.fam[f][j + 1].Ir[cc].data(), kc, row);
-                            row += pad_ld(kc);
-                        }
-                    }
-                }
-            }
-        }
-        DevBuf<int32_t> didx;
-        DevBuf<double> dlr, tmp;
-        DevBuf<XferJob> dxj;
-        didx.upload(idx, s);
-        dlr.upload(lr, s);
-        dxj.upload(xj, s);
-        tmp.alloc(in_len);
-        H2W_HIP(hipMemsetAsync(tmp.p, 0, in_len * 8, s));
-        launch_xfer(dxj.p, (int)xj.size(), didx.p, dlr.p, tmp.p, h.kp, s);
-        H2W_HIP(hipGetLastError());
-        launch_to_fp32(tmp.p, h.mat32.p + h.mat_len, in_len, s);
-        H2W_HIP(hipGetLastError());
-        // compact copies of the leading M2L sub-blocks
-        std::vector<CopyJob> cj;
-        for (auto &pr : m2l_in) {
-            const M2LBlk &m = m2l[pr.first];
-            const FamLevel &fl = h.fam[m.f][m.j];
-            CopyJob J;
-            J.src = m.off;
-            J.dst = h.mat_len + pr.second;
-            J.m = fl.kin[m.b];
-            J.n = fl.kin[m.p];
-            J.lds = (int32_t)pad_ld(fl.rank[m.b]);
-            J.ldd = (int32_t)pad_ld(fl.kin[m.b]);
-            cj.push_back(J);
-        }
-        DevBuf<CopyJob> dcj;
-        dcj.upload(cj, s);
-        launch_copy32(dcj.p, (int)cj.size(), h.mat32.p, s);
-        H2W_HIP(hipGetLastError());
-        H2W_HIP(hipStreamSynchronize(s));
-    }
-
     // ---- GEMV work lists, one per launch of the matvec
     std::vector<WorkList> WL;
     auto fam_up_item = [&](WorkList &W, int f, int j, int b, int64_t src) {
@@ -1256,137 +1421,7 @@ void assemble(Handle &h) {
         se += h.launch_elems[q];
     }
     h.stored_elems = se;
-    if (inner_on) {
-        const int64_t base = h.mat_len;  // the inner operator follows the main operators in the FP32 pool
-        std::vector<WorkList> IL;
-        {
-            WorkList W;
-            for (int f = 0; f < 2; ++f) {
-                if (!fam_on[f] || P.nL < nfam_lo[f]) continue;
-                FamLevel &fl = h.fam[f][P.nL];
-                for (int b = 0; b < fl.nb; ++b)
-                    if (fl.eint_off[b] >= 0)
-                        add_item(W, fl.o_off[b], fl.kin[b],
-                                 {TermSpec{base + fl.eint_off[b], h.off_x + P.bstart[P.nL][b], (int32_t)P.bcount[P.nL][b]}},
-                                 false, (int)pad_ld(fl.kin[b]));
-            }
-            for (auto &p : h.pairs)
-                if (p.rank > 0)
-                    add_item(W, p.t_off, p.rank,
-                             {TermSpec{p.ar_off, h.off_x + P.bstart[p.level][p.v], (int32_t)P.bcount[p.level][p.v]}},
-                             false, (int)pad_ld(p.rank));
-            IL.push_back(std::move(W));
-        }
-        for (int j = P.nL - 1; j >= 1; --j) {
-            WorkList W;
-            for (int f = 0; f < 2; ++f) {
-                if (!fam_on[f] || j < nfam_lo[f]) continue;
-                FamLevel &fl = h.fam[f][j];
-                FamLevel &ch = h.fam[f][j + 1];
-                for (int b = 0; b < fl.nb; ++b) {
-                    if (fl.eint_off[b] < 0) continue;
-                    const int64_t ldet = pad_ld(fl.kin[b]);
-                    std::vector<TermSpec> t;
-                    int64_t col = 0;
-                    for (int c = 0; c < P.nchild; ++c) {
-                        const int cc = b * P.nchild + c;
-                        if (ch.kin[cc] > 0)
-                            t.push_back(TermSpec{base + fl.eint_off[b] + col * ldet, ch.o_off[cc], (int32_t)ch.kin[cc]});
-                        col += pad_ld(ch.kin[cc]);
-                    }
-                    add_item(W, fl.o_off[b], fl.kin[b], t, false, (int)ldet);
-                }
-            }
-            IL.push_back(std::move(W));
-        }
-        {
-            WorkList W;
-            W.role = 1;
-            std::vector<std::vector<std::vector<std::vector<TermSpec>>>> per(2);
-            for (int f = 0; f < 2; ++f) {
-                if (!fam_on[f]) continue;
-                per[f].assign(P.nL + 1, {});
-                for (int j = nfam_lo[f]; j <= P.nL; ++j) per[f][j].assign(h.fam[f][j].nb, {});
-            }
-            for (auto &pr : m2l_in) {
-                const M2LBlk &m = m2l[pr.first];
-                const FamLevel &fl = h.fam[m.f][m.j];
-                per[m.f][m.j][m.b].push_back(TermSpec{base + pr.second, fl.o_off[m.p], (int32_t)fl.kin[m.p]});
-            }
-            for (int f = 0; f < 2; ++f) {
-                if (!fam_on[f]) continue;
-                for (int j = nfam_lo[f]; j <= P.nL; ++j) {
-                    FamLevel &fl = h.fam[f][j];
-                    for (int b = 0; b < fl.nb; ++b)
-                        if (fl.kin[b] > 0)
-                            add_item(W, fl.u_off[b], fl.kin[b], per[f][j][b], false, (int)pad_ld(fl.kin[b]));
-                }
-            }
-            std::vector<std::vector<TermSpec>> pn(nleaf);
-            for (auto &nb : nearb)
-                pn[nb.b].push_back(TermSpec{nb.off, h.off_x + P.bstart[P.nL][nb.n], (int32_t)P.bcount[P.nL][nb.n]});
-            for (int b = 0; b < nleaf; ++b)
-                add_item(W, h.off_y + P.bstart[P.nL][b], (int)P.bcount[P.nL][b], pn[b], false,
-                         (int)pad_ld(P.bcount[P.nL][b]));
-            IL.push_back(std::move(W));
-        }
-        for (int j = 1; j <= P.nL - 1; ++j) {
-            WorkList W;
-            for (int f = 0; f < 2; ++f) {
-                if (!fam_on[f] || j < nfam_lo[f]) continue;
-                FamLevel &fl = h.fam[f][j];
-                FamLevel &ch = h.fam[f][j + 1];
-                for (int b = 0; b < fl.nb; ++b) {
-                    if (fl.ein_off[b] < 0) continue;
-                    int64_t m = 0;
-                    for (int c = 0; c < P.nchild; ++c) m += pad_ld(ch.kin[b * P.nchild + c]);
-                    int64_t row = 0;
-                    for (int c = 0; c < P.nchild; ++c) {
-                        const int cc = b * P.nchild + c;
-                        if (ch.kin[cc] > 0)
-                            add_item(W, ch.u_off[cc], ch.kin[cc],
-                                     {TermSpec{base + fl.ein_off[b], fl.u_off[b], (int32_t)fl.kin[b]}}, false,
-                                     (int)pad_ld(m), (int)row);
-                        row += pad_ld(ch.kin[cc]);
-                    }
-                }
-            }
-            IL.push_back(std::move(W));
-        }
-        {
-            WorkList W;
-            for (int b = 0; b < nleaf; ++b) {
-                std::vector<TermSpec> t;
-                for (int f = 0; f < 2; ++f) {
-                    if (!fam_on[f] || P.nL < nfam_lo[f]) continue;
-                    FamLevel &fl = h.fam[f][P.nL];
-                    if (fl.ein_off[b] >= 0) t.push_back(TermSpec{base + fl.ein_off[b], fl.u_off[b], (int32_t)fl.kin[b]});
-                }
-                add_item(W, h.off_y + P.bstart[P.nL][b], (int)P.bcount[P.nL][b], t, false,
-                         (int)pad_ld(P.bcount[P.nL][b]));
-            }
-            IL.push_back(std::move(W));
-        }
-        if (!h.pairs.empty()) {
-            WorkList W;
-            W.role = 2;
-            std::vector<std::vector<std::vector<TermSpec>>> per(P.nL + 1);
-            for (int j = 1; j <= P.nL; ++j) per[j].assign(P.nboxes[j], {});
-            for (auto &p : h.pairs)
-                if (p.rank > 0) per[p.level][p.b].push_back(TermSpec{p.ac_off, p.t_off, (int32_t)p.rank});
-            for (int j = 1; j <= P.nL; ++j)
-                for (int b = 0; b < P.nboxes[j]; ++b)
-                    add_item(W, h.off_y + P.bstart[j][b], (int)P.bcount[j][b], per[j][b], true,
-                             (int)pad_ld(P.bcount[j][b]));
-            IL.push_back(std::move(W));
-        }
-        upload_lists(IL, h.inner, s);
-        h.inner_elems = 0;
-        for (auto e : h.inner.elems) h.inner_elems += e;
-    } else {
-        h.inner.launches.clear();
-        h.inner_elems = 0;
-    }
+    build_inner(h, h.opts.ir_tol_pow);
     H2W_HIP(hipStreamSynchronize(s));
     phase("work lists");
     if (verbose)
@@ -1404,16 +1439,20 @@ void run_launch(Handle &h, const Launch &Lh, int precision) {
     hipStream_t s = h.stream;
     const bool otf = (precision == H2W_FP32_OTF || precision == H2W_FP64_OTF) && Lh.role == 1;
     if (precision == H2W_FP32_INNER) {
-        launch_bgemv32(h.inner.d_items.p + Lh.item0, (int)Lh.nitems, h.inner.d_terms.p, h.mat32.p, h.vec32.p, Lh.role,
-                       s);
+        launch_bgemv32(h.inner.d_items.p + Lh.item0, (int)Lh.nitems, h.inner.d_terms.p, h.mat32.p, h.mat32in.p,
+                       h.vec32.p, Lh.role, Lh.unr, s);
         return;
     }
     if (is32(precision)) {
         if (otf) launch_otf32(h.d_items.p + Lh.item0, (int)Lh.nitems, h.d_terms.p, h.sp, h.vec32.p, s);
-        else launch_bgemv32(h.d_items.p + Lh.item0, (int)Lh.nitems, h.d_terms.p, h.mat32.p, h.vec32.p, Lh.role, s);
+        else
+            launch_bgemv32(h.d_items.p + Lh.item0, (int)Lh.nitems, h.d_terms.p, h.mat32.p, nullptr, h.vec32.p, Lh.role,
+                           Lh.unr32, s);
     } else {
         if (otf) launch_otf64(h.d_items.p + Lh.item0, (int)Lh.nitems, h.d_terms.p, h.sp, h.vec64.p, s);
-        else launch_bgemv64(h.d_items.p + Lh.item0, (int)Lh.nitems, h.d_terms.p, h.mat64.p, h.vec64.p, Lh.role, s);
+        else
+            launch_bgemv64(h.d_items.p + Lh.item0, (int)Lh.nitems, h.d_terms.p, h.mat64.p, nullptr, h.vec64.p, Lh.role,
+                           Lh.unr, s);
     }
 }
 
@@ -1471,6 +1510,13 @@ void drop_graphs(Handle &h) {
             (void)hipGraphExecDestroy(g);
             g = nullptr;
         }
+}
+
+void drop_graph(Handle &h, int precision) {
+    if (h.graph[precision]) {
+        (void)hipGraphExecDestroy(h.graph[precision]);
+        h.graph[precision] = nullptr;
+    }
 }
 
 // device-pointer matvec on one vector; tree-ordered in/out when tree == true
@@ -1537,6 +1583,18 @@ double dev_norm(Handle &h, const double *v) {
     H2W_HIP(hipMemcpyAsync(&r, h.g_h.p, 8, hipMemcpyDeviceToHost, h.stream));
     H2W_HIP(hipStreamSynchronize(h.stream));
     return std::sqrt(r);
+}
+
+double dev_dot(Handle &h, const double *a, const double *b) {
+    int nb = mdot_blocks(h.N);
+    h.g_part.ensure((size_t)nb * 4);
+    h.g_h.ensure(8);
+    launch_mdot(a, h.N, 1, b, h.N, h.g_part.p, nb, h.stream);
+    launch_mdot_reduce(h.g_part.p, nb, 1, h.g_h.p, h.stream);
+    double r = 0.0;
+    H2W_HIP(hipMemcpyAsync(&r, h.g_h.p, 8, hipMemcpyDeviceToHost, h.stream));
+    H2W_HIP(hipStreamSynchronize(h.stream));
+    return r;
 }
 
 void gen_rot(double dx, double dy, double &cs, double &sn) {
@@ -1638,6 +1696,188 @@ GmresOut gmres_core(Handle &h, const double *b, double *x, int max_iter, double 
 }
 
 }  // namespace
+
+// GMRES on the tree-ordered right-hand side bt (device), starting from and
+// updating h.g_x: plain FP64 GMRES, or FP64 iterative refinement whose
+// corrections come from GMRES on an FP32 operator.
+h2w_gmres_stats solve_tree(Handle &h, const double *bt, int max_iter, double tol, int mode) {
+    hipStream_t s = h.stream;
+    const int64_t N = h.N;
+    const int inner_prec = mode == H2W_GMRES_FP32_OTF_IR ? H2W_FP32_OTF
+                           : mode == H2W_GMRES_FP32_INNER_IR ? H2W_FP32_INNER
+                                                              : H2W_FP32;
+    double t0 = now_s();
+    h2w_gmres_stats S;
+    std::memset(&S, 0, sizeof(S));
+    if (mode == H2W_GMRES_FP64) {
+        auto o = gmres_core(h, bt, h.g_x.p, max_iter, tol, H2W_FP64, false, -1.0);
+        S.iterations = o.iters;
+        S.resid = o.resid;
+        S.matvecs_fp64 = o.matvecs;
+        S.converged = o.conv;
+    } else {
+        // FP64 iterative refinement: r = b - A x in FP64, correction from GMRES on the FP32 operator
+        double normb = dev_norm(h, bt);
+        if (normb == 0.0) normb = 1.0;
+        int total = 0, outer = 0;
+        double rn = 0.0;
+        while (true) {
+            matvec_dev(h, h.g_x.p, h.g_d.p, H2W_FP64, true);
+            S.matvecs_fp64++;
+            if (outer == 0) {
+                // start from alpha x0 with alpha minimising |b - alpha A x0| (no extra product):
+                // the first correction then starts from a residual no larger than b
+                const double dd = dev_dot(h, h.g_d.p, h.g_d.p);
+                const double alpha = dd > 0.0 ? dev_dot(h, h.g_d.p, bt) / dd : 1.0;
+                if (std::isfinite(alpha) && alpha != 1.0) {
+                    launch_scal(alpha, h.g_x.p, N, s);
+                    launch_scal(alpha, h.g_d.p, N, s);
+                }
+            }
+            H2W_HIP(hipMemcpyAsync(h.g_r.p, bt, N * 8, hipMemcpyDeviceToDevice, s));
+            launch_axpby(-1.0, h.g_d.p, 1.0, h.g_r.p, N, s);
+            double nr = dev_norm(h, h.g_r.p);
+            rn = nr / normb;
+            if (rn <= tol || total >= max_iter || outer >= 50) break;
+            const double floor_eta = mode == H2W_GMRES_FP32_INNER_IR ? std::pow(10.0, -1.0 * h.opts.ir_tol_pow) : 1e-6;
+            double eta = std::min(1e-2, std::max(floor_eta, 0.1 * tol / rn));
+            launch_fill(h.g_d.p, N, 0.0, s);
+            auto o = gmres_core(h, h.g_r.p, h.g_d.p, max_iter - total, eta, inner_prec, true, nr);
+            total += o.iters;
+            S.matvecs_fp32 += o.matvecs;
+            launch_axpby(1.0, h.g_d.p, 1.0, h.g_x.p, N, s);
+            ++outer;
+            if (o.iters == 0) break;
+        }
+        S.iterations = total;
+        S.outer_iterations = outer;
+        S.resid = rn;
+        S.converged = rn <= tol;
+    }
+    H2W_HIP(hipStreamSynchronize(s));
+    S.seconds = now_s() - t0;
+    return S;
+}
+
+
+// ------------------------------------------------------------------ autotuner
+namespace {
+
+struct EventPair {
+    hipEvent_t a = nullptr, b = nullptr;
+    EventPair() {
+        H2W_HIP(hipEventCreate(&a));
+        H2W_HIP(hipEventCreate(&b));
+    }
+    ~EventPair() {
+        if (a) (void)hipEventDestroy(a);
+        if (b) (void)hipEventDestroy(b);
+    }
+    EventPair(const EventPair &) = delete;
+    EventPair &operator=(const EventPair &) = delete;
+    double ms(hipStream_t s) {
+        H2W_HIP(hipEventSynchronize(b));
+        float m = 0.f;
+        H2W_HIP(hipEventElapsedTime(&m, a, b));
+        return m;
+    }
+};
+
+double time_launch(Handle &h, const Launch &Lh, int precision, int nrep, EventPair &ev) {
+    run_launch(h, Lh, precision);
+    H2W_HIP(hipEventRecord(ev.a, h.stream));
+    for (int r = 0; r < nrep; ++r) run_launch(h, Lh, precision);
+    H2W_HIP(hipEventRecord(ev.b, h.stream));
+    H2W_HIP(hipGetLastError());
+    return ev.ms(h.stream) / nrep;
+}
+
+// unit input in the pool's x segment, zero outputs
+void tune_inputs(Handle &h, int precision) {
+    if (is32(precision)) launch_fill32(h.vec32.p + h.off_x, h.N, 1.0f, h.stream);
+    else launch_fill(h.vec64.p + h.off_x, h.N, 1.0, h.stream);
+    zero_outputs(h, precision);
+}
+
+}  // namespace
+
+// one whole matvec (graph replay when enabled), ms
+double time_matvec(Handle &h, int precision, int nrep) {
+    EventPair ev;
+    tune_inputs(h, precision);
+    run_launches(h, precision);
+    H2W_HIP(hipEventRecord(ev.a, h.stream));
+    for (int r = 0; r < nrep; ++r) run_launches(h, precision);
+    H2W_HIP(hipEventRecord(ev.b, h.stream));
+    return ev.ms(h.stream) / nrep;
+}
+
+// Per launch of one precision's matvec: 2, 4 or 8 column loads in flight per lane,
+// whichever runs fastest (4 unless another is at least 2 % faster).
+void tune_unroll(Handle &h, int precision) {
+    std::vector<Launch> &L = precision == H2W_FP32_INNER ? h.inner.launches : h.launches;
+    if (L.empty()) return;
+    EventPair ev;
+    tune_inputs(h, precision);
+    for (auto &Lh : L) {
+        int &u = precision == H2W_FP32 ? Lh.unr32 : Lh.unr;
+        const int64_t elems = precision == H2W_FP32_INNER ? h.inner.elems[&Lh - &L[0]] : h.launch_elems[&Lh - &L[0]];
+        const int nrep = elems > (int64_t)1 << 26 ? 3 : 8;
+        u = 4;
+        double tb = time_launch(h, Lh, precision, nrep, ev);
+        int best = 4;
+        for (int c : {2, 8}) {
+            u = c;
+            double t = time_launch(h, Lh, precision, nrep, ev);
+            if (t < 0.98 * tb) {
+                tb = t;
+                best = c;
+            }
+        }
+        u = best;
+    }
+    H2W_HIP(hipStreamSynchronize(h.stream));
+    drop_graph(h, precision);
+}
+
+// Inner tolerance of the refinement: the fastest converged solve of A x = bt (x0 = 0)
+// over 10^-p, p = 4..8 below tol_pow.
+void tune_inner(Handle &h, const double *bt, double tol, int max_iter, bool unroll, h2w_tune *out) {
+    const int p0 = h.opts.ir_tol_pow;
+    int best = 0;
+    double tbest = 1e300;
+    out->n_cand = 0;
+    for (int p = 4; p <= 8 && p < h.tol_pow && out->n_cand < H2W_TUNE_MAX_CAND; ++p) {
+        build_inner(h, p);
+        if (h.inner.launches.empty()) continue;
+        if (unroll) tune_unroll(h, H2W_FP32_INNER);
+        time_matvec(h, H2W_FP32_INNER, 1);  // captures the graph outside the timed solves
+        time_matvec(h, H2W_FP64, 1);
+        h2w_gmres_stats S{}, S2{};
+        for (int rep = 0; rep < 2; ++rep) {
+            launch_fill(h.g_x.p, h.N, 0.0, h.stream);
+            S2 = solve_tree(h, bt, max_iter, tol, H2W_GMRES_FP32_INNER_IR);
+            if (rep == 0 || S2.seconds < S.seconds) S = S2;
+        }
+        const int c = out->n_cand++;
+        out->cand_ir_tol_pow[c] = p;
+        out->cand_seconds[c] = S.seconds;
+        out->cand_iterations[c] = S.iterations;
+        out->cand_outer[c] = S.outer_iterations;
+        out->cand_converged[c] = S.converged;
+        out->cand_inner_bytes[c] = h.inner_elems * 4;
+        if (S.converged && S.seconds < tbest) {
+            tbest = S.seconds;
+            best = p;
+        }
+    }
+    if (best == 0) best = p0;
+    if (best != h.opts.ir_tol_pow || h.inner.launches.empty()) {
+        build_inner(h, best);
+        if (unroll) tune_unroll(h, H2W_FP32_INNER);
+    }
+    out->ir_tol_pow = h.opts.ir_tol_pow;
+}
 }  // namespace h2w
 
 // ------------------------------------------------------------------ C ABI
@@ -1820,9 +2060,6 @@ int h2w_gmres(h2w_handle *h, const double *b, double *x, int max_iter, double to
     if (mode != H2W_GMRES_FP64 && !h->mat32.p) throw Error{H2W_ESTATE, "FP32 shadow not built"};
     if (mode == H2W_GMRES_FP32_INNER_IR && h->inner.launches.empty())
         throw Error{H2W_ESTATE, "the inner operator was not built (opts.ir_tol_pow)"};
-    const int inner_prec = mode == H2W_GMRES_FP32_OTF_IR ? H2W_FP32_OTF
-                           : mode == H2W_GMRES_FP32_INNER_IR ? H2W_FP32_INNER
-                                                              : H2W_FP32;
     H2W_HIP(hipSetDevice(h->opts.device));
     const int64_t N = h->N;
     hipStream_t s = h->stream;
@@ -1844,46 +2081,7 @@ int h2w_gmres(h2w_handle *h, const double *b, double *x, int max_iter, double to
     h2w::launch_permute_in(bsrc, N, bt.p, N, h->d_perm.p, N, 1, s);
     h2w::launch_permute_in(xsrc, N, h->g_x.p, N, h->d_perm.p, N, 1, s);
     H2W_HIP(hipStreamSynchronize(s));
-    double t0 = h2w::now_s();
-    h2w_gmres_stats S;
-    std::memset(&S, 0, sizeof(S));
-    if (mode == H2W_GMRES_FP64) {
-        auto o = h2w::gmres_core(*h, bt.p, h->g_x.p, max_iter, tol, H2W_FP64, false, -1.0);
-        S.iterations = o.iters;
-        S.resid = o.resid;
-        S.matvecs_fp64 = o.matvecs;
-        S.converged = o.conv;
-    } else {
-        // FP64 iterative refinement: r = b - A x in FP64, correction from GMRES on the FP32 operator
-        double normb = h2w::dev_norm(*h, bt.p);
-        if (normb == 0.0) normb = 1.0;
-        int total = 0, outer = 0;
-        double rn = 0.0;
-        while (true) {
-            h2w::matvec_dev(*h, h->g_x.p, h->g_d.p, H2W_FP64, true);
-            S.matvecs_fp64++;
-            H2W_HIP(hipMemcpyAsync(h->g_r.p, bt.p, N * 8, hipMemcpyDeviceToDevice, s));
-            h2w::launch_axpby(-1.0, h->g_d.p, 1.0, h->g_r.p, N, s);
-            double nr = h2w::dev_norm(*h, h->g_r.p);
-            rn = nr / normb;
-            if (rn <= tol || total >= max_iter || outer >= 50) break;
-            const double floor_eta = mode == H2W_GMRES_FP32_INNER_IR ? std::pow(10.0, -1.0 * h->opts.ir_tol_pow) : 1e-6;
-            double eta = std::min(1e-2, std::max(floor_eta, 0.1 * tol / rn));
-            h2w::launch_fill(h->g_d.p, N, 0.0, s);
-            auto o = h2w::gmres_core(*h, h->g_r.p, h->g_d.p, max_iter - total, eta, inner_prec, true, nr);
-            total += o.iters;
-            S.matvecs_fp32 += o.matvecs;
-            h2w::launch_axpby(1.0, h->g_d.p, 1.0, h->g_x.p, N, s);
-            ++outer;
-            if (o.iters == 0) break;
-        }
-        S.iterations = total;
-        S.outer_iterations = outer;
-        S.resid = rn;
-        S.converged = rn <= tol;
-    }
-    H2W_HIP(hipStreamSynchronize(s));
-    S.seconds = h2w::now_s() - t0;
+    h2w_gmres_stats S = h2w::solve_tree(*h, bt.p, max_iter, tol, mode);
     double *xdst = where == H2W_HOST ? h->d_stage_out.p : x;
     h2w::launch_permute_out(h->g_x.p, N, xdst, N, h->d_perm.p, N, 1, s);
     if (where == H2W_HOST) H2W_HIP(hipMemcpyAsync(x, h->d_stage_out.p, N * 8, hipMemcpyDeviceToHost, s));
@@ -2096,6 +2294,64 @@ int h2w_bench_matvec(h2w_handle *h, int nrep, int precision, h2w_bench *out) {
     for (auto &e : ev) (void)hipEventDestroy(e);
     (void)hipEventDestroy(t0);
     (void)hipEventDestroy(t1);
+    return H2W_OK;
+    H2W_CATCH
+}
+
+int h2w_set_inner_tolerance(h2w_handle *h, int ir_tol_pow) {
+    H2W_TRY
+    if (!h) throw Error{H2W_EINVAL, "NULL handle"};
+    if (!h->assembled) throw Error{H2W_ESTATE, "not assembled"};
+    if (!h->mat32.p) throw Error{H2W_ESTATE, "the inner operator needs the FP32 shadow (opts.fp32_shadow)"};
+    if (ir_tol_pow < 0 || (ir_tol_pow > 0 && ir_tol_pow >= h->tol_pow))
+        throw Error{H2W_EINVAL, "ir_tol_pow must be 0 or in (0, tol_pow)"};
+    H2W_HIP(hipSetDevice(h->opts.device));
+    h2w::build_inner(*h, ir_tol_pow);
+    return H2W_OK;
+    H2W_CATCH
+}
+
+int h2w_autotune(h2w_handle *h, const double *b, int where, double tol, int max_iter, int flags, h2w_tune *out) {
+    H2W_TRY
+    if (!h || !out) throw Error{H2W_EINVAL, "NULL argument"};
+    if (!h->assembled) throw Error{H2W_ESTATE, "not assembled"};
+    if (flags & ~(H2W_TUNE_UNROLL | H2W_TUNE_INNER)) throw Error{H2W_EINVAL, "unknown flags"};
+    if ((flags & H2W_TUNE_INNER) && (!b || max_iter < 1 || !(tol > 0.0)))
+        throw Error{H2W_EINVAL, "inner tuning needs b, tol > 0 and max_iter >= 1"};
+    if ((flags & H2W_TUNE_INNER) && !h->mat32.p) throw Error{H2W_ESTATE, "FP32 shadow not built"};
+    H2W_HIP(hipSetDevice(h->opts.device));
+    std::memset(out, 0, sizeof(*out));
+    const int64_t N = h->N;
+    hipStream_t s = h->stream;
+    const int precs[3] = {H2W_FP64, H2W_FP32, H2W_FP32_INNER};
+    auto have = [&](int p) {
+        return p == H2W_FP64 || (h->mat32.p && (p != H2W_FP32_INNER || !h->inner.launches.empty()));
+    };
+    for (int q = 0; q < 3; ++q) out->matvec_ms_before[q] = have(precs[q]) ? h2w::time_matvec(*h, precs[q], 5) : 0.0;
+    if (flags & H2W_TUNE_UNROLL)
+        for (int q = 0; q < 2; ++q)
+            if (have(precs[q])) h2w::tune_unroll(*h, precs[q]);
+    if (flags & H2W_TUNE_INNER) {
+        h->g_x.ensure(N);
+        h->g_r.ensure(N);
+        h->g_d.ensure(N);
+        h2w::DevBuf<double> bin, bt;
+        bt.alloc(N);
+        const double *bsrc = b;
+        if (where == H2W_HOST) {
+            bin.alloc(N);
+            H2W_HIP(hipMemcpyAsync(bin.p, b, N * 8, hipMemcpyHostToDevice, s));
+            bsrc = bin.p;
+        }
+        h2w::launch_permute_in(bsrc, N, bt.p, N, h->d_perm.p, N, 1, s);
+        H2W_HIP(hipStreamSynchronize(s));
+        h2w::tune_inner(*h, bt.p, tol, max_iter, (flags & H2W_TUNE_UNROLL) != 0, out);
+    } else {
+        if ((flags & H2W_TUNE_UNROLL) && have(H2W_FP32_INNER)) h2w::tune_unroll(*h, H2W_FP32_INNER);
+        out->ir_tol_pow = h->inner.launches.empty() ? 0 : h->opts.ir_tol_pow;
+    }
+    for (int q = 0; q < 3; ++q) out->matvec_ms_after[q] = have(precs[q]) ? h2w::time_matvec(*h, precs[q], 5) : 0.0;
+    H2W_HIP(hipStreamSynchronize(s));
     return H2W_OK;
     H2W_CATCH
 }

@@ -490,13 +490,16 @@ __global__ void k_cur(const CurJob *__restrict__ jobs, const int32_t *__restrict
 // or atomically.  Blocks are column-major with leading dimensions padded to a
 // multiple of 4, so every VEC group is aligned and stays inside the column.
 // ROLE only names the instance (0: transfer passes, 1: M2L + near field,
-// 2: semi-nested column factors) so that profiles separate the launches.
-template <typename T, int VEC, int ROLE>
+// 2: semi-nested column factors) so that profiles separate the launches; UNR
+// is the number of column loads each lane keeps in flight.  Bit 1 of an item's
+// flags selects the operator pool A1 (the inner operator of the refinement)
+// instead of A0.
+template <typename T, int VEC, int ROLE, int UNR>
 __global__ __launch_bounds__(256) void k_bgemv(const GItem *__restrict__ items, const GTerm *__restrict__ terms,
-                                               const T *__restrict__ A, T *vec) {
+                                               const T *__restrict__ A0, const T *__restrict__ A1, T *vec) {
     typedef T V __attribute__((ext_vector_type(VEC)));
-    constexpr int UNR = 4;
     const GItem it = items[blockIdx.x];
+    const T *__restrict__ A = (it.flags & 2) ? A1 : A0;
     const int lane = threadIdx.x & 63;
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int rows = it.rows;
@@ -551,13 +554,13 @@ __global__ __launch_bounds__(256) void k_bgemv(const GItem *__restrict__ items, 
     }
 }
 
-// leading sub-block copies inside the FP32 pool (compact inner-operator blocks)
-__global__ void k_copy32(const CopyJob *__restrict__ jobs, float *pool) {
+// leading sub-block copies from the FP32 pool into the inner-operator buffer
+__global__ void k_copy32(const CopyJob *__restrict__ jobs, const float *__restrict__ src, float *dst) {
     const CopyJob jb = jobs[blockIdx.x];
     const int64_t tot = (int64_t)jb.m * jb.n;
     for (int64_t e = threadIdx.x; e < tot; e += blockDim.x) {
         int i = (int)(e % jb.m), j = (int)(e / jb.m);
-        pool[jb.dst + (int64_t)j * jb.ldd + i] = pool[jb.src + (int64_t)j * jb.lds + i];
+        dst[jb.dst + (int64_t)j * jb.ldd + i] = src[jb.src + (int64_t)j * jb.lds + i];
     }
 }
 
@@ -794,6 +797,11 @@ __global__ void k_axpby(double a, const double *__restrict__ x, double b, double
         y[i] = b == 0.0 ? a * x[i] : a * x[i] + b * y[i];
 }
 
+__global__ void k_scal(double a, double *y, int64_t n) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        y[i] *= a;
+}
+
 inline int grid_for(int64_t n, int nt = 256, int cap = 8192) {
     int64_t g = (n + nt - 1) / nt;
     if (g < 1) g = 1;
@@ -829,26 +837,35 @@ void launch_cur(const CurJob *jobs, int njobs, const int32_t *idx, const double 
                 hipStream_t s) {
     if (njobs > 0) hipLaunchKernelGGL(k_cur, dim3(njobs), dim3(256), 0, s, jobs, idx, lr, A, kp);
 }
-template <typename T>
-static void launch_bgemv_t(const GItem *items, int nitems, const GTerm *terms, const T *A, T *vec, int role,
-                           hipStream_t s) {
-    if (nitems <= 0) return;
+template <typename T, int ROLE>
+static void launch_bgemv_r(const GItem *items, int nitems, const GTerm *terms, const T *A0, const T *A1, T *vec,
+                           int unr, hipStream_t s) {
     constexpr int VEC = sizeof(T) == 8 ? 2 : 4;
-    if (role == 1) hipLaunchKernelGGL((k_bgemv<T, VEC, 1>), dim3(nitems), dim3(256), 0, s, items, terms, A, vec);
-    else if (role == 2)
-        hipLaunchKernelGGL((k_bgemv<T, VEC, 2>), dim3(nitems), dim3(256), 0, s, items, terms, A, vec);
-    else hipLaunchKernelGGL((k_bgemv<T, VEC, 0>), dim3(nitems), dim3(256), 0, s, items, terms, A, vec);
+    if (unr <= 2)
+        hipLaunchKernelGGL((k_bgemv<T, VEC, ROLE, 2>), dim3(nitems), dim3(256), 0, s, items, terms, A0, A1, vec);
+    else if (unr >= 8)
+        hipLaunchKernelGGL((k_bgemv<T, VEC, ROLE, 8>), dim3(nitems), dim3(256), 0, s, items, terms, A0, A1, vec);
+    else hipLaunchKernelGGL((k_bgemv<T, VEC, ROLE, 4>), dim3(nitems), dim3(256), 0, s, items, terms, A0, A1, vec);
 }
-void launch_copy32(const CopyJob *jobs, int njobs, float *pool, hipStream_t s) {
-    if (njobs > 0) hipLaunchKernelGGL(k_copy32, dim3(njobs), dim3(256), 0, s, jobs, pool);
+template <typename T>
+static void launch_bgemv_t(const GItem *items, int nitems, const GTerm *terms, const T *A0, const T *A1, T *vec,
+                           int role, int unr, hipStream_t s) {
+    if (nitems <= 0) return;
+    if (!A1) A1 = A0;
+    if (role == 1) launch_bgemv_r<T, 1>(items, nitems, terms, A0, A1, vec, unr, s);
+    else if (role == 2) launch_bgemv_r<T, 2>(items, nitems, terms, A0, A1, vec, unr, s);
+    else launch_bgemv_r<T, 0>(items, nitems, terms, A0, A1, vec, unr, s);
 }
-void launch_bgemv64(const GItem *items, int nitems, const GTerm *terms, const double *A, double *vec, int role,
-                    hipStream_t s) {
-    launch_bgemv_t<double>(items, nitems, terms, A, vec, role, s);
+void launch_copy32(const CopyJob *jobs, int njobs, const float *src, float *dst, hipStream_t s) {
+    if (njobs > 0) hipLaunchKernelGGL(k_copy32, dim3(njobs), dim3(256), 0, s, jobs, src, dst);
 }
-void launch_bgemv32(const GItem *items, int nitems, const GTerm *terms, const float *A, float *vec, int role,
-                    hipStream_t s) {
-    launch_bgemv_t<float>(items, nitems, terms, A, vec, role, s);
+void launch_bgemv64(const GItem *items, int nitems, const GTerm *terms, const double *A0, const double *A1,
+                    double *vec, int role, int unr, hipStream_t s) {
+    launch_bgemv_t<double>(items, nitems, terms, A0, A1, vec, role, unr, s);
+}
+void launch_bgemv32(const GItem *items, int nitems, const GTerm *terms, const float *A0, const float *A1, float *vec,
+                    int role, int unr, hipStream_t s) {
+    launch_bgemv_t<float>(items, nitems, terms, A0, A1, vec, role, unr, s);
 }
 void launch_otf32(const GItem *items, int nitems, const GTerm *terms, SlotPoints sp, float *vec, hipStream_t s) {
     if (nitems > 0)
@@ -921,6 +938,9 @@ void launch_vec_update(const double *V, int64_t ldv, int k, const double *c, dou
 }
 void launch_axpby(double a, const double *x, double b, double *y, int64_t n, hipStream_t s) {
     hipLaunchKernelGGL(k_axpby, dim3(grid_for(n)), dim3(256), 0, s, a, x, b, y, n);
+}
+void launch_scal(double a, double *y, int64_t n, hipStream_t s) {
+    hipLaunchKernelGGL(k_scal, dim3(grid_for(n)), dim3(256), 0, s, a, y, n);
 }
 
 }  // namespace h2w

@@ -170,6 +170,34 @@ typedef struct {
 } h2w_bench;
 int h2w_bench_matvec(h2w_handle *h, int nrep, int precision, h2w_bench *out);
 
+/* Rebuild the truncated FP32 inner operator of H2W_GMRES_FP32_INNER_IR for tolerance
+ * 10^-ir_tol_pow (0 < ir_tol_pow < tol_pow; 0 removes it).  No counterpart in the reference
+ * (which has one FP64 operator); the stored FP64 / FP32 operators are untouched. */
+int h2w_set_inner_tolerance(h2w_handle *h, int ir_tol_pow);
+
+/* Autotuner (no counterpart in the reference).  With H2W_TUNE_UNROLL it times every launch of
+ * the FP64, FP32 and FP32-inner matvecs with 2, 4 and 8 column loads in flight per lane and
+ * keeps the fastest per launch.  With H2W_TUNE_INNER it solves A x = b (x0 = 0) with
+ * H2W_GMRES_FP32_INNER_IR for each inner tolerance 10^-p, p in 4..8 below tol_pow, and keeps
+ * the fastest solve that converges.  b has N entries in input order, on the host or the device
+ * (`where`); the handle's own x is unchanged. */
+#define H2W_TUNE_UNROLL 1
+#define H2W_TUNE_INNER 2
+#define H2W_TUNE_MAX_CAND 8
+typedef struct {
+    int ir_tol_pow;                          /* chosen inner exponent (0: no inner operator) */
+    int n_cand;
+    int cand_ir_tol_pow[H2W_TUNE_MAX_CAND];
+    double cand_seconds[H2W_TUNE_MAX_CAND];  /* solve time, best of two runs */
+    int cand_iterations[H2W_TUNE_MAX_CAND];  /* inner (FP32) GMRES iterations */
+    int cand_outer[H2W_TUNE_MAX_CAND];       /* refinement steps */
+    int cand_converged[H2W_TUNE_MAX_CAND];
+    int64_t cand_inner_bytes[H2W_TUNE_MAX_CAND]; /* FP32 bytes one inner matvec streams */
+    double matvec_ms_before[3];              /* FP64, FP32, FP32 inner matvec before the unroll tuning */
+    double matvec_ms_after[3];
+} h2w_tune;
+int h2w_autotune(h2w_handle *h, const double *b, int where, double tol, int max_iter, int flags, h2w_tune *out);
+
 /* Tree-order permutation: perm[t] = input index of the t-th point in tree order
  * (leaf boxes' chargeLocations concatenated, 3D/codes/HODLR3Dtree.hpp:2489-2503). */
 int h2w_tree_order(const h2w_handle *h, int32_t *perm);
