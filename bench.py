@@ -32,6 +32,7 @@ sys.path.insert(0, ROOT)
 METRIC = "GMRES time-to-1e-10 (s) & matvec GB/s: FP32+refine batched vs FP64 GPU, 1 GPU"
 HBM_PEAK_GBPS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
+MFMA_PEAK_TFLOPS = {"f32": 157.3, "f64": 78.6}  # dense matrix-core peaks of MI355X
 # flops of one 3D RBF entry evaluated on the fly (add/mul = 1, fma = 2, sqrt/log = 1):
 # 3 differences, R^2 (mul + 2 fma), sqrt, log, scale, accumulate (fma)
 FLOP_PER_ENTRY = 3 + 5 + 1 + 1 + 1 + 2
@@ -157,6 +158,7 @@ def main():
     ap.add_argument("--tol-pow", type=int, default=10)
     ap.add_argument("--cpu-side", type=int, default=24, help="grid side of the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-multi", action="store_true", help="skip the 16-right-hand-side section")
     ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (default launch shapes, inner 1e-6)")
     ap.add_argument("--pmc-probe", action="store_true",
                     help="assemble, then run 3 FP64 and 3 FP32-inner matvecs and a 1 GiB device copy "
@@ -294,6 +296,44 @@ def main():
     roof, roof2 = (roof_in, roof_st) if t_in >= t_st else (roof_st, roof_in)
     copy_bw = hip.copy_gbps()
     log(f"device-to-device copy: {copy_bw:.0f} GB/s (read + write)")
+
+    # 16 right-hand sides per pass on the matrix cores: the product and 16 lockstep solves
+    multi = None
+    if not args.no_multi:
+        multi = {"nrhs": 16, "matvec": {}, "gmres": {}}
+        peak = {"fp64": MFMA_PEAK_TFLOPS["f64"], "fp32": MFMA_PEAK_TFLOPS["f32"], "fp32_inner": MFMA_PEAK_TFLOPS["f32"]}
+        for prec, name, s_op in ((h2w.FP64, "fp64", 8), (h2w.FP32, "fp32", 4), (h2w.FP32_INNER, "fp32_inner", 4)):
+            r = A.bench_matvec_multi(5, prec)
+            ms = r["ms_per_pass"]
+            elems = (st["stored_bytes_fp64"] // 8 if name != "fp32_inner"
+                     else A.stats()["stored_bytes_inner_fp32"] // 4)  # after tuning
+            nbytes = s_op * elems + 2 * 8 * N * 16
+            tflops = 2.0 * elems * 16 / (ms * 1e-3) * 1e-12
+            multi["matvec"][name] = {"ms_per_16": round(ms, 4), "ms_per_rhs": round(ms / 16, 4),
+                                     "alg_GBps": round(nbytes / (ms * 1e-3) * 1e-9, 1),
+                                     "TFLOPs": round(tflops, 2), "mfma_frac": round(tflops / peak[name], 4),
+                                     "speedup_vs_16_single": round(16 * mv[name]["ms"] / ms, 2)}
+            log(f"16-wide matvec {name}: {ms:.3f} ms per pass ({ms / 16:.3f} ms per right-hand side, "
+                f"{tflops:.1f} TFLOP/s, x{16 * mv[name]['ms'] / ms:.2f} vs 16 single products)")
+        XT = np.random.default_rng(6).uniform(-1.0, 1.0, (16, N))
+        BM = np.empty_like(XT)
+        for c in range(16):
+            BM[c] = A.dense_rows(None, XT[c])
+        for name, mode in (("fp64", h2w.GMRES_FP64), ("fp32_inner_ir", h2w.GMRES_FP32_INNER_IR)):
+            A.gmres_multi(BM[:2], None, 500, gtol, mode)  # warm-up (graph capture)
+            t0 = time.time()
+            XM, sts = A.gmres_multi(BM, None, 500, gtol, mode)
+            dt = time.time() - t0
+            errs = [float(np.linalg.norm(XM[c] - XT[c]) / np.linalg.norm(XT[c])) for c in range(16)]
+            multi["gmres"][name] = {"s_16": round(sts[0]["seconds"], 4), "s_16_host_incl": round(dt, 4),
+                                    "max_rel_error": max(errs), "all_converged": all(x["converged"] for x in sts),
+                                    "max_iterations": max(x["iterations"] for x in sts),
+                                    "matvecs_fp64": sts[0]["matvecs_fp64"], "matvecs_fp32": sts[0]["matvecs_fp32"]}
+            log(f"16 solves {name}: {sts[0]['seconds'] * 1e3:.1f} ms ({sts[0]['seconds'] * 1e3 / 16:.2f} ms per "
+                f"right-hand side), max error {max(errs):.2e}")
+        multi["gmres"]["fp32_inner_ir_vs_fp64"] = round(multi["gmres"]["fp64"]["s_16"]
+                                                       / multi["gmres"]["fp32_inner_ir"]["s_16"], 3)
+        multi["gmres"]["vs_16_single_fp64_solves"] = round(16 * t64 / multi["gmres"]["fp32_inner_ir"]["s_16"], 2)
     for r in (roof_in, roof_st):
         r["copy_GBps"] = round(copy_bw, 1)
         r["frac_of_copy"] = round(r["achieved"] / copy_bw, 4)
@@ -312,7 +352,7 @@ def main():
            "matvec": {k: {kk: (round(vv, 4) if isinstance(vv, float) else vv) for kk, vv in v.items()}
                       for k, v in mv.items()},
            "assembly_s": round(st["assembly_seconds"], 2), "memory_elements": M,
-           "autotune": tune,
+           "autotune": tune, "multi_rhs": multi,
            "roofline": roof, "roofline_other": roof2, "roofline_otf": roof_otf,
            "inner_operator_bytes_fp32": st["stored_bytes_inner_fp32"], "cpu_baseline": None}
     if not args.no_cpu:

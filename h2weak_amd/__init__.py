@@ -35,7 +35,7 @@ GMRES_FP64, GMRES_FP32_IR, GMRES_FP32_OTF_IR, GMRES_FP32_INNER_IR = 0, 1, 2, 3
 EXPORTS = ("h2w_default_opts", "h2w_create", "h2w_assemble", "h2w_matvec", "h2w_matvec_ex", "h2w_gmres",
            "h2w_dense_matvec", "h2w_get_stats", "h2w_tree_order", "h2w_box_skeleton", "h2w_interaction_list",
            "h2w_near_list", "h2w_destroy", "h2w_last_error", "h2w_bench_matvec", "h2w_set_inner_tolerance",
-           "h2w_autotune")
+           "h2w_autotune", "h2w_gmres_multi", "h2w_bench_matvec_multi")
 TUNE_UNROLL, TUNE_INNER = 1, 2
 
 
@@ -114,6 +114,8 @@ def lib():
     L.h2w_last_error.restype = ctypes.c_char_p
     L.h2w_bench_matvec.argtypes = [p, i, i, ctypes.POINTER(BenchResult)]
     L.h2w_set_inner_tolerance.argtypes = [p, i]
+    L.h2w_bench_matvec_multi.argtypes = [p, i, i, ctypes.POINTER(BenchResult)]
+    L.h2w_gmres_multi.argtypes = [p, p, i64, p, i64, i, i, d, i, i, ctypes.POINTER(GmresStats)]
     L.h2w_autotune.argtypes = [p, p, i, d, i, i, ctypes.POINTER(Tune)]
     _lib = L
     return L
@@ -205,6 +207,18 @@ class Initialisation:
             _check(rc)
         return x, st.as_dict()
 
+    def gmres_multi(self, B, X0=None, max_iter=500, tol=1e-12, mode=GMRES_FP64):
+        """nrhs independent solves (rows of B), 16 at a time in lockstep on the matrix cores."""
+        B = _f64(np.atleast_2d(B))
+        nrhs = B.shape[0]
+        X = np.ones_like(B) if X0 is None else np.array(np.atleast_2d(X0), dtype=np.float64)
+        st = (GmresStats * nrhs)()
+        rc = lib().h2w_gmres_multi(self._h, _ptr(B), self.N, _ptr(X), self.N, nrhs, int(max_iter), float(tol), mode,
+                                   HOST, st)
+        if rc not in (H2W_OK, H2W_ENOCONV):
+            _check(rc)
+        return X, [st[c].as_dict() for c in range(nrhs)]
+
     def stats(self):
         st = Stats()
         _check(lib().h2w_get_stats(self._h, ctypes.byref(st)))
@@ -271,6 +285,13 @@ class Initialisation:
                                 "converged": bool(t.cand_converged[q]), "inner_bytes": t.cand_inner_bytes[q]}
                                for q in range(n)],
                 "matvec_ms_before": list(t.matvec_ms_before), "matvec_ms_after": list(t.matvec_ms_after)}
+
+
+    def bench_matvec_multi(self, nrep=10, precision=FP64):
+        """HIP-event time of one 16-wide product (16 right-hand sides per pass)."""
+        r = BenchResult()
+        _check(lib().h2w_bench_matvec_multi(self._h, int(nrep), precision, ctypes.byref(r)))
+        return {"ms_per_pass": r.ms_per_matvec, "nrhs": 16}
 
 
 class IterativeSolver:

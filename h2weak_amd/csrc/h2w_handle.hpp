@@ -1,0 +1,218 @@
+// h2w_handle.hpp -- the library handle and the host-side types shared by the
+// translation units of libh2w (assembly and single-vector runtime in
+// h2w_ops.hip, kNR-wide and batched runtime in multi.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cmath>
+#include <string>
+#include <vector>
+
+#include "h2w_internal.hpp"
+
+namespace h2w {
+
+
+constexpr double kPI = 3.141592653589793238462643383279502884;
+
+inline double now_s() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+template <typename T>
+struct DevBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    DevBuf() {}
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    ~DevBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    void alloc(size_t cnt) {
+        release();
+        if (cnt == 0) return;
+        hipError_t e = hipMalloc(&p, cnt * sizeof(T));
+        if (e != hipSuccess) {
+            p = nullptr;
+            throw Error{H2W_ENOMEM, "hipMalloc of " + std::to_string(cnt * sizeof(T)) + " bytes failed"};
+        }
+        n = cnt;
+    }
+    void ensure(size_t cnt) {
+        if (cnt > n) alloc(cnt);
+    }
+    void upload(const std::vector<T> &v, hipStream_t s) {
+        ensure(v.size());
+        if (!v.empty()) H2W_HIP(hipMemcpyAsync(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s));
+    }
+};
+
+inline int64_t align_up(int64_t x, int64_t a) { return (x + a - 1) / a * a; }
+
+// one ACA problem: rows/cols are tree-order point indices
+struct AcaBlock {
+    std::vector<int32_t> rows, cols;
+    int rank = 0;
+    std::vector<int32_t> rb, cb;  // local pivot indices
+    std::vector<double> lr;       // L then R, rank x rank column-major each
+    std::vector<double> hist;     // per step: |u||v| and normS after the step
+};
+
+// skeleton data of one family at one level
+struct FamLevel {
+    int nb = 0;
+    std::vector<int> rank;
+    std::vector<std::vector<int32_t>> Ir, Ic;  // tree-order indices
+    std::vector<std::vector<double>> lr;
+    std::vector<std::vector<double>> hist;     // ACA history (|u||v|, normS) per step
+    std::vector<int> kin;                      // rank of the inner (truncated) operator
+    std::vector<int64_t> ein_off, eint_off;    // inner transfers in the FP32 pool
+    std::vector<std::vector<int32_t>> S;       // transfer row set (leaf: box points; else children's Ir)
+    std::vector<int64_t> o_off, u_off;         // vec pool
+    std::vector<int64_t> e_off, et_off;        // mat pool
+};
+
+struct Pair {                                  // semi-nested vertex pair (B <- V) at a level
+    int level, b, v;
+    int rank;
+    std::vector<int32_t> Ir, Ic;
+    std::vector<double> lr;
+    int64_t ac_off, ar_off, t_off;
+};
+
+struct Launch {
+    int64_t item0, nitems, term0, nterms;
+    int role;      // 0 transfer pass, 1 M2L + near field, 2 semi-nested column factors
+    int unr = 4;   // columns in flight per wave (autotuned), FP64 / inner FP32 lists
+    int unr32 = 4; // the same for the stored FP32 copy
+};
+
+struct WorkList {
+    std::vector<GItem> items;
+    std::vector<GTerm> terms;
+    int role = 0;
+};
+
+// Launch lists as uploaded to the device
+struct LaunchSet {
+    std::vector<Launch> launches;
+    std::vector<int64_t> elems;
+    DevBuf<GItem> d_items;
+    DevBuf<GTerm> d_terms;
+};
+
+struct TermSpec {
+    int64_t a, x;
+    int32_t cols;
+};
+
+struct M2LBlk {  // one stored M2L block K(Ir_B, Ir_F): family, level, box, partner, pool offset
+    int f, j, b, p;
+    int64_t off;
+};
+struct NearBlk {  // one stored near-field block K(P(B), P(N))
+    int b, n;
+    int64_t off;
+};
+
+
+inline void gen_rot(double dx, double dy, double &cs, double &sn) {
+    if (dy == 0.0) {
+        cs = 1.0;
+        sn = 0.0;
+    } else if (std::fabs(dy) > std::fabs(dx)) {
+        double t = dx / dy;
+        sn = 1.0 / std::sqrt(1.0 + t * t);
+        cs = t * sn;
+    } else {
+        double t = dy / dx;
+        cs = 1.0 / std::sqrt(1.0 + t * t);
+        sn = t * cs;
+    }
+}
+inline void app_rot(double &dx, double &dy, double cs, double sn) {
+    double t = cs * dx + sn * dy;
+    dy = -sn * dx + cs * dy;
+    dx = t;
+}
+
+}  // namespace h2w
+
+struct h2w_handle {
+    int dim = 3, variant = 0, choice = 0, tol_pow = 10, n_max = 64;
+    int64_t N = 0;
+    double L = 1.0;
+    h2w_opts opts;
+    h2w::Plan plan;
+    bool assembled = false;
+    double diag = 0.0;
+    double memory = 0.0;
+    double assembly_seconds = 0.0;
+    int aca_reruns = 0;
+    hipStream_t stream = nullptr;
+    h2w::KParams kp;
+    h2w::DevBuf<double> d_px, d_py, d_pz;
+    h2w::DevBuf<int32_t> d_perm;
+    h2w::DevBuf<double> mat64;
+    h2w::DevBuf<float> mat32;
+    int64_t mat_len = 0;
+    h2w::DevBuf<double> vec64;
+    h2w::DevBuf<float> vec32;
+    h2w::DevBuf<double> vec64m;            // the vector pool with kNR right-hand sides interleaved
+    h2w::DevBuf<float> vec32m;
+    h2w::DevBuf<float> sx32, sy32, sz32;   // slot coordinates (FP32)
+    h2w::DevBuf<float> sxl, syl, szl;      // FP32 remainders of the slot coordinates
+    h2w::DevBuf<double> sx64, sy64, sz64;  // slot coordinates (FP64)
+    h2w::DevBuf<int32_t> spid;             // slot point ids
+    h2w::SlotPoints sp;
+    int64_t vec_len = 0, off_x = 0, off_y = 0;
+    std::vector<h2w::FamLevel> fam[2];
+    std::vector<h2w::Pair> pairs;
+    std::vector<h2w::Launch> launches;
+    std::vector<int64_t> launch_elems;  // operator elements each launch streams
+    std::vector<h2w::M2LBlk> m2l;
+    std::vector<h2w::NearBlk> nearb;
+    h2w::DevBuf<float> mat32in;         // the inner operator's transfers and compact M2L blocks
+    h2w::LaunchSet inner;               // the refinement's inner operator (ranks truncated)
+    hipGraphExec_t graph[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};  // one per precision
+    hipGraphExec_t graph_m[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};  // the same, kNR right-hand sides
+    bool use_graph = true;
+    int64_t inner_elems = 0;
+    h2w::DevBuf<h2w::GItem> d_items;
+    h2w::DevBuf<h2w::GTerm> d_terms;
+    int64_t stored_elems = 0;
+    // staging for host-pointer calls and GMRES
+    h2w::DevBuf<double> d_stage_in, d_stage_out, d_dense;
+    h2w::DevBuf<double> g_V, g_w, g_r, g_b, g_x, g_h, g_part, g_d;
+    int64_t g_Vcols = 0;
+    // kNR-wide GMRES (interleaved N x kNR vectors)
+    h2w::DevBuf<double> m_V, m_w, m_r, m_x, m_d, m_b, m_h, m_part, m_s;
+    int64_t m_Vcols = 0;
+};
+
+namespace h2w {
+
+using Handle = h2w_handle;
+
+// matvec runtime (h2w_ops.hip)
+bool is32(int precision);
+const std::vector<Launch> &launches_for(Handle &h, int precision);
+void drop_graphs(Handle &h);
+void drop_graph(Handle &h, int precision);
+void matvec_dev(Handle &h, const double *x, double *y, int precision, bool tree);
+
+// kNR-wide runtime (multi.hip)
+// Y = A X for nrhs <= kNR columns (column stride ldx / ldy), device pointers; tree = tree order
+void matvec_multi_dev(Handle &h, const double *X, int64_t ldx, double *Y, int64_t ldy, int nrhs, int precision,
+                      bool tree);
+// nrhs <= kNR GMRES solves / FP64 refinements on interleaved tree-order B and X (device)
+void solve_tree_m(Handle &h, const double *B, double *X, int nrhs, int max_iter, double tol, int mode,
+                  h2w_gmres_stats *st);
+
+}  // namespace h2w

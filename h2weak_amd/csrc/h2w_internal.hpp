@@ -131,6 +131,7 @@ struct GItem {
 };
 
 constexpr int kRowTile = 128;  // rows of one GEMV item
+constexpr int kNR = 16;        // right-hand sides of one multi-vector pass
 
 // ------------------------------------------------------------------ launch wrappers (kernels.hip)
 void launch_aca(const AcaJob *jobs, int njobs, const int32_t *idx, double *ws, int64_t ws_cap,
@@ -175,6 +176,28 @@ void launch_axpby(double a, const double *x, double b, double *y, int64_t n, hip
 void launch_scal(double a, double *y, int64_t n, hipStream_t s);  // y *= a
 void launch_vec_update(const double *V, int64_t ldv, int k, const double *c, double *x, int64_t n, hipStream_t s);
 
+// kNR right-hand sides: vector pools and work vectors hold element (slot, r) at slot * kNR + r
+void launch_bgemm64(const GItem *items, int nitems, const GTerm *terms, const double *A0, const double *A1,
+                    double *vec, int role, hipStream_t s);
+void launch_bgemm32(const GItem *items, int nitems, const GTerm *terms, const float *A0, const float *A1, float *vec,
+                    int role, hipStream_t s);
+// dst(t, r) = X(perm[t] + r ldx) for r < nrhs, 0 beyond (perm == NULL: identity)
+void launch_gather_m64(const double *X, int64_t ldx, int nrhs, const int32_t *perm, int64_t N, double *dst,
+                       hipStream_t s);
+void launch_gather_m32(const double *X, int64_t ldx, int nrhs, const int32_t *perm, int64_t N, float *dst,
+                       hipStream_t s);
+void launch_scatter_m64(const double *src, int64_t N, double *Y, int64_t ldy, int nrhs, const int32_t *perm,
+                        hipStream_t s);
+void launch_scatter_m32(const float *src, int64_t N, double *Y, int64_t ldy, int nrhs, const int32_t *perm,
+                        hipStream_t s);
+int mdot16_blocks(int64_t n);
+void launch_mdot16(const double *V, int64_t ldv, int k, const double *w, int64_t n, double *partial, int nblk,
+                   hipStream_t s);
+void launch_mdot16_reduce(const double *partial, int nblk, int k, double *out, hipStream_t s);
+void launch_mgemv16(const double *V, int64_t ldv, int k, const double *h, double sign, double *w, int64_t n,
+                    hipStream_t s);
+void launch_scale16(const double *src, const double *sc, double *dst, int64_t n, hipStream_t s);
+void launch_axpy16(const double *a, const double *x, double *y, int64_t n, hipStream_t s);
 int mdot_blocks(int64_t n);
 
 }  // namespace h2w

@@ -25,175 +25,17 @@
 #include <string>
 #include <vector>
 
-#include "h2w_internal.hpp"
+#include "h2w_handle.hpp"
 
 namespace h2w {
 
 static thread_local std::string g_err;
 void set_error(const std::string &m) { g_err = m; }
 
-namespace {
-
-constexpr double kPI = 3.141592653589793238462643383279502884;
-
-double now_s() {
-    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
-}
-
-template <typename T>
-struct DevBuf {
-    T *p = nullptr;
-    size_t n = 0;
-    DevBuf() {}
-    DevBuf(const DevBuf &) = delete;
-    DevBuf &operator=(const DevBuf &) = delete;
-    ~DevBuf() { release(); }
-    void release() {
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        n = 0;
-    }
-    void alloc(size_t cnt) {
-        release();
-        if (cnt == 0) return;
-        hipError_t e = hipMalloc(&p, cnt * sizeof(T));
-        if (e != hipSuccess) {
-            p = nullptr;
-            throw Error{H2W_ENOMEM, "hipMalloc of " + std::to_string(cnt * sizeof(T)) + " bytes failed"};
-        }
-        n = cnt;
-    }
-    void ensure(size_t cnt) {
-        if (cnt > n) alloc(cnt);
-    }
-    void upload(const std::vector<T> &v, hipStream_t s) {
-        ensure(v.size());
-        if (!v.empty()) H2W_HIP(hipMemcpyAsync(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s));
-    }
-};
-
-int64_t align_up(int64_t x, int64_t a) { return (x + a - 1) / a * a; }
-
-// one ACA problem: rows/cols are tree-order point indices
-struct AcaBlock {
-    std::vector<int32_t> rows, cols;
-    int rank = 0;
-    std::vector<int32_t> rb, cb;  // local pivot indices
-    std::vector<double> lr;       // L then R, rank x rank column-major each
-    std::vector<double> hist;     // per step: |u||v| and normS after the step
-};
-
-// skeleton data of one family at one level
-struct FamLevel {
-    int nb = 0;
-    std::vector<int> rank;
-    std::vector<std::vector<int32_t>> Ir, Ic;  // tree-order indices
-    std::vector<std::vector<double>> lr;
-    std::vector<std::vector<double>> hist;     // ACA history (|u||v|, normS) per step
-    std::vector<int> kin;                      // rank of the inner (truncated) operator
-    std::vector<int64_t> ein_off, eint_off;    // inner transfers in the FP32 pool
-    std::vector<std::vector<int32_t>> S;       // transfer row set (leaf: box points; else children's Ir)
-    std::vector<int64_t> o_off, u_off;         // vec pool
-    std::vector<int64_t> e_off, et_off;        // mat pool
-};
-
-struct Pair {                                  // semi-nested vertex pair (B <- V) at a level
-    int level, b, v;
-    int rank;
-    std::vector<int32_t> Ir, Ic;
-    std::vector<double> lr;
-    int64_t ac_off, ar_off, t_off;
-};
-
-struct Launch {
-    int64_t item0, nitems, term0, nterms;
-    int role;      // 0 transfer pass, 1 M2L + near field, 2 semi-nested column factors
-    int unr = 4;   // columns in flight per wave (autotuned), FP64 / inner FP32 lists
-    int unr32 = 4; // the same for the stored FP32 copy
-};
-
-struct WorkList {
-    std::vector<GItem> items;
-    std::vector<GTerm> terms;
-    int role = 0;
-};
-
-// Launch lists as uploaded to the device
-struct LaunchSet {
-    std::vector<Launch> launches;
-    std::vector<int64_t> elems;
-    DevBuf<GItem> d_items;
-    DevBuf<GTerm> d_terms;
-};
-
-struct TermSpec {
-    int64_t a, x;
-    int32_t cols;
-};
-
-struct M2LBlk {  // one stored M2L block K(Ir_B, Ir_F): family, level, box, partner, pool offset
-    int f, j, b, p;
-    int64_t off;
-};
-struct NearBlk {  // one stored near-field block K(P(B), P(N))
-    int b, n;
-    int64_t off;
-};
-
-}  // namespace
-
 }  // namespace h2w
-
-struct h2w_handle {
-    int dim = 3, variant = 0, choice = 0, tol_pow = 10, n_max = 64;
-    int64_t N = 0;
-    double L = 1.0;
-    h2w_opts opts;
-    h2w::Plan plan;
-    bool assembled = false;
-    double diag = 0.0;
-    double memory = 0.0;
-    double assembly_seconds = 0.0;
-    int aca_reruns = 0;
-    hipStream_t stream = nullptr;
-    h2w::KParams kp;
-    h2w::DevBuf<double> d_px, d_py, d_pz;
-    h2w::DevBuf<int32_t> d_perm;
-    h2w::DevBuf<double> mat64;
-    h2w::DevBuf<float> mat32;
-    int64_t mat_len = 0;
-    h2w::DevBuf<double> vec64;
-    h2w::DevBuf<float> vec32;
-    h2w::DevBuf<float> sx32, sy32, sz32;   // slot coordinates (FP32)
-    h2w::DevBuf<float> sxl, syl, szl;      // FP32 remainders of the slot coordinates
-    h2w::DevBuf<double> sx64, sy64, sz64;  // slot coordinates (FP64)
-    h2w::DevBuf<int32_t> spid;             // slot point ids
-    h2w::SlotPoints sp;
-    int64_t vec_len = 0, off_x = 0, off_y = 0;
-    std::vector<h2w::FamLevel> fam[2];
-    std::vector<h2w::Pair> pairs;
-    std::vector<h2w::Launch> launches;
-    std::vector<int64_t> launch_elems;  // operator elements each launch streams
-    std::vector<h2w::M2LBlk> m2l;
-    std::vector<h2w::NearBlk> nearb;
-    h2w::DevBuf<float> mat32in;         // the inner operator's transfers and compact M2L blocks
-    h2w::LaunchSet inner;               // the refinement's inner operator (ranks truncated)
-    hipGraphExec_t graph[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};  // one per precision
-    bool use_graph = true;
-    int64_t inner_elems = 0;
-    h2w::DevBuf<h2w::GItem> d_items;
-    h2w::DevBuf<h2w::GTerm> d_terms;
-    int64_t stored_elems = 0;
-    // staging for host-pointer calls and GMRES
-    h2w::DevBuf<double> d_stage_in, d_stage_out, d_dense;
-    h2w::DevBuf<double> g_V, g_w, g_r, g_b, g_x, g_h, g_part, g_d;
-    int64_t g_Vcols = 0;
-};
 
 namespace h2w {
 namespace {
-
-using Handle = h2w_handle;
 
 int ref_nlevels(int dim, int64_t N, int n_max, bool grid) {
     if (grid && dim > 1) {
@@ -707,7 +549,6 @@ void upload_lists(std::vector<WorkList> &WL, LaunchSet &LS, hipStream_t s) {
 // recomputed from the leading blocks of L and R.  Lives in its own FP32 buffer
 // (work items flagged with pool 1); rebuilt by h2w_set_inner_tolerance and the
 // autotuner.
-void drop_graph(Handle &h, int precision);
 
 void build_inner(Handle &h, int irp) {
     drop_graph(h, H2W_FP32_INNER);
@@ -992,7 +833,6 @@ void build_inner(Handle &h, int irp) {
 }
 
 // ------------------------------------------------------------------ assembly
-void drop_graphs(Handle &h);
 
 void assemble(Handle &h) {
     drop_graphs(h);
@@ -1433,6 +1273,8 @@ void assemble(Handle &h) {
 
 // ------------------------------------------------------------------ matvec
 // y_tree = A x_tree inside the vector pool (x already at off_x)
+}  // namespace
+
 bool is32(int precision) { return precision == H2W_FP32 || precision == H2W_FP32_OTF || precision == H2W_FP32_INNER; }
 
 void run_launch(Handle &h, const Launch &Lh, int precision) {
@@ -1504,19 +1346,17 @@ void run_launches(Handle &h, int precision) {
     H2W_HIP(hipGraphLaunch(h.graph[precision], h.stream));
 }
 
-void drop_graphs(Handle &h) {
-    for (auto &g : h.graph)
-        if (g) {
-            (void)hipGraphExecDestroy(g);
-            g = nullptr;
+// graphs of the single-vector and the kNR-wide matvec
+void drop_graph(Handle &h, int precision) {
+    for (hipGraphExec_t *g : {&h.graph[precision], &h.graph_m[precision]})
+        if (*g) {
+            (void)hipGraphExecDestroy(*g);
+            *g = nullptr;
         }
 }
 
-void drop_graph(Handle &h, int precision) {
-    if (h.graph[precision]) {
-        (void)hipGraphExecDestroy(h.graph[precision]);
-        h.graph[precision] = nullptr;
-    }
+void drop_graphs(Handle &h) {
+    for (int p = 0; p < 5; ++p) drop_graph(h, p);
 }
 
 // device-pointer matvec on one vector; tree-ordered in/out when tree == true
@@ -1540,7 +1380,6 @@ void matvec_dev(Handle &h, const double *x, double *y, int precision, bool tree)
     H2W_HIP(hipGetLastError());
 }
 
-}  // namespace
 }  // namespace h2w
 
 namespace h2w {
@@ -1597,25 +1436,6 @@ double dev_dot(Handle &h, const double *a, const double *b) {
     return r;
 }
 
-void gen_rot(double dx, double dy, double &cs, double &sn) {
-    if (dy == 0.0) {
-        cs = 1.0;
-        sn = 0.0;
-    } else if (std::fabs(dy) > std::fabs(dx)) {
-        double t = dx / dy;
-        sn = 1.0 / std::sqrt(1.0 + t * t);
-        cs = t * sn;
-    } else {
-        double t = dy / dx;
-        cs = 1.0 / std::sqrt(1.0 + t * t);
-        sn = t * cs;
-    }
-}
-void app_rot(double &dx, double &dy, double cs, double sn) {
-    double t = cs * dx + sn * dy;
-    dy = -sn * dx + cs * dy;
-    dx = t;
-}
 
 // Solves A x = b (tree order, device).  x_zero: x0 = 0 and no initial matvec.
 // normb_in > 0 overrides ||b|| in the stopping test.
@@ -1644,8 +1464,9 @@ GmresOut gmres_core(Handle &h, const double *b, double *x, int max_iter, double 
         return out;
     }
     const int m = max_iter;
-    std::vector<double> H((size_t)(m + 1) * std::max(m, 1), 0.0), sv(m + 1, 0.0), cs(m + 1, 0.0), sn(m + 1, 0.0);
-    auto Hh = [&](int i, int j) -> double & { return H[(size_t)j * (m + 1) + i]; };
+    std::vector<std::vector<double>> H;  // Hessenberg columns as they are produced (rows 0..j+1)
+    std::vector<double> sv(m + 1, 0.0), cs(m + 1, 0.0), sn(m + 1, 0.0);
+    auto Hh = [&](int i, int j) -> double & { return H[j][i]; };
     ensure_V(h, std::min<int64_t>(m + 1, 32));
     launch_axpby(1.0 / beta, r, 0.0, h.g_V.p, N, s);
     sv[0] = beta;
@@ -1669,6 +1490,7 @@ GmresOut gmres_core(Handle &h, const double *b, double *x, int max_iter, double 
         launch_mdot_reduce(h.g_part.p, nb, 1, h.g_h.p + 2 * k, s);
         H2W_HIP(hipMemcpyAsync(hh.data(), h.g_h.p, (2 * k + 1) * 8, hipMemcpyDeviceToHost, s));
         H2W_HIP(hipStreamSynchronize(s));
+        H.emplace_back(i + 2, 0.0);
         for (int q = 0; q < k; ++q) Hh(q, i) = hh[q] + hh[k + q];
         Hh(i + 1, i) = std::sqrt(hh[2 * k]);
         launch_axpby(1.0 / Hh(i + 1, i), w, 0.0, V + (int64_t)(i + 1) * N, N, s);
@@ -2028,6 +1850,27 @@ int h2w_matvec_ex(h2w_handle *h, const double *x, double *y, int nrhs, int64_t l
     if (precision < H2W_FP64 || precision > H2W_FP32_INNER) throw Error{H2W_EINVAL, "bad precision"};
     H2W_HIP(hipSetDevice(h->opts.device));
     const int64_t N = h->N;
+    if (nrhs > 1 && (precision == H2W_FP64 || precision == H2W_FP32 || precision == H2W_FP32_INNER)) {
+        // kNR right-hand sides per pass on the matrix cores
+        const double *xs = x;
+        double *ys = y;
+        h2w::DevBuf<double> hx, hy;
+        if (where == H2W_HOST) {
+            hx.alloc((size_t)ld * nrhs);
+            hy.alloc((size_t)ld * nrhs);
+            H2W_HIP(hipMemcpyAsync(hx.p, x, (size_t)ld * nrhs * 8, hipMemcpyHostToDevice, h->stream));
+            xs = hx.p;
+            ys = hy.p;
+        }
+        for (int c0 = 0; c0 < nrhs; c0 += h2w::kNR)
+            h2w::matvec_multi_dev(*h, xs + c0 * ld, ld, ys + c0 * ld, ld, std::min(h2w::kNR, nrhs - c0), precision,
+                                  order == H2W_ORDER_TREE);
+        if (where == H2W_HOST)
+            for (int c = 0; c < nrhs; ++c)
+                H2W_HIP(hipMemcpyAsync(y + c * ld, hy.p + c * ld, N * 8, hipMemcpyDeviceToHost, h->stream));
+        H2W_HIP(hipStreamSynchronize(h->stream));
+        return H2W_OK;
+    }
     for (int r = 0; r < nrhs; ++r) {
         const double *xr = x + r * ld;
         double *yr = y + r * ld;
@@ -2088,6 +1931,51 @@ int h2w_gmres(h2w_handle *h, const double *b, double *x, int max_iter, double to
     H2W_HIP(hipStreamSynchronize(s));
     if (st) *st = S;
     return S.converged ? H2W_OK : H2W_ENOCONV;
+    H2W_CATCH
+}
+
+int h2w_gmres_multi(h2w_handle *h, const double *B, int64_t ldb, double *X, int64_t ldx, int nrhs, int max_iter,
+                    double tol, int mode, int where, h2w_gmres_stats *st) {
+    H2W_TRY
+    if (!h || !B || !X || !st) throw Error{H2W_EINVAL, "NULL argument"};
+    if (!h->assembled) throw Error{H2W_ESTATE, "h2w_gmres_multi before h2w_assemble"};
+    if (max_iter < 1 || nrhs < 1 || ldb < h->N || ldx < h->N) throw Error{H2W_EINVAL, "bad max_iter / nrhs / ld"};
+    if (mode != H2W_GMRES_FP64 && mode != H2W_GMRES_FP32_IR && mode != H2W_GMRES_FP32_INNER_IR)
+        throw Error{H2W_EINVAL, "several right-hand sides: modes FP64, FP32_IR and FP32_INNER_IR"};
+    if (mode != H2W_GMRES_FP64 && !h->mat32.p) throw Error{H2W_ESTATE, "FP32 shadow not built"};
+    if (mode == H2W_GMRES_FP32_INNER_IR && h->inner.launches.empty())
+        throw Error{H2W_ESTATE, "the inner operator was not built (opts.ir_tol_pow)"};
+    H2W_HIP(hipSetDevice(h->opts.device));
+    const int64_t N = h->N;
+    hipStream_t s = h->stream;
+    const int64_t len = N * h2w::kNR;
+    h2w::DevBuf<double> hb, hx, bt, xt;
+    const double *bs = B;
+    double *xs = X;
+    if (where == H2W_HOST) {
+        hb.alloc((size_t)ldb * nrhs);
+        hx.alloc((size_t)ldx * nrhs);
+        H2W_HIP(hipMemcpyAsync(hb.p, B, (size_t)ldb * nrhs * 8, hipMemcpyHostToDevice, s));
+        H2W_HIP(hipMemcpyAsync(hx.p, X, (size_t)ldx * nrhs * 8, hipMemcpyHostToDevice, s));
+        bs = hb.p;
+        xs = hx.p;
+    }
+    bt.alloc(len);
+    xt.alloc(len);
+    for (int c0 = 0; c0 < nrhs; c0 += h2w::kNR) {
+        const int nc = std::min(h2w::kNR, nrhs - c0);
+        h2w::launch_gather_m64(bs + c0 * ldb, ldb, nc, h->d_perm.p, N, bt.p, s);
+        h2w::launch_gather_m64(xs + c0 * ldx, ldx, nc, h->d_perm.p, N, xt.p, s);
+        h2w::solve_tree_m(*h, bt.p, xt.p, nc, max_iter, tol, mode, st + c0);
+        h2w::launch_scatter_m64(xt.p, N, xs + c0 * ldx, ldx, nc, h->d_perm.p, s);
+    }
+    if (where == H2W_HOST)
+        for (int c = 0; c < nrhs; ++c)
+            H2W_HIP(hipMemcpyAsync(X + c * ldx, hx.p + c * ldx, N * 8, hipMemcpyDeviceToHost, s));
+    H2W_HIP(hipStreamSynchronize(s));
+    for (int c = 0; c < nrhs; ++c)
+        if (!st[c].converged) return H2W_ENOCONV;
+    return H2W_OK;
     H2W_CATCH
 }
 

@@ -554,6 +554,138 @@ __global__ __launch_bounds__(256) void k_bgemv(const GItem *__restrict__ items, 
     }
 }
 
+// ------------------------------------------------------------------ batched skinny GEMM, kNR right-hand sides
+// The same work items as k_bgemv, on a vector pool holding kNR right-hand
+// sides per slot (element (slot, rhs) at slot * kNR + rhs): every item is a
+// small dense product Y[rows x kNR] += A[rows x cols] X[cols x kNR] on the
+// matrix cores (16x16x4 MFMA, FP32 or FP64).
+//
+// Lane l (m = l % 16, q = l / 16) reads VEC consecutive rows of column c0 + q
+// with one 16-byte load: rows VEC m .. VEC m + VEC-1 of a 16 VEC-row group (a
+// quarter-wave reads 256 contiguous bytes of a column).  Component u of that
+// vector is the MFMA's A(m, q) for the strided row set {VEC m + u}, so VEC
+// MFMAs consume one load and X(c0 + q, m) -- kNR contiguous elements per
+// column -- is the B operand of all of them.  An MFMA's D(row(q, v), m) then
+// belongs to item row  group * 16 VEC + VEC row(q, v) + u, where row(q, v) =
+// 4q + v (FP32) or q + 4v (FP64) are the two instructions' result layouts,
+// checked on gfx950.  The four waves take interleaved 4-column slices, two in
+// flight at a time; their partial products are summed through LDS by wave 0.
+template <typename T>
+struct Mfma16;
+template <>
+struct Mfma16<float> {
+    typedef float v4 __attribute__((ext_vector_type(4)));
+    static __device__ __forceinline__ v4 op(float a, float b, v4 c) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int q, int v) { return 4 * q + v; }
+};
+template <>
+struct Mfma16<double> {
+    typedef double v4 __attribute__((ext_vector_type(4)));
+    static __device__ __forceinline__ v4 op(double a, double b, v4 c) {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int q, int v) { return q + 4 * v; }
+};
+
+template <typename T, int ROLE>
+__global__ __launch_bounds__(256) void k_bgemm(const GItem *__restrict__ items, const GTerm *__restrict__ terms,
+                                               const T *__restrict__ A0, const T *__restrict__ A1, T *vec) {
+    typedef typename Mfma16<T>::v4 V4;
+    constexpr int VEC = 16 / sizeof(T);     // rows per lane load
+    constexpr int RG = 16 * VEC;            // rows of one load group
+    constexpr int NG = kRowTile / RG;       // load groups of an item
+    constexpr int NA = NG * VEC;            // accumulators
+    constexpr int UNR = 2;                  // 4-column slices in flight per wave
+    typedef T VL __attribute__((ext_vector_type(VEC)));
+    const GItem it = items[blockIdx.x];
+    const T *__restrict__ A = (it.flags & 2) ? A1 : A0;
+    const int lane = threadIdx.x & 63;
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int rows = it.rows;
+    const int ng = (rows + RG - 1) / RG;
+    const int m = lane & 15, q = lane >> 4;
+    V4 acc[NA];
+#pragma unroll
+    for (int a = 0; a < NA; ++a) acc[a] = (V4)0;
+    const int64_t lda = it.lda;
+    for (int tt = it.t0; tt < it.t1; ++tt) {
+        const GTerm tm = terms[tt];
+        const T *a = A + tm.a + it.row0 + VEC * m;
+        const T *x = vec + tm.x * kNR + m;
+        const int cols = tm.cols;
+        for (int c0 = 4 * wv; c0 < cols; c0 += 16 * UNR) {
+            VL av[UNR][NG];
+            T bx[UNR];
+#pragma unroll
+            for (int u = 0; u < UNR; ++u) {
+                const int c = c0 + 16 * u + q;
+                const bool cin = c < cols;
+                bx[u] = cin ? x[(int64_t)c * kNR] : T(0);
+                const T *ac = a + (int64_t)c * lda;
+#pragma unroll
+                for (int g = 0; g < NG; ++g)
+                    av[u][g] = (g < ng && cin && RG * g + VEC * m < rows)
+                                   ? __builtin_nontemporal_load((const VL *)(ac + RG * g))
+                                   : (VL)0;
+            }
+#pragma unroll
+            for (int u = 0; u < UNR; ++u)
+#pragma unroll
+                for (int g = 0; g < NG; ++g)
+                    if (g < ng)
+#pragma unroll
+                        for (int e = 0; e < VEC; ++e) acc[g * VEC + e] = Mfma16<T>::op(av[u][g][e], bx[u], acc[g * VEC + e]);
+        }
+    }
+    // item row held by accumulator (g, e), result element v of this lane
+    auto row_of = [&](int g, int e, int v) { return RG * g + VEC * Mfma16<T>::row(q, v) + e; };
+    // sum the four waves' partial products: 3 and 2 into 1 and 0, then 1 into 0
+    __shared__ T red[2][kRowTile][kNR];
+    auto put = [&](int slot) {
+#pragma unroll
+        for (int g = 0; g < NG; ++g)
+            if (g < ng)
+#pragma unroll
+                for (int e = 0; e < VEC; ++e)
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) red[slot][row_of(g, e, v)][m] = acc[g * VEC + e][v];
+    };
+    auto add = [&](int slot) {
+#pragma unroll
+        for (int g = 0; g < NG; ++g)
+            if (g < ng)
+#pragma unroll
+                for (int e = 0; e < VEC; ++e)
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) acc[g * VEC + e][v] += red[slot][row_of(g, e, v)][m];
+    };
+    if (wv >= 2) put(wv - 2);
+    __syncthreads();
+    if (wv < 2) add(wv);
+    __syncthreads();
+    if (wv == 1) put(0);
+    __syncthreads();
+    if (wv == 0) {
+        add(0);
+        T *y = vec + it.y * kNR + m;
+#pragma unroll
+        for (int g = 0; g < NG; ++g)
+            if (g < ng)
+#pragma unroll
+                for (int e = 0; e < VEC; ++e)
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) {
+                        const int r = row_of(g, e, v);
+                        if (r < rows) {
+                            if (it.flags & 1) atomicAdd(y + (int64_t)r * kNR, acc[g * VEC + e][v]);
+                            else y[(int64_t)r * kNR] += acc[g * VEC + e][v];
+                        }
+                    }
+    }
+}
+
 // leading sub-block copies from the FP32 pool into the inner-operator buffer
 __global__ void k_copy32(const CopyJob *__restrict__ jobs, const float *__restrict__ src, float *dst) {
     const CopyJob jb = jobs[blockIdx.x];
@@ -802,6 +934,91 @@ __global__ void k_scal(double a, double *y, int64_t n) {
         y[i] *= a;
 }
 
+
+// ------------------------------------------------------------------ kNR-wide vectors (interleaved N x kNR)
+// element (t, r) at t * kNR + r; one dot product, norm or update per column r
+template <typename T>
+__global__ void k_gather_m(const double *__restrict__ X, int64_t ldx, int nrhs, const int32_t *__restrict__ perm,
+                           int64_t N, T *__restrict__ dst) {
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < N * kNR; e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t t = e / kNR;
+        const int r = (int)(e % kNR);
+        const int64_t src = perm ? perm[t] : t;
+        dst[e] = r < nrhs ? (T)X[src + r * ldx] : T(0);
+    }
+}
+template <typename T>
+__global__ void k_scatter_m(const T *__restrict__ src, int64_t N, double *__restrict__ Y, int64_t ldy, int nrhs,
+                            const int32_t *__restrict__ perm) {
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < N * kNR; e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t t = e / kNR;
+        const int r = (int)(e % kNR);
+        if (r < nrhs) Y[(perm ? perm[t] : t) + r * ldy] = (double)src[e];
+    }
+}
+// partial[(j * nblk + b) * kNR + r] = sum over block b's slots of V_j(t, r) w(t, r)
+__global__ __launch_bounds__(256) void k_mdot16(const double *__restrict__ V, int64_t ldv,
+                                                const double *__restrict__ w, int64_t n,
+                                                double *__restrict__ partial) {
+    const int b = blockIdx.x, j = blockIdx.y, nblk = gridDim.x;
+    const double *v = V + (int64_t)j * ldv;
+    const int r = threadIdx.x & 15;
+    double acc = 0.0;
+    for (int64_t e = (int64_t)b * 256 + threadIdx.x; e < n * kNR; e += (int64_t)nblk * 256) acc = fma(v[e], w[e], acc);
+    acc += __shfl_xor(acc, 16, 64);
+    acc += __shfl_xor(acc, 32, 64);
+    __shared__ double sh[4][kNR];
+    if ((threadIdx.x & 63) < 16) sh[threadIdx.x >> 6][r] = acc;
+    __syncthreads();
+    if (threadIdx.x < 16)
+        partial[((int64_t)j * nblk + b) * kNR + r] = sh[0][r] + sh[1][r] + sh[2][r] + sh[3][r];
+}
+__global__ __launch_bounds__(256) void k_mdot16_reduce(const double *__restrict__ partial, int nblk,
+                                                       double *__restrict__ out) {
+    const int j = blockIdx.x;
+    const int r = threadIdx.x & 15, q = threadIdx.x >> 4;
+    double acc = 0.0;
+    for (int b = q; b < nblk; b += 16) acc += partial[((int64_t)j * nblk + b) * kNR + r];
+    acc += __shfl_xor(acc, 16, 64);
+    acc += __shfl_xor(acc, 32, 64);
+    __shared__ double sh[4][kNR];
+    if ((threadIdx.x & 63) < 16) sh[threadIdx.x >> 6][r] = acc;
+    __syncthreads();
+    if (threadIdx.x < 16) out[(int64_t)j * kNR + r] = sh[0][r] + sh[1][r] + sh[2][r] + sh[3][r];
+}
+// w(t, r) += sign * sum_j V_j(t, r) h(j, r)
+__global__ __launch_bounds__(256) void k_mgemv16(const double *__restrict__ V, int64_t ldv, int k,
+                                                 const double *__restrict__ h, double *__restrict__ w, int64_t n,
+                                                 double sign) {
+    __shared__ double hs[256 * kNR];
+    for (int64_t e0 = blockIdx.x * (int64_t)blockDim.x; e0 < n * kNR; e0 += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t e = e0 + threadIdx.x;
+        const int r = threadIdx.x & 15;
+        double acc = 0.0;
+        for (int l0 = 0; l0 < k; l0 += 256) {
+            const int l1 = min(k, l0 + 256);
+            __syncthreads();
+            for (int q = threadIdx.x; q < (l1 - l0) * kNR; q += blockDim.x) hs[q] = h[(int64_t)l0 * kNR + q];
+            __syncthreads();
+            if (e < n * kNR)
+                for (int l = l0; l < l1; ++l) acc = fma(V[(int64_t)l * ldv + e], hs[(l - l0) * kNR + r], acc);
+        }
+        if (e < n * kNR) w[e] += sign * acc;
+    }
+}
+// dst(t, r) = sc(r) * src(t, r)
+__global__ void k_scale16(const double *__restrict__ src, const double *__restrict__ sc, double *__restrict__ dst,
+                          int64_t n) {
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n * kNR; e += (int64_t)gridDim.x * blockDim.x)
+        dst[e] = sc[e % kNR] * src[e];
+}
+// y(t, r) += a(r) * x(t, r)
+__global__ void k_axpy16(const double *__restrict__ a, const double *__restrict__ x, double *__restrict__ y,
+                         int64_t n) {
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n * kNR; e += (int64_t)gridDim.x * blockDim.x)
+        y[e] += a[e % kNR] * x[e];
+}
+
 inline int grid_for(int64_t n, int nt = 256, int cap = 8192) {
     int64_t g = (n + nt - 1) / nt;
     if (g < 1) g = 1;
@@ -941,6 +1158,60 @@ void launch_axpby(double a, const double *x, double b, double *y, int64_t n, hip
 }
 void launch_scal(double a, double *y, int64_t n, hipStream_t s) {
     hipLaunchKernelGGL(k_scal, dim3(grid_for(n)), dim3(256), 0, s, a, y, n);
+}
+
+
+template <typename T>
+static void launch_bgemm_t(const GItem *items, int nitems, const GTerm *terms, const T *A0, const T *A1, T *vec,
+                           int role, hipStream_t s) {
+    if (nitems <= 0) return;
+    if (!A1) A1 = A0;
+    if (role == 1) hipLaunchKernelGGL((k_bgemm<T, 1>), dim3(nitems), dim3(256), 0, s, items, terms, A0, A1, vec);
+    else if (role == 2) hipLaunchKernelGGL((k_bgemm<T, 2>), dim3(nitems), dim3(256), 0, s, items, terms, A0, A1, vec);
+    else hipLaunchKernelGGL((k_bgemm<T, 0>), dim3(nitems), dim3(256), 0, s, items, terms, A0, A1, vec);
+}
+void launch_bgemm64(const GItem *items, int nitems, const GTerm *terms, const double *A0, const double *A1,
+                    double *vec, int role, hipStream_t s) {
+    launch_bgemm_t<double>(items, nitems, terms, A0, A1, vec, role, s);
+}
+void launch_bgemm32(const GItem *items, int nitems, const GTerm *terms, const float *A0, const float *A1, float *vec,
+                    int role, hipStream_t s) {
+    launch_bgemm_t<float>(items, nitems, terms, A0, A1, vec, role, s);
+}
+void launch_gather_m64(const double *X, int64_t ldx, int nrhs, const int32_t *perm, int64_t N, double *dst,
+                       hipStream_t s) {
+    hipLaunchKernelGGL(k_gather_m<double>, dim3(grid_for(N * kNR)), dim3(256), 0, s, X, ldx, nrhs, perm, N, dst);
+}
+void launch_gather_m32(const double *X, int64_t ldx, int nrhs, const int32_t *perm, int64_t N, float *dst,
+                       hipStream_t s) {
+    hipLaunchKernelGGL(k_gather_m<float>, dim3(grid_for(N * kNR)), dim3(256), 0, s, X, ldx, nrhs, perm, N, dst);
+}
+void launch_scatter_m64(const double *src, int64_t N, double *Y, int64_t ldy, int nrhs, const int32_t *perm,
+                        hipStream_t s) {
+    hipLaunchKernelGGL(k_scatter_m<double>, dim3(grid_for(N * kNR)), dim3(256), 0, s, src, N, Y, ldy, nrhs, perm);
+}
+void launch_scatter_m32(const float *src, int64_t N, double *Y, int64_t ldy, int nrhs, const int32_t *perm,
+                        hipStream_t s) {
+    hipLaunchKernelGGL(k_scatter_m<float>, dim3(grid_for(N * kNR)), dim3(256), 0, s, src, N, Y, ldy, nrhs, perm);
+}
+int mdot16_blocks(int64_t n) { return (int)std::max<int64_t>(1, std::min<int64_t>(512, (n * kNR + 4095) / 4096)); }
+void launch_mdot16(const double *V, int64_t ldv, int k, const double *w, int64_t n, double *partial, int nblk,
+                   hipStream_t s) {
+    if (k > 0) hipLaunchKernelGGL(k_mdot16, dim3(nblk, k), dim3(256), 0, s, V, ldv, w, n, partial);
+}
+void launch_mdot16_reduce(const double *partial, int nblk, int k, double *out, hipStream_t s) {
+    if (k > 0) hipLaunchKernelGGL(k_mdot16_reduce, dim3(k), dim3(256), 0, s, partial, nblk, out);
+}
+void launch_mgemv16(const double *V, int64_t ldv, int k, const double *h, double sign, double *w, int64_t n,
+                    hipStream_t s) {
+    if (k > 0)
+        hipLaunchKernelGGL(k_mgemv16, dim3(grid_for(n * kNR, 256, 4096)), dim3(256), 0, s, V, ldv, k, h, w, n, sign);
+}
+void launch_scale16(const double *src, const double *sc, double *dst, int64_t n, hipStream_t s) {
+    hipLaunchKernelGGL(k_scale16, dim3(grid_for(n * kNR)), dim3(256), 0, s, src, sc, dst, n);
+}
+void launch_axpy16(const double *a, const double *x, double *y, int64_t n, hipStream_t s) {
+    hipLaunchKernelGGL(k_axpy16, dim3(grid_for(n * kNR)), dim3(256), 0, s, a, x, y, n);
 }
 
 }  // namespace h2w

@@ -135,9 +135,11 @@ int h2w_assemble(h2w_handle *h);
  * (3D/codes/matvec.hpp:63-83, :115-136; 1D/codes/matvec.hpp:51-65,89-103). */
 int h2w_matvec(h2w_handle *h, const double *x, double *y, int where);
 
-/* General form: precision H2W_FP64, H2W_FP32 (FP32 operators, FP32 accumulation), H2W_FP32_OTF
- * or H2W_FP64_OTF (x and y are FP64 either way), order H2W_ORDER_INPUT or H2W_ORDER_TREE, nrhs right-hand sides
- * stored one after another with leading dimension ld (>= N). */
+/* General form: precision H2W_FP64, H2W_FP32 (FP32 operators, FP32 accumulation), H2W_FP32_INNER,
+ * H2W_FP32_OTF or H2W_FP64_OTF (x and y are FP64 either way), order H2W_ORDER_INPUT or H2W_ORDER_TREE,
+ * nrhs right-hand sides stored one after another with leading dimension ld (>= N).  With nrhs > 1 and a
+ * stored-operator precision the product runs 16 right-hand sides per pass on the matrix cores
+ * (each operator block read once per pass); the on-the-fly precisions loop over the columns. */
 int h2w_matvec_ex(h2w_handle *h, const double *x, double *y, int nrhs, int64_t ld, int precision, int order,
                   int where);
 
@@ -147,6 +149,15 @@ int h2w_matvec_ex(h2w_handle *h, const double *x, double *y, int nrhs, int64_t l
  * Returns H2W_ENOCONV (with x updated) when max_iter is reached first. */
 int h2w_gmres(h2w_handle *h, const double *b, double *x, int max_iter, double tol, int mode, int where,
               h2w_gmres_stats *stats);
+
+/* nrhs independent GMRES solves A x_c = b_c (columns of B and X, leading dimensions ldb, ldx >= N),
+ * 16 at a time in lockstep: every Arnoldi step is one 16-wide product on the matrix cores, each
+ * column keeps its own Krylov basis, Hessenberg matrix, Givens rotations and stopping test (those of
+ * h2w_gmres), and stops at its own iteration.  Modes H2W_GMRES_FP64, H2W_GMRES_FP32_IR and
+ * H2W_GMRES_FP32_INNER_IR.  stats receives nrhs entries.  No counterpart in the reference, whose
+ * driver solves one right-hand side (1D/codes/gmres.hpp:91-200). */
+int h2w_gmres_multi(h2w_handle *h, const double *B, int64_t ldb, double *X, int64_t ldx, int nrhs, int max_iter,
+                    double tol, int mode, int where, h2w_gmres_stats *stats);
 
 /* Dense FP64 product of selected rows of A with x, entries evaluated on the fly on the GPU.
  * rows == NULL means all N rows.  Replaces `getTruePoten` (3D/codes/HODLR3Dtree.hpp:2630-2643). */
@@ -169,6 +180,9 @@ typedef struct {
                                           2 semi-nested column factors                       */
 } h2w_bench;
 int h2w_bench_matvec(h2w_handle *h, int nrep, int precision, h2w_bench *out);
+/* The same for the 16-wide product (matrix cores, 16 right-hand sides per pass): ms_per_matvec is the
+ * time of one pass over all 16 columns (launch_role[0] = -2). */
+int h2w_bench_matvec_multi(h2w_handle *h, int nrep, int precision, h2w_bench *out);
 
 /* Rebuild the truncated FP32 inner operator of H2W_GMRES_FP32_INNER_IR for tolerance
  * 10^-ir_tol_pow (0 < ir_tol_pow < tol_pow; 0 removes it).  No counterpart in the reference
