@@ -35,7 +35,8 @@ GMRES_FP64, GMRES_FP32_IR, GMRES_FP32_OTF_IR, GMRES_FP32_INNER_IR = 0, 1, 2, 3
 EXPORTS = ("h2w_default_opts", "h2w_create", "h2w_assemble", "h2w_matvec", "h2w_matvec_ex", "h2w_gmres",
            "h2w_dense_matvec", "h2w_get_stats", "h2w_tree_order", "h2w_box_skeleton", "h2w_interaction_list",
            "h2w_near_list", "h2w_destroy", "h2w_last_error", "h2w_bench_matvec", "h2w_set_inner_tolerance",
-           "h2w_autotune", "h2w_gmres_multi", "h2w_bench_matvec_multi")
+           "h2w_autotune", "h2w_gmres_multi", "h2w_bench_matvec_multi", "h2w_batch_create", "h2w_batch_matvec",
+           "h2w_batch_gmres", "h2w_batch_bench_matvec", "h2w_batch_destroy")
 TUNE_UNROLL, TUNE_INNER = 1, 2
 
 
@@ -115,6 +116,11 @@ def lib():
     L.h2w_bench_matvec.argtypes = [p, i, i, ctypes.POINTER(BenchResult)]
     L.h2w_set_inner_tolerance.argtypes = [p, i]
     L.h2w_bench_matvec_multi.argtypes = [p, i, i, ctypes.POINTER(BenchResult)]
+    L.h2w_batch_create.argtypes = [ctypes.POINTER(p), i, ctypes.POINTER(p)]
+    L.h2w_batch_matvec.argtypes = [p, p, p, i, i, i]
+    L.h2w_batch_gmres.argtypes = [p, p, p, i, i, d, i, i, ctypes.POINTER(GmresStats)]
+    L.h2w_batch_bench_matvec.argtypes = [p, i, i, ctypes.POINTER(BenchResult)]
+    L.h2w_batch_destroy.argtypes = [p]
     L.h2w_gmres_multi.argtypes = [p, p, i64, p, i64, i, i, d, i, i, ctypes.POINTER(GmresStats)]
     L.h2w_autotune.argtypes = [p, p, i, d, i, i, ctypes.POINTER(Tune)]
     _lib = L
@@ -292,6 +298,64 @@ class Initialisation:
         r = BenchResult()
         _check(lib().h2w_bench_matvec_multi(self._h, int(nrep), precision, ctypes.byref(r)))
         return {"ms_per_pass": r.ms_per_matvec, "nrhs": 16}
+
+
+class Batch:
+    """Independent systems (assembled Initialisation objects on one device) multiplied and
+    solved together: one launch per phase for all of them, 16 right-hand sides per pass.
+    Vectors are lists with one (nrhs, N_s) array per system."""
+
+    def __init__(self, systems):
+        self.systems = list(systems)  # keeps the handles alive as long as the batch
+        arr = (ctypes.c_void_p * len(self.systems))(*[s._h for s in self.systems])
+        h = ctypes.c_void_p()
+        _check(lib().h2w_batch_create(arr, len(self.systems), ctypes.byref(h)))
+        self._h = h
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            lib().h2w_batch_destroy(h)
+            self._h = None
+
+    def _pack(self, Xs):
+        Xs = [np.atleast_2d(np.asarray(x, dtype=np.float64)) for x in Xs]
+        if len(Xs) != len(self.systems) or any(x.shape[1] != s.N for x, s in zip(Xs, self.systems)):
+            raise ValueError("one (nrhs, N_s) array per system")
+        nrhs = Xs[0].shape[0]
+        if any(x.shape[0] != nrhs for x in Xs):
+            raise ValueError("every system takes the same number of right-hand sides")
+        return np.ascontiguousarray(np.concatenate([x.ravel() for x in Xs])), nrhs
+
+    def _unpack(self, flat, nrhs):
+        out, o = [], 0
+        for s in self.systems:
+            out.append(flat[o:o + nrhs * s.N].reshape(nrhs, s.N))
+            o += nrhs * s.N
+        return out
+
+    def matvec(self, Xs, precision=FP64):
+        X, nrhs = self._pack(Xs)
+        Y = np.empty_like(X)
+        _check(lib().h2w_batch_matvec(self._h, _ptr(X), _ptr(Y), nrhs, precision, HOST))
+        return self._unpack(Y, nrhs)
+
+    def gmres(self, Bs, X0s=None, max_iter=500, tol=1e-10, mode=GMRES_FP64):
+        B, nrhs = self._pack(Bs)
+        X = np.ones_like(B) if X0s is None else self._pack(X0s)[0].copy()
+        st = (GmresStats * (len(self.systems) * nrhs))()
+        rc = lib().h2w_batch_gmres(self._h, _ptr(B), _ptr(X), nrhs, int(max_iter), float(tol), mode, HOST, st)
+        if rc not in (H2W_OK, H2W_ENOCONV):
+            _check(rc)
+        stats = [[st[s * nrhs + r].as_dict() for r in range(nrhs)] for s in range(len(self.systems))]
+        return self._unpack(X, nrhs), stats
+
+    def bench_matvec(self, nrep=10, precision=FP64):
+        r = BenchResult()
+        _check(lib().h2w_batch_bench_matvec(self._h, int(nrep), precision, ctypes.byref(r)))
+        n = r.n_launch
+        return {"ms_per_pass": r.ms_per_matvec, "nrhs": 16, "launch_bytes": list(r.launch_bytes[:n]),
+                "launch_role": list(r.launch_role[:n])}
 
 
 class IterativeSolver:

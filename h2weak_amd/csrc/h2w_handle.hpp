@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include <chrono>
+#include <functional>
 #include <cmath>
 #include <string>
 #include <vector>
@@ -86,9 +87,15 @@ struct Pair {                                  // semi-nested vertex pair (B <- 
     int64_t ac_off, ar_off, t_off;
 };
 
+// Position of a launch in the matvec: stage 0 up pass (step = levels above the
+// leaves), 1 M2L + near field, 2 down pass (step = levels above the leaves),
+// 3 semi-nested column factors.  Batches of systems of different depths merge
+// their launches by (stage, step): up steps from the leaves, down steps ending
+// at the leaves.
 struct Launch {
     int64_t item0, nitems, term0, nterms;
     int role;      // 0 transfer pass, 1 M2L + near field, 2 semi-nested column factors
+    int stage = 0, step = 0;
     int unr = 4;   // columns in flight per wave (autotuned), FP64 / inner FP32 lists
     int unr32 = 4; // the same for the stored FP32 copy
 };
@@ -97,12 +104,15 @@ struct WorkList {
     std::vector<GItem> items;
     std::vector<GTerm> terms;
     int role = 0;
+    int stage = 0, step = 0;
 };
 
 // Launch lists as uploaded to the device
 struct LaunchSet {
     std::vector<Launch> launches;
     std::vector<int64_t> elems;
+    std::vector<GItem> h_items;  // host copies (terms ranges absolute), for merging into batches
+    std::vector<GTerm> h_terms;
     DevBuf<GItem> d_items;
     DevBuf<GTerm> d_terms;
 };
@@ -141,6 +151,12 @@ inline void app_rot(double &dx, double &dy, double cs, double sn) {
     dy = -sn * dx + cs * dy;
     dx = t;
 }
+
+// device work buffers of a lockstep solve
+struct LockBufs {
+    DevBuf<double> V, w, r, d, b, part, h, sc;
+    int64_t Vcols = 0;
+};
 
 }  // namespace h2w
 
@@ -184,16 +200,21 @@ struct h2w_handle {
     hipGraphExec_t graph_m[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};  // the same, kNR right-hand sides
     bool use_graph = true;
     int64_t inner_elems = 0;
+    int inner_gen = 0;                  // incremented whenever the inner operator is rebuilt
     h2w::DevBuf<h2w::GItem> d_items;
     h2w::DevBuf<h2w::GTerm> d_terms;
+    std::vector<h2w::GItem> h_items;    // host copies of the main lists
+    std::vector<h2w::GTerm> h_terms;
     int64_t stored_elems = 0;
     // staging for host-pointer calls and GMRES
     h2w::DevBuf<double> d_stage_in, d_stage_out, d_dense;
     h2w::DevBuf<double> g_V, g_w, g_r, g_b, g_x, g_h, g_part, g_d;
     int64_t g_Vcols = 0;
-    // kNR-wide GMRES (interleaved N x kNR vectors)
-    h2w::DevBuf<double> m_V, m_w, m_r, m_x, m_d, m_b, m_h, m_part, m_s;
-    int64_t m_Vcols = 0;
+    // kNR-wide lockstep GMRES (interleaved N x kNR vectors)
+    h2w::LockBufs mbuf;
+    h2w::DevBuf<int2> m_blk;      // dot-product blocks of the one-system lockstep vector
+    h2w::DevBuf<int32_t> m_sblk;
+    int m_nblk = 0;
 };
 
 namespace h2w {
@@ -214,5 +235,27 @@ void matvec_multi_dev(Handle &h, const double *X, int64_t ldx, double *Y, int64_
 // nrhs <= kNR GMRES solves / FP64 refinements on interleaved tree-order B and X (device)
 void solve_tree_m(Handle &h, const double *B, double *X, int nrhs, int max_iter, double tol, int mode,
                   h2w_gmres_stats *st);
+
+// Lockstep solves over S systems x kNR columns on a lockstep vector (n slots, the
+// systems' interleaved tree-order vectors one after another).  Column (s, r) is
+// right-hand side r of system s.
+struct Lockstep {
+    hipStream_t s = nullptr;
+    int64_t n = 0;                    // slots
+    int S = 1;                        // systems
+    const int32_t *seg = nullptr;     // system of every slot (device; NULL: S == 1)
+    const int2 *blk = nullptr;        // dot-product blocks (device): [first, one-past-last) slot
+    int nblk = 0;
+    const int32_t *sblk = nullptr;    // blocks of system s: sblk[s] .. sblk[s+1]-1 (device)
+    std::vector<double> ir_floor;     // per system: 10^-ir_tol_pow of its inner operator
+    std::function<void(const double *, double *, int)> matvec;  // Y = A X, interleaved, a precision
+    LockBufs *buf = nullptr;
+};
+// Split S consecutive slot ranges (system s: [start[s], start[s+1])) into dot-product blocks.
+void lock_blocks(const std::vector<int64_t> &start, std::vector<int2> &blk, std::vector<int32_t> &sblk);
+// nrhs <= kNR right-hand sides per system: B, X lockstep vectors (device); st: S x nrhs,
+// entry (s, r) at s * nrhs + r.  Modes H2W_GMRES_FP64, _FP32_IR, _FP32_INNER_IR.
+void lock_solve(Lockstep &L, const double *B, double *X, int nrhs, int max_iter, double tol, int mode,
+                h2w_gmres_stats *st);
 
 }  // namespace h2w

@@ -190,14 +190,27 @@ void launch_scatter_m64(const double *src, int64_t N, double *Y, int64_t ldy, in
                         hipStream_t s);
 void launch_scatter_m32(const float *src, int64_t N, double *Y, int64_t ldy, int nrhs, const int32_t *perm,
                         hipStream_t s);
-int mdot16_blocks(int64_t n);
-void launch_mdot16(const double *V, int64_t ldv, int k, const double *w, int64_t n, double *partial, int nblk,
+// batched systems: the operator pools of system f >> 8 of an item come from tab[2 f + 0/1]
+void launch_bgemm64_tab(const GItem *items, int nitems, const GTerm *terms, const double *const *tab, double *vec,
+                        int role, hipStream_t s);
+void launch_bgemm32_tab(const GItem *items, int nitems, const GTerm *terms, const float *const *tab, float *vec,
+                        int role, hipStream_t s);
+void launch_map_in64(const double *X, const int32_t *map, int64_t n, double *pool, hipStream_t s);
+void launch_map_in32(const double *X, const int32_t *map, int64_t n, float *pool, hipStream_t s);
+void launch_map_out64(const double *pool, const int32_t *map, int64_t n, double *Y, hipStream_t s);
+void launch_map_out32(const float *pool, const int32_t *map, int64_t n, double *Y, hipStream_t s);
+void launch_gather_b(const double *X, const int64_t *src, const int32_t *ld, int nrhs, int64_t n, double *dst,
+                     hipStream_t s);
+void launch_scatter_b(const double *v, const int64_t *src, const int32_t *ld, int nrhs, int64_t n, double *Y,
+                      hipStream_t s);
+// lockstep solves over S systems x kNR columns (segmented interleaved vectors, seg == NULL: S == 1)
+void launch_bdot16(const double *V, int64_t ldv, int k, const double *w, const int2 *blk, int nblk, double *partial,
                    hipStream_t s);
-void launch_mdot16_reduce(const double *partial, int nblk, int k, double *out, hipStream_t s);
-void launch_mgemv16(const double *V, int64_t ldv, int k, const double *h, double sign, double *w, int64_t n,
-                    hipStream_t s);
-void launch_scale16(const double *src, const double *sc, double *dst, int64_t n, hipStream_t s);
-void launch_axpy16(const double *a, const double *x, double *y, int64_t n, hipStream_t s);
+void launch_bdot16_reduce(const double *partial, int nblk, int k, const int32_t *sblk, int S, double *out,
+                          hipStream_t s);
+void launch_bgemv16(const double *V, int64_t ldv, int k, const double *h, int S, double sign, double *w, int64_t n,
+                    const int32_t *seg, hipStream_t s);
+void launch_bscale16(const double *src, const double *sc, double *dst, int64_t n, const int32_t *seg, hipStream_t s);
 int mdot_blocks(int64_t n);
 
 }  // namespace h2w

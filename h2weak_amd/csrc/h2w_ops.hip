@@ -527,6 +527,8 @@ void upload_lists(std::vector<WorkList> &WL, LaunchSet &LS, hipStream_t s) {
         Lh.term0 = (int64_t)terms.size();
         Lh.nterms = (int64_t)W.terms.size();
         Lh.role = W.role;
+        Lh.stage = W.stage;
+        Lh.step = W.step;
         int64_t e = 0;
         for (auto it : W.items) {
             for (int q = it.t0; q < it.t1; ++q) e += (int64_t)it.rows * W.terms[q].cols;
@@ -540,6 +542,8 @@ void upload_lists(std::vector<WorkList> &WL, LaunchSet &LS, hipStream_t s) {
     }
     LS.d_items.upload(items, s);
     LS.d_terms.upload(terms, s);
+    LS.h_items = std::move(items);
+    LS.h_terms = std::move(terms);
 }
 
 // ------------------------------------------------------------------ the refinement's inner operator
@@ -552,6 +556,7 @@ void upload_lists(std::vector<WorkList> &WL, LaunchSet &LS, hipStream_t s) {
 
 void build_inner(Handle &h, int irp) {
     drop_graph(h, H2W_FP32_INNER);
+    ++h.inner_gen;
     const Plan &P = h.plan;
     hipStream_t s = h.stream;
     const int nfam_lo[2] = {2, 1};
@@ -720,6 +725,7 @@ void build_inner(Handle &h, int irp) {
         }
         for (int j = P.nL - 1; j >= 1; --j) {
             WorkList W;
+            W.step = P.nL - j;
             for (int f = 0; f < 2; ++f) {
                 if (!fam_on[f] || j < nfam_lo[f]) continue;
                 FamLevel &fl = h.fam[f][j];
@@ -743,6 +749,7 @@ void build_inner(Handle &h, int irp) {
         {
             WorkList W;
             W.role = 1;
+            W.stage = 1;
             std::vector<std::vector<std::vector<std::vector<TermSpec>>>> per(2);
             for (int f = 0; f < 2; ++f) {
                 if (!fam_on[f]) continue;
@@ -773,6 +780,8 @@ void build_inner(Handle &h, int irp) {
         }
         for (int j = 1; j <= P.nL - 1; ++j) {
             WorkList W;
+            W.stage = 2;
+            W.step = P.nL - j;
             for (int f = 0; f < 2; ++f) {
                 if (!fam_on[f] || j < nfam_lo[f]) continue;
                 FamLevel &fl = h.fam[f][j];
@@ -796,6 +805,7 @@ void build_inner(Handle &h, int irp) {
         }
         {
             WorkList W;
+            W.stage = 2;
             for (int b = 0; b < nleaf; ++b) {
                 std::vector<TermSpec> t;
                 for (int f = 0; f < 2; ++f) {
@@ -811,6 +821,7 @@ void build_inner(Handle &h, int irp) {
         if (!h.pairs.empty()) {
             WorkList W;
             W.role = 2;
+            W.stage = 3;
             std::vector<std::vector<std::vector<TermSpec>>> per(P.nL + 1);
             for (int j = 1; j <= P.nL; ++j) per[j].assign(P.nboxes[j], {});
             for (auto &p : h.pairs)
@@ -1156,6 +1167,7 @@ void assemble(Handle &h) {
     // up, levels nL-1 .. 1
     for (int j = P.nL - 1; j >= 1; --j) {
         WorkList W;
+        W.step = P.nL - j;
         for (int f = 0; f < 2; ++f) {
             if (!fam_on[f] || j < nfam_lo[f]) continue;
             for (int b = 0; b < P.nboxes[j]; ++b)
@@ -1167,6 +1179,7 @@ void assemble(Handle &h) {
     {
         WorkList W;
         W.role = 1;
+        W.stage = 1;
         for (int f = 0; f < 2; ++f) {
             if (!fam_on[f]) continue;
             for (int j = nfam_lo[f]; j <= P.nL; ++j) {
@@ -1190,6 +1203,8 @@ void assemble(Handle &h) {
     // down, levels 1 .. nL-1
     for (int j = 1; j <= P.nL - 1; ++j) {
         WorkList W;
+        W.stage = 2;
+        W.step = P.nL - j;
         for (int f = 0; f < 2; ++f) {
             if (!fam_on[f] || j < nfam_lo[f]) continue;
             FamLevel &fl = h.fam[f][j];
@@ -1205,6 +1220,7 @@ void assemble(Handle &h) {
     // down, leaf level: far + vertex into y
     {
         WorkList W;
+        W.stage = 2;
         for (int b = 0; b < nleaf; ++b) {
             std::vector<TermSpec> t;
             for (int f = 0; f < 2; ++f) {
@@ -1221,6 +1237,7 @@ void assemble(Handle &h) {
     if (!h.pairs.empty()) {
         WorkList W;
         W.role = 2;
+        W.stage = 3;
         std::vector<std::vector<std::vector<TermSpec>>> per(P.nL + 1);
         for (int j = 1; j <= P.nL; ++j) per[j].assign(P.nboxes[j], {});
         for (auto &p : h.pairs)
@@ -1242,6 +1259,8 @@ void assemble(Handle &h) {
         Lh.term0 = (int64_t)terms.size();
         Lh.nterms = (int64_t)W.terms.size();
         Lh.role = W.role;
+        Lh.stage = W.stage;
+        Lh.step = W.step;
         for (auto it : W.items) {
             it.t0 += (int32_t)Lh.term0;
             it.t1 += (int32_t)Lh.term0;
@@ -1252,6 +1271,8 @@ void assemble(Handle &h) {
     }
     h.d_items.upload(items, s);
     h.d_terms.upload(terms, s);
+    h.h_items = items;
+    h.h_terms = terms;
     int64_t se = 0;
     h.launch_elems.assign(h.launches.size(), 0);
     for (size_t q = 0; q < h.launches.size(); ++q) {

@@ -591,7 +591,8 @@ struct Mfma16<double> {
 
 template <typename T, int ROLE>
 __global__ __launch_bounds__(256) void k_bgemm(const GItem *__restrict__ items, const GTerm *__restrict__ terms,
-                                               const T *__restrict__ A0, const T *__restrict__ A1, T *vec) {
+                                               const T *__restrict__ A0, const T *__restrict__ A1,
+                                               const T *const *__restrict__ tab, T *vec) {
     typedef typename Mfma16<T>::v4 V4;
     constexpr int VEC = 16 / sizeof(T);     // rows per lane load
     constexpr int RG = 16 * VEC;            // rows of one load group
@@ -600,7 +601,8 @@ __global__ __launch_bounds__(256) void k_bgemm(const GItem *__restrict__ items, 
     constexpr int UNR = 2;                  // 4-column slices in flight per wave
     typedef T VL __attribute__((ext_vector_type(VEC)));
     const GItem it = items[blockIdx.x];
-    const T *__restrict__ A = (it.flags & 2) ? A1 : A0;
+    // batched systems: pool pair of system flags >> 8 from the table
+    const T *__restrict__ A = tab ? tab[2 * (it.flags >> 8) + ((it.flags >> 1) & 1)] : ((it.flags & 2) ? A1 : A0);
     const int lane = threadIdx.x & 63;
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int rows = it.rows;
@@ -956,15 +958,22 @@ __global__ void k_scatter_m(const T *__restrict__ src, int64_t N, double *__rest
         if (r < nrhs) Y[(perm ? perm[t] : t) + r * ldy] = (double)src[e];
     }
 }
-// partial[(j * nblk + b) * kNR + r] = sum over block b's slots of V_j(t, r) w(t, r)
-__global__ __launch_bounds__(256) void k_mdot16(const double *__restrict__ V, int64_t ldv,
-                                                const double *__restrict__ w, int64_t n,
+// ---- lockstep solves: S systems x kNR columns.  A lockstep vector holds the
+// systems' interleaved tree-order vectors one after another (n slots in all);
+// column (s, r) is right-hand side r of system s, seg[t] the system of slot t
+// (NULL: a single system).  Dot products reduce per block of slots (blk[b] =
+// first and one-past-last slot, all in one system), then per system over the
+// system's blocks sblk[s] .. sblk[s+1]-1.
+// partial[(j * nblk + b) * kNR + r] = sum over block b of V_j(t, r) w(t, r)
+__global__ __launch_bounds__(256) void k_bdot16(const double *__restrict__ V, int64_t ldv,
+                                                const double *__restrict__ w, const int2 *__restrict__ blk,
                                                 double *__restrict__ partial) {
     const int b = blockIdx.x, j = blockIdx.y, nblk = gridDim.x;
     const double *v = V + (int64_t)j * ldv;
+    const int2 rg = blk[b];
     const int r = threadIdx.x & 15;
     double acc = 0.0;
-    for (int64_t e = (int64_t)b * 256 + threadIdx.x; e < n * kNR; e += (int64_t)nblk * 256) acc = fma(v[e], w[e], acc);
+    for (int64_t e = (int64_t)rg.x * kNR + threadIdx.x; e < (int64_t)rg.y * kNR; e += 256) acc = fma(v[e], w[e], acc);
     acc += __shfl_xor(acc, 16, 64);
     acc += __shfl_xor(acc, 32, 64);
     __shared__ double sh[4][kNR];
@@ -973,50 +982,67 @@ __global__ __launch_bounds__(256) void k_mdot16(const double *__restrict__ V, in
     if (threadIdx.x < 16)
         partial[((int64_t)j * nblk + b) * kNR + r] = sh[0][r] + sh[1][r] + sh[2][r] + sh[3][r];
 }
-__global__ __launch_bounds__(256) void k_mdot16_reduce(const double *__restrict__ partial, int nblk,
-                                                       double *__restrict__ out) {
-    const int j = blockIdx.x;
+// out[(j * S + s) * kNR + r] = sum over system s's blocks
+__global__ __launch_bounds__(256) void k_bdot16_reduce(const double *__restrict__ partial, int nblk,
+                                                       const int32_t *__restrict__ sblk, double *__restrict__ out) {
+    const int j = blockIdx.x, sy = blockIdx.y, S = gridDim.y;
     const int r = threadIdx.x & 15, q = threadIdx.x >> 4;
     double acc = 0.0;
-    for (int b = q; b < nblk; b += 16) acc += partial[((int64_t)j * nblk + b) * kNR + r];
+    for (int b = sblk[sy] + q; b < sblk[sy + 1]; b += 16) acc += partial[((int64_t)j * nblk + b) * kNR + r];
     acc += __shfl_xor(acc, 16, 64);
     acc += __shfl_xor(acc, 32, 64);
     __shared__ double sh[4][kNR];
     if ((threadIdx.x & 63) < 16) sh[threadIdx.x >> 6][r] = acc;
     __syncthreads();
-    if (threadIdx.x < 16) out[(int64_t)j * kNR + r] = sh[0][r] + sh[1][r] + sh[2][r] + sh[3][r];
+    if (threadIdx.x < 16) out[((int64_t)j * S + sy) * kNR + r] = sh[0][r] + sh[1][r] + sh[2][r] + sh[3][r];
 }
-// w(t, r) += sign * sum_j V_j(t, r) h(j, r)
-__global__ __launch_bounds__(256) void k_mgemv16(const double *__restrict__ V, int64_t ldv, int k,
-                                                 const double *__restrict__ h, double *__restrict__ w, int64_t n,
-                                                 double sign) {
-    __shared__ double hs[256 * kNR];
-    for (int64_t e0 = blockIdx.x * (int64_t)blockDim.x; e0 < n * kNR; e0 += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t e = e0 + threadIdx.x;
-        const int r = threadIdx.x & 15;
+// w(t, r) += sign * sum_j V_j(t, r) h(j, seg(t), r)
+__global__ __launch_bounds__(256) void k_bgemv16(const double *__restrict__ V, int64_t ldv, int k,
+                                                 const double *__restrict__ h, int S, double sign,
+                                                 double *__restrict__ w, int64_t n, const int32_t *__restrict__ seg) {
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n * kNR; e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t col = (seg ? (int64_t)seg[e / kNR] * kNR : 0) + (e % kNR);
         double acc = 0.0;
-        for (int l0 = 0; l0 < k; l0 += 256) {
-            const int l1 = min(k, l0 + 256);
-            __syncthreads();
-            for (int q = threadIdx.x; q < (l1 - l0) * kNR; q += blockDim.x) hs[q] = h[(int64_t)l0 * kNR + q];
-            __syncthreads();
-            if (e < n * kNR)
-                for (int l = l0; l < l1; ++l) acc = fma(V[(int64_t)l * ldv + e], hs[(l - l0) * kNR + r], acc);
-        }
-        if (e < n * kNR) w[e] += sign * acc;
+        for (int l = 0; l < k; ++l) acc = fma(V[(int64_t)l * ldv + e], h[(int64_t)l * S * kNR + col], acc);
+        w[e] += sign * acc;
     }
 }
-// dst(t, r) = sc(r) * src(t, r)
-__global__ void k_scale16(const double *__restrict__ src, const double *__restrict__ sc, double *__restrict__ dst,
-                          int64_t n) {
+// dst(t, r) = sc(seg(t), r) * src(t, r)
+__global__ void k_bscale16(const double *__restrict__ src, const double *__restrict__ sc, double *dst, int64_t n,
+                           const int32_t *__restrict__ seg) {
     for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n * kNR; e += (int64_t)gridDim.x * blockDim.x)
-        dst[e] = sc[e % kNR] * src[e];
+        dst[e] = sc[(seg ? (int64_t)seg[e / kNR] * kNR : 0) + (e % kNR)] * src[e];
 }
-// y(t, r) += a(r) * x(t, r)
-__global__ void k_axpy16(const double *__restrict__ a, const double *__restrict__ x, double *__restrict__ y,
-                         int64_t n) {
+
+
+// batch vector pool <-> lockstep vectors: lockstep slot t lives at pool slot map[t]
+template <typename T>
+__global__ void k_map_in(const double *__restrict__ X, const int32_t *__restrict__ map, int64_t n, T *__restrict__ pool) {
     for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n * kNR; e += (int64_t)gridDim.x * blockDim.x)
-        y[e] += a[e % kNR] * x[e];
+        pool[(int64_t)map[e / kNR] * kNR + e % kNR] = (T)X[e];
+}
+template <typename T>
+__global__ void k_map_out(const T *__restrict__ pool, const int32_t *__restrict__ map, int64_t n, double *__restrict__ Y) {
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n * kNR; e += (int64_t)gridDim.x * blockDim.x)
+        Y[e] = (double)pool[(int64_t)map[e / kNR] * kNR + e % kNR];
+}
+// column-major caller blocks <-> lockstep vectors: lockstep slot t reads row src[t] of a block with
+// leading dimension ld[t] (per-system blocks concatenated), columns r < nrhs
+__global__ void k_gather_b(const double *__restrict__ X, const int64_t *__restrict__ src, const int32_t *__restrict__ ld,
+                           int nrhs, int64_t n, double *__restrict__ dst) {
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n * kNR; e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t t = e / kNR;
+        const int r = (int)(e % kNR);
+        dst[e] = r < nrhs ? X[src[t] + (int64_t)r * ld[t]] : 0.0;
+    }
+}
+__global__ void k_scatter_b(const double *__restrict__ src_v, const int64_t *__restrict__ src,
+                            const int32_t *__restrict__ ld, int nrhs, int64_t n, double *__restrict__ Y) {
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n * kNR; e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t t = e / kNR;
+        const int r = (int)(e % kNR);
+        if (r < nrhs) Y[src[t] + (int64_t)r * ld[t]] = src_v[e];
+    }
 }
 
 inline int grid_for(int64_t n, int nt = 256, int cap = 8192) {
@@ -1162,21 +1188,31 @@ void launch_scal(double a, double *y, int64_t n, hipStream_t s) {
 
 
 template <typename T>
-static void launch_bgemm_t(const GItem *items, int nitems, const GTerm *terms, const T *A0, const T *A1, T *vec,
-                           int role, hipStream_t s) {
+static void launch_bgemm_t(const GItem *items, int nitems, const GTerm *terms, const T *A0, const T *A1,
+                           const T *const *tab, T *vec, int role, hipStream_t s) {
     if (nitems <= 0) return;
     if (!A1) A1 = A0;
-    if (role == 1) hipLaunchKernelGGL((k_bgemm<T, 1>), dim3(nitems), dim3(256), 0, s, items, terms, A0, A1, vec);
-    else if (role == 2) hipLaunchKernelGGL((k_bgemm<T, 2>), dim3(nitems), dim3(256), 0, s, items, terms, A0, A1, vec);
-    else hipLaunchKernelGGL((k_bgemm<T, 0>), dim3(nitems), dim3(256), 0, s, items, terms, A0, A1, vec);
+    if (role == 1)
+        hipLaunchKernelGGL((k_bgemm<T, 1>), dim3(nitems), dim3(256), 0, s, items, terms, A0, A1, tab, vec);
+    else if (role == 2)
+        hipLaunchKernelGGL((k_bgemm<T, 2>), dim3(nitems), dim3(256), 0, s, items, terms, A0, A1, tab, vec);
+    else hipLaunchKernelGGL((k_bgemm<T, 0>), dim3(nitems), dim3(256), 0, s, items, terms, A0, A1, tab, vec);
 }
 void launch_bgemm64(const GItem *items, int nitems, const GTerm *terms, const double *A0, const double *A1,
                     double *vec, int role, hipStream_t s) {
-    launch_bgemm_t<double>(items, nitems, terms, A0, A1, vec, role, s);
+    launch_bgemm_t<double>(items, nitems, terms, A0, A1, nullptr, vec, role, s);
 }
 void launch_bgemm32(const GItem *items, int nitems, const GTerm *terms, const float *A0, const float *A1, float *vec,
                     int role, hipStream_t s) {
-    launch_bgemm_t<float>(items, nitems, terms, A0, A1, vec, role, s);
+    launch_bgemm_t<float>(items, nitems, terms, A0, A1, nullptr, vec, role, s);
+}
+void launch_bgemm64_tab(const GItem *items, int nitems, const GTerm *terms, const double *const *tab, double *vec,
+                        int role, hipStream_t s) {
+    launch_bgemm_t<double>(items, nitems, terms, nullptr, nullptr, tab, vec, role, s);
+}
+void launch_bgemm32_tab(const GItem *items, int nitems, const GTerm *terms, const float *const *tab, float *vec,
+                        int role, hipStream_t s) {
+    launch_bgemm_t<float>(items, nitems, terms, nullptr, nullptr, tab, vec, role, s);
 }
 void launch_gather_m64(const double *X, int64_t ldx, int nrhs, const int32_t *perm, int64_t N, double *dst,
                        hipStream_t s) {
@@ -1194,24 +1230,41 @@ void launch_scatter_m32(const float *src, int64_t N, double *Y, int64_t ldy, int
                         hipStream_t s) {
     hipLaunchKernelGGL(k_scatter_m<float>, dim3(grid_for(N * kNR)), dim3(256), 0, s, src, N, Y, ldy, nrhs, perm);
 }
-int mdot16_blocks(int64_t n) { return (int)std::max<int64_t>(1, std::min<int64_t>(512, (n * kNR + 4095) / 4096)); }
-void launch_mdot16(const double *V, int64_t ldv, int k, const double *w, int64_t n, double *partial, int nblk,
+void launch_bdot16(const double *V, int64_t ldv, int k, const double *w, const int2 *blk, int nblk, double *partial,
                    hipStream_t s) {
-    if (k > 0) hipLaunchKernelGGL(k_mdot16, dim3(nblk, k), dim3(256), 0, s, V, ldv, w, n, partial);
+    if (k > 0) hipLaunchKernelGGL(k_bdot16, dim3(nblk, k), dim3(256), 0, s, V, ldv, w, blk, partial);
 }
-void launch_mdot16_reduce(const double *partial, int nblk, int k, double *out, hipStream_t s) {
-    if (k > 0) hipLaunchKernelGGL(k_mdot16_reduce, dim3(k), dim3(256), 0, s, partial, nblk, out);
+void launch_bdot16_reduce(const double *partial, int nblk, int k, const int32_t *sblk, int S, double *out,
+                          hipStream_t s) {
+    if (k > 0) hipLaunchKernelGGL(k_bdot16_reduce, dim3(k, S), dim3(256), 0, s, partial, nblk, sblk, out);
 }
-void launch_mgemv16(const double *V, int64_t ldv, int k, const double *h, double sign, double *w, int64_t n,
-                    hipStream_t s) {
+void launch_bgemv16(const double *V, int64_t ldv, int k, const double *h, int S, double sign, double *w, int64_t n,
+                    const int32_t *seg, hipStream_t s) {
     if (k > 0)
-        hipLaunchKernelGGL(k_mgemv16, dim3(grid_for(n * kNR, 256, 4096)), dim3(256), 0, s, V, ldv, k, h, w, n, sign);
+        hipLaunchKernelGGL(k_bgemv16, dim3(grid_for(n * kNR, 256, 4096)), dim3(256), 0, s, V, ldv, k, h, S, sign, w, n,
+                           seg);
 }
-void launch_scale16(const double *src, const double *sc, double *dst, int64_t n, hipStream_t s) {
-    hipLaunchKernelGGL(k_scale16, dim3(grid_for(n * kNR)), dim3(256), 0, s, src, sc, dst, n);
+void launch_bscale16(const double *src, const double *sc, double *dst, int64_t n, const int32_t *seg, hipStream_t s) {
+    hipLaunchKernelGGL(k_bscale16, dim3(grid_for(n * kNR)), dim3(256), 0, s, src, sc, dst, n, seg);
 }
-void launch_axpy16(const double *a, const double *x, double *y, int64_t n, hipStream_t s) {
-    hipLaunchKernelGGL(k_axpy16, dim3(grid_for(n * kNR)), dim3(256), 0, s, a, x, y, n);
+void launch_map_in64(const double *X, const int32_t *map, int64_t n, double *pool, hipStream_t s) {
+    hipLaunchKernelGGL(k_map_in<double>, dim3(grid_for(n * kNR)), dim3(256), 0, s, X, map, n, pool);
 }
-
+void launch_map_in32(const double *X, const int32_t *map, int64_t n, float *pool, hipStream_t s) {
+    hipLaunchKernelGGL(k_map_in<float>, dim3(grid_for(n * kNR)), dim3(256), 0, s, X, map, n, pool);
+}
+void launch_map_out64(const double *pool, const int32_t *map, int64_t n, double *Y, hipStream_t s) {
+    hipLaunchKernelGGL(k_map_out<double>, dim3(grid_for(n * kNR)), dim3(256), 0, s, pool, map, n, Y);
+}
+void launch_map_out32(const float *pool, const int32_t *map, int64_t n, double *Y, hipStream_t s) {
+    hipLaunchKernelGGL(k_map_out<float>, dim3(grid_for(n * kNR)), dim3(256), 0, s, pool, map, n, Y);
+}
+void launch_gather_b(const double *X, const int64_t *src, const int32_t *ld, int nrhs, int64_t n, double *dst,
+                     hipStream_t s) {
+    hipLaunchKernelGGL(k_gather_b, dim3(grid_for(n * kNR)), dim3(256), 0, s, X, src, ld, nrhs, n, dst);
+}
+void launch_scatter_b(const double *v, const int64_t *src, const int32_t *ld, int nrhs, int64_t n, double *Y,
+                      hipStream_t s) {
+    hipLaunchKernelGGL(k_scatter_b, dim3(grid_for(n * kNR)), dim3(256), 0, s, v, src, ld, nrhs, n, Y);
+}
 }  // namespace h2w

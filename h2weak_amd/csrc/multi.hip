@@ -97,73 +97,102 @@ void matvec_m_tree(Handle &h, const double *X, double *Y, int precision) {
     }
 }
 
-// per-column dot products of two interleaved vectors, to the host
-void dots16(Handle &h, const double *a, const double *b, double *out) {
-    const int nb = mdot16_blocks(h.N);
-    h.m_part.ensure((size_t)nb * kNR);
-    h.m_h.ensure(kNR);
-    launch_mdot16(a, 0, 1, b, h.N, h.m_part.p, nb, h.stream);
-    launch_mdot16_reduce(h.m_part.p, nb, 1, h.m_h.p, h.stream);
-    H2W_HIP(hipMemcpyAsync(out, h.m_h.p, kNR * 8, hipMemcpyDeviceToHost, h.stream));
-    H2W_HIP(hipStreamSynchronize(h.stream));
+}  // namespace
+
+// ------------------------------------------------------------------ lockstep solves
+void lock_blocks(const std::vector<int64_t> &start, std::vector<int2> &blk, std::vector<int32_t> &sblk) {
+    const int S = (int)start.size() - 1;
+    const int64_t n = start[S] - start[0];
+    const int64_t chunk = std::max<int64_t>(64, (n + 1023) / 1024);  // ~1024 blocks over the whole vector
+    blk.clear();
+    sblk.assign(S + 1, 0);
+    for (int s = 0; s < S; ++s) {
+        sblk[s] = (int32_t)blk.size();
+        for (int64_t a = start[s]; a < start[s + 1]; a += chunk)
+            blk.push_back(make_int2((int)a, (int)std::min(start[s + 1], a + chunk)));
+    }
+    sblk[S] = (int32_t)blk.size();
 }
 
-void set_scale(Handle &h, const double *sc) {
-    h.m_s.ensure(kNR);
-    H2W_HIP(hipMemcpyAsync(h.m_s.p, sc, kNR * 8, hipMemcpyHostToDevice, h.stream));
+namespace {
+
+// per-column dot products of two lockstep vectors, to the host (S x kNR values)
+void ldots(Lockstep &L, const double *a, const double *b, double *out) {
+    LockBufs &B = *L.buf;
+    B.part.ensure((size_t)L.nblk * kNR);
+    B.h.ensure((size_t)L.S * kNR);
+    launch_bdot16(a, 0, 1, b, L.blk, L.nblk, B.part.p, L.s);
+    launch_bdot16_reduce(B.part.p, L.nblk, 1, L.sblk, L.S, B.h.p, L.s);
+    H2W_HIP(hipMemcpyAsync(out, B.h.p, (size_t)L.S * kNR * 8, hipMemcpyDeviceToHost, L.s));
+    H2W_HIP(hipStreamSynchronize(L.s));
 }
 
-void ensure_Vm(Handle &h, int64_t cols) {
-    if (cols <= h.m_Vcols) return;
-    const int64_t len = h.N * kNR;
-    int64_t nc = std::max<int64_t>(cols, std::max<int64_t>(8, h.m_Vcols * 2));
+// dst(t, r) = sc(s(t), r) src(t, r) with sc from the host
+void lscale(Lockstep &L, const std::vector<double> &sc, const double *src, double *dst) {
+    LockBufs &B = *L.buf;
+    B.sc.ensure(sc.size());
+    H2W_HIP(hipMemcpyAsync(B.sc.p, sc.data(), sc.size() * 8, hipMemcpyHostToDevice, L.s));
+    launch_bscale16(src, B.sc.p, dst, L.n, L.seg, L.s);
+}
+
+void ensure_V(Lockstep &L, int64_t cols) {
+    LockBufs &B = *L.buf;
+    if (cols <= B.Vcols) return;
+    const int64_t len = L.n * kNR;
+    int64_t nc = std::max<int64_t>(cols, std::max<int64_t>(8, B.Vcols * 2));
     double *np = nullptr;
     if (hipMalloc(&np, (size_t)nc * len * 8) != hipSuccess) throw Error{H2W_ENOMEM, "GMRES basis allocation failed"};
-    if (h.m_Vcols > 0)
-        H2W_HIP(hipMemcpyAsync(np, h.m_V.p, (size_t)h.m_Vcols * len * 8, hipMemcpyDeviceToDevice, h.stream));
-    H2W_HIP(hipStreamSynchronize(h.stream));
-    h.m_V.release();
-    h.m_V.p = np;
-    h.m_V.n = (size_t)nc * len;
-    h.m_Vcols = nc;
+    if (B.Vcols > 0) H2W_HIP(hipMemcpyAsync(np, B.V.p, (size_t)B.Vcols * len * 8, hipMemcpyDeviceToDevice, L.s));
+    H2W_HIP(hipStreamSynchronize(L.s));
+    B.V.release();
+    B.V.p = np;
+    B.V.n = (size_t)nc * len;
+    B.Vcols = nc;
 }
 
-struct GmresOutM {
-    int iters[kNR] = {};
-    double resid[kNR] = {};
-    bool conv[kNR] = {};
+struct LockOut {
+    std::vector<int> iters;
+    std::vector<double> resid;
+    std::vector<char> conv;
     int matvecs = 0;
 };
 
-// kNR unrestarted GMRES solves of A x_r = b_r in lockstep (the algorithm of the
-// single-vector core: CGS2 Arnoldi, Givens, resid = |s_{i+1}| / normb).  Column r
-// stops at its own tolerance tol[r]; columns with active[r] false are skipped.
-// x_zero: x0 = 0 and no initial product.  normb_in[r] > 0 overrides |b_r|.
-GmresOutM gmres_core_m(Handle &h, const double *b, double *x, const bool *active, int max_iter, const double *tol,
-                       int prec, bool x_zero, const double *normb_in) {
-    const int64_t N = h.N, len = N * kNR;
-    hipStream_t s = h.stream;
-    GmresOutM out;
-    h.m_w.ensure(len);
-    h.m_r.ensure(len);
-    double *w = h.m_w.p, *r = h.m_r.p;
-    double normb[kNR], beta[kNR], tmp[kNR];
-    if (normb_in) std::memcpy(tmp, normb_in, sizeof(tmp));
+// Unrestarted GMRES on every active column in lockstep: the single-vector core's
+// algorithm (CGS2 Arnoldi, Givens rotations, resid = |s_{i+1}| / normb, back-
+// substitution) per column, one batched product per step for all columns.
+// Column c stops at its own tolerance tol[c]; a stopped column's basis vectors
+// are zero from then on.  x_zero: x0 = 0 and no initial product; normb_in[c] > 0
+// overrides |b_c|.
+LockOut lock_gmres(Lockstep &L, const double *b, double *x, const std::vector<char> &active, int max_iter,
+                   const std::vector<double> &tol, int prec, bool x_zero, const double *normb_in) {
+    const int C = L.S * kNR;
+    const int64_t len = L.n * kNR;
+    hipStream_t s = L.s;
+    LockBufs &B = *L.buf;
+    LockOut out;
+    out.iters.assign(C, 0);
+    out.resid.assign(C, 0.0);
+    out.conv.assign(C, 0);
+    B.w.ensure(len);
+    B.r.ensure(len);
+    double *w = B.w.p, *r = B.r.p;
+    std::vector<double> normb(C), beta(C), tmp(C);
+    if (normb_in) std::copy(normb_in, normb_in + C, tmp.begin());
     else {
-        dots16(h, b, b, tmp);
+        ldots(L, b, b, tmp.data());
         for (double &v : tmp) v = std::sqrt(v);
     }
-    for (int c = 0; c < kNR; ++c) normb[c] = tmp[c] > 0.0 ? tmp[c] : 1.0;
+    for (int c = 0; c < C; ++c) normb[c] = tmp[c] > 0.0 ? tmp[c] : 1.0;
     H2W_HIP(hipMemcpyAsync(r, b, len * 8, hipMemcpyDeviceToDevice, s));
     if (!x_zero) {
-        matvec_m_tree(h, x, w, prec);
+        L.matvec(x, w, prec);
         out.matvecs++;
         launch_axpby(-1.0, w, 1.0, r, len, s);
     }
-    dots16(h, r, r, tmp);
-    bool run[kNR];
+    ldots(L, r, r, tmp.data());
+    std::vector<char> run(C, 0);
     int nrun = 0;
-    for (int c = 0; c < kNR; ++c) {
+    for (int c = 0; c < C; ++c) {
         beta[c] = std::sqrt(tmp[c]);
         out.resid[c] = beta[c] / normb[c];
         run[c] = active[c] && out.resid[c] > tol[c];
@@ -173,46 +202,44 @@ GmresOutM gmres_core_m(Handle &h, const double *b, double *x, const bool *active
     if (nrun == 0) return out;
     const int m = max_iter;
     const size_t ld = (size_t)m + 1;
-    // Hessenberg columns as they are produced: column j holds rows 0..j+1 of every solve
-    std::vector<std::vector<double>> H;
-    std::vector<double> sv(ld * kNR, 0.0), cs(ld * kNR, 0.0), sn(ld * kNR, 0.0);
-    auto Hh = [&](int c, int i, int j) -> double & { return H[j][(size_t)i * kNR + c]; };
-    std::vector<double> Y((size_t)m * kNR, 0.0);  // update coefficients, (j, c) at j * kNR + c
-    ensure_Vm(h, std::min<int64_t>(m + 1, 16));
-    for (int c = 0; c < kNR; ++c) {
+    std::vector<std::vector<double>> H;  // Hessenberg column j: rows 0..j+1 of every column c at i * C + c
+    std::vector<double> sv(ld * C, 0.0), cs(ld * C, 0.0), sn(ld * C, 0.0);
+    auto Hh = [&](int c, int i, int j) -> double & { return H[j][(size_t)i * C + c]; };
+    std::vector<double> Y;  // update coefficients (j, c) at j * C + c
+    ensure_V(L, std::min<int64_t>(m + 1, 16));
+    for (int c = 0; c < C; ++c) {
         tmp[c] = run[c] ? 1.0 / beta[c] : 0.0;
         sv[c * ld] = beta[c];
     }
-    set_scale(h, tmp);
-    launch_scale16(r, h.m_s.p, h.m_V.p, N, s);
-    const int nb = mdot16_blocks(N);
-    h.m_part.ensure((size_t)nb * (m + 2) * kNR);
-    h.m_h.ensure((size_t)(2 * (m + 2) + 2) * kNR);
-    std::vector<double> hh((size_t)(2 * (m + 2) + 2) * kNR);
+    lscale(L, tmp, r, B.V.p);
+    B.part.ensure((size_t)L.nblk * (m + 2) * kNR);
+    B.h.ensure((size_t)(2 * (m + 2) + 2) * C);
+    std::vector<double> hh((size_t)(2 * (m + 2) + 2) * C);
     int last = 0;
     for (int i = 0; i < m && nrun > 0; ++i) {
-        ensure_Vm(h, i + 2);
-        double *V = h.m_V.p;
-        matvec_m_tree(h, V + (int64_t)i * len, w, prec);
+        ensure_V(L, i + 2);
+        double *V = B.V.p;
+        L.matvec(V + (int64_t)i * len, w, prec);
         out.matvecs++;
         const int k = i + 1;
-        launch_mdot16(V, len, k, w, N, h.m_part.p, nb, s);
-        launch_mdot16_reduce(h.m_part.p, nb, k, h.m_h.p, s);
-        launch_mgemv16(V, len, k, h.m_h.p, -1.0, w, N, s);
-        launch_mdot16(V, len, k, w, N, h.m_part.p, nb, s);
-        launch_mdot16_reduce(h.m_part.p, nb, k, h.m_h.p + (size_t)k * kNR, s);
-        launch_mgemv16(V, len, k, h.m_h.p + (size_t)k * kNR, -1.0, w, N, s);
-        launch_mdot16(w, 0, 1, w, N, h.m_part.p, nb, s);
-        launch_mdot16_reduce(h.m_part.p, nb, 1, h.m_h.p + (size_t)2 * k * kNR, s);
-        H2W_HIP(hipMemcpyAsync(hh.data(), h.m_h.p, (size_t)(2 * k + 1) * kNR * 8, hipMemcpyDeviceToHost, s));
+        launch_bdot16(V, len, k, w, L.blk, L.nblk, B.part.p, s);
+        launch_bdot16_reduce(B.part.p, L.nblk, k, L.sblk, L.S, B.h.p, s);
+        launch_bgemv16(V, len, k, B.h.p, L.S, -1.0, w, L.n, L.seg, s);
+        launch_bdot16(V, len, k, w, L.blk, L.nblk, B.part.p, s);
+        launch_bdot16_reduce(B.part.p, L.nblk, k, L.sblk, L.S, B.h.p + (size_t)k * C, s);
+        launch_bgemv16(V, len, k, B.h.p + (size_t)k * C, L.S, -1.0, w, L.n, L.seg, s);
+        launch_bdot16(w, 0, 1, w, L.blk, L.nblk, B.part.p, s);
+        launch_bdot16_reduce(B.part.p, L.nblk, 1, L.sblk, L.S, B.h.p + (size_t)2 * k * C, s);
+        H2W_HIP(hipMemcpyAsync(hh.data(), B.h.p, (size_t)(2 * k + 1) * C * 8, hipMemcpyDeviceToHost, s));
         H2W_HIP(hipStreamSynchronize(s));
         last = i;
-        H.emplace_back((size_t)(i + 2) * kNR, 0.0);
-        for (int c = 0; c < kNR; ++c) {
+        H.emplace_back((size_t)(i + 2) * C, 0.0);
+        Y.resize((size_t)(i + 1) * C, 0.0);
+        for (int c = 0; c < C; ++c) {
             tmp[c] = 0.0;
             if (!run[c]) continue;
-            for (int q = 0; q < k; ++q) Hh(c, q, i) = hh[(size_t)q * kNR + c] + hh[(size_t)(k + q) * kNR + c];
-            Hh(c, i + 1, i) = std::sqrt(hh[(size_t)2 * k * kNR + c]);
+            for (int q = 0; q < k; ++q) Hh(c, q, i) = hh[(size_t)q * C + c] + hh[(size_t)(k + q) * C + c];
+            Hh(c, i + 1, i) = std::sqrt(hh[(size_t)2 * k * C + c]);
             if (Hh(c, i + 1, i) > 0.0) tmp[c] = 1.0 / Hh(c, i + 1, i);
             double *csc = &cs[c * ld], *snc = &sn[c * ld], *svc = &sv[c * ld];
             for (int q = 0; q < i; ++q) app_rot(Hh(c, q, i), Hh(c, q + 1, i), csc[q], snc[q]);
@@ -222,27 +249,23 @@ GmresOutM gmres_core_m(Handle &h, const double *b, double *x, const bool *active
             out.resid[c] = std::fabs(svc[i + 1]) / normb[c];
             out.iters[c] = i + 1;
             if (out.resid[c] < tol[c] || i == m - 1 || tmp[c] == 0.0) {
-                // back-substitution for this column; its later basis vectors stay zero
                 std::vector<double> y(svc, svc + i + 1);
                 for (int q = i; q >= 0; q--) {
                     y[q] /= Hh(c, q, q);
                     for (int jj = q - 1; jj >= 0; jj--) y[jj] -= Hh(c, jj, q) * y[q];
                 }
-                for (int q = 0; q <= i; ++q) Y[(size_t)q * kNR + c] = y[q];
+                for (int q = 0; q <= i; ++q) Y[(size_t)q * C + c] = y[q];
                 out.conv[c] = out.resid[c] < tol[c];
-                run[c] = false;
+                run[c] = 0;
                 tmp[c] = 0.0;
                 --nrun;
             }
         }
-        if (nrun > 0) {
-            set_scale(h, tmp);
-            launch_scale16(w, h.m_s.p, V + (int64_t)(i + 1) * len, N, s);
-        }
+        if (nrun > 0) lscale(L, tmp, w, V + (int64_t)(i + 1) * len);
     }
-    // x += V Y over the basis vectors every column used
-    H2W_HIP(hipMemcpyAsync(h.m_h.p, Y.data(), (size_t)(last + 1) * kNR * 8, hipMemcpyHostToDevice, s));
-    launch_mgemv16(h.m_V.p, len, last + 1, h.m_h.p, 1.0, x, N, s);
+    // x += V Y over the basis vectors the columns used
+    H2W_HIP(hipMemcpyAsync(B.h.p, Y.data(), (size_t)(last + 1) * C * 8, hipMemcpyHostToDevice, s));
+    launch_bgemv16(B.V.p, len, last + 1, B.h.p, L.S, 1.0, x, L.n, L.seg, s);
     H2W_HIP(hipStreamSynchronize(s));
     return out;
 }
@@ -268,87 +291,81 @@ void matvec_multi_dev(Handle &h, const double *X, int64_t ldx, double *Y, int64_
     H2W_HIP(hipGetLastError());
 }
 
-// nrhs <= kNR solves on interleaved tree-order vectors: B (device, read) and X
-// (device, x0 on entry, the solutions on return).  Per-column statistics in st.
-void solve_tree_m(Handle &h, const double *B, double *X, int nrhs, int max_iter, double tol, int mode,
-                  h2w_gmres_stats *st) {
-    hipStream_t s = h.stream;
-    const int64_t N = h.N, len = N * kNR;
+void lock_solve(Lockstep &L, const double *B, double *X, int nrhs, int max_iter, double tol, int mode,
+                h2w_gmres_stats *st) {
+    hipStream_t s = L.s;
+    const int S = L.S, C = S * kNR;
+    const int64_t len = L.n * kNR;
     const int inner_prec = mode == H2W_GMRES_FP32_INNER_IR ? H2W_FP32_INNER : H2W_FP32;
     const double t0 = now_s();
-    bool active[kNR];
-    double tolv[kNR];
-    for (int c = 0; c < kNR; ++c) {
-        active[c] = c < nrhs;
-        tolv[c] = tol;
-    }
-    for (int c = 0; c < nrhs; ++c) std::memset(&st[c], 0, sizeof(h2w_gmres_stats));
+    std::vector<char> active(C, 0);
+    for (int sy = 0; sy < S; ++sy)
+        for (int r = 0; r < nrhs; ++r) active[sy * kNR + r] = 1;
+    auto stat = [&](int c) -> h2w_gmres_stats & { return st[(c / kNR) * nrhs + c % kNR]; };
+    std::memset(st, 0, sizeof(h2w_gmres_stats) * S * nrhs);
     if (mode == H2W_GMRES_FP64) {
-        GmresOutM o = gmres_core_m(h, B, X, active, max_iter, tolv, H2W_FP64, false, nullptr);
-        for (int c = 0; c < nrhs; ++c) {
-            st[c].iterations = o.iters[c];
-            st[c].resid = o.resid[c];
-            st[c].matvecs_fp64 = o.matvecs;
-            st[c].converged = o.conv[c];
-        }
+        LockOut o = lock_gmres(L, B, X, active, max_iter, std::vector<double>(C, tol), H2W_FP64, false, nullptr);
+        for (int c = 0; c < C; ++c)
+            if (active[c]) {
+                stat(c).iterations = o.iters[c];
+                stat(c).resid = o.resid[c];
+                stat(c).matvecs_fp64 = o.matvecs;
+                stat(c).converged = o.conv[c];
+            }
     } else {
         // FP64 refinement per column: r = b - A x on the stored FP64 operators, corrections from
-        // kNR-wide GMRES on the FP32 (inner) operator; converged columns drop out (zero residual)
-        h.m_d.ensure(len);
-        h.m_b.ensure(len);
-        double *d = h.m_d.p, *r = h.m_b.p;
-        double nb2[kNR], rr[kNR], normb[kNR], rn[kNR], tmp[kNR];
-        dots16(h, B, B, nb2);
-        for (int c = 0; c < kNR; ++c) normb[c] = nb2[c] > 0.0 ? std::sqrt(nb2[c]) : 1.0;
+        // lockstep GMRES on the FP32 (inner) operators; converged columns drop out (zero residual)
+        LockBufs &Bf = *L.buf;
+        Bf.d.ensure(len);
+        Bf.b.ensure(len);
+        double *d = Bf.d.p, *r = Bf.b.p;
+        std::vector<double> nb2(C), rr(C), normb(C), rn(C), tmp(C), nr(C), eta(C), dd(C), db(C);
+        ldots(L, B, B, nb2.data());
+        for (int c = 0; c < C; ++c) normb[c] = nb2[c] > 0.0 ? std::sqrt(nb2[c]) : 1.0;
         int outer = 0, total = 0, mv64 = 0, mv32 = 0;
-        const double floor_eta = mode == H2W_GMRES_FP32_INNER_IR ? std::pow(10.0, -1.0 * h.opts.ir_tol_pow) : 1e-6;
-        bool run[kNR];
+        std::vector<char> run(C, 0);
         while (true) {
-            matvec_m_tree(h, X, d, H2W_FP64);
+            L.matvec(X, d, H2W_FP64);
             ++mv64;
             if (outer == 0) {
-                // x0 -> alpha_r x0 with alpha_r minimising |b_r - alpha_r A x0_r|
-                double dd[kNR], db[kNR];
-                dots16(h, d, d, dd);
-                dots16(h, d, B, db);
-                for (int c = 0; c < kNR; ++c) {
+                // x0 -> alpha_c x0 with alpha_c minimising |b_c - alpha_c A x0_c|
+                ldots(L, d, d, dd.data());
+                ldots(L, d, B, db.data());
+                for (int c = 0; c < C; ++c) {
                     const double a = dd[c] > 0.0 ? db[c] / dd[c] : 1.0;
                     tmp[c] = std::isfinite(a) ? a : 1.0;
                 }
-                set_scale(h, tmp);
-                launch_scale16(X, h.m_s.p, X, N, s);
-                launch_scale16(d, h.m_s.p, d, N, s);
+                lscale(L, tmp, X, X);
+                lscale(L, tmp, d, d);
             }
             H2W_HIP(hipMemcpyAsync(r, B, len * 8, hipMemcpyDeviceToDevice, s));
             launch_axpby(-1.0, d, 1.0, r, len, s);
-            dots16(h, r, r, rr);
+            ldots(L, r, r, rr.data());
             int nrun = 0;
-            for (int c = 0; c < kNR; ++c) {
+            for (int c = 0; c < C; ++c) {
                 rn[c] = std::sqrt(rr[c]) / normb[c];
                 run[c] = active[c] && rn[c] > tol && total < max_iter;
                 nrun += run[c];
-                if (c < nrhs) {
-                    st[c].resid = rn[c];
-                    st[c].converged = rn[c] <= tol;
+                if (active[c]) {
+                    stat(c).resid = rn[c];
+                    stat(c).converged = rn[c] <= tol;
                 }
             }
             if (nrun == 0 || outer >= 50) break;
-            double nr[kNR], eta[kNR];
-            for (int c = 0; c < kNR; ++c) {
+            for (int c = 0; c < C; ++c) {
+                const double fl = mode == H2W_GMRES_FP32_INNER_IR ? L.ir_floor[c / kNR] : 1e-6;
                 nr[c] = std::sqrt(rr[c]);
-                eta[c] = run[c] ? std::min(1e-2, std::max(floor_eta, 0.1 * tol / rn[c])) : 1.0;
+                eta[c] = run[c] ? std::min(1e-2, std::max(fl, 0.1 * tol / rn[c])) : 1.0;
                 tmp[c] = run[c] ? 1.0 : 0.0;
             }
-            // converged columns get a zero right-hand side
-            set_scale(h, tmp);
-            launch_scale16(r, h.m_s.p, r, N, s);
+            lscale(L, tmp, r, r);  // converged columns get a zero right-hand side
             launch_fill(d, len, 0.0, s);
-            GmresOutM o = gmres_core_m(h, r, d, run, max_iter - total, eta, inner_prec, true, nr);
+            LockOut o = lock_gmres(L, r, d, run, max_iter - total, eta, inner_prec, true, nr.data());
             int most = 0;
-            for (int c = 0; c < nrhs; ++c)
+            for (int c = 0; c < C; ++c)
                 if (run[c]) {
-                    st[c].iterations += o.iters[c];
-                    st[c].outer_iterations++;
+                    stat(c).iterations += o.iters[c];
+                    stat(c).outer_iterations++;
                     most = std::max(most, o.iters[c]);
                 }
             total += most;
@@ -357,14 +374,40 @@ void solve_tree_m(Handle &h, const double *B, double *X, int nrhs, int max_iter,
             ++outer;
             if (most == 0) break;
         }
-        for (int c = 0; c < nrhs; ++c) {
-            st[c].matvecs_fp64 = mv64;
-            st[c].matvecs_fp32 = mv32;
-        }
+        for (int c = 0; c < C; ++c)
+            if (active[c]) {
+                stat(c).matvecs_fp64 = mv64;
+                stat(c).matvecs_fp32 = mv32;
+            }
     }
     H2W_HIP(hipStreamSynchronize(s));
     const double dt = now_s() - t0;
-    for (int c = 0; c < nrhs; ++c) st[c].seconds = dt;
+    for (int q = 0; q < S * nrhs; ++q) st[q].seconds = dt;
+}
+
+// nrhs <= kNR solves of one system on interleaved tree-order vectors: B (device,
+// read) and X (device, x0 on entry, the solutions on return)
+void solve_tree_m(Handle &h, const double *B, double *X, int nrhs, int max_iter, double tol, int mode,
+                  h2w_gmres_stats *st) {
+    if (!h.m_nblk) {
+        std::vector<int2> blk;
+        std::vector<int32_t> sblk;
+        lock_blocks({0, h.N}, blk, sblk);
+        h.m_blk.upload(blk, h.stream);
+        h.m_sblk.upload(sblk, h.stream);
+        h.m_nblk = (int)blk.size();
+    }
+    Lockstep L;
+    L.s = h.stream;
+    L.n = h.N;
+    L.S = 1;
+    L.blk = h.m_blk.p;
+    L.nblk = h.m_nblk;
+    L.sblk = h.m_sblk.p;
+    L.ir_floor = {std::pow(10.0, -1.0 * h.opts.ir_tol_pow)};
+    L.matvec = [&h](const double *x, double *y, int prec) { matvec_m_tree(h, x, y, prec); };
+    L.buf = &h.mbuf;
+    lock_solve(L, B, X, nrhs, max_iter, tol, mode, st);
 }
 
 }  // namespace h2w

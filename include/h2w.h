@@ -212,6 +212,24 @@ typedef struct {
 } h2w_tune;
 int h2w_autotune(h2w_handle *h, const double *b, int where, double tol, int max_iter, int flags, h2w_tune *out);
 
+/* ---- batches of independent systems (no counterpart in the reference, which solves one system per run).
+ * A batch refers to assembled handles on one device (they must outlive it and not be re-assembled; an
+ * inner operator rebuilt after h2w_batch_create makes H2W_FP32_INNER calls fail with H2W_ESTATE).  Every
+ * product is one launch per phase for all systems (their work lists merged by phase, up passes aligned at
+ * the leaves, down passes ending there), 16 right-hand sides per pass on the matrix cores.  Vectors: for
+ * each system in order an N_s x nrhs column-major block (leading dimension N_s), blocks concatenated. */
+typedef struct h2w_batch h2w_batch;
+int h2w_batch_create(h2w_handle *const *systems, int nsys, h2w_batch **out);
+/* Y = A X for every system (precision H2W_FP64, H2W_FP32 or H2W_FP32_INNER). */
+int h2w_batch_matvec(h2w_batch *b, const double *X, double *Y, int nrhs, int precision, int where);
+/* nsys x nrhs independent solves in lockstep (the per-column algorithm of h2w_gmres_multi); stats has
+ * nsys * nrhs entries, system-major. */
+int h2w_batch_gmres(h2w_batch *b, const double *B, double *X, int nrhs, int max_iter, double tol, int mode,
+                    int where, h2w_gmres_stats *stats);
+/* HIP-event time of one batched 16-wide product; launch_bytes / launch_role per merged launch. */
+int h2w_batch_bench_matvec(h2w_batch *b, int nrep, int precision, h2w_bench *out);
+void h2w_batch_destroy(h2w_batch *b);
+
 /* Tree-order permutation: perm[t] = input index of the t-th point in tree order
  * (leaf boxes' chargeLocations concatenated, 3D/codes/HODLR3Dtree.hpp:2489-2503). */
 int h2w_tree_order(const h2w_handle *h, int32_t *perm);

@@ -1,0 +1,68 @@
+"""GPU: batches of independent systems.  A batched product (one launch per
+phase for every system, 16 right-hand sides per pass) equals each system's own
+16-wide product; lockstep solves over all (system, right-hand side) columns
+reach each column's tolerance with the single-system solve's accuracy.  The
+systems differ in size, depth, kernel and format."""
+import numpy as np
+import pytest
+
+import h2weak_amd as h2w
+
+pytestmark = pytest.mark.gpu
+
+# dim, N, n_max, tol_pow, choice, variant, seed
+SYSTEMS = [
+    (3, 2000, 64, 8, 1, 0, 1),
+    (3, 9000, 64, 8, 1, 0, 2),    # one level deeper
+    (3, 5000, 64, 8, 0, 1, 3),    # semi-nested IE
+    (2, 3000, 40, 8, 1, 0, 4),
+]
+
+
+@pytest.fixture(scope="module")
+def batch():
+    sys_ = []
+    for dim, N, nmax, tp, ch, var, seed in SYSTEMS:
+        P = np.random.default_rng(seed).uniform(-1, 1, (dim, N))
+        g = h2w.Initialisation(dim, N, nmax, 1.0, tp, ch, var, points=P)
+        g.tree_initialisation()
+        sys_.append(g)
+    return h2w.Batch(sys_), sys_
+
+
+def _rel(a, b):
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+def test_depths_differ(batch):
+    _, sys_ = batch
+    assert len({g.stats()["nlevels"] for g in sys_}) >= 2
+
+
+@pytest.mark.parametrize("nrhs", [3, 16, 19])
+def test_batched_product(batch, nrhs):
+    B, sys_ = batch
+    rng = np.random.default_rng(nrhs)
+    Xs = [rng.normal(size=(nrhs, g.N)) for g in sys_]
+    for prec, tol in ((h2w.FP64, 1e-13), (h2w.FP32, 1e-6), (h2w.FP32_INNER, 1e-6)):
+        Ys = B.matvec(Xs, precision=prec)
+        for X, Y, g in zip(Xs, Ys, sys_):
+            ref = g.matvec_multi(X, precision=prec)
+            for c in range(nrhs):
+                assert _rel(Y[c], ref[c]) <= tol, (prec, c)
+
+
+def test_batched_solves(batch):
+    B, sys_ = batch
+    rng = np.random.default_rng(7)
+    Xt = [rng.normal(size=(4, g.N)) for g in sys_]
+    Bs = [g.matvec_multi(x) for g, x in zip(sys_, Xt)]
+    gtol = 1e-8
+    for mode in (h2w.GMRES_FP64, h2w.GMRES_FP32_IR, h2w.GMRES_FP32_INNER_IR):
+        Xs, st = B.gmres(Bs, None, 300, gtol, mode)
+        for s, g in enumerate(sys_):
+            for c in range(4):
+                assert st[s][c]["converged"], (mode, s, c, st[s][c])
+                assert _rel(g.mat_vec_prod(Xs[s][c]), Bs[s][c]) <= 2 * gtol
+                x1, _ = g.gmres(Bs[s][c], None, 300, gtol, mode)
+                assert _rel(Xs[s][c], Xt[s][c]) <= 10 * _rel(x1, Xt[s][c]) + 1e-12, (mode, s, c)
