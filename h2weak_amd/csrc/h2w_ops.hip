@@ -139,18 +139,24 @@ struct AcaPool {
     int64_t cap = 0;  // doubles
 };
 
-void ensure_pool(Handle &h, AcaPool &P) {
-    if (P.cap > 0) return;
+// The workspace holds the U and V columns of every block of a batch.  It is sized
+// to what the blocks at hand are estimated to need (within the budget: opts or
+// half of the free device memory, at most 96 GB) and grows when a later call
+// needs more, so small systems do not pay for a large allocation.
+void ensure_pool(Handle &h, AcaPool &P, int64_t need_doubles) {
     int64_t budget = h.opts.workspace_bytes;
     if (budget <= 0) {
         size_t fr = 0, tot = 0;
         H2W_HIP(hipMemGetInfo(&fr, &tot));
-        budget = std::min<int64_t>((int64_t)96 << 30, (int64_t)(fr / 2));
+        budget = std::min<int64_t>((int64_t)96 << 30, (int64_t)(fr / 2) + P.cap * 8);
         budget = std::max<int64_t>(budget, (int64_t)64 << 20);
     }
-    P.cap = budget / 8;
+    const int64_t want = std::min<int64_t>(budget / 8, std::max<int64_t>(need_doubles + need_doubles / 4, 8 << 20));
+    if (P.cap >= want) return;
+    P.ws.release();
+    P.cap = want;
     P.ws.alloc(P.cap);
-    P.top.alloc(1);
+    if (!P.top.p) P.top.alloc(1);
 }
 
 void run_aca(Handle &h, std::vector<AcaBlock> &blocks, int rank_hint, AcaPool &pool) {
@@ -158,12 +164,14 @@ void run_aca(Handle &h, std::vector<AcaBlock> &blocks, int rank_hint, AcaPool &p
     const double tol = std::pow(10.0, -1.0 * h.tol_pow);
     hipStream_t s = h.stream;
     const bool verbose = std::getenv("H2W_VERBOSE") != nullptr;
-    ensure_pool(h, pool);
     std::vector<int> est(blocks.size());
+    int64_t need = 0;
     for (size_t b = 0; b < blocks.size(); ++b) {
         int mn = (int)std::min(blocks[b].rows.size(), blocks[b].cols.size());
         est[b] = std::min(mn, std::max(1, rank_hint));
+        need += (int64_t)(blocks[b].rows.size() + blocks[b].cols.size()) * est[b];
     }
+    ensure_pool(h, pool, need);
     std::vector<size_t> todo(blocks.size());
     std::iota(todo.begin(), todo.end(), 0);
     DevBuf<uint8_t> flags;
@@ -303,6 +311,12 @@ void run_aca(Handle &h, std::vector<AcaBlock> &blocks, int rank_hint, AcaPool &p
             }
         }
         todo.swap(rerun);
+        if (!todo.empty()) {
+            // re-runs go with larger estimates: let the workspace grow to hold them
+            int64_t need2 = 0;
+            for (size_t b : todo) need2 += (int64_t)(blocks[b].rows.size() + blocks[b].cols.size()) * est[b];
+            ensure_pool(h, pool, need2);
+        }
     }
 }
 
