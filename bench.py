@@ -149,6 +149,64 @@ def pmc_reduce(d, side, tp):
     print(json.dumps(out))
 
 
+def bench_batch(S=32, tol_pow=8, seed=5):
+    """32 independent 3D H2* RBF systems, N log-uniform in [4096, 65536] (seed 5), points U([-1,1]^3),
+    n_max 64, ACA and GMRES tolerance 1e-8, 16 right-hand sides each (X_true ~ N(0,1), B = A X_true):
+    batched products (one launch per phase for all systems, 16 right-hand sides per pass) and lockstep
+    solves of all 512 columns, against plain FP64 solves one right-hand side at a time."""
+    import h2weak_amd as h2w
+    rng = np.random.default_rng(seed)
+    Ns = [int(v) for v in np.rint(np.exp(rng.uniform(np.log(4096), np.log(65536), S)))]
+    gtol = 10.0 ** (-tol_pow)
+    t0 = time.time()
+    systems = []
+    for q, N in enumerate(Ns):
+        P = np.random.default_rng(100 + q).uniform(-1.0, 1.0, (3, N))
+        g = h2w.Initialisation(3, N, 64, 1.0, tol_pow, h2w.CHOICE_RBF, h2w.NESTED, points=P)
+        g.tree_initialisation()
+        systems.append(g)
+    t_asm = time.time() - t0
+    log(f"batch: {S} systems, N {min(Ns)}..{max(Ns)} (sum {sum(Ns)}), assembly {t_asm:.1f} s")
+    B = h2w.Batch(systems)
+    out = {"systems": S, "N": Ns, "nrhs": 16, "tol": gtol, "assembly_s": round(t_asm, 2),
+           "nlevels": sorted({g.stats()["nlevels"] for g in systems}), "matvec": {}, "gmres": {}}
+    sts = [g.stats() for g in systems]
+    for prec, name, key, s_op in ((h2w.FP64, "fp64", "stored_bytes_fp64", 8), (h2w.FP32, "fp32", "stored_bytes_fp32", 4),
+                                  (h2w.FP32_INNER, "fp32_inner", "stored_bytes_inner_fp32", 4)):
+        r = B.bench_matvec(5, prec)
+        nbytes = sum(st[key] for st in sts) + 2 * 8 * 16 * sum(Ns)
+        out["matvec"][name] = {"ms_per_16": round(r["ms_per_pass"], 4), "launches": len(r["launch_role"]),
+                               "alg_GBps": round(nbytes / (r["ms_per_pass"] * 1e-3) * 1e-9, 1)}
+        log(f"batch 16-wide product {name}: {r['ms_per_pass']:.3f} ms, {len(r['launch_role'])} launches")
+    Xt = [np.random.default_rng(200 + q).normal(size=(16, g.N)) for q, g in enumerate(systems)]
+    Bs = B.matvec(Xt)
+    for name, mode in (("fp64", h2w.GMRES_FP64), ("fp32_inner_ir", h2w.GMRES_FP32_INNER_IR),
+                       ("fp32_ir", h2w.GMRES_FP32_IR)):
+        B.gmres([b[:1] for b in Bs], None, 300, gtol, mode)  # warm-up (graph capture)
+        Xs, st = B.gmres(Bs, None, 300, gtol, mode)
+        cols = [c for row in st for c in row]
+        err = max(float(np.linalg.norm(Xs[q][c] - Xt[q][c]) / np.linalg.norm(Xt[q][c]))
+                  for q in range(S) for c in range(16))
+        out["gmres"][name] = {"s_512": round(cols[0]["seconds"], 4), "ms_per_solve": round(cols[0]["seconds"] / 512 * 1e3, 4),
+                              "all_converged": all(c["converged"] for c in cols),
+                              "max_iterations": max(c["iterations"] for c in cols), "max_rel_error": err}
+        log(f"batch solves {name}: {cols[0]['seconds'] * 1e3:.1f} ms for 512 columns, max error {err:.2e}")
+    # plain FP64, one right-hand side at a time: every 4th system's first column, scaled to 512 solves
+    ts = []
+    for q in range(0, S, 4):
+        _, s1 = systems[q].gmres(Bs[q][0], None, 300, gtol, h2w.GMRES_FP64)
+        ts.append(s1["seconds"])
+    seq = float(np.mean(ts)) * 512
+    out["sequential_fp64_single_rhs_s"] = round(seq, 3)
+    out["sequential_sample"] = "mean of one single-RHS FP64 solve on every 4th system, x512"
+    out["speedup_fp32_inner_ir_vs_sequential_fp64"] = round(seq / out["gmres"]["fp32_inner_ir"]["s_512"], 2)
+    out["speedup_fp32_inner_ir_vs_batched_fp64"] = round(out["gmres"]["fp64"]["s_512"]
+                                                         / out["gmres"]["fp32_inner_ir"]["s_512"], 3)
+    log(f"batch: sequential FP64 estimate {seq:.2f} s; batched FP32-inner IR x{out['speedup_fp32_inner_ir_vs_sequential_fp64']}")
+    del B, systems
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -159,6 +217,7 @@ def main():
     ap.add_argument("--cpu-side", type=int, default=24, help="grid side of the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-multi", action="store_true", help="skip the 16-right-hand-side section")
+    ap.add_argument("--no-batch", action="store_true", help="skip the 32-system batch section")
     ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (default launch shapes, inner 1e-6)")
     ap.add_argument("--pmc-probe", action="store_true",
                     help="assemble, then run 3 FP64 and 3 FP32-inner matvecs and a 1 GiB device copy "
@@ -338,6 +397,12 @@ def main():
         r["copy_GBps"] = round(copy_bw, 1)
         r["frac_of_copy"] = round(r["achieved"] / copy_bw, 4)
 
+    inner_bytes = A.stats()["stored_bytes_inner_fp32"]
+    del A  # frees the large system's operators before the batch section
+    import gc
+    gc.collect()
+    batch = None if args.no_batch else bench_batch()
+
     out = {"metric": METRIC, "value": round(t_step, 6), "unit": "s", "n_gpus": 1, "steps": args.steps,
            "warmup": args.warmup, "ms_per_step": round(t_step * 1e3, 3), "higher_is_better": False,
            "scaling": "weak", "vs_baseline": None, "dtype": "f32 operators, f64 refinement",
@@ -352,9 +417,9 @@ def main():
            "matvec": {k: {kk: (round(vv, 4) if isinstance(vv, float) else vv) for kk, vv in v.items()}
                       for k, v in mv.items()},
            "assembly_s": round(st["assembly_seconds"], 2), "memory_elements": M,
-           "autotune": tune, "multi_rhs": multi,
+           "autotune": tune, "multi_rhs": multi, "batch_c4": batch,
            "roofline": roof, "roofline_other": roof2, "roofline_otf": roof_otf,
-           "inner_operator_bytes_fp32": st["stored_bytes_inner_fp32"], "cpu_baseline": None}
+           "inner_operator_bytes_fp32": inner_bytes, "cpu_baseline": None}
     if not args.no_cpu:
         log(f"CPU baseline (oracle) on grid {args.cpu_side}^3")
         out["cpu_baseline"] = cpu_baseline(args.cpu_side, res["fp64"]["stats"]["matvecs_fp64"], M, tp, gtol)
