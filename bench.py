@@ -207,6 +207,44 @@ def bench_batch(S=32, tol_pow=8, seed=5):
     return out
 
 
+def bench_c5(N=262144, tol_pow=10, seed=7):
+    """Learned rank predictor at N = 262144, 3D RBF, tol 1e-10, uniform and Gaussian (sigma 0.2) points:
+    assembly with the heuristic workspace estimates and with the shipped model, and the model's accuracy on
+    every ACA block (these two point sets are not in its training harvest)."""
+    import h2weak_amd as h2w
+    from h2weak_amd import rank_mlp
+    params, meta = rank_mlp.load(rank_mlp.default_model_path(3, 1, tol_pow))
+    out = {"workload": f"3D H2* RBF N={N} n_max=64 tol 1e-{tol_pow}, uniform and gauss(0.2) points (seed {seed})",
+           "model": {k: meta[k] for k in ("features", "hidden", "activation", "params", "train_samples", "train_sets")},
+           "sets": {}}
+    for kind in ("uniform", "gauss"):
+        P = rank_mlp.points(kind, N, seed)
+        row = {}
+        mem = None
+        for use in (False, True):
+            g = h2w.Initialisation(3, N, 64, 1.0, tol_pow, h2w.CHOICE_RBF, h2w.NESTED, points=P)
+            if use:
+                g.set_rank_model(params)
+            g.tree_initialisation()
+            st = g.stats()
+            mem = st["memory"] if mem is None else mem
+            assert st["memory"] == mem  # the model never changes the operator
+            r = {"assembly_s": round(st["assembly_seconds"], 2), "aca_batches": st["aca_batches"],
+                 "aca_reruns": st["aca_reruns"], "aca_pool_GB": round(st["aca_pool_bytes"] * 1e-9, 2)}
+            if use:
+                F, R = g.rank_samples()
+                pl = g.rank_predict(F)
+                est = np.ceil(1.2 * np.maximum(0, np.exp2(pl) - 1)) + 4
+                r.update({"blocks": int(len(R)), "mae_log2": round(float(np.mean(np.abs(pl - np.log2(1 + R)))), 4),
+                          "under_estimates": st["rank_pred_under"],
+                          "reserved_over_actual": round(float(np.sum(est) / max(1, np.sum(R))), 3)})
+            row["model" if use else "heuristic"] = r
+            log(f"C5 {kind} {'model' if use else 'heuristic'}: {r}")
+            del g
+        out["sets"][kind] = row
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -225,7 +263,12 @@ def main():
     ap.add_argument("--pmc-reduce", metavar="DIR",
                     help="read DIR/fetch_counter_collection.csv and DIR/write_counter_collection.csv written by "
                          "two rocprofv3 --pmc runs of --pmc-probe and write profiles/pmc_traffic.json (no GPU)")
+    ap.add_argument("--c5", action="store_true",
+                    help="rank-predictor measurement (N = 262144 uniform and Gaussian, heuristic vs model)")
     args = ap.parse_args()
+    if args.c5:
+        bench_c5()
+        return
     if args.pmc_reduce:
         pmc_reduce(args.pmc_reduce, args.side, args.tol_pow)
         return

@@ -36,7 +36,9 @@ EXPORTS = ("h2w_default_opts", "h2w_create", "h2w_assemble", "h2w_matvec", "h2w_
            "h2w_dense_matvec", "h2w_get_stats", "h2w_tree_order", "h2w_box_skeleton", "h2w_interaction_list",
            "h2w_near_list", "h2w_destroy", "h2w_last_error", "h2w_bench_matvec", "h2w_set_inner_tolerance",
            "h2w_autotune", "h2w_gmres_multi", "h2w_bench_matvec_multi", "h2w_batch_create", "h2w_batch_matvec",
-           "h2w_batch_gmres", "h2w_batch_bench_matvec", "h2w_batch_destroy")
+           "h2w_batch_gmres", "h2w_batch_bench_matvec", "h2w_batch_destroy", "h2w_set_rank_model",
+           "h2w_rank_samples", "h2w_rank_predict")
+RANK_FEATURES, RANK_PARAMS = 16, 9473
 TUNE_UNROLL, TUNE_INNER = 1, 2
 
 
@@ -59,7 +61,9 @@ class Stats(ctypes.Structure):
                 ("max_rank_vertex", ctypes.c_int), ("mean_rank_far", ctypes.c_double),
                 ("mean_rank_vertex", ctypes.c_double), ("stored_bytes_fp64", ctypes.c_int64),
                 ("stored_bytes_fp32", ctypes.c_int64), ("n_launches_matvec", ctypes.c_int),
-                ("aca_reruns", ctypes.c_int), ("stored_bytes_inner_fp32", ctypes.c_int64)]
+                ("aca_reruns", ctypes.c_int), ("stored_bytes_inner_fp32", ctypes.c_int64),
+                ("aca_batches", ctypes.c_int), ("aca_pool_bytes", ctypes.c_int64), ("rank_pred_blocks", ctypes.c_int),
+                ("rank_pred_under", ctypes.c_int), ("rank_pred_mae_log2", ctypes.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -121,6 +125,10 @@ def lib():
     L.h2w_batch_gmres.argtypes = [p, p, p, i, i, d, i, i, ctypes.POINTER(GmresStats)]
     L.h2w_batch_bench_matvec.argtypes = [p, i, i, ctypes.POINTER(BenchResult)]
     L.h2w_batch_destroy.argtypes = [p]
+    L.h2w_set_rank_model.argtypes = [p, p, i64]
+    L.h2w_rank_samples.argtypes = [p, p, p, i64]
+    L.h2w_rank_samples.restype = i64
+    L.h2w_rank_predict.argtypes = [p, p, i64, p]
     L.h2w_gmres_multi.argtypes = [p, p, i64, p, i64, i, i, d, i, i, ctypes.POINTER(GmresStats)]
     L.h2w_autotune.argtypes = [p, p, i, d, i, i, ctypes.POINTER(Tune)]
     _lib = L
@@ -269,6 +277,32 @@ class Initialisation:
         return {"ms_per_matvec": r.ms_per_matvec, "launch_ms": list(r.launch_ms[:n]),
                 "launch_bytes": list(r.launch_bytes[:n]), "launch_role": list(r.launch_role[:n])}
 
+
+    def set_rank_model(self, params):
+        """BF16 parameters (uint16 array of RANK_PARAMS) of the rank predictor; None removes it.
+        Takes effect at the next tree_initialisation()."""
+        if params is None:
+            _check(lib().h2w_set_rank_model(self._h, None, 0))
+            return
+        w = np.ascontiguousarray(params, dtype=np.uint16)
+        _check(lib().h2w_set_rank_model(self._h, _ptr(w), w.size))
+
+    def rank_samples(self):
+        """(features [n, 16] float32, ranks [n] int32) of every ACA block of the last assembly."""
+        n = lib().h2w_rank_samples(self._h, None, None, 0)
+        if n < 0:
+            _check(-n)
+        F = np.empty((n, RANK_FEATURES), dtype=np.float32)
+        R = np.empty(n, dtype=np.int32)
+        lib().h2w_rank_samples(self._h, _ptr(F), _ptr(R), n)
+        return F, R
+
+    def rank_predict(self, feat):
+        """log2(1 + rank) predicted by the GPU kernel for feature rows [n, 16]."""
+        F = np.ascontiguousarray(feat, dtype=np.float32)
+        out = np.empty(F.shape[0], dtype=np.float32)
+        _check(lib().h2w_rank_predict(self._h, _ptr(F), F.shape[0], _ptr(out)))
+        return out
 
     def set_inner_tolerance(self, ir_tol_pow):
         """Rebuild the truncated FP32 inner operator for tolerance 10^-ir_tol_pow (0 removes it)."""

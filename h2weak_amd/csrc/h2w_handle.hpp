@@ -5,6 +5,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <array>
 #include <chrono>
 #include <functional>
 #include <cmath>
@@ -57,8 +58,16 @@ struct DevBuf {
 inline int64_t align_up(int64_t x, int64_t a) { return (x + a - 1) / a * a; }
 
 // one ACA problem: rows/cols are tree-order point indices
+constexpr int kRankFeatures = 16;  // geometry features of one ACA block (rank predictor input)
+constexpr int kRankHidden = 64;    // hidden width of the rank predictor (three hidden layers)
+// parameters: W1 (F x H), b1, W2 (H x H), b2, W3 (H x H), b3, w4 (H), b4 -- row-major, BF16
+constexpr int kRankParams = kRankFeatures * kRankHidden + kRankHidden + 2 * (kRankHidden * kRankHidden + kRankHidden) +
+                            kRankHidden + 1;
+
 struct AcaBlock {
     std::vector<int32_t> rows, cols;
+    std::array<float, kRankFeatures> feat{};
+    bool has_feat = false;
     int rank = 0;
     std::vector<int32_t> rb, cb;  // local pivot indices
     std::vector<double> lr;       // L then R, rank x rank column-major each
@@ -171,6 +180,13 @@ struct h2w_handle {
     double memory = 0.0;
     double assembly_seconds = 0.0;
     int aca_reruns = 0;
+    int aca_batches = 0;                // k_aca launch batches of the last assembly
+    int64_t aca_pool_bytes = 0;         // largest ACA workspace of the last assembly
+    h2w::DevBuf<uint16_t> rank_model;   // BF16 rank-predictor parameters (empty: heuristic estimates)
+    int rank_pred_blocks = 0, rank_pred_under = 0;  // predicted blocks / blocks whose rank exceeded the estimate
+    double rank_pred_abs_log2 = 0.0;    // sum of |log2(1 + predicted) - log2(1 + rank)|
+    std::vector<std::array<float, h2w::kRankFeatures>> sample_feat;  // harvest: every ACA block of the
+    std::vector<int32_t> sample_rank;                                 // last assembly with its rank
     hipStream_t stream = nullptr;
     h2w::KParams kp;
     h2w::DevBuf<double> d_px, d_py, d_pz;

@@ -6,6 +6,7 @@
 
 #include <cstdint>
 #include <string>
+#include <array>
 #include <vector>
 
 #include "../../include/h2w.h"
@@ -62,6 +63,9 @@ struct Plan {
     std::vector<std::vector<std::vector<int32_t>>> il;  // [level][box] interaction list
     std::vector<std::vector<std::vector<uint8_t>>> ilv; // vertex flags of il entries
     std::vector<std::vector<int32_t>> near;             // [leaf box] near neighbours (slot order)
+    std::vector<std::vector<std::array<int, 3>>> icoord; // [level][box] integer box coordinates
+    std::vector<std::vector<std::array<float, 6>>> bbox; // [level][box] bounding box of its points (min, max);
+                                                         // empty boxes: min > max
 
     void build(int dim, int64_t N, const double *pts_soa, double L, int nlevels);
     std::vector<int32_t> vert_of(int j, int b) const;
@@ -203,6 +207,8 @@ void launch_gather_b(const double *X, const int64_t *src, const int32_t *ld, int
                      hipStream_t s);
 void launch_scatter_b(const double *v, const int64_t *src, const int32_t *ld, int nrhs, int64_t n, double *Y,
                       hipStream_t s);
+// rank predictor: out[i] = log2(1 + predicted rank) of block i (16 features each, BF16 parameters)
+void launch_rank_mlp(const float *feat, int n, const uint16_t *w, float *out, hipStream_t s);
 // lockstep solves over S systems x kNR columns (segmented interleaved vectors, seg == NULL: S == 1)
 void launch_bdot16(const double *V, int64_t ldv, int k, const double *w, const int2 *blk, int nblk, double *partial,
                    hipStream_t s);

@@ -159,21 +159,94 @@ void ensure_pool(Handle &h, AcaPool &P, int64_t need_doubles) {
     if (!P.top.p) P.top.alloc(1);
 }
 
+// Geometry features of an ACA block for the rank predictor: family, level, row and
+// column counts, the box's and the partners' point counts, the interaction set's
+// size and centre separations (in box half-widths), how much of their boxes the
+// points fill, and the box centre.  All of order one.
+void block_features(const Handle &h, int fam, int j, int b, const std::vector<int32_t> &partners, int64_t m,
+                    int64_t n, AcaBlock &B) {
+    const Plan &P = h.plan;
+    const double hw = P.L / double(1 << j);
+    auto centre = [&](int bx, int d) { return d < P.dim ? -P.L + (2.0 * P.icoord[j][bx][d] + 1.0) * hw : 0.0; };
+    auto fill = [&](int bx) {
+        const auto &bb = P.bbox[j][bx];
+        if (bb[0] > bb[3]) return 0.0;
+        double e = 0.0;
+        for (int d = 0; d < P.dim; ++d) e += bb[3 + d] - bb[d];
+        return e / (P.dim * 2.0 * hw);
+    };
+    double sum_d = 0.0, min_d = 0.0, sum_fill = 0.0;
+    int64_t pcount = 0;
+    for (size_t q = 0; q < partners.size(); ++q) {
+        const int p = partners[q];
+        double d2 = 0.0;
+        for (int d = 0; d < 3; ++d) d2 += (centre(p, d) - centre(b, d)) * (centre(p, d) - centre(b, d));
+        const double dist = std::sqrt(d2) / hw;
+        sum_d += dist;
+        min_d = q == 0 ? dist : std::min(min_d, dist);
+        sum_fill += fill(p);
+        pcount += P.bcount[j][p];
+    }
+    const double np = (double)partners.size();
+    float *f = B.feat.data();
+    f[0] = fam == 0;
+    f[1] = fam == 1;
+    f[2] = fam == 2;
+    f[3] = (float)(j / 8.0);
+    f[4] = (float)(std::log2(1.0 + m) / 16.0);
+    f[5] = (float)(std::log2(1.0 + n) / 20.0);
+    f[6] = (float)(std::log2(1.0 + P.bcount[j][b]) / 16.0);
+    f[7] = (float)(std::log2(1.0 + pcount) / 20.0);
+    f[8] = (float)(np / 64.0);
+    f[9] = (float)(np > 0 ? sum_d / np / 8.0 : 0.0);
+    f[10] = (float)(min_d / 8.0);
+    f[11] = (float)fill(b);
+    f[12] = (float)(np > 0 ? sum_fill / np : 0.0);
+    for (int d = 0; d < 3; ++d) f[13 + d] = (float)(centre(b, d) / P.L);
+    B.has_feat = true;
+}
+
 void run_aca(Handle &h, std::vector<AcaBlock> &blocks, int rank_hint, AcaPool &pool) {
     if (blocks.empty()) return;
     const double tol = std::pow(10.0, -1.0 * h.tol_pow);
     hipStream_t s = h.stream;
     const bool verbose = std::getenv("H2W_VERBOSE") != nullptr;
     std::vector<int> est(blocks.size());
-    int64_t need = 0;
     for (size_t b = 0; b < blocks.size(); ++b) {
         int mn = (int)std::min(blocks[b].rows.size(), blocks[b].cols.size());
         est[b] = std::min(mn, std::max(1, rank_hint));
-        need += (int64_t)(blocks[b].rows.size() + blocks[b].cols.size()) * est[b];
     }
-    ensure_pool(h, pool, need);
     std::vector<size_t> todo(blocks.size());
     std::iota(todo.begin(), todo.end(), 0);
+    // learned rank estimates: the workspace each block reserves, and rank-homogeneous batches
+    // (blocks in descending predicted rank); the cross itself still stops at the tolerance
+    bool pred = h.rank_model.p != nullptr;
+    for (auto &B : blocks) pred = pred && B.has_feat;
+    std::vector<float> plog;
+    if (pred) {
+        const int nb = (int)blocks.size();
+        std::vector<float> F((size_t)nb * kRankFeatures);
+        for (int b = 0; b < nb; ++b) std::copy(blocks[b].feat.begin(), blocks[b].feat.end(), &F[(size_t)b * kRankFeatures]);
+        DevBuf<float> dF, dO;
+        dF.upload(F, s);
+        dO.alloc(nb);
+        launch_rank_mlp(dF.p, nb, h.rank_model.p, dO.p, s);
+        H2W_HIP(hipGetLastError());
+        plog.resize(nb);
+        H2W_HIP(hipMemcpyAsync(plog.data(), dO.p, (size_t)nb * 4, hipMemcpyDeviceToHost, s));
+        H2W_HIP(hipStreamSynchronize(s));
+        for (int b = 0; b < nb; ++b) {
+            const int mn = (int)std::min(blocks[b].rows.size(), blocks[b].cols.size());
+            const double pr = std::max(0.0, std::exp2((double)plog[b]) - 1.0);
+            est[b] = std::max(1, std::min(mn, (int)std::ceil(1.2 * pr) + 4));
+        }
+        std::stable_sort(todo.begin(), todo.end(), [&](size_t a, size_t c) { return est[a] > est[c]; });
+    }
+    const std::vector<int> est0 = est;
+    int64_t need = 0;
+    for (size_t b = 0; b < blocks.size(); ++b) need += (int64_t)(blocks[b].rows.size() + blocks[b].cols.size()) * est[b];
+    ensure_pool(h, pool, need);
+    h.aca_pool_bytes = std::max<int64_t>(h.aca_pool_bytes, pool.cap * 8);
     DevBuf<uint8_t> flags;
     DevBuf<int32_t> didx, drank, dstat, drp, dcp;
     DevBuf<double> dhist;
@@ -238,6 +311,7 @@ void run_aca(Handle &h, std::vector<AcaBlock> &blocks, int rank_hint, AcaPool &p
             dcp.ensure(std::max<int64_t>(po, 1));
             dhist.ensure(std::max<int64_t>(2 * po, 2));
             H2W_HIP(hipMemsetAsync(pool.top.p, 0, sizeof(unsigned long long), s));
+            ++h.aca_batches;
             double tb = now_s();
             if (nbig > 0)
                 launch_aca(djobs.p, (int)nbig, didx.p, pool.ws.p, pool.cap, pool.top.p, dtab.p, flags.p, drank.p,
@@ -279,6 +353,15 @@ void run_aca(Handle &h, std::vector<AcaBlock> &blocks, int rank_hint, AcaPool &p
                 }
                 AcaBlock &B = blocks[b];
                 B.rank = rk[q];
+                if (B.has_feat) {
+                    h.sample_feat.push_back(B.feat);
+                    h.sample_rank.push_back(rk[q]);
+                }
+                if (pred) {
+                    h.rank_pred_blocks++;
+                    h.rank_pred_under += rk[q] > est0[b];
+                    h.rank_pred_abs_log2 += std::fabs((double)plog[b] - std::log2(1.0 + rk[q]));
+                }
                 B.rb.assign(rp.begin() + jobs[q].piv, rp.begin() + jobs[q].piv + rk[q]);
                 B.cb.assign(cp.begin() + jobs[q].piv, cp.begin() + jobs[q].piv + rk[q]);
                 B.hist.assign(hs.begin() + 2 * jobs[q].piv, hs.begin() + 2 * (jobs[q].piv + rk[q]));
@@ -347,7 +430,9 @@ void far_nca(Handle &h, AcaPool &pool) {
         std::vector<AcaBlock> blocks(fl.nb);
         for (int b = 0; b < fl.nb; ++b) {
             blocks[b].rows = fl.S[b];
-            for (int f : P.far_of(j, b)) blocks[b].cols.insert(blocks[b].cols.end(), fl.S[f].begin(), fl.S[f].end());
+            const auto far = P.far_of(j, b);
+            for (int f : far) blocks[b].cols.insert(blocks[b].cols.end(), fl.S[f].begin(), fl.S[f].end());
+            block_features(h, 0, j, b, far, (int64_t)blocks[b].rows.size(), (int64_t)blocks[b].cols.size(), blocks[b]);
         }
         // the far rank is bounded by the row-set size (children's skeletons)
         int hint = 0;
@@ -384,7 +469,8 @@ void vertex_nca(Handle &h, AcaPool &pool) {
         std::vector<AcaBlock> blocks(fl.nb);
         for (int b = 0; b < fl.nb; ++b) {
             blocks[b].rows = range_list(P.bstart[j][b], P.bcount[j][b]);
-            for (int v : P.vert_of(j, b)) {
+            const auto ver = P.vert_of(j, b);
+            for (int v : ver) {
                 auto r = range_list(P.bstart[j][v], P.bcount[j][v]);
                 blocks[b].cols.insert(blocks[b].cols.end(), r.begin(), r.end());
             }
@@ -392,6 +478,7 @@ void vertex_nca(Handle &h, AcaPool &pool) {
                 auto &pic = F[j - 1].Ic[b / P.nchild];
                 blocks[b].cols.insert(blocks[b].cols.end(), pic.begin(), pic.end());
             }
+            block_features(h, 1, j, b, ver, (int64_t)blocks[b].rows.size(), (int64_t)blocks[b].cols.size(), blocks[b]);
         }
         // ranks shrink with the box size: start from twice the parent level's largest rank
         int hint = std::max(16, h.opts.rank_cap);
@@ -450,6 +537,7 @@ void cur_pairs(Handle &h, AcaPool &pool) {
                 AcaBlock B;
                 B.rows = range_list(P.bstart[j][b], P.bcount[j][b]);
                 B.cols = range_list(P.bstart[j][v], P.bcount[j][v]);
+                block_features(h, 2, j, b, {v}, (int64_t)B.rows.size(), (int64_t)B.cols.size(), B);
                 blocks.push_back(std::move(B));
             }
     run_aca(h, blocks, std::max(16, h.opts.rank_cap), pool);
@@ -862,6 +950,12 @@ void build_inner(Handle &h, int irp) {
 void assemble(Handle &h) {
     drop_graphs(h);
     double t0 = now_s();
+    h.aca_reruns = h.aca_batches = 0;
+    h.aca_pool_bytes = 0;
+    h.rank_pred_blocks = h.rank_pred_under = 0;
+    h.rank_pred_abs_log2 = 0.0;
+    h.sample_feat.clear();
+    h.sample_rank.clear();
     const bool verbose = std::getenv("H2W_VERBOSE") != nullptr;
     auto phase = [&](const char *what) {
         if (verbose) {
@@ -2069,6 +2163,11 @@ int h2w_get_stats(const h2w_handle *h, h2w_stats *st) {
     st->diag = h->diag;
     st->assembly_seconds = h->assembly_seconds;
     st->aca_reruns = h->aca_reruns;
+    st->aca_batches = h->aca_batches;
+    st->aca_pool_bytes = h->aca_pool_bytes;
+    st->rank_pred_blocks = h->rank_pred_blocks;
+    st->rank_pred_under = h->rank_pred_under;
+    st->rank_pred_mae_log2 = h->rank_pred_blocks ? h->rank_pred_abs_log2 / h->rank_pred_blocks : 0.0;
     for (int f = 0; f < 2; ++f) {
         int mx = 0;
         double sum = 0.0;
@@ -2275,6 +2374,55 @@ int h2w_autotune(h2w_handle *h, const double *b, int where, double tol, int max_
     }
     for (int q = 0; q < 3; ++q) out->matvec_ms_after[q] = have(precs[q]) ? h2w::time_matvec(*h, precs[q], 5) : 0.0;
     H2W_HIP(hipStreamSynchronize(s));
+    return H2W_OK;
+    H2W_CATCH
+}
+
+int h2w_set_rank_model(h2w_handle *h, const uint16_t *params, int64_t count) {
+    H2W_TRY
+    if (!h) throw Error{H2W_EINVAL, "NULL handle"};
+    if (params && count != h2w::kRankParams)
+        throw Error{H2W_EINVAL, "rank model: expected " + std::to_string(h2w::kRankParams) + " BF16 parameters"};
+    if (h->opts.plan_only) throw Error{H2W_ESTATE, "handle was created with plan_only"};
+    H2W_HIP(hipSetDevice(h->opts.device));
+    if (!params) {
+        h->rank_model.release();
+        return H2W_OK;
+    }
+    std::vector<uint16_t> v(params, params + count);
+    h->rank_model.upload(v, h->stream);
+    H2W_HIP(hipStreamSynchronize(h->stream));
+    return H2W_OK;
+    H2W_CATCH
+}
+
+int64_t h2w_rank_samples(const h2w_handle *h, float *feat, int32_t *rank, int64_t cap) {
+    if (!h) {
+        h2w::set_error("NULL handle");
+        return -H2W_EINVAL;
+    }
+    const int64_t n = (int64_t)h->sample_rank.size();
+    for (int64_t q = 0; q < std::min(n, cap); ++q) {
+        if (feat) std::copy(h->sample_feat[q].begin(), h->sample_feat[q].end(), feat + q * h2w::kRankFeatures);
+        if (rank) rank[q] = h->sample_rank[q];
+    }
+    return n;
+}
+
+int h2w_rank_predict(h2w_handle *h, const float *feat, int64_t n, float *out) {
+    H2W_TRY
+    if (!h || (!feat && n > 0) || (!out && n > 0) || n < 0) throw Error{H2W_EINVAL, "bad arguments"};
+    if (!h->rank_model.p) throw Error{H2W_ESTATE, "no rank model (h2w_set_rank_model)"};
+    H2W_HIP(hipSetDevice(h->opts.device));
+    if (n == 0) return H2W_OK;
+    std::vector<float> F(feat, feat + n * h2w::kRankFeatures);
+    h2w::DevBuf<float> dF, dO;
+    dF.upload(F, h->stream);
+    dO.alloc(n);
+    h2w::launch_rank_mlp(dF.p, (int)n, h->rank_model.p, dO.p, h->stream);
+    H2W_HIP(hipGetLastError());
+    H2W_HIP(hipMemcpyAsync(out, dO.p, n * 4, hipMemcpyDeviceToHost, h->stream));
+    H2W_HIP(hipStreamSynchronize(h->stream));
     return H2W_OK;
     H2W_CATCH
 }

@@ -1045,6 +1045,57 @@ __global__ void k_scatter_b(const double *__restrict__ src_v, const int64_t *__r
     }
 }
 
+
+// ------------------------------------------------------------------ rank predictor
+// A 16-64-64-64-1 perceptron with GELU (erf form) activations predicting
+// log2(1 + rank) of an ACA block from its geometry features.  BF16 parameters
+// (expanded to FP32 in LDS), FP32 accumulation, one thread per block.
+// Parameter layout: W1 (F x H, row-major: input-major), b1, W2 (H x H), b2,
+// W3 (H x H), b3, w4 (H), b4.
+constexpr int kRF = 16, kRH = 64;
+constexpr int kRP = kRF * kRH + kRH + 2 * (kRH * kRH + kRH) + kRH + 1;
+__device__ __forceinline__ float bf16_f32(uint16_t v) { return __uint_as_float((uint32_t)v << 16); }
+__device__ __forceinline__ float gelu_erf(float x) { return 0.5f * x * (1.0f + erff(x * 0.70710678118654752f)); }
+
+__global__ __launch_bounds__(256) void k_rank_mlp(const float *__restrict__ feat, int n,
+                                                  const uint16_t *__restrict__ w, float *__restrict__ out) {
+    __shared__ float sw[kRP];
+    for (int q = threadIdx.x; q < kRP; q += blockDim.x) sw[q] = bf16_f32(w[q]);
+    __syncthreads();
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float *W1 = sw, *b1 = W1 + kRF * kRH, *W2 = b1 + kRH, *b2 = W2 + kRH * kRH, *W3 = b2 + kRH;
+    const float *b3 = W3 + kRH * kRH, *w4 = b3 + kRH, *b4 = w4 + kRH;
+    float x[kRF], h1[kRH], h2[kRH];
+#pragma unroll
+    for (int f = 0; f < kRF; ++f) x[f] = feat[(int64_t)i * kRF + f];
+#pragma unroll
+    for (int o = 0; o < kRH; ++o) {
+        float a = b1[o];
+#pragma unroll
+        for (int f = 0; f < kRF; ++f) a = fmaf(x[f], W1[f * kRH + o], a);
+        h1[o] = gelu_erf(a);
+    }
+#pragma unroll
+    for (int o = 0; o < kRH; ++o) {
+        float a = b2[o];
+#pragma unroll
+        for (int k = 0; k < kRH; ++k) a = fmaf(h1[k], W2[k * kRH + o], a);
+        h2[o] = gelu_erf(a);
+    }
+#pragma unroll
+    for (int o = 0; o < kRH; ++o) {
+        float a = b3[o];
+#pragma unroll
+        for (int k = 0; k < kRH; ++k) a = fmaf(h2[k], W3[k * kRH + o], a);
+        h1[o] = gelu_erf(a);
+    }
+    float y = b4[0];
+#pragma unroll
+    for (int k = 0; k < kRH; ++k) y = fmaf(h1[k], w4[k], y);
+    out[i] = y;
+}
+
 inline int grid_for(int64_t n, int nt = 256, int cap = 8192) {
     int64_t g = (n + nt - 1) / nt;
     if (g < 1) g = 1;
@@ -1266,5 +1317,8 @@ void launch_gather_b(const double *X, const int64_t *src, const int32_t *ld, int
 void launch_scatter_b(const double *v, const int64_t *src, const int32_t *ld, int nrhs, int64_t n, double *Y,
                       hipStream_t s) {
     hipLaunchKernelGGL(k_scatter_b, dim3(grid_for(n * kNR)), dim3(256), 0, s, v, src, ld, nrhs, n, Y);
+}
+void launch_rank_mlp(const float *feat, int n, const uint16_t *w, float *out, hipStream_t s) {
+    if (n > 0) hipLaunchKernelGGL(k_rank_mlp, dim3((n + 255) / 256), dim3(256), 0, s, feat, n, w, out);
 }
 }  // namespace h2w

@@ -171,6 +171,32 @@ void Plan::build(int dim_, int64_t N_, const double *pts, double L_, int nlevels
             at[j][lin] = b;
         }
     }
+    icoord = ic;
+    // bounding boxes of the boxes' points, leaves up
+    bbox.assign(nL + 1, {});
+    for (int j = nL; j >= 0; --j) {
+        bbox[j].assign(nboxes[j], {1e30f, 1e30f, 1e30f, -1e30f, -1e30f, -1e30f});
+        for (int b = 0; b < nboxes[j]; ++b) {
+            auto &bb = bbox[j][b];
+            if (j == nL) {
+                for (int64_t t = bstart[j][b]; t < bstart[j][b] + bcount[j][b]; ++t) {
+                    const double c[3] = {px[t], dim > 1 ? py[t] : 0.0, dim > 2 ? pz[t] : 0.0};
+                    for (int d = 0; d < 3; ++d) {
+                        bb[d] = std::min(bb[d], (float)c[d]);
+                        bb[3 + d] = std::max(bb[3 + d], (float)c[d]);
+                    }
+                }
+            } else {
+                for (int c = 0; c < nchild; ++c) {
+                    const auto &cb = bbox[j + 1][b * nchild + c];
+                    for (int d = 0; d < 3; ++d) {
+                        bb[d] = std::min(bb[d], cb[d]);
+                        bb[3 + d] = std::max(bb[3 + d], cb[3 + d]);
+                    }
+                }
+            }
+        }
+    }
     auto lookup = [&](int j, const std::array<int, 3> &q) -> int {
         int side = 1 << j;
         int64_t lin = 0;

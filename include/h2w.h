@@ -95,6 +95,11 @@ typedef struct {
     int n_launches_matvec;   /* kernel launches per matvec                                       */
     int aca_reruns;          /* blocks re-run because they outgrew their workspace batch         */
     int64_t stored_bytes_inner_fp32; /* bytes one H2W_FP32_INNER matvec streams                      */
+    int aca_batches;         /* ACA launch batches of the assembly                                 */
+    int64_t aca_pool_bytes;  /* largest ACA workspace of the assembly                              */
+    int rank_pred_blocks;    /* blocks whose workspace came from the rank model (0: no model)      */
+    int rank_pred_under;     /* of those, blocks whose rank exceeded the estimate (re-run)         */
+    double rank_pred_mae_log2; /* mean |log2(1 + predicted) - log2(1 + rank)| over those blocks    */
 } h2w_stats;
 
 typedef struct {
@@ -211,6 +216,24 @@ typedef struct {
     double matvec_ms_after[3];
 } h2w_tune;
 int h2w_autotune(h2w_handle *h, const double *b, int where, double tol, int max_iter, int flags, h2w_tune *out);
+
+/* ---- learned rank predictor (no counterpart in the reference).  A 16-64-64-64-1 perceptron (GELU,
+ * BF16 parameters, FP32 accumulation, evaluated on the GPU) predicts log2(1 + rank) of every ACA block
+ * from 16 geometry features (family, level, row/column and point counts, interaction-set size and
+ * separations, box fill, box centre).  With a model set, each block reserves workspace for 1.2 x its
+ * predicted rank + 4 and blocks are batched in descending predicted rank; the adaptive cross still decides
+ * pivots and ranks (a block that outgrows its estimate is re-run), so the assembled operator does not
+ * depend on the model.  Parameter layout: W1 (16 x 64, input-major), b1, W2 (64 x 64), b2, W3, b3,
+ * w4 (64), b4. */
+#define H2W_RANK_FEATURES 16
+#define H2W_RANK_PARAMS 9473
+/* params == NULL removes the model; takes effect at the next h2w_assemble. */
+int h2w_set_rank_model(h2w_handle *h, const uint16_t *params, int64_t count);
+/* (features, rank) of every ACA block of the last assembly (the training harvest); copies up to cap
+ * samples (feat: cap x H2W_RANK_FEATURES) and returns the number available (or -H2W_*). */
+int64_t h2w_rank_samples(const h2w_handle *h, float *feat, int32_t *rank, int64_t cap);
+/* The model's output log2(1 + rank) for n feature rows (host pointers), by the same GPU kernel. */
+int h2w_rank_predict(h2w_handle *h, const float *feat, int64_t n, float *out);
 
 /* ---- batches of independent systems (no counterpart in the reference, which solves one system per run).
  * A batch refers to assembled handles on one device (they must outlive it and not be re-assembled; an
