@@ -256,6 +256,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-multi", action="store_true", help="skip the 16-right-hand-side section")
     ap.add_argument("--no-batch", action="store_true", help="skip the 32-system batch section")
+    ap.add_argument("--no-rank-model", action="store_true", help="assemble with heuristic workspace estimates")
     ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (default launch shapes, inner 1e-6)")
     ap.add_argument("--pmc-probe", action="store_true",
                     help="assemble, then run 3 FP64 and 3 FP32-inner matvecs and a 1 GiB device copy "
@@ -284,6 +285,10 @@ def main():
     log(f"workload: 3D H2* RBF grid {side}^3 N={N} n_max=64 tol 1e-{tp}")
     t0 = time.time()
     A = h2w.Initialisation(3, N, 64, 1.0, tp, h2w.CHOICE_RBF, h2w.NESTED)
+    from h2weak_amd import rank_mlp
+    mpath = None if args.no_rank_model else rank_mlp.default_model_path(3, 1, tp)
+    if mpath:
+        A.set_rank_model(rank_mlp.load(mpath)[0])  # workspace estimates only; the operator is unchanged
     A.tree_initialisation()
     st = A.stats()
     log(f"assembly {time.time() - t0:.1f} s, M = {st['memory']:.4e} elements ({st['memory'] * 8e-9:.2f} GB fp64)")
@@ -460,6 +465,8 @@ def main():
            "matvec": {k: {kk: (round(vv, 4) if isinstance(vv, float) else vv) for kk, vv in v.items()}
                       for k, v in mv.items()},
            "assembly_s": round(st["assembly_seconds"], 2), "memory_elements": M,
+           "assembly_rank_model": {k: st[k] for k in ("rank_pred_blocks", "rank_pred_under", "rank_pred_mae_log2",
+                                                      "aca_batches", "aca_reruns")},
            "autotune": tune, "multi_rhs": multi, "batch_c4": batch,
            "roofline": roof, "roofline_other": roof2, "roofline_otf": roof_otf,
            "inner_operator_bytes_fp32": inner_bytes, "cpu_baseline": None}
