@@ -207,6 +207,61 @@ def bench_batch(S=32, tol_pow=8, seed=5):
     return out
 
 
+# The reference's eight published runs (uniform grids, x0 = 1): parameters, and the printed iteration count,
+# storage (GB) and wall times of assembly and GMRES on its CPU (hardware unstated).
+TRANSCRIPTS = {
+    "1d_nested": dict(dim=1, variant=0, N=250000, n_max=100, tol_pow=12, choice=1, gtol=1e-12, max_iter=500,
+                      iters=11, storage_gb=0.658597, asm_s=47.0822, gmres_s=1.49966, src="1D/results/nhodlr.txt"),
+    "1d_semi": dict(dim=1, variant=1, N=250000, n_max=100, tol_pow=12, choice=1, gtol=1e-12, max_iter=500,
+                    iters=11, storage_gb=1.61, asm_s=26.4311, gmres_s=2.37944, src="1D/results/sn_hodlr.txt"),
+    "2d_nested": dict(dim=2, variant=0, N=90000, n_max=400, tol_pow=12, choice=1, gtol=1e-12, max_iter=500,
+                      iters=28, storage_gb=2.47347, asm_s=68.1836, gmres_s=4.57271, src="2D/results/nhodlr.txt"),
+    "2d_semi": dict(dim=2, variant=1, N=90000, n_max=400, tol_pow=12, choice=1, gtol=1e-12, max_iter=500,
+                    iters=28, storage_gb=3.03009, asm_s=37.1335, gmres_s=4.75639, src="2D/results/sn_hodlr.txt"),
+    "2d_hstar": dict(dim=2, variant=2, N=90000, n_max=400, tol_pow=12, choice=1, gtol=1e-12, max_iter=500,
+                     iters=28, storage_gb=4.17922, asm_s=39.9024, gmres_s=7.14276, src="2D/results/hodlr.txt"),
+    "3d_nested": dict(dim=3, variant=0, N=42875, n_max=512, tol_pow=7, choice=0, gtol=1e-10, max_iter=1000,
+                      iters=5, storage_gb=2.87549, asm_s=50.4586, gmres_s=1.59765, src="3D/results/nhodlr.txt"),
+    "3d_semi": dict(dim=3, variant=1, N=42875, n_max=512, tol_pow=7, choice=0, gtol=1e-10, max_iter=1000,
+                    iters=5, storage_gb=3.02412, asm_s=55.2427, gmres_s=2.11972, src="3D/results/sn_hodlr.txt"),
+    "3d_hstar": dict(dim=3, variant=2, N=42875, n_max=512, tol_pow=7, choice=0, gtol=1e-10, max_iter=1000,
+                     iters=5, storage_gb=4.39326, asm_s=129.971, gmres_s=3.54523, src="3D/results/hodlr.txt"),
+}
+
+
+def bench_transcripts():
+    """The reference's published runs on the GPU: assembly and GMRES wall time beside the published CPU
+    times, with the storage count, iterations and solution error of each."""
+    import h2weak_amd as h2w
+    out = {}
+    for name, c in TRANSCRIPTS.items():
+        t0 = time.time()
+        A = h2w.Initialisation(c["dim"], c["N"], c["n_max"], 1.0, c["tol_pow"], c["choice"], c["variant"])
+        A.tree_initialisation()
+        t_asm = time.time() - t0
+        st = A.stats()
+        xt = np.random.default_rng(2).uniform(-1.0, 1.0, c["N"])
+        b = A.dense_rows(None, xt)
+        row = {"nlevels": st["nlevels"], "storage_gb": st["memory"] * 8e-9, "published_storage_gb": c["storage_gb"],
+               "assembly_s": round(t_asm, 3), "published_assembly_s": c["asm_s"],
+               "assembly_speedup": round(c["asm_s"] / t_asm, 1), "source": c["src"]}
+        for mode, nm in ((h2w.GMRES_FP64, "fp64"), (h2w.GMRES_FP32_INNER_IR, "fp32_inner_ir")):
+            A.gmres(b, None, c["max_iter"], c["gtol"], mode)  # warm-up (graph capture)
+            x, gs = A.gmres(b, None, c["max_iter"], c["gtol"], mode)
+            err = float(np.linalg.norm(x - xt) / np.linalg.norm(xt))
+            row[nm] = {"s": round(gs["seconds"], 5), "iterations": gs["iterations"], "resid": gs["resid"],
+                       "rel_error": err, "converged": bool(gs["converged"]),
+                       "speedup_vs_published": round(c["gmres_s"] / gs["seconds"], 1)}
+        row["published_iterations"] = c["iters"]
+        row["published_gmres_s"] = c["gmres_s"]
+        log(f"{name}: storage {row['storage_gb']:.6g} GB (published {c['storage_gb']}), assembly {t_asm:.2f} s "
+            f"(published {c['asm_s']} s), GMRES FP64 {row['fp64']['s'] * 1e3:.1f} ms / {row['fp64']['iterations']} "
+            f"its (published {c['gmres_s']} s / {c['iters']}), FP32-inner IR {row['fp32_inner_ir']['s'] * 1e3:.1f} ms")
+        out[name] = row
+        del A
+    print(json.dumps({"reference_transcripts_on_gpu": out}), flush=True)
+
+
 def bench_c5(N=262144, tol_pow=10, seed=7):
     """Learned rank predictor at N = 262144, 3D RBF, tol 1e-10, uniform and Gaussian (sigma 0.2) points:
     assembly with the heuristic workspace estimates and with the shipped model, and the model's accuracy on
@@ -266,7 +321,12 @@ def main():
                          "two rocprofv3 --pmc runs of --pmc-probe and write profiles/pmc_traffic.json (no GPU)")
     ap.add_argument("--c5", action="store_true",
                     help="rank-predictor measurement (N = 262144 uniform and Gaussian, heuristic vs model)")
+    ap.add_argument("--transcripts", action="store_true",
+                    help="the reference's eight published runs on the GPU, beside the published CPU times")
     args = ap.parse_args()
+    if args.transcripts:
+        bench_transcripts()
+        return
     if args.c5:
         bench_c5()
         return

@@ -37,7 +37,7 @@ EXPORTS = ("h2w_default_opts", "h2w_create", "h2w_assemble", "h2w_matvec", "h2w_
            "h2w_near_list", "h2w_destroy", "h2w_last_error", "h2w_bench_matvec", "h2w_set_inner_tolerance",
            "h2w_autotune", "h2w_gmres_multi", "h2w_bench_matvec_multi", "h2w_batch_create", "h2w_batch_matvec",
            "h2w_batch_gmres", "h2w_batch_bench_matvec", "h2w_batch_destroy", "h2w_set_rank_model",
-           "h2w_rank_samples", "h2w_rank_predict")
+           "h2w_rank_samples", "h2w_rank_predict", "h2w_save", "h2w_load")
 RANK_FEATURES, RANK_PARAMS = 16, 9473
 TUNE_UNROLL, TUNE_INNER = 1, 2
 
@@ -129,6 +129,8 @@ def lib():
     L.h2w_rank_samples.argtypes = [p, p, p, i64]
     L.h2w_rank_samples.restype = i64
     L.h2w_rank_predict.argtypes = [p, p, i64, p]
+    L.h2w_save.argtypes = [p, ctypes.c_char_p]
+    L.h2w_load.argtypes = [ctypes.c_char_p, ctypes.POINTER(Opts), ctypes.POINTER(p)]
     L.h2w_gmres_multi.argtypes = [p, p, i64, p, i64, i, i, d, i, i, ctypes.POINTER(GmresStats)]
     L.h2w_autotune.argtypes = [p, p, i, d, i, i, ctypes.POINTER(Tune)]
     _lib = L
@@ -277,6 +279,28 @@ class Initialisation:
         return {"ms_per_matvec": r.ms_per_matvec, "launch_ms": list(r.launch_ms[:n]),
                 "launch_bytes": list(r.launch_bytes[:n]), "launch_role": list(r.launch_role[:n])}
 
+
+    def save(self, path):
+        """Write the assembled operator's skeletons (h2w_save)."""
+        _check(lib().h2w_save(self._h, os.fsencode(path)))
+
+    @classmethod
+    def load(cls, path, **opts):
+        """An assembled Initialisation from a file written by save(); no cross approximation runs."""
+        o = Opts()
+        lib().h2w_default_opts(ctypes.byref(o))
+        for k, v in opts.items():
+            setattr(o, k, v)
+        h = ctypes.c_void_p()
+        _check(lib().h2w_load(os.fsencode(path), ctypes.byref(o), ctypes.byref(h)))
+        self = cls.__new__(cls)
+        self._h = h
+        self._pts = None
+        st = Stats()
+        _check(lib().h2w_get_stats(h, ctypes.byref(st)))
+        self.dim, self.N = st.dim, st.N
+        self.nLevels = st.nlevels
+        return self
 
     def set_rank_model(self, params):
         """BF16 parameters (uint16 array of RANK_PARAMS) of the rank predictor; None removes it.

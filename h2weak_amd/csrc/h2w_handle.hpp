@@ -237,6 +237,9 @@ namespace h2w {
 
 using Handle = h2w_handle;
 
+// assembly from skeletons already in h.fam / h.pairs (h2w_ops.hip)
+void rebuild_operators(Handle &h);
+
 // matvec runtime (h2w_ops.hip)
 bool is32(int precision);
 const std::vector<Launch> &launches_for(Handle &h, int precision);

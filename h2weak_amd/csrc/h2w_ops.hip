@@ -947,6 +947,8 @@ void build_inner(Handle &h, int irp) {
 
 // ------------------------------------------------------------------ assembly
 
+void assemble_operators(Handle &h, double t0);
+
 void assemble(Handle &h) {
     drop_graphs(h);
     double t0 = now_s();
@@ -964,8 +966,6 @@ void assemble(Handle &h) {
         }
     };
     const Plan &P = h.plan;
-    hipStream_t s = h.stream;
-    h.aca_reruns = 0;
     {
         AcaPool pool;
         if (P.dim >= 2 && h.variant != H2W_NON_NESTED) far_nca(h, pool);
@@ -978,6 +978,23 @@ void assemble(Handle &h) {
         }
         phase("vertex NCA / CUR");
     }
+    assemble_operators(h, t0);
+}
+
+// Everything after the skeletons: pools, transfer / M2L / near-field / CUR blocks,
+// the FP32 shadow, work lists and the inner operator.  Reads h.fam (ranks, pivots,
+// L and R, ACA histories, row sets S) and h.pairs only, so it also rebuilds an
+// operator from saved skeletons.
+void assemble_operators(Handle &h, double t0) {
+    const bool verbose = std::getenv("H2W_VERBOSE") != nullptr;
+    auto phase = [&](const char *what) {
+        if (verbose) {
+            (void)hipStreamSynchronize(h.stream);
+            std::fprintf(stderr, "[h2w] %-28s %8.3f s\n", what, now_s() - t0);
+        }
+    };
+    const Plan &P = h.plan;
+    hipStream_t s = h.stream;
     const int nfam_lo[2] = {2, 1};  // first level of each family
     const bool fam_on[2] = {P.dim >= 2 && h.variant != H2W_NON_NESTED, h.variant == H2W_NESTED};
 
@@ -1403,6 +1420,12 @@ void assemble(Handle &h) {
 // ------------------------------------------------------------------ matvec
 // y_tree = A x_tree inside the vector pool (x already at off_x)
 }  // namespace
+
+// operator phase of the assembly from skeletons already in the handle (loading a saved operator)
+void rebuild_operators(Handle &h) {
+    drop_graphs(h);
+    assemble_operators(h, now_s());
+}
 
 bool is32(int precision) { return precision == H2W_FP32 || precision == H2W_FP32_OTF || precision == H2W_FP32_INNER; }
 

@@ -217,6 +217,16 @@ typedef struct {
 } h2w_tune;
 int h2w_autotune(h2w_handle *h, const double *b, int where, double tol, int max_iter, int flags, h2w_tune *out);
 
+/* ---- saving and loading an assembled operator (no counterpart in the reference, which re-assembles on
+ * every run).  The file holds the skeletons (per box the rank, row and column pivots, L and R, and the
+ * cross's per-step history; per semi-nested / non-nested pair the same), the input points and the build
+ * parameters -- not the operator pools.  h2w_load rebuilds the plan and re-evaluates every operator block
+ * with the same kernels: the loaded operator equals the saved one bit for bit, and no cross approximation
+ * runs.  The file ends with an FNV-1a hash; truncated or altered files are refused (H2W_EINVAL). */
+int h2w_save(const h2w_handle *h, const char *path);
+/* opts: device, fp32_shadow, ir_tol_pow, ... of the new handle (nlevels and compat come from the file). */
+int h2w_load(const char *path, const h2w_opts *opts, h2w_handle **out);
+
 /* ---- learned rank predictor (no counterpart in the reference).  A 16-64-64-64-1 perceptron (GELU,
  * BF16 parameters, FP32 accumulation, evaluated on the GPU) predicts log2(1 + rank) of every ACA block
  * from 16 geometry features (family, level, row/column and point counts, interaction-set size and
