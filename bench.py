@@ -371,8 +371,8 @@ def main():
     if not args.no_tune:
         t0 = time.time()
         tune = A.autotune(b, tol=gtol, max_iter=500)
-        log(f"autotune {time.time() - t0:.1f} s: inner tolerance 1e-{tune['ir_tol_pow']}; "
-            + ", ".join(f"1e-{c['ir_tol_pow']}: {c['seconds'] * 1e3:.1f} ms ({c['iterations']}+{c['outer']})"
+        log(f"autotune {time.time() - t0:.1f} s: inner tolerance 10^-{tune['ir_tol_exp']:g}; "
+            + ", ".join(f"10^-{c['ir_tol_exp']:g}: {c['seconds'] * 1e3:.1f} ms ({c['iterations']}+{c['outer']})"
                         for c in tune["candidates"])
             + f"; matvec ms FP64/FP32/inner {['%.3f' % v for v in tune['matvec_ms_before']]} -> "
             f"{['%.3f' % v for v in tune['matvec_ms_after']]}")

@@ -63,7 +63,8 @@ class Stats(ctypes.Structure):
                 ("stored_bytes_fp32", ctypes.c_int64), ("n_launches_matvec", ctypes.c_int),
                 ("aca_reruns", ctypes.c_int), ("stored_bytes_inner_fp32", ctypes.c_int64),
                 ("aca_batches", ctypes.c_int), ("aca_pool_bytes", ctypes.c_int64), ("rank_pred_blocks", ctypes.c_int),
-                ("rank_pred_under", ctypes.c_int), ("rank_pred_mae_log2", ctypes.c_double)]
+                ("rank_pred_under", ctypes.c_int), ("rank_pred_mae_log2", ctypes.c_double),
+                ("inner_tol_exp", ctypes.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -84,10 +85,10 @@ class BenchResult(ctypes.Structure):
 
 
 class Tune(ctypes.Structure):
-    _fields_ = [("ir_tol_pow", ctypes.c_int), ("n_cand", ctypes.c_int), ("cand_ir_tol_pow", ctypes.c_int * 8),
-                ("cand_seconds", ctypes.c_double * 8), ("cand_iterations", ctypes.c_int * 8),
-                ("cand_outer", ctypes.c_int * 8), ("cand_converged", ctypes.c_int * 8),
-                ("cand_inner_bytes", ctypes.c_int64 * 8), ("matvec_ms_before", ctypes.c_double * 3),
+    _fields_ = [("ir_tol_exp", ctypes.c_double), ("n_cand", ctypes.c_int), ("cand_ir_tol_exp", ctypes.c_double * 12),
+                ("cand_seconds", ctypes.c_double * 12), ("cand_iterations", ctypes.c_int * 12),
+                ("cand_outer", ctypes.c_int * 12), ("cand_converged", ctypes.c_int * 12),
+                ("cand_inner_bytes", ctypes.c_int64 * 12), ("matvec_ms_before", ctypes.c_double * 3),
                 ("matvec_ms_after", ctypes.c_double * 3)]
 
 
@@ -343,8 +344,8 @@ class Initialisation:
         _check(lib().h2w_autotune(self._h, _ptr(bb) if bb is not None else None, HOST, float(tol),
                                   int(max_iter), int(flags), ctypes.byref(t)))
         n = t.n_cand
-        return {"ir_tol_pow": t.ir_tol_pow,
-                "candidates": [{"ir_tol_pow": t.cand_ir_tol_pow[q], "seconds": t.cand_seconds[q],
+        return {"ir_tol_exp": t.ir_tol_exp,
+                "candidates": [{"ir_tol_exp": t.cand_ir_tol_exp[q], "seconds": t.cand_seconds[q],
                                 "iterations": t.cand_iterations[q], "outer": t.cand_outer[q],
                                 "converged": bool(t.cand_converged[q]), "inner_bytes": t.cand_inner_bytes[q]}
                                for q in range(n)],

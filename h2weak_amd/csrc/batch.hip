@@ -191,7 +191,7 @@ Lockstep lockstep_of(Batch &B) {
     L.blk = B.blk.p;
     L.nblk = B.nblk;
     L.sblk = B.sblk.p;
-    for (auto *h : B.sys) L.ir_floor.push_back(std::pow(10.0, -1.0 * std::max(1, h->opts.ir_tol_pow)));
+    for (auto *h : B.sys) L.ir_floor.push_back(std::pow(10.0, -std::max(1.0, h->ir_exp)));
     L.matvec = [&B](const double *x, double *y, int prec) { batch_matvec_lock(B, x, y, prec); };
     L.buf = &B.buf;
     return L;

@@ -217,6 +217,7 @@ struct h2w_handle {
     bool use_graph = true;
     int64_t inner_elems = 0;
     int inner_gen = 0;                  // incremented whenever the inner operator is rebuilt
+    double ir_exp = 0.0;                // the inner operator's tolerance exponent p (10^-p), 0: none
     h2w::DevBuf<h2w::GItem> d_items;
     h2w::DevBuf<h2w::GTerm> d_terms;
     std::vector<h2w::GItem> h_items;    // host copies of the main lists
@@ -266,7 +267,7 @@ struct Lockstep {
     const int2 *blk = nullptr;        // dot-product blocks (device): [first, one-past-last) slot
     int nblk = 0;
     const int32_t *sblk = nullptr;    // blocks of system s: sblk[s] .. sblk[s+1]-1 (device)
-    std::vector<double> ir_floor;     // per system: 10^-ir_tol_pow of its inner operator
+    std::vector<double> ir_floor;     // per system: 10^-p of its inner operator
     std::function<void(const double *, double *, int)> matvec;  // Y = A X, interleaved, a precision
     LockBufs *buf = nullptr;
 };

@@ -656,7 +656,7 @@ void upload_lists(std::vector<WorkList> &WL, LaunchSet &LS, hipStream_t s) {
 // (work items flagged with pool 1); rebuilt by h2w_set_inner_tolerance and the
 // autotuner.
 
-void build_inner(Handle &h, int irp) {
+void build_inner(Handle &h, double irp) {
     drop_graph(h, H2W_FP32_INNER);
     ++h.inner_gen;
     const Plan &P = h.plan;
@@ -942,7 +942,7 @@ void build_inner(Handle &h, int irp) {
         h.inner_elems = 0;
     }
     H2W_HIP(hipStreamSynchronize(s));
-    h.opts.ir_tol_pow = inner_on ? irp : 0;
+    h.ir_exp = inner_on ? irp : 0.0;
 }
 
 // ------------------------------------------------------------------ assembly
@@ -1407,7 +1407,7 @@ void assemble_operators(Handle &h, double t0) {
         se += h.launch_elems[q];
     }
     h.stored_elems = se;
-    build_inner(h, h.opts.ir_tol_pow);
+    build_inner(h, (double)h.opts.ir_tol_pow);
     H2W_HIP(hipStreamSynchronize(s));
     phase("work lists");
     if (verbose)
@@ -1713,7 +1713,7 @@ h2w_gmres_stats solve_tree(Handle &h, const double *bt, int max_iter, double tol
             double nr = dev_norm(h, h.g_r.p);
             rn = nr / normb;
             if (rn <= tol || total >= max_iter || outer >= 50) break;
-            const double floor_eta = mode == H2W_GMRES_FP32_INNER_IR ? std::pow(10.0, -1.0 * h.opts.ir_tol_pow) : 1e-6;
+            const double floor_eta = mode == H2W_GMRES_FP32_INNER_IR ? std::pow(10.0, -h.ir_exp) : 1e-6;
             double eta = std::min(1e-2, std::max(floor_eta, 0.1 * tol / rn));
             launch_fill(h.g_d.p, N, 0.0, s);
             auto o = gmres_core(h, h.g_r.p, h.g_d.p, max_iter - total, eta, inner_prec, true, nr);
@@ -1814,43 +1814,54 @@ void tune_unroll(Handle &h, int precision) {
     drop_graph(h, precision);
 }
 
-// Inner tolerance of the refinement: the fastest converged solve of A x = bt (x0 = 0)
-// over 10^-p, p = 4..8 below tol_pow.
+// Inner tolerance of the refinement: 10^-p for p = 4, 4.5, ..., 8 below tol_pow.  Each
+// candidate solves A x = bt from x0 = 0 and from x0 = 1 (the reference's start); its score
+// is the slower of the two, raised by a fifth when a final residual sits within 3x of the
+// tolerance (a rounding change could then cost another refinement step).  The best score
+// is kept.
 void tune_inner(Handle &h, const double *bt, double tol, int max_iter, bool unroll, h2w_tune *out) {
-    const int p0 = h.opts.ir_tol_pow;
-    int best = 0;
-    double tbest = 1e300;
+    const double p0 = h.ir_exp > 0 ? h.ir_exp : (double)h.opts.ir_tol_pow;
+    double best = 0.0, sbest = 1e300;
     out->n_cand = 0;
-    for (int p = 4; p <= 8 && p < h.tol_pow && out->n_cand < H2W_TUNE_MAX_CAND; ++p) {
+    for (int q = 8; q <= 16 && 0.5 * q < h.tol_pow && out->n_cand < H2W_TUNE_MAX_CAND; ++q) {
+        const double p = 0.5 * q;
         build_inner(h, p);
         if (h.inner.launches.empty()) continue;
         if (unroll) tune_unroll(h, H2W_FP32_INNER);
-        time_matvec(h, H2W_FP32_INNER, 1);  // captures the graph outside the timed solves
+        time_matvec(h, H2W_FP32_INNER, 1);  // captures the graphs outside the timed solves
         time_matvec(h, H2W_FP64, 1);
-        h2w_gmres_stats S{}, S2{};
-        for (int rep = 0; rep < 2; ++rep) {
-            launch_fill(h.g_x.p, h.N, 0.0, h.stream);
-            S2 = solve_tree(h, bt, max_iter, tol, H2W_GMRES_FP32_INNER_IR);
-            if (rep == 0 || S2.seconds < S.seconds) S = S2;
+        h2w_gmres_stats S{};
+        double worst = 0.0, rmax = 0.0;
+        bool conv = true;
+        for (int start = 0; start < 2; ++start) {
+            launch_fill(h.g_x.p, h.N, start == 0 ? 0.0 : 1.0, h.stream);
+            h2w_gmres_stats S2 = solve_tree(h, bt, max_iter, tol, H2W_GMRES_FP32_INNER_IR);
+            conv = conv && S2.converged;
+            rmax = std::max(rmax, S2.resid);
+            if (S2.seconds > worst) {
+                worst = S2.seconds;
+                S = S2;
+            }
         }
+        const double score = worst * (rmax > tol / 3.0 ? 1.2 : 1.0);
         const int c = out->n_cand++;
-        out->cand_ir_tol_pow[c] = p;
-        out->cand_seconds[c] = S.seconds;
+        out->cand_ir_tol_exp[c] = p;
+        out->cand_seconds[c] = worst;
         out->cand_iterations[c] = S.iterations;
         out->cand_outer[c] = S.outer_iterations;
-        out->cand_converged[c] = S.converged;
+        out->cand_converged[c] = conv;
         out->cand_inner_bytes[c] = h.inner_elems * 4;
-        if (S.converged && S.seconds < tbest) {
-            tbest = S.seconds;
+        if (conv && score < sbest) {
+            sbest = score;
             best = p;
         }
     }
-    if (best == 0) best = p0;
-    if (best != h.opts.ir_tol_pow || h.inner.launches.empty()) {
+    if (best == 0.0) best = p0;
+    if (best != h.ir_exp || h.inner.launches.empty()) {
         build_inner(h, best);
         if (unroll) tune_unroll(h, H2W_FP32_INNER);
     }
-    out->ir_tol_pow = h.opts.ir_tol_pow;
+    out->ir_tol_exp = h.ir_exp;
 }
 }  // namespace h2w
 
@@ -2187,6 +2198,7 @@ int h2w_get_stats(const h2w_handle *h, h2w_stats *st) {
     st->assembly_seconds = h->assembly_seconds;
     st->aca_reruns = h->aca_reruns;
     st->aca_batches = h->aca_batches;
+    st->inner_tol_exp = h->inner.launches.empty() ? 0.0 : h->ir_exp;
     st->aca_pool_bytes = h->aca_pool_bytes;
     st->rank_pred_blocks = h->rank_pred_blocks;
     st->rank_pred_under = h->rank_pred_under;
@@ -2351,7 +2363,7 @@ int h2w_set_inner_tolerance(h2w_handle *h, int ir_tol_pow) {
     if (ir_tol_pow < 0 || (ir_tol_pow > 0 && ir_tol_pow >= h->tol_pow))
         throw Error{H2W_EINVAL, "ir_tol_pow must be 0 or in (0, tol_pow)"};
     H2W_HIP(hipSetDevice(h->opts.device));
-    h2w::build_inner(*h, ir_tol_pow);
+    h2w::build_inner(*h, (double)ir_tol_pow);
     return H2W_OK;
     H2W_CATCH
 }
@@ -2393,7 +2405,7 @@ int h2w_autotune(h2w_handle *h, const double *b, int where, double tol, int max_
         h2w::tune_inner(*h, bt.p, tol, max_iter, (flags & H2W_TUNE_UNROLL) != 0, out);
     } else {
         if ((flags & H2W_TUNE_UNROLL) && have(H2W_FP32_INNER)) h2w::tune_unroll(*h, H2W_FP32_INNER);
-        out->ir_tol_pow = h->inner.launches.empty() ? 0 : h->opts.ir_tol_pow;
+        out->ir_tol_exp = h->inner.launches.empty() ? 0.0 : h->ir_exp;
     }
     for (int q = 0; q < 3; ++q) out->matvec_ms_after[q] = have(precs[q]) ? h2w::time_matvec(*h, precs[q], 5) : 0.0;
     H2W_HIP(hipStreamSynchronize(s));

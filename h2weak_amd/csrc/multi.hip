@@ -404,7 +404,7 @@ void solve_tree_m(Handle &h, const double *B, double *X, int nrhs, int max_iter,
     L.blk = h.m_blk.p;
     L.nblk = h.m_nblk;
     L.sblk = h.m_sblk.p;
-    L.ir_floor = {std::pow(10.0, -1.0 * h.opts.ir_tol_pow)};
+    L.ir_floor = {std::pow(10.0, -std::max(1.0, h.ir_exp))};
     L.matvec = [&h](const double *x, double *y, int prec) { matvec_m_tree(h, x, y, prec); };
     L.buf = &h.mbuf;
     lock_solve(L, B, X, nrhs, max_iter, tol, mode, st);

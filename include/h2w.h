@@ -100,6 +100,7 @@ typedef struct {
     int rank_pred_blocks;    /* blocks whose workspace came from the rank model (0: no model)      */
     int rank_pred_under;     /* of those, blocks whose rank exceeded the estimate (re-run)         */
     double rank_pred_mae_log2; /* mean |log2(1 + predicted) - log2(1 + rank)| over those blocks    */
+    double inner_tol_exp;    /* tolerance exponent p (10^-p) of the inner operator, 0 if none          */
 } h2w_stats;
 
 typedef struct {
@@ -196,20 +197,21 @@ int h2w_set_inner_tolerance(h2w_handle *h, int ir_tol_pow);
 
 /* Autotuner (no counterpart in the reference).  With H2W_TUNE_UNROLL it times every launch of
  * the FP64, FP32 and FP32-inner matvecs with 2, 4 and 8 column loads in flight per lane and
- * keeps the fastest per launch.  With H2W_TUNE_INNER it solves A x = b (x0 = 0) with
- * H2W_GMRES_FP32_INNER_IR for each inner tolerance 10^-p, p in 4..8 below tol_pow, and keeps
- * the fastest solve that converges.  b has N entries in input order, on the host or the device
- * (`where`); the handle's own x is unchanged. */
+ * keeps the fastest per launch.  With H2W_TUNE_INNER it rebuilds the inner operator for
+ * tolerances 10^-p, p = 4, 4.5, ..., 8 below tol_pow, solves A x = b with
+ * H2W_GMRES_FP32_INNER_IR from x0 = 0 and from x0 = 1, scores each p by the slower solve
+ * (raised by a fifth when a final residual sits within 3x of tol) and keeps the best.
+ * b has N entries in input order, on the host or the device (`where`). */
 #define H2W_TUNE_UNROLL 1
 #define H2W_TUNE_INNER 2
-#define H2W_TUNE_MAX_CAND 8
+#define H2W_TUNE_MAX_CAND 12
 typedef struct {
-    int ir_tol_pow;                          /* chosen inner exponent (0: no inner operator) */
+    double ir_tol_exp;                       /* chosen inner exponent p (0: no inner operator) */
     int n_cand;
-    int cand_ir_tol_pow[H2W_TUNE_MAX_CAND];
-    double cand_seconds[H2W_TUNE_MAX_CAND];  /* solve time, best of two runs */
-    int cand_iterations[H2W_TUNE_MAX_CAND];  /* inner (FP32) GMRES iterations */
-    int cand_outer[H2W_TUNE_MAX_CAND];       /* refinement steps */
+    double cand_ir_tol_exp[H2W_TUNE_MAX_CAND];
+    double cand_seconds[H2W_TUNE_MAX_CAND];  /* the slower of the two solves */
+    int cand_iterations[H2W_TUNE_MAX_CAND];  /* inner (FP32) GMRES iterations of that solve */
+    int cand_outer[H2W_TUNE_MAX_CAND];       /* refinement steps of that solve */
     int cand_converged[H2W_TUNE_MAX_CAND];
     int64_t cand_inner_bytes[H2W_TUNE_MAX_CAND]; /* FP32 bytes one inner matvec streams */
     double matvec_ms_before[3];              /* FP64, FP32, FP32 inner matvec before the unroll tuning */

@@ -61,11 +61,10 @@ def test_autotune_keeps_parity(pair):
     with pytest.raises(h2w.H2WError):
         g.autotune(None, flags=h2w.TUNE_INNER)
     t = g.autotune(b, tol=1e-10, max_iter=300)
-    assert [c["ir_tol_pow"] for c in t["candidates"]] == [4, 5, 6, 7, 8]
+    assert [c["ir_tol_exp"] for c in t["candidates"]] == [4, 4.5, 5, 5.5, 6, 6.5, 7, 7.5, 8]
     assert all(c["converged"] for c in t["candidates"])
-    assert t["ir_tol_pow"] in (4, 5, 6, 7, 8)
-    best = min(t["candidates"], key=lambda c: c["seconds"])
-    assert best["ir_tol_pow"] == t["ir_tol_pow"]
+    assert t["ir_tol_exp"] in [c["ir_tol_exp"] for c in t["candidates"]]
+    assert g.stats()["inner_tol_exp"] == t["ir_tol_exp"]
     assert all(v > 0 for v in t["matvec_ms_after"])
     # products after tuning: same operator, other summation order at most
     assert _rel(g.mat_vec_prod(x), ref) <= 1e-12
