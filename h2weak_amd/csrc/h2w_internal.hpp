@@ -138,6 +138,26 @@ constexpr int kRowTile = 128;  // rows of one GEMV item
 constexpr int kNR = 16;        // right-hand sides of one multi-vector pass
 
 // ------------------------------------------------------------------ launch wrappers (kernels.hip)
+// wide cross for a few very large blocks (state per block, partials per workgroup)
+struct WideState {
+    int32_t rank, rp, cp, l, next_rp, done, status, found, dots_rank, pad;
+    double piv, vn2, un, vn, normS;
+    int64_t off;
+};
+struct WidePart {
+    double ss, a;
+    int32_t idx, pad;
+    double sv;
+};
+void launch_wide_init(const AcaJob *jobs, int nb, int maxmn, WideState *state, uint8_t *flags, int64_t *tab,
+                      unsigned long long *ws_top, int64_t ws_cap, hipStream_t s);
+// one pivot step of every block; maxrank bounds the ranks reached so far (+1)
+void launch_wide_step(const AcaJob *jobs, int nb, int maxm, int maxn, int maxrank, const int32_t *idx, double *ws,
+                      int64_t *tab, uint8_t *flags, WideState *state, WidePart *part, int pstride, double *dots,
+                      int dstride, int32_t *rowpiv, int32_t *colpiv, double *hist, unsigned long long *ws_top,
+                      int64_t ws_cap, KParams kp, double tol, hipStream_t s);
+void launch_wide_finish(const WideState *state, int nb, int32_t *rank_out, int32_t *status_out, int32_t *ndone,
+                        hipStream_t s);
 void launch_aca(const AcaJob *jobs, int njobs, const int32_t *idx, double *ws, int64_t ws_cap,
                 unsigned long long *ws_top, int64_t *tab, uint8_t *flags, int32_t *rank, int32_t *status,
                 int32_t *rowpiv, int32_t *colpiv, KParams kp, double tol, double *hist, int big, hipStream_t s);

@@ -206,6 +206,49 @@ void block_features(const Handle &h, int fam, int j, int b, const std::vector<in
     B.has_feat = true;
 }
 
+// The wide cross of the first nh jobs of a batch (device copies at djobs): pivot steps
+// are enqueued 16 at a time; between groups the host reads how many blocks are done.
+// Every step advances a block's rank, moves its first-row search on, or finishes it, so
+// the loop ends within max(m) + max(tcap) + 1 steps.
+void run_wide(Handle &h, const std::vector<AcaJob> &jobs, size_t nh, const AcaJob *djobs, const int32_t *didx,
+              AcaPool &pool, int64_t *dtab, uint8_t *dflags, int32_t *drank, int32_t *dstat, int32_t *drp,
+              int32_t *dcp, double *dhist, double tol) {
+    hipStream_t s = h.stream;
+    const int nb = (int)nh;
+    int maxm = 0, maxn = 0, maxt = 0;
+    for (size_t q = 0; q < nh; ++q) {
+        maxm = std::max(maxm, jobs[q].m);
+        maxn = std::max(maxn, jobs[q].n);
+        maxt = std::max(maxt, jobs[q].tcap);
+    }
+    const int pstride = std::max(1, std::max((maxm + 2047) / 2048, (maxn + 2047) / 2048));
+    const int dstride = std::max(maxt, 1);
+    DevBuf<WideState> st;
+    DevBuf<WidePart> part;
+    DevBuf<double> dots;
+    DevBuf<int32_t> ndone;
+    st.alloc(nb);
+    part.alloc((size_t)nb * pstride);
+    dots.alloc((size_t)nb * dstride);
+    ndone.alloc(1);
+    launch_wide_init(djobs, nb, maxm + maxn, st.p, dflags, dtab, pool.top.p, pool.cap, s);
+    const int64_t limit = (int64_t)maxm + maxt + 1;
+    int64_t steps = 0;
+    while (true) {
+        for (int c = 0; c < 16; ++c, ++steps)
+            launch_wide_step(djobs, nb, maxm, maxn, (int)std::min<int64_t>(steps + 1, dstride), didx, pool.ws.p, dtab,
+                             dflags, st.p, part.p, pstride, dots.p, dstride, drp, dcp, dhist, pool.top.p, pool.cap,
+                             h.kp, tol, s);
+        launch_wide_finish(st.p, nb, drank, dstat, ndone.p, s);
+        H2W_HIP(hipGetLastError());
+        int nd = 0;
+        H2W_HIP(hipMemcpyAsync(&nd, ndone.p, 4, hipMemcpyDeviceToHost, s));
+        H2W_HIP(hipStreamSynchronize(s));
+        if (nd == nb) break;
+        if (steps > limit) throw Error{H2W_EHIP, "wide cross did not finish within its step bound"};
+    }
+}
+
 void run_aca(Handle &h, std::vector<AcaBlock> &blocks, int rank_hint, AcaPool &pool) {
     if (blocks.empty()) return;
     const double tol = std::pow(10.0, -1.0 * h.tol_pow);
@@ -254,7 +297,9 @@ void run_aca(Handle &h, std::vector<AcaBlock> &blocks, int rank_hint, AcaPool &p
     DevBuf<AcaJob> djobs;
     DevBuf<LrJob> dlr;
     DevBuf<double> dlrout;
-    const size_t kBig = 16384;  // m + n above which a block gets a 1024-thread workgroup
+    const size_t kBig = 16384;    // m + n above which a block gets a 1024-thread workgroup
+    // m + n above which a block's cross is spread over many workgroups (H2W_WIDE_MIN overrides)
+    const size_t kHuge = std::getenv("H2W_WIDE_MIN") ? (size_t)std::atoll(std::getenv("H2W_WIDE_MIN")) : 8192;
     while (!todo.empty()) {
         std::vector<size_t> rerun;
         size_t pos = 0;
@@ -272,13 +317,14 @@ void run_aca(Handle &h, std::vector<AcaBlock> &blocks, int rank_hint, AcaPool &p
                 ++pos;
                 if (batch.size() >= 65535) break;
             }
-            // big blocks first, each group contiguous
-            std::stable_partition(batch.begin(), batch.end(), [&](size_t b) {
-                return blocks[b].rows.size() + blocks[b].cols.size() > kBig;
-            });
-            size_t nbig = 0;
-            while (nbig < batch.size() && blocks[batch[nbig]].rows.size() + blocks[batch[nbig]].cols.size() > kBig)
-                ++nbig;
+            // huge blocks first (wide cross), then big ones (1024-thread workgroups), each group contiguous
+            auto mn_of = [&](size_t b) { return blocks[b].rows.size() + blocks[b].cols.size(); };
+            std::stable_partition(batch.begin(), batch.end(), [&](size_t b) { return mn_of(b) > kBig; });
+            std::stable_partition(batch.begin(), batch.end(), [&](size_t b) { return mn_of(b) > kHuge; });
+            size_t nhuge = 0, nbig = 0;
+            while (nhuge < batch.size() && mn_of(batch[nhuge]) > kHuge) ++nhuge;
+            nbig = nhuge;
+            while (nbig < batch.size() && mn_of(batch[nbig]) > kBig) ++nbig;
             std::vector<AcaJob> jobs(batch.size());
             std::vector<int32_t> idx;
             idx.reserve(isz);
@@ -313,9 +359,11 @@ void run_aca(Handle &h, std::vector<AcaBlock> &blocks, int rank_hint, AcaPool &p
             H2W_HIP(hipMemsetAsync(pool.top.p, 0, sizeof(unsigned long long), s));
             ++h.aca_batches;
             double tb = now_s();
-            if (nbig > 0)
-                launch_aca(djobs.p, (int)nbig, didx.p, pool.ws.p, pool.cap, pool.top.p, dtab.p, flags.p, drank.p,
-                           dstat.p, drp.p, dcp.p, h.kp, tol, dhist.p, 1, s);
+            if (nhuge > 0) run_wide(h, jobs, nhuge, djobs.p, didx.p, pool, dtab.p, flags.p, drank.p, dstat.p, drp.p,
+                                    dcp.p, dhist.p, tol);
+            if (nbig > nhuge)
+                launch_aca(djobs.p + nhuge, (int)(nbig - nhuge), didx.p, pool.ws.p, pool.cap, pool.top.p, dtab.p,
+                           flags.p, drank.p + nhuge, dstat.p + nhuge, drp.p, dcp.p, h.kp, tol, dhist.p, 1, s);
             if (batch.size() > nbig)
                 launch_aca(djobs.p + nbig, (int)(batch.size() - nbig), didx.p, pool.ws.p, pool.cap, pool.top.p,
                            dtab.p, flags.p, drank.p + nbig, dstat.p + nbig, drp.p, dcp.p, h.kp, tol, dhist.p, 0, s);
@@ -553,6 +601,34 @@ void cur_pairs(Handle &h, AcaPool &pool) {
     }
 }
 
+// The rows of a transfer (columns of a CUR row factor) are independent: sub-jobs of
+// 64 rows (256 columns) spread one large box over many workgroups.
+std::vector<XferJob> split_xfer(const std::vector<XferJob> &in) {
+    std::vector<XferJob> out;
+    for (const XferJob &J : in)
+        for (int r0 = 0; r0 < J.m; r0 += 64) {
+            XferJob K = J;
+            K.m = std::min(64, J.m - r0);
+            K.s = J.s + r0;
+            K.e = J.e + r0;
+            K.et = J.et + (int64_t)r0 * J.ldet;
+            out.push_back(K);
+        }
+    return out;
+}
+std::vector<CurJob> split_cur(const std::vector<CurJob> &in) {
+    std::vector<CurJob> out;
+    for (const CurJob &J : in)
+        for (int c0 = 0; c0 < J.n; c0 += 256) {
+            CurJob K = J;
+            K.n = std::min(256, J.n - c0);
+            K.v = J.v + c0;
+            K.a = J.a + (int64_t)c0 * J.lda;
+            out.push_back(K);
+        }
+    return out;
+}
+
 // ------------------------------------------------------------------ work lists
 constexpr int64_t kItemWork = 128 * 4096;  // elements of A per GEMV item before column chunking
 
@@ -776,6 +852,7 @@ void build_inner(Handle &h, double irp) {
         DevBuf<XferJob> dxj;
         didx.upload(idx, s);
         dlr.upload(lr, s);
+        xj = split_xfer(xj);
         dxj.upload(xj, s);
         tmp.alloc(in_len);
         H2W_HIP(hipMemsetAsync(tmp.p, 0, in_len * 8, s));
@@ -1205,7 +1282,9 @@ void assemble_operators(Handle &h, double t0) {
         DevBuf<CurJob> dcj;
         didx.upload(idx, s);
         dlr.upload(lr, s);
+        xj = split_xfer(xj);
         dxj.upload(xj, s);
+        cj = split_cur(cj);
         dcj.upload(cj, s);
         launch_xfer(dxj.p, (int)xj.size(), didx.p, dlr.p, h.mat64.p, h.kp, s);
         H2W_HIP(hipGetLastError());

@@ -385,6 +385,306 @@ __global__ __launch_bounds__(NT) void k_aca(const AcaJob *__restrict__ jobs, con
     }
 }
 
+// ------------------------------------------------------------------ wide ACA
+// The cross of k_aca for a few very large blocks, each spread over many
+// workgroups: one pivot step is a sequence of launches over all such blocks --
+// residual row (wide), row reduction, residual column (wide), column reduction
+// and acceptance, normS dot products (wide), normS update and the next step's
+// reservation.  Residual entries subtract the earlier terms in rank order as in
+// k_aca, so rows, columns and pivots are the same numbers; only the sums of
+// squares and dot products are reduced in another order.  Per-block state lives
+// in WideState; a finished block (converged, exhausted, or out of workspace:
+// status 1) is skipped by every later launch.
+constexpr int kWideCols = 2048;  // row / column entries one workgroup of the wide passes covers
+
+__device__ __forceinline__ void wide_reserve(WideState &st, const AcaJob &jb, int64_t *tab,
+                                             unsigned long long *ws_top, int64_t ws_cap) {
+    int64_t off = -1;
+    if (st.rank < jb.tcap) {
+        unsigned long long o = atomicAdd(ws_top, (unsigned long long)(jb.m + jb.n));
+        if ((int64_t)o + jb.m + jb.n <= ws_cap) off = (int64_t)o;
+    }
+    if (off < 0) {
+        st.status = 1;
+        st.done = 1;
+        return;
+    }
+    st.off = off;
+    tab[2 * st.rank] = off;
+    tab[2 * st.rank + 1] = off + jb.m;
+}
+
+// first launch: clear the used-row/column flags, reserve step 0, start the first-row search
+__global__ __launch_bounds__(256) void k_wide_init(const AcaJob *__restrict__ jobs, WideState *__restrict__ state,
+                                                   uint8_t *__restrict__ flags, int64_t *__restrict__ tab_all,
+                                                   unsigned long long *ws_top, int64_t ws_cap) {
+    const AcaJob jb = jobs[blockIdx.y];
+    const int64_t tot = (int64_t)jb.m + jb.n;
+    for (int64_t e = blockIdx.x * 256 + threadIdx.x; e < tot; e += (int64_t)gridDim.x * 256) flags[jb.wsF + e] = 0;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        WideState st{};
+        st.rank = 0;
+        st.rp = 0;
+        st.l = 0;
+        st.next_rp = INT_MAX;
+        st.done = (jb.m == 0 || jb.n == 0 || jb.tcap == 0);
+        if (!st.done) wide_reserve(st, jb, tab_all + jb.tab, ws_top, ws_cap);
+        state[blockIdx.y] = st;
+    }
+}
+
+// residual row rp -> V slot of the current step; partial sum of squares and argmax over unused columns
+__global__ __launch_bounds__(256) void k_wide_row(const AcaJob *__restrict__ jobs, const int32_t *__restrict__ idx,
+                                                  const double *__restrict__ wsr, double *__restrict__ ws,
+                                                  const int64_t *__restrict__ tab_all, const uint8_t *__restrict__ flags,
+                                                  const WideState *__restrict__ state, WidePart *__restrict__ part,
+                                                  int pstride, KParams kp) {
+    const AcaJob jb = jobs[blockIdx.y];
+    const WideState st = state[blockIdx.y];
+    const int s0 = blockIdx.x * kWideCols;
+    if (st.done || s0 >= jb.n) return;
+    const int s1 = min(jb.n, s0 + kWideCols);
+    const int rank = st.rank, rp = st.rp;
+    const int32_t *rows = idx + jb.rows, *cols = idx + jb.cols;
+    const int64_t *tab = tab_all + jb.tab;
+    const uint8_t *cu = flags + jb.wsF + jb.m;
+    double *Vr = ws + st.off + jb.m;
+    __shared__ double coef[kChunk];
+    __shared__ int64_t cofs[kChunk];
+    __shared__ BlockRed<256> red;
+    PointI pr = load_point(kp, rows[rp]);
+    AMax am{-1.0, INT_MAX, 0.0};
+    double vn2 = 0.0;
+    const int nch = max(1, (rank + kChunk - 1) / kChunk);
+    for (int ch = 0; ch < nch; ++ch) {
+        const int l0 = ch * kChunk, l1 = min(rank, l0 + kChunk);
+        for (int q = l0 + threadIdx.x; q < l1; q += 256) {
+            coef[q - l0] = wsr[tab[2 * q] + rp];
+            cofs[q - l0] = tab[2 * q + 1];
+        }
+        __syncthreads();
+        for (int s = s0 + threadIdx.x; s < s1; s += 256) {
+            double r = ch == 0 ? kentry(kp, pr, cols[s]) : Vr[s];
+            for (int q = l0; q < l1; ++q) r -= coef[q - l0] * wsr[cofs[q - l0] + s];
+            Vr[s] = r;
+            if (ch == nch - 1) {
+                vn2 += r * r;
+                if (!cu[s]) amax_update(am, r, s);
+            }
+        }
+        __syncthreads();
+    }
+    double dummy = 0.0;
+    block_reduce<256>(red, vn2, dummy, am);
+    if (threadIdx.x == 0) part[(int64_t)blockIdx.y * pstride + blockIdx.x] = WidePart{vn2, am.a, am.idx, 0, am.sv};
+}
+
+// reduce a wide pass's partials: sum of squares and the first argmax
+__device__ __forceinline__ void wide_reduce(const WidePart *p, int np, double &ss, AMax &am) {
+    __shared__ BlockRed<256> red;
+    ss = 0.0;
+    am = AMax{-1.0, INT_MAX, 0.0};
+    for (int w = threadIdx.x; w < np; w += 256) {
+        ss += p[w].ss;
+        AMax o{p[w].a, p[w].idx, p[w].sv};
+        amax_merge(am, o);
+    }
+    double dummy = 0.0;
+    block_reduce<256>(red, ss, dummy, am);
+}
+
+// row reduction: the pivot column (first step: a row with a nonzero entry, else the next row)
+__global__ __launch_bounds__(256) void k_wide_row_reduce(const AcaJob *__restrict__ jobs,
+                                                         WideState *__restrict__ state,
+                                                         const WidePart *__restrict__ part, int pstride) {
+    const AcaJob jb = jobs[blockIdx.x];
+    WideState st = state[blockIdx.x];
+    if (st.done) return;
+    const int np = (jb.n + kWideCols - 1) / kWideCols;
+    double vn2;
+    AMax am;
+    wide_reduce(part + (int64_t)blockIdx.x * pstride, np, vn2, am);
+    if (threadIdx.x != 0) return;
+    if (st.rank == 0) {
+        if (am.idx != INT_MAX && am.a > 1e-36) {
+            st.cp = am.idx;
+            st.piv = am.sv;
+            st.vn2 = vn2;
+            st.found = 1;
+        } else if (++st.l >= jb.m) {
+            st.done = 1;  // every row is zero: rank 0
+        } else {
+            st.rp = st.l;
+        }
+    } else if (am.idx == INT_MAX) {
+        st.done = 1;  // no unused column left
+    } else {
+        st.cp = am.idx;
+        st.piv = am.sv;
+        st.vn2 = vn2;
+    }
+    state[blockIdx.x] = st;
+}
+
+// residual column cp -> U slot (divided by the pivot); partial sum of squares, argmax over unused rows != rp
+__global__ __launch_bounds__(256) void k_wide_col(const AcaJob *__restrict__ jobs, const int32_t *__restrict__ idx,
+                                                  const double *__restrict__ wsr, double *__restrict__ ws,
+                                                  const int64_t *__restrict__ tab_all, const uint8_t *__restrict__ flags,
+                                                  const WideState *__restrict__ state, WidePart *__restrict__ part,
+                                                  int pstride, KParams kp) {
+    const AcaJob jb = jobs[blockIdx.y];
+    const WideState st = state[blockIdx.y];
+    const int t0 = blockIdx.x * kWideCols;
+    if (st.done || t0 >= jb.m || (st.rank == 0 && !st.found)) return;
+    const int t1 = min(jb.m, t0 + kWideCols);
+    const int rank = st.rank, rp = st.rp, cp = st.cp;
+    const double piv = st.piv;
+    const int32_t *rows = idx + jb.rows, *cols = idx + jb.cols;
+    const int64_t *tab = tab_all + jb.tab;
+    const uint8_t *ru = flags + jb.wsF;
+    double *Ur = ws + st.off;
+    __shared__ double coef[kChunk];
+    __shared__ int64_t cofs[kChunk];
+    __shared__ BlockRed<256> red;
+    PointI pc = load_point(kp, cols[cp]);
+    AMax an{-1.0, INT_MAX, 0.0};
+    double un2 = 0.0;
+    const int nch = max(1, (rank + kChunk - 1) / kChunk);
+    for (int ch = 0; ch < nch; ++ch) {
+        const int l0 = ch * kChunk, l1 = min(rank, l0 + kChunk);
+        for (int q = l0 + threadIdx.x; q < l1; q += 256) {
+            coef[q - l0] = wsr[tab[2 * q + 1] + cp];
+            cofs[q - l0] = tab[2 * q];
+        }
+        __syncthreads();
+        for (int t = t0 + threadIdx.x; t < t1; t += 256) {
+            double c = ch == 0 ? kentry(kp, pc, rows[t]) : Ur[t];
+            for (int q = l0; q < l1; ++q) c -= coef[q - l0] * wsr[cofs[q - l0] + t];
+            if (ch == nch - 1) {
+                double u = c / piv;
+                Ur[t] = u;
+                un2 += u * u;
+                if (t != rp && !ru[t]) amax_update(an, c, t);
+            } else {
+                Ur[t] = c;
+            }
+        }
+        __syncthreads();
+    }
+    double dummy = 0.0;
+    block_reduce<256>(red, un2, dummy, an);
+    if (threadIdx.x == 0) part[(int64_t)blockIdx.y * pstride + blockIdx.x] = WidePart{un2, an.a, an.idx, 0, an.sv};
+}
+
+// column reduction and acceptance of the step; the first step also checks the loop condition
+// and reserves the next step (later steps do that after the normS update)
+__global__ __launch_bounds__(256) void k_wide_accept(const AcaJob *__restrict__ jobs, WideState *__restrict__ state,
+                                                     const WidePart *__restrict__ part, int pstride,
+                                                     uint8_t *__restrict__ flags, int32_t *__restrict__ rowpiv,
+                                                     int32_t *__restrict__ colpiv, double *__restrict__ hist_all,
+                                                     int64_t *__restrict__ tab_all, unsigned long long *ws_top,
+                                                     int64_t ws_cap, double tol) {
+    const AcaJob jb = jobs[blockIdx.x];
+    WideState st = state[blockIdx.x];
+    if (st.done || (st.rank == 0 && !st.found)) return;
+    const int np = (jb.m + kWideCols - 1) / kWideCols;
+    double un2;
+    AMax an;
+    wide_reduce(part + (int64_t)blockIdx.x * pstride, np, un2, an);
+    if (threadIdx.x != 0) return;
+    const double un = sqrt(un2), vn = sqrt(st.vn2);
+    uint8_t *ru = flags + jb.wsF, *cu = ru + jb.m;
+    double *hist = hist_all + 2 * jb.piv;
+    if (st.rank > 0 && (un < 1e-36 || vn < 1e-36)) {
+        st.done = 1;  // the step is dropped
+        state[blockIdx.x] = st;
+        return;
+    }
+    rowpiv[jb.piv + st.rank] = st.rp;
+    colpiv[jb.piv + st.rank] = st.cp;
+    ru[st.rp] = 1;
+    cu[st.cp] = 1;
+    st.un = un;
+    st.vn = vn;
+    st.next_rp = an.idx;
+    ++st.rank;
+    if (st.rank == 1) {
+        // the first term enters without a normS update (normS stays 0)
+        hist[0] = un * vn;
+        hist[1] = 0.0;
+        st.normS = 0.0;
+        const int mn = min(jb.m, jb.n);
+        if (!(un * vn > tol * st.normS && st.rank < mn) || st.next_rp == INT_MAX) st.done = 1;
+        else {
+            st.rp = st.next_rp;
+            wide_reserve(st, jb, tab_all + jb.tab, ws_top, ws_cap);
+        }
+    }
+    state[blockIdx.x] = st;
+}
+
+// |<U_q, u><V_q, v>| of the newest term (u, v) with every term q <= newest, one workgroup per q
+__global__ __launch_bounds__(256) void k_wide_dots(const AcaJob *__restrict__ jobs, const double *__restrict__ ws,
+                                                   const int64_t *__restrict__ tab_all,
+                                                   const WideState *__restrict__ state, double *__restrict__ dots,
+                                                   int dstride) {
+    const AcaJob jb = jobs[blockIdx.y];
+    const WideState st = state[blockIdx.y];
+    const int q = blockIdx.x;
+    if (st.done || st.rank < 2 || q >= st.rank || st.dots_rank == st.rank) return;
+    const int64_t *tab = tab_all + jb.tab;
+    const double *Uq = ws + tab[2 * q], *Vq = ws + tab[2 * q + 1];
+    const double *Uk = ws + tab[2 * (st.rank - 1)], *Vk = ws + tab[2 * (st.rank - 1) + 1];
+    double du = 0.0, dv = 0.0;
+    for (int t = threadIdx.x; t < jb.m; t += 256) du += Uq[t] * Uk[t];
+    for (int s = threadIdx.x; s < jb.n; s += 256) dv += Vq[s] * Vk[s];
+    __shared__ BlockRed<256> red;
+    AMax dummy{-1.0, INT_MAX, 0.0};
+    block_reduce<256>(red, du, dv, dummy);
+    if (threadIdx.x == 0) dots[(int64_t)blockIdx.y * dstride + q] = fabs(du * dv);
+}
+
+// normS update of the newest term, loop condition, next step's reservation
+__global__ __launch_bounds__(256) void k_wide_norm(const AcaJob *__restrict__ jobs, WideState *__restrict__ state,
+                                                   const double *__restrict__ dots, int dstride,
+                                                   double *__restrict__ hist_all, int64_t *__restrict__ tab_all,
+                                                   unsigned long long *ws_top, int64_t ws_cap, double tol) {
+    const AcaJob jb = jobs[blockIdx.x];
+    WideState st = state[blockIdx.x];
+    if (st.done || st.rank < 2 || st.dots_rank == st.rank) return;
+    double S = 0.0, dummy = 0.0;
+    for (int q = threadIdx.x; q < st.rank; q += 256) S += dots[(int64_t)blockIdx.x * dstride + q];
+    __shared__ BlockRed<256> red;
+    AMax am{-1.0, INT_MAX, 0.0};
+    block_reduce<256>(red, S, dummy, am);
+    if (threadIdx.x != 0) return;
+    const double uv = st.un * st.vn;
+    st.normS = sqrt(st.normS * st.normS + uv * uv + 2.0 * S);
+    double *hist = hist_all + 2 * jb.piv;
+    hist[2 * (st.rank - 1)] = uv;
+    hist[2 * (st.rank - 1) + 1] = st.normS;
+    st.dots_rank = st.rank;
+    const int mn = min(jb.m, jb.n);
+    if (!(uv > tol * st.normS && st.rank < mn) || st.next_rp == INT_MAX) st.done = 1;
+    else {
+        st.rp = st.next_rp;
+        wide_reserve(st, jb, tab_all + jb.tab, ws_top, ws_cap);
+    }
+    state[blockIdx.x] = st;
+}
+
+__global__ void k_wide_finish(const WideState *__restrict__ state, int nb, int32_t *__restrict__ rank_out,
+                              int32_t *__restrict__ status_out, int32_t *__restrict__ ndone) {
+    int done = 0;
+    for (int b = threadIdx.x; b < nb; b += blockDim.x) {
+        rank_out[b] = state[b].rank;
+        status_out[b] = state[b].status;
+        done += state[b].done;
+    }
+    done = (int)wave_sum((double)done);
+    if (threadIdx.x == 0) *ndone = done;
+}
+
 // L (unit lower) and R (upper) of the cross: L(i,j) = U_j[rb_i], R(j,i) = V_j[cb_i]
 // (2D/codes/aca.hpp:419-445)
 __global__ void k_lr_extract(const LrJob *__restrict__ jobs, const double *__restrict__ ws,
@@ -1320,5 +1620,33 @@ void launch_scatter_b(const double *v, const int64_t *src, const int32_t *ld, in
 }
 void launch_rank_mlp(const float *feat, int n, const uint16_t *w, float *out, hipStream_t s) {
     if (n > 0) hipLaunchKernelGGL(k_rank_mlp, dim3((n + 255) / 256), dim3(256), 0, s, feat, n, w, out);
+}
+void launch_wide_init(const AcaJob *jobs, int nb, int maxmn, WideState *state, uint8_t *flags, int64_t *tab,
+                      unsigned long long *ws_top, int64_t ws_cap, hipStream_t s) {
+    hipLaunchKernelGGL(k_wide_init, dim3(std::max(1, std::min(1024, (maxmn + 2047) / 2048)), nb), dim3(256), 0, s,
+                       jobs, state, flags, tab, ws_top, ws_cap);
+}
+void launch_wide_step(const AcaJob *jobs, int nb, int maxm, int maxn, int maxrank, const int32_t *idx, double *ws,
+                      int64_t *tab, uint8_t *flags, WideState *state, WidePart *part, int pstride, double *dots,
+                      int dstride, int32_t *rowpiv, int32_t *colpiv, double *hist, unsigned long long *ws_top,
+                      int64_t ws_cap, KParams kp, double tol, hipStream_t s) {
+    const int gr = std::max(1, (maxn + kWideCols - 1) / kWideCols), gc = std::max(1, (maxm + kWideCols - 1) / kWideCols);
+    hipLaunchKernelGGL(k_wide_row, dim3(gr, nb), dim3(256), 0, s, jobs, idx, ws, ws, tab, flags, state, part, pstride,
+                       kp);
+    hipLaunchKernelGGL(k_wide_row_reduce, dim3(nb), dim3(256), 0, s, jobs, state, part, pstride);
+    hipLaunchKernelGGL(k_wide_col, dim3(gc, nb), dim3(256), 0, s, jobs, idx, ws, ws, tab, flags, state, part, pstride,
+                       kp);
+    hipLaunchKernelGGL(k_wide_accept, dim3(nb), dim3(256), 0, s, jobs, state, part, pstride, flags, rowpiv, colpiv,
+                       hist, tab, ws_top, ws_cap, tol);
+    if (maxrank >= 2) {
+        hipLaunchKernelGGL(k_wide_dots, dim3(std::max(1, maxrank), nb), dim3(256), 0, s, jobs, ws, tab, state, dots,
+                           dstride);
+        hipLaunchKernelGGL(k_wide_norm, dim3(nb), dim3(256), 0, s, jobs, state, dots, dstride, hist, tab, ws_top,
+                           ws_cap, tol);
+    }
+}
+void launch_wide_finish(const WideState *state, int nb, int32_t *rank_out, int32_t *status_out, int32_t *ndone,
+                        hipStream_t s) {
+    hipLaunchKernelGGL(k_wide_finish, dim3(1), dim3(64), 0, s, state, nb, rank_out, status_out, ndone);
 }
 }  // namespace h2w
