@@ -319,11 +319,24 @@ def main():
     ap.add_argument("--pmc-reduce", metavar="DIR",
                     help="read DIR/fetch_counter_collection.csv and DIR/write_counter_collection.csv written by "
                          "two rocprofv3 --pmc runs of --pmc-probe and write profiles/pmc_traffic.json (no GPU)")
+    ap.add_argument("--assembly-only", action="store_true",
+                    help="assemble the workload once (heuristic rank estimates) and print the time")
     ap.add_argument("--c5", action="store_true",
                     help="rank-predictor measurement (N = 262144 uniform and Gaussian, heuristic vs model)")
     ap.add_argument("--transcripts", action="store_true",
                     help="the reference's eight published runs on the GPU, beside the published CPU times")
     args = ap.parse_args()
+    if args.assembly_only:
+        import h2weak_amd as h2w
+        N = args.side ** 3
+        A = h2w.Initialisation(3, N, 64, 1.0, args.tol_pow, h2w.CHOICE_RBF, h2w.NESTED)
+        t0 = time.time()
+        A.tree_initialisation()
+        st = A.stats()
+        print(json.dumps({"workload": f"3D H2* RBF grid {args.side}^3 tol 1e-{args.tol_pow}",
+                          "assembly_s": round(time.time() - t0, 3), "memory": st["memory"],
+                          "max_rank_far": st["max_rank_far"], "max_rank_vertex": st["max_rank_vertex"]}), flush=True)
+        return
     if args.transcripts:
         bench_transcripts()
         return

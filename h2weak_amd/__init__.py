@@ -414,7 +414,7 @@ class Batch:
         _check(lib().h2w_batch_bench_matvec(self._h, int(nrep), precision, ctypes.byref(r)))
         n = r.n_launch
         return {"ms_per_pass": r.ms_per_matvec, "nrhs": 16, "launch_bytes": list(r.launch_bytes[:n]),
-                "launch_role": list(r.launch_role[:n])}
+                "launch_role": list(r.launch_role[:n]), "launch_ms": list(r.launch_ms[:n])}
 
 
 class IterativeSolver:

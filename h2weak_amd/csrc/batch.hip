@@ -419,7 +419,32 @@ int h2w_batch_bench_matvec(h2w_batch *B, int nrep, int precision, h2w_bench *out
     const auto &M = B->lists[precision == H2W_FP32_INNER ? 1 : 0];
     out->n_launch = (int)std::min<size_t>(M.launches.size(), H2W_MAX_LAUNCHES);
     const int es = h2w::is32(precision) ? 4 : 8;
+    // per-launch durations: the launches one by one between events (eager, same kernels as the graph)
+    std::vector<hipEvent_t> ev(out->n_launch + 1);
+    for (auto &e : ev) H2W_HIP(hipEventCreate(&e));
+    std::vector<double> acc(out->n_launch, 0.0);
+    for (int r = 0; r < nrep; ++r) {
+        H2W_HIP(hipEventRecord(ev[0], s));
+        for (int q = 0; q < out->n_launch; ++q) {
+            const auto &L = M.launches[q];
+            if (h2w::is32(precision))
+                h2w::launch_bgemm32_tab(M.d_items.p + L.item0, (int)L.nitems, M.d_terms.p, B->tab32.p, B->vec32m.p,
+                                        L.role, s);
+            else
+                h2w::launch_bgemm64_tab(M.d_items.p + L.item0, (int)L.nitems, M.d_terms.p, B->tab64.p, B->vec64m.p,
+                                        L.role, s);
+            H2W_HIP(hipEventRecord(ev[q + 1], s));
+        }
+        H2W_HIP(hipEventSynchronize(ev[out->n_launch]));
+        for (int q = 0; q < out->n_launch; ++q) {
+            float m2 = 0.f;
+            H2W_HIP(hipEventElapsedTime(&m2, ev[q], ev[q + 1]));
+            acc[q] += m2;
+        }
+    }
+    for (auto &e : ev) (void)hipEventDestroy(e);
     for (int q = 0; q < out->n_launch; ++q) {
+        out->launch_ms[q] = acc[q] / nrep;
         out->launch_role[q] = M.launches[q].role;
         out->launch_bytes[q] = M.elems[q] * es;
     }

@@ -868,8 +868,10 @@ __global__ __launch_bounds__(256) void k_bgemv(const GItem *__restrict__ items, 
 // column -- is the B operand of all of them.  An MFMA's D(row(q, v), m) then
 // belongs to item row  group * 16 VEC + VEC row(q, v) + u, where row(q, v) =
 // 4q + v (FP32) or q + 4v (FP64) are the two instructions' result layouts,
-// checked on gfx950.  The four waves take interleaved 4-column slices, two in
-// flight at a time; their partial products are summed through LDS by wave 0.
+// checked on gfx950.  An item's columns, term after term, form 4-column chunks;
+// the four waves take every fourth chunk (a scalar cursor walks the terms), four
+// chunks in flight per wave whatever the terms' widths, and their partial
+// products are summed through LDS by wave 0.
 template <typename T>
 struct Mfma16;
 template <>
@@ -889,20 +891,15 @@ struct Mfma16<double> {
     static __device__ __forceinline__ int row(int q, int v) { return q + 4 * v; }
 };
 
-template <typename T, int ROLE>
-__global__ __launch_bounds__(256) void k_bgemm(const GItem *__restrict__ items, const GTerm *__restrict__ terms,
-                                               const T *__restrict__ A0, const T *__restrict__ A1,
-                                               const T *const *__restrict__ tab, T *vec) {
+// One item with VEC rows per lane load, NG load groups (rows <= 16 VEC NG) and UNR
+// 4-column chunks in flight per wave.  red: the workgroup's LDS reduction buffer.
+template <typename T, int VEC, int NG, int UNR>
+__device__ __forceinline__ void bgemm_item(const GItem &it, const GTerm *__restrict__ terms, const T *__restrict__ A,
+                                           T *vec, T (*red)[kRowTile][kNR]) {
     typedef typename Mfma16<T>::v4 V4;
-    constexpr int VEC = 16 / sizeof(T);     // rows per lane load
-    constexpr int RG = 16 * VEC;            // rows of one load group
-    constexpr int NG = kRowTile / RG;       // load groups of an item
-    constexpr int NA = NG * VEC;            // accumulators
-    constexpr int UNR = 2;                  // 4-column slices in flight per wave
     typedef T VL __attribute__((ext_vector_type(VEC)));
-    const GItem it = items[blockIdx.x];
-    // batched systems: pool pair of system flags >> 8 from the table
-    const T *__restrict__ A = tab ? tab[2 * (it.flags >> 8) + ((it.flags >> 1) & 1)] : ((it.flags & 2) ? A1 : A0);
+    constexpr int RG = 16 * VEC;  // rows of one load group
+    constexpr int NA = NG * VEC;  // accumulators
     const int lane = threadIdx.x & 63;
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int rows = it.rows;
@@ -912,39 +909,55 @@ __global__ __launch_bounds__(256) void k_bgemm(const GItem *__restrict__ items, 
 #pragma unroll
     for (int a = 0; a < NA; ++a) acc[a] = (V4)0;
     const int64_t lda = it.lda;
-    for (int tt = it.t0; tt < it.t1; ++tt) {
-        const GTerm tm = terms[tt];
-        const T *a = A + tm.a + it.row0 + VEC * m;
-        const T *x = vec + tm.x * kNR + m;
-        const int cols = tm.cols;
-        for (int c0 = 4 * wv; c0 < cols; c0 += 16 * UNR) {
-            VL av[UNR][NG];
-            T bx[UNR];
+    // The item's columns, term after term, in 4-column chunks; wave wv takes chunks
+    // wv, wv + 4, ... through a cursor (term tt, chunk ck of that term), so UNR chunks
+    // are in flight whatever the terms' widths.
+    int tt = it.t0, ck = wv;
+    auto settle = [&]() {
+        while (tt < it.t1) {
+            const int nk = (terms[tt].cols + 3) >> 2;
+            if (ck < nk) break;
+            ck -= nk;
+            ++tt;
+        }
+    };
+    settle();
+    while (tt < it.t1) {
+        VL av[UNR][NG];
+        T bx[UNR];
+        bool ok[UNR];
 #pragma unroll
-            for (int u = 0; u < UNR; ++u) {
-                const int c = c0 + 16 * u + q;
-                const bool cin = c < cols;
-                bx[u] = cin ? x[(int64_t)c * kNR] : T(0);
-                const T *ac = a + (int64_t)c * lda;
+        for (int u = 0; u < UNR; ++u) {
+            ok[u] = tt < it.t1;
+            bx[u] = T(0);
 #pragma unroll
-                for (int g = 0; g < NG; ++g)
-                    av[u][g] = (g < ng && cin && RG * g + VEC * m < rows)
-                                   ? __builtin_nontemporal_load((const VL *)(ac + RG * g))
-                                   : (VL)0;
+            for (int g = 0; g < NG; ++g) av[u][g] = (VL)0;
+            if (ok[u]) {
+                const GTerm tm = terms[tt];
+                const int c = 4 * ck + q;
+                if (c < tm.cols) {
+                    bx[u] = vec[(tm.x + c) * kNR + m];
+                    const T *ac = A + tm.a + it.row0 + VEC * m + (int64_t)c * lda;
+#pragma unroll
+                    for (int g = 0; g < NG; ++g)
+                        if (g < ng && RG * g + VEC * m < rows) av[u][g] = __builtin_nontemporal_load((const VL *)(ac + RG * g));
+                }
+                ck += 4;
+                settle();
             }
+        }
 #pragma unroll
-            for (int u = 0; u < UNR; ++u)
+        for (int u = 0; u < UNR; ++u)
+            if (ok[u])
 #pragma unroll
                 for (int g = 0; g < NG; ++g)
                     if (g < ng)
 #pragma unroll
                         for (int e = 0; e < VEC; ++e) acc[g * VEC + e] = Mfma16<T>::op(av[u][g][e], bx[u], acc[g * VEC + e]);
-        }
     }
     // item row held by accumulator (g, e), result element v of this lane
     auto row_of = [&](int g, int e, int v) { return RG * g + VEC * Mfma16<T>::row(q, v) + e; };
     // sum the four waves' partial products: 3 and 2 into 1 and 0, then 1 into 0
-    __shared__ T red[2][kRowTile][kNR];
     auto put = [&](int slot) {
 #pragma unroll
         for (int g = 0; g < NG; ++g)
@@ -986,6 +999,18 @@ __global__ __launch_bounds__(256) void k_bgemm(const GItem *__restrict__ items, 
                         }
                     }
     }
+}
+
+template <typename T, int ROLE>
+__global__ __launch_bounds__(256) void k_bgemm(const GItem *__restrict__ items, const GTerm *__restrict__ terms,
+                                               const T *__restrict__ A0, const T *__restrict__ A1,
+                                               const T *const *__restrict__ tab, T *vec) {
+    __shared__ T red[2][kRowTile][kNR];
+    const GItem it = items[blockIdx.x];
+    // batched systems: pool pair of system flags >> 8 from the table
+    const T *__restrict__ A = tab ? tab[2 * (it.flags >> 8) + ((it.flags >> 1) & 1)] : ((it.flags & 2) ? A1 : A0);
+    constexpr int VEC = 16 / sizeof(T);
+    bgemm_item<T, VEC, kRowTile / (16 * VEC), 4>(it, terms, A, vec, red);
 }
 
 // leading sub-block copies from the FP32 pool into the inner-operator buffer
