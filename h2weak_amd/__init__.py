@@ -353,10 +353,13 @@ class Initialisation:
 
 
     def bench_matvec_multi(self, nrep=10, precision=FP64):
-        """HIP-event time of one 16-wide product (16 right-hand sides per pass)."""
+        """HIP-event time of one 16-wide product (16 right-hand sides per pass) and the operator bytes
+        it multiplies (launch_bytes[0]; symmetric blocks count twice)."""
         r = BenchResult()
         _check(lib().h2w_bench_matvec_multi(self._h, int(nrep), precision, ctypes.byref(r)))
-        return {"ms_per_pass": r.ms_per_matvec, "nrhs": 16}
+        n = r.n_launch
+        return {"ms_per_pass": r.ms_per_matvec, "nrhs": 16, "launch_bytes": list(r.launch_bytes[:n]),
+                "launch_ms": list(r.launch_ms[:n]), "launch_role": list(r.launch_role[:n])}
 
 
 class Batch:

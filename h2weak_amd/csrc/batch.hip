@@ -60,6 +60,7 @@ void merge_lists(Batch &B, int which) {
         int s;
         const Launch *L;
         const LaunchSet *ls;  // null: the handle's main lists
+        int role;
     };
     auto key_of = [](const Launch &L) {
         // up steps from the leaves, down steps ending at the leaves
@@ -69,7 +70,15 @@ void merge_lists(Batch &B, int which) {
     for (int s = 0; s < B.S; ++s) {
         Handle &h = *B.sys[s];
         const std::vector<Launch> &Ls = which == 0 ? h.launches : h.inner.launches;
-        for (auto &L : Ls) groups[key_of(L)].push_back(Src{s, &L, which == 0 ? nullptr : &h.inner});
+        const LaunchSet &tr = which == 0 ? h.trans : h.inner_trans;
+        for (auto &L : Ls) {
+            if (L.role == 3) {
+                // the gather of the single-vector path: the transposed products instead
+                for (auto &Lt : tr.launches) groups[key_of(L)].push_back(Src{s, &Lt, &tr, 3});
+                continue;
+            }
+            groups[key_of(L)].push_back(Src{s, &L, which == 0 ? nullptr : &h.inner, L.role});
+        }
     }
     LaunchSet &M = B.lists[which];
     M.launches.clear();
@@ -80,7 +89,7 @@ void merge_lists(Batch &B, int which) {
         Launch Lm;
         Lm.item0 = (int64_t)items.size();
         Lm.term0 = (int64_t)terms.size();
-        Lm.role = kv.second.front().L->role;
+        Lm.role = kv.second.front().role;
         Lm.stage = kv.first.first;
         Lm.step = kv.second.front().L->step;
         int64_t e = 0;
@@ -101,6 +110,7 @@ void merge_lists(Batch &B, int which) {
                 it.t0 = t0;
                 it.t1 = (int32_t)terms.size();
                 it.y += vb;
+                it.xb += vb;
                 it.flags = (it.flags & 3) | (src.s << 8);
                 items.push_back(it);
             }
@@ -126,16 +136,26 @@ void check_batch(Batch &B, int precision) {
     }
 }
 
-void run_batch_eager(Batch &B, int precision) {
+void run_batch_launch(Batch &B, const Launch &L, int precision) {
     hipStream_t s = B.stream;
     const LaunchSet &M = B.lists[precision == H2W_FP32_INNER ? 1 : 0];
     if (is32(precision)) {
-        for (auto &L : M.launches)
+        if (L.role == 3)
+            launch_bgemm_t32(M.d_items.p + L.item0, (int)L.nitems, M.d_terms.p, nullptr, nullptr, B.tab32.p,
+                             B.vec32m.p, s);
+        else
             launch_bgemm32_tab(M.d_items.p + L.item0, (int)L.nitems, M.d_terms.p, B.tab32.p, B.vec32m.p, L.role, s);
     } else {
-        for (auto &L : M.launches)
+        if (L.role == 3)
+            launch_bgemm_t64(M.d_items.p + L.item0, (int)L.nitems, M.d_terms.p, nullptr, nullptr, B.tab64.p,
+                             B.vec64m.p, s);
+        else
             launch_bgemm64_tab(M.d_items.p + L.item0, (int)L.nitems, M.d_terms.p, B.tab64.p, B.vec64m.p, L.role, s);
     }
+}
+
+void run_batch_eager(Batch &B, int precision) {
+    for (auto &L : B.lists[precision == H2W_FP32_INNER ? 1 : 0].launches) run_batch_launch(B, L, precision);
     H2W_HIP(hipGetLastError());
 }
 
@@ -426,13 +446,7 @@ int h2w_batch_bench_matvec(h2w_batch *B, int nrep, int precision, h2w_bench *out
     for (int r = 0; r < nrep; ++r) {
         H2W_HIP(hipEventRecord(ev[0], s));
         for (int q = 0; q < out->n_launch; ++q) {
-            const auto &L = M.launches[q];
-            if (h2w::is32(precision))
-                h2w::launch_bgemm32_tab(M.d_items.p + L.item0, (int)L.nitems, M.d_terms.p, B->tab32.p, B->vec32m.p,
-                                        L.role, s);
-            else
-                h2w::launch_bgemm64_tab(M.d_items.p + L.item0, (int)L.nitems, M.d_terms.p, B->tab64.p, B->vec64m.p,
-                                        L.role, s);
+            h2w::run_batch_launch(*B, M.launches[q], precision);
             H2W_HIP(hipEventRecord(ev[q + 1], s));
         }
         H2W_HIP(hipEventSynchronize(ev[out->n_launch]));

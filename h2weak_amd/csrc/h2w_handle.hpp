@@ -129,6 +129,8 @@ struct LaunchSet {
 struct TermSpec {
     int64_t a, x;
     int32_t cols;
+    int64_t t = -1;  // symmetric term: staging offset of its transposed product
+    int32_t ld = 0;  // transposed term: leading dimension of the stored block
 };
 
 struct M2LBlk {  // one stored M2L block K(Ir_B, Ir_F): family, level, box, partner, pool offset
@@ -204,6 +206,7 @@ struct h2w_handle {
     h2w::DevBuf<int32_t> spid;             // slot point ids
     h2w::SlotPoints sp;
     int64_t vec_len = 0, off_x = 0, off_y = 0;
+    int64_t off_one = 0, ones_len = 0;  // ones segment of the vector pools (x of the staging gather)
     std::vector<h2w::FamLevel> fam[2];
     std::vector<h2w::Pair> pairs;
     std::vector<h2w::Launch> launches;
@@ -212,6 +215,11 @@ struct h2w_handle {
     std::vector<h2w::NearBlk> nearb;
     h2w::DevBuf<float> mat32in;         // the inner operator's transfers and compact M2L blocks
     h2w::LaunchSet inner;               // the refinement's inner operator (ranks truncated)
+    h2w::LaunchSet trans, inner_trans;  // 16-wide path: K(b, p)^T x_b of the stored blocks (role 3)
+    h2w::LaunchSet otf;                 // on-the-fly path: the M2L + near-field items, every direction
+    int64_t stag_len = 0, inner_stag_len = 0;  // staging elements of the symmetric blocks' transposed products
+    h2w::DevBuf<double> stag64;
+    h2w::DevBuf<float> stag32;
     hipGraphExec_t graph[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};  // one per precision
     hipGraphExec_t graph_m[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};  // the same, kNR right-hand sides
     bool use_graph = true;

@@ -122,11 +122,13 @@ struct CopyJob {   // B(0:m, 0:n) = A(0:m, 0:n): leading sub-block copy between 
 struct GTerm {
     int64_t a;     // element offset of A (column-major, leading dimension = item.lda) at row 0
     int64_t x;     // element offset of x in the vector pool
+    int64_t t;     // symmetric term: staging offset of A^T x_item (cols entries), else -1
     int32_t cols;
-    int32_t pad;
+    int32_t ld;    // transposed term (16-wide transposed launch): leading dimension of the stored block
 };
 struct GItem {
     int64_t y;     // element offset of the output segment (first row of this tile)
+    int64_t xb;    // offset of this tile's rows in the input vector (symmetric terms' transposed part)
     int32_t rows;  // <= kRowTile
     int32_t lda;
     int32_t row0;  // first row of this tile inside the terms' matrices
@@ -169,11 +171,12 @@ void launch_xfer(const XferJob *jobs, int njobs, const int32_t *idx, const doubl
 void launch_cur(const CurJob *jobs, int njobs, const int32_t *idx, const double *lr, double *A, KParams kp,
                 hipStream_t s);
 void launch_copy32(const CopyJob *jobs, int njobs, const float *src, float *dst, hipStream_t s);
-// operator pool A0, or A1 for items whose flags have bit 1 set; unr in {2, 4, 8}
+// operator pool A0, or A1 for items whose flags have bit 1 set; unr in {2, 4, 8}.  Role 1:
+// symmetric terms (t >= 0) also write A^T x_item to stag.  Role 3: the staging gather.
 void launch_bgemv64(const GItem *items, int nitems, const GTerm *terms, const double *A0, const double *A1,
-                    double *vec, int role, int unr, hipStream_t s);
-void launch_bgemv32(const GItem *items, int nitems, const GTerm *terms, const float *A0, const float *A1, float *vec,
-                    int role, int unr, hipStream_t s);
+                    double *stag, double *vec, int role, int unr, hipStream_t s);
+void launch_bgemv32(const GItem *items, int nitems, const GTerm *terms, const float *A0, const float *A1,
+                    float *stag, float *vec, int role, int unr, hipStream_t s);
 void launch_otf32(const GItem *items, int nitems, const GTerm *terms, SlotPoints sp, float *vec, hipStream_t s);
 void launch_otf64(const GItem *items, int nitems, const GTerm *terms, SlotPoints sp, double *vec, hipStream_t s);
 void launch_to_fp32(const double *a, float *b, int64_t n, hipStream_t s);
@@ -200,6 +203,11 @@ void launch_axpby(double a, const double *x, double b, double *y, int64_t n, hip
 void launch_scal(double a, double *y, int64_t n, hipStream_t s);  // y *= a
 void launch_vec_update(const double *V, int64_t ldv, int k, const double *c, double *x, int64_t n, hipStream_t s);
 
+// kNR right-hand sides, transposed products of the stored symmetric blocks (role 3 lists)
+void launch_bgemm_t64(const GItem *items, int nitems, const GTerm *terms, const double *A0, const double *A1,
+                      const double *const *tab, double *vec, hipStream_t s);
+void launch_bgemm_t32(const GItem *items, int nitems, const GTerm *terms, const float *A0, const float *A1,
+                      const float *const *tab, float *vec, hipStream_t s);
 // kNR right-hand sides: vector pools and work vectors hold element (slot, r) at slot * kNR + r
 void launch_bgemm64(const GItem *items, int nitems, const GTerm *terms, const double *A0, const double *A1,
                     double *vec, int role, hipStream_t s);

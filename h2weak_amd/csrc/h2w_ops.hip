@@ -644,12 +644,13 @@ void add_item(WorkList &W, int64_t y, int rows, const std::vector<TermSpec> &ter
         if ((int64_t)tr * C <= kItemWork) {
             GItem it;
             it.y = y + r0;
+            it.xb = 0;
             it.rows = tr;
             it.lda = lda;
             it.row0 = row_base + r0;
             it.t0 = (int32_t)W.terms.size();
             for (auto &t : terms)
-                if (t.cols > 0) W.terms.push_back(GTerm{t.a, t.x, t.cols, 0});
+                if (t.cols > 0) W.terms.push_back(GTerm{t.a, t.x, -1, t.cols, 0});
             it.t1 = (int32_t)W.terms.size();
             it.flags = (atomic ? 1 : 0) | (pool << 1);
             W.items.push_back(it);
@@ -661,13 +662,14 @@ void add_item(WorkList &W, int64_t y, int rows, const std::vector<TermSpec> &ter
             it.lda = lda;
             it.row0 = row_base + r0;
             it.y = y + r0;
+            it.xb = 0;
             it.flags = 1 | (pool << 1);
             int64_t used = 0;
             it.t0 = (int32_t)W.terms.size();
             for (auto &t : terms) {
                 for (int64_t c0 = 0; c0 < t.cols;) {
                     int64_t take = std::min<int64_t>(t.cols - c0, maxc - used);
-                    W.terms.push_back(GTerm{t.a + c0 * lda, t.x + c0, (int32_t)take, 0});
+                    W.terms.push_back(GTerm{t.a + c0 * lda, t.x + c0, -1, (int32_t)take, 0});
                     used += take;
                     c0 += take;
                     if (used == maxc) {
@@ -686,6 +688,134 @@ void add_item(WorkList &W, int64_t y, int rows, const std::vector<TermSpec> &ter
 
 // rank at which the ACA would have stopped for tolerance tau: the first accepted
 // step k >= 1 with |u_k||v_k| <= tau * normS_k ends the cross (2D/codes/aca.hpp:363)
+// Items whose terms are never split by columns: a row tile's terms are grouped into
+// items of at most kItemWork elements (a tile split over several items accumulates
+// atomically).  A symmetric term (t >= 0) of row tile k writes the transposed product
+// of its rows to staging slot t + k pad_ld(cols).
+void add_item_whole(WorkList &W, int64_t y, int64_t xb, int rows, const std::vector<TermSpec> &terms, int lda,
+                    int pool) {
+    if (rows <= 0) return;
+    std::vector<const TermSpec *> ts;
+    for (auto &t : terms)
+        if (t.cols > 0) ts.push_back(&t);
+    if (ts.empty()) return;
+    for (int r0 = 0, k = 0; r0 < rows; r0 += kRowTile, ++k) {
+        const int tr = std::min(kRowTile, rows - r0);
+        std::vector<std::pair<size_t, size_t>> groups;  // [first, last) of ts per item
+        size_t g0 = 0;
+        int64_t work = 0;
+        for (size_t q = 0; q < ts.size(); ++q) {
+            const int64_t w = (int64_t)tr * ts[q]->cols;
+            if (q > g0 && work + w > kItemWork) {
+                groups.push_back({g0, q});
+                g0 = q;
+                work = 0;
+            }
+            work += w;
+        }
+        groups.push_back({g0, ts.size()});
+        for (auto &g : groups) {
+            GItem it;
+            it.y = y + r0;
+            it.xb = xb + r0;
+            it.rows = tr;
+            it.lda = lda;
+            it.row0 = r0;
+            it.t0 = (int32_t)W.terms.size();
+            for (size_t q = g.first; q < g.second; ++q) {
+                const TermSpec &t = *ts[q];
+                W.terms.push_back(GTerm{t.a, t.x, t.t >= 0 ? t.t + (int64_t)k * pad_ld(t.cols) : -1, t.cols, t.ld});
+            }
+            it.t1 = (int32_t)W.terms.size();
+            it.flags = (groups.size() > 1 ? 1 : 0) | (pool << 1);
+            W.items.push_back(it);
+        }
+    }
+}
+
+// One group of boxes whose pairwise blocks are stored once per unordered pair (the
+// M2L blocks of one family and level, or the near field): box b has output offset
+// y[b], input offset x[b] and size r[b]; up[b] lists the stored blocks K(b, p), p > b
+// (r[b] x r[p], leading dimension pad_ld(r[b])), self[b] the block K(b, b) or -1.
+struct SymGroup {
+    std::vector<int64_t> y, x;
+    std::vector<int> r;
+    std::vector<std::vector<std::pair<int, int64_t>>> up;
+    std::vector<int64_t> self;
+    int pool = 0;
+};
+// The four lists of such a group: fused items (box b: y_b += K(b, p) x_p over its stored
+// blocks, each block's K(b, p)^T x_b into a staging slot of p), the gather (y_p += the
+// staging slots of p, a GEMV with the ones segment), the 16-wide path's transposed items
+// (box p: y_p += K(b, p)^T x_b, b < p) and the on-the-fly path's directed items.
+struct SymLists {
+    WorkList fused, gather, trans, otf;
+    int64_t stag = 0;  // staging elements
+};
+void sym_lists(const SymGroup &G, SymLists &L, int64_t off_one) {
+    const int nb = (int)G.r.size();
+    std::vector<std::vector<std::pair<int, int64_t>>> down(nb);
+    std::vector<std::vector<int64_t>> slot(nb);  // slot offset of each up[b] entry
+    for (int b = 0; b < nb; ++b) slot[b].assign(G.up[b].size(), -1);
+    for (int b = 0; b < nb; ++b)
+        for (auto &pe : G.up[b]) down[pe.first].push_back({b, pe.second});
+    std::vector<int64_t> base(nb, -1);
+    std::vector<int> nslot(nb, 0);
+    {
+        std::vector<size_t> cur(nb, 0);  // next up[b] entry to assign (partners come in increasing p)
+        for (int p = 0; p < nb; ++p) {
+            int c = 0;
+            for (auto &be : down[p]) c += (G.r[be.first] + kRowTile - 1) / kRowTile;
+            nslot[p] = G.r[p] > 0 ? c : 0;
+            if (nslot[p] > 0) {
+                base[p] = L.stag;
+                L.stag += align_up((int64_t)c * pad_ld(G.r[p]), 32);
+            }
+            int col = 0;
+            for (auto &be : down[p]) {
+                const int b = be.first;
+                size_t k = 0;
+                while (k < G.up[b].size() && G.up[b][k].first != p) ++k;
+                if (base[p] >= 0) slot[b][k] = base[p] + (int64_t)col * pad_ld(G.r[p]);
+                col += (G.r[b] + kRowTile - 1) / kRowTile;
+            }
+            (void)cur;
+        }
+    }
+    for (int b = 0; b < nb; ++b) {
+        if (G.r[b] <= 0) continue;
+        std::vector<TermSpec> t;
+        if (G.self[b] >= 0) t.push_back(TermSpec{G.self[b], G.x[b], G.r[b]});
+        for (size_t k = 0; k < G.up[b].size(); ++k) {
+            const int p = G.up[b][k].first;
+            TermSpec ts{G.up[b][k].second, G.x[p], G.r[p]};
+            ts.t = slot[b][k];
+            t.push_back(ts);
+        }
+        add_item_whole(L.fused, G.y[b], G.x[b], G.r[b], t, (int)pad_ld(G.r[b]), G.pool);
+    }
+    for (int p = 0; p < nb; ++p)
+        if (nslot[p] > 0)
+            add_item_whole(L.gather, G.y[p], 0, G.r[p], {TermSpec{base[p], off_one, nslot[p]}}, (int)pad_ld(G.r[p]),
+                           0);
+    for (int p = 0; p < nb; ++p) {
+        std::vector<TermSpec> t;
+        for (auto &be : down[p]) {
+            TermSpec ts{be.second, G.x[be.first], G.r[be.first]};
+            ts.ld = (int32_t)pad_ld(G.r[be.first]);
+            t.push_back(ts);
+        }
+        add_item_whole(L.trans, G.y[p], 0, G.r[p], t, (int)pad_ld(G.r[p]), G.pool);
+    }
+    for (int b = 0; b < nb; ++b) {
+        std::vector<TermSpec> t;
+        if (G.self[b] >= 0) t.push_back(TermSpec{0, G.x[b], G.r[b]});
+        for (auto &pe : G.up[b]) t.push_back(TermSpec{0, G.x[pe.first], G.r[pe.first]});
+        for (auto &be : down[b]) t.push_back(TermSpec{0, G.x[be.first], G.r[be.first]});
+        add_item(L.otf, G.y[b], G.r[b], t, false, (int)pad_ld(G.r[b]));
+    }
+}
+
 int inner_rank(const std::vector<double> &hist, int rank, double tau) {
     for (int k = 1; k < rank && 2 * k + 1 < (int)hist.size(); ++k)
         if (hist[2 * k] <= tau * hist[2 * k + 1]) return k + 1;
@@ -926,36 +1056,63 @@ void build_inner(Handle &h, double irp) {
             IL.push_back(std::move(W));
         }
         {
-            WorkList W;
-            W.role = 1;
-            W.stage = 1;
-            std::vector<std::vector<std::vector<std::vector<TermSpec>>>> per(2);
-            for (int f = 0; f < 2; ++f) {
-                if (!fam_on[f]) continue;
-                per[f].assign(P.nL + 1, {});
-                for (int j = nfam_lo[f]; j <= P.nL; ++j) per[f][j].assign(h.fam[f][j].nb, {});
-            }
-            for (auto &pr : m2l_in) {
-                const M2LBlk &m = m2l[pr.first];
-                const FamLevel &fl = h.fam[m.f][m.j];
-                per[m.f][m.j][m.b].push_back(TermSpec{base + pr.second, fl.o_off[m.p], (int32_t)fl.kin[m.p]});
-            }
+            // the inner M2L blocks (compact leading sub-blocks, one per unordered pair, second
+            // pool) and the stored near field, fused, then the gather of their transposed products
+            SymLists SL;
             for (int f = 0; f < 2; ++f) {
                 if (!fam_on[f]) continue;
                 for (int j = nfam_lo[f]; j <= P.nL; ++j) {
                     FamLevel &fl = h.fam[f][j];
-                    for (int b = 0; b < fl.nb; ++b)
-                        if (fl.kin[b] > 0)
-                            add_item(W, fl.u_off[b], fl.kin[b], per[f][j][b], false, (int)pad_ld(fl.kin[b]), 0, 1);
+                    SymGroup G;
+                    G.y.assign(fl.u_off.begin(), fl.u_off.end());
+                    G.x.assign(fl.o_off.begin(), fl.o_off.end());
+                    G.r = fl.kin;
+                    G.up.assign(fl.nb, {});
+                    G.self.assign(fl.nb, -1);
+                    G.pool = 1;
+                    for (auto &pr : m2l_in) {
+                        const M2LBlk &m = m2l[pr.first];
+                        if (m.f == f && m.j == j) G.up[m.b].push_back({m.p, base + pr.second});
+                    }
+                    sym_lists(G, SL, h.off_one);
                 }
             }
-            std::vector<std::vector<TermSpec>> pn(nleaf);
-            for (auto &nb : nearb)
-                pn[nb.b].push_back(TermSpec{nb.off, h.off_x + P.bstart[P.nL][nb.n], (int32_t)P.bcount[P.nL][nb.n]});
-            for (int b = 0; b < nleaf; ++b)
-                add_item(W, h.off_y + P.bstart[P.nL][b], (int)P.bcount[P.nL][b], pn[b], false,
-                         (int)pad_ld(P.bcount[P.nL][b]));
-            IL.push_back(std::move(W));
+            {
+                SymGroup G;
+                G.y.resize(nleaf);
+                G.x.resize(nleaf);
+                G.r.resize(nleaf);
+                G.up.assign(nleaf, {});
+                G.self.assign(nleaf, -1);
+                for (int b = 0; b < nleaf; ++b) {
+                    G.y[b] = h.off_y + P.bstart[P.nL][b];
+                    G.x[b] = h.off_x + P.bstart[P.nL][b];
+                    G.r[b] = (int)P.bcount[P.nL][b];
+                }
+                for (auto &nb : nearb) {
+                    if (nb.n == nb.b) G.self[nb.b] = nb.off;
+                    else G.up[nb.b].push_back({nb.n, nb.off});
+                }
+                sym_lists(G, SL, h.off_one);
+            }
+            SL.fused.role = 1;
+            SL.fused.stage = 1;
+            SL.gather.role = 3;
+            SL.gather.stage = 1;
+            SL.gather.step = 1;
+            IL.push_back(std::move(SL.fused));
+            IL.push_back(std::move(SL.gather));
+            SL.trans.role = 3;
+            SL.trans.stage = 1;
+            SL.trans.step = 1;
+            std::vector<WorkList> tl;
+            tl.push_back(std::move(SL.trans));
+            upload_lists(tl, h.inner_trans, s);
+            h.inner_stag_len = SL.stag;
+            const float *was = h.stag32.p;
+            h.stag32.ensure((size_t)std::max<int64_t>(32, std::max(h.stag_len, h.inner_stag_len)));
+            if (h.stag32.p != was) drop_graph(h, H2W_FP32);  // its staging pointer moved
+            H2W_HIP(hipMemsetAsync(h.stag32.p, 0, h.stag32.n * 4, s));
         }
         for (int j = 1; j <= P.nL - 1; ++j) {
             WorkList W;
@@ -1012,8 +1169,9 @@ void build_inner(Handle &h, double irp) {
             IL.push_back(std::move(W));
         }
         upload_lists(IL, h.inner, s);
-        h.inner_elems = 0;
-        for (auto e : h.inner.elems) h.inner_elems += e;
+        h.inner_elems = 0;  // operator elements (the gather reads staging slots)
+        for (size_t q = 0; q < h.inner.launches.size(); ++q)
+            if (h.inner.launches[q].role != 3) h.inner_elems += h.inner.elems[q];
     } else {
         h.inner.launches.clear();
         h.inner_elems = 0;
@@ -1075,8 +1233,33 @@ void assemble_operators(Handle &h, double t0) {
     const int nfam_lo[2] = {2, 1};  // first level of each family
     const bool fam_on[2] = {P.dim >= 2 && h.variant != H2W_NON_NESTED, h.variant == H2W_NESTED};
 
-    // ---- vector pool
+    // ---- vector pool: a segment of ones (the staging gather's x), then x, y, skeleton vectors
     int64_t vo = 0;
+    {
+        // the gather of box p sums one staging slot per row tile of every lower partner
+        int64_t mx = 1;
+        for (int f = 0; f < 2; ++f) {
+            if (!fam_on[f]) continue;
+            for (int j = nfam_lo[f]; j <= P.nL; ++j) {
+                const FamLevel &fl = h.fam[f][j];
+                for (int p = 0; p < fl.nb; ++p) {
+                    int64_t c = 0;
+                    for (int b : (f == 0 ? P.far_of(j, p) : P.vert_of(j, p)))
+                        if (b < p) c += (fl.rank[b] + kRowTile - 1) / kRowTile;
+                    mx = std::max(mx, c);
+                }
+            }
+        }
+        for (int p = 0; p < P.nboxes[P.nL]; ++p) {
+            int64_t c = 0;
+            for (int b : P.near[p])
+                if (b < p) c += (P.bcount[P.nL][b] + kRowTile - 1) / kRowTile;
+            mx = std::max(mx, c);
+        }
+        h.off_one = vo;
+        h.ones_len = mx;
+        vo += align_up(mx, 32);
+    }
     h.off_x = vo;
     vo += align_up(P.N, 32);
     h.off_y = vo;
@@ -1139,8 +1322,10 @@ void assemble_operators(Handle &h, double t0) {
                 auto partners = f == 0 ? P.far_of(j, b) : P.vert_of(j, b);
                 for (int p : partners) {
                     int64_t e = (int64_t)fl.rank[b] * fl.rank[p];
-                    memory += (double)e;
-                    if (e == 0) continue;
+                    memory += (double)e;  // the reference counts both directions
+                    // K(Ir_p, Ir_b) = K(Ir_b, Ir_p)^T entry for entry (the kernel depends on |x - y|
+                    // only): one block per unordered pair, the lower-numbered box's
+                    if (e == 0 || p < b) continue;
                     m2l.push_back(M2LBlk{f, j, b, p, take(pad_ld(fl.rank[b]) * fl.rank[p])});
                 }
             }
@@ -1156,7 +1341,7 @@ void assemble_operators(Handle &h, double t0) {
         for (int n : lst) {
             int64_t e = P.bcount[P.nL][b] * P.bcount[P.nL][n];
             if (count_near) memory += (double)e;
-            if (e == 0) continue;
+            if (e == 0 || n < b) continue;  // one block per unordered pair (self blocks kept)
             nearb.push_back(NearBlk{b, n, take(pad_ld(P.bcount[P.nL][b]) * P.bcount[P.nL][n])});
         }
     }
@@ -1174,7 +1359,9 @@ void assemble_operators(Handle &h, double t0) {
     h.mat64.alloc(h.mat_len);
     H2W_HIP(hipMemsetAsync(h.mat64.p, 0, h.mat_len * 8, s));
     h.vec64.alloc(h.vec_len);
+    launch_fill(h.vec64.p + h.off_one, h.ones_len, 1.0, s);
     h.vec32.alloc(h.vec_len);
+    launch_fill32(h.vec32.p + h.off_one, h.ones_len, 1.0f, s);
     {
         // the point of every vector slot, for on-the-fly near-field / M2L evaluation
         std::vector<int32_t> pid(h.vec_len, -1);
@@ -1379,30 +1566,61 @@ void assemble_operators(Handle &h, double t0) {
         }
         WL.push_back(std::move(W));
     }
-    // M2L (every level, both families) + near field
+    // M2L (every level, both families) + near field, one block per unordered pair: the fused
+    // launch, then the gather of the transposed products
     {
-        WorkList W;
-        W.role = 1;
-        W.stage = 1;
+        SymLists SL;
         for (int f = 0; f < 2; ++f) {
             if (!fam_on[f]) continue;
             for (int j = nfam_lo[f]; j <= P.nL; ++j) {
                 FamLevel &fl = h.fam[f][j];
-                std::vector<std::vector<TermSpec>> per(fl.nb);
+                SymGroup G;
+                G.y.assign(fl.u_off.begin(), fl.u_off.end());
+                G.x.assign(fl.o_off.begin(), fl.o_off.end());
+                G.r = fl.rank;
+                G.up.assign(fl.nb, {});
+                G.self.assign(fl.nb, -1);
                 for (auto &m : m2l)
-                    if (m.f == f && m.j == j)
-                        per[m.b].push_back(TermSpec{m.off, fl.o_off[m.p], (int32_t)fl.rank[m.p]});
-                for (int b = 0; b < fl.nb; ++b)
-                    add_item(W, fl.u_off[b], fl.rank[b], per[b], false, (int)pad_ld(fl.rank[b]));
+                    if (m.f == f && m.j == j) G.up[m.b].push_back({m.p, m.off});
+                sym_lists(G, SL, h.off_one);
             }
         }
-        std::vector<std::vector<TermSpec>> per(nleaf);
-        for (auto &nb : nearb)
-            per[nb.b].push_back(TermSpec{nb.off, h.off_x + P.bstart[P.nL][nb.n], (int32_t)P.bcount[P.nL][nb.n]});
-        for (int b = 0; b < nleaf; ++b)
-            add_item(W, h.off_y + P.bstart[P.nL][b], (int)P.bcount[P.nL][b], per[b], false,
-                     (int)pad_ld(P.bcount[P.nL][b]));
-        WL.push_back(std::move(W));
+        {
+            SymGroup G;
+            G.y.resize(nleaf);
+            G.x.resize(nleaf);
+            G.r.resize(nleaf);
+            G.up.assign(nleaf, {});
+            G.self.assign(nleaf, -1);
+            for (int b = 0; b < nleaf; ++b) {
+                G.y[b] = h.off_y + P.bstart[P.nL][b];
+                G.x[b] = h.off_x + P.bstart[P.nL][b];
+                G.r[b] = (int)P.bcount[P.nL][b];
+            }
+            for (auto &nb : nearb) {
+                if (nb.n == nb.b) G.self[nb.b] = nb.off;
+                else G.up[nb.b].push_back({nb.n, nb.off});
+            }
+            sym_lists(G, SL, h.off_one);
+        }
+        SL.fused.role = 1;
+        SL.fused.stage = 1;
+        SL.gather.role = 3;
+        SL.gather.stage = 1;
+        SL.gather.step = 1;
+        WL.push_back(std::move(SL.fused));
+        WL.push_back(std::move(SL.gather));
+        h.stag_len = SL.stag;
+        SL.trans.role = 3;
+        SL.trans.stage = 1;
+        SL.trans.step = 1;
+        SL.otf.role = 1;
+        SL.otf.stage = 1;
+        std::vector<WorkList> tl, ol;
+        tl.push_back(std::move(SL.trans));
+        ol.push_back(std::move(SL.otf));
+        upload_lists(tl, h.trans, s);
+        upload_lists(ol, h.otf, s);
     }
     // down, levels 1 .. nL-1
     for (int j = 1; j <= P.nL - 1; ++j) {
@@ -1477,13 +1695,19 @@ void assemble_operators(Handle &h, double t0) {
     h.d_terms.upload(terms, s);
     h.h_items = items;
     h.h_terms = terms;
+    h.stag64.alloc((size_t)std::max<int64_t>(h.stag_len, 32));
+    H2W_HIP(hipMemsetAsync(h.stag64.p, 0, h.stag64.n * 8, s));
+    if (h.opts.fp32_shadow) {
+        h.stag32.ensure((size_t)std::max<int64_t>(std::max(h.stag_len, h.inner_stag_len), 32));
+        H2W_HIP(hipMemsetAsync(h.stag32.p, 0, h.stag32.n * 4, s));
+    }
     int64_t se = 0;
     h.launch_elems.assign(h.launches.size(), 0);
     for (size_t q = 0; q < h.launches.size(); ++q) {
         const Launch &Lh = h.launches[q];
         for (int64_t i = Lh.item0; i < Lh.item0 + Lh.nitems; ++i)
             for (int t = items[i].t0; t < items[i].t1; ++t) h.launch_elems[q] += (int64_t)items[i].rows * terms[t].cols;
-        se += h.launch_elems[q];
+        if (Lh.role != 3) se += h.launch_elems[q];  // operator elements (the gather reads staging slots)
     }
     h.stored_elems = se;
     build_inner(h, (double)h.opts.ir_tol_pow);
@@ -1508,24 +1732,33 @@ void rebuild_operators(Handle &h) {
 
 bool is32(int precision) { return precision == H2W_FP32 || precision == H2W_FP32_OTF || precision == H2W_FP32_INNER; }
 
+// Stage 1 is one fused launch (role 1: every stored block K(b, p), p >= b, applied to
+// x_p, the transposed products K(b, p)^T x_b written to staging slots) and one gather
+// (role 3: y_p += the staging slots of p, a GEMV over the staging buffer with the ones
+// segment).  The on-the-fly precisions evaluate the blocks from the points instead,
+// over the directed lists, and skip the gather.
 void run_launch(Handle &h, const Launch &Lh, int precision) {
     hipStream_t s = h.stream;
-    const bool otf = (precision == H2W_FP32_OTF || precision == H2W_FP64_OTF) && Lh.role == 1;
-    if (precision == H2W_FP32_INNER) {
-        launch_bgemv32(h.inner.d_items.p + Lh.item0, (int)Lh.nitems, h.inner.d_terms.p, h.mat32.p, h.mat32in.p,
-                       h.vec32.p, Lh.role, Lh.unr, s);
+    const bool otf = (precision == H2W_FP32_OTF || precision == H2W_FP64_OTF) && (Lh.role == 1 || Lh.role == 3);
+    if (otf) {
+        if (Lh.role == 3) return;
+        for (auto &Lo : h.otf.launches) {
+            if (is32(precision))
+                launch_otf32(h.otf.d_items.p + Lo.item0, (int)Lo.nitems, h.otf.d_terms.p, h.sp, h.vec32.p, s);
+            else
+                launch_otf64(h.otf.d_items.p + Lo.item0, (int)Lo.nitems, h.otf.d_terms.p, h.sp, h.vec64.p, s);
+        }
         return;
     }
-    if (is32(precision)) {
-        if (otf) launch_otf32(h.d_items.p + Lh.item0, (int)Lh.nitems, h.d_terms.p, h.sp, h.vec32.p, s);
-        else
-            launch_bgemv32(h.d_items.p + Lh.item0, (int)Lh.nitems, h.d_terms.p, h.mat32.p, nullptr, h.vec32.p, Lh.role,
-                           Lh.unr32, s);
+    if (precision == H2W_FP32_INNER) {
+        launch_bgemv32(h.inner.d_items.p + Lh.item0, (int)Lh.nitems, h.inner.d_terms.p,
+                       Lh.role == 3 ? h.stag32.p : h.mat32.p, h.mat32in.p, h.stag32.p, h.vec32.p, Lh.role, Lh.unr, s);
+    } else if (is32(precision)) {
+        launch_bgemv32(h.d_items.p + Lh.item0, (int)Lh.nitems, h.d_terms.p, Lh.role == 3 ? h.stag32.p : h.mat32.p,
+                       nullptr, h.stag32.p, h.vec32.p, Lh.role, Lh.unr32, s);
     } else {
-        if (otf) launch_otf64(h.d_items.p + Lh.item0, (int)Lh.nitems, h.d_terms.p, h.sp, h.vec64.p, s);
-        else
-            launch_bgemv64(h.d_items.p + Lh.item0, (int)Lh.nitems, h.d_terms.p, h.mat64.p, nullptr, h.vec64.p, Lh.role,
-                           Lh.unr, s);
+        launch_bgemv64(h.d_items.p + Lh.item0, (int)Lh.nitems, h.d_terms.p, Lh.role == 3 ? h.stag64.p : h.mat64.p,
+                       nullptr, h.stag64.p, h.vec64.p, Lh.role, Lh.unr, s);
     }
 }
 

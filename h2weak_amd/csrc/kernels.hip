@@ -794,9 +794,51 @@ __global__ void k_cur(const CurJob *__restrict__ jobs, const int32_t *__restrict
 // is the number of column loads each lane keeps in flight.  Bit 1 of an item's
 // flags selects the operator pool A1 (the inner operator of the refinement)
 // instead of A0.
+// Streaming load through a global-address-space pointer: operator pointers taken from a
+// table in memory are generic to the compiler, whose flat loads would also count against
+// the LDS wait counter.
+template <typename V>
+__device__ __forceinline__ V ld_nt(const V *p) {
+    return __builtin_nontemporal_load((const __attribute__((address_space(1))) V *)p);
+}
+
+// Value of lane (lane ^ O) for a power of two O < 64: quad permutes (DPP) for 1 and 2,
+// ds_swizzle in its bit-mask mode for 4 .. 16, ds_bpermute for 32.  Every lane of the
+// aligned group of 2 O lanes must be active.
+template <int O>
+__device__ __forceinline__ int xor_lane(int v) {
+    if constexpr (O == 1) return __builtin_amdgcn_mov_dpp(v, 0xB1, 0xF, 0xF, false);
+    else if constexpr (O == 2) return __builtin_amdgcn_mov_dpp(v, 0x4E, 0xF, 0xF, false);
+    else if constexpr (O < 32) return __builtin_amdgcn_ds_swizzle(v, 0x1F | (O << 10));
+    else return __builtin_amdgcn_ds_bpermute((int)(((threadIdx.x & 63) ^ O) << 2), v);
+}
+template <int O>
+__device__ __forceinline__ float xor_lane(float v) {
+    return __int_as_float(xor_lane<O>(__float_as_int(v)));
+}
+template <int O>
+__device__ __forceinline__ double xor_lane(double v) {
+    const int64_t b = __double_as_longlong(v);
+    const int lo = xor_lane<O>((int)(uint32_t)b), hi = xor_lane<O>((int)(b >> 32));
+    return __longlong_as_double((int64_t)(((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo));
+}
+// the same with O known only after unrolling
+template <typename T>
+__device__ __forceinline__ T xor_lane_n(T v, int o) {
+    switch (o) {
+    case 1: return xor_lane<1>(v);
+    case 2: return xor_lane<2>(v);
+    case 4: return xor_lane<4>(v);
+    case 8: return xor_lane<8>(v);
+    case 16: return xor_lane<16>(v);
+    default: return xor_lane<32>(v);
+    }
+}
+
 template <typename T, int VEC, int ROLE, int UNR>
 __global__ __launch_bounds__(256) void k_bgemv(const GItem *__restrict__ items, const GTerm *__restrict__ terms,
-                                               const T *__restrict__ A0, const T *__restrict__ A1, T *vec) {
+                                               const T *__restrict__ A0, const T *__restrict__ A1, T *vec,
+                                               T *__restrict__ stag) {
     typedef T V __attribute__((ext_vector_type(VEC)));
     const GItem it = items[blockIdx.x];
     const T *__restrict__ A = (it.flags & 2) ? A1 : A0;
@@ -810,6 +852,10 @@ __global__ __launch_bounds__(256) void k_bgemv(const GItem *__restrict__ items, 
     const bool act = r0 < rows;
     __shared__ T part[4][kRowTile];
     V acc = (V)0;
+    // this tile's own input rows: the x of the transposed products of symmetric terms
+    T xb[VEC];
+#pragma unroll
+    for (int v = 0; v < VEC; ++v) xb[v] = (ROLE == 1 && act && r0 + v < rows) ? vec[it.xb + r0 + v] : T(0);
     const int64_t lda = it.lda;
     const int step = 4 * cpw;
     for (int tt = it.t0; tt < it.t1; ++tt) {
@@ -817,6 +863,7 @@ __global__ __launch_bounds__(256) void k_bgemv(const GItem *__restrict__ items, 
         const T *a = A + tm.a + it.row0 + r0;
         const T *x = vec + tm.x;
         const int cols = tm.cols;
+        const bool sym = ROLE == 1 && tm.t >= 0;
         for (int c = wv * cpw + q; c < cols + 0; c += step * UNR) {
             V av[UNR];
             T xv[UNR];
@@ -824,7 +871,7 @@ __global__ __launch_bounds__(256) void k_bgemv(const GItem *__restrict__ items, 
             for (int u = 0; u < UNR; ++u) {
                 const int cc = c + u * step;
                 if (act && cc < cols) {
-                    av[u] = __builtin_nontemporal_load((const V *)(a + (int64_t)cc * lda));
+                    av[u] = ld_nt((const V *)(a + (int64_t)cc * lda));
                     xv[u] = x[cc];
                 } else {
                     av[u] = (V)0;
@@ -835,6 +882,49 @@ __global__ __launch_bounds__(256) void k_bgemv(const GItem *__restrict__ items, 
             for (int u = 0; u < UNR; ++u)
 #pragma unroll
                 for (int v = 0; v < VEC; ++v) acc[v] = fma(av[u][v], xv[u], acc[v]);
+            if (sym) {
+                // columns c + u step of the block dotted with this tile's rows of x, summed
+                // over the lpc lanes that hold them.  A butterfly first halves the UNR
+                // values per exchange (lane bit o keeps one half, sends the other), so that
+                // after log2 UNR exchanges lane l holds column ui(l) summed over its UNR-lane
+                // group; plain exchanges then finish the sums, and the first UNR lanes of
+                // every column group write them.
+                T tp[UNR];
+#pragma unroll
+                for (int u = 0; u < UNR; ++u) {
+                    tp[u] = T(0);
+#pragma unroll
+                    for (int v = 0; v < VEC; ++v) tp[u] = fma(av[u][v], xb[v], tp[u]);
+                }
+                if (lpc >= UNR) {
+                    int ui = 0;
+#pragma unroll
+                    for (int k = UNR, o = 1; k > 1; k >>= 1, o <<= 1) {
+                        const bool hi = lane & o;
+#pragma unroll
+                        for (int i = 0; i < k / 2; ++i) {
+                            const T snd = hi ? tp[i] : tp[i + k / 2];
+                            const T keep = hi ? tp[i + k / 2] : tp[i];
+                            tp[i] = keep + xor_lane_n(snd, o);
+                        }
+                        ui += hi ? k / 2 : 0;
+                    }
+#pragma unroll
+                    for (int o = UNR; o < 64; o <<= 1)
+                        if (o < lpc) tp[0] += xor_lane_n(tp[0], o);
+                    const int cc = c + ui * step;
+                    if ((lane % lpc) < UNR && cc < cols) stag[tm.t + cc] = tp[0];
+                } else {
+#pragma unroll
+                    for (int u = 0; u < UNR; ++u) {
+#pragma unroll
+                        for (int o = 1; o < UNR; o <<= 1)
+                            if (o < lpc) tp[u] += xor_lane_n(tp[u], o);
+                        const int cc = c + u * step;
+                        if ((lane % lpc) == 0 && cc < cols) stag[tm.t + cc] = tp[u];
+                    }
+                }
+            }
         }
     }
 #pragma unroll
@@ -940,7 +1030,7 @@ __device__ __forceinline__ void bgemm_item(const GItem &it, const GTerm *__restr
                     const T *ac = A + tm.a + it.row0 + VEC * m + (int64_t)c * lda;
 #pragma unroll
                     for (int g = 0; g < NG; ++g)
-                        if (g < ng && RG * g + VEC * m < rows) av[u][g] = __builtin_nontemporal_load((const VL *)(ac + RG * g));
+                        if (g < ng && RG * g + VEC * m < rows) av[u][g] = ld_nt((const VL *)(ac + RG * g));
                 }
                 ck += 4;
                 settle();
@@ -1011,6 +1101,151 @@ __global__ __launch_bounds__(256) void k_bgemm(const GItem *__restrict__ items, 
     const T *__restrict__ A = tab ? tab[2 * (it.flags >> 8) + ((it.flags >> 1) & 1)] : ((it.flags & 2) ? A1 : A0);
     constexpr int VEC = 16 / sizeof(T);
     bgemm_item<T, VEC, kRowTile / (16 * VEC), 4>(it, terms, A, vec, red);
+}
+
+// Transposed products of the symmetric blocks on the 16-wide path:
+// Y[j, n] += sum_i K[i, row0 + j] X[i, n] for an item's output rows j (box p) and
+// terms K(b, p) (leading dimension tm.ld, inner dimension tm.cols = r_b), through LDS.
+// The MFMA's A operand wants an output row (a column of K) per lane % 16 and the inner
+// index per lane / 16, while K's contiguous direction is the inner one: loading that
+// layout directly reads 64-byte runs per column.  Instead a slab is 16 VEC inner
+// indices (256 bytes of every column); wave w owns output groups w and w + 4 (16 rows each) of the item and reads
+// its 16-column groups slab by slab with 16 lanes per column (whole 256-byte runs),
+// writes them to its own LDS region and reads them back in the MFMA's A layout (lane
+// (m, q): VEC inner indices of column 16 g + m).  The next slab's loads are in flight
+// while a slab is multiplied: the slab's B operands are loaded before them, since
+// vector loads retire in order.  No wave reads another wave's region, and every wave
+// writes only its own output rows.
+template <typename T>
+__global__ __launch_bounds__(256) void k_bgemm_tl(const GItem *__restrict__ items, const GTerm *__restrict__ terms,
+                                                  const T *__restrict__ A0, const T *__restrict__ A1,
+                                                  const T *const *__restrict__ tab, T *vec) {
+    typedef typename Mfma16<T>::v4 V4;
+    constexpr int VEC = 16 / sizeof(T);
+    constexpr int SL = 16 * VEC;       // inner indices of one slab
+    constexpr int LD = SL + VEC;       // LDS row: one column's slab, padded by 16 bytes
+    typedef T VL __attribute__((ext_vector_type(VEC)));
+    __shared__ __attribute__((aligned(16))) T lds[4][32][LD];
+    const GItem it = items[blockIdx.x];
+    const T *__restrict__ A = tab ? tab[2 * (it.flags >> 8) + ((it.flags >> 1) & 1)] : ((it.flags & 2) ? A1 : A0);
+    const int lane = threadIdx.x & 63;
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int rows = it.rows;
+    const int ng = (rows + 15) >> 4;
+    const int m = lane & 15, q = lane >> 4;
+    // Items of one or two groups split their slabs over the waves instead: wave w takes
+    // group w % ng and every ns-th slab from the (w / ng)-th, and the partial products are
+    // summed through LDS at the end.
+    const int ns = ng <= 2 ? 4 / ng : 1;
+    const int g0 = ns > 1 ? wv % ng : wv;  // first group of this wave (the second is wv + 4)
+    const int nown = ns > 1 ? 1 : (wv < ng) + (wv + 4 < ng);
+    if (nown == 0) return;  // (ns == 1 only: no barrier follows)
+    T(*reg)[LD] = lds[wv];
+    V4 acc[2] = {(V4)0, (V4)0};
+    int lt = it.t0, lk = ns > 1 ? wv / ng : 0;  // next slab to load (term, slab of the term)
+    auto settle = [&]() {
+        while (lt < it.t1) {
+            const int nk = (terms[lt].cols + SL - 1) / SL;
+            if (lk < nk) break;
+            lk -= nk;
+            ++lt;
+        }
+    };
+    int64_t sx = 0;          // loaded slab: input offset, inner size, first inner index
+    int scols = 0, si0 = 0;
+    int okm = 0, vlim = 0;  // loaded slab: lanes' valid loads, valid inner indices from this lane's first
+    VL L[2][4];
+    // load the next slab into L (16 lanes per column, this lane's VEC inner indices)
+    // (past the last slab it still loads, from the last term, and the loads are dropped)
+    auto next = [&]() -> bool {
+        settle();
+        const bool has = lt < it.t1;
+        const GTerm tm = terms[has ? lt : it.t1 - 1];
+        sx = tm.x;
+        scols = tm.cols;
+        si0 = has ? lk * SL : tm.cols;
+        const int io = si0 + VEC * m;
+        // unconditional loads (lanes out of range read the term's first element, dropped when
+        // staged): loads under a branch, or values used at once, would make the next wait
+        // drain all of them
+        okm = 0;
+        vlim = tm.cols - io;
+#pragma unroll
+        for (int gi = 0; gi < 2; ++gi)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int c = 16 * (gi ? wv + 4 : g0) + q + 4 * j;  // item row = column of the block
+                const bool ok = gi < nown && c < rows && io < tm.cols;
+                okm |= ok ? 1 << (4 * gi + j) : 0;
+                L[gi][j] = ld_nt((const VL *)(A + tm.a + (ok ? (int64_t)(it.row0 + c) * tm.ld + io : 0)));
+            }
+        lk += has ? ns : 0;
+        return has;
+    };
+    auto wave_sync = [] {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    };
+    bool more = next();
+    while (more) {
+        // staged with every entry outside the block set to zero, so that the B operands of
+        // inner indices beyond the block (clamped loads of a valid x) multiply zeros
+#pragma unroll
+        for (int gi = 0; gi < 2; ++gi)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                VL v = L[gi][j];
+#pragma unroll
+                for (int e = 0; e < VEC; ++e) v[e] = ((okm >> (4 * gi + j)) & 1) && e < vlim ? v[e] : T(0);
+                *(VL *)&reg[16 * gi + q + 4 * j][VEC * m] = v;
+            }
+        T bx[4][VEC];
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            const int i = si0 + 4 * VEC * s + VEC * q;
+#pragma unroll
+            for (int e = 0; e < VEC; ++e) bx[s][e] = vec[(sx + min(i + e, scols - 1)) * kNR + m];
+        }
+        more = next();
+        wave_sync();
+#pragma unroll
+        for (int s = 0; s < 4; ++s)
+#pragma unroll
+            for (int gi = 0; gi < 2; ++gi)
+                if (gi < nown) {
+                    const VL a = *(const VL *)&reg[16 * gi + m][4 * VEC * s + VEC * q];
+#pragma unroll
+                    for (int e = 0; e < VEC; ++e) acc[gi] = Mfma16<T>::op(a[e], bx[s][e], acc[gi]);
+                }
+        wave_sync();
+    }
+    if (ns > 1) {
+        // partial products of group g0 from waves g0, g0 + ng, ..: summed by wave g0
+        T(*red)[kNR] = (T(*)[kNR])&lds[wv][0][0];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) red[Mfma16<T>::row(q, v)][m] = acc[0][v];
+        __syncthreads();
+        if (wv >= ng) return;
+        for (int o = wv + ng; o < 4; o += ng) {
+            T(*ro)[kNR] = (T(*)[kNR])&lds[o][0][0];
+#pragma unroll
+            for (int v = 0; v < 4; ++v) acc[0][v] += ro[Mfma16<T>::row(q, v)][m];
+        }
+    }
+#pragma unroll
+    for (int gi = 0; gi < 2; ++gi)
+        if (gi < nown) {
+            T *y = vec + it.y * kNR + m;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int r = 16 * (gi ? wv + 4 : g0) + Mfma16<T>::row(q, v);
+                if (r < rows) {
+                    if (it.flags & 1) atomicAdd(y + (int64_t)r * kNR, acc[gi][v]);
+                    else y[(int64_t)r * kNR] += acc[gi][v];
+                }
+            }
+        }
 }
 
 // leading sub-block copies from the FP32 pool into the inner-operator buffer
@@ -1457,34 +1692,36 @@ void launch_cur(const CurJob *jobs, int njobs, const int32_t *idx, const double 
     if (njobs > 0) hipLaunchKernelGGL(k_cur, dim3(njobs), dim3(256), 0, s, jobs, idx, lr, A, kp);
 }
 template <typename T, int ROLE>
-static void launch_bgemv_r(const GItem *items, int nitems, const GTerm *terms, const T *A0, const T *A1, T *vec,
-                           int unr, hipStream_t s) {
+static void launch_bgemv_r(const GItem *items, int nitems, const GTerm *terms, const T *A0, const T *A1, T *stag,
+                           T *vec, int unr, hipStream_t s) {
     constexpr int VEC = sizeof(T) == 8 ? 2 : 4;
     if (unr <= 2)
-        hipLaunchKernelGGL((k_bgemv<T, VEC, ROLE, 2>), dim3(nitems), dim3(256), 0, s, items, terms, A0, A1, vec);
+        hipLaunchKernelGGL((k_bgemv<T, VEC, ROLE, 2>), dim3(nitems), dim3(256), 0, s, items, terms, A0, A1, vec, stag);
     else if (unr >= 8)
-        hipLaunchKernelGGL((k_bgemv<T, VEC, ROLE, 8>), dim3(nitems), dim3(256), 0, s, items, terms, A0, A1, vec);
-    else hipLaunchKernelGGL((k_bgemv<T, VEC, ROLE, 4>), dim3(nitems), dim3(256), 0, s, items, terms, A0, A1, vec);
+        hipLaunchKernelGGL((k_bgemv<T, VEC, ROLE, 8>), dim3(nitems), dim3(256), 0, s, items, terms, A0, A1, vec, stag);
+    else
+        hipLaunchKernelGGL((k_bgemv<T, VEC, ROLE, 4>), dim3(nitems), dim3(256), 0, s, items, terms, A0, A1, vec, stag);
 }
 template <typename T>
-static void launch_bgemv_t(const GItem *items, int nitems, const GTerm *terms, const T *A0, const T *A1, T *vec,
-                           int role, int unr, hipStream_t s) {
+static void launch_bgemv_t(const GItem *items, int nitems, const GTerm *terms, const T *A0, const T *A1, T *stag,
+                           T *vec, int role, int unr, hipStream_t s) {
     if (nitems <= 0) return;
     if (!A1) A1 = A0;
-    if (role == 1) launch_bgemv_r<T, 1>(items, nitems, terms, A0, A1, vec, unr, s);
-    else if (role == 2) launch_bgemv_r<T, 2>(items, nitems, terms, A0, A1, vec, unr, s);
-    else launch_bgemv_r<T, 0>(items, nitems, terms, A0, A1, vec, unr, s);
+    if (role == 1) launch_bgemv_r<T, 1>(items, nitems, terms, A0, A1, stag, vec, unr, s);
+    else if (role == 2) launch_bgemv_r<T, 2>(items, nitems, terms, A0, A1, stag, vec, unr, s);
+    else if (role == 3) launch_bgemv_r<T, 3>(items, nitems, terms, A0, A1, stag, vec, unr, s);
+    else launch_bgemv_r<T, 0>(items, nitems, terms, A0, A1, stag, vec, unr, s);
 }
 void launch_copy32(const CopyJob *jobs, int njobs, const float *src, float *dst, hipStream_t s) {
     if (njobs > 0) hipLaunchKernelGGL(k_copy32, dim3(njobs), dim3(256), 0, s, jobs, src, dst);
 }
 void launch_bgemv64(const GItem *items, int nitems, const GTerm *terms, const double *A0, const double *A1,
-                    double *vec, int role, int unr, hipStream_t s) {
-    launch_bgemv_t<double>(items, nitems, terms, A0, A1, vec, role, unr, s);
+                    double *stag, double *vec, int role, int unr, hipStream_t s) {
+    launch_bgemv_t<double>(items, nitems, terms, A0, A1, stag, vec, role, unr, s);
 }
-void launch_bgemv32(const GItem *items, int nitems, const GTerm *terms, const float *A0, const float *A1, float *vec,
-                    int role, int unr, hipStream_t s) {
-    launch_bgemv_t<float>(items, nitems, terms, A0, A1, vec, role, unr, s);
+void launch_bgemv32(const GItem *items, int nitems, const GTerm *terms, const float *A0, const float *A1,
+                    float *stag, float *vec, int role, int unr, hipStream_t s) {
+    launch_bgemv_t<float>(items, nitems, terms, A0, A1, stag, vec, role, unr, s);
 }
 void launch_otf32(const GItem *items, int nitems, const GTerm *terms, SlotPoints sp, float *vec, hipStream_t s) {
     if (nitems > 0)
@@ -1673,5 +1910,17 @@ void launch_wide_step(const AcaJob *jobs, int nb, int maxm, int maxn, int maxran
 void launch_wide_finish(const WideState *state, int nb, int32_t *rank_out, int32_t *status_out, int32_t *ndone,
                         hipStream_t s) {
     hipLaunchKernelGGL(k_wide_finish, dim3(1), dim3(64), 0, s, state, nb, rank_out, status_out, ndone);
+}
+void launch_bgemm_t64(const GItem *items, int nitems, const GTerm *terms, const double *A0, const double *A1,
+                      const double *const *tab, double *vec, hipStream_t s) {
+    if (nitems > 0)
+        hipLaunchKernelGGL(k_bgemm_tl<double>, dim3(nitems), dim3(256), 0, s, items, terms, A0, A1 ? A1 : A0, tab,
+                           vec);
+}
+void launch_bgemm_t32(const GItem *items, int nitems, const GTerm *terms, const float *A0, const float *A1,
+                      const float *const *tab, float *vec, hipStream_t s) {
+    if (nitems > 0)
+        hipLaunchKernelGGL(k_bgemm_tl<float>, dim3(nitems), dim3(256), 0, s, items, terms, A0, A1 ? A1 : A0, tab,
+                           vec);
 }
 }  // namespace h2w

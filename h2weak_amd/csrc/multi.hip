@@ -25,26 +25,57 @@ void ensure_pool_m(Handle &h, int precision) {
     else h.vec64m.ensure((size_t)h.vec_len * kNR);
 }
 
-void run_launches_m_eager(Handle &h, int precision) {
-    const auto &L = launches_for(h, precision);
+// The fused stage-1 launch applies the stored blocks K(b, p), p >= b, directly; the
+// single-vector path's gather (role 3) is replaced by the transposed products
+// K(b, p)^T X_b of the role-3 lists (h.trans, h.inner_trans) on the matrix cores.
+// One step of the sequence: a launch of the precision's lists, or (trans) launch Lh of
+// its role-3 lists.
+void run_launch_m(Handle &h, const Launch &Lh, int precision, bool trans) {
     hipStream_t s = h.stream;
-    const int64_t nout = (h.vec_len - h.off_y) * kNR;
-    if (precision == H2W_FP32_INNER) {
-        launch_fill32(h.vec32m.p + h.off_y * kNR, nout, 0.0f, s);
-        for (auto &Lh : L)
-            launch_bgemm32(h.inner.d_items.p + Lh.item0, (int)Lh.nitems, h.inner.d_terms.p, h.mat32.p, h.mat32in.p,
-                           h.vec32m.p, Lh.role, s);
+    if (trans) {
+        const LaunchSet &T = precision == H2W_FP32_INNER ? h.inner_trans : h.trans;
+        if (precision == H2W_FP32_INNER)
+            launch_bgemm_t32(T.d_items.p + Lh.item0, (int)Lh.nitems, T.d_terms.p, h.mat32.p, h.mat32in.p, nullptr,
+                             h.vec32m.p, s);
+        else if (precision == H2W_FP32)
+            launch_bgemm_t32(T.d_items.p + Lh.item0, (int)Lh.nitems, T.d_terms.p, h.mat32.p, nullptr, nullptr,
+                             h.vec32m.p, s);
+        else
+            launch_bgemm_t64(T.d_items.p + Lh.item0, (int)Lh.nitems, T.d_terms.p, h.mat64.p, nullptr, nullptr,
+                             h.vec64m.p, s);
+    } else if (precision == H2W_FP32_INNER) {
+        launch_bgemm32(h.inner.d_items.p + Lh.item0, (int)Lh.nitems, h.inner.d_terms.p, h.mat32.p, h.mat32in.p,
+                       h.vec32m.p, Lh.role, s);
     } else if (precision == H2W_FP32) {
-        launch_fill32(h.vec32m.p + h.off_y * kNR, nout, 0.0f, s);
-        for (auto &Lh : L)
-            launch_bgemm32(h.d_items.p + Lh.item0, (int)Lh.nitems, h.d_terms.p, h.mat32.p, nullptr, h.vec32m.p,
-                           Lh.role, s);
+        launch_bgemm32(h.d_items.p + Lh.item0, (int)Lh.nitems, h.d_terms.p, h.mat32.p, nullptr, h.vec32m.p, Lh.role,
+                       s);
     } else {
-        launch_fill(h.vec64m.p + h.off_y * kNR, nout, 0.0, s);
-        for (auto &Lh : L)
-            launch_bgemm64(h.d_items.p + Lh.item0, (int)Lh.nitems, h.d_terms.p, h.mat64.p, nullptr, h.vec64m.p,
-                           Lh.role, s);
+        launch_bgemm64(h.d_items.p + Lh.item0, (int)Lh.nitems, h.d_terms.p, h.mat64.p, nullptr, h.vec64m.p, Lh.role,
+                       s);
     }
+}
+
+// the sequence as (launch, trans) steps, output zeroing not included
+std::vector<std::pair<const Launch *, bool>> steps_m(Handle &h, int precision) {
+    std::vector<std::pair<const Launch *, bool>> st;
+    const LaunchSet &T = precision == H2W_FP32_INNER ? h.inner_trans : h.trans;
+    for (auto &Lh : launches_for(h, precision)) {
+        if (Lh.role == 3)
+            for (auto &Lt : T.launches) st.push_back({&Lt, true});
+        else st.push_back({&Lh, false});
+    }
+    return st;
+}
+
+void zero_outputs_m(Handle &h, int precision) {
+    const int64_t nout = (h.vec_len - h.off_y) * kNR;
+    if (is32(precision)) launch_fill32(h.vec32m.p + h.off_y * kNR, nout, 0.0f, h.stream);
+    else launch_fill(h.vec64m.p + h.off_y * kNR, nout, 0.0, h.stream);
+}
+
+void run_launches_m_eager(Handle &h, int precision) {
+    zero_outputs_m(h, precision);
+    for (auto &st : steps_m(h, precision)) run_launch_m(h, *st.first, precision, st.second);
     H2W_HIP(hipGetLastError());
 }
 
@@ -440,9 +471,49 @@ extern "C" int h2w_bench_matvec_multi(h2w_handle *h, int nrep, int precision, h2
         out->ms_per_matvec = ms / nrep;
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
-        out->n_launch = 1;
+        // entry 0: the whole pass and the operator bytes it multiplies (every stored element
+        // once, the symmetric blocks twice: direct and transposed products); entries 1..: the
+        // launches one by one between events (eager), role 4 for the transposed products
+        const bool in = precision == H2W_FP32_INNER;
+        const h2w::LaunchSet &tr = in ? h->inner_trans : h->trans;
+        const int es = h2w::is32(precision) ? 4 : 8;
+        int64_t el = in ? h->inner_elems : h->stored_elems;
+        for (auto e : tr.elems) el += e;
         out->launch_ms[0] = out->ms_per_matvec;
         out->launch_role[0] = -2;
+        out->launch_bytes[0] = el * es;
+        const auto st = h2w::steps_m(*h, precision);
+        const int nl = (int)std::min<size_t>(st.size(), H2W_MAX_LAUNCHES - 1);
+        std::vector<hipEvent_t> ev(nl + 1);
+        for (auto &e : ev) H2W_HIP(hipEventCreate(&e));
+        std::vector<double> acc(nl, 0.0);
+        for (int r = 0; r < nrep; ++r) {
+            h2w::zero_outputs_m(*h, precision);
+            H2W_HIP(hipEventRecord(ev[0], s));
+            for (int q = 0; q < nl; ++q) {
+                h2w::run_launch_m(*h, *st[q].first, precision, st[q].second);
+                H2W_HIP(hipEventRecord(ev[q + 1], s));
+            }
+            H2W_HIP(hipEventSynchronize(ev[nl]));
+            for (int q = 0; q < nl; ++q) {
+                float m2 = 0.f;
+                H2W_HIP(hipEventElapsedTime(&m2, ev[q], ev[q + 1]));
+                acc[q] += m2;
+            }
+        }
+        for (auto &e : ev) (void)hipEventDestroy(e);
+        const auto &L = h2w::launches_for(*h, precision);
+        for (int q = 0; q < nl; ++q) {
+            const h2w::Launch *Lp = st[q].first;
+            out->launch_ms[q + 1] = acc[q] / nrep;
+            out->launch_role[q + 1] = st[q].second ? 4 : Lp->role;
+            int64_t e = 0;
+            if (st[q].second) e = tr.elems[Lp - tr.launches.data()];
+            else if (in) e = h->inner.elems[Lp - L.data()];
+            else e = h->launch_elems[Lp - L.data()];
+            out->launch_bytes[q + 1] = e * es;
+        }
+        out->n_launch = nl + 1;
         return H2W_OK;
     } catch (const Error &e) {
         h2w::set_error(e.msg);
