@@ -135,18 +135,53 @@ def pmc_reduce(d, side, tp):
     f_fetch, f_write = nb / per["fetch"][kc][1], nb / per["write"][kc][1]
     out = {"calibration": {"copy_bytes": int(nb), "fetch_factor": round(f_fetch, 5), "write_factor": round(f_write, 5),
                            "note": "bytes = counter KiB * 1024 * factor; factors from the 1 GiB device copy"}}
+    probe = None  # the probe's own line (launch bytes), from the logs of the two runs
+    for fn in sorted(os.listdir(d)):
+        if fn.endswith(".log"):
+            for line in open(os.path.join(d, fn)):
+                if line.startswith('{"pmc_probe"'):
+                    probe = json.loads(line)
     for prec, key in (("double", "fp64_m2l_near"), ("float", "fp32inner_m2l_near")):
         pat = re.compile(r"k_bgemv<%s, \d+, 1[,>]" % prec)
         ks = sorted(k for k, (n, v) in per["fetch"].items() if pat.search(n) and k in per["write"])
         rd = [per["fetch"][k][1] * f_fetch for k in ks]
         wr = [per["write"][k][1] * f_write for k in ks]
         tot = sorted(a + b for a, b in zip(rd, wr))
-        out[f"3d_rbf_{side}_{tp}_{key}"] = {"dispatches": len(ks), "fetch_bytes": int(sorted(rd)[len(rd) // 2]),
-                                           "write_bytes": int(sorted(wr)[len(wr) // 2]),
-                                           "hbm_bytes_per_launch": int(tot[len(tot) // 2])}
+        rec = {"dispatches": len(ks), "fetch_bytes": int(sorted(rd)[len(rd) // 2]),
+               "write_bytes": int(sorted(wr)[len(wr) // 2]), "hbm_bytes_per_launch": int(tot[len(tot) // 2])}
+        if probe:
+            tag, es = ("fp64", 8) if prec == "double" else ("fp32_inner", 4)
+            q = probe[f"launch_role_{tag}"].index(1)
+            alg = probe[f"launch_bytes_{tag}"][q] + 2 * es * probe["N"]  # the bench's algorithmic bytes
+            rec["algorithmic_bytes"] = int(alg)
+            rec["traffic_over_algorithmic"] = round(rec["hbm_bytes_per_launch"] / alg, 4)
+        out[f"3d_rbf_{side}_{tp}_{key}"] = rec
     path = os.path.join(ROOT, "profiles", "pmc_traffic.json")
     json.dump(out, open(path, "w"), indent=1)
     print(json.dumps(out))
+
+
+def kernel_table(path, top=25):
+    """Markdown table of a rocprofv3 --stats kernel summary (<prefix>_kernel_stats.csv), top kernels by
+    total time, names shortened to the template-id (no GPU)."""
+    import csv
+    rows = []
+    for r in csv.DictReader(open(path)):
+        name = r["Name"].replace("(anonymous namespace)::", "")
+        depth, cut = 0, len(name)
+        for i, ch in enumerate(name):  # the argument list: the first '(' outside template brackets
+            depth += ch == "<"
+            depth -= ch == ">"
+            if ch == "(" and depth == 0:
+                cut = i
+                break
+        name = name[:cut].replace("void ", "").replace("h2w::", "").strip()
+        rows.append((float(r["TotalDurationNs"]), int(r["Calls"]), float(r["AverageNs"]), float(r["Percentage"]), name))
+    rows.sort(reverse=True)
+    print("| total ms | calls | average us | % | kernel |")
+    print("|---:|---:|---:|---:|---|")
+    for tot, calls, avg, pct, name in rows[:top]:
+        print(f"| {tot * 1e-6:.1f} | {calls} | {avg * 1e-3:.1f} | {pct:.1f} | `{name}` |")
 
 
 def bench_batch(S=32, tol_pow=8, seed=5):
@@ -325,7 +360,12 @@ def main():
                     help="rank-predictor measurement (N = 262144 uniform and Gaussian, heuristic vs model)")
     ap.add_argument("--transcripts", action="store_true",
                     help="the reference's eight published runs on the GPU, beside the published CPU times")
+    ap.add_argument("--kernel-table", metavar="CSV",
+                    help="print a rocprofv3 --stats kernel summary CSV as a markdown table (no GPU)")
     args = ap.parse_args()
+    if args.kernel_table:
+        kernel_table(args.kernel_table)
+        return
     if args.assembly_only:
         import h2weak_amd as h2w
         N = args.side ** 3
@@ -373,9 +413,11 @@ def main():
         src, dst = hip.malloc(nb), hip.malloc(nb)
         hip.d2d(dst, src, nb)
         hip.sync()
+        r64, r32 = A.bench_matvec(1, h2w.FP64), A.bench_matvec(1, h2w.FP32_INNER)
         print(json.dumps({"pmc_probe": True, "copy_bytes": nb, "workload": f"3D RBF grid {side}^3 tol 1e-{tp}",
-                          "launch_elems_fp64": A.bench_matvec(1, h2w.FP64)["launch_bytes"],
-                          "launch_elems_fp32_inner": A.bench_matvec(1, h2w.FP32_INNER)["launch_bytes"]}))
+                          "N": N, "launch_bytes_fp64": r64["launch_bytes"], "launch_role_fp64": r64["launch_role"],
+                          "launch_bytes_fp32_inner": r32["launch_bytes"],
+                          "launch_role_fp32_inner": r32["launch_role"]}), flush=True)
         return
     xt = np.random.default_rng(2).uniform(-1.0, 1.0, N)
     b = A.dense_rows(None, xt)
@@ -473,12 +515,19 @@ def main():
     tr_path = os.path.join(ROOT, "profiles", "pmc_traffic.json")
     if os.path.exists(tr_path):
         tr = json.load(open(tr_path))
-        key = f"3d_rbf_{side}_{tp}_fp64_m2l_near"
-        if key in tr:
-            roof_st["traffic"] = tr[key]["hbm_bytes_per_launch"]
-        key = f"3d_rbf_{side}_{tp}_fp32inner_m2l_near"
-        if key in tr:
-            roof_in["traffic"] = tr[key]["hbm_bytes_per_launch"]
+        # measured HBM bytes of the same launch; when the tuner chose another inner operator than
+        # the counter runs measured, the measured traffic/algorithmic ratio times this launch's bytes
+        for roof, key in ((roof_st, f"3d_rbf_{side}_{tp}_fp64_m2l_near"), (roof_in, f"3d_rbf_{side}_{tp}_fp32inner_m2l_near")):
+            rec = tr.get(key)
+            if not rec or "algorithmic_bytes" not in rec:
+                continue
+            if abs(rec["algorithmic_bytes"] - roof["algorithmic_bytes"]) <= 1e-3 * roof["algorithmic_bytes"]:
+                roof["traffic"] = rec["hbm_bytes_per_launch"]
+                roof["traffic_source"] = "rocprofv3 FETCH_SIZE + WRITE_SIZE of this launch"
+            else:
+                roof["traffic"] = int(rec["traffic_over_algorithmic"] * roof["algorithmic_bytes"])
+                roof["traffic_source"] = (f"measured traffic/algorithmic ratio {rec['traffic_over_algorithmic']} of the "
+                                          f"same kernel on a launch of {rec['algorithmic_bytes'] / 1e9:.2f} GB, scaled")
     roof, roof2 = (roof_in, roof_st) if t_in >= t_st else (roof_st, roof_in)
     copy_bw = hip.copy_gbps()
     log(f"device-to-device copy: {copy_bw:.0f} GB/s (read + write)")
