@@ -206,7 +206,6 @@ struct h2w_handle {
     h2w::DevBuf<int32_t> spid;             // slot point ids
     h2w::SlotPoints sp;
     int64_t vec_len = 0, off_x = 0, off_y = 0;
-    int64_t off_one = 0, ones_len = 0;  // ones segment of the vector pools (x of the staging gather)
     std::vector<h2w::FamLevel> fam[2];
     std::vector<h2w::Pair> pairs;
     std::vector<h2w::Launch> launches;

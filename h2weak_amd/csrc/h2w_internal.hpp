@@ -203,6 +203,11 @@ void launch_axpby(double a, const double *x, double b, double *y, int64_t n, hip
 void launch_scal(double a, double *y, int64_t n, hipStream_t s);  // y *= a
 void launch_vec_update(const double *V, int64_t ldv, int k, const double *c, double *x, int64_t n, hipStream_t s);
 
+// the gather of the staging slots (role 3 launches of the single-vector path)
+void launch_gather64(const GItem *items, int nitems, const GTerm *terms, const double *stag, double *vec,
+                     hipStream_t s);
+void launch_gather32(const GItem *items, int nitems, const GTerm *terms, const float *stag, float *vec,
+                     hipStream_t s);
 // kNR right-hand sides, transposed products of the stored symmetric blocks (role 3 lists)
 void launch_bgemm_t64(const GItem *items, int nitems, const GTerm *terms, const double *A0, const double *A1,
                       const double *const *tab, double *vec, hipStream_t s);

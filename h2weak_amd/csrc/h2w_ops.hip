@@ -690,8 +690,12 @@ void add_item(WorkList &W, int64_t y, int rows, const std::vector<TermSpec> &ter
 // step k >= 1 with |u_k||v_k| <= tau * normS_k ends the cross (2D/codes/aca.hpp:363)
 // Items whose terms are never split by columns: a row tile's terms are grouped into
 // items of at most kItemWork elements (a tile split over several items accumulates
-// atomically).  A symmetric term (t >= 0) of row tile k writes the transposed product
-// of its rows to staging slot t + k pad_ld(cols).
+// atomically).  Terms keep their t (a symmetric term's staging slot, assigned per row tile
+// by the caller).
+// Staging slots start on their own cache lines (32 elements: 128 bytes in FP32, 256 in
+// FP64): a line shared by the slots of two items would be written back half-filled.
+inline int64_t slot_ld(int64_t r) { return align_up(r, 32); }
+
 void add_item_whole(WorkList &W, int64_t y, int64_t xb, int rows, const std::vector<TermSpec> &terms, int lda,
                     int pool) {
     if (rows <= 0) return;
@@ -699,7 +703,7 @@ void add_item_whole(WorkList &W, int64_t y, int64_t xb, int rows, const std::vec
     for (auto &t : terms)
         if (t.cols > 0) ts.push_back(&t);
     if (ts.empty()) return;
-    for (int r0 = 0, k = 0; r0 < rows; r0 += kRowTile, ++k) {
+    for (int r0 = 0; r0 < rows; r0 += kRowTile) {
         const int tr = std::min(kRowTile, rows - r0);
         std::vector<std::pair<size_t, size_t>> groups;  // [first, last) of ts per item
         size_t g0 = 0;
@@ -724,7 +728,7 @@ void add_item_whole(WorkList &W, int64_t y, int64_t xb, int rows, const std::vec
             it.t0 = (int32_t)W.terms.size();
             for (size_t q = g.first; q < g.second; ++q) {
                 const TermSpec &t = *ts[q];
-                W.terms.push_back(GTerm{t.a, t.x, t.t >= 0 ? t.t + (int64_t)k * pad_ld(t.cols) : -1, t.cols, t.ld});
+                W.terms.push_back(GTerm{t.a, t.x, t.t, t.cols, t.ld});
             }
             it.t1 = (int32_t)W.terms.size();
             it.flags = (groups.size() > 1 ? 1 : 0) | (pool << 1);
@@ -745,59 +749,63 @@ struct SymGroup {
     int pool = 0;
 };
 // The four lists of such a group: fused items (box b: y_b += K(b, p) x_p over its stored
-// blocks, each block's K(b, p)^T x_b into a staging slot of p), the gather (y_p += the
-// staging slots of p, a GEMV with the ones segment), the 16-wide path's transposed items
-// (box p: y_p += K(b, p)^T x_b, b < p) and the on-the-fly path's directed items.
+// blocks, each block's K(b, p)^T x_b into a staging slot), the gather (y_p += the staging
+// slots of p), the 16-wide path's transposed items (box p: y_p += K(b, p)^T x_b, b < p) and
+// the on-the-fly path's directed items.  Slots are laid out in the order the fused items
+// write them (item after item, term after term), so that every fused item writes one
+// contiguous range; the gather of p reads its slots from the ranges of the items of its
+// lower partners (its terms' a are the slot offsets).
 struct SymLists {
     WorkList fused, gather, trans, otf;
     int64_t stag = 0;  // staging elements
 };
-void sym_lists(const SymGroup &G, SymLists &L, int64_t off_one) {
+void sym_lists(const SymGroup &G, SymLists &L) {
     const int nb = (int)G.r.size();
     std::vector<std::vector<std::pair<int, int64_t>>> down(nb);
-    std::vector<std::vector<int64_t>> slot(nb);  // slot offset of each up[b] entry
-    for (int b = 0; b < nb; ++b) slot[b].assign(G.up[b].size(), -1);
     for (int b = 0; b < nb; ++b)
         for (auto &pe : G.up[b]) down[pe.first].push_back({b, pe.second});
-    std::vector<int64_t> base(nb, -1);
-    std::vector<int> nslot(nb, 0);
-    {
-        std::vector<size_t> cur(nb, 0);  // next up[b] entry to assign (partners come in increasing p)
-        for (int p = 0; p < nb; ++p) {
-            int c = 0;
-            for (auto &be : down[p]) c += (G.r[be.first] + kRowTile - 1) / kRowTile;
-            nslot[p] = G.r[p] > 0 ? c : 0;
-            if (nslot[p] > 0) {
-                base[p] = L.stag;
-                L.stag += align_up((int64_t)c * pad_ld(G.r[p]), 32);
-            }
-            int col = 0;
-            for (auto &be : down[p]) {
-                const int b = be.first;
-                size_t k = 0;
-                while (k < G.up[b].size() && G.up[b][k].first != p) ++k;
-                if (base[p] >= 0) slot[b][k] = base[p] + (int64_t)col * pad_ld(G.r[p]);
-                col += (G.r[b] + kRowTile - 1) / kRowTile;
-            }
-            (void)cur;
-        }
-    }
+    // fused items; a symmetric term carries its partner p in t until its slot is assigned
+    const size_t item0 = L.fused.items.size();
     for (int b = 0; b < nb; ++b) {
         if (G.r[b] <= 0) continue;
         std::vector<TermSpec> t;
         if (G.self[b] >= 0) t.push_back(TermSpec{G.self[b], G.x[b], G.r[b]});
-        for (size_t k = 0; k < G.up[b].size(); ++k) {
-            const int p = G.up[b][k].first;
-            TermSpec ts{G.up[b][k].second, G.x[p], G.r[p]};
-            ts.t = slot[b][k];
+        for (auto &pe : G.up[b]) {
+            TermSpec ts{pe.second, G.x[pe.first], G.r[pe.first]};
+            ts.t = pe.first;
             t.push_back(ts);
         }
         add_item_whole(L.fused, G.y[b], G.x[b], G.r[b], t, (int)pad_ld(G.r[b]), G.pool);
     }
-    for (int p = 0; p < nb; ++p)
-        if (nslot[p] > 0)
-            add_item_whole(L.gather, G.y[p], 0, G.r[p], {TermSpec{base[p], off_one, nslot[p]}}, (int)pad_ld(G.r[p]),
-                           0);
+    std::vector<std::vector<int64_t>> slots(nb);  // slot offsets of p, in order of the fused items
+    for (size_t q = item0; q < L.fused.items.size(); ++q) {
+        const GItem &it = L.fused.items[q];
+        for (int32_t k = it.t0; k < it.t1; ++k) {
+            GTerm &tm = L.fused.terms[k];
+            if (tm.t < 0) continue;
+            const int p = (int)tm.t;
+            tm.t = L.stag;
+            slots[p].push_back(L.stag);
+            L.stag += slot_ld(G.r[p]);
+        }
+    }
+    // gather: one item per row tile of p, one term per slot (a = slot offset)
+    for (int p = 0; p < nb; ++p) {
+        if (slots[p].empty()) continue;
+        for (int r0 = 0; r0 < G.r[p]; r0 += kRowTile) {
+            GItem it;
+            it.y = G.y[p] + r0;
+            it.xb = 0;
+            it.rows = std::min(kRowTile, G.r[p] - r0);
+            it.lda = 0;
+            it.row0 = r0;
+            it.t0 = (int32_t)L.gather.terms.size();
+            for (int64_t o : slots[p]) L.gather.terms.push_back(GTerm{o, 0, -1, 1, 0});
+            it.t1 = (int32_t)L.gather.terms.size();
+            it.flags = 0;
+            L.gather.items.push_back(it);
+        }
+    }
     for (int p = 0; p < nb; ++p) {
         std::vector<TermSpec> t;
         for (auto &be : down[p]) {
@@ -1074,7 +1082,7 @@ void build_inner(Handle &h, double irp) {
                         const M2LBlk &m = m2l[pr.first];
                         if (m.f == f && m.j == j) G.up[m.b].push_back({m.p, base + pr.second});
                     }
-                    sym_lists(G, SL, h.off_one);
+                    sym_lists(G, SL);
                 }
             }
             {
@@ -1093,7 +1101,7 @@ void build_inner(Handle &h, double irp) {
                     if (nb.n == nb.b) G.self[nb.b] = nb.off;
                     else G.up[nb.b].push_back({nb.n, nb.off});
                 }
-                sym_lists(G, SL, h.off_one);
+                sym_lists(G, SL);
             }
             SL.fused.role = 1;
             SL.fused.stage = 1;
@@ -1233,33 +1241,8 @@ void assemble_operators(Handle &h, double t0) {
     const int nfam_lo[2] = {2, 1};  // first level of each family
     const bool fam_on[2] = {P.dim >= 2 && h.variant != H2W_NON_NESTED, h.variant == H2W_NESTED};
 
-    // ---- vector pool: a segment of ones (the staging gather's x), then x, y, skeleton vectors
+    // ---- vector pool: x, y, skeleton vectors
     int64_t vo = 0;
-    {
-        // the gather of box p sums one staging slot per row tile of every lower partner
-        int64_t mx = 1;
-        for (int f = 0; f < 2; ++f) {
-            if (!fam_on[f]) continue;
-            for (int j = nfam_lo[f]; j <= P.nL; ++j) {
-                const FamLevel &fl = h.fam[f][j];
-                for (int p = 0; p < fl.nb; ++p) {
-                    int64_t c = 0;
-                    for (int b : (f == 0 ? P.far_of(j, p) : P.vert_of(j, p)))
-                        if (b < p) c += (fl.rank[b] + kRowTile - 1) / kRowTile;
-                    mx = std::max(mx, c);
-                }
-            }
-        }
-        for (int p = 0; p < P.nboxes[P.nL]; ++p) {
-            int64_t c = 0;
-            for (int b : P.near[p])
-                if (b < p) c += (P.bcount[P.nL][b] + kRowTile - 1) / kRowTile;
-            mx = std::max(mx, c);
-        }
-        h.off_one = vo;
-        h.ones_len = mx;
-        vo += align_up(mx, 32);
-    }
     h.off_x = vo;
     vo += align_up(P.N, 32);
     h.off_y = vo;
@@ -1359,9 +1342,7 @@ void assemble_operators(Handle &h, double t0) {
     h.mat64.alloc(h.mat_len);
     H2W_HIP(hipMemsetAsync(h.mat64.p, 0, h.mat_len * 8, s));
     h.vec64.alloc(h.vec_len);
-    launch_fill(h.vec64.p + h.off_one, h.ones_len, 1.0, s);
     h.vec32.alloc(h.vec_len);
-    launch_fill32(h.vec32.p + h.off_one, h.ones_len, 1.0f, s);
     {
         // the point of every vector slot, for on-the-fly near-field / M2L evaluation
         std::vector<int32_t> pid(h.vec_len, -1);
@@ -1582,7 +1563,7 @@ void assemble_operators(Handle &h, double t0) {
                 G.self.assign(fl.nb, -1);
                 for (auto &m : m2l)
                     if (m.f == f && m.j == j) G.up[m.b].push_back({m.p, m.off});
-                sym_lists(G, SL, h.off_one);
+                sym_lists(G, SL);
             }
         }
         {
@@ -1601,7 +1582,7 @@ void assemble_operators(Handle &h, double t0) {
                 if (nb.n == nb.b) G.self[nb.b] = nb.off;
                 else G.up[nb.b].push_back({nb.n, nb.off});
             }
-            sym_lists(G, SL, h.off_one);
+            sym_lists(G, SL);
         }
         SL.fused.role = 1;
         SL.fused.stage = 1;
@@ -1734,9 +1715,8 @@ bool is32(int precision) { return precision == H2W_FP32 || precision == H2W_FP32
 
 // Stage 1 is one fused launch (role 1: every stored block K(b, p), p >= b, applied to
 // x_p, the transposed products K(b, p)^T x_b written to staging slots) and one gather
-// (role 3: y_p += the staging slots of p, a GEMV over the staging buffer with the ones
-// segment).  The on-the-fly precisions evaluate the blocks from the points instead,
-// over the directed lists, and skip the gather.
+// (role 3: y_p += the staging slots of p).  The on-the-fly precisions evaluate the
+// blocks from the points instead, over the directed lists, and skip the gather.
 void run_launch(Handle &h, const Launch &Lh, int precision) {
     hipStream_t s = h.stream;
     const bool otf = (precision == H2W_FP32_OTF || precision == H2W_FP64_OTF) && (Lh.role == 1 || Lh.role == 3);
@@ -1750,15 +1730,18 @@ void run_launch(Handle &h, const Launch &Lh, int precision) {
         }
         return;
     }
-    if (precision == H2W_FP32_INNER) {
-        launch_bgemv32(h.inner.d_items.p + Lh.item0, (int)Lh.nitems, h.inner.d_terms.p,
-                       Lh.role == 3 ? h.stag32.p : h.mat32.p, h.mat32in.p, h.stag32.p, h.vec32.p, Lh.role, Lh.unr, s);
+    const bool in = precision == H2W_FP32_INNER;
+    const GItem *items = (in ? h.inner.d_items.p : h.d_items.p) + Lh.item0;
+    const GTerm *terms = in ? h.inner.d_terms.p : h.d_terms.p;
+    if (Lh.role == 3) {
+        if (is32(precision)) launch_gather32(items, (int)Lh.nitems, terms, h.stag32.p, h.vec32.p, s);
+        else launch_gather64(items, (int)Lh.nitems, terms, h.stag64.p, h.vec64.p, s);
+    } else if (in) {
+        launch_bgemv32(items, (int)Lh.nitems, terms, h.mat32.p, h.mat32in.p, h.stag32.p, h.vec32.p, Lh.role, Lh.unr, s);
     } else if (is32(precision)) {
-        launch_bgemv32(h.d_items.p + Lh.item0, (int)Lh.nitems, h.d_terms.p, Lh.role == 3 ? h.stag32.p : h.mat32.p,
-                       nullptr, h.stag32.p, h.vec32.p, Lh.role, Lh.unr32, s);
+        launch_bgemv32(items, (int)Lh.nitems, terms, h.mat32.p, nullptr, h.stag32.p, h.vec32.p, Lh.role, Lh.unr32, s);
     } else {
-        launch_bgemv64(h.d_items.p + Lh.item0, (int)Lh.nitems, h.d_terms.p, Lh.role == 3 ? h.stag64.p : h.mat64.p,
-                       nullptr, h.stag64.p, h.vec64.p, Lh.role, Lh.unr, s);
+        launch_bgemv64(items, (int)Lh.nitems, terms, h.mat64.p, nullptr, h.stag64.p, h.vec64.p, Lh.role, Lh.unr, s);
     }
 }
 

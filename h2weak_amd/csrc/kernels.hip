@@ -835,12 +835,76 @@ __device__ __forceinline__ T xor_lane_n(T v, int o) {
     }
 }
 
+// Raw buffer access (gfx9 resource word 3): a lane whose offset lies at or past the
+// resource's size reads zeros and writes nothing, without a branch.
+constexpr int kBufWord3 = 0x00020000;
+constexpr uint32_t kOutOfRange = 0xFFFFFFF0u;
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const void *base, uint32_t bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), (short)0, (int)bytes, kBufWord3);
+}
+template <typename T>
+__device__ __forceinline__ T buf_ld(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+    if constexpr (sizeof(T) == 8) return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+    else return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
+}
+template <typename V>
+__device__ __forceinline__ V buf_ld16_nt(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+    return __builtin_bit_cast(V, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 2));
+}
+// N consecutive elements (N sizeof(T) = 8, 16, 32 or 64 bytes) in 8- or 16-byte loads
+template <typename T, int N>
+__device__ __forceinline__ void buf_ld_n(T (&out)[N], __amdgpu_buffer_rsrc_t r, uint32_t off) {
+    constexpr int B = N * (int)sizeof(T);
+    if constexpr (B == 8) {
+        typedef unsigned int U2 __attribute__((ext_vector_type(2)));
+        typedef T W __attribute__((ext_vector_type(N)));
+        const W w = __builtin_bit_cast(W, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+#pragma unroll
+        for (int k = 0; k < N; ++k) out[k] = w[k];
+    } else {
+        constexpr int E = 16 / sizeof(T);
+        typedef T W __attribute__((ext_vector_type(E)));
+#pragma unroll
+        for (int p = 0; p < B / 16; ++p) {
+            const W w = __builtin_bit_cast(W, __builtin_amdgcn_raw_buffer_load_b128(r, off + 16 * p, 0, 0));
+#pragma unroll
+            for (int k = 0; k < E; ++k) out[E * p + k] = w[k];
+        }
+    }
+}
+template <typename T>
+__device__ __forceinline__ void buf_st(T v, __amdgpu_buffer_rsrc_t r, uint32_t off) {
+    typedef unsigned int U2 __attribute__((ext_vector_type(2)));
+    if constexpr (sizeof(T) == 8) __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(U2, v), r, off, 0, 0);
+    else __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), r, off, 0, 0);
+}
+
+// Work lists are never written by a kernel: read through the constant address space, their
+// wave-uniform loads stay scalar after the kernel's own stores (a vector load there would
+// wait for every load in flight).
+typedef const __attribute__((address_space(4))) GTerm CGTerm;
+__device__ __forceinline__ GTerm term_at(const CGTerm *p, int k) {
+    GTerm t;
+    t.a = p[k].a;
+    t.x = p[k].x;
+    t.t = p[k].t;
+    t.cols = p[k].cols;
+    t.ld = p[k].ld;
+    return t;
+}
+
+// The column loop of an item walks (term, column) iterations with a wave-uniform
+// cursor; each iteration is UNR column loads per lane.  Two iterations are in flight:
+// the loads of the next are issued before the current one is multiplied.  Every load
+// and staging store is unconditional (lanes out of range address past the buffer's
+// end), so the compiler's waits count exactly the outstanding operations.
 template <typename T, int VEC, int ROLE, int UNR>
 __global__ __launch_bounds__(256) void k_bgemv(const GItem *__restrict__ items, const GTerm *__restrict__ terms,
                                                const T *__restrict__ A0, const T *__restrict__ A1, T *vec,
                                                T *__restrict__ stag) {
     typedef T V __attribute__((ext_vector_type(VEC)));
     const GItem it = items[blockIdx.x];
+    const CGTerm *const tms = (const CGTerm *)terms;
     const T *__restrict__ A = (it.flags & 2) ? A1 : A0;
     const int lane = threadIdx.x & 63;
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -857,75 +921,112 @@ __global__ __launch_bounds__(256) void k_bgemv(const GItem *__restrict__ items, 
 #pragma unroll
     for (int v = 0; v < VEC; ++v) xb[v] = (ROLE == 1 && act && r0 + v < rows) ? vec[it.xb + r0 + v] : T(0);
     const int64_t lda = it.lda;
-    const int step = 4 * cpw;
-    for (int tt = it.t0; tt < it.t1; ++tt) {
-        const GTerm tm = terms[tt];
-        const T *a = A + tm.a + it.row0 + r0;
-        const T *x = vec + tm.x;
-        const int cols = tm.cols;
-        const bool sym = ROLE == 1 && tm.t >= 0;
-        for (int c = wv * cpw + q; c < cols + 0; c += step * UNR) {
-            V av[UNR];
-            T xv[UNR];
+    // a wave takes chunks of cpw UNR consecutive columns, lane group q the UNR columns
+    // c0 + q UNR + u (one vector load of their x); the four waves take every fourth chunk
+    const int chunk = cpw * UNR;
+    const uint32_t roff = (uint32_t)((it.row0 + (act ? r0 : 0)) * sizeof(T));  // this lane's rows in a column
+    const __amdgpu_buffer_rsrc_t srs = buf_rsrc(stag, 0x7FFFFFFFu);
+    struct Iter {
+        V av[UNR];
+        T xv[UNR];
+        int cols, cb;
+        int64_t t;
+    };
+    int tt = it.t0, c0 = wv * chunk;  // next iteration: term, first column of the wave's chunk
+    auto settle = [&]() {
+        while (tt < it.t1 && c0 >= tms[tt].cols) {
+            ++tt;
+            c0 = wv * chunk;
+        }
+    };
+    // loads of the next iteration (past the last one: a term's resources with no valid lane)
+    auto load = [&](Iter &B) -> bool {
+        const bool has = tt < it.t1;
+        const GTerm tm = term_at(tms, has ? tt : it.t1 - 1);
+        B.cols = has ? tm.cols : 0;
+        B.cb = c0 + q * UNR;
+        B.t = tm.t;
+        const __amdgpu_buffer_rsrc_t ra = buf_rsrc(A + tm.a, (uint32_t)((int64_t)tm.cols * lda * sizeof(T)));
+        const __amdgpu_buffer_rsrc_t rx = buf_rsrc(vec + tm.x, (uint32_t)(B.cols * sizeof(T)));
+#pragma unroll
+        for (int u = 0; u < UNR; ++u) {
+            const int cc = B.cb + u;
+            const bool ok = act && cc < B.cols;
+            B.av[u] = buf_ld16_nt<V>(ra, ok ? (uint32_t)(cc * lda * sizeof(T)) + roff : kOutOfRange);
+        }
+        // (columns past the term's end within the group read neighbouring values or zeros:
+        // their A entries are zero and they are dropped below)
+        buf_ld_n<T, UNR>(B.xv, rx, act && B.cb < B.cols ? (uint32_t)(B.cb * sizeof(T)) : kOutOfRange);
+        if (has) {
+            c0 += 4 * chunk;
+            settle();
+        }
+        return has;
+    };
+    auto compute = [&](const Iter &B) {
+#pragma unroll
+        for (int u = 0; u < UNR; ++u) {
+            const T xu = B.cb + u < B.cols ? B.xv[u] : T(0);
+#pragma unroll
+            for (int v = 0; v < VEC; ++v) acc[v] = fma(B.av[u][v], xu, acc[v]);
+        }
+        if (ROLE == 1) {
+            // columns cb + u of the block dotted with this tile's rows of x, summed
+            // over the lpc lanes that hold them.  A butterfly first halves the UNR values per
+            // exchange (lane bit o keeps one half, sends the other), so that after log2 UNR
+            // exchanges lane l holds column ui(l) summed over its UNR-lane group; plain
+            // exchanges then finish the sums, and the first UNR lanes of every column group
+            // store them (non-symmetric terms, t < 0, store nothing).
+            T tp[UNR];
 #pragma unroll
             for (int u = 0; u < UNR; ++u) {
-                const int cc = c + u * step;
-                if (act && cc < cols) {
-                    av[u] = ld_nt((const V *)(a + (int64_t)cc * lda));
-                    xv[u] = x[cc];
-                } else {
-                    av[u] = (V)0;
-                    xv[u] = T(0);
-                }
+                tp[u] = T(0);
+#pragma unroll
+                for (int v = 0; v < VEC; ++v) tp[u] = fma(B.av[u][v], xb[v], tp[u]);
             }
+            if (lpc >= UNR) {
+                int ui = 0;
 #pragma unroll
-            for (int u = 0; u < UNR; ++u)
+                for (int k = UNR, o = 1; k > 1; k >>= 1, o <<= 1) {
+                    const bool hi = lane & o;
 #pragma unroll
-                for (int v = 0; v < VEC; ++v) acc[v] = fma(av[u][v], xv[u], acc[v]);
-            if (sym) {
-                // columns c + u step of the block dotted with this tile's rows of x, summed
-                // over the lpc lanes that hold them.  A butterfly first halves the UNR
-                // values per exchange (lane bit o keeps one half, sends the other), so that
-                // after log2 UNR exchanges lane l holds column ui(l) summed over its UNR-lane
-                // group; plain exchanges then finish the sums, and the first UNR lanes of
-                // every column group write them.
-                T tp[UNR];
+                    for (int i = 0; i < k / 2; ++i) {
+                        const T snd = hi ? tp[i] : tp[i + k / 2];
+                        const T keep = hi ? tp[i + k / 2] : tp[i];
+                        tp[i] = keep + xor_lane_n(snd, o);
+                    }
+                    ui += hi ? k / 2 : 0;
+                }
+#pragma unroll
+                for (int o = UNR; o < 64; o <<= 1)
+                    if (o < lpc) tp[0] += xor_lane_n(tp[0], o);
+                const int cc = B.cb + ui;
+                const bool st = (lane % lpc) < UNR && cc < B.cols && B.t >= 0;
+                buf_st<T>(tp[0], srs, st ? (uint32_t)((B.t + cc) * sizeof(T)) : kOutOfRange);
+            } else {
+                // columns of fewer than UNR lanes (short tiles): every lane sums all UNR values
+                // over its group, lane k of the group stores the columns u = k mod lpc
 #pragma unroll
                 for (int u = 0; u < UNR; ++u) {
-                    tp[u] = T(0);
 #pragma unroll
-                    for (int v = 0; v < VEC; ++v) tp[u] = fma(av[u][v], xb[v], tp[u]);
-                }
-                if (lpc >= UNR) {
-                    int ui = 0;
-#pragma unroll
-                    for (int k = UNR, o = 1; k > 1; k >>= 1, o <<= 1) {
-                        const bool hi = lane & o;
-#pragma unroll
-                        for (int i = 0; i < k / 2; ++i) {
-                            const T snd = hi ? tp[i] : tp[i + k / 2];
-                            const T keep = hi ? tp[i + k / 2] : tp[i];
-                            tp[i] = keep + xor_lane_n(snd, o);
-                        }
-                        ui += hi ? k / 2 : 0;
-                    }
-#pragma unroll
-                    for (int o = UNR; o < 64; o <<= 1)
-                        if (o < lpc) tp[0] += xor_lane_n(tp[0], o);
-                    const int cc = c + ui * step;
-                    if ((lane % lpc) < UNR && cc < cols) stag[tm.t + cc] = tp[0];
-                } else {
-#pragma unroll
-                    for (int u = 0; u < UNR; ++u) {
-#pragma unroll
-                        for (int o = 1; o < UNR; o <<= 1)
-                            if (o < lpc) tp[u] += xor_lane_n(tp[u], o);
-                        const int cc = c + u * step;
-                        if ((lane % lpc) == 0 && cc < cols) stag[tm.t + cc] = tp[u];
-                    }
+                    for (int o = 1; o < UNR; o <<= 1)
+                        if (o < lpc) tp[u] += xor_lane_n(tp[u], o);
+                    const int cc = B.cb + u;
+                    const bool st = (u % lpc) == (lane % lpc) && cc < B.cols && B.t >= 0;
+                    buf_st<T>(tp[u], srs, st ? (uint32_t)((B.t + cc) * sizeof(T)) : kOutOfRange);
                 }
             }
         }
+    };
+    Iter B0, B1;
+    settle();
+    bool h0 = load(B0), h1 = load(B1);
+    while (h0) {
+        compute(B0);
+        h0 = load(B0);
+        if (!h1) break;
+        compute(B1);
+        h1 = load(B1);
     }
 #pragma unroll
     for (int v = 0; v < VEC; ++v)
@@ -941,6 +1042,39 @@ __global__ __launch_bounds__(256) void k_bgemv(const GItem *__restrict__ items, 
         T *y = vec + it.y + t;
         if (it.flags & 1) atomicAdd(y, s);
         else *y += s;
+    }
+}
+
+// Gather of the staging slots: y[row0 + r] += sum over the item's terms of
+// stag[a + row0 + r] (one term per slot; a slot holds one block's transposed product,
+// all rows of the output box).  Threads r and r + 128 take alternate slots, four loads
+// in flight each.
+template <typename T>
+__global__ __launch_bounds__(256) void k_gather(const GItem *__restrict__ items, const GTerm *__restrict__ terms,
+                                                const T *__restrict__ stag, T *vec) {
+    const GItem it = items[blockIdx.x];
+    const CGTerm *const tms = (const CGTerm *)terms;
+    __shared__ T part[2][kRowTile];
+    const int r = threadIdx.x & (kRowTile - 1), h = threadIdx.x / kRowTile;
+    T s0 = T(0), s1 = T(0);
+    if (r < it.rows) {
+        const T *src = stag + it.row0 + r;
+        int tt = it.t0 + h;
+        for (; tt + 6 < it.t1; tt += 8) {
+            const T a = src[tms[tt].a], b = src[tms[tt + 2].a], c = src[tms[tt + 4].a], d = src[tms[tt + 6].a];
+            s0 += a + b;
+            s1 += c + d;
+        }
+        for (; tt < it.t1; tt += 2) s0 += src[tms[tt].a];
+    }
+    part[h][r] = s0 + s1;
+    __syncthreads();
+    if (threadIdx.x < it.rows) {
+        const int t = threadIdx.x;
+        const T v = part[0][t] + part[1][t];
+        T *y = vec + it.y + t;
+        if (it.flags & 1) atomicAdd(y, v);
+        else *y += v;
     }
 }
 
@@ -1709,7 +1843,6 @@ static void launch_bgemv_t(const GItem *items, int nitems, const GTerm *terms, c
     if (!A1) A1 = A0;
     if (role == 1) launch_bgemv_r<T, 1>(items, nitems, terms, A0, A1, stag, vec, unr, s);
     else if (role == 2) launch_bgemv_r<T, 2>(items, nitems, terms, A0, A1, stag, vec, unr, s);
-    else if (role == 3) launch_bgemv_r<T, 3>(items, nitems, terms, A0, A1, stag, vec, unr, s);
     else launch_bgemv_r<T, 0>(items, nitems, terms, A0, A1, stag, vec, unr, s);
 }
 void launch_copy32(const CopyJob *jobs, int njobs, const float *src, float *dst, hipStream_t s) {
@@ -1922,5 +2055,13 @@ void launch_bgemm_t32(const GItem *items, int nitems, const GTerm *terms, const 
     if (nitems > 0)
         hipLaunchKernelGGL(k_bgemm_tl<float>, dim3(nitems), dim3(256), 0, s, items, terms, A0, A1 ? A1 : A0, tab,
                            vec);
+}
+void launch_gather64(const GItem *items, int nitems, const GTerm *terms, const double *stag, double *vec,
+                     hipStream_t s) {
+    if (nitems > 0) hipLaunchKernelGGL(k_gather<double>, dim3(nitems), dim3(256), 0, s, items, terms, stag, vec);
+}
+void launch_gather32(const GItem *items, int nitems, const GTerm *terms, const float *stag, float *vec,
+                     hipStream_t s) {
+    if (nitems > 0) hipLaunchKernelGGL(k_gather<float>, dim3(nitems), dim3(256), 0, s, items, terms, stag, vec);
 }
 }  // namespace h2w
