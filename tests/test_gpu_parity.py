@@ -38,6 +38,9 @@ CASES = [
     (3, 4000, 32, 8, 1, 1, "gauss"),
     (2, 48 * 48, 36, 10, 1, 2, "grid"),
     (3, 16 ** 3, 27, 7, 0, 2, "grid"),
+    # leaves of 256 points: near-field blocks and output boxes of two 128-row tiles
+    (2, 64 * 64, 256, 10, 1, 0, "grid"),
+    (2, 64 * 64, 256, 11, 0, 1, "rand"),
 ]
 
 
