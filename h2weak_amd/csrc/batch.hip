@@ -117,6 +117,9 @@ void merge_lists(Batch &B, int which) {
         }
         Lm.nitems = (int64_t)items.size() - Lm.item0;
         Lm.nterms = (int64_t)terms.size() - Lm.term0;
+        int64_t sc = 0;
+        for (int64_t t = Lm.term0; t < (int64_t)terms.size(); ++t) sc += terms[t].cols;
+        Lm.avg_cols = Lm.nterms ? (int)(sc / Lm.nterms) : 0;
         if (Lm.nitems == 0) continue;
         M.launches.push_back(Lm);
         M.elems.push_back(e);
@@ -142,13 +145,13 @@ void run_batch_launch(Batch &B, const Launch &L, int precision) {
     if (is32(precision)) {
         if (L.role == 3)
             launch_bgemm_t32(M.d_items.p + L.item0, (int)L.nitems, M.d_terms.p, nullptr, nullptr, B.tab32.p,
-                             B.vec32m.p, s);
+                             B.vec32m.p, trans_narrow(L, true), s);
         else
             launch_bgemm32_tab(M.d_items.p + L.item0, (int)L.nitems, M.d_terms.p, B.tab32.p, B.vec32m.p, L.role, s);
     } else {
         if (L.role == 3)
             launch_bgemm_t64(M.d_items.p + L.item0, (int)L.nitems, M.d_terms.p, nullptr, nullptr, B.tab64.p,
-                             B.vec64m.p, s);
+                             B.vec64m.p, trans_narrow(L, false), s);
         else
             launch_bgemm64_tab(M.d_items.p + L.item0, (int)L.nitems, M.d_terms.p, B.tab64.p, B.vec64m.p, L.role, s);
     }

@@ -107,7 +107,10 @@ struct Launch {
     int stage = 0, step = 0;
     int unr = 4;   // columns in flight per wave (autotuned), FP64 / inner FP32 lists
     int unr32 = 4; // the same for the stored FP32 copy
+    int avg_cols = 0;  // mean inner size of the terms (picks the transposed 16-wide kernel's slabs)
 };
+// narrow slabs for the transposed 16-wide products when the blocks have few rows
+inline int trans_narrow(const Launch &L, bool fp32) { return L.avg_cols <= (fp32 ? 32 : 16); }
 
 struct WorkList {
     std::vector<GItem> items;

@@ -209,10 +209,12 @@ void launch_gather64(const GItem *items, int nitems, const GTerm *terms, const d
 void launch_gather32(const GItem *items, int nitems, const GTerm *terms, const float *stag, float *vec,
                      hipStream_t s);
 // kNR right-hand sides, transposed products of the stored symmetric blocks (role 3 lists)
+// (narrow: slabs of 4 VEC inner indices, 16 columns per load, four slabs per step -- for
+// blocks of few rows; else slabs of 16 VEC inner indices, 4 columns per load)
 void launch_bgemm_t64(const GItem *items, int nitems, const GTerm *terms, const double *A0, const double *A1,
-                      const double *const *tab, double *vec, hipStream_t s);
+                      const double *const *tab, double *vec, int narrow, hipStream_t s);
 void launch_bgemm_t32(const GItem *items, int nitems, const GTerm *terms, const float *A0, const float *A1,
-                      const float *const *tab, float *vec, hipStream_t s);
+                      const float *const *tab, float *vec, int narrow, hipStream_t s);
 // kNR right-hand sides: vector pools and work vectors hold element (slot, r) at slot * kNR + r
 void launch_bgemm64(const GItem *items, int nitems, const GTerm *terms, const double *A0, const double *A1,
                     double *vec, int role, hipStream_t s);

@@ -845,7 +845,9 @@ void upload_lists(std::vector<WorkList> &WL, LaunchSet &LS, hipStream_t s) {
         Lh.role = W.role;
         Lh.stage = W.stage;
         Lh.step = W.step;
-        int64_t e = 0;
+        int64_t e = 0, sc = 0;
+        for (auto &tm : W.terms) sc += tm.cols;
+        Lh.avg_cols = W.terms.empty() ? 0 : (int)(sc / (int64_t)W.terms.size());
         for (auto it : W.items) {
             for (int q = it.t0; q < it.t1; ++q) e += (int64_t)it.rows * W.terms[q].cols;
             it.t0 += (int32_t)Lh.term0;

@@ -1250,23 +1250,29 @@ __global__ __launch_bounds__(256) void k_bgemm(const GItem *__restrict__ items, 
 // while a slab is multiplied: the slab's B operands are loaded before them, since
 // vector loads retire in order.  No wave reads another wave's region, and every wave
 // writes only its own output rows.
-template <typename T>
+template <typename T, int LPC>
 __global__ __launch_bounds__(256) void k_bgemm_tl(const GItem *__restrict__ items, const GTerm *__restrict__ terms,
                                                   const T *__restrict__ A0, const T *__restrict__ A1,
                                                   const T *const *__restrict__ tab, T *vec) {
     typedef typename Mfma16<T>::v4 V4;
     constexpr int VEC = 16 / sizeof(T);
-    constexpr int SL = 16 * VEC;       // inner indices of one slab
+    constexpr int SL = LPC * VEC;      // inner indices of one slab (LPC lanes per column)
     constexpr int LD = SL + VEC;       // LDS row: one column's slab, padded by 16 bytes
+    constexpr int CPL = 64 / LPC;      // columns per load instruction
+    constexpr int NJ = 16 / CPL;       // load instructions per 16-column group
+    constexpr int NSUB = LPC / 4;      // MFMA k-steps (4 VEC inner indices) per slab
+    constexpr int U = 16 / LPC;        // slabs per iteration
     typedef T VL __attribute__((ext_vector_type(VEC)));
-    __shared__ __attribute__((aligned(16))) T lds[4][32][LD];
+    __shared__ __attribute__((aligned(16))) T lds[4][U][32][LD];
     const GItem it = items[blockIdx.x];
+    const CGTerm *const tms = (const CGTerm *)terms;
     const T *__restrict__ A = tab ? tab[2 * (it.flags >> 8) + ((it.flags >> 1) & 1)] : ((it.flags & 2) ? A1 : A0);
     const int lane = threadIdx.x & 63;
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int rows = it.rows;
     const int ng = (rows + 15) >> 4;
-    const int m = lane & 15, q = lane >> 4;
+    const int m = lane & 15, q = lane >> 4;       // MFMA operand lane
+    const int il = lane % LPC, cl = lane / LPC;   // load lane: inner chunk, column
     // Items of one or two groups split their slabs over the waves instead: wave w takes
     // group w % ng and every ns-th slab from the (w / ng)-th, and the partial products are
     // summed through LDS at the end.
@@ -1274,44 +1280,44 @@ __global__ __launch_bounds__(256) void k_bgemm_tl(const GItem *__restrict__ item
     const int g0 = ns > 1 ? wv % ng : wv;  // first group of this wave (the second is wv + 4)
     const int nown = ns > 1 ? 1 : (wv < ng) + (wv + 4 < ng);
     if (nown == 0) return;  // (ns == 1 only: no barrier follows)
-    T(*reg)[LD] = lds[wv];
     V4 acc[2] = {(V4)0, (V4)0};
     int lt = it.t0, lk = ns > 1 ? wv / ng : 0;  // next slab to load (term, slab of the term)
     auto settle = [&]() {
         while (lt < it.t1) {
-            const int nk = (terms[lt].cols + SL - 1) / SL;
+            const int nk = (tms[lt].cols + SL - 1) / SL;
             if (lk < nk) break;
             lk -= nk;
             ++lt;
         }
     };
-    int64_t sx = 0;          // loaded slab: input offset, inner size, first inner index
-    int scols = 0, si0 = 0;
-    int okm = 0, vlim = 0;  // loaded slab: lanes' valid loads, valid inner indices from this lane's first
-    VL L[2][4];
-    // load the next slab into L (16 lanes per column, this lane's VEC inner indices)
-    // (past the last slab it still loads, from the last term, and the loads are dropped)
-    auto next = [&]() -> bool {
+    struct Slab {
+        int64_t x;         // input offset
+        int cols, i0;      // inner size, first inner index
+        int okm, vlim;     // lanes' valid loads, valid inner indices from this lane's first
+    };
+    Slab S[U];
+    VL L[U][2][NJ];
+    // load the next slab into L[u] (past the last slab it still loads, from the last term,
+    // and the loads are dropped: loads under a branch, or values used at once, would make
+    // the next wait drain all of them)
+    auto next = [&](int u) -> bool {
         settle();
         const bool has = lt < it.t1;
-        const GTerm tm = terms[has ? lt : it.t1 - 1];
-        sx = tm.x;
-        scols = tm.cols;
-        si0 = has ? lk * SL : tm.cols;
-        const int io = si0 + VEC * m;
-        // unconditional loads (lanes out of range read the term's first element, dropped when
-        // staged): loads under a branch, or values used at once, would make the next wait
-        // drain all of them
-        okm = 0;
-        vlim = tm.cols - io;
+        const GTerm tm = term_at(tms, has ? lt : it.t1 - 1);
+        S[u].x = tm.x;
+        S[u].cols = tm.cols;
+        S[u].i0 = has ? lk * SL : tm.cols;
+        const int io = S[u].i0 + VEC * il;
+        S[u].okm = 0;
+        S[u].vlim = tm.cols - io;
 #pragma unroll
         for (int gi = 0; gi < 2; ++gi)
 #pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                const int c = 16 * (gi ? wv + 4 : g0) + q + 4 * j;  // item row = column of the block
+            for (int j = 0; j < NJ; ++j) {
+                const int c = 16 * (gi ? wv + 4 : g0) + cl + CPL * j;  // item row = column of the block
                 const bool ok = gi < nown && c < rows && io < tm.cols;
-                okm |= ok ? 1 << (4 * gi + j) : 0;
-                L[gi][j] = ld_nt((const VL *)(A + tm.a + (ok ? (int64_t)(it.row0 + c) * tm.ld + io : 0)));
+                S[u].okm |= ok ? 1 << (NJ * gi + j) : 0;
+                L[u][gi][j] = ld_nt((const VL *)(A + tm.a + (ok ? (int64_t)(it.row0 + c) * tm.ld + io : 0)));
             }
         lk += has ? ns : 0;
         return has;
@@ -1321,48 +1327,61 @@ __global__ __launch_bounds__(256) void k_bgemm_tl(const GItem *__restrict__ item
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     };
-    bool more = next();
+    bool more = false;
+#pragma unroll
+    for (int u = 0; u < U; ++u) more |= next(u);
     while (more) {
         // staged with every entry outside the block set to zero, so that the B operands of
         // inner indices beyond the block (clamped loads of a valid x) multiply zeros
+        T bx[U][NSUB][VEC];
+        Slab C[U];
 #pragma unroll
-        for (int gi = 0; gi < 2; ++gi)
-#pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                VL v = L[gi][j];
-#pragma unroll
-                for (int e = 0; e < VEC; ++e) v[e] = ((okm >> (4 * gi + j)) & 1) && e < vlim ? v[e] : T(0);
-                *(VL *)&reg[16 * gi + q + 4 * j][VEC * m] = v;
-            }
-        T bx[4][VEC];
-#pragma unroll
-        for (int s = 0; s < 4; ++s) {
-            const int i = si0 + 4 * VEC * s + VEC * q;
-#pragma unroll
-            for (int e = 0; e < VEC; ++e) bx[s][e] = vec[(sx + min(i + e, scols - 1)) * kNR + m];
-        }
-        more = next();
-        wave_sync();
-#pragma unroll
-        for (int s = 0; s < 4; ++s)
+        for (int u = 0; u < U; ++u) {
 #pragma unroll
             for (int gi = 0; gi < 2; ++gi)
-                if (gi < nown) {
-                    const VL a = *(const VL *)&reg[16 * gi + m][4 * VEC * s + VEC * q];
 #pragma unroll
-                    for (int e = 0; e < VEC; ++e) acc[gi] = Mfma16<T>::op(a[e], bx[s][e], acc[gi]);
+                for (int j = 0; j < NJ; ++j) {
+                    VL v = L[u][gi][j];
+#pragma unroll
+                    for (int e = 0; e < VEC; ++e)
+                        v[e] = ((S[u].okm >> (NJ * gi + j)) & 1) && e < S[u].vlim ? v[e] : T(0);
+                    *(VL *)&lds[wv][u][16 * gi + cl + CPL * j][VEC * il] = v;
                 }
+#pragma unroll
+            for (int s = 0; s < NSUB; ++s) {
+                const int i = S[u].i0 + 4 * VEC * s + VEC * q;
+#pragma unroll
+                for (int e = 0; e < VEC; ++e) bx[u][s][e] = vec[(S[u].x + min(i + e, S[u].cols - 1)) * kNR + m];
+            }
+            C[u] = S[u];
+        }
+        more = false;
+#pragma unroll
+        for (int u = 0; u < U; ++u) more |= next(u);
+        wave_sync();
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if (C[u].i0 < C[u].cols)  // (a slab past the last one multiplies nothing)
+#pragma unroll
+                for (int s = 0; s < NSUB; ++s)
+#pragma unroll
+                    for (int gi = 0; gi < 2; ++gi)
+                        if (gi < nown) {
+                            const VL a = *(const VL *)&lds[wv][u][16 * gi + m][4 * VEC * s + VEC * q];
+#pragma unroll
+                            for (int e = 0; e < VEC; ++e) acc[gi] = Mfma16<T>::op(a[e], bx[u][s][e], acc[gi]);
+                        }
         wave_sync();
     }
     if (ns > 1) {
         // partial products of group g0 from waves g0, g0 + ng, ..: summed by wave g0
-        T(*red)[kNR] = (T(*)[kNR])&lds[wv][0][0];
+        T(*red)[kNR] = (T(*)[kNR])&lds[wv][0][0][0];
 #pragma unroll
         for (int v = 0; v < 4; ++v) red[Mfma16<T>::row(q, v)][m] = acc[0][v];
         __syncthreads();
         if (wv >= ng) return;
         for (int o = wv + ng; o < 4; o += ng) {
-            T(*ro)[kNR] = (T(*)[kNR])&lds[o][0][0];
+            T(*ro)[kNR] = (T(*)[kNR])&lds[o][0][0][0];
 #pragma unroll
             for (int v = 0; v < 4; ++v) acc[0][v] += ro[Mfma16<T>::row(q, v)][m];
         }
@@ -2044,17 +2063,22 @@ void launch_wide_finish(const WideState *state, int nb, int32_t *rank_out, int32
                         hipStream_t s) {
     hipLaunchKernelGGL(k_wide_finish, dim3(1), dim3(64), 0, s, state, nb, rank_out, status_out, ndone);
 }
+template <typename T>
+static void launch_bgemm_tl(const GItem *items, int nitems, const GTerm *terms, const T *A0, const T *A1,
+                            const T *const *tab, T *vec, int narrow, hipStream_t s) {
+    if (nitems <= 0) return;
+    if (narrow)
+        hipLaunchKernelGGL((k_bgemm_tl<T, 4>), dim3(nitems), dim3(256), 0, s, items, terms, A0, A1 ? A1 : A0, tab, vec);
+    else
+        hipLaunchKernelGGL((k_bgemm_tl<T, 16>), dim3(nitems), dim3(256), 0, s, items, terms, A0, A1 ? A1 : A0, tab, vec);
+}
 void launch_bgemm_t64(const GItem *items, int nitems, const GTerm *terms, const double *A0, const double *A1,
-                      const double *const *tab, double *vec, hipStream_t s) {
-    if (nitems > 0)
-        hipLaunchKernelGGL(k_bgemm_tl<double>, dim3(nitems), dim3(256), 0, s, items, terms, A0, A1 ? A1 : A0, tab,
-                           vec);
+                      const double *const *tab, double *vec, int narrow, hipStream_t s) {
+    launch_bgemm_tl<double>(items, nitems, terms, A0, A1, tab, vec, narrow, s);
 }
 void launch_bgemm_t32(const GItem *items, int nitems, const GTerm *terms, const float *A0, const float *A1,
-                      const float *const *tab, float *vec, hipStream_t s) {
-    if (nitems > 0)
-        hipLaunchKernelGGL(k_bgemm_tl<float>, dim3(nitems), dim3(256), 0, s, items, terms, A0, A1 ? A1 : A0, tab,
-                           vec);
+                      const float *const *tab, float *vec, int narrow, hipStream_t s) {
+    launch_bgemm_tl<float>(items, nitems, terms, A0, A1, tab, vec, narrow, s);
 }
 void launch_gather64(const GItem *items, int nitems, const GTerm *terms, const double *stag, double *vec,
                      hipStream_t s) {

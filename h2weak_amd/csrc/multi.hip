@@ -36,13 +36,13 @@ void run_launch_m(Handle &h, const Launch &Lh, int precision, bool trans) {
         const LaunchSet &T = precision == H2W_FP32_INNER ? h.inner_trans : h.trans;
         if (precision == H2W_FP32_INNER)
             launch_bgemm_t32(T.d_items.p + Lh.item0, (int)Lh.nitems, T.d_terms.p, h.mat32.p, h.mat32in.p, nullptr,
-                             h.vec32m.p, s);
+                             h.vec32m.p, trans_narrow(Lh, true), s);
         else if (precision == H2W_FP32)
             launch_bgemm_t32(T.d_items.p + Lh.item0, (int)Lh.nitems, T.d_terms.p, h.mat32.p, nullptr, nullptr,
-                             h.vec32m.p, s);
+                             h.vec32m.p, trans_narrow(Lh, true), s);
         else
             launch_bgemm_t64(T.d_items.p + Lh.item0, (int)Lh.nitems, T.d_terms.p, h.mat64.p, nullptr, nullptr,
-                             h.vec64m.p, s);
+                             h.vec64m.p, trans_narrow(Lh, false), s);
     } else if (precision == H2W_FP32_INNER) {
         launch_bgemm32(h.inner.d_items.p + Lh.item0, (int)Lh.nitems, h.inner.d_terms.p, h.mat32.p, h.mat32in.p,
                        h.vec32m.p, Lh.role, s);
