@@ -387,8 +387,8 @@ def main():
         pmc_reduce(args.pmc_reduce, args.side, args.tol_pow)
         return
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); the sharded "
-              "design is described in DESIGN.md and not launched", file=sys.stderr)
+        print(f"bench.py: runs on one GPU only (asked for --gpus {args.gpus}); sharding the operator by "
+              "subtree over several GPUs is not implemented", file=sys.stderr)
         sys.exit(2)
     import h2weak_amd as h2w
 

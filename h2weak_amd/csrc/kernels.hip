@@ -789,11 +789,12 @@ __global__ void k_cur(const CurJob *__restrict__ jobs, const int32_t *__restrict
 // cross-lane adds, then across the waves in LDS, and added to y exclusively
 // or atomically.  Blocks are column-major with leading dimensions padded to a
 // multiple of 4, so every VEC group is aligned and stays inside the column.
-// ROLE only names the instance (0: transfer passes, 1: M2L + near field,
-// 2: semi-nested column factors) so that profiles separate the launches; UNR
-// is the number of column loads each lane keeps in flight.  Bit 1 of an item's
-// flags selects the operator pool A1 (the inner operator of the refinement)
-// instead of A0.
+// ROLE 1 (the fused M2L + near-field launch) also forms the transposed products
+// of the symmetric terms; 0 (transfer passes) and 2 (semi-nested column factors)
+// only name the instance so that profiles separate the launches.  UNR is the
+// number of columns each lane loads per iteration.  Bit 1 of an item's flags
+// selects the operator pool A1 (the inner operator of the refinement) instead
+// of A0.
 // Streaming load through a global-address-space pointer: operator pointers taken from a
 // table in memory are generic to the compiler, whose flat loads would also count against
 // the LDS wait counter.
