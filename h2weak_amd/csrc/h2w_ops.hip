@@ -737,6 +737,59 @@ void add_item_whole(WorkList &W, int64_t y, int64_t xb, int rows, const std::vec
     }
 }
 
+// A launch of few large items (the transfer passes near the root) leaves most of the
+// GPU idle while a handful of workgroups stream their items: split such items by
+// columns (the pieces accumulate atomically) until the launch has about kMinItems
+// items of at least kMinPiece elements.  Lists with staging terms are left alone.
+constexpr int64_t kMinItems = 2048, kMinPiece = 128 * 64;
+void rebalance(WorkList &W) {
+    if ((int64_t)W.items.size() >= kMinItems) return;
+    int64_t E = 0;
+    for (auto &it : W.items)
+        for (int32_t k = it.t0; k < it.t1; ++k) {
+            if (W.terms[k].t >= 0) return;
+            E += (int64_t)it.rows * W.terms[k].cols;
+        }
+    const int64_t target = std::max<int64_t>(E / kMinItems, kMinPiece);
+    WorkList out;
+    out.role = W.role;
+    out.stage = W.stage;
+    out.step = W.step;
+    for (const GItem &it : W.items) {
+        int64_t work = 0;
+        for (int32_t k = it.t0; k < it.t1; ++k) work += (int64_t)it.rows * W.terms[k].cols;
+        GItem nt = it;
+        nt.t0 = (int32_t)out.terms.size();
+        if (work <= 2 * target) {
+            for (int32_t k = it.t0; k < it.t1; ++k) out.terms.push_back(W.terms[k]);
+            nt.t1 = (int32_t)out.terms.size();
+            out.items.push_back(nt);
+            continue;
+        }
+        const int64_t maxc = std::max<int64_t>(1, target / it.rows);
+        nt.flags |= 1;
+        int64_t used = 0;
+        for (int32_t k = it.t0; k < it.t1; ++k) {
+            const GTerm &t = W.terms[k];
+            for (int64_t c0 = 0; c0 < t.cols;) {
+                const int64_t take = std::min<int64_t>(t.cols - c0, maxc - used);
+                out.terms.push_back(GTerm{t.a + c0 * it.lda, t.x + c0, -1, (int32_t)take, t.ld});
+                used += take;
+                c0 += take;
+                if (used == maxc) {
+                    nt.t1 = (int32_t)out.terms.size();
+                    out.items.push_back(nt);
+                    nt.t0 = nt.t1;
+                    used = 0;
+                }
+            }
+        }
+        nt.t1 = (int32_t)out.terms.size();
+        if (nt.t1 > nt.t0) out.items.push_back(nt);
+    }
+    W = std::move(out);
+}
+
 // One group of boxes whose pairwise blocks are stored once per unordered pair (the
 // M2L blocks of one family and level, or the near field): box b has output offset
 // y[b], input offset x[b] and size r[b]; up[b] lists the stored blocks K(b, p), p > b
@@ -1178,6 +1231,8 @@ void build_inner(Handle &h, double irp) {
                              (int)pad_ld(P.bcount[j][b]));
             IL.push_back(std::move(W));
         }
+        for (auto &W : IL)
+            if (W.role == 0) rebalance(W);
         upload_lists(IL, h.inner, s);
         h.inner_elems = 0;  // operator elements (the gather reads staging slots)
         for (size_t q = 0; q < h.inner.launches.size(); ++q)
@@ -1655,6 +1710,8 @@ void assemble_operators(Handle &h, double t0) {
     }
     std::vector<GItem> items;
     std::vector<GTerm> terms;
+    for (auto &W : WL)
+        if (W.role == 0) rebalance(W);
     h.launches.clear();
     for (auto &W : WL) {
         if (W.items.empty()) continue;
