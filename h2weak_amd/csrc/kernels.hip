@@ -843,6 +843,13 @@ constexpr uint32_t kOutOfRange = 0xFFFFFFF0u;
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const void *base, uint32_t bytes) {
     return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), (short)0, (int)bytes, kBufWord3);
 }
+// An offset the compiler cannot see through: keeps a load whose offset is selected per
+// lane a single unconditional load (instead of one load per alternative under branches).
+// The asm is empty and loads nothing.
+__device__ __forceinline__ uint32_t opaque_off(uint32_t v) {
+    asm volatile("" : "+v"(v));
+    return v;
+}
 template <typename T>
 __device__ __forceinline__ T buf_ld(__amdgpu_buffer_rsrc_t r, uint32_t off) {
     if constexpr (sizeof(T) == 8) return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
@@ -1117,7 +1124,11 @@ struct Mfma16<double> {
 };
 
 // One item with VEC rows per lane load, NG load groups (rows <= 16 VEC NG) and UNR
-// 4-column chunks in flight per wave.  red: the workgroup's LDS reduction buffer.
+// 4-column chunks per iteration, two iterations in flight (the next one's loads are
+// issued before the current one is multiplied).  Loads are raw buffer loads whose
+// out-of-range lanes read zeros, so no branch guards them and the waits before the
+// MFMAs count only the current iteration's loads.  red: the workgroup's LDS
+// reduction buffer.
 template <typename T, int VEC, int NG, int UNR>
 __device__ __forceinline__ void bgemm_item(const GItem &it, const GTerm *__restrict__ terms, const T *__restrict__ A,
                                            T *vec, T (*red)[kRowTile][kNR]) {
@@ -1125,6 +1136,7 @@ __device__ __forceinline__ void bgemm_item(const GItem &it, const GTerm *__restr
     typedef T VL __attribute__((ext_vector_type(VEC)));
     constexpr int RG = 16 * VEC;  // rows of one load group
     constexpr int NA = NG * VEC;  // accumulators
+    const CGTerm *const tms = (const CGTerm *)terms;
     const int lane = threadIdx.x & 63;
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int rows = it.rows;
@@ -1135,50 +1147,70 @@ __device__ __forceinline__ void bgemm_item(const GItem &it, const GTerm *__restr
     for (int a = 0; a < NA; ++a) acc[a] = (V4)0;
     const int64_t lda = it.lda;
     // The item's columns, term after term, in 4-column chunks; wave wv takes chunks
-    // wv, wv + 4, ... through a cursor (term tt, chunk ck of that term), so UNR chunks
-    // are in flight whatever the terms' widths.
+    // wv, wv + 4, ... through a cursor (term tt, chunk ck of that term).
     int tt = it.t0, ck = wv;
     auto settle = [&]() {
         while (tt < it.t1) {
-            const int nk = (terms[tt].cols + 3) >> 2;
+            const int nk = (tms[tt].cols + 3) >> 2;
             if (ck < nk) break;
             ck -= nk;
             ++tt;
         }
     };
-    settle();
-    while (tt < it.t1) {
+    struct Iter {
         VL av[UNR][NG];
         T bx[UNR];
-        bool ok[UNR];
+        bool has;
+    };
+    // this lane's rows of every load group (out of range: past the end of any resource)
+    uint32_t roff[NG];
+#pragma unroll
+    for (int g = 0; g < NG; ++g)
+        roff[g] = (g < ng && RG * g + VEC * m < rows) ? (uint32_t)((it.row0 + VEC * m + RG * g) * sizeof(T)) : kOutOfRange;
+    auto load = [&](Iter &B) -> bool {
+        B.has = tt < it.t1;
 #pragma unroll
         for (int u = 0; u < UNR; ++u) {
-            ok[u] = tt < it.t1;
-            bx[u] = T(0);
+            const bool has = tt < it.t1;
+            const GTerm tm = term_at(tms, has ? tt : it.t1 - 1);
+            const int c = 4 * ck + q;
+            const bool ok = has && c < tm.cols;
+            const __amdgpu_buffer_rsrc_t ra = buf_rsrc(A + tm.a, (uint32_t)((int64_t)tm.cols * lda * sizeof(T)));
+            const __amdgpu_buffer_rsrc_t rx = buf_rsrc(vec + tm.x * kNR, (uint32_t)((int64_t)tm.cols * kNR * sizeof(T)));
+            B.bx[u] = buf_ld<T>(rx, opaque_off(ok ? (uint32_t)((c * kNR + m) * sizeof(T)) : kOutOfRange));
+            const uint32_t coff = ok ? (uint32_t)(c * lda * sizeof(T)) : kOutOfRange;
 #pragma unroll
-            for (int g = 0; g < NG; ++g) av[u][g] = (VL)0;
-            if (ok[u]) {
-                const GTerm tm = terms[tt];
-                const int c = 4 * ck + q;
-                if (c < tm.cols) {
-                    bx[u] = vec[(tm.x + c) * kNR + m];
-                    const T *ac = A + tm.a + it.row0 + VEC * m + (int64_t)c * lda;
-#pragma unroll
-                    for (int g = 0; g < NG; ++g)
-                        if (g < ng && RG * g + VEC * m < rows) av[u][g] = ld_nt((const VL *)(ac + RG * g));
-                }
+            for (int g = 0; g < NG; ++g)
+                B.av[u][g] =
+                    buf_ld16_nt<VL>(ra, opaque_off(ok && roff[g] != kOutOfRange ? coff + roff[g] : kOutOfRange));
+            if (has) {
                 ck += 4;
                 settle();
             }
         }
+        return B.has;
+    };
+    auto compute = [&](const Iter &B) {
 #pragma unroll
         for (int u = 0; u < UNR; ++u)
-            if (ok[u])
 #pragma unroll
-                for (int g = 0; g < NG; ++g)
-                    if (g < ng)
+            for (int g = 0; g < NG; ++g)
+                if (g < ng)
 #pragma unroll
-                        for (int e = 0; e < VEC; ++e) acc[g * VEC + e] = Mfma16<T>::op(av[u][g][e], bx[u], acc[g * VEC + e]);
+                    for (int e = 0; e < VEC; ++e)
+                        acc[g * VEC + e] = Mfma16<T>::op(B.av[u][g][e], B.bx[u], acc[g * VEC + e]);
+    };
+    settle();
+    // (a buffer loaded past the last chunk holds zeros: multiplying it adds nothing, so
+    // both halves run unconditionally and the loop has a single path for the waits)
+    Iter B0, B1;
+    bool h0 = load(B0);
+    load(B1);
+    while (h0) {
+        compute(B0);
+        h0 = load(B0);
+        compute(B1);
+        load(B1);
     }
     // item row held by accumulator (g, e), result element v of this lane
     auto row_of = [&](int g, int e, int v) { return RG * g + VEC * Mfma16<T>::row(q, v) + e; };
@@ -1235,7 +1267,7 @@ __global__ __launch_bounds__(256) void k_bgemm(const GItem *__restrict__ items, 
     // batched systems: pool pair of system flags >> 8 from the table
     const T *__restrict__ A = tab ? tab[2 * (it.flags >> 8) + ((it.flags >> 1) & 1)] : ((it.flags & 2) ? A1 : A0);
     constexpr int VEC = 16 / sizeof(T);
-    bgemm_item<T, VEC, kRowTile / (16 * VEC), 4>(it, terms, A, vec, red);
+    bgemm_item<T, VEC, kRowTile / (16 * VEC), 2>(it, terms, A, vec, red);
 }
 
 // Transposed products of the symmetric blocks on the 16-wide path:
