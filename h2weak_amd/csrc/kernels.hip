@@ -1026,15 +1026,17 @@ __global__ __launch_bounds__(256) void k_bgemv(const GItem *__restrict__ items, 
             }
         }
     };
+    // (an iteration loaded past the last one reads nothing and stores nothing: both halves
+    // run unconditionally, so the loop has a single path for the waits)
     Iter B0, B1;
     settle();
-    bool h0 = load(B0), h1 = load(B1);
+    bool h0 = load(B0);
+    load(B1);
     while (h0) {
         compute(B0);
         h0 = load(B0);
-        if (!h1) break;
         compute(B1);
-        h1 = load(B1);
+        load(B1);
     }
 #pragma unroll
     for (int v = 0; v < VEC; ++v)
