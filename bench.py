@@ -475,7 +475,7 @@ def main():
                              (h2w.FP32_OTF, "fp32_otf", 4), (h2w.FP64_OTF, "fp64_otf", 8)):
         r = A.bench_matvec(10, prec)
         t = r["ms_per_matvec"] * 1e-3
-        # streamed: every launch's operator (or staging) bytes; effective: the reference's
+        # streamed: every launch's operator bytes; effective: the reference's
         # storage count M (both directions of every pair) over the same time
         streamed = sum(r["launch_bytes"]) + 2 * 8 * N
         mv[name] = {"ms": r["ms_per_matvec"],

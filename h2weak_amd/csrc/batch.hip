@@ -72,12 +72,9 @@ void merge_lists(Batch &B, int which) {
         const std::vector<Launch> &Ls = which == 0 ? h.launches : h.inner.launches;
         const LaunchSet &tr = which == 0 ? h.trans : h.inner_trans;
         for (auto &L : Ls) {
-            if (L.role == 3) {
-                // the gather of the single-vector path: the transposed products instead
-                for (auto &Lt : tr.launches) groups[key_of(L)].push_back(Src{s, &Lt, &tr, 3});
-                continue;
-            }
             groups[key_of(L)].push_back(Src{s, &L, which == 0 ? nullptr : &h.inner, L.role});
+            if (L.role == 1)  // the transposed products of the symmetric blocks, after the fused launch
+                for (auto &Lt : tr.launches) groups[key_of(Lt)].push_back(Src{s, &Lt, &tr, 3});
         }
     }
     LaunchSet &M = B.lists[which];

@@ -132,7 +132,7 @@ struct LaunchSet {
 struct TermSpec {
     int64_t a, x;
     int32_t cols;
-    int64_t t = -1;  // symmetric term: staging offset of its transposed product
+    int64_t t = -1;  // symmetric term: output offset of its transposed product (the partner box's y)
     int32_t ld = 0;  // transposed term: leading dimension of the stored block
 };
 
@@ -219,9 +219,6 @@ struct h2w_handle {
     h2w::LaunchSet inner;               // the refinement's inner operator (ranks truncated)
     h2w::LaunchSet trans, inner_trans;  // 16-wide path: K(b, p)^T x_b of the stored blocks (role 3)
     h2w::LaunchSet otf;                 // on-the-fly path: the M2L + near-field items, every direction
-    int64_t stag_len = 0, inner_stag_len = 0;  // staging elements of the symmetric blocks' transposed products
-    h2w::DevBuf<double> stag64;
-    h2w::DevBuf<float> stag32;
     hipGraphExec_t graph[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};  // one per precision
     hipGraphExec_t graph_m[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};  // the same, kNR right-hand sides
     bool use_graph = true;

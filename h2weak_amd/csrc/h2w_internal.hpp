@@ -122,7 +122,7 @@ struct CopyJob {   // B(0:m, 0:n) = A(0:m, 0:n): leading sub-block copy between 
 struct GTerm {
     int64_t a;     // element offset of A (column-major, leading dimension = item.lda) at row 0
     int64_t x;     // element offset of x in the vector pool
-    int64_t t;     // symmetric term: staging offset of A^T x_item (cols entries), else -1
+    int64_t t;     // symmetric term: vector-pool offset receiving A^T x_item (cols entries), else -1
     int32_t cols;
     int32_t ld;    // transposed term (16-wide transposed launch): leading dimension of the stored block
 };
@@ -133,7 +133,9 @@ struct GItem {
     int32_t lda;
     int32_t row0;  // first row of this tile inside the terms' matrices
     int32_t t0, t1;
-    int32_t flags; // bit 0: atomic add (else exclusive); bit 1: operator in the second pool
+    int32_t flags; // bit 0: atomic add (else exclusive); bit 1: operator in the second pool;
+                   // bit 2: fused M2L + near-field item (atomic in the single-vector product);
+                   // bits 8..: system of a batch
 };
 
 constexpr int kRowTile = 128;  // rows of one GEMV item
@@ -172,11 +174,11 @@ void launch_cur(const CurJob *jobs, int njobs, const int32_t *idx, const double 
                 hipStream_t s);
 void launch_copy32(const CopyJob *jobs, int njobs, const float *src, float *dst, hipStream_t s);
 // operator pool A0, or A1 for items whose flags have bit 1 set; unr in {2, 4, 8}.  Role 1:
-// symmetric terms (t >= 0) also write A^T x_item to stag.  Role 3: the staging gather.
+// symmetric terms (t >= 0) also add A^T x_item to vec[t ..] atomically.
 void launch_bgemv64(const GItem *items, int nitems, const GTerm *terms, const double *A0, const double *A1,
-                    double *stag, double *vec, int role, int unr, hipStream_t s);
+                    double *vec, int role, int unr, hipStream_t s);
 void launch_bgemv32(const GItem *items, int nitems, const GTerm *terms, const float *A0, const float *A1,
-                    float *stag, float *vec, int role, int unr, hipStream_t s);
+                    float *vec, int role, int unr, hipStream_t s);
 void launch_otf32(const GItem *items, int nitems, const GTerm *terms, SlotPoints sp, float *vec, hipStream_t s);
 void launch_otf64(const GItem *items, int nitems, const GTerm *terms, SlotPoints sp, double *vec, hipStream_t s);
 void launch_to_fp32(const double *a, float *b, int64_t n, hipStream_t s);
@@ -203,11 +205,6 @@ void launch_axpby(double a, const double *x, double b, double *y, int64_t n, hip
 void launch_scal(double a, double *y, int64_t n, hipStream_t s);  // y *= a
 void launch_vec_update(const double *V, int64_t ldv, int k, const double *c, double *x, int64_t n, hipStream_t s);
 
-// the gather of the staging slots (role 3 launches of the single-vector path)
-void launch_gather64(const GItem *items, int nitems, const GTerm *terms, const double *stag, double *vec,
-                     hipStream_t s);
-void launch_gather32(const GItem *items, int nitems, const GTerm *terms, const float *stag, float *vec,
-                     hipStream_t s);
 // kNR right-hand sides, transposed products of the stored symmetric blocks (role 3 lists)
 // (narrow: slabs of 4 VEC inner indices, 16 columns per load, four slabs per step -- for
 // blocks of few rows; else slabs of 16 VEC inner indices, 4 columns per load)

@@ -690,12 +690,7 @@ void add_item(WorkList &W, int64_t y, int rows, const std::vector<TermSpec> &ter
 // step k >= 1 with |u_k||v_k| <= tau * normS_k ends the cross (2D/codes/aca.hpp:363)
 // Items whose terms are never split by columns: a row tile's terms are grouped into
 // items of at most kItemWork elements (a tile split over several items accumulates
-// atomically).  Terms keep their t (a symmetric term's staging slot, assigned per row tile
-// by the caller).
-// Staging slots start on their own cache lines (32 elements: 128 bytes in FP32, 256 in
-// FP64): a line shared by the slots of two items would be written back half-filled.
-inline int64_t slot_ld(int64_t r) { return align_up(r, 32); }
-
+// atomically).  Terms keep their t (a symmetric term's transposed-product output).
 void add_item_whole(WorkList &W, int64_t y, int64_t xb, int rows, const std::vector<TermSpec> &terms, int lda,
                     int pool) {
     if (rows <= 0) return;
@@ -740,7 +735,7 @@ void add_item_whole(WorkList &W, int64_t y, int64_t xb, int rows, const std::vec
 // A launch of few large items (the transfer passes near the root) leaves most of the
 // GPU idle while a handful of workgroups stream their items: split such items by
 // columns (the pieces accumulate atomically) until the launch has about kMinItems
-// items of at least kMinPiece elements.  Lists with staging terms are left alone.
+// items of at least kMinPiece elements.  Lists with symmetric terms are left alone.
 constexpr int64_t kMinItems = 2048, kMinPiece = 128 * 64;
 void rebalance(WorkList &W) {
     if ((int64_t)W.items.size() >= kMinItems) return;
@@ -801,23 +796,20 @@ struct SymGroup {
     std::vector<int64_t> self;
     int pool = 0;
 };
-// The four lists of such a group: fused items (box b: y_b += K(b, p) x_p over its stored
-// blocks, each block's K(b, p)^T x_b into a staging slot), the gather (y_p += the staging
-// slots of p), the 16-wide path's transposed items (box p: y_p += K(b, p)^T x_b, b < p) and
-// the on-the-fly path's directed items.  Slots are laid out in the order the fused items
-// write them (item after item, term after term), so that every fused item writes one
-// contiguous range; the gather of p reads its slots from the ranges of the items of its
-// lower partners (its terms' a are the slot offsets).
+// The three lists of such a group: fused items (box b: y_b += K(b, p) x_p over its stored
+// blocks, and each block's K(b, p)^T x_b added to y_p; in the single-vector product every
+// fused item accumulates atomically, since its rows also receive the transposed products of
+// other items), the
+// 16-wide path's transposed items (box p: y_p += K(b, p)^T X_b, b < p) and the on-the-fly
+// path's directed items.
 struct SymLists {
-    WorkList fused, gather, trans, otf;
-    int64_t stag = 0;  // staging elements
+    WorkList fused, trans, otf;
 };
 void sym_lists(const SymGroup &G, SymLists &L) {
     const int nb = (int)G.r.size();
     std::vector<std::vector<std::pair<int, int64_t>>> down(nb);
     for (int b = 0; b < nb; ++b)
         for (auto &pe : G.up[b]) down[pe.first].push_back({b, pe.second});
-    // fused items; a symmetric term carries its partner p in t until its slot is assigned
     const size_t item0 = L.fused.items.size();
     for (int b = 0; b < nb; ++b) {
         if (G.r[b] <= 0) continue;
@@ -825,40 +817,14 @@ void sym_lists(const SymGroup &G, SymLists &L) {
         if (G.self[b] >= 0) t.push_back(TermSpec{G.self[b], G.x[b], G.r[b]});
         for (auto &pe : G.up[b]) {
             TermSpec ts{pe.second, G.x[pe.first], G.r[pe.first]};
-            ts.t = pe.first;
+            ts.t = G.y[pe.first];
             t.push_back(ts);
         }
         add_item_whole(L.fused, G.y[b], G.x[b], G.r[b], t, (int)pad_ld(G.r[b]), G.pool);
     }
-    std::vector<std::vector<int64_t>> slots(nb);  // slot offsets of p, in order of the fused items
-    for (size_t q = item0; q < L.fused.items.size(); ++q) {
-        const GItem &it = L.fused.items[q];
-        for (int32_t k = it.t0; k < it.t1; ++k) {
-            GTerm &tm = L.fused.terms[k];
-            if (tm.t < 0) continue;
-            const int p = (int)tm.t;
-            tm.t = L.stag;
-            slots[p].push_back(L.stag);
-            L.stag += slot_ld(G.r[p]);
-        }
-    }
-    // gather: one item per row tile of p, one term per slot (a = slot offset)
-    for (int p = 0; p < nb; ++p) {
-        if (slots[p].empty()) continue;
-        for (int r0 = 0; r0 < G.r[p]; r0 += kRowTile) {
-            GItem it;
-            it.y = G.y[p] + r0;
-            it.xb = 0;
-            it.rows = std::min(kRowTile, G.r[p] - r0);
-            it.lda = 0;
-            it.row0 = r0;
-            it.t0 = (int32_t)L.gather.terms.size();
-            for (int64_t o : slots[p]) L.gather.terms.push_back(GTerm{o, 0, -1, 1, 0});
-            it.t1 = (int32_t)L.gather.terms.size();
-            it.flags = 0;
-            L.gather.items.push_back(it);
-        }
-    }
+    // (flag bit 2: the single-vector kernel adds y atomically; the 16-wide path applies the
+    // transposes in a later launch and keeps exclusive writers)
+    for (size_t q = item0; q < L.fused.items.size(); ++q) L.fused.items[q].flags |= 4;
     for (int p = 0; p < nb; ++p) {
         std::vector<TermSpec> t;
         for (auto &be : down[p]) {
@@ -1120,7 +1086,7 @@ void build_inner(Handle &h, double irp) {
         }
         {
             // the inner M2L blocks (compact leading sub-blocks, one per unordered pair, second
-            // pool) and the stored near field, fused, then the gather of their transposed products
+            // pool) and the stored near field, both directions in one fused launch
             SymLists SL;
             for (int f = 0; f < 2; ++f) {
                 if (!fam_on[f]) continue;
@@ -1160,22 +1126,13 @@ void build_inner(Handle &h, double irp) {
             }
             SL.fused.role = 1;
             SL.fused.stage = 1;
-            SL.gather.role = 3;
-            SL.gather.stage = 1;
-            SL.gather.step = 1;
             IL.push_back(std::move(SL.fused));
-            IL.push_back(std::move(SL.gather));
             SL.trans.role = 3;
             SL.trans.stage = 1;
             SL.trans.step = 1;
             std::vector<WorkList> tl;
             tl.push_back(std::move(SL.trans));
             upload_lists(tl, h.inner_trans, s);
-            h.inner_stag_len = SL.stag;
-            const float *was = h.stag32.p;
-            h.stag32.ensure((size_t)std::max<int64_t>(32, std::max(h.stag_len, h.inner_stag_len)));
-            if (h.stag32.p != was) drop_graph(h, H2W_FP32);  // its staging pointer moved
-            H2W_HIP(hipMemsetAsync(h.stag32.p, 0, h.stag32.n * 4, s));
         }
         for (int j = 1; j <= P.nL - 1; ++j) {
             WorkList W;
@@ -1234,9 +1191,8 @@ void build_inner(Handle &h, double irp) {
         for (auto &W : IL)
             if (W.role == 0) rebalance(W);
         upload_lists(IL, h.inner, s);
-        h.inner_elems = 0;  // operator elements (the gather reads staging slots)
-        for (size_t q = 0; q < h.inner.launches.size(); ++q)
-            if (h.inner.launches[q].role != 3) h.inner_elems += h.inner.elems[q];
+        h.inner_elems = 0;
+        for (auto e : h.inner.elems) h.inner_elems += e;
     } else {
         h.inner.launches.clear();
         h.inner_elems = 0;
@@ -1604,8 +1560,8 @@ void assemble_operators(Handle &h, double t0) {
         }
         WL.push_back(std::move(W));
     }
-    // M2L (every level, both families) + near field, one block per unordered pair: the fused
-    // launch, then the gather of the transposed products
+    // M2L (every level, both families) + near field, one block per unordered pair, both
+    // directions in one fused launch
     {
         SymLists SL;
         for (int f = 0; f < 2; ++f) {
@@ -1643,12 +1599,7 @@ void assemble_operators(Handle &h, double t0) {
         }
         SL.fused.role = 1;
         SL.fused.stage = 1;
-        SL.gather.role = 3;
-        SL.gather.stage = 1;
-        SL.gather.step = 1;
         WL.push_back(std::move(SL.fused));
-        WL.push_back(std::move(SL.gather));
-        h.stag_len = SL.stag;
         SL.trans.role = 3;
         SL.trans.stage = 1;
         SL.trans.step = 1;
@@ -1735,19 +1686,13 @@ void assemble_operators(Handle &h, double t0) {
     h.d_terms.upload(terms, s);
     h.h_items = items;
     h.h_terms = terms;
-    h.stag64.alloc((size_t)std::max<int64_t>(h.stag_len, 32));
-    H2W_HIP(hipMemsetAsync(h.stag64.p, 0, h.stag64.n * 8, s));
-    if (h.opts.fp32_shadow) {
-        h.stag32.ensure((size_t)std::max<int64_t>(std::max(h.stag_len, h.inner_stag_len), 32));
-        H2W_HIP(hipMemsetAsync(h.stag32.p, 0, h.stag32.n * 4, s));
-    }
     int64_t se = 0;
     h.launch_elems.assign(h.launches.size(), 0);
     for (size_t q = 0; q < h.launches.size(); ++q) {
         const Launch &Lh = h.launches[q];
         for (int64_t i = Lh.item0; i < Lh.item0 + Lh.nitems; ++i)
             for (int t = items[i].t0; t < items[i].t1; ++t) h.launch_elems[q] += (int64_t)items[i].rows * terms[t].cols;
-        if (Lh.role != 3) se += h.launch_elems[q];  // operator elements (the gather reads staging slots)
+        se += h.launch_elems[q];
     }
     h.stored_elems = se;
     build_inner(h, (double)h.opts.ir_tol_pow);
@@ -1772,15 +1717,13 @@ void rebuild_operators(Handle &h) {
 
 bool is32(int precision) { return precision == H2W_FP32 || precision == H2W_FP32_OTF || precision == H2W_FP32_INNER; }
 
-// Stage 1 is one fused launch (role 1: every stored block K(b, p), p >= b, applied to
-// x_p, the transposed products K(b, p)^T x_b written to staging slots) and one gather
-// (role 3: y_p += the staging slots of p).  The on-the-fly precisions evaluate the
-// blocks from the points instead, over the directed lists, and skip the gather.
+// Stage 1 is one fused launch (role 1): every stored block K(b, p), p >= b, applied to
+// x_p, and its transpose to x_b with the product added to y_p.  The on-the-fly
+// precisions evaluate the blocks from the points instead, over the directed lists.
 void run_launch(Handle &h, const Launch &Lh, int precision) {
     hipStream_t s = h.stream;
-    const bool otf = (precision == H2W_FP32_OTF || precision == H2W_FP64_OTF) && (Lh.role == 1 || Lh.role == 3);
+    const bool otf = (precision == H2W_FP32_OTF || precision == H2W_FP64_OTF) && Lh.role == 1;
     if (otf) {
-        if (Lh.role == 3) return;
         for (auto &Lo : h.otf.launches) {
             if (is32(precision))
                 launch_otf32(h.otf.d_items.p + Lo.item0, (int)Lo.nitems, h.otf.d_terms.p, h.sp, h.vec32.p, s);
@@ -1792,16 +1735,10 @@ void run_launch(Handle &h, const Launch &Lh, int precision) {
     const bool in = precision == H2W_FP32_INNER;
     const GItem *items = (in ? h.inner.d_items.p : h.d_items.p) + Lh.item0;
     const GTerm *terms = in ? h.inner.d_terms.p : h.d_terms.p;
-    if (Lh.role == 3) {
-        if (is32(precision)) launch_gather32(items, (int)Lh.nitems, terms, h.stag32.p, h.vec32.p, s);
-        else launch_gather64(items, (int)Lh.nitems, terms, h.stag64.p, h.vec64.p, s);
-    } else if (in) {
-        launch_bgemv32(items, (int)Lh.nitems, terms, h.mat32.p, h.mat32in.p, h.stag32.p, h.vec32.p, Lh.role, Lh.unr, s);
-    } else if (is32(precision)) {
-        launch_bgemv32(items, (int)Lh.nitems, terms, h.mat32.p, nullptr, h.stag32.p, h.vec32.p, Lh.role, Lh.unr32, s);
-    } else {
-        launch_bgemv64(items, (int)Lh.nitems, terms, h.mat64.p, nullptr, h.stag64.p, h.vec64.p, Lh.role, Lh.unr, s);
-    }
+    if (in) launch_bgemv32(items, (int)Lh.nitems, terms, h.mat32.p, h.mat32in.p, h.vec32.p, Lh.role, Lh.unr, s);
+    else if (is32(precision))
+        launch_bgemv32(items, (int)Lh.nitems, terms, h.mat32.p, nullptr, h.vec32.p, Lh.role, Lh.unr32, s);
+    else launch_bgemv64(items, (int)Lh.nitems, terms, h.mat64.p, nullptr, h.vec64.p, Lh.role, Lh.unr, s);
 }
 
 void zero_outputs(Handle &h, int precision) {

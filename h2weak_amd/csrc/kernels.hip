@@ -880,12 +880,6 @@ __device__ __forceinline__ void buf_ld_n(T (&out)[N], __amdgpu_buffer_rsrc_t r, 
         }
     }
 }
-template <typename T>
-__device__ __forceinline__ void buf_st(T v, __amdgpu_buffer_rsrc_t r, uint32_t off) {
-    typedef unsigned int U2 __attribute__((ext_vector_type(2)));
-    if constexpr (sizeof(T) == 8) __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(U2, v), r, off, 0, 0);
-    else __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), r, off, 0, 0);
-}
 
 // Work lists are never written by a kernel: read through the constant address space, their
 // wave-uniform loads stay scalar after the kernel's own stores (a vector load there would
@@ -904,12 +898,11 @@ __device__ __forceinline__ GTerm term_at(const CGTerm *p, int k) {
 // The column loop of an item walks (term, column) iterations with a wave-uniform
 // cursor; each iteration is UNR column loads per lane.  Two iterations are in flight:
 // the loads of the next are issued before the current one is multiplied.  Every load
-// and staging store is unconditional (lanes out of range address past the buffer's
-// end), so the compiler's waits count exactly the outstanding operations.
+// is unconditional (lanes out of range address past the buffer's end), so the
+// compiler's waits count exactly the outstanding operations.
 template <typename T, int VEC, int ROLE, int UNR>
 __global__ __launch_bounds__(256) void k_bgemv(const GItem *__restrict__ items, const GTerm *__restrict__ terms,
-                                               const T *__restrict__ A0, const T *__restrict__ A1, T *vec,
-                                               T *__restrict__ stag) {
+                                               const T *__restrict__ A0, const T *__restrict__ A1, T *vec) {
     typedef T V __attribute__((ext_vector_type(VEC)));
     const GItem it = items[blockIdx.x];
     const CGTerm *const tms = (const CGTerm *)terms;
@@ -933,7 +926,6 @@ __global__ __launch_bounds__(256) void k_bgemv(const GItem *__restrict__ items, 
     // c0 + q UNR + u (one vector load of their x); the four waves take every fourth chunk
     const int chunk = cpw * UNR;
     const uint32_t roff = (uint32_t)((it.row0 + (act ? r0 : 0)) * sizeof(T));  // this lane's rows in a column
-    const __amdgpu_buffer_rsrc_t srs = buf_rsrc(stag, 0x7FFFFFFFu);
     struct Iter {
         V av[UNR];
         T xv[UNR];
@@ -979,12 +971,13 @@ __global__ __launch_bounds__(256) void k_bgemv(const GItem *__restrict__ items, 
             for (int v = 0; v < VEC; ++v) acc[v] = fma(B.av[u][v], xu, acc[v]);
         }
         if (ROLE == 1) {
-            // columns cb + u of the block dotted with this tile's rows of x, summed
-            // over the lpc lanes that hold them.  A butterfly first halves the UNR values per
-            // exchange (lane bit o keeps one half, sends the other), so that after log2 UNR
-            // exchanges lane l holds column ui(l) summed over its UNR-lane group; plain
-            // exchanges then finish the sums, and the first UNR lanes of every column group
-            // store them (non-symmetric terms, t < 0, store nothing).
+            // columns cb + u of the block dotted with this tile's rows of x, summed over the
+            // lpc lanes that hold them, and added to the partner box's output y_p (offset t)
+            // with atomics.  A butterfly first halves the UNR values per exchange (lane bit o
+            // keeps one half, sends the other), so that after log2 UNR exchanges lane l holds
+            // column ui(l) summed over its UNR-lane group; plain exchanges then finish the
+            // sums, and the first UNR lanes of every column group add them (non-symmetric
+            // terms, t < 0, add nothing).
             T tp[UNR];
 #pragma unroll
             for (int u = 0; u < UNR; ++u) {
@@ -1009,24 +1002,22 @@ __global__ __launch_bounds__(256) void k_bgemv(const GItem *__restrict__ items, 
                 for (int o = UNR; o < 64; o <<= 1)
                     if (o < lpc) tp[0] += xor_lane_n(tp[0], o);
                 const int cc = B.cb + ui;
-                const bool st = (lane % lpc) < UNR && cc < B.cols && B.t >= 0;
-                buf_st<T>(tp[0], srs, st ? (uint32_t)((B.t + cc) * sizeof(T)) : kOutOfRange);
+                if ((lane % lpc) < UNR && cc < B.cols && B.t >= 0) atomicAdd(vec + B.t + cc, tp[0]);
             } else {
                 // columns of fewer than UNR lanes (short tiles): every lane sums all UNR values
-                // over its group, lane k of the group stores the columns u = k mod lpc
+                // over its group, lane k of the group adds the columns u = k mod lpc
 #pragma unroll
                 for (int u = 0; u < UNR; ++u) {
 #pragma unroll
                     for (int o = 1; o < UNR; o <<= 1)
                         if (o < lpc) tp[u] += xor_lane_n(tp[u], o);
                     const int cc = B.cb + u;
-                    const bool st = (u % lpc) == (lane % lpc) && cc < B.cols && B.t >= 0;
-                    buf_st<T>(tp[u], srs, st ? (uint32_t)((B.t + cc) * sizeof(T)) : kOutOfRange);
+                    if ((u % lpc) == (lane % lpc) && cc < B.cols && B.t >= 0) atomicAdd(vec + B.t + cc, tp[u]);
                 }
             }
         }
     };
-    // (an iteration loaded past the last one reads nothing and stores nothing: both halves
+    // (an iteration loaded past the last one reads nothing and adds nothing: both halves
     // run unconditionally, so the loop has a single path for the waits)
     Iter B0, B1;
     settle();
@@ -1050,41 +1041,8 @@ __global__ __launch_bounds__(256) void k_bgemv(const GItem *__restrict__ items, 
         const int t = threadIdx.x;
         T s = part[0][t] + part[1][t] + part[2][t] + part[3][t];
         T *y = vec + it.y + t;
-        if (it.flags & 1) atomicAdd(y, s);
+        if (it.flags & 5) atomicAdd(y, s);  // split items; fused items (transposes land in y too)
         else *y += s;
-    }
-}
-
-// Gather of the staging slots: y[row0 + r] += sum over the item's terms of
-// stag[a + row0 + r] (one term per slot; a slot holds one block's transposed product,
-// all rows of the output box).  Threads r and r + 128 take alternate slots, four loads
-// in flight each.
-template <typename T>
-__global__ __launch_bounds__(256) void k_gather(const GItem *__restrict__ items, const GTerm *__restrict__ terms,
-                                                const T *__restrict__ stag, T *vec) {
-    const GItem it = items[blockIdx.x];
-    const CGTerm *const tms = (const CGTerm *)terms;
-    __shared__ T part[2][kRowTile];
-    const int r = threadIdx.x & (kRowTile - 1), h = threadIdx.x / kRowTile;
-    T s0 = T(0), s1 = T(0);
-    if (r < it.rows) {
-        const T *src = stag + it.row0 + r;
-        int tt = it.t0 + h;
-        for (; tt + 6 < it.t1; tt += 8) {
-            const T a = src[tms[tt].a], b = src[tms[tt + 2].a], c = src[tms[tt + 4].a], d = src[tms[tt + 6].a];
-            s0 += a + b;
-            s1 += c + d;
-        }
-        for (; tt < it.t1; tt += 2) s0 += src[tms[tt].a];
-    }
-    part[h][r] = s0 + s1;
-    __syncthreads();
-    if (threadIdx.x < it.rows) {
-        const int t = threadIdx.x;
-        const T v = part[0][t] + part[1][t];
-        T *y = vec + it.y + t;
-        if (it.flags & 1) atomicAdd(y, v);
-        else *y += v;
     }
 }
 
@@ -1880,35 +1838,35 @@ void launch_cur(const CurJob *jobs, int njobs, const int32_t *idx, const double 
     if (njobs > 0) hipLaunchKernelGGL(k_cur, dim3(njobs), dim3(256), 0, s, jobs, idx, lr, A, kp);
 }
 template <typename T, int ROLE>
-static void launch_bgemv_r(const GItem *items, int nitems, const GTerm *terms, const T *A0, const T *A1, T *stag,
-                           T *vec, int unr, hipStream_t s) {
+static void launch_bgemv_r(const GItem *items, int nitems, const GTerm *terms, const T *A0, const T *A1, T *vec,
+                           int unr, hipStream_t s) {
     constexpr int VEC = sizeof(T) == 8 ? 2 : 4;
     if (unr <= 2)
-        hipLaunchKernelGGL((k_bgemv<T, VEC, ROLE, 2>), dim3(nitems), dim3(256), 0, s, items, terms, A0, A1, vec, stag);
+        hipLaunchKernelGGL((k_bgemv<T, VEC, ROLE, 2>), dim3(nitems), dim3(256), 0, s, items, terms, A0, A1, vec);
     else if (unr >= 8)
-        hipLaunchKernelGGL((k_bgemv<T, VEC, ROLE, 8>), dim3(nitems), dim3(256), 0, s, items, terms, A0, A1, vec, stag);
+        hipLaunchKernelGGL((k_bgemv<T, VEC, ROLE, 8>), dim3(nitems), dim3(256), 0, s, items, terms, A0, A1, vec);
     else
-        hipLaunchKernelGGL((k_bgemv<T, VEC, ROLE, 4>), dim3(nitems), dim3(256), 0, s, items, terms, A0, A1, vec, stag);
+        hipLaunchKernelGGL((k_bgemv<T, VEC, ROLE, 4>), dim3(nitems), dim3(256), 0, s, items, terms, A0, A1, vec);
 }
 template <typename T>
-static void launch_bgemv_t(const GItem *items, int nitems, const GTerm *terms, const T *A0, const T *A1, T *stag,
-                           T *vec, int role, int unr, hipStream_t s) {
+static void launch_bgemv_t(const GItem *items, int nitems, const GTerm *terms, const T *A0, const T *A1, T *vec,
+                           int role, int unr, hipStream_t s) {
     if (nitems <= 0) return;
     if (!A1) A1 = A0;
-    if (role == 1) launch_bgemv_r<T, 1>(items, nitems, terms, A0, A1, stag, vec, unr, s);
-    else if (role == 2) launch_bgemv_r<T, 2>(items, nitems, terms, A0, A1, stag, vec, unr, s);
-    else launch_bgemv_r<T, 0>(items, nitems, terms, A0, A1, stag, vec, unr, s);
+    if (role == 1) launch_bgemv_r<T, 1>(items, nitems, terms, A0, A1, vec, unr, s);
+    else if (role == 2) launch_bgemv_r<T, 2>(items, nitems, terms, A0, A1, vec, unr, s);
+    else launch_bgemv_r<T, 0>(items, nitems, terms, A0, A1, vec, unr, s);
 }
 void launch_copy32(const CopyJob *jobs, int njobs, const float *src, float *dst, hipStream_t s) {
     if (njobs > 0) hipLaunchKernelGGL(k_copy32, dim3(njobs), dim3(256), 0, s, jobs, src, dst);
 }
 void launch_bgemv64(const GItem *items, int nitems, const GTerm *terms, const double *A0, const double *A1,
-                    double *stag, double *vec, int role, int unr, hipStream_t s) {
-    launch_bgemv_t<double>(items, nitems, terms, A0, A1, stag, vec, role, unr, s);
+                    double *vec, int role, int unr, hipStream_t s) {
+    launch_bgemv_t<double>(items, nitems, terms, A0, A1, vec, role, unr, s);
 }
 void launch_bgemv32(const GItem *items, int nitems, const GTerm *terms, const float *A0, const float *A1,
-                    float *stag, float *vec, int role, int unr, hipStream_t s) {
-    launch_bgemv_t<float>(items, nitems, terms, A0, A1, stag, vec, role, unr, s);
+                    float *vec, int role, int unr, hipStream_t s) {
+    launch_bgemv_t<float>(items, nitems, terms, A0, A1, vec, role, unr, s);
 }
 void launch_otf32(const GItem *items, int nitems, const GTerm *terms, SlotPoints sp, float *vec, hipStream_t s) {
     if (nitems > 0)
@@ -2114,13 +2072,5 @@ void launch_bgemm_t64(const GItem *items, int nitems, const GTerm *terms, const 
 void launch_bgemm_t32(const GItem *items, int nitems, const GTerm *terms, const float *A0, const float *A1,
                       const float *const *tab, float *vec, int narrow, hipStream_t s) {
     launch_bgemm_tl<float>(items, nitems, terms, A0, A1, tab, vec, narrow, s);
-}
-void launch_gather64(const GItem *items, int nitems, const GTerm *terms, const double *stag, double *vec,
-                     hipStream_t s) {
-    if (nitems > 0) hipLaunchKernelGGL(k_gather<double>, dim3(nitems), dim3(256), 0, s, items, terms, stag, vec);
-}
-void launch_gather32(const GItem *items, int nitems, const GTerm *terms, const float *stag, float *vec,
-                     hipStream_t s) {
-    if (nitems > 0) hipLaunchKernelGGL(k_gather<float>, dim3(nitems), dim3(256), 0, s, items, terms, stag, vec);
 }
 }  // namespace h2w

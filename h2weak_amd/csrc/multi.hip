@@ -26,8 +26,8 @@ void ensure_pool_m(Handle &h, int precision) {
 }
 
 // The fused stage-1 launch applies the stored blocks K(b, p), p >= b, directly; the
-// single-vector path's gather (role 3) is replaced by the transposed products
-// K(b, p)^T X_b of the role-3 lists (h.trans, h.inner_trans) on the matrix cores.
+// transposed products K(b, p)^T X_b follow as the launches of the role-3 lists
+// (h.trans, h.inner_trans) on the matrix cores.
 // One step of the sequence: a launch of the precision's lists, or (trans) launch Lh of
 // its role-3 lists.
 void run_launch_m(Handle &h, const Launch &Lh, int precision, bool trans) {
@@ -60,9 +60,9 @@ std::vector<std::pair<const Launch *, bool>> steps_m(Handle &h, int precision) {
     std::vector<std::pair<const Launch *, bool>> st;
     const LaunchSet &T = precision == H2W_FP32_INNER ? h.inner_trans : h.trans;
     for (auto &Lh : launches_for(h, precision)) {
-        if (Lh.role == 3)
+        st.push_back({&Lh, false});
+        if (Lh.role == 1)
             for (auto &Lt : T.launches) st.push_back({&Lt, true});
-        else st.push_back({&Lh, false});
     }
     return st;
 }

@@ -184,15 +184,14 @@ typedef struct {
     int64_t launch_bytes[H2W_MAX_LAUNCHES];
     int launch_role[H2W_MAX_LAUNCHES]; /* -1 zero fill, 0 transfer pass, 1 M2L + near field (one
                                           stored block per unordered pair, applied both ways),
-                                          2 semi-nested column factors, 3 gather of the
-                                          transposed products (reads staging slots)          */
+                                          2 semi-nested column factors                       */
 } h2w_bench;
 int h2w_bench_matvec(h2w_handle *h, int nrep, int precision, h2w_bench *out);
 /* The same for the 16-wide product (matrix cores, 16 right-hand sides per pass): ms_per_matvec is the
  * time of one pass over all 16 columns.  Entry 0 is the whole pass (launch_role[0] = -2,
  * launch_bytes[0] the operator bytes one pass multiplies: the symmetric blocks count twice, for
  * their direct and transposed products); entries 1.. are its launches one by one (role 4: the
- * transposed products of the symmetric blocks, which replace the gather). */
+ * transposed products of the symmetric blocks, a launch of their own on this path). */
 int h2w_bench_matvec_multi(h2w_handle *h, int nrep, int precision, h2w_bench *out);
 
 /* Rebuild the truncated FP32 inner operator of H2W_GMRES_FP32_INNER_IR for tolerance
