@@ -16,6 +16,10 @@ CASES = [
     (3, 4000, 32, 8, 0, 1, "rand"),       # (H2+H)*, IE: CUR column factors (atomic items)
     (2, 48 * 48, 36, 10, 0, 2, "grid"),   # H*
     (1, 4096, 64, 12, 1, 0, "rand"),      # 1D
+    # the transposed products' two slab shapes: blocks of few rows (narrow slabs in every
+    # precision) and 256-point leaves (wide slabs, output boxes of two row tiles)
+    (3, 16 ** 3, 27, 4, 1, 0, "grid"),
+    (2, 64 * 64, 256, 12, 1, 0, "grid"),
 ]
 
 
