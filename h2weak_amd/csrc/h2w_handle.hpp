@@ -103,7 +103,8 @@ struct Pair {                                  // semi-nested vertex pair (B <- 
 // at the leaves.
 struct Launch {
     int64_t item0, nitems, term0, nterms;
-    int role;      // 0 transfer pass, 1 M2L + near field, 2 semi-nested column factors
+    int role;      // 0 transfer pass, 1 M2L + near field, 2 semi-nested column factors,
+                   // 3 transposed products of the symmetric blocks (16-wide lists)
     int stage = 0, step = 0;
     int unr = 4;   // columns in flight per wave (autotuned), FP64 / inner FP32 lists
     int unr32 = 4; // the same for the stored FP32 copy
