@@ -910,7 +910,9 @@ __global__ __launch_bounds__(256) void k_bgemv(const GItem *__restrict__ items, 
     const int lane = threadIdx.x & 63;
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int rows = it.rows;
-    int lpc = 1;
+    // (the fused launch gives a column at least UNR lanes, so that its transposed sums
+    // end on UNR distinct lanes; the extra lanes of a short tile are inactive)
+    int lpc = ROLE == 1 ? UNR : 1;
     while (lpc * VEC < rows) lpc <<= 1;
     const int cpw = 64 / lpc;
     const int q = lane / lpc, r0 = (lane % lpc) * VEC;
@@ -978,43 +980,33 @@ __global__ __launch_bounds__(256) void k_bgemv(const GItem *__restrict__ items, 
             // column ui(l) summed over its UNR-lane group; plain exchanges then finish the
             // sums, and the first UNR lanes of every column group add them (non-symmetric
             // terms, t < 0, add nothing).
-            T tp[UNR];
+            // (a vector, not an array: the selects below then pick values, where selects
+            // between array elements become a dynamically indexed array)
+            typedef T VU __attribute__((ext_vector_type(UNR)));
+            VU tp;
 #pragma unroll
             for (int u = 0; u < UNR; ++u) {
                 tp[u] = T(0);
 #pragma unroll
                 for (int v = 0; v < VEC; ++v) tp[u] = fma(B.av[u][v], xb[v], tp[u]);
             }
-            if (lpc >= UNR) {
-                int ui = 0;
+            int ui = 0;
 #pragma unroll
-                for (int k = UNR, o = 1; k > 1; k >>= 1, o <<= 1) {
-                    const bool hi = lane & o;
+            for (int k = UNR, o = 1; k > 1; k >>= 1, o <<= 1) {
+                const bool hi = lane & o;
 #pragma unroll
-                    for (int i = 0; i < k / 2; ++i) {
-                        const T snd = hi ? tp[i] : tp[i + k / 2];
-                        const T keep = hi ? tp[i + k / 2] : tp[i];
-                        tp[i] = keep + xor_lane_n(snd, o);
-                    }
-                    ui += hi ? k / 2 : 0;
+                for (int i = 0; i < k / 2; ++i) {
+                    const T snd = hi ? tp[i] : tp[i + k / 2];
+                    const T keep = hi ? tp[i + k / 2] : tp[i];
+                    tp[i] = keep + xor_lane_n(snd, o);
                 }
-#pragma unroll
-                for (int o = UNR; o < 64; o <<= 1)
-                    if (o < lpc) tp[0] += xor_lane_n(tp[0], o);
-                const int cc = B.cb + ui;
-                if ((lane % lpc) < UNR && cc < B.cols && B.t >= 0) atomicAdd(vec + B.t + cc, tp[0]);
-            } else {
-                // columns of fewer than UNR lanes (short tiles): every lane sums all UNR values
-                // over its group, lane k of the group adds the columns u = k mod lpc
-#pragma unroll
-                for (int u = 0; u < UNR; ++u) {
-#pragma unroll
-                    for (int o = 1; o < UNR; o <<= 1)
-                        if (o < lpc) tp[u] += xor_lane_n(tp[u], o);
-                    const int cc = B.cb + u;
-                    if ((u % lpc) == (lane % lpc) && cc < B.cols && B.t >= 0) atomicAdd(vec + B.t + cc, tp[u]);
-                }
+                ui += hi ? k / 2 : 0;
             }
+#pragma unroll
+            for (int o = UNR; o < 64; o <<= 1)
+                if (o < lpc) tp[0] += xor_lane_n(tp[0], o);
+            const int cc = B.cb + ui;
+            if ((lane % lpc) < UNR && cc < B.cols && B.t >= 0) atomicAdd(vec + B.t + cc, tp[0]);
         }
     };
     // (an iteration loaded past the last one reads nothing and adds nothing: both halves
