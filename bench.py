@@ -530,7 +530,8 @@ def main():
                                           f"same kernel on a launch of {rec['algorithmic_bytes'] / 1e9:.2f} GB, scaled")
     roof, roof2 = (roof_in, roof_st) if t_in >= t_st else (roof_st, roof_in)
     copy_bw = hip.copy_gbps()
-    log(f"device-to-device copy: {copy_bw:.0f} GB/s (read + write)")
+    read_bw = h2w.bench_stream_read(4 << 30, 5)
+    log(f"device-to-device copy: {copy_bw:.0f} GB/s (read + write); read-only stream: {read_bw:.0f} GB/s")
 
     # 16 right-hand sides per pass on the matrix cores: the product and 16 lockstep solves
     multi = None
@@ -574,6 +575,8 @@ def main():
     for r in (roof_in, roof_st):
         r["copy_GBps"] = round(copy_bw, 1)
         r["frac_of_copy"] = round(r["achieved"] / copy_bw, 4)
+        r["stream_read_GBps"] = round(read_bw, 1)
+        r["frac_of_stream_read"] = round(r["achieved"] / read_bw, 4)
 
     inner_bytes = A.stats()["stored_bytes_inner_fp32"]
     del A  # frees the large system's operators before the batch section

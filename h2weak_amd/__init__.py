@@ -37,7 +37,7 @@ EXPORTS = ("h2w_default_opts", "h2w_create", "h2w_assemble", "h2w_matvec", "h2w_
            "h2w_near_list", "h2w_destroy", "h2w_last_error", "h2w_bench_matvec", "h2w_set_inner_tolerance",
            "h2w_autotune", "h2w_gmres_multi", "h2w_bench_matvec_multi", "h2w_batch_create", "h2w_batch_matvec",
            "h2w_batch_gmres", "h2w_batch_bench_matvec", "h2w_batch_destroy", "h2w_set_rank_model",
-           "h2w_rank_samples", "h2w_rank_predict", "h2w_save", "h2w_load")
+           "h2w_rank_samples", "h2w_rank_predict", "h2w_save", "h2w_load", "h2w_bench_stream_read")
 RANK_FEATURES, RANK_PARAMS = 16, 9473
 TUNE_UNROLL, TUNE_INNER = 1, 2
 
@@ -134,6 +134,7 @@ def lib():
     L.h2w_load.argtypes = [ctypes.c_char_p, ctypes.POINTER(Opts), ctypes.POINTER(p)]
     L.h2w_gmres_multi.argtypes = [p, p, i64, p, i64, i, i, d, i, i, ctypes.POINTER(GmresStats)]
     L.h2w_autotune.argtypes = [p, p, i, d, i, i, ctypes.POINTER(Tune)]
+    L.h2w_bench_stream_read.argtypes = [i, i64, i, ctypes.POINTER(d)]
     _lib = L
     return L
 
@@ -149,6 +150,14 @@ def _f64(a):
 
 def _ptr(a):
     return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def bench_stream_read(nbytes=4 << 30, nrep=5, device=0):
+    """Read bandwidth (GB/s) of a plain read-only stream over nbytes of device memory: the
+    achievable rate a streaming kernel is compared with."""
+    g = ctypes.c_double()
+    _check(lib().h2w_bench_stream_read(int(device), int(nbytes), int(nrep), ctypes.byref(g)))
+    return g.value
 
 
 class Initialisation:

@@ -183,6 +183,10 @@ void launch_otf32(const GItem *items, int nitems, const GTerm *terms, SlotPoints
 void launch_otf64(const GItem *items, int nitems, const GTerm *terms, SlotPoints sp, double *vec, hipStream_t s);
 void launch_to_fp32(const double *a, float *b, int64_t n, hipStream_t s);
 void launch_to_fp64(const float *a, double *b, int64_t n, hipStream_t s);
+// read-only stream of bytes (whole chunks of stream_chunk_bytes(); the rest is not read) by grid
+// workgroups of 256; out: grid words, written only in the rare case the loaded words fold to a fixed value
+int64_t stream_chunk_bytes();
+void launch_stream_read(const void *p, int64_t bytes, uint32_t *out, int grid, hipStream_t s);
 void launch_fill(double *a, int64_t n, double v, hipStream_t s);
 void launch_fill32(float *a, int64_t n, float v, hipStream_t s);
 // tree-order gather/scatter with optional precision change, nrhs vectors of leading dimension ld

@@ -2580,6 +2580,55 @@ int h2w_near_list(const h2w_handle *h, int level, int box, int32_t *nb, int cap)
     return (int)l.size();
 }
 
+int h2w_bench_stream_read(int device, int64_t bytes, int nrep, double *gbps) {
+    H2W_TRY
+    if (!gbps || nrep < 1 || bytes < (1 << 20)) throw Error{H2W_EINVAL, "bad argument"};
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+        throw Error{H2W_ENOLIB, "no HIP device visible; libh2w has no CPU fallback"};
+    if (device < 0 || device >= ndev) throw Error{H2W_EINVAL, "device ordinal out of range"};
+    H2W_HIP(hipSetDevice(device));
+    bytes -= bytes % h2w::stream_chunk_bytes();
+    int cus = 0;
+    H2W_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
+    const int grid = 16 * std::max(cus, 1);
+    h2w::DevBuf<uint8_t> buf;
+    h2w::DevBuf<uint32_t> sink;
+    buf.alloc(bytes);
+    sink.alloc(grid);
+    hipStream_t s = nullptr;
+    H2W_HIP(hipStreamCreate(&s));
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    double ms_total = 0.0;
+    try {
+        H2W_HIP(hipEventCreate(&e0));
+        H2W_HIP(hipEventCreate(&e1));
+        H2W_HIP(hipMemsetAsync(buf.p, 0x5A, bytes, s));
+        h2w::launch_stream_read(buf.p, bytes, sink.p, grid, s);  // warm-up
+        H2W_HIP(hipEventRecord(e0, s));
+        for (int r = 0; r < nrep; ++r) h2w::launch_stream_read(buf.p, bytes, sink.p, grid, s);
+        H2W_HIP(hipEventRecord(e1, s));
+        H2W_HIP(hipGetLastError());
+        H2W_HIP(hipEventSynchronize(e1));
+        float ms = 0.f;
+        H2W_HIP(hipEventElapsedTime(&ms, e0, e1));
+        ms_total = ms;
+    } catch (...) {
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+        (void)hipStreamSynchronize(s);
+        (void)hipStreamDestroy(s);
+        throw;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    H2W_HIP(hipStreamSynchronize(s));
+    H2W_HIP(hipStreamDestroy(s));
+    *gbps = (double)bytes * nrep / (ms_total * 1e-3) * 1e-9;
+    return H2W_OK;
+    H2W_CATCH
+}
+
 int h2w_bench_matvec(h2w_handle *h, int nrep, int precision, h2w_bench *out) {
     H2W_TRY
     if (!h || !out || nrep < 1) throw Error{H2W_EINVAL, "bad argument"};

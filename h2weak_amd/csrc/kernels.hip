@@ -1523,6 +1523,27 @@ __global__ void k_to_fp64(const float *a, double *b, int64_t n) {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
         b[i] = (double)a[i];
 }
+// Read-only stream (the read bandwidth a streaming kernel is compared with): chunks of
+// kStreamUnr x 256 16-byte words, one chunk per workgroup and trip (lane t loads words
+// t, t + 256, ... of the chunk, all nontemporal and issued before any is used), chunks
+// strided over the grid.  The words are folded into one that is written only when it
+// equals a value it practically never takes, so the loads stay live and nothing is written.
+constexpr int kStreamUnr = 8;
+typedef uint32_t Word16 __attribute__((ext_vector_type(4)));
+__global__ __launch_bounds__(256) void k_stream_read(const Word16 *__restrict__ p, int64_t nchunks,
+                                                     uint32_t *__restrict__ out) {
+    uint32_t acc = 0;
+    for (int64_t c = blockIdx.x; c < nchunks; c += gridDim.x) {
+        const Word16 *q = p + c * (kStreamUnr * 256) + threadIdx.x;
+        Word16 w[kStreamUnr];
+#pragma unroll
+        for (int u = 0; u < kStreamUnr; ++u) w[u] = __builtin_nontemporal_load(q + u * 256);
+#pragma unroll
+        for (int u = 0; u < kStreamUnr; ++u) acc = (acc ^ w[u].x ^ w[u].y) + (w[u].z ^ w[u].w);
+    }
+    if (acc == 0x9E3779B9u) out[blockIdx.x] = acc;
+}
+
 template <typename T>
 __global__ void k_fill(T *a, int64_t n, T v) {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
@@ -1877,6 +1898,10 @@ void launch_to_fp32(const double *a, float *b, int64_t n, hipStream_t s) {
 }
 void launch_to_fp64(const float *a, double *b, int64_t n, hipStream_t s) {
     if (n > 0) hipLaunchKernelGGL(k_to_fp64, dim3(grid_for(n)), dim3(256), 0, s, a, b, n);
+}
+int64_t stream_chunk_bytes() { return (int64_t)kStreamUnr * 256 * 16; }
+void launch_stream_read(const void *p, int64_t bytes, uint32_t *out, int grid, hipStream_t s) {
+    hipLaunchKernelGGL(k_stream_read, dim3(grid), dim3(256), 0, s, (const Word16 *)p, bytes / stream_chunk_bytes(), out);
 }
 void launch_fill(double *a, int64_t n, double v, hipStream_t s) {
     if (n > 0) hipLaunchKernelGGL(k_fill<double>, dim3(grid_for(n)), dim3(256), 0, s, a, n, v);

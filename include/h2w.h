@@ -193,6 +193,11 @@ int h2w_bench_matvec(h2w_handle *h, int nrep, int precision, h2w_bench *out);
  * their direct and transposed products); entries 1.. are its launches one by one (role 4: the
  * transposed products of the symmetric blocks, a launch of their own on this path). */
 int h2w_bench_matvec_multi(h2w_handle *h, int nrep, int precision, h2w_bench *out);
+/* No counterpart in the reference: a measurement helper.  Read bandwidth of a plain read-only
+ * stream over `bytes` (>= 1 MiB, rounded down to a multiple of 32 KiB) of device memory on `device`, averaged over
+ * nrep launches after one warm-up (HIP events; allocates and frees its own buffer): the achievable
+ * rate the streaming products are compared with, beside the published 8 TB/s. */
+int h2w_bench_stream_read(int device, int64_t bytes, int nrep, double *gbps);
 
 /* Rebuild the truncated FP32 inner operator of H2W_GMRES_FP32_INNER_IR for tolerance
  * 10^-ir_tol_pow (0 < ir_tol_pow < tol_pow; 0 removes it).  No counterpart in the reference

@@ -197,3 +197,13 @@ def test_graph_replays_match(built):
             assert np.linalg.norm(y - r) <= 1e-12 * np.linalg.norm(r)
             g.mat_vec_prod(x, precision=h2w.FP32)
         g.dense_rows([0, 1, 2], xs[0])
+
+
+def test_stream_read_bandwidth():
+    """The read-only stream helper the bench compares the products with: a positive rate no
+    higher than the published HBM peak, and argument checks."""
+    g = h2w.bench_stream_read(1 << 30, 3)
+    assert 500.0 < g < 8000.0
+    with pytest.raises(h2w.H2WError) as e:
+        h2w.bench_stream_read(1024, 1)
+    assert e.value.code == h2w.H2W_EINVAL
