@@ -424,24 +424,16 @@ int h2w_batch_bench_matvec(h2w_batch *B, int nrep, int precision, h2w_bench *out
     ly.alloc((size_t)B->n * h2w::kNR);
     h2w::launch_fill(lx.p, B->n * h2w::kNR, 1.0, s);
     h2w::batch_matvec_lock(*B, lx.p, ly.p, precision);
-    hipEvent_t e0, e1;
-    H2W_HIP(hipEventCreate(&e0));
-    H2W_HIP(hipEventCreate(&e1));
+    h2w::Event e0, e1;
     H2W_HIP(hipEventRecord(e0, s));
     for (int r = 0; r < nrep; ++r) h2w::batch_matvec_lock(*B, lx.p, ly.p, precision);
     H2W_HIP(hipEventRecord(e1, s));
-    H2W_HIP(hipEventSynchronize(e1));
-    float ms = 0.f;
-    H2W_HIP(hipEventElapsedTime(&ms, e0, e1));
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    out->ms_per_matvec = ms / nrep;
+    out->ms_per_matvec = h2w::elapsed_ms(e0, e1) / nrep;
     const auto &M = B->lists[precision == H2W_FP32_INNER ? 1 : 0];
     out->n_launch = (int)std::min<size_t>(M.launches.size(), H2W_MAX_LAUNCHES);
     const int es = h2w::is32(precision) ? 4 : 8;
     // per-launch durations: the launches one by one between events (eager, same kernels as the graph)
-    std::vector<hipEvent_t> ev(out->n_launch + 1);
-    for (auto &e : ev) H2W_HIP(hipEventCreate(&e));
+    std::vector<h2w::Event> ev(out->n_launch + 1);
     std::vector<double> acc(out->n_launch, 0.0);
     for (int r = 0; r < nrep; ++r) {
         H2W_HIP(hipEventRecord(ev[0], s));
@@ -449,14 +441,8 @@ int h2w_batch_bench_matvec(h2w_batch *B, int nrep, int precision, h2w_bench *out
             h2w::run_batch_launch(*B, M.launches[q], precision);
             H2W_HIP(hipEventRecord(ev[q + 1], s));
         }
-        H2W_HIP(hipEventSynchronize(ev[out->n_launch]));
-        for (int q = 0; q < out->n_launch; ++q) {
-            float m2 = 0.f;
-            H2W_HIP(hipEventElapsedTime(&m2, ev[q], ev[q + 1]));
-            acc[q] += m2;
-        }
+        for (int q = 0; q < out->n_launch; ++q) acc[q] += h2w::elapsed_ms(ev[q], ev[q + 1]);
     }
-    for (auto &e : ev) (void)hipEventDestroy(e);
     for (int q = 0; q < out->n_launch; ++q) {
         out->launch_ms[q] = acc[q] / nrep;
         out->launch_role[q] = M.launches[q].role;

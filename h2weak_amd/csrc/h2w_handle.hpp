@@ -23,6 +23,26 @@ inline double now_s() {
     return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
+// A HIP event owned by the scope: created at construction, destroyed on every exit path.
+struct Event {
+    hipEvent_t e = nullptr;
+    Event() { H2W_HIP(hipEventCreate(&e)); }
+    ~Event() {
+        if (e) (void)hipEventDestroy(e);
+    }
+    Event(const Event &) = delete;
+    Event &operator=(const Event &) = delete;
+    Event(Event &&o) noexcept : e(o.e) { o.e = nullptr; }
+    operator hipEvent_t() const { return e; }
+};
+// milliseconds between two recorded events (waits for the second)
+inline double elapsed_ms(const Event &a, const Event &b) {
+    H2W_HIP(hipEventSynchronize(b));
+    float m = 0.f;
+    H2W_HIP(hipEventElapsedTime(&m, a, b));
+    return m;
+}
+
 template <typename T>
 struct DevBuf {
     T *p = nullptr;

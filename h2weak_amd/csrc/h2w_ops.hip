@@ -2029,23 +2029,8 @@ h2w_gmres_stats solve_tree(Handle &h, const double *bt, int max_iter, double tol
 namespace {
 
 struct EventPair {
-    hipEvent_t a = nullptr, b = nullptr;
-    EventPair() {
-        H2W_HIP(hipEventCreate(&a));
-        H2W_HIP(hipEventCreate(&b));
-    }
-    ~EventPair() {
-        if (a) (void)hipEventDestroy(a);
-        if (b) (void)hipEventDestroy(b);
-    }
-    EventPair(const EventPair &) = delete;
-    EventPair &operator=(const EventPair &) = delete;
-    double ms(hipStream_t s) {
-        H2W_HIP(hipEventSynchronize(b));
-        float m = 0.f;
-        H2W_HIP(hipEventElapsedTime(&m, a, b));
-        return m;
-    }
+    Event a, b;
+    double ms(hipStream_t) { return elapsed_ms(a, b); }
 };
 
 double time_launch(Handle &h, const Launch &Lh, int precision, int nrep, EventPair &ev) {
@@ -2596,34 +2581,16 @@ int h2w_bench_stream_read(int device, int64_t bytes, int nrep, double *gbps) {
     h2w::DevBuf<uint32_t> sink;
     buf.alloc(bytes);
     sink.alloc(grid);
-    hipStream_t s = nullptr;
-    H2W_HIP(hipStreamCreate(&s));
-    hipEvent_t e0 = nullptr, e1 = nullptr;
-    double ms_total = 0.0;
-    try {
-        H2W_HIP(hipEventCreate(&e0));
-        H2W_HIP(hipEventCreate(&e1));
-        H2W_HIP(hipMemsetAsync(buf.p, 0x5A, bytes, s));
-        h2w::launch_stream_read(buf.p, bytes, sink.p, grid, s);  // warm-up
-        H2W_HIP(hipEventRecord(e0, s));
-        for (int r = 0; r < nrep; ++r) h2w::launch_stream_read(buf.p, bytes, sink.p, grid, s);
-        H2W_HIP(hipEventRecord(e1, s));
-        H2W_HIP(hipGetLastError());
-        H2W_HIP(hipEventSynchronize(e1));
-        float ms = 0.f;
-        H2W_HIP(hipEventElapsedTime(&ms, e0, e1));
-        ms_total = ms;
-    } catch (...) {
-        if (e0) (void)hipEventDestroy(e0);
-        if (e1) (void)hipEventDestroy(e1);
-        (void)hipStreamSynchronize(s);
-        (void)hipStreamDestroy(s);
-        throw;
-    }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    H2W_HIP(hipStreamSynchronize(s));
-    H2W_HIP(hipStreamDestroy(s));
+    // (on the null stream, which waits for the device's other work before it starts)
+    h2w::Event e0, e1;
+    H2W_HIP(hipMemset(buf.p, 0x5A, bytes));
+    h2w::launch_stream_read(buf.p, bytes, sink.p, grid, nullptr);  // warm-up
+    H2W_HIP(hipEventRecord(e0, nullptr));
+    for (int r = 0; r < nrep; ++r) h2w::launch_stream_read(buf.p, bytes, sink.p, grid, nullptr);
+    H2W_HIP(hipEventRecord(e1, nullptr));
+    H2W_HIP(hipGetLastError());
+    const double ms_total = h2w::elapsed_ms(e0, e1);
+    H2W_HIP(hipDeviceSynchronize());
     *gbps = (double)bytes * nrep / (ms_total * 1e-3) * 1e-9;
     return H2W_OK;
     H2W_CATCH
@@ -2644,22 +2611,16 @@ int h2w_bench_matvec(h2w_handle *h, int nrep, int precision, h2w_bench *out) {
     h2w::launch_fill(h->vec64.p + h->off_x, N, 1.0, s);
     h2w::launch_fill32(h->vec32.p + h->off_x, N, 1.0f, s);
     const int nl = std::min<int>((int)h2w::launches_for(*h, precision).size(), H2W_MAX_LAUNCHES - 1);
-    std::vector<hipEvent_t> ev(nl + 2);
-    for (auto &e : ev) H2W_HIP(hipEventCreate(&e));
+    std::vector<h2w::Event> ev(nl + 2);
     std::vector<double> acc(nl + 1, 0.0);
     // warm-up
     h2w::run_launches(*h, precision);
     H2W_HIP(hipStreamSynchronize(s));
-    hipEvent_t t0, t1;
-    H2W_HIP(hipEventCreate(&t0));
-    H2W_HIP(hipEventCreate(&t1));
+    h2w::Event t0, t1;
     H2W_HIP(hipEventRecord(t0, s));
     for (int r = 0; r < nrep; ++r) h2w::run_launches(*h, precision);
     H2W_HIP(hipEventRecord(t1, s));
-    H2W_HIP(hipEventSynchronize(t1));
-    float ms = 0.f;
-    H2W_HIP(hipEventElapsedTime(&ms, t0, t1));
-    out->ms_per_matvec = ms / nrep;
+    out->ms_per_matvec = h2w::elapsed_ms(t0, t1) / nrep;
     // per-launch durations: fill + each bgemv launch
     for (int r = 0; r < nrep; ++r) {
         H2W_HIP(hipEventRecord(ev[0], s));
@@ -2669,12 +2630,7 @@ int h2w_bench_matvec(h2w_handle *h, int nrep, int precision, h2w_bench *out) {
             h2w::run_launch(*h, LL[q], precision);
             H2W_HIP(hipEventRecord(ev[q + 2], s));
         }
-        H2W_HIP(hipEventSynchronize(ev[nl + 1]));
-        for (int q = 0; q <= nl; ++q) {
-            float m2 = 0.f;
-            H2W_HIP(hipEventElapsedTime(&m2, ev[q], ev[q + 1]));
-            acc[q] += m2;
-        }
+        for (int q = 0; q <= nl; ++q) acc[q] += h2w::elapsed_ms(ev[q], ev[q + 1]);
     }
     out->n_launch = nl + 1;
     const int es = h2w::is32(precision) ? 4 : 8;
@@ -2688,9 +2644,6 @@ int h2w_bench_matvec(h2w_handle *h, int nrep, int precision, h2w_bench *out) {
         out->launch_bytes[q + 1] =
             (precision == H2W_FP32_INNER ? h->inner.elems[q] : h->launch_elems[q]) * es;
     }
-    for (auto &e : ev) (void)hipEventDestroy(e);
-    (void)hipEventDestroy(t0);
-    (void)hipEventDestroy(t1);
     return H2W_OK;
     H2W_CATCH
 }

@@ -459,18 +459,11 @@ extern "C" int h2w_bench_matvec_multi(h2w_handle *h, int nrep, int precision, h2
         if (h2w::is32(precision)) h2w::launch_fill32(h->vec32m.p + h->off_x * h2w::kNR, nx, 1.0f, s);
         else h2w::launch_fill(h->vec64m.p + h->off_x * h2w::kNR, nx, 1.0, s);
         h2w::run_launches_m(*h, precision);
-        hipEvent_t e0, e1;
-        H2W_HIP(hipEventCreate(&e0));
-        H2W_HIP(hipEventCreate(&e1));
+        h2w::Event e0, e1;
         H2W_HIP(hipEventRecord(e0, s));
         for (int r = 0; r < nrep; ++r) h2w::run_launches_m(*h, precision);
         H2W_HIP(hipEventRecord(e1, s));
-        H2W_HIP(hipEventSynchronize(e1));
-        float ms = 0.f;
-        H2W_HIP(hipEventElapsedTime(&ms, e0, e1));
-        out->ms_per_matvec = ms / nrep;
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(e1);
+        out->ms_per_matvec = h2w::elapsed_ms(e0, e1) / nrep;
         // entry 0: the whole pass and the operator bytes it multiplies (every stored element
         // once, the symmetric blocks twice: direct and transposed products); entries 1..: the
         // launches one by one between events (eager), role 4 for the transposed products
@@ -484,8 +477,7 @@ extern "C" int h2w_bench_matvec_multi(h2w_handle *h, int nrep, int precision, h2
         out->launch_bytes[0] = el * es;
         const auto st = h2w::steps_m(*h, precision);
         const int nl = (int)std::min<size_t>(st.size(), H2W_MAX_LAUNCHES - 1);
-        std::vector<hipEvent_t> ev(nl + 1);
-        for (auto &e : ev) H2W_HIP(hipEventCreate(&e));
+        std::vector<h2w::Event> ev(nl + 1);
         std::vector<double> acc(nl, 0.0);
         for (int r = 0; r < nrep; ++r) {
             h2w::zero_outputs_m(*h, precision);
@@ -494,14 +486,8 @@ extern "C" int h2w_bench_matvec_multi(h2w_handle *h, int nrep, int precision, h2
                 h2w::run_launch_m(*h, *st[q].first, precision, st[q].second);
                 H2W_HIP(hipEventRecord(ev[q + 1], s));
             }
-            H2W_HIP(hipEventSynchronize(ev[nl]));
-            for (int q = 0; q < nl; ++q) {
-                float m2 = 0.f;
-                H2W_HIP(hipEventElapsedTime(&m2, ev[q], ev[q + 1]));
-                acc[q] += m2;
-            }
+            for (int q = 0; q < nl; ++q) acc[q] += h2w::elapsed_ms(ev[q], ev[q + 1]);
         }
-        for (auto &e : ev) (void)hipEventDestroy(e);
         const auto &L = h2w::launches_for(*h, precision);
         for (int q = 0; q < nl; ++q) {
             const h2w::Launch *Lp = st[q].first;

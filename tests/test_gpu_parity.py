@@ -207,3 +207,19 @@ def test_stream_read_bandwidth():
     with pytest.raises(h2w.H2WError) as e:
         h2w.bench_stream_read(1024, 1)
     assert e.value.code == h2w.H2W_EINVAL
+
+
+def test_bench_helpers(built):
+    """The timing entry points (single, 16-wide, batch): positive times, one fused launch
+    per product with its operator bytes, roles as documented in include/h2w.h."""
+    o, g, N = built(CASES[6])
+    for prec in (h2w.FP64, h2w.FP32, h2w.FP32_INNER):
+        r = g.bench_matvec(2, prec)
+        assert r["ms_per_matvec"] > 0 and r["launch_role"][0] == -1
+        assert r["launch_role"].count(1) == 1 and r["launch_bytes"][r["launch_role"].index(1)] > 0
+        assert all(t >= 0 for t in r["launch_ms"])
+        m = g.bench_matvec_multi(2, prec)
+        assert m["ms_per_pass"] > 0 and m["launch_role"][0] == -2 and 4 in m["launch_role"]
+        assert m["launch_bytes"][0] > r["launch_bytes"][r["launch_role"].index(1)]
+    b = h2w.Batch([g]).bench_matvec(2, h2w.FP64)
+    assert b["ms_per_pass"] > 0 and len(b["launch_ms"]) == len(b["launch_role"]) > 0
