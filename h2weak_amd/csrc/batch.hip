@@ -118,6 +118,7 @@ void merge_lists(Batch &B, int which) {
         for (int64_t t = Lm.term0; t < (int64_t)terms.size(); ++t) sc += terms[t].cols;
         Lm.avg_cols = Lm.nterms ? (int)(sc / Lm.nterms) : 0;
         if (Lm.nitems == 0) continue;
+        Lm.avg_item_elems = e / Lm.nitems;
         M.launches.push_back(Lm);
         M.elems.push_back(e);
     }
@@ -144,13 +145,15 @@ void run_batch_launch(Batch &B, const Launch &L, int precision) {
             launch_bgemm_t32(M.d_items.p + L.item0, (int)L.nitems, M.d_terms.p, nullptr, nullptr, B.tab32.p,
                              B.vec32m.p, trans_narrow(L, true), s);
         else
-            launch_bgemm32_tab(M.d_items.p + L.item0, (int)L.nitems, M.d_terms.p, B.tab32.p, B.vec32m.p, L.role, s);
+            launch_bgemm32_tab(M.d_items.p + L.item0, (int)L.nitems, M.d_terms.p, B.tab32.p, B.vec32m.p, L.role,
+                               per_wave_items(L), s);
     } else {
         if (L.role == 3)
             launch_bgemm_t64(M.d_items.p + L.item0, (int)L.nitems, M.d_terms.p, nullptr, nullptr, B.tab64.p,
                              B.vec64m.p, trans_narrow(L, false), s);
         else
-            launch_bgemm64_tab(M.d_items.p + L.item0, (int)L.nitems, M.d_terms.p, B.tab64.p, B.vec64m.p, L.role, s);
+            launch_bgemm64_tab(M.d_items.p + L.item0, (int)L.nitems, M.d_terms.p, B.tab64.p, B.vec64m.p, L.role,
+                               per_wave_items(L), s);
     }
 }
 

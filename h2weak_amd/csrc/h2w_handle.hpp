@@ -129,7 +129,14 @@ struct Launch {
     int unr = 4;   // columns in flight per wave (autotuned), FP64 / inner FP32 lists
     int unr32 = 4; // the same for the stored FP32 copy
     int avg_cols = 0;  // mean inner size of the terms (picks the transposed 16-wide kernel's slabs)
+    int64_t avg_item_elems = 0;  // mean operator elements per item
 };
+// one item per wave on the 16-wide path: launches of enough items that are small on average
+// (the transfer passes at the leaves of small systems), where a workgroup's set-up and the
+// reduction across its waves would outweigh the item
+inline bool per_wave_items(const Launch &L) {
+    return L.nitems >= 2048 && L.avg_item_elems > 0 && L.avg_item_elems <= 4096;
+}
 // narrow slabs for the transposed 16-wide products when the blocks have few rows
 inline int trans_narrow(const Launch &L, bool fp32) { return L.avg_cols <= (fp32 ? 32 : 16); }
 

@@ -216,11 +216,12 @@ void launch_bgemm_t64(const GItem *items, int nitems, const GTerm *terms, const 
                       const double *const *tab, double *vec, int narrow, hipStream_t s);
 void launch_bgemm_t32(const GItem *items, int nitems, const GTerm *terms, const float *A0, const float *A1,
                       const float *const *tab, float *vec, int narrow, hipStream_t s);
-// kNR right-hand sides: vector pools and work vectors hold element (slot, r) at slot * kNR + r
+// kNR right-hand sides: vector pools and work vectors hold element (slot, r) at slot * kNR + r;
+// per_wave: one item per wave instead of per workgroup (launches of many small items)
 void launch_bgemm64(const GItem *items, int nitems, const GTerm *terms, const double *A0, const double *A1,
-                    double *vec, int role, hipStream_t s);
+                    double *vec, int role, bool per_wave, hipStream_t s);
 void launch_bgemm32(const GItem *items, int nitems, const GTerm *terms, const float *A0, const float *A1, float *vec,
-                    int role, hipStream_t s);
+                    int role, bool per_wave, hipStream_t s);
 // dst(t, r) = X(perm[t] + r ldx) for r < nrhs, 0 beyond (perm == NULL: identity)
 void launch_gather_m64(const double *X, int64_t ldx, int nrhs, const int32_t *perm, int64_t N, double *dst,
                        hipStream_t s);
@@ -232,9 +233,9 @@ void launch_scatter_m32(const float *src, int64_t N, double *Y, int64_t ldy, int
                         hipStream_t s);
 // batched systems: the operator pools of system f >> 8 of an item come from tab[2 f + 0/1]
 void launch_bgemm64_tab(const GItem *items, int nitems, const GTerm *terms, const double *const *tab, double *vec,
-                        int role, hipStream_t s);
+                        int role, bool per_wave, hipStream_t s);
 void launch_bgemm32_tab(const GItem *items, int nitems, const GTerm *terms, const float *const *tab, float *vec,
-                        int role, hipStream_t s);
+                        int role, bool per_wave, hipStream_t s);
 void launch_map_in64(const double *X, const int32_t *map, int64_t n, double *pool, hipStream_t s);
 void launch_map_in32(const double *X, const int32_t *map, int64_t n, float *pool, hipStream_t s);
 void launch_map_out64(const double *pool, const int32_t *map, int64_t n, double *Y, hipStream_t s);

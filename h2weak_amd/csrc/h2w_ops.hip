@@ -874,6 +874,7 @@ void upload_lists(std::vector<WorkList> &WL, LaunchSet &LS, hipStream_t s) {
             items.push_back(it);
         }
         terms.insert(terms.end(), W.terms.begin(), W.terms.end());
+        Lh.avg_item_elems = e / Lh.nitems;
         LS.launches.push_back(Lh);
         LS.elems.push_back(e);
     }
@@ -1692,6 +1693,7 @@ void assemble_operators(Handle &h, double t0) {
         const Launch &Lh = h.launches[q];
         for (int64_t i = Lh.item0; i < Lh.item0 + Lh.nitems; ++i)
             for (int t = items[i].t0; t < items[i].t1; ++t) h.launch_elems[q] += (int64_t)items[i].rows * terms[t].cols;
+        h.launches[q].avg_item_elems = h.launch_elems[q] / Lh.nitems;
         se += h.launch_elems[q];
     }
     h.stored_elems = se;

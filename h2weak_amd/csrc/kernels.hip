@@ -1079,9 +1079,11 @@ struct Mfma16<double> {
 // 4-column chunks per iteration, two iterations in flight (the next one's loads are
 // issued before the current one is multiplied).  Loads are raw buffer loads whose
 // out-of-range lanes read zeros, so no branch guards them and the waits before the
-// MFMAs count only the current iteration's loads.  red: the workgroup's LDS
-// reduction buffer.
-template <typename T, int VEC, int NG, int UNR>
+// MFMAs count only the current iteration's loads.  WPI waves share the item: 4 (the
+// waves take every fourth chunk and sum their products through red, the workgroup's
+// LDS reduction buffer) or 1 (the wave takes every chunk and writes its rows itself;
+// red is unused and no barrier is reached).
+template <typename T, int VEC, int NG, int UNR, int WPI>
 __device__ __forceinline__ void bgemm_item(const GItem &it, const GTerm *__restrict__ terms, const T *__restrict__ A,
                                            T *vec, T (*red)[kRowTile][kNR]) {
     typedef typename Mfma16<T>::v4 V4;
@@ -1100,7 +1102,7 @@ __device__ __forceinline__ void bgemm_item(const GItem &it, const GTerm *__restr
     const int64_t lda = it.lda;
     // The item's columns, term after term, in 4-column chunks; wave wv takes chunks
     // wv, wv + 4, ... through a cursor (term tt, chunk ck of that term).
-    int tt = it.t0, ck = wv;
+    int tt = it.t0, ck = WPI == 4 ? wv : 0;
     auto settle = [&]() {
         while (tt < it.t1) {
             const int nk = (tms[tt].cols + 3) >> 2;
@@ -1136,7 +1138,7 @@ __device__ __forceinline__ void bgemm_item(const GItem &it, const GTerm *__restr
                 B.av[u][g] =
                     buf_ld16_nt<VL>(ra, opaque_off(ok && roff[g] != kOutOfRange ? coff + roff[g] : kOutOfRange));
             if (has) {
-                ck += 4;
+                ck += WPI;
                 settle();
             }
         }
@@ -1185,14 +1187,16 @@ __device__ __forceinline__ void bgemm_item(const GItem &it, const GTerm *__restr
 #pragma unroll
                     for (int v = 0; v < 4; ++v) acc[g * VEC + e][v] += red[slot][row_of(g, e, v)][m];
     };
-    if (wv >= 2) put(wv - 2);
-    __syncthreads();
-    if (wv < 2) add(wv);
-    __syncthreads();
-    if (wv == 1) put(0);
-    __syncthreads();
-    if (wv == 0) {
-        add(0);
+    if constexpr (WPI == 4) {
+        if (wv >= 2) put(wv - 2);
+        __syncthreads();
+        if (wv < 2) add(wv);
+        __syncthreads();
+        if (wv == 1) put(0);
+        __syncthreads();
+        if (wv == 0) add(0);
+    }
+    if (WPI == 1 || wv == 0) {
         T *y = vec + it.y * kNR + m;
 #pragma unroll
         for (int g = 0; g < NG; ++g)
@@ -1210,16 +1214,24 @@ __device__ __forceinline__ void bgemm_item(const GItem &it, const GTerm *__restr
     }
 }
 
-template <typename T, int ROLE>
-__global__ __launch_bounds__(256) void k_bgemm(const GItem *__restrict__ items, const GTerm *__restrict__ terms,
-                                               const T *__restrict__ A0, const T *__restrict__ A1,
-                                               const T *const *__restrict__ tab, T *vec) {
-    __shared__ T red[2][kRowTile][kNR];
-    const GItem it = items[blockIdx.x];
+// WPI 4: one item per workgroup; WPI 1: one item per wave (launches of many small items:
+// no cross-wave reduction, four items in flight per workgroup), nitems bounds the last one.
+template <typename T, int ROLE, int WPI>
+__global__ __launch_bounds__(256) void k_bgemm(const GItem *__restrict__ items, int nitems,
+                                               const GTerm *__restrict__ terms, const T *__restrict__ A0,
+                                               const T *__restrict__ A1, const T *const *__restrict__ tab, T *vec) {
+    const int idx = WPI == 4 ? (int)blockIdx.x : 4 * (int)blockIdx.x + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    if (idx >= nitems) return;  // (WPI 1 only: whole waves, and no barrier follows)
+    const GItem it = items[idx];
     // batched systems: pool pair of system flags >> 8 from the table
     const T *__restrict__ A = tab ? tab[2 * (it.flags >> 8) + ((it.flags >> 1) & 1)] : ((it.flags & 2) ? A1 : A0);
     constexpr int VEC = 16 / sizeof(T);
-    bgemm_item<T, VEC, kRowTile / (16 * VEC), 2>(it, terms, A, vec, red);
+    if constexpr (WPI == 4) {
+        __shared__ T red[2][kRowTile][kNR];
+        bgemm_item<T, VEC, kRowTile / (16 * VEC), 2, 4>(it, terms, A, vec, red);
+    } else {
+        bgemm_item<T, VEC, kRowTile / (16 * VEC), 2, 1>(it, terms, A, vec, nullptr);
+    }
 }
 
 // Transposed products of the symmetric blocks on the 16-wide path:
@@ -1962,32 +1974,40 @@ void launch_scal(double a, double *y, int64_t n, hipStream_t s) {
 }
 
 
+template <typename T, int ROLE>
+static void launch_bgemm_r(const GItem *items, int nitems, const GTerm *terms, const T *A0, const T *A1,
+                           const T *const *tab, T *vec, bool per_wave, hipStream_t s) {
+    if (per_wave)
+        hipLaunchKernelGGL((k_bgemm<T, ROLE, 1>), dim3((nitems + 3) / 4), dim3(256), 0, s, items, nitems, terms, A0, A1,
+                           tab, vec);
+    else
+        hipLaunchKernelGGL((k_bgemm<T, ROLE, 4>), dim3(nitems), dim3(256), 0, s, items, nitems, terms, A0, A1, tab,
+                           vec);
+}
 template <typename T>
 static void launch_bgemm_t(const GItem *items, int nitems, const GTerm *terms, const T *A0, const T *A1,
-                           const T *const *tab, T *vec, int role, hipStream_t s) {
+                           const T *const *tab, T *vec, int role, bool per_wave, hipStream_t s) {
     if (nitems <= 0) return;
     if (!A1) A1 = A0;
-    if (role == 1)
-        hipLaunchKernelGGL((k_bgemm<T, 1>), dim3(nitems), dim3(256), 0, s, items, terms, A0, A1, tab, vec);
-    else if (role == 2)
-        hipLaunchKernelGGL((k_bgemm<T, 2>), dim3(nitems), dim3(256), 0, s, items, terms, A0, A1, tab, vec);
-    else hipLaunchKernelGGL((k_bgemm<T, 0>), dim3(nitems), dim3(256), 0, s, items, terms, A0, A1, tab, vec);
+    if (role == 1) launch_bgemm_r<T, 1>(items, nitems, terms, A0, A1, tab, vec, per_wave, s);
+    else if (role == 2) launch_bgemm_r<T, 2>(items, nitems, terms, A0, A1, tab, vec, per_wave, s);
+    else launch_bgemm_r<T, 0>(items, nitems, terms, A0, A1, tab, vec, per_wave, s);
 }
 void launch_bgemm64(const GItem *items, int nitems, const GTerm *terms, const double *A0, const double *A1,
-                    double *vec, int role, hipStream_t s) {
-    launch_bgemm_t<double>(items, nitems, terms, A0, A1, nullptr, vec, role, s);
+                    double *vec, int role, bool per_wave, hipStream_t s) {
+    launch_bgemm_t<double>(items, nitems, terms, A0, A1, nullptr, vec, role, per_wave, s);
 }
 void launch_bgemm32(const GItem *items, int nitems, const GTerm *terms, const float *A0, const float *A1, float *vec,
-                    int role, hipStream_t s) {
-    launch_bgemm_t<float>(items, nitems, terms, A0, A1, nullptr, vec, role, s);
+                    int role, bool per_wave, hipStream_t s) {
+    launch_bgemm_t<float>(items, nitems, terms, A0, A1, nullptr, vec, role, per_wave, s);
 }
 void launch_bgemm64_tab(const GItem *items, int nitems, const GTerm *terms, const double *const *tab, double *vec,
-                        int role, hipStream_t s) {
-    launch_bgemm_t<double>(items, nitems, terms, nullptr, nullptr, tab, vec, role, s);
+                        int role, bool per_wave, hipStream_t s) {
+    launch_bgemm_t<double>(items, nitems, terms, nullptr, nullptr, tab, vec, role, per_wave, s);
 }
 void launch_bgemm32_tab(const GItem *items, int nitems, const GTerm *terms, const float *const *tab, float *vec,
-                        int role, hipStream_t s) {
-    launch_bgemm_t<float>(items, nitems, terms, nullptr, nullptr, tab, vec, role, s);
+                        int role, bool per_wave, hipStream_t s) {
+    launch_bgemm_t<float>(items, nitems, terms, nullptr, nullptr, tab, vec, role, per_wave, s);
 }
 void launch_gather_m64(const double *X, int64_t ldx, int nrhs, const int32_t *perm, int64_t N, double *dst,
                        hipStream_t s) {

@@ -45,13 +45,13 @@ void run_launch_m(Handle &h, const Launch &Lh, int precision, bool trans) {
                              h.vec64m.p, trans_narrow(Lh, false), s);
     } else if (precision == H2W_FP32_INNER) {
         launch_bgemm32(h.inner.d_items.p + Lh.item0, (int)Lh.nitems, h.inner.d_terms.p, h.mat32.p, h.mat32in.p,
-                       h.vec32m.p, Lh.role, s);
+                       h.vec32m.p, Lh.role, per_wave_items(Lh), s);
     } else if (precision == H2W_FP32) {
         launch_bgemm32(h.d_items.p + Lh.item0, (int)Lh.nitems, h.d_terms.p, h.mat32.p, nullptr, h.vec32m.p, Lh.role,
-                       s);
+                       per_wave_items(Lh), s);
     } else {
         launch_bgemm64(h.d_items.p + Lh.item0, (int)Lh.nitems, h.d_terms.p, h.mat64.p, nullptr, h.vec64m.p, Lh.role,
-                       s);
+                       per_wave_items(Lh), s);
     }
 }
 
