@@ -1983,14 +1983,25 @@ h2w_gmres_stats solve_tree(Handle &h, const double *bt, int max_iter, double tol
         S.matvecs_fp64 = o.matvecs;
         S.converged = o.conv;
     } else {
-        // FP64 iterative refinement: r = b - A x in FP64, correction from GMRES on the FP32 operator
+        // FP64 iterative refinement: r = b - A x in FP64, correction from GMRES on the FP32 operator.
+        // The first residual is the FP32 operator's own: the first correction then solves the
+        // FP32 system itself (x1 = A32^-1 b to the inner tolerance, as a low-precision first
+        // solve), which is as close to the solution as a correction of the FP64 residual of x0
+        // would be, one FP64 product earlier.  Convergence is only ever decided on an FP64
+        // residual.
         double normb = dev_norm(h, bt);
         if (normb == 0.0) normb = 1.0;
         int total = 0, outer = 0;
         double rn = 0.0;
+        bool fp64_resid = max_iter < 1;  // (no correction allowed: report x0's FP64 residual)
         while (true) {
-            matvec_dev(h, h.g_x.p, h.g_d.p, H2W_FP64, true);
-            S.matvecs_fp64++;
+            if (fp64_resid) {
+                matvec_dev(h, h.g_x.p, h.g_d.p, H2W_FP64, true);
+                S.matvecs_fp64++;
+            } else {
+                matvec_dev(h, h.g_x.p, h.g_d.p, inner_prec, true);
+                S.matvecs_fp32++;
+            }
             if (outer == 0) {
                 // start from alpha x0 with alpha minimising |b - alpha A x0| (no extra product):
                 // the first correction then starts from a residual no larger than b
@@ -2005,7 +2016,7 @@ h2w_gmres_stats solve_tree(Handle &h, const double *bt, int max_iter, double tol
             launch_axpby(-1.0, h.g_d.p, 1.0, h.g_r.p, N, s);
             double nr = dev_norm(h, h.g_r.p);
             rn = nr / normb;
-            if (rn <= tol || total >= max_iter || outer >= 50) break;
+            if (fp64_resid && (rn <= tol || total >= max_iter || outer >= 50)) break;
             const double floor_eta = mode == H2W_GMRES_FP32_INNER_IR ? std::pow(10.0, -h.ir_exp) : 1e-6;
             double eta = std::min(1e-2, std::max(floor_eta, 0.1 * tol / rn));
             launch_fill(h.g_d.p, N, 0.0, s);
@@ -2014,7 +2025,8 @@ h2w_gmres_stats solve_tree(Handle &h, const double *bt, int max_iter, double tol
             S.matvecs_fp32 += o.matvecs;
             launch_axpby(1.0, h.g_d.p, 1.0, h.g_x.p, N, s);
             ++outer;
-            if (o.iters == 0) break;
+            if (o.iters == 0 && fp64_resid) break;
+            fp64_resid = true;
         }
         S.iterations = total;
         S.outer_iterations = outer;

@@ -345,7 +345,9 @@ void lock_solve(Lockstep &L, const double *B, double *X, int nrhs, int max_iter,
             }
     } else {
         // FP64 refinement per column: r = b - A x on the stored FP64 operators, corrections from
-        // lockstep GMRES on the FP32 (inner) operators; converged columns drop out (zero residual)
+        // lockstep GMRES on the FP32 (inner) operators; converged columns drop out (zero residual).
+        // The first residual is the FP32 operators' own (the first correction solves the FP32
+        // systems themselves); convergence is only decided on FP64 residuals.
         LockBufs &Bf = *L.buf;
         Bf.d.ensure(len);
         Bf.b.ensure(len);
@@ -355,9 +357,10 @@ void lock_solve(Lockstep &L, const double *B, double *X, int nrhs, int max_iter,
         for (int c = 0; c < C; ++c) normb[c] = nb2[c] > 0.0 ? std::sqrt(nb2[c]) : 1.0;
         int outer = 0, total = 0, mv64 = 0, mv32 = 0;
         std::vector<char> run(C, 0);
+        bool fp64_resid = max_iter < 1;  // (no correction allowed: report x0's FP64 residual)
         while (true) {
-            L.matvec(X, d, H2W_FP64);
-            ++mv64;
+            L.matvec(X, d, fp64_resid ? H2W_FP64 : inner_prec);
+            ++(fp64_resid ? mv64 : mv32);
             if (outer == 0) {
                 // x0 -> alpha_c x0 with alpha_c minimising |b_c - alpha_c A x0_c|
                 ldots(L, d, d, dd.data());
@@ -375,14 +378,14 @@ void lock_solve(Lockstep &L, const double *B, double *X, int nrhs, int max_iter,
             int nrun = 0;
             for (int c = 0; c < C; ++c) {
                 rn[c] = std::sqrt(rr[c]) / normb[c];
-                run[c] = active[c] && rn[c] > tol && total < max_iter;
+                run[c] = active[c] && (!fp64_resid || (rn[c] > tol && total < max_iter));
                 nrun += run[c];
-                if (active[c]) {
+                if (active[c] && fp64_resid) {
                     stat(c).resid = rn[c];
                     stat(c).converged = rn[c] <= tol;
                 }
             }
-            if (nrun == 0 || outer >= 50) break;
+            if (nrun == 0 || (fp64_resid && outer >= 50)) break;
             for (int c = 0; c < C; ++c) {
                 const double fl = mode == H2W_GMRES_FP32_INNER_IR ? L.ir_floor[c / kNR] : 1e-6;
                 nr[c] = std::sqrt(rr[c]);
@@ -403,7 +406,8 @@ void lock_solve(Lockstep &L, const double *B, double *X, int nrhs, int max_iter,
             mv32 += o.matvecs;
             launch_axpby(1.0, d, 1.0, X, len, s);
             ++outer;
-            if (most == 0) break;
+            if (most == 0 && fp64_resid) break;
+            fp64_resid = true;
         }
         for (int c = 0; c < C; ++c)
             if (active[c]) {
