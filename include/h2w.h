@@ -115,7 +115,9 @@ typedef struct {
 
 #define H2W_GMRES_FP64 0         /* unrestarted GMRES, stored FP64 operators                          */
 #define H2W_GMRES_FP32_IR 1      /* FP64 iterative refinement (stored FP64 residuals) around GMRES on
-                                    the stored FP32 operators                                         */
+                                    the stored FP32 operators; the first correction starts from the
+                                    FP32 operator's residual of x0 (convergence is decided on FP64
+                                    residuals only)                                                   */
 #define H2W_GMRES_FP32_OTF_IR 2  /* as H2W_GMRES_FP32_IR with the inner operator's near-field and M2L
                                     blocks evaluated on the fly in FP32                               */
 #define H2W_GMRES_FP32_INNER_IR 3 /* as H2W_GMRES_FP32_IR with the truncated FP32 inner operator     */
