@@ -223,3 +223,28 @@ def test_bench_helpers(built):
         assert m["launch_bytes"][0] > r["launch_bytes"][r["launch_role"].index(1)]
     b = h2w.Batch([g]).bench_matvec(2, h2w.FP64)
     assert b["ms_per_pass"] > 0 and len(b["launch_ms"]) == len(b["launch_role"]) > 0
+
+
+@pytest.mark.parametrize("mode", [h2w.GMRES_FP32_IR, h2w.GMRES_FP32_INNER_IR])
+def test_refinement_reports_fp64_residuals(built, mode):
+    """The refinement's first correction starts from the FP32 operator's residual, but what it
+    reports and decides on is an FP64 residual: with too few iterations allowed, and when x0
+    already solves the system (max_iter 0 is refused, as for FP64 GMRES)."""
+    o, g, N = built(CASES[6])
+    xt = np.random.default_rng(5).normal(size=N)
+    b = g.mat_vec_prod(xt)
+    x0 = np.ones(N)
+
+    def true_resid(x):
+        return np.linalg.norm(b - g.mat_vec_prod(x)) / np.linalg.norm(b)
+
+    with pytest.raises(h2w.H2WError) as e:
+        g.gmres(b, x0, 0, 1e-10, mode)
+    assert e.value.code == h2w.H2W_EINVAL
+    for its in (1, 3):
+        x, s = g.gmres(b, x0, its, 1e-10, mode)
+        assert s["iterations"] <= its and not s["converged"] and s["matvecs_fp64"] >= 1
+        assert s["resid"] == pytest.approx(true_resid(x), rel=1e-6)
+    x, s = g.gmres(b, xt, 300, 1e-10, mode)
+    assert s["converged"] and true_resid(x) <= 2e-10
+    assert np.linalg.norm(x - xt) <= 1e-8 * np.linalg.norm(xt)
