@@ -261,6 +261,7 @@ struct h2w_handle {
     // staging for host-pointer calls and GMRES
     h2w::DevBuf<double> d_stage_in, d_stage_out, d_dense;
     h2w::DevBuf<double> g_V, g_w, g_r, g_b, g_x, g_h, g_part, g_d;
+    h2w::DevBuf<double> g_U, g_coef;  // recycled refinement space: U (k columns), combination coefficients
     int64_t g_Vcols = 0;
     // kNR-wide lockstep GMRES (interleaved N x kNR vectors)
     h2w::LockBufs mbuf;

@@ -1841,6 +1841,14 @@ struct GmresOut {
     double resid = 0.0;
     bool conv = false;
     int matvecs = 0;
+    std::vector<std::vector<double>> H;  // Hessenberg columns (rows 0..j+1) of a plain solve
+};
+
+// A space kept from one correction of the refinement for the next ones (same operator):
+// C = A U with orthonormal C, k columns; C sits in the first k columns of the GMRES basis
+// buffer, so that one fused pass orthogonalises against C and the new Arnoldi vectors.
+struct Recycled {
+    int k = 0;
 };
 
 void ensure_V(Handle &h, int64_t cols) {
@@ -1885,7 +1893,7 @@ double dev_dot(Handle &h, const double *a, const double *b) {
 // Solves A x = b (tree order, device).  x_zero: x0 = 0 and no initial matvec.
 // normb_in > 0 overrides ||b|| in the stopping test.
 GmresOut gmres_core(Handle &h, const double *b, double *x, int max_iter, double tol, int prec, bool x_zero,
-                    double normb_in) {
+                    double normb_in, const Recycled *rc = nullptr, bool keep_h = false) {
     const int64_t N = h.N;
     hipStream_t s = h.stream;
     GmresOut out;
@@ -1893,6 +1901,13 @@ GmresOut gmres_core(Handle &h, const double *b, double *x, int max_iter, double 
     h.g_b.ensure(N);  // residual of this solve
     double *w = h.g_w.p, *r = h.g_b.p;
     double normb = normb_in > 0 ? normb_in : dev_norm(h, b);
+    // recycled space (x starts at zero): x = U C^T r, r -= C C^T r, and every Arnoldi vector is
+    // orthogonalised against C as well (its coefficients B = C^T A v_j enter the update of x)
+    const int kc = rc && x_zero ? rc->k : 0;
+    const int m = max_iter;
+    const int nb = mdot_blocks(N);
+    h.g_part.ensure((size_t)nb * (kc + m + 2));
+    h.g_h.ensure((size_t)2 * (kc + m + 2) + 2);
     if (x_zero) {
         H2W_HIP(hipMemcpyAsync(r, b, N * 8, hipMemcpyDeviceToDevice, s));
     } else {
@@ -1901,6 +1916,12 @@ GmresOut gmres_core(Handle &h, const double *b, double *x, int max_iter, double 
         H2W_HIP(hipMemcpyAsync(r, b, N * 8, hipMemcpyDeviceToDevice, s));
         launch_axpby(-1.0, w, 1.0, r, N, s);
     }
+    if (kc > 0) {
+        launch_mdot(h.g_V.p, N, kc, r, N, h.g_part.p, nb, s);
+        launch_mdot_reduce(h.g_part.p, nb, kc, h.g_h.p, s);
+        launch_mgemv_sub(h.g_V.p, N, kc, h.g_h.p, r, N, s);
+        launch_vec_update(h.g_U.p, N, kc, h.g_h.p, x, N, s);
+    }
     double beta = dev_norm(h, r);
     if (normb == 0.0) normb = 1;
     out.resid = beta / normb;
@@ -1908,36 +1929,38 @@ GmresOut gmres_core(Handle &h, const double *b, double *x, int max_iter, double 
         out.conv = true;
         return out;
     }
-    const int m = max_iter;
     std::vector<std::vector<double>> H;  // Hessenberg columns as they are produced (rows 0..j+1)
+    std::vector<std::vector<double>> Bc; // (recycled) C^T A v_j
     std::vector<double> sv(m + 1, 0.0), cs(m + 1, 0.0), sn(m + 1, 0.0);
     auto Hh = [&](int i, int j) -> double & { return H[j][i]; };
-    ensure_V(h, std::min<int64_t>(m + 1, 32));
-    launch_axpby(1.0 / beta, r, 0.0, h.g_V.p, N, s);
+    ensure_V(h, kc + std::min<int64_t>(m + 1, 32));
+    launch_axpby(1.0 / beta, r, 0.0, h.g_V.p + (int64_t)kc * N, N, s);
     sv[0] = beta;
-    const int nb = mdot_blocks(N);
-    h.g_part.ensure((size_t)nb * (m + 2));
-    h.g_h.ensure((size_t)2 * (m + 2) + 2);
-    std::vector<double> hh(2 * (m + 2) + 2);
+    std::vector<double> hh(2 * (kc + m + 2) + 2);
     for (int i = 0; i < m; ++i) {
-        ensure_V(h, i + 2);
-        double *V = h.g_V.p;
+        ensure_V(h, kc + i + 2);
+        double *C = h.g_V.p, *V = h.g_V.p + (int64_t)kc * N;
         matvec_dev(h, V + (int64_t)i * N, w, prec, true);
         out.matvecs++;
-        const int k = i + 1;
-        launch_mdot(V, N, k, w, N, h.g_part.p, nb, s);
+        const int k = kc + i + 1;  // vectors orthogonalised against: C, v_0 .. v_i (contiguous)
+        launch_mdot(C, N, k, w, N, h.g_part.p, nb, s);
         launch_mdot_reduce(h.g_part.p, nb, k, h.g_h.p, s);
-        launch_mgemv_sub(V, N, k, h.g_h.p, w, N, s);
-        launch_mdot(V, N, k, w, N, h.g_part.p, nb, s);
+        launch_mgemv_sub(C, N, k, h.g_h.p, w, N, s);
+        launch_mdot(C, N, k, w, N, h.g_part.p, nb, s);
         launch_mdot_reduce(h.g_part.p, nb, k, h.g_h.p + k, s);
-        launch_mgemv_sub(V, N, k, h.g_h.p + k, w, N, s);
+        launch_mgemv_sub(C, N, k, h.g_h.p + k, w, N, s);
         launch_mdot(w, N, 1, w, N, h.g_part.p, nb, s);
         launch_mdot_reduce(h.g_part.p, nb, 1, h.g_h.p + 2 * k, s);
         H2W_HIP(hipMemcpyAsync(hh.data(), h.g_h.p, (2 * k + 1) * 8, hipMemcpyDeviceToHost, s));
         H2W_HIP(hipStreamSynchronize(s));
+        if (kc > 0) {
+            Bc.emplace_back(kc);
+            for (int q = 0; q < kc; ++q) Bc.back()[q] = hh[q] + hh[k + q];
+        }
         H.emplace_back(i + 2, 0.0);
-        for (int q = 0; q < k; ++q) Hh(q, i) = hh[q] + hh[k + q];
+        for (int q = 0; q <= i; ++q) Hh(q, i) = hh[kc + q] + hh[k + kc + q];
         Hh(i + 1, i) = std::sqrt(hh[2 * k]);
+        if (keep_h) out.H.push_back(H.back());  // (before the rotations)
         launch_axpby(1.0 / Hh(i + 1, i), w, 0.0, V + (int64_t)(i + 1) * N, N, s);
         for (int q = 0; q < i; ++q) app_rot(Hh(q, i), Hh(q + 1, i), cs[q], sn[q]);
         gen_rot(Hh(i, i), Hh(i + 1, i), cs[i], sn[i]);
@@ -1952,14 +1975,82 @@ GmresOut gmres_core(Handle &h, const double *b, double *x, int max_iter, double 
                 y[q] /= Hh(q, q);
                 for (int jj = q - 1; jj >= 0; jj--) y[jj] -= Hh(jj, q) * y[q];
             }
-            H2W_HIP(hipMemcpyAsync(h.g_h.p, y.data(), (i + 1) * 8, hipMemcpyHostToDevice, s));
+            // x += V y - U (B y)
+            std::vector<double> cy(y);
+            if (kc > 0) {
+                cy.resize(i + 1 + kc, 0.0);
+                for (int jj = 0; jj <= i; ++jj)
+                    for (int q = 0; q < kc; ++q) cy[i + 1 + q] -= Bc[jj][q] * y[jj];
+            }
+            H2W_HIP(hipMemcpyAsync(h.g_h.p, cy.data(), cy.size() * 8, hipMemcpyHostToDevice, s));
             launch_vec_update(V, N, i + 1, h.g_h.p, x, N, s);
+            if (kc > 0) launch_vec_update(h.g_U.p, N, kc, h.g_h.p + i + 1, x, N, s);
             H2W_HIP(hipStreamSynchronize(s));
             out.conv = out.resid < tol;
             return out;
         }
     }
     return out;
+}
+
+// The space of a finished correction kept for the next ones: with the Arnoldi relation
+// A V_j = V_{j+1} Hbar and Hbar = Q R (thin QR, host), U = V_j R^-1 and C = V_{j+1} Q satisfy
+// A U = C with orthonormal C.  C replaces the first j basis columns, U goes to g_U.  Returns
+// false (nothing kept) for fewer than two iterations or a near-singular R.
+bool keep_space(Handle &h, const GmresOut &o, Recycled &rc) {
+    rc.k = 0;
+    const int j = (int)o.H.size();
+    if (j < 2 || o.iters != j) return false;
+    const int64_t N = h.N;
+    hipStream_t s = h.stream;
+    // Hbar columns (j+1 rows), Gram-Schmidt twice in double
+    std::vector<std::vector<double>> Q(j, std::vector<double>(j + 1, 0.0));
+    std::vector<std::vector<double>> R(j, std::vector<double>(j, 0.0));  // R[col][row]
+    for (int c = 0; c < j; ++c) {
+        std::vector<double> v(j + 1, 0.0);
+        for (int q = 0; q < (int)o.H[c].size() && q <= j; ++q) v[q] = o.H[c][q];
+        for (int pass = 0; pass < 2; ++pass)
+            for (int p = 0; p < c; ++p) {
+                double d = 0.0;
+                for (int q = 0; q <= j; ++q) d += Q[p][q] * v[q];
+                for (int q = 0; q <= j; ++q) v[q] -= d * Q[p][q];
+                R[c][p] += d;
+            }
+        double nv = 0.0;
+        for (int q = 0; q <= j; ++q) nv += v[q] * v[q];
+        nv = std::sqrt(nv);
+        const double ref = c == 0 ? nv : R[0][0];
+        if (!std::isfinite(nv) || !(nv > 1e-12 * ref)) return false;
+        R[c][c] = nv;
+        for (int q = 0; q <= j; ++q) Q[c][q] = v[q] / nv;
+    }
+    // Rinv (upper triangular), column by column: R Rinv[:, c] = e_c
+    std::vector<std::vector<double>> Ri(j, std::vector<double>(j, 0.0));
+    for (int c = 0; c < j; ++c)
+        for (int q = c; q >= 0; --q) {
+            double v = q == c ? 1.0 : 0.0;
+            for (int p = q + 1; p <= c; ++p) v -= R[p][q] * Ri[c][p];
+            Ri[c][q] = v / R[q][q];
+        }
+    // coefficients: C_c = V Q[:, c] (j+1 terms), U_c = V Ri[:, c] (j terms, padded to j+1)
+    std::vector<double> coef((size_t)2 * j * (j + 1), 0.0);
+    for (int c = 0; c < j; ++c)
+        for (int q = 0; q <= j; ++q) {
+            coef[(size_t)c * (j + 1) + q] = Q[c][q];
+            coef[(size_t)(j + c) * (j + 1) + q] = q < j ? Ri[c][q] : 0.0;
+        }
+    h.g_coef.ensure(coef.size());
+    H2W_HIP(hipMemcpyAsync(h.g_coef.p, coef.data(), coef.size() * 8, hipMemcpyHostToDevice, s));
+    h.g_U.ensure((size_t)2 * j * N);  // U, then C before it moves into the basis
+    launch_fill(h.g_U.p, 2 * j * N, 0.0, s);
+    for (int c = 0; c < j; ++c) {
+        launch_vec_update(h.g_V.p, N, j + 1, h.g_coef.p + (size_t)c * (j + 1), h.g_U.p + (int64_t)(j + c) * N, N, s);
+        launch_vec_update(h.g_V.p, N, j + 1, h.g_coef.p + (size_t)(j + c) * (j + 1), h.g_U.p + (int64_t)c * N, N, s);
+    }
+    H2W_HIP(hipMemcpyAsync(h.g_V.p, h.g_U.p + (int64_t)j * N, (size_t)j * N * 8, hipMemcpyDeviceToDevice, s));
+    H2W_HIP(hipGetLastError());
+    rc.k = j;
+    return true;
 }
 
 }  // namespace
@@ -1994,6 +2085,7 @@ h2w_gmres_stats solve_tree(Handle &h, const double *bt, int max_iter, double tol
         int total = 0, outer = 0;
         double rn = 0.0;
         bool fp64_resid = max_iter < 1;  // (no correction allowed: report x0's FP64 residual)
+        Recycled rc;
         while (true) {
             if (fp64_resid) {
                 matvec_dev(h, h.g_x.p, h.g_d.p, H2W_FP64, true);
@@ -2020,7 +2112,10 @@ h2w_gmres_stats solve_tree(Handle &h, const double *bt, int max_iter, double tol
             const double floor_eta = mode == H2W_GMRES_FP32_INNER_IR ? std::pow(10.0, -h.ir_exp) : 1e-6;
             double eta = std::min(1e-2, std::max(floor_eta, 0.1 * tol / rn));
             launch_fill(h.g_d.p, N, 0.0, s);
-            auto o = gmres_core(h, h.g_r.p, h.g_d.p, max_iter - total, eta, inner_prec, true, nr);
+            // later corrections (same FP32 operator) recycle the first one's Krylov space
+            auto o = gmres_core(h, h.g_r.p, h.g_d.p, max_iter - total, eta, inner_prec, true, nr,
+                                rc.k > 0 ? &rc : nullptr, outer == 0);
+            if (outer == 0) keep_space(h, o, rc);
             total += o.iters;
             S.matvecs_fp32 += o.matvecs;
             launch_axpby(1.0, h.g_d.p, 1.0, h.g_x.p, N, s);

@@ -58,11 +58,13 @@ def test_batched_solves(batch):
     Xt = [rng.normal(size=(4, g.N)) for g in sys_]
     Bs = [g.matvec_multi(x) for g, x in zip(sys_, Xt)]
     gtol = 1e-8
+    # accuracy reference: each column's own single FP64 solve to the same tolerance
+    e64 = [[_rel(g.gmres(Bs[s][c], None, 300, gtol, h2w.GMRES_FP64)[0], Xt[s][c]) for c in range(4)]
+           for s, g in enumerate(sys_)]
     for mode in (h2w.GMRES_FP64, h2w.GMRES_FP32_IR, h2w.GMRES_FP32_INNER_IR):
         Xs, st = B.gmres(Bs, None, 300, gtol, mode)
         for s, g in enumerate(sys_):
             for c in range(4):
                 assert st[s][c]["converged"], (mode, s, c, st[s][c])
                 assert _rel(g.mat_vec_prod(Xs[s][c]), Bs[s][c]) <= 2 * gtol
-                x1, _ = g.gmres(Bs[s][c], None, 300, gtol, mode)
-                assert _rel(Xs[s][c], Xt[s][c]) <= 10 * _rel(x1, Xt[s][c]) + 1e-12, (mode, s, c)
+                assert _rel(Xs[s][c], Xt[s][c]) <= 10 * e64[s][c] + 1e-12, (mode, s, c)

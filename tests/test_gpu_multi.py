@@ -71,16 +71,17 @@ def test_multi_rhs_gmres(built, case):
     Xt = rng.normal(size=(5, N))
     B = np.stack([o.dense_matvec(x) for x in Xt])
     gtol = 1e-10 if tp >= 10 else 1e-8
+    single64 = [g.gmres(B[c], None, 300, gtol, h2w.GMRES_FP64) for c in range(5)]
     for mode in (h2w.GMRES_FP64, h2w.GMRES_FP32_IR, h2w.GMRES_FP32_INNER_IR):
         if mode == h2w.GMRES_FP32_INNER_IR and tp <= 6:
             continue
         X, st = g.gmres_multi(B, None, 300, gtol, mode)
         for c in range(5):
-            x1, s1 = g.gmres(B[c], None, 300, gtol, mode)
+            x1, s1 = single64[c]
             assert st[c]["converged"], (mode, c, st[c])
             # the true residual of every column meets the tolerance
             assert _rel(g.mat_vec_prod(X[c]), B[c]) <= 2 * gtol
-            # and the solution is as accurate as the single-vector solve's
+            # and the solution is as accurate as the single-vector FP64 solve's
             e1, em = _rel(x1, Xt[c]), _rel(X[c], Xt[c])
             assert em <= 10 * e1 + 1e-13, (mode, c, em, e1)
             if mode == h2w.GMRES_FP64:
