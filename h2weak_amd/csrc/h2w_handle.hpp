@@ -198,6 +198,8 @@ inline void app_rot(double &dx, double &dy, double cs, double sn) {
 struct LockBufs {
     DevBuf<double> V, w, r, d, b, part, h, sc;
     int64_t Vcols = 0;
+    DevBuf<double> U, coef;  // recycled refinement space (U; C sits in the first rk basis columns)
+    int rk = 0;
 };
 
 }  // namespace h2w

@@ -193,9 +193,13 @@ struct LockOut {
 // substitution) per column, one batched product per step for all columns.
 // Column c stops at its own tolerance tol[c]; a stopped column's basis vectors
 // are zero from then on.  x_zero: x0 = 0 and no initial product; normb_in[c] > 0
-// overrides |b_c|.
+// overrides |b_c|.  kc > 0 (with x_zero): recycle the space kept by lock_keep_space
+// (C in the first kc basis columns, U in buf->U), as the single-vector core does.
+// hkeep: receives the Hessenberg columns before the rotations, and every running
+// column's last basis vector is kept, for lock_keep_space.
 LockOut lock_gmres(Lockstep &L, const double *b, double *x, const std::vector<char> &active, int max_iter,
-                   const std::vector<double> &tol, int prec, bool x_zero, const double *normb_in) {
+                   const std::vector<double> &tol, int prec, bool x_zero, const double *normb_in, int kc = 0,
+                   std::vector<std::vector<double>> *hkeep = nullptr) {
     const int C = L.S * kNR;
     const int64_t len = L.n * kNR;
     hipStream_t s = L.s;
@@ -219,6 +223,17 @@ LockOut lock_gmres(Lockstep &L, const double *b, double *x, const std::vector<ch
         L.matvec(x, w, prec);
         out.matvecs++;
         launch_axpby(-1.0, w, 1.0, r, len, s);
+        kc = 0;
+    }
+    const int m = max_iter;
+    B.part.ensure((size_t)L.nblk * (kc + m + 2) * kNR);
+    B.h.ensure((size_t)(2 * (kc + m + 2) + 2) * C);
+    if (kc > 0) {
+        // x = U C^T r, r -= C C^T r (per column)
+        launch_bdot16(B.V.p, len, kc, r, L.blk, L.nblk, B.part.p, s);
+        launch_bdot16_reduce(B.part.p, L.nblk, kc, L.sblk, L.S, B.h.p, s);
+        launch_bgemv16(B.V.p, len, kc, B.h.p, L.S, -1.0, r, L.n, L.seg, s);
+        launch_bgemv16(B.U.p, len, kc, B.h.p, L.S, 1.0, x, L.n, L.seg, s);
     }
     ldots(L, r, r, tmp.data());
     std::vector<char> run(C, 0);
@@ -230,35 +245,36 @@ LockOut lock_gmres(Lockstep &L, const double *b, double *x, const std::vector<ch
         out.conv[c] = active[c] && !run[c];
         nrun += run[c];
     }
-    if (nrun == 0) return out;
-    const int m = max_iter;
+    if (nrun == 0) {
+        H2W_HIP(hipStreamSynchronize(s));
+        return out;
+    }
     const size_t ld = (size_t)m + 1;
     std::vector<std::vector<double>> H;  // Hessenberg column j: rows 0..j+1 of every column c at i * C + c
     std::vector<double> sv(ld * C, 0.0), cs(ld * C, 0.0), sn(ld * C, 0.0);
     auto Hh = [&](int c, int i, int j) -> double & { return H[j][(size_t)i * C + c]; };
     std::vector<double> Y;  // update coefficients (j, c) at j * C + c
-    ensure_V(L, std::min<int64_t>(m + 1, 16));
+    std::vector<std::vector<double>> Bst;  // (recycled) C^T A v_i: (q, c) at q * C + c
+    ensure_V(L, kc + std::min<int64_t>(m + 1, 16));
     for (int c = 0; c < C; ++c) {
         tmp[c] = run[c] ? 1.0 / beta[c] : 0.0;
         sv[c * ld] = beta[c];
     }
-    lscale(L, tmp, r, B.V.p);
-    B.part.ensure((size_t)L.nblk * (m + 2) * kNR);
-    B.h.ensure((size_t)(2 * (m + 2) + 2) * C);
-    std::vector<double> hh((size_t)(2 * (m + 2) + 2) * C);
+    lscale(L, tmp, r, B.V.p + (int64_t)kc * len);
+    std::vector<double> hh((size_t)(2 * (kc + m + 2) + 2) * C);
     int last = 0;
     for (int i = 0; i < m && nrun > 0; ++i) {
-        ensure_V(L, i + 2);
-        double *V = B.V.p;
+        ensure_V(L, kc + i + 2);
+        double *Cb = B.V.p, *V = B.V.p + (int64_t)kc * len;
         L.matvec(V + (int64_t)i * len, w, prec);
         out.matvecs++;
-        const int k = i + 1;
-        launch_bdot16(V, len, k, w, L.blk, L.nblk, B.part.p, s);
+        const int k = kc + i + 1;  // vectors orthogonalised against: C, v_0 .. v_i (contiguous)
+        launch_bdot16(Cb, len, k, w, L.blk, L.nblk, B.part.p, s);
         launch_bdot16_reduce(B.part.p, L.nblk, k, L.sblk, L.S, B.h.p, s);
-        launch_bgemv16(V, len, k, B.h.p, L.S, -1.0, w, L.n, L.seg, s);
-        launch_bdot16(V, len, k, w, L.blk, L.nblk, B.part.p, s);
+        launch_bgemv16(Cb, len, k, B.h.p, L.S, -1.0, w, L.n, L.seg, s);
+        launch_bdot16(Cb, len, k, w, L.blk, L.nblk, B.part.p, s);
         launch_bdot16_reduce(B.part.p, L.nblk, k, L.sblk, L.S, B.h.p + (size_t)k * C, s);
-        launch_bgemv16(V, len, k, B.h.p + (size_t)k * C, L.S, -1.0, w, L.n, L.seg, s);
+        launch_bgemv16(Cb, len, k, B.h.p + (size_t)k * C, L.S, -1.0, w, L.n, L.seg, s);
         launch_bdot16(w, 0, 1, w, L.blk, L.nblk, B.part.p, s);
         launch_bdot16_reduce(B.part.p, L.nblk, 1, L.sblk, L.S, B.h.p + (size_t)2 * k * C, s);
         H2W_HIP(hipMemcpyAsync(hh.data(), B.h.p, (size_t)(2 * k + 1) * C * 8, hipMemcpyDeviceToHost, s));
@@ -266,12 +282,19 @@ LockOut lock_gmres(Lockstep &L, const double *b, double *x, const std::vector<ch
         last = i;
         H.emplace_back((size_t)(i + 2) * C, 0.0);
         Y.resize((size_t)(i + 1) * C, 0.0);
+        if (kc > 0) Bst.emplace_back((size_t)kc * C, 0.0);
         for (int c = 0; c < C; ++c) {
             tmp[c] = 0.0;
             if (!run[c]) continue;
-            for (int q = 0; q < k; ++q) Hh(c, q, i) = hh[(size_t)q * C + c] + hh[(size_t)(k + q) * C + c];
+            for (int q = 0; q < kc; ++q) Bst.back()[(size_t)q * C + c] = hh[(size_t)q * C + c] + hh[(size_t)(k + q) * C + c];
+            for (int q = 0; q <= i; ++q)
+                Hh(c, q, i) = hh[(size_t)(kc + q) * C + c] + hh[(size_t)(k + kc + q) * C + c];
             Hh(c, i + 1, i) = std::sqrt(hh[(size_t)2 * k * C + c]);
             if (Hh(c, i + 1, i) > 0.0) tmp[c] = 1.0 / Hh(c, i + 1, i);
+        }
+        if (hkeep) hkeep->push_back(H.back());  // (before the rotations)
+        for (int c = 0; c < C; ++c) {
+            if (!run[c]) continue;
             double *csc = &cs[c * ld], *snc = &sn[c * ld], *svc = &sv[c * ld];
             for (int q = 0; q < i; ++q) app_rot(Hh(c, q, i), Hh(c, q + 1, i), csc[q], snc[q]);
             gen_rot(Hh(c, i, i), Hh(c, i + 1, i), csc[i], snc[i]);
@@ -288,17 +311,102 @@ LockOut lock_gmres(Lockstep &L, const double *b, double *x, const std::vector<ch
                 for (int q = 0; q <= i; ++q) Y[(size_t)q * C + c] = y[q];
                 out.conv[c] = out.resid[c] < tol[c];
                 run[c] = 0;
-                tmp[c] = 0.0;
+                if (!hkeep) tmp[c] = 0.0;  // (kept: the column's last basis vector enters C)
                 --nrun;
             }
         }
-        if (nrun > 0) lscale(L, tmp, w, V + (int64_t)(i + 1) * len);
+        if (nrun > 0 || hkeep) lscale(L, tmp, w, V + (int64_t)(i + 1) * len);
     }
-    // x += V Y over the basis vectors the columns used
+    // x += V Y over the basis vectors the columns used, - U (B Y) with a recycled space
     H2W_HIP(hipMemcpyAsync(B.h.p, Y.data(), (size_t)(last + 1) * C * 8, hipMemcpyHostToDevice, s));
-    launch_bgemv16(B.V.p, len, last + 1, B.h.p, L.S, 1.0, x, L.n, L.seg, s);
+    launch_bgemv16(B.V.p + (int64_t)kc * len, len, last + 1, B.h.p, L.S, 1.0, x, L.n, L.seg, s);
+    if (kc > 0) {
+        std::vector<double> by((size_t)kc * C, 0.0);
+        for (int c = 0; c < C; ++c)
+            for (int jj = 0; jj < out.iters[c] && jj < (int)Bst.size(); ++jj)
+                for (int q = 0; q < kc; ++q) by[(size_t)q * C + c] -= Bst[jj][(size_t)q * C + c] * Y[(size_t)jj * C + c];
+        B.coef.ensure(by.size());
+        H2W_HIP(hipMemcpyAsync(B.coef.p, by.data(), by.size() * 8, hipMemcpyHostToDevice, s));
+        launch_bgemv16(B.U.p, len, kc, B.coef.p, L.S, 1.0, x, L.n, L.seg, s);
+    }
     H2W_HIP(hipStreamSynchronize(s));
     return out;
+}
+
+// The space of a finished lockstep correction kept for the next ones, per column as the
+// single-vector refinement does (U = V_j R^-1, C = V_{j+1} Q from Hbar = QR of the column's
+// Hessenberg matrix, j its iteration count).  Columns of fewer than two iterations or with a
+// near-singular R keep nothing (zero C and U); k is the largest j kept.
+void lock_keep_space(Lockstep &L, const std::vector<std::vector<double>> &Hk, const LockOut &o) {
+    LockBufs &B = *L.buf;
+    B.rk = 0;
+    const int C = L.S * kNR;
+    const int64_t len = L.n * kNR;
+    hipStream_t s = L.s;
+    std::vector<int> jc(C, 0);
+    std::vector<std::vector<std::vector<double>>> Qc(C), Ric(C);
+    int k = 0;
+    for (int c = 0; c < C; ++c) {
+        const int j = o.iters[c];
+        if (j < 2 || j > (int)Hk.size()) continue;
+        std::vector<std::vector<double>> Q(j, std::vector<double>(j + 1, 0.0)), R(j, std::vector<double>(j, 0.0));
+        bool ok = true;
+        for (int col = 0; col < j && ok; ++col) {
+            std::vector<double> v(j + 1, 0.0);
+            for (int q = 0; q <= col + 1; ++q) v[q] = Hk[col][(size_t)q * C + c];
+            for (int pass = 0; pass < 2; ++pass)
+                for (int p = 0; p < col; ++p) {
+                    double d = 0.0;
+                    for (int q = 0; q <= j; ++q) d += Q[p][q] * v[q];
+                    for (int q = 0; q <= j; ++q) v[q] -= d * Q[p][q];
+                    R[col][p] += d;
+                }
+            double nv = 0.0;
+            for (int q = 0; q <= j; ++q) nv += v[q] * v[q];
+            nv = std::sqrt(nv);
+            const double ref = col == 0 ? nv : R[0][0];
+            if (!std::isfinite(nv) || !(nv > 1e-12 * ref)) ok = false;
+            else {
+                R[col][col] = nv;
+                for (int q = 0; q <= j; ++q) Q[col][q] = v[q] / nv;
+            }
+        }
+        if (!ok) continue;
+        std::vector<std::vector<double>> Ri(j, std::vector<double>(j, 0.0));
+        for (int col = 0; col < j; ++col)
+            for (int q = col; q >= 0; --q) {
+                double v = q == col ? 1.0 : 0.0;
+                for (int p = q + 1; p <= col; ++p) v -= R[p][q] * Ri[col][p];
+                Ri[col][q] = v / R[q][q];
+            }
+        jc[c] = j;
+        Qc[c] = std::move(Q);
+        Ric[c] = std::move(Ri);
+        k = std::max(k, j);
+    }
+    if (k == 0) return;
+    // output vector p of C: coefficients (q, c) = Q_c[p][q] over basis vectors q <= k; of U: Ri_c[p][q]
+    const size_t per = (size_t)(k + 1) * C;
+    std::vector<double> coef((size_t)2 * k * per, 0.0);
+    for (int c = 0; c < C; ++c)
+        for (int p = 0; p < jc[c]; ++p)
+            for (int q = 0; q <= jc[c]; ++q) {
+                coef[(size_t)p * per + (size_t)q * C + c] = Qc[c][p][q];
+                if (q < jc[c]) coef[(size_t)(k + p) * per + (size_t)q * C + c] = Ric[c][p][q];
+            }
+    B.coef.ensure(coef.size());
+    H2W_HIP(hipMemcpyAsync(B.coef.p, coef.data(), coef.size() * 8, hipMemcpyHostToDevice, s));
+    B.U.ensure((size_t)2 * k * len);  // U, then C before it moves into the basis
+    launch_fill(B.U.p, 2 * k * len, 0.0, s);
+    for (int p = 0; p < k; ++p) {
+        launch_bgemv16(B.V.p, len, k + 1, B.coef.p + (size_t)p * per, L.S, 1.0, B.U.p + (int64_t)(k + p) * len, L.n,
+                       L.seg, s);
+        launch_bgemv16(B.V.p, len, k + 1, B.coef.p + (size_t)(k + p) * per, L.S, 1.0, B.U.p + (int64_t)p * len, L.n,
+                       L.seg, s);
+    }
+    H2W_HIP(hipMemcpyAsync(B.V.p, B.U.p + (int64_t)k * len, (size_t)k * len * 8, hipMemcpyDeviceToDevice, s));
+    H2W_HIP(hipStreamSynchronize(s));
+    B.rk = k;
 }
 
 }  // namespace
@@ -394,7 +502,11 @@ void lock_solve(Lockstep &L, const double *B, double *X, int nrhs, int max_iter,
             }
             lscale(L, tmp, r, r);  // converged columns get a zero right-hand side
             launch_fill(d, len, 0.0, s);
-            LockOut o = lock_gmres(L, r, d, run, max_iter - total, eta, inner_prec, true, nr.data());
+            // later corrections recycle the first one's Krylov space (per column)
+            std::vector<std::vector<double>> hk;
+            LockOut o = lock_gmres(L, r, d, run, max_iter - total, eta, inner_prec, true, nr.data(),
+                                   outer == 0 ? 0 : L.buf->rk, outer == 0 ? &hk : nullptr);
+            if (outer == 0) lock_keep_space(L, hk, o);
             int most = 0;
             for (int c = 0; c < C; ++c)
                 if (run[c]) {
