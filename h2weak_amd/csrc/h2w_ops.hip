@@ -2109,7 +2109,11 @@ h2w_gmres_stats solve_tree(Handle &h, const double *bt, int max_iter, double tol
             double nr = dev_norm(h, h.g_r.p);
             rn = nr / normb;
             if (fp64_resid && (rn <= tol || total >= max_iter || outer >= 50)) break;
-            const double floor_eta = mode == H2W_GMRES_FP32_INNER_IR ? std::pow(10.0, -h.ir_exp) : 1e-6;
+            // (the first correction, from the FP32 residual, stops at sqrt(tol): the FP64 residual
+            // after it is bounded by the FP32 operator's error anyway, and the second correction
+            // finishes from there)
+            double floor_eta = mode == H2W_GMRES_FP32_INNER_IR ? std::pow(10.0, -h.ir_exp) : 1e-6;
+            if (!fp64_resid) floor_eta = std::max(floor_eta, std::sqrt(tol));
             double eta = std::min(1e-2, std::max(floor_eta, 0.1 * tol / rn));
             launch_fill(h.g_d.p, N, 0.0, s);
             // later corrections (same FP32 operator) recycle the first one's Krylov space
@@ -2228,7 +2232,7 @@ void tune_inner(Handle &h, const double *bt, double tol, int max_iter, bool unro
                 S = S2;
             }
         }
-        const double score = worst * (rmax > tol / 3.0 ? 1.2 : 1.0);
+        const double score = worst * (rmax > tol / 2.0 ? 1.2 : 1.0);
         const int c = out->n_cand++;
         out->cand_ir_tol_exp[c] = p;
         out->cand_seconds[c] = worst;

@@ -495,7 +495,8 @@ void lock_solve(Lockstep &L, const double *B, double *X, int nrhs, int max_iter,
             }
             if (nrun == 0 || (fp64_resid && outer >= 50)) break;
             for (int c = 0; c < C; ++c) {
-                const double fl = mode == H2W_GMRES_FP32_INNER_IR ? L.ir_floor[c / kNR] : 1e-6;
+                double fl = mode == H2W_GMRES_FP32_INNER_IR ? L.ir_floor[c / kNR] : 1e-6;
+                if (outer == 0) fl = std::max(fl, std::sqrt(tol));  // (as the single-vector refinement)
                 nr[c] = std::sqrt(rr[c]);
                 eta[c] = run[c] ? std::min(1e-2, std::max(fl, 0.1 * tol / rn[c])) : 1.0;
                 tmp[c] = run[c] ? 1.0 : 0.0;
