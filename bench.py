@@ -351,6 +351,8 @@ def main():
     ap.add_argument("--pmc-probe", action="store_true",
                     help="assemble, then run 3 FP64 and 3 FP32-inner matvecs and a 1 GiB device copy "
                          "(the calibration kernel) for rocprofv3 --pmc; prints what it ran")
+    ap.add_argument("--inner-tol", type=float, default=6.5,
+                    help="inner-operator tolerance exponent of --pmc-probe (the tuner's usual choice at 64^3)")
     ap.add_argument("--pmc-reduce", metavar="DIR",
                     help="read DIR/fetch_counter_collection.csv and DIR/write_counter_collection.csv written by "
                          "two rocprofv3 --pmc runs of --pmc-probe and write profiles/pmc_traffic.json (no GPU)")
@@ -407,6 +409,7 @@ def main():
     log(f"assembly {time.time() - t0:.1f} s, M = {st['memory']:.4e} elements ({st['memory'] * 8e-9:.2f} GB fp64)")
     if args.pmc_probe:
         hip = Hip()
+        A.set_inner_tolerance(args.inner_tol)
         A.bench_matvec(3, h2w.FP64)
         A.bench_matvec(3, h2w.FP32_INNER)
         nb = 1 << 30
@@ -415,6 +418,7 @@ def main():
         hip.sync()
         r64, r32 = A.bench_matvec(1, h2w.FP64), A.bench_matvec(1, h2w.FP32_INNER)
         print(json.dumps({"pmc_probe": True, "copy_bytes": nb, "workload": f"3D RBF grid {side}^3 tol 1e-{tp}",
+                          "inner_tol_exp": args.inner_tol,
                           "N": N, "launch_bytes_fp64": r64["launch_bytes"], "launch_role_fp64": r64["launch_role"],
                           "launch_bytes_fp32_inner": r32["launch_bytes"],
                           "launch_role_fp32_inner": r32["launch_role"]}), flush=True)

@@ -119,7 +119,7 @@ def lib():
     L.h2w_destroy.argtypes = [p]
     L.h2w_last_error.restype = ctypes.c_char_p
     L.h2w_bench_matvec.argtypes = [p, i, i, ctypes.POINTER(BenchResult)]
-    L.h2w_set_inner_tolerance.argtypes = [p, i]
+    L.h2w_set_inner_tolerance.argtypes = [p, d]
     L.h2w_bench_matvec_multi.argtypes = [p, i, i, ctypes.POINTER(BenchResult)]
     L.h2w_batch_create.argtypes = [ctypes.POINTER(p), i, ctypes.POINTER(p)]
     L.h2w_batch_matvec.argtypes = [p, p, p, i, i, i]
@@ -339,8 +339,9 @@ class Initialisation:
         return out
 
     def set_inner_tolerance(self, ir_tol_pow):
-        """Rebuild the truncated FP32 inner operator for tolerance 10^-ir_tol_pow (0 removes it)."""
-        _check(lib().h2w_set_inner_tolerance(self._h, int(ir_tol_pow)))
+        """Rebuild the truncated FP32 inner operator for tolerance 10^-ir_tol_pow (0 removes it;
+        fractional exponents as the autotuner's half decades)."""
+        _check(lib().h2w_set_inner_tolerance(self._h, float(ir_tol_pow)))
 
     def autotune(self, b=None, tol=1e-10, max_iter=1000, flags=TUNE_UNROLL | TUNE_INNER):
         """GEMV unroll depth per launch and inner tolerance of the refinement (b in input order)."""

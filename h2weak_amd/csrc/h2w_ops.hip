@@ -2761,15 +2761,15 @@ int h2w_bench_matvec(h2w_handle *h, int nrep, int precision, h2w_bench *out) {
     H2W_CATCH
 }
 
-int h2w_set_inner_tolerance(h2w_handle *h, int ir_tol_pow) {
+int h2w_set_inner_tolerance(h2w_handle *h, double ir_tol_pow) {
     H2W_TRY
     if (!h) throw Error{H2W_EINVAL, "NULL handle"};
     if (!h->assembled) throw Error{H2W_ESTATE, "not assembled"};
     if (!h->mat32.p) throw Error{H2W_ESTATE, "the inner operator needs the FP32 shadow (opts.fp32_shadow)"};
-    if (ir_tol_pow < 0 || (ir_tol_pow > 0 && ir_tol_pow >= h->tol_pow))
+    if (!(ir_tol_pow >= 0.0) || (ir_tol_pow > 0.0 && ir_tol_pow >= h->tol_pow))
         throw Error{H2W_EINVAL, "ir_tol_pow must be 0 or in (0, tol_pow)"};
     H2W_HIP(hipSetDevice(h->opts.device));
-    h2w::build_inner(*h, (double)ir_tol_pow);
+    h2w::build_inner(*h, ir_tol_pow);
     return H2W_OK;
     H2W_CATCH
 }

@@ -202,9 +202,10 @@ int h2w_bench_matvec_multi(h2w_handle *h, int nrep, int precision, h2w_bench *ou
 int h2w_bench_stream_read(int device, int64_t bytes, int nrep, double *gbps);
 
 /* Rebuild the truncated FP32 inner operator of H2W_GMRES_FP32_INNER_IR for tolerance
- * 10^-ir_tol_pow (0 < ir_tol_pow < tol_pow; 0 removes it).  No counterpart in the reference
+ * 10^-ir_tol_pow (0 < ir_tol_pow < tol_pow, fractional exponents allowed as the autotuner's half
+ * decades; 0 removes it).  No counterpart in the reference
  * (which has one FP64 operator); the stored FP64 / FP32 operators are untouched. */
-int h2w_set_inner_tolerance(h2w_handle *h, int ir_tol_pow);
+int h2w_set_inner_tolerance(h2w_handle *h, double ir_tol_pow);
 
 /* Autotuner (no counterpart in the reference).  With H2W_TUNE_UNROLL it times every launch of
  * the FP64, FP32 and FP32-inner matvecs with 2, 4 and 8 column loads in flight per lane and
