@@ -33,20 +33,21 @@ def test_inner_tolerance_controls_truncation(pair):
     y64 = g.mat_vec_prod(x)
     assert _rel(y64, o.mat_vec_prod(x)) <= 1e-12
     sizes = []
-    for p in (4, 6, 8):
+    for p in (4, 6, 6.5, 8):  # (fractional exponents: the tuner's half decades)
         g.set_inner_tolerance(p)
         yin = g.mat_vec_prod(x, precision=h2w.FP32_INNER)
         assert _rel(yin, y64) <= 30 * 10.0 ** (-p) + 2e-6, p
         sizes.append(g.stats()["stored_bytes_inner_fp32"])
     # looser tolerance, fewer pivots kept
-    assert sizes[0] <= sizes[1] <= sizes[2]
+    assert sizes[0] <= sizes[1] <= sizes[2] <= sizes[3]
     g.set_inner_tolerance(0)
     with pytest.raises(h2w.H2WError) as e:
         g.mat_vec_prod(x, precision=h2w.FP32_INNER)
     assert e.value.code == h2w.H2W_ESTATE
-    with pytest.raises(h2w.H2WError) as e:
-        g.set_inner_tolerance(CASE[3])
-    assert e.value.code == h2w.H2W_EINVAL
+    for bad in (CASE[3], CASE[3] + 0.5, -1.0):
+        with pytest.raises(h2w.H2WError) as e:
+            g.set_inner_tolerance(bad)
+        assert e.value.code == h2w.H2W_EINVAL
     g.set_inner_tolerance(6)
 
 
