@@ -362,6 +362,8 @@ def main():
                     help="rank-predictor measurement (N = 262144 uniform and Gaussian, heuristic vs model)")
     ap.add_argument("--transcripts", action="store_true",
                     help="the reference's eight published runs on the GPU, beside the published CPU times")
+    ap.add_argument("--batch-only", action="store_true",
+                    help="only the 32-system batch section (16 right-hand sides each, tol 1e-8); prints its JSON")
     ap.add_argument("--kernel-table", metavar="CSV",
                     help="print a rocprofv3 --stats kernel summary CSV as a markdown table (no GPU)")
     args = ap.parse_args()
@@ -384,6 +386,9 @@ def main():
         return
     if args.c5:
         bench_c5()
+        return
+    if args.batch_only:
+        print(json.dumps({"batch_c4": bench_batch()}), flush=True)
         return
     if args.pmc_reduce:
         pmc_reduce(args.pmc_reduce, args.side, args.tol_pow)
