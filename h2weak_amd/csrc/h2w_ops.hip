@@ -2124,7 +2124,8 @@ h2w_gmres_stats solve_tree(Handle &h, const double *bt, int max_iter, double tol
             S.matvecs_fp32 += o.matvecs;
             launch_axpby(1.0, h.g_d.p, 1.0, h.g_x.p, N, s);
             ++outer;
-            if (o.iters == 0 && fp64_resid) break;
+            // (no iteration and no recycled projection: the correction was zero)
+            if (o.iters == 0 && fp64_resid && rc.k == 0) break;
             fp64_resid = true;
         }
         S.iterations = total;

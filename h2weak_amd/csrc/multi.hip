@@ -519,7 +519,8 @@ void lock_solve(Lockstep &L, const double *B, double *X, int nrhs, int max_iter,
             mv32 += o.matvecs;
             launch_axpby(1.0, d, 1.0, X, len, s);
             ++outer;
-            if (most == 0 && fp64_resid) break;
+            // (no iteration and no recycled projection: the corrections were zero)
+            if (most == 0 && fp64_resid && L.buf->rk == 0) break;
             fp64_resid = true;
         }
         for (int c = 0; c < C; ++c)
