@@ -43,6 +43,33 @@ inline double elapsed_ms(const Event &a, const Event &b) {
     return m;
 }
 
+// Page-locked host buffer (hipHostMalloc) for the small per-iteration read-backs of GMRES: a
+// device-to-host copy into it is one DMA transfer, without the runtime's staging copy.
+struct PinBuf {
+    double *p = nullptr;
+    size_t n = 0;
+    PinBuf() {}
+    PinBuf(const PinBuf &) = delete;
+    PinBuf &operator=(const PinBuf &) = delete;
+    ~PinBuf() { release(); }
+    void release() {
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    double *ensure(size_t cnt) {
+        if (cnt > n) {
+            release();
+            if (hipHostMalloc((void **)&p, cnt * sizeof(double), hipHostMallocDefault) != hipSuccess) {
+                p = nullptr;
+                throw Error{H2W_ENOMEM, "hipHostMalloc of " + std::to_string(cnt * sizeof(double)) + " bytes failed"};
+            }
+            n = cnt;
+        }
+        return p;
+    }
+};
+
 template <typename T>
 struct DevBuf {
     T *p = nullptr;
@@ -197,6 +224,7 @@ inline void app_rot(double &dx, double &dy, double cs, double sn) {
 // device work buffers of a lockstep solve
 struct LockBufs {
     DevBuf<double> V, w, r, d, b, part, h, sc;
+    PinBuf pin;  // read-backs of the Hessenberg columns
     int64_t Vcols = 0;
     DevBuf<double> U, coef;  // recycled refinement space (U; C sits in the first rk basis columns)
     int rk = 0;
@@ -264,6 +292,7 @@ struct h2w_handle {
     h2w::DevBuf<double> d_stage_in, d_stage_out, d_dense;
     h2w::DevBuf<double> g_V, g_w, g_r, g_b, g_x, g_h, g_part, g_d;
     h2w::DevBuf<double> g_U, g_coef;  // recycled refinement space: U (k columns), combination coefficients
+    h2w::PinBuf g_pin;                // GMRES read-backs (Hessenberg columns)
     int64_t g_Vcols = 0;
     // kNR-wide lockstep GMRES (interleaved N x kNR vectors)
     h2w::LockBufs mbuf;

@@ -1936,7 +1936,7 @@ GmresOut gmres_core(Handle &h, const double *b, double *x, int max_iter, double 
     ensure_V(h, kc + std::min<int64_t>(m + 1, 32));
     launch_axpby(1.0 / beta, r, 0.0, h.g_V.p + (int64_t)kc * N, N, s);
     sv[0] = beta;
-    std::vector<double> hh(2 * (kc + m + 2) + 2);
+    double *hh = h.g_pin.ensure(2 * (kc + m + 2) + 2);
     for (int i = 0; i < m; ++i) {
         ensure_V(h, kc + i + 2);
         double *C = h.g_V.p, *V = h.g_V.p + (int64_t)kc * N;
@@ -1951,7 +1951,7 @@ GmresOut gmres_core(Handle &h, const double *b, double *x, int max_iter, double 
         launch_mgemv_sub(C, N, k, h.g_h.p + k, w, N, s);
         launch_mdot(w, N, 1, w, N, h.g_part.p, nb, s);
         launch_mdot_reduce(h.g_part.p, nb, 1, h.g_h.p + 2 * k, s);
-        H2W_HIP(hipMemcpyAsync(hh.data(), h.g_h.p, (2 * k + 1) * 8, hipMemcpyDeviceToHost, s));
+        H2W_HIP(hipMemcpyAsync(hh, h.g_h.p, (2 * k + 1) * 8, hipMemcpyDeviceToHost, s));
         H2W_HIP(hipStreamSynchronize(s));
         if (kc > 0) {
             Bc.emplace_back(kc);

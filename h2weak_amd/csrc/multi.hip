@@ -261,7 +261,7 @@ LockOut lock_gmres(Lockstep &L, const double *b, double *x, const std::vector<ch
         sv[c * ld] = beta[c];
     }
     lscale(L, tmp, r, B.V.p + (int64_t)kc * len);
-    std::vector<double> hh((size_t)(2 * (kc + m + 2) + 2) * C);
+    double *hh = B.pin.ensure((size_t)(2 * (kc + m + 2) + 2) * C);
     int last = 0;
     for (int i = 0; i < m && nrun > 0; ++i) {
         ensure_V(L, kc + i + 2);
@@ -277,7 +277,7 @@ LockOut lock_gmres(Lockstep &L, const double *b, double *x, const std::vector<ch
         launch_bgemv16(Cb, len, k, B.h.p + (size_t)k * C, L.S, -1.0, w, L.n, L.seg, s);
         launch_bdot16(w, 0, 1, w, L.blk, L.nblk, B.part.p, s);
         launch_bdot16_reduce(B.part.p, L.nblk, 1, L.sblk, L.S, B.h.p + (size_t)2 * k * C, s);
-        H2W_HIP(hipMemcpyAsync(hh.data(), B.h.p, (size_t)(2 * k + 1) * C * 8, hipMemcpyDeviceToHost, s));
+        H2W_HIP(hipMemcpyAsync(hh, B.h.p, (size_t)(2 * k + 1) * C * 8, hipMemcpyDeviceToHost, s));
         H2W_HIP(hipStreamSynchronize(s));
         last = i;
         H.emplace_back((size_t)(i + 2) * C, 0.0);
