@@ -307,6 +307,8 @@ using Handle = h2w_handle;
 
 // assembly from skeletons already in h.fam / h.pairs (h2w_ops.hip)
 void rebuild_operators(Handle &h);
+// stream and device copies of the points of a handle created with plan_only (h2w_ops.hip)
+void attach_device(Handle &h);
 
 // matvec runtime (h2w_ops.hip)
 bool is32(int precision);

@@ -897,6 +897,8 @@ void build_inner(Handle &h, double irp) {
     ++h.inner_gen;
     const Plan &P = h.plan;
     hipStream_t s = h.stream;
+    // the inner buffers and lists are freed and reallocated below: nothing queued may still read them
+    H2W_HIP(hipStreamSynchronize(s));
     const int nfam_lo[2] = {2, 1};
     const bool fam_on[2] = {P.dim >= 2 && h.variant != H2W_NON_NESTED, h.variant == H2W_NESTED};
     const int nleaf = P.nboxes[P.nL];
@@ -1791,8 +1793,12 @@ void run_launches(Handle &h, int precision) {
     H2W_HIP(hipGraphLaunch(h.graph[precision], h.stream));
 }
 
-// graphs of the single-vector and the kNR-wide matvec
+// graphs of the single-vector and the kNR-wide matvec.  The stream is drained first, so no
+// replay of an exec is in flight when it is destroyed, and callers that go on to free or
+// reallocate the buffers it read (build_inner) free idle memory.
 void drop_graph(Handle &h, int precision) {
+    if (!h.graph[precision] && !h.graph_m[precision]) return;
+    if (h.stream) (void)hipStreamSynchronize(h.stream);  // (also on error paths: never throws)
     for (hipGraphExec_t *g : {&h.graph[precision], &h.graph_m[precision]})
         if (*g) {
             (void)hipGraphExecDestroy(*g);
@@ -1961,14 +1967,19 @@ GmresOut gmres_core(Handle &h, const double *b, double *x, int max_iter, double 
         for (int q = 0; q <= i; ++q) Hh(q, i) = hh[kc + q] + hh[k + kc + q];
         Hh(i + 1, i) = std::sqrt(hh[2 * k]);
         if (keep_h) out.H.push_back(H.back());  // (before the rotations)
-        launch_axpby(1.0 / Hh(i + 1, i), w, 0.0, V + (int64_t)(i + 1) * N, N, s);
+        // H(i+1, i) = 0: the Krylov space is invariant and the least-squares solution is exact
+        // (the reference's gap, 1D/codes/gmres.hpp:153-154, divides by it); the next basis
+        // vector is left zero, the rotation below leaves s_{i+1} = 0 and the solve ends here
+        const bool breakdown = !(Hh(i + 1, i) > 0.0);
+        if (breakdown) launch_fill(V + (int64_t)(i + 1) * N, N, 0.0, s);
+        else launch_axpby(1.0 / Hh(i + 1, i), w, 0.0, V + (int64_t)(i + 1) * N, N, s);
         for (int q = 0; q < i; ++q) app_rot(Hh(q, i), Hh(q + 1, i), cs[q], sn[q]);
         gen_rot(Hh(i, i), Hh(i + 1, i), cs[i], sn[i]);
         app_rot(Hh(i, i), Hh(i + 1, i), cs[i], sn[i]);
         app_rot(sv[i], sv[i + 1], cs[i], sn[i]);
         out.resid = std::fabs(sv[i + 1]) / normb;
         out.iters = i + 1;
-        bool done = out.resid < tol || i == m - 1;
+        bool done = out.resid < tol || i == m - 1 || breakdown;
         if (done) {
             std::vector<double> y(sv.begin(), sv.begin() + i + 1);
             for (int q = i; q >= 0; q--) {
@@ -1986,7 +1997,7 @@ GmresOut gmres_core(Handle &h, const double *b, double *x, int max_iter, double 
             launch_vec_update(V, N, i + 1, h.g_h.p, x, N, s);
             if (kc > 0) launch_vec_update(h.g_U.p, N, kc, h.g_h.p + i + 1, x, N, s);
             H2W_HIP(hipStreamSynchronize(s));
-            out.conv = out.resid < tol;
+            out.conv = out.resid < tol || (breakdown && out.resid == 0.0);
             return out;
         }
     }
@@ -2255,6 +2266,31 @@ void tune_inner(Handle &h, const double *bt, double tol, int max_iter, bool unro
 }
 }  // namespace h2w
 
+namespace h2w {
+// The device side of a handle whose plan is built: stream, tree-order points and
+// permutation on the device.  Needs a visible HIP device.
+void attach_device(Handle &h) {
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+        throw Error{H2W_ENOLIB, "no HIP device visible; libh2w has no CPU fallback"};
+    if (h.opts.device < 0 || h.opts.device >= ndev) throw Error{H2W_EINVAL, "device ordinal out of range"};
+    h.opts.plan_only = 0;
+    H2W_HIP(hipSetDevice(h.opts.device));
+    H2W_HIP(hipStreamCreateWithFlags(&h.stream, hipStreamNonBlocking));
+    h.d_px.upload(h.plan.px, h.stream);
+    h.d_py.upload(h.plan.py, h.stream);
+    h.d_pz.upload(h.plan.pz, h.stream);
+    h.d_perm.upload(h.plan.perm, h.stream);
+    H2W_HIP(hipStreamSynchronize(h.stream));
+    h.kp.dim = h.dim;
+    h.kp.choice = h.choice;
+    h.kp.diag = h.diag;
+    h.kp.px = h.d_px.p;
+    h.kp.py = h.d_py.p;
+    h.kp.pz = h.d_pz.p;
+}
+}  // namespace h2w
+
 // ------------------------------------------------------------------ C ABI
 using h2w::Error;
 
@@ -2362,23 +2398,7 @@ int h2w_create(int dim, int variant, int64_t N, const double *points, int n_max,
     h->use_graph = std::getenv("H2W_NO_GRAPH") == nullptr;
     h->plan.build(dim, N, points, L, nl);
     h->diag = h2w::diag_value(dim, choice, N, o.compat);
-    if (o.plan_only) {
-        *out = h.release();
-        return H2W_OK;
-    }
-    H2W_HIP(hipSetDevice(o.device));
-    H2W_HIP(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
-    h->d_px.upload(h->plan.px, h->stream);
-    h->d_py.upload(h->plan.py, h->stream);
-    h->d_pz.upload(h->plan.pz, h->stream);
-    h->d_perm.upload(h->plan.perm, h->stream);
-    H2W_HIP(hipStreamSynchronize(h->stream));
-    h->kp.dim = dim;
-    h->kp.choice = choice;
-    h->kp.diag = h->diag;
-    h->kp.px = h->d_px.p;
-    h->kp.py = h->d_py.p;
-    h->kp.pz = h->d_pz.p;
+    if (!o.plan_only) h2w::attach_device(*h);
     *out = h.release();
     return H2W_OK;
     H2W_CATCH

@@ -20,6 +20,8 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
+#include <algorithm>
+#include <array>
 #include <cstring>
 #include <memory>
 #include <string>
@@ -190,16 +192,28 @@ int h2w_load(const char *path, const h2w_opts *opts, h2w_handle **out) {
         if (opts) o = *opts;
         o.nlevels = nL;
         o.compat = hdr[6];
-        o.plan_only = 0;
+        // The file is read and checked completely against a host-only plan before any device
+        // is touched: every index it holds is used to address points or pool blocks later.
+        const int want_device = !o.plan_only;
+        o.plan_only = 1;
         int rc = h2w_create(dim, variant, N, pts.data(), n_max, L, tol_pow, choice, &o, &h);
         if (rc != H2W_OK) throw Error{rc, h2w_last_error()};
         const h2w::Plan &P = h->plan;
         const double t0 = h2w::now_s();
+        auto pivots_ok = [&](const std::vector<int32_t> &v) {
+            for (int32_t i : v)
+                if (i < 0 || i >= N) return false;
+            return true;
+        };
+        // levels each family's skeletons exist on (far: 2..nL, vertex: 1..nL; see the assembly)
+        const bool fam_used[2] = {dim >= 2 && variant != H2W_NON_NESTED, variant == H2W_NESTED};
+        const int fam_lo[2] = {2, 1};
         for (int f = 0; f < 2; ++f) {
             h->fam[f].assign(P.nL + 1, h2w::FamLevel());
             for (int j = 0; j <= P.nL; ++j) {
                 const int32_t nb = r.get<int32_t>();
-                if (nb != 0 && nb != P.nboxes[j]) throw Error{H2W_EINVAL, "corrupt operator file (box count)"};
+                const bool used = fam_used[f] && j >= fam_lo[f];
+                if (nb != (used ? P.nboxes[j] : 0)) throw Error{H2W_EINVAL, "corrupt operator file (box count)"};
                 h2w::FamLevel &fl = h->fam[f][j];
                 fl.nb = nb;
                 fl.rank.resize(nb);
@@ -209,27 +223,41 @@ int h2w_load(const char *path, const h2w_opts *opts, h2w_handle **out) {
                 fl.hist.resize(nb);
                 for (int b = 0; b < nb; ++b) {
                     const int32_t rk = r.get<int32_t>();
-                    if (rk < 0 || rk > N) throw Error{H2W_EINVAL, "corrupt operator file (rank)"};
+                    if (rk < 0 || rk > P.bcount[j][b]) throw Error{H2W_EINVAL, "corrupt operator file (rank)"};
                     fl.rank[b] = rk;
                     r.vec(fl.Ir[b], rk, N);
                     r.vec(fl.Ic[b], rk, N);
                     r.vec(fl.lr[b], (int64_t)2 * rk * rk, (int64_t)2 * rk * rk);
                     const int32_t nh = r.get<int32_t>();
                     r.vec(fl.hist[b], nh, (int64_t)2 * N);
-                    for (int k = 0; k < rk; ++k)
-                        if (fl.Ir[b][k] < 0 || fl.Ir[b][k] >= N || fl.Ic[b][k] < 0 || fl.Ic[b][k] >= N)
-                            throw Error{H2W_EINVAL, "corrupt operator file (pivot)"};
+                    if (!pivots_ok(fl.Ir[b]) || !pivots_ok(fl.Ic[b]))
+                        throw Error{H2W_EINVAL, "corrupt operator file (pivot)"};
                 }
             }
         }
+        // a far skeleton is taken from its row set: the leaf's points, else the children's skeletons
+        if (fam_used[0])
+            for (int j = P.nL - 1; j >= fam_lo[0]; --j)
+                for (int b = 0; b < P.nboxes[j]; ++b) {
+                    int64_t rows = 0;
+                    for (int c = 0; c < P.nchild; ++c) rows += h->fam[0][j + 1].rank[b * P.nchild + c];
+                    if (h->fam[0][j].rank[b] > rows) throw Error{H2W_EINVAL, "corrupt operator file (rank)"};
+                }
+        // pairs: exactly the directed pairs the assembly makes, in its order
+        std::vector<std::array<int, 3>> expect;
+        if (variant != H2W_NESTED)
+            for (int j = 1; j <= P.nL; ++j)
+                for (int b = 0; b < P.nboxes[j]; ++b)
+                    for (int v : (variant == H2W_NON_NESTED ? P.il[j][b] : P.vert_of(j, b))) expect.push_back({j, b, v});
         const int64_t np = r.get<int64_t>();
-        if (np < 0 || np > ((int64_t)1 << 31)) throw Error{H2W_EINVAL, "corrupt operator file (pairs)"};
+        if (np != (int64_t)expect.size()) throw Error{H2W_EINVAL, "corrupt operator file (pair count)"};
         h->pairs.assign((size_t)np, h2w::Pair());
-        for (auto &p : h->pairs) {
+        for (int64_t q = 0; q < np; ++q) {
+            h2w::Pair &p = h->pairs[q];
             int32_t v[4];
             r.raw(v, sizeof(v));
-            if (v[0] < 1 || v[0] > P.nL || v[1] < 0 || v[1] >= P.nboxes[v[0]] || v[2] < 0 ||
-                v[2] >= P.nboxes[v[0]] || v[3] < 0 || v[3] > N)
+            if (v[0] != expect[q][0] || v[1] != expect[q][1] || v[2] != expect[q][2] || v[3] < 0 ||
+                v[3] > std::min(P.bcount[v[0]][v[1]], P.bcount[v[0]][v[2]]))
                 throw Error{H2W_EINVAL, "corrupt operator file (pair)"};
             p.level = v[0];
             p.b = v[1];
@@ -238,13 +266,20 @@ int h2w_load(const char *path, const h2w_opts *opts, h2w_handle **out) {
             r.vec(p.Ir, p.rank, N);
             r.vec(p.Ic, p.rank, N);
             r.vec(p.lr, (int64_t)2 * p.rank * p.rank, (int64_t)2 * p.rank * p.rank);
+            if (!pivots_ok(p.Ir) || !pivots_ok(p.Ic)) throw Error{H2W_EINVAL, "corrupt operator file (pair pivot)"};
         }
         const uint64_t want = r.hash;
         const uint64_t got = r.get<uint64_t>();
         if (got != want) throw Error{H2W_EINVAL, "operator file checksum mismatch"};
+        char extra;
+        if (std::fread(&extra, 1, 1, fp.get()) != 0) throw Error{H2W_EINVAL, "corrupt operator file (trailing bytes)"};
         if (P.dim >= 2 && variant != H2W_NON_NESTED) h2w::derive_S(*h, 0, 2);
         if (variant == H2W_NESTED) h2w::derive_S(*h, 1, 1);
-        H2W_HIP(hipSetDevice(h->opts.device));
+        if (!want_device) {  // checked only: no device work
+            *out = h;
+            return H2W_OK;
+        }
+        h2w::attach_device(*h);
         h2w::rebuild_operators(*h);
         h->assembly_seconds = h2w::now_s() - t0;
         *out = h;

@@ -236,9 +236,13 @@ int h2w_autotune(h2w_handle *h, const double *b, int where, double tol, int max_
  * cross's per-step history; per semi-nested / non-nested pair the same), the input points and the build
  * parameters -- not the operator pools.  h2w_load rebuilds the plan and re-evaluates every operator block
  * with the same kernels: the loaded operator equals the saved one bit for bit, and no cross approximation
- * runs.  The file ends with an FNV-1a hash; truncated or altered files are refused (H2W_EINVAL). */
+ * runs.  The file ends with an FNV-1a hash; truncated or altered files are refused (H2W_EINVAL).  The hash
+ * only detects accidents: every count, rank, box, pair and pivot index in the file is also checked against
+ * the plan rebuilt from its points before any device work, so a crafted file is refused as well. */
 int h2w_save(const h2w_handle *h, const char *path);
-/* opts: device, fp32_shadow, ir_tol_pow, ... of the new handle (nlevels and compat come from the file). */
+/* opts: device, fp32_shadow, ir_tol_pow, ... of the new handle (nlevels and compat come from the file).
+ * With opts->plan_only = 1 the file is read and checked only, no device is touched, and the handle
+ * answers plan queries (h2w_tree_order, h2w_interaction_list, ...). */
 int h2w_load(const char *path, const h2w_opts *opts, h2w_handle **out);
 
 /* ---- learned rank predictor (no counterpart in the reference).  A 16-64-64-64-1 perceptron (GELU,

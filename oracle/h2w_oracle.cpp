@@ -679,9 +679,46 @@ struct Oracle {
             }
         }
     }
+    // M2L blocks K(Ir_B, Ir_F) of one level, stored for both directions (3D/codes/HODLR3Dtree.hpp:1978-1998,
+    // 2164-2185).  Each unordered pair is evaluated once, by its smaller box, and the transpose is
+    // copied to the other box.  Boxes run in parallel (assembly is not what the CPU baseline times);
+    // every box map is written by one thread only.
+    void assemble_m2l(int j, bool ver) {
+        const int nb = nBoxes[j];
+        std::vector<std::vector<std::pair<int, Mat>>> up(nb);
+#pragma omp parallel for schedule(dynamic, 1)
+        for (int k = 0; k < nb; ++k) {
+            Box &B = tree[j][k];
+            for (size_t i = 0; i < B.il.size(); ++i) {
+                int f = B.il[i];
+                if ((bool)B.vert[i] != ver || f < k) continue;
+                up[k].push_back({f, K.block(ver ? B.Irv : B.Ir, ver ? tree[j][f].Irv : tree[j][f].Ir)});
+            }
+        }
+#pragma omp parallel for schedule(dynamic, 1)
+        for (int k = 0; k < nb; ++k) {
+            Box &B = tree[j][k];
+            auto &dst = ver ? B.M2L_ver : B.M2L_far;
+            for (auto &e : up[k]) dst[e.first] = e.second;
+            for (size_t i = 0; i < B.il.size(); ++i) {
+                int f = B.il[i];
+                if ((bool)B.vert[i] != ver || f >= k) continue;
+                bool found = false;
+                for (auto &e : up[f])
+                    if (e.first == k) {
+                        dst[f] = transpose(e.second);
+                        found = true;
+                    }
+                if (!found) dst[f] = K.block(ver ? B.Irv : B.Ir, ver ? tree[j][f].Irv : tree[j][f].Ir);
+            }
+        }
+    }
     void assemble_far() {
         if (dim == 1) return;
+        // boxes of one level are independent (the reference loops over them serially; the pivots do
+        // not depend on the order, the row/column gathers inside the cross then run serially)
         for (int j = nL; j >= 2; --j)
+#pragma omp parallel for schedule(dynamic, 1)
             for (int k = 0; k < nBoxes[j]; ++k) {
                 Box &B = tree[j][k];
                 std::vector<int> rows, cols;
@@ -701,19 +738,11 @@ struct Oracle {
                     B.L2P = Mat((int)rows.size(), 0);
                 }
             }
-        for (int j = 2; j <= nL; ++j)
-            for (int k = 0; k < nBoxes[j]; ++k) {
-                Box &B = tree[j][k];
-                for (size_t i = 0; i < B.il.size(); ++i) {
-                    if (B.vert[i]) continue;
-                    int f = B.il[i];
-                    if (!B.M2L_far.count(f)) B.M2L_far[f] = K.block(B.Ir, tree[j][f].Ir);
-                    if (!tree[j][f].M2L_far.count(k)) tree[j][f].M2L_far[k] = transpose(B.M2L_far[f]);
-                }
-            }
+        for (int j = 2; j <= nL; ++j) assemble_m2l(j, false);
     }
     void assemble_vertex_nested() {
         for (int j = 1; j <= nL; ++j)
+#pragma omp parallel for schedule(dynamic, 1)
             for (int k = 0; k < nBoxes[j]; ++k) {
                 Box &B = tree[j][k];
                 std::vector<int> cols;
@@ -733,17 +762,9 @@ struct Oracle {
                 for (int r : a.rb) B.Irv.push_back(B.pts[r]);
                 for (int c : a.cb) B.Icv.push_back(cols[c]);
             }
+        for (int j = 1; j <= nL; ++j) assemble_m2l(j, true);
         for (int j = 1; j <= nL; ++j)
-            for (int k = 0; k < nBoxes[j]; ++k) {
-                Box &B = tree[j][k];
-                for (size_t i = 0; i < B.il.size(); ++i) {
-                    if (!B.vert[i]) continue;
-                    int f = B.il[i];
-                    if (!B.M2L_ver.count(f)) B.M2L_ver[f] = K.block(B.Irv, tree[j][f].Irv);
-                    if (!tree[j][f].M2L_ver.count(k)) tree[j][f].M2L_ver[k] = transpose(B.M2L_ver[f]);
-                }
-            }
-        for (int j = 1; j <= nL; ++j)
+#pragma omp parallel for schedule(dynamic, 1)
             for (int k = 0; k < nBoxes[j]; ++k) {
                 Box &B = tree[j][k];
                 if (j == nL) {
@@ -757,6 +778,7 @@ struct Oracle {
     }
     void assemble_vertex_semi() {
         for (int j = 1; j <= nL; ++j)
+#pragma omp parallel for schedule(dynamic, 1)
             for (int k = 0; k < nBoxes[j]; ++k) {
                 Box &B = tree[j][k];
                 for (size_t i = 0; i < B.il.size(); ++i) {

@@ -1,8 +1,9 @@
 """GPU: batches of independent systems.  A batched product (one launch per
 phase for every system, 16 right-hand sides per pass) equals each system's own
-16-wide product; lockstep solves over all (system, right-hand side) columns
-reach each column's tolerance with the single-system solve's accuracy.  The
-systems differ in size, depth, kernel and format."""
+16-wide product and matches the oracle's product of the same system; lockstep
+solves over all (system, right-hand side) columns of right-hand sides from the
+dense FP64 product reach each column's tolerance with the single-system solve's
+accuracy.  The systems differ in size, depth, kernel and format."""
 import numpy as np
 import pytest
 
@@ -39,6 +40,23 @@ def test_depths_differ(batch):
     assert len({g.stats()["nlevels"] for g in sys_}) >= 2
 
 
+def test_batched_product_against_oracle(batch):
+    """The batched FP64 product of every system against the oracle's product of the same system
+    (same skeletons: the pivots are the oracle's) and against the dense FP64 product."""
+    import oracle
+    B, sys_ = batch
+    Xs = [np.random.default_rng(11).normal(size=(2, g.N)) for g in sys_]
+    Ys = B.matvec(Xs)
+    for (dim, N, nmax, tp, ch, var, seed), X, Y, g in zip(SYSTEMS, Xs, Ys, sys_):
+        P = np.random.default_rng(seed).uniform(-1, 1, (dim, N))
+        o = oracle.Oracle(dim, N, nmax, 1.0, tp, ch, var, points=P)
+        o.tree_initialisation()
+        for c in range(2):
+            yo, yd = o.mat_vec_prod(X[c]), g.dense_rows(None, X[c])
+            assert _rel(Y[c], yo) <= 1e-11, (N, c, _rel(Y[c], yo))
+            assert _rel(Y[c], yd) <= 10 * _rel(yo, yd) + 1e-14
+
+
 @pytest.mark.parametrize("nrhs", [3, 16, 19])
 def test_batched_product(batch, nrhs):
     B, sys_ = batch
@@ -56,7 +74,8 @@ def test_batched_solves(batch):
     B, sys_ = batch
     rng = np.random.default_rng(7)
     Xt = [rng.normal(size=(4, g.N)) for g in sys_]
-    Bs = [g.matvec_multi(x) for g, x in zip(sys_, Xt)]
+    # right-hand sides from the dense FP64 product (not the compressed one), so errors are against x_true
+    Bs = [np.stack([g.dense_rows(None, x[c]) for c in range(4)]) for g, x in zip(sys_, Xt)]
     gtol = 1e-8
     # accuracy reference: each column's own single FP64 solve to the same tolerance
     e64 = [[_rel(g.gmres(Bs[s][c], None, 300, gtol, h2w.GMRES_FP64)[0], Xt[s][c]) for c in range(4)]

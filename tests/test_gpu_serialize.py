@@ -1,13 +1,16 @@
 """GPU: saving and loading an assembled operator.  The loaded operator (plan
 rebuilt from the saved points, skeletons put back, blocks re-evaluated) equals
-the saved one: storage count, pivots and products to the last bit up to atomic
-ordering, for every format; altered or truncated files are refused."""
+the saved one (storage count, pivots and products to the last bit up to atomic
+ordering, for every format) and the oracle's assembly of the same problem
+(pivots, the reference's storage count, the product to rounding); altered or
+truncated files are refused."""
 import os
 
 import numpy as np
 import pytest
 
 import h2weak_amd as h2w
+import oracle
 
 pytestmark = pytest.mark.gpu
 
@@ -45,6 +48,20 @@ def test_save_load_roundtrip(case, tmp_path):
             for box in range(0, (1 << dim) ** j, 7):
                 assert a.skeleton(f, j, box) == b.skeleton(f, j, box)
     x = np.random.default_rng(1).normal(size=N)
+    # the loaded operator against the oracle's assembly of the same problem: its skeletons are the
+    # oracle's pivots, and its product is the oracle's product to rounding
+    o = oracle.Oracle(dim, N, nmax, 1.0, tp, ch, var, points=P)
+    o.tree_initialisation()
+    same = tot = 0
+    for f in fams:
+        for j in range(2 if f == 0 else 1, sa["nlevels"] + 1):
+            for box in range((1 << dim) ** j):
+                tot += 1
+                same += b.skeleton(f, j, box) == o.pivots(f, j, box)
+    assert same == tot, (same, tot)
+    assert sb["memory"] == o.memory
+    yo = o.mat_vec_prod(x)
+    assert np.linalg.norm(b.mat_vec_prod(x) - yo) <= 1e-12 * np.linalg.norm(yo)
     for prec in (h2w.FP64, h2w.FP32, h2w.FP32_INNER):
         if prec == h2w.FP32_INNER and tp <= 6:
             continue
