@@ -37,7 +37,8 @@ EXPORTS = ("h2w_default_opts", "h2w_create", "h2w_assemble", "h2w_matvec", "h2w_
            "h2w_near_list", "h2w_destroy", "h2w_last_error", "h2w_bench_matvec", "h2w_set_inner_tolerance",
            "h2w_autotune", "h2w_gmres_multi", "h2w_bench_matvec_multi", "h2w_batch_create", "h2w_batch_matvec",
            "h2w_batch_gmres", "h2w_batch_bench_matvec", "h2w_batch_destroy", "h2w_set_rank_model",
-           "h2w_rank_samples", "h2w_rank_predict", "h2w_save", "h2w_load", "h2w_bench_stream_read")
+           "h2w_rank_samples", "h2w_rank_predict", "h2w_save", "h2w_load", "h2w_bench_stream_read",
+           "h2w_dense_matvec_noalloc")
 RANK_FEATURES, RANK_PARAMS = 16, 9473
 TUNE_UNROLL, TUNE_INNER = 1, 2
 
@@ -111,6 +112,7 @@ def lib():
     L.h2w_matvec_ex.argtypes = [p, p, p, i, i64, i, i, i]
     L.h2w_gmres.argtypes = [p, p, p, i, d, i, i, ctypes.POINTER(GmresStats)]
     L.h2w_dense_matvec.argtypes = [p, p, p, p, i64, i]
+    L.h2w_dense_matvec_noalloc.argtypes = [p, p, p, p, i64]
     L.h2w_get_stats.argtypes = [p, ctypes.POINTER(Stats)]
     L.h2w_tree_order.argtypes = [p, p]
     L.h2w_box_skeleton.argtypes = [p, i, i, i, p, p, i]

@@ -254,6 +254,7 @@ struct h2w_handle {
     h2w::KParams kp;
     h2w::DevBuf<double> d_px, d_py, d_pz;
     h2w::DevBuf<int32_t> d_perm;
+    h2w::DevBuf<int32_t> d_iperm;          // input index -> tree index
     h2w::DevBuf<double> mat64;
     h2w::DevBuf<float> mat32;
     int64_t mat_len = 0;
@@ -309,6 +310,9 @@ using Handle = h2w_handle;
 void rebuild_operators(Handle &h);
 // stream and device copies of the points of a handle created with plan_only (h2w_ops.hip)
 void attach_device(Handle &h);
+// device address of the per-GPU staging array of h2w_dense_matvec_noalloc, looked up once per
+// device with that device current (noalloc.hip)
+void noalloc_setup(int device);
 
 // matvec runtime (h2w_ops.hip)
 bool is32(int precision);

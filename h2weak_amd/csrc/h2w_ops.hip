@@ -2281,7 +2281,11 @@ void attach_device(Handle &h) {
     h.d_py.upload(h.plan.py, h.stream);
     h.d_pz.upload(h.plan.pz, h.stream);
     h.d_perm.upload(h.plan.perm, h.stream);
+    std::vector<int32_t> iperm(h.N);
+    for (int64_t t = 0; t < h.N; ++t) iperm[h.plan.perm[t]] = (int32_t)t;
+    h.d_iperm.upload(iperm, h.stream);
     H2W_HIP(hipStreamSynchronize(h.stream));
+    noalloc_setup(h.opts.device);
     h.kp.dim = h.dim;
     h.kp.choice = h.choice;
     h.kp.diag = h.diag;

@@ -171,6 +171,17 @@ int h2w_gmres_multi(h2w_handle *h, const double *B, int64_t ldb, double *X, int6
  * rows == NULL means all N rows.  Replaces `getTruePoten` (3D/codes/HODLR3Dtree.hpp:2630-2643). */
 int h2w_dense_matvec(h2w_handle *h, const double *x, double *y, const int64_t *rows, int64_t nrows, int where);
 
+/* No counterpart in the reference: for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The product of h2w_dense_matvec with
+ * device pointers only (x: N doubles and y: nrows doubles in input order, rows: nrows int64 input-order
+ * row indices or NULL for all N rows), without any allocation: its staging (x in tree order, the rows in
+ * tree order, a count of bad row indices) lives in one fixed device array per GPU of 12 MiB, set up when
+ * the first handle on that GPU is created and held until the process ends.  Limits: N <= 2^20 points and
+ * nrows <= 2^20 rows per call (larger: H2W_EINVAL, use h2w_dense_matvec).  A row index outside [0, N)
+ * gives H2W_EINVAL after the call (y is then unspecified).  Every call on one GPU shares that array:
+ * calls that overlap on one GPU (other host threads, other handles' streams) race. */
+int h2w_dense_matvec_noalloc(h2w_handle *h, const double *x, double *y, const int64_t *rows, int64_t nrows);
+
 int h2w_get_stats(const h2w_handle *h, h2w_stats *stats);
 
 /* Timing of the matvec on device-resident data (no counterpart in the reference, whose
