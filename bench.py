@@ -91,29 +91,94 @@ class Hip:
         return 2.0 * nbytes * reps / (ms.value * 1e-3) * 1e-9
 
 
-def cpu_baseline(n_side, gmres_matvecs_full, memory_full, tol_pow, gtol):
-    """Oracle (port of the reference's CPU algorithm, OpenMP on all host cores) on a
-    bounded sample: the same workload at n_side^3 points.  Its GMRES time-to-tolerance
-    is scaled to the full workload by the ratio of streamed work (M x matvecs)."""
+def _omp_env():
+    """OpenMP placement for the CPU baseline, as the reference's runs would use it: one thread per
+    core, threads bound close to each other (set before the oracle's OpenMP runtime starts)."""
+    os.environ.setdefault("OMP_PROC_BIND", "close")
+    os.environ.setdefault("OMP_PLACES", "cores")
+
+
+def _cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_solve(side, tol_pow, gtol, b=None, reps=3):
+    """The oracle (C++/OpenMP restatement of the reference's CPU algorithm) on the 3D H2* RBF grid
+    side^3: assembly, then GMRES to gtol from x0 = 1 timed with a wall clock around the solve only
+    (as the reference brackets it, 1D/codes/main.cpp:37-40), median of reps solves."""
+    _omp_env()
+    N = side ** 3
+    if N > 100000:
+        os.environ.setdefault("H2W_ORACLE_PROGRESS", "1")  # (the oracle reports each finished level)
     import oracle
-    N = n_side ** 3
-    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     t0 = time.time()
     o = oracle.Oracle(3, N, 64, 1.0, tol_pow, 1, 0)
     o.tree_initialisation()
     t_asm = time.time() - t0
+    log(f"oracle {side}^3: assembly {t_asm:.1f} s")
     xt = np.random.default_rng(2).uniform(-1, 1, N)
-    b = o.dense_matvec(xt)
-    t0 = time.time()
-    x, it, resid, nmv = o.gmres(b, None, 500, gtol)
-    t_gm = time.time() - t0
-    scale = (memory_full * gmres_matvecs_full) / (o.memory * nmv)
+    if b is None:
+        b = o.dense_matvec(xt)
+    ts = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        x, it, resid, nmv = o.gmres(b, None, 500, gtol)
+        ts.append(time.perf_counter() - t0)
+        log(f"oracle {side}^3: GMRES {it} iterations in {ts[-1]:.3f} s")
+    # one oracle matvec, for its bandwidth on the reference's storage count
+    xr = np.random.default_rng(3).normal(size=N)
+    t0 = time.perf_counter()
+    o.mat_vec_prod(xr)
+    t_mv = time.perf_counter() - t0
     err = float(np.linalg.norm(x - xt) / np.linalg.norm(xt))
-    return {"value": t_gm * scale, "unit": "s", "cores": cores, "kind": "port",
-            "sample": (f"oracle on 3D H2* RBF grid {n_side}^3 (N={N}), n_max=64, tol 1e-{tol_pow}: assembly "
-                       f"{t_asm:.1f} s, GMRES {it} its / {nmv} matvecs in {t_gm:.3f} s (rel. error {err:.1e}); "
-                       f"scaled x{scale:.1f} by M*matvecs to N=262144"),
-            "sample_gmres_s": t_gm, "sample_assembly_s": t_asm, "sample_iterations": it}
+    return {"N": N, "side": side, "assembly_s": round(t_asm, 2), "gmres_s": float(np.median(ts)),
+            "gmres_runs_s": [round(v, 4) for v in ts], "iterations": it, "matvecs": nmv, "resid": resid,
+            "rel_error": err, "memory": o.memory, "matvec_s": t_mv,
+            "matvec_effective_GBps": round((8 * o.memory + 16 * N) / t_mv * 1e-9, 2),
+            "threads": oracle.threads(), "cpu_model": _cpu_model(),
+            "omp": {k: os.environ.get(k) for k in ("OMP_NUM_THREADS", "OMP_PROC_BIND", "OMP_PLACES")}}
+
+
+def gpu_solve_at(side, tol_pow, gtol, reps=3):
+    """The GPU path on the same grid as a CPU sample: plain FP64 GMRES and the FP32-inner refinement
+    (default inner tolerance, no tuning), b from the GPU's dense FP64 product, x0 = 1, median of reps."""
+    import h2weak_amd as h2w
+    N = side ** 3
+    A = h2w.Initialisation(3, N, 64, 1.0, tol_pow, h2w.CHOICE_RBF, h2w.NESTED)
+    A.tree_initialisation()
+    xt = np.random.default_rng(2).uniform(-1, 1, N)
+    b = A.dense_rows(None, xt)
+    out = {}
+    for name, mode in (("fp64", h2w.GMRES_FP64), ("fp32_inner_ir", h2w.GMRES_FP32_INNER_IR)):
+        A.gmres(b, None, 500, gtol, mode)  # warm-up (graph capture)
+        runs = [A.gmres(b, None, 500, gtol, mode) for _ in range(reps)]
+        out[name] = {"s": float(np.median([r[1]["seconds"] for r in runs])), "iterations": runs[-1][1]["iterations"],
+                     "rel_error": float(np.linalg.norm(runs[-1][0] - xt) / np.linalg.norm(xt))}
+    return out, b
+
+
+def cpu_baseline(side, tol_pow, gtol):
+    """CPU baseline on a bounded sample of the workload: the oracle on the 3D H2* RBF grid side^3, its
+    GMRES time measured (not scaled), beside the GPU's time on the same grid and the same b."""
+    gpu, b = gpu_solve_at(side, tol_pow, gtol)
+    c = cpu_solve(side, tol_pow, gtol, b=b)
+    return {"value": round(c["gmres_s"], 6), "unit": "s", "cores": c["threads"], "kind": "port",
+            "sample": (f"oracle (C++/OpenMP restatement of the reference) on 3D H2* RBF grid {side}^3 (N={c['N']}), "
+                       f"n_max=64, tol 1e-{tol_pow}, same b as the GPU run on that grid: GMRES to {gtol:g} from x0 = 1, "
+                       f"{c['iterations']} iterations, median of 3 solves; {c['threads']} OpenMP threads on "
+                       f"{c['cpu_model']}, OMP_PROC_BIND={c['omp']['OMP_PROC_BIND']} OMP_PLACES={c['omp']['OMP_PLACES']}"),
+            "sample_N": c["N"], "sample_assembly_s": c["assembly_s"], "sample_iterations": c["iterations"],
+            "sample_rel_error": c["rel_error"], "sample_runs_s": c["gmres_runs_s"],
+            "sample_matvec_effective_GBps": c["matvec_effective_GBps"], "cpu_model": c["cpu_model"],
+            "gpu_same_sample": gpu,
+            "speedup_gpu_fp32_inner_ir_same_sample": round(c["gmres_s"] / gpu["fp32_inner_ir"]["s"], 2),
+            "speedup_gpu_fp64_same_sample": round(c["gmres_s"] / gpu["fp64"]["s"], 2)}
 
 
 def pmc_reduce(d, side, tp):
@@ -242,6 +307,92 @@ def bench_batch(S=32, tol_pow=8, seed=5):
     return out
 
 
+# The other benchmark configurations: (dim, variant, choice, N, n_max, tol_pow, gmres tol, points, x_true, modes)
+CONFIGS = {
+    "c1_1d_rbf_65536": dict(dim=1, variant=0, choice=1, N=65536, n_max=100, tol_pow=12, gtol=1e-12,
+                            points="U(-1,1) seed 1", xt="U(-1,1) seed 2"),
+    "c2_1d_ie_250000": dict(dim=1, variant=0, choice=0, N=250000, n_max=100, tol_pow=12, gtol=1e-12,
+                            points="U(-1,1) seed 1", xt="U(-1,1) seed 2"),
+    "c2_1d_rbf_250000": dict(dim=1, variant=0, choice=1, N=250000, n_max=100, tol_pow=12, gtol=1e-12,
+                             points="U(-1,1) seed 1", xt="U(-1,1) seed 2"),
+    "c3_2d_semi_ie_400x400": dict(dim=2, variant=1, choice=0, N=160000, n_max=64, tol_pow=10, gtol=1e-10,
+                                  points="grid", xt="N(0,1) seed 3"),
+}
+
+
+def bench_configs(names=None, cpu=True):
+    """The 1D and 2D benchmark configurations at their full sizes: on the GPU the FP64 product's
+    bandwidth, plain FP64 GMRES and the FP32-inner refinement; beside them the oracle (the reference's
+    CPU algorithm) on the host cores, its FP64 GMRES timed on the same b.  b = A x_true from the GPU's
+    dense FP64 product; x0 = 1; errors against x_true, and 1024 sampled product rows against the dense
+    FP64 rows."""
+    import h2weak_amd as h2w
+    out = {}
+    for name, c in CONFIGS.items():
+        if names and name not in names:
+            continue
+        N, dim = c["N"], c["dim"]
+        P = None if c["points"] == "grid" else np.random.default_rng(1).uniform(-1, 1, (dim, N))
+        t0 = time.time()
+        A = h2w.Initialisation(dim, N, c["n_max"], 1.0, c["tol_pow"], c["choice"], c["variant"], points=P)
+        A.tree_initialisation()
+        t_asm = time.time() - t0
+        st = A.stats()
+        xt = (np.random.default_rng(3).normal(size=N) if c["xt"].startswith("N(")
+              else np.random.default_rng(2).uniform(-1.0, 1.0, N))
+        b = A.dense_rows(None, xt)
+        rows = np.sort(np.random.default_rng(4).choice(N, 1024, replace=False))
+        y = A.mat_vec_prod(xt)
+        row_err = float(np.linalg.norm(y[rows] - b[rows]) / np.linalg.norm(b[rows]))
+        r = A.bench_matvec(10, h2w.FP64)
+        streamed = sum(r["launch_bytes"]) + 2 * 8 * N
+        row = {"config": c, "nlevels": st["nlevels"], "gpu_assembly_s": round(t_asm, 3), "memory": st["memory"],
+               "sampled_rows_rel_error": row_err,
+               "gpu_matvec_fp64": {"ms": round(r["ms_per_matvec"], 4),
+                                   "streamed_GBps": round(streamed / (r["ms_per_matvec"] * 1e-3) * 1e-9, 1),
+                                   "hbm_frac": round(streamed / (r["ms_per_matvec"] * 1e-3) * 1e-9 / HBM_PEAK_GBPS, 4),
+                                   "effective_GBps": round((8 * st["memory"] + 16 * N) / (r["ms_per_matvec"] * 1e-3) * 1e-9, 1)},
+               "gpu_gmres": {}}
+        for mode, nm in ((h2w.GMRES_FP64, "fp64"), (h2w.GMRES_FP32_INNER_IR, "fp32_inner_ir")):
+            A.gmres(b, None, 500, c["gtol"], mode)  # warm-up (graph capture)
+            runs = [A.gmres(b, None, 500, c["gtol"], mode) for _ in range(3)]
+            x, gs = runs[-1]
+            row["gpu_gmres"][nm] = {"s": float(np.median([q[1]["seconds"] for q in runs])),
+                                    "iterations": gs["iterations"], "resid": gs["resid"],
+                                    "rel_error": float(np.linalg.norm(x - xt) / np.linalg.norm(xt))}
+        log(f"{name}: GPU assembly {t_asm:.2f} s, matvec {row['gpu_matvec_fp64']}, GMRES {row['gpu_gmres']}")
+        del A
+        if cpu:
+            _omp_env()
+            import oracle
+            t0 = time.time()
+            o = oracle.Oracle(dim, N, c["n_max"], 1.0, c["tol_pow"], c["choice"], c["variant"], points=P)
+            o.tree_initialisation()
+            t_oasm = time.time() - t0
+            ts = []
+            for _ in range(3):
+                t0 = time.perf_counter()
+                x, it, resid, nmv = o.gmres(b, None, 500, c["gtol"])
+                ts.append(time.perf_counter() - t0)
+            t0 = time.perf_counter()
+            o.mat_vec_prod(xt)
+            t_mv = time.perf_counter() - t0
+            row["cpu_oracle"] = {"assembly_s": round(t_oasm, 2), "gmres_s": float(np.median(ts)),
+                                 "gmres_runs_s": [round(v, 4) for v in ts], "iterations": it, "resid": resid,
+                                 "rel_error": float(np.linalg.norm(x - xt) / np.linalg.norm(xt)),
+                                 "matvec_s": round(t_mv, 4),
+                                 "matvec_effective_GBps": round((8 * o.memory + 16 * N) / t_mv * 1e-9, 2),
+                                 "memory": o.memory, "threads": oracle.threads(), "cpu_model": _cpu_model(),
+                                 "omp": {k: os.environ.get(k) for k in ("OMP_PROC_BIND", "OMP_PLACES")}}
+            row["speedup_gpu_fp64_vs_cpu"] = round(row["cpu_oracle"]["gmres_s"] / row["gpu_gmres"]["fp64"]["s"], 1)
+            row["speedup_gpu_fp32_inner_ir_vs_cpu"] = round(row["cpu_oracle"]["gmres_s"]
+                                                            / row["gpu_gmres"]["fp32_inner_ir"]["s"], 1)
+            log(f"{name}: CPU oracle {row['cpu_oracle']}")
+            del o
+        out[name] = row
+    print(json.dumps({"configs": out}), flush=True)
+
+
 # The reference's eight published runs (uniform grids, x0 = 1): parameters, and the printed iteration count,
 # storage (GB) and wall times of assembly and GMRES on its CPU (hardware unstated).
 TRANSCRIPTS = {
@@ -327,7 +478,11 @@ def bench_c5(N=262144, tol_pow=10, seed=7):
                 est = np.ceil(1.2 * np.maximum(0, np.exp2(pl) - 1)) + 4
                 r.update({"blocks": int(len(R)), "mae_log2": round(float(np.mean(np.abs(pl - np.log2(1 + R)))), 4),
                           "under_estimates": st["rank_pred_under"],
-                          "reserved_over_actual": round(float(np.sum(est) / max(1, np.sum(R))), 3)})
+                          "reserved_over_actual": round(float(np.sum(est) / max(1, np.sum(R))), 3),
+                          "fixed_rank_blocks": st["rank_fixed_blocks"],
+                          "fixed_rank_fallback_resumed": st["rank_fixed_resumed"],
+                          "fixed_rank_extra_steps": st["rank_fixed_extra_steps"],
+                          "fixed_rank_sampled_rel_err_max": st["rank_fixed_sampled_err"]})
             row["model" if use else "heuristic"] = r
             log(f"C5 {kind} {'model' if use else 'heuristic'}: {r}")
             del g
@@ -342,7 +497,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--side", type=int, default=64, help="grid points per axis (N = side^3)")
     ap.add_argument("--tol-pow", type=int, default=10)
-    ap.add_argument("--cpu-side", type=int, default=24, help="grid side of the CPU-baseline sample")
+    ap.add_argument("--cpu-side", type=int, default=32, help="grid side of the CPU-baseline sample")
+    ap.add_argument("--cpu-full", action="store_true",
+                    help="only the CPU baseline at the full workload (the oracle at --side^3, several minutes); "
+                         "prints its JSON line (kept under profiles/ and cited by the default run)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-multi", action="store_true", help="skip the 16-right-hand-side section")
     ap.add_argument("--no-batch", action="store_true", help="skip the 32-system batch section")
@@ -360,6 +518,8 @@ def main():
                     help="assemble the workload once (heuristic rank estimates) and print the time")
     ap.add_argument("--c5", action="store_true",
                     help="rank-predictor measurement (N = 262144 uniform and Gaussian, heuristic vs model)")
+    ap.add_argument("--configs", action="store_true",
+                    help="the 1D / 2D benchmark configurations at full size, GPU and CPU oracle side by side")
     ap.add_argument("--transcripts", action="store_true",
                     help="the reference's eight published runs on the GPU, beside the published CPU times")
     ap.add_argument("--batch-only", action="store_true",
@@ -384,6 +544,9 @@ def main():
     if args.transcripts:
         bench_transcripts()
         return
+    if args.configs:
+        bench_configs(cpu=not args.no_cpu)
+        return
     if args.c5:
         bench_c5()
         return
@@ -392,6 +555,17 @@ def main():
         return
     if args.pmc_reduce:
         pmc_reduce(args.pmc_reduce, args.side, args.tol_pow)
+        return
+    if args.cpu_full:
+        import h2weak_amd as h2w
+        N = args.side ** 3
+        G = h2w.Initialisation(3, N, 64, 1.0, args.tol_pow, h2w.CHOICE_RBF, h2w.NESTED)  # (dense product only)
+        b = G.dense_rows(None, np.random.default_rng(2).uniform(-1, 1, N))
+        del G
+        c = cpu_solve(args.side, args.tol_pow, 10.0 ** -args.tol_pow, b=b)
+        print(json.dumps({"cpu_full": c, "workload": f"3D H2* RBF grid {args.side}^3 N={N} n_max=64 tol 1e-{args.tol_pow}",
+                          "command": "python bench.py --cpu-full", "b": "GPU dense FP64 product of x_true ~ U(-1,1) seed 2"}),
+              flush=True)
         return
     if args.gpus != 1:
         print(f"bench.py: runs on one GPU only (asked for --gpus {args.gpus}); sharding the operator by "
@@ -433,8 +607,11 @@ def main():
     log("b = A x_true from the GPU dense FP64 product")
     tune = None
     if not args.no_tune:
+        # the tuner solves a right-hand side of its own (x_tune ~ U(-1,1), seed 12), never the timed b
+        b_tune = A.dense_rows(None, np.random.default_rng(12).uniform(-1.0, 1.0, N))
         t0 = time.time()
-        tune = A.autotune(b, tol=gtol, max_iter=500)
+        tune = A.autotune(b_tune, tol=gtol, max_iter=500)
+        tune["rhs"] = "b_tune = A x_tune, x_tune ~ U(-1,1) seed 12 (the timed b uses seed 2)"
         log(f"autotune {time.time() - t0:.1f} s: inner tolerance 10^-{tune['ir_tol_exp']:g}; "
             + ", ".join(f"10^-{c['ir_tol_exp']:g}: {c['seconds'] * 1e3:.1f} ms ({c['iterations']}+{c['outer']})"
                         for c in tune["candidates"])
@@ -612,10 +789,24 @@ def main():
            "autotune": tune, "multi_rhs": multi, "batch_c4": batch,
            "roofline": roof, "roofline_other": roof2, "roofline_otf": roof_otf,
            "inner_operator_bytes_fp32": inner_bytes, "cpu_baseline": None}
+    # GPU time-to-tolerance over FP64 GPU at equal batching: one right-hand side, 16 in lockstep, 512 in a batch
+    out["speedup_vs_fp64_like_for_like"] = {
+        "single_rhs": round(t64 / t_step, 3),
+        "rhs16_lockstep": multi["gmres"]["fp32_inner_ir_vs_fp64"] if multi else None,
+        "batch512": batch["speedup_fp32_inner_ir_vs_batched_fp64"] if batch else None}
     if not args.no_cpu:
         log(f"CPU baseline (oracle) on grid {args.cpu_side}^3")
-        out["cpu_baseline"] = cpu_baseline(args.cpu_side, res["fp64"]["stats"]["matvecs_fp64"], M, tp, gtol)
-        out["speedup_vs_cpu"] = round(out["cpu_baseline"]["value"] / t_step, 2)
+        out["cpu_baseline"] = cpu_baseline(args.cpu_side, tp, gtol)
+        out["speedup_vs_cpu_same_sample"] = out["cpu_baseline"]["speedup_gpu_fp32_inner_ir_same_sample"]
+    # the oracle at the full workload, timed once on a GPU box's host cores (bench.py --cpu-full)
+    full = os.path.join(ROOT, "profiles", f"cpu_oracle_3d_rbf_grid{side}.json")
+    if os.path.exists(full):
+        c = json.load(open(full))["cpu_full"]
+        if c["N"] == N:
+            out["cpu_full_size"] = {"gmres_s": c["gmres_s"], "threads": c["threads"], "cpu_model": c["cpu_model"],
+                                    "iterations": c["iterations"], "rel_error": c["rel_error"],
+                                    "source": os.path.relpath(full, ROOT)}
+            out["speedup_vs_cpu"] = round(c["gmres_s"] / t_step, 2)
     print(json.dumps(out), flush=True)
 
 

@@ -65,7 +65,9 @@ class Stats(ctypes.Structure):
                 ("aca_reruns", ctypes.c_int), ("stored_bytes_inner_fp32", ctypes.c_int64),
                 ("aca_batches", ctypes.c_int), ("aca_pool_bytes", ctypes.c_int64), ("rank_pred_blocks", ctypes.c_int),
                 ("rank_pred_under", ctypes.c_int), ("rank_pred_mae_log2", ctypes.c_double),
-                ("inner_tol_exp", ctypes.c_double)]
+                ("inner_tol_exp", ctypes.c_double), ("rank_fixed_blocks", ctypes.c_int),
+                ("rank_fixed_resumed", ctypes.c_int), ("rank_fixed_extra_steps", ctypes.c_int64),
+                ("rank_fixed_sampled_err", ctypes.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -248,6 +248,10 @@ struct h2w_handle {
     h2w::DevBuf<uint16_t> rank_model;   // BF16 rank-predictor parameters (empty: heuristic estimates)
     int rank_pred_blocks = 0, rank_pred_under = 0;  // predicted blocks / blocks whose rank exceeded the estimate
     double rank_pred_abs_log2 = 0.0;    // sum of |log2(1 + predicted) - log2(1 + rank)|
+    int rank_fix_blocks = 0;            // wide blocks that ran a fixed-rank phase
+    int rank_fix_resumed = 0;           // of those, blocks the stopping rule sent on adaptively
+    int64_t rank_fix_extra_steps = 0;   // pivot steps taken beyond where the stopping rule ended a block
+    double rank_fix_sampled_err = 0.0;  // largest sampled-entry relative error of those blocks
     std::vector<std::array<float, h2w::kRankFeatures>> sample_feat;  // harvest: every ACA block of the
     std::vector<int32_t> sample_rank;                                 // last assembly with its rank
     hipStream_t stream = nullptr;

@@ -144,7 +144,9 @@ constexpr int kNR = 16;        // right-hand sides of one multi-vector pass
 // ------------------------------------------------------------------ launch wrappers (kernels.hip)
 // wide cross for a few very large blocks (state per block, partials per workgroup)
 struct WideState {
-    int32_t rank, rp, cp, l, next_rp, done, status, found, dots_rank, pad;
+    // status: 0 running / finished, 1 out of workspace (re-run), 2 parked at the end of its fixed
+    // phase (kfix steps without the normS update), 3 ended inside its fixed phase (exhausted)
+    int32_t rank, rp, cp, l, next_rp, done, status, found, dots_rank, kfix;
     double piv, vn2, un, vn, normS;
     int64_t off;
 };
@@ -162,6 +164,27 @@ void launch_wide_step(const AcaJob *jobs, int nb, int maxm, int maxn, int maxran
                       int64_t ws_cap, KParams kp, double tol, hipStream_t s);
 void launch_wide_finish(const WideState *state, int nb, int32_t *rank_out, int32_t *status_out, int32_t *ndone,
                         hipStream_t s);
+// fixed-rank phase (AcaJob.pad = kfix > 1 steps): one pivot step without the normS update
+void launch_wide_fixed_step(const AcaJob *jobs, int nb, int maxm, int maxn, const int32_t *idx, double *ws,
+                            int64_t *tab, uint8_t *flags, WideState *state, WidePart *part, int pstride,
+                            int32_t *rowpiv, int32_t *colpiv, double *hist, unsigned long long *ws_top, int64_t ws_cap,
+                            KParams kp, double tol, hipStream_t s);
+// Gram products of parked blocks: for tile job (block, qi, ki) of 64 x 64 term pairs, prod[off + k(k+1)/2 + q]
+// = |<U_q, U_k>| |<V_q, V_k>| for q <= k < rank
+struct GramJob {
+    int32_t blk, qi, ki, pad;
+    int64_t off;
+};
+void launch_wide_gram(const GramJob *gj, int njobs, const AcaJob *jobs, const WideState *state, const double *ws,
+                      const int64_t *tab, double *prod, hipStream_t s);
+// the judged blocks: final_rank[b] >= 0 ends block b at that rank; -1 resumes the adaptive cross with
+// normS = normS_in[b]; -2 leaves the block as it is
+void launch_wide_judge(const AcaJob *jobs, int nb, WideState *state, const int32_t *final_rank, const double *normS_in,
+                       int64_t *tab, unsigned long long *ws_top, int64_t ws_cap, hipStream_t s);
+// sampled-entry check of finished blocks: err2[b], ref2[b] over ns entries (i, j) drawn from a hash of
+// (b, sample): |A(i,j) - sum_q U_q[i] V_q[j]|^2 and |A(i,j)|^2
+void launch_aca_sample(const AcaJob *jobs, int nb, const int32_t *rank, const int32_t *idx, const double *ws,
+                       const int64_t *tab, KParams kp, int ns, double *err2, double *ref2, hipStream_t s);
 void launch_aca(const AcaJob *jobs, int njobs, const int32_t *idx, double *ws, int64_t ws_cap,
                 unsigned long long *ws_top, int64_t *tab, uint8_t *flags, int32_t *rank, int32_t *status,
                 int32_t *rowpiv, int32_t *colpiv, KParams kp, double tol, double *hist, int big, hipStream_t s);

@@ -206,20 +206,28 @@ void block_features(const Handle &h, int fam, int j, int b, const std::vector<in
     B.has_feat = true;
 }
 
-// The wide cross of the first nh jobs of a batch (device copies at djobs): pivot steps
-// are enqueued 16 at a time; between groups the host reads how many blocks are done.
-// Every step advances a block's rank, moves its first-row search on, or finishes it, so
-// the loop ends within max(m) + max(tcap) + 1 steps.
+// The wide cross of the first nh jobs of a batch (device copies at djobs).  Blocks with a
+// fixed-rank phase (AcaJob.pad = kfix > 1, from the rank model) first take kfix pivot steps
+// without the normS update and stopping test -- every launch enqueued at once, no host reads
+// in between -- and are then judged by the reference's own stopping rule, applied to all their
+// steps at once: the normS sequence follows from the Gram products |<U_q,U_k>||<V_q,V_k>| of
+// their terms, the block ends at the first step the rule stops (the pivots up to there are the
+// ones the adaptive cross takes, since the rule never influences a pivot) or, when the rule
+// holds for every step (rank under-predicted), it continues adaptively from there.  Adaptive
+// steps are enqueued 16 at a time; between groups the host reads how many blocks are done.
+// Every step advances a block's rank, moves its first-row search on, or finishes it, so the
+// loop ends within max(m) + max(tcap) + 1 steps.
 void run_wide(Handle &h, const std::vector<AcaJob> &jobs, size_t nh, const AcaJob *djobs, const int32_t *didx,
               AcaPool &pool, int64_t *dtab, uint8_t *dflags, int32_t *drank, int32_t *dstat, int32_t *drp,
               int32_t *dcp, double *dhist, double tol) {
     hipStream_t s = h.stream;
     const int nb = (int)nh;
-    int maxm = 0, maxn = 0, maxt = 0;
+    int maxm = 0, maxn = 0, maxt = 0, maxfix = 0;
     for (size_t q = 0; q < nh; ++q) {
         maxm = std::max(maxm, jobs[q].m);
         maxn = std::max(maxn, jobs[q].n);
         maxt = std::max(maxt, jobs[q].tcap);
+        maxfix = std::max(maxfix, jobs[q].pad);
     }
     const int pstride = std::max(1, std::max((maxm + 2047) / 2048, (maxn + 2047) / 2048));
     const int dstride = std::max(maxt, 1);
@@ -234,11 +242,98 @@ void run_wide(Handle &h, const std::vector<AcaJob> &jobs, size_t nh, const AcaJo
     launch_wide_init(djobs, nb, maxm + maxn, st.p, dflags, dtab, pool.top.p, pool.cap, s);
     const int64_t limit = (int64_t)maxm + maxt + 1;
     int64_t steps = 0;
+    if (maxfix > 1) {
+        // ---- fixed phase (a block whose first rows are zero spends steps on its first-row search:
+        // enqueue more until every block has parked or ended)
+        std::vector<WideState> hs(nb);
+        int64_t todo = maxfix;
+        while (true) {
+            for (int64_t c = 0; c < todo; ++c, ++steps)
+                launch_wide_fixed_step(djobs, nb, maxm, maxn, didx, pool.ws.p, dtab, dflags, st.p, part.p, pstride,
+                                       drp, dcp, dhist, pool.top.p, pool.cap, h.kp, tol, s);
+            H2W_HIP(hipGetLastError());
+            H2W_HIP(hipMemcpyAsync(hs.data(), st.p, (size_t)nb * sizeof(WideState), hipMemcpyDeviceToHost, s));
+            H2W_HIP(hipStreamSynchronize(s));
+            int running = 0;
+            for (auto &w : hs) running += !w.done;
+            if (running == 0) break;
+            if (steps > limit) throw Error{H2W_EHIP, "wide cross (fixed phase) did not finish within its step bound"};
+            todo = 16;
+        }
+        // ---- Gram products of the judged blocks (status 2: parked at kfix, 3: ended inside the phase)
+        std::vector<GramJob> gj;
+        std::vector<int64_t> poff(nb, -1);
+        int64_t plen = 0;
+        for (int b = 0; b < nb; ++b) {
+            if (hs[b].status != 2 && hs[b].status != 3) continue;
+            const int K = hs[b].rank;
+            poff[b] = plen;
+            plen += (int64_t)K * (K + 1) / 2;
+            const int nt = (K + 63) / 64;
+            for (int ki = 0; ki < nt; ++ki)
+                for (int qi = 0; qi <= ki; ++qi) gj.push_back(GramJob{b, qi, ki, 0, poff[b]});
+        }
+        std::vector<int32_t> fin(nb, -2);
+        std::vector<double> nrm(nb, 0.0);
+        if (!gj.empty()) {
+            DevBuf<GramJob> dgj;
+            DevBuf<double> dprod;
+            dgj.upload(gj, s);
+            dprod.alloc(std::max<int64_t>(plen, 1));
+            launch_wide_gram(dgj.p, (int)gj.size(), djobs, st.p, pool.ws.p, dtab, dprod.p, s);
+            H2W_HIP(hipGetLastError());
+            std::vector<double> prod(plen);
+            H2W_HIP(hipMemcpyAsync(prod.data(), dprod.p, plen * 8, hipMemcpyDeviceToHost, s));
+            H2W_HIP(hipStreamSynchronize(s));
+            for (int b = 0; b < nb; ++b) {
+                if (poff[b] < 0) continue;
+                const int K = hs[b].rank;
+                const int64_t ho = 2 * jobs[b].piv;
+                std::vector<double> hb(2 * (size_t)K);
+                H2W_HIP(hipMemcpyAsync(hb.data(), dhist + ho, 2 * (size_t)K * 8, hipMemcpyDeviceToHost, s));
+                H2W_HIP(hipStreamSynchronize(s));
+                // the reference's recurrence (2D/codes/aca.hpp:404-409): the first term enters with
+                // normS = 0; term k adds (|u_k||v_k|)^2 + 2 sum_{q <= k} |<U_q,u_k>||<V_q,v_k>|
+                const int mn = std::min(jobs[b].m, jobs[b].n);
+                double normS = 0.0;
+                int stop = -1;
+                for (int k = 0; k < K; ++k) {
+                    const double uv = hb[2 * k];
+                    if (k > 0) {
+                        double S = 0.0;
+                        const double *pk = prod.data() + poff[b] + (int64_t)k * (k + 1) / 2;
+                        for (int q = 0; q <= k; ++q) S += pk[q];
+                        normS = std::sqrt(normS * normS + uv * uv + 2.0 * S);
+                    }
+                    hb[2 * k + 1] = normS;
+                    if (!(uv > tol * normS && k + 1 < mn)) {
+                        stop = k + 1;
+                        break;
+                    }
+                }
+                const int kept = stop > 0 ? stop : K;
+                if (stop > 0 || hs[b].status == 3) {
+                    fin[b] = kept;
+                    h.rank_fix_extra_steps += K - kept;
+                } else {
+                    fin[b] = -1;  // continue adaptively
+                    nrm[b] = normS;
+                    h.rank_fix_resumed++;
+                }
+                h.rank_fix_blocks++;
+                H2W_HIP(hipMemcpyAsync(dhist + ho, hb.data(), 2 * (size_t)kept * 8, hipMemcpyHostToDevice, s));
+                H2W_HIP(hipStreamSynchronize(s));
+            }
+            DevBuf<int32_t> dfin;
+            DevBuf<double> dnrm;
+            dfin.upload(fin, s);
+            dnrm.upload(nrm, s);
+            launch_wide_judge(djobs, nb, st.p, dfin.p, dnrm.p, dtab, pool.top.p, pool.cap, s);
+            H2W_HIP(hipGetLastError());
+            H2W_HIP(hipStreamSynchronize(s));
+        }
+    }
     while (true) {
-        for (int c = 0; c < 16; ++c, ++steps)
-            launch_wide_step(djobs, nb, maxm, maxn, (int)std::min<int64_t>(steps + 1, dstride), didx, pool.ws.p, dtab,
-                             dflags, st.p, part.p, pstride, dots.p, dstride, drp, dcp, dhist, pool.top.p, pool.cap,
-                             h.kp, tol, s);
         launch_wide_finish(st.p, nb, drank, dstat, ndone.p, s);
         H2W_HIP(hipGetLastError());
         int nd = 0;
@@ -246,6 +341,30 @@ void run_wide(Handle &h, const std::vector<AcaJob> &jobs, size_t nh, const AcaJo
         H2W_HIP(hipStreamSynchronize(s));
         if (nd == nb) break;
         if (steps > limit) throw Error{H2W_EHIP, "wide cross did not finish within its step bound"};
+        for (int c = 0; c < 16; ++c, ++steps)
+            launch_wide_step(djobs, nb, maxm, maxn, (int)std::min<int64_t>(steps + 1, dstride), didx, pool.ws.p, dtab,
+                             dflags, st.p, part.p, pstride, dots.p, dstride, drp, dcp, dhist, pool.top.p, pool.cap,
+                             h.kp, tol, s);
+    }
+    // sampled entries of the fixed-rank blocks against their cross approximation
+    if (maxfix > 1) {
+        std::vector<int> sel;
+        for (int b = 0; b < nb; ++b)
+            if (jobs[b].pad > 1) sel.push_back(b);
+        DevBuf<double> e2, a2;
+        e2.alloc(nb);
+        a2.alloc(nb);
+        launch_aca_sample(djobs, nb, drank, didx, pool.ws.p, dtab, h.kp, 256, e2.p, a2.p, s);
+        H2W_HIP(hipGetLastError());
+        std::vector<double> he(nb), ha(nb);
+        std::vector<int32_t> hst(nb);
+        H2W_HIP(hipMemcpyAsync(he.data(), e2.p, nb * 8, hipMemcpyDeviceToHost, s));
+        H2W_HIP(hipMemcpyAsync(ha.data(), a2.p, nb * 8, hipMemcpyDeviceToHost, s));
+        H2W_HIP(hipMemcpyAsync(hst.data(), dstat, nb * 4, hipMemcpyDeviceToHost, s));
+        H2W_HIP(hipStreamSynchronize(s));
+        for (int b : sel)
+            if (hst[b] == 0 && ha[b] > 0.0)
+                h.rank_fix_sampled_err = std::max(h.rank_fix_sampled_err, std::sqrt(he[b] / ha[b]));
     }
 }
 
@@ -286,6 +405,13 @@ void run_aca(Handle &h, std::vector<AcaBlock> &blocks, int rank_hint, AcaPool &p
         std::stable_sort(todo.begin(), todo.end(), [&](size_t a, size_t c) { return est[a] > est[c]; });
     }
     const std::vector<int> est0 = est;
+    // fixed-rank phase of the wide cross: 3 % above the predicted rank plus 2 steps (H2W_NO_FIXED_RANK
+    // keeps the adaptive cross throughout)
+    const bool fixed_off = std::getenv("H2W_NO_FIXED_RANK") != nullptr;
+    auto kfix_of = [&](size_t b) {
+        const double pr = pred ? std::max(0.0, std::exp2((double)plog[b]) - 1.0) : 0.0;
+        return std::max(2, (int)std::lround(1.03 * pr) + 2);
+    };
     int64_t need = 0;
     for (size_t b = 0; b < blocks.size(); ++b) need += (int64_t)(blocks[b].rows.size() + blocks[b].cols.size()) * est[b];
     ensure_pool(h, pool, need);
@@ -335,7 +461,8 @@ void run_aca(Handle &h, std::vector<AcaBlock> &blocks, int rank_hint, AcaPool &p
                 J.m = (int32_t)B.rows.size();
                 J.n = (int32_t)B.cols.size();
                 J.tcap = std::min(std::min(J.m, J.n), 8192);
-                J.pad = 0;
+                // wide blocks with a predicted rank: a fixed-rank phase of about that many steps
+                J.pad = (pred && q < nhuge && !fixed_off) ? std::min(J.tcap, kfix_of(batch[q])) : 0;
                 J.rows = (int64_t)idx.size();
                 idx.insert(idx.end(), B.rows.begin(), B.rows.end());
                 J.cols = (int64_t)idx.size();
@@ -1215,6 +1342,9 @@ void assemble(Handle &h) {
     h.aca_pool_bytes = 0;
     h.rank_pred_blocks = h.rank_pred_under = 0;
     h.rank_pred_abs_log2 = 0.0;
+    h.rank_fix_blocks = h.rank_fix_resumed = 0;
+    h.rank_fix_extra_steps = 0;
+    h.rank_fix_sampled_err = 0.0;
     h.sample_feat.clear();
     h.sample_rank.clear();
     const bool verbose = std::getenv("H2W_VERBOSE") != nullptr;
@@ -2617,6 +2747,10 @@ int h2w_get_stats(const h2w_handle *h, h2w_stats *st) {
     st->rank_pred_blocks = h->rank_pred_blocks;
     st->rank_pred_under = h->rank_pred_under;
     st->rank_pred_mae_log2 = h->rank_pred_blocks ? h->rank_pred_abs_log2 / h->rank_pred_blocks : 0.0;
+    st->rank_fixed_blocks = h->rank_fix_blocks;
+    st->rank_fixed_resumed = h->rank_fix_resumed;
+    st->rank_fixed_extra_steps = h->rank_fix_extra_steps;
+    st->rank_fixed_sampled_err = h->rank_fix_sampled_err;
     for (int f = 0; f < 2; ++f) {
         int mx = 0;
         double sum = 0.0;
@@ -2781,6 +2915,13 @@ int h2w_bench_matvec(h2w_handle *h, int nrep, int precision, h2w_bench *out) {
         out->launch_role[q + 1] = Lh.role;
         out->launch_bytes[q + 1] =
             (precision == H2W_FP32_INNER ? h->inner.elems[q] : h->launch_elems[q]) * es;
+        if ((precision == H2W_FP32_OTF || precision == H2W_FP64_OTF) && Lh.role == 1) {
+            // the on-the-fly launches evaluate every entry of the directed lists (both
+            // directions of each pair), element size times that count
+            int64_t e = 0;
+            for (auto v : h->otf.elems) e += v;
+            out->launch_bytes[q + 1] = e * es;
+        }
     }
     return H2W_OK;
     H2W_CATCH

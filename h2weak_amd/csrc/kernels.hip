@@ -428,6 +428,7 @@ __global__ __launch_bounds__(256) void k_wide_init(const AcaJob *__restrict__ jo
         st.l = 0;
         st.next_rp = INT_MAX;
         st.done = (jb.m == 0 || jb.n == 0 || jb.tcap == 0);
+        st.kfix = jb.pad;
         if (!st.done) wide_reserve(st, jb, tab_all + jb.tab, ws_top, ws_cap);
         state[blockIdx.y] = st;
     }
@@ -595,8 +596,10 @@ __global__ __launch_bounds__(256) void k_wide_accept(const AcaJob *__restrict__ 
     const double un = sqrt(un2), vn = sqrt(st.vn2);
     uint8_t *ru = flags + jb.wsF, *cu = ru + jb.m;
     double *hist = hist_all + 2 * jb.piv;
+    const bool fixed = st.kfix > 1 && st.dots_rank >= 0;  // (dots_rank < 0: the fixed phase was judged)
     if (st.rank > 0 && (un < 1e-36 || vn < 1e-36)) {
         st.done = 1;  // the step is dropped
+        if (fixed && st.rank >= 2) st.status = 3;
         state[blockIdx.x] = st;
         return;
     }
@@ -608,6 +611,23 @@ __global__ __launch_bounds__(256) void k_wide_accept(const AcaJob *__restrict__ 
     st.vn = vn;
     st.next_rp = an.idx;
     ++st.rank;
+    if (fixed) {
+        // fixed phase: no normS update and no stopping test until kfix steps are taken (the stopping
+        // rule is applied afterwards to all of them at once, from their Gram products)
+        hist[2 * (st.rank - 1)] = un * vn;
+        hist[2 * (st.rank - 1) + 1] = 0.0;
+        st.dots_rank = st.rank;
+        const int mn = min(jb.m, jb.n);
+        if (st.rank >= st.kfix || st.rank >= mn || st.next_rp == INT_MAX) {
+            st.done = 1;
+            st.status = st.rank >= st.kfix && st.rank < mn && st.next_rp != INT_MAX ? 2 : 3;
+        } else {
+            st.rp = st.next_rp;
+            wide_reserve(st, jb, tab_all + jb.tab, ws_top, ws_cap);
+        }
+        state[blockIdx.x] = st;
+        return;
+    }
     if (st.rank == 1) {
         // the first term enters without a normS update (normS stays 0)
         hist[0] = un * vn;
@@ -683,6 +703,124 @@ __global__ void k_wide_finish(const WideState *__restrict__ state, int nb, int32
     }
     done = (int)wave_sum((double)done);
     if (threadIdx.x == 0) *ndone = done;
+}
+
+// Gram products of the terms of a parked block: one workgroup per 64 x 64 tile of term pairs
+// (q in tile qi, k in tile ki, qi <= ki); each thread accumulates 4 x 4 dot products of U
+// columns, then of V columns, over 32-entry slabs staged in LDS.  prod = |<U_q,U_k>||<V_q,V_k>|
+// for q <= k (packed lower-triangular by k).
+constexpr int kGramT = 64, kGramS = 32;
+__device__ __forceinline__ void gram_tile(const double *ws, const int64_t *tab, int part, int len, int q0, int nq,
+                                          int k0, int nk, double (&acc)[4][4]) {
+    __shared__ double sq[kGramS][kGramT + 1], sk[kGramS][kGramT + 1];
+    const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;  // q = q0 + 4 tx + a, k = k0 + 4 ty + b
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) acc[a][b] = 0.0;
+    for (int t0 = 0; t0 < len; t0 += kGramS) {
+        __syncthreads();
+        for (int e = threadIdx.x; e < kGramS * kGramT; e += 256) {
+            const int c = e / kGramS, t = e % kGramS;  // column c of the tile, entry t of the slab
+            const bool tin = t0 + t < len;
+            sq[t][c] = (c < nq && tin) ? ws[tab[2 * (q0 + c) + part] + t0 + t] : 0.0;
+            sk[t][c] = (c < nk && tin) ? ws[tab[2 * (k0 + c) + part] + t0 + t] : 0.0;
+        }
+        __syncthreads();
+#pragma unroll 4
+        for (int t = 0; t < kGramS; ++t) {
+            double x[4], y[4];
+#pragma unroll
+            for (int a = 0; a < 4; ++a) x[a] = sq[t][4 * tx + a];
+#pragma unroll
+            for (int b = 0; b < 4; ++b) y[b] = sk[t][4 * ty + b];
+#pragma unroll
+            for (int a = 0; a < 4; ++a)
+#pragma unroll
+                for (int b = 0; b < 4; ++b) acc[a][b] = fma(x[a], y[b], acc[a][b]);
+        }
+    }
+}
+__global__ __launch_bounds__(256) void k_wide_gram(const GramJob *__restrict__ gj, const AcaJob *__restrict__ jobs,
+                                                   const WideState *__restrict__ state, const double *__restrict__ ws,
+                                                   const int64_t *__restrict__ tab_all, double *__restrict__ prod) {
+    const GramJob g = gj[blockIdx.x];
+    const AcaJob jb = jobs[g.blk];
+    const int K = state[g.blk].rank;
+    const int64_t *tab = tab_all + jb.tab;
+    const int q0 = g.qi * kGramT, k0 = g.ki * kGramT;
+    const int nq = min(kGramT, K - q0), nk = min(kGramT, K - k0);
+    double gu[4][4], gv[4][4];
+    gram_tile(ws, tab, 0, jb.m, q0, nq, k0, nk, gu);
+    gram_tile(ws, tab, 1, jb.n, q0, nq, k0, nk, gv);
+    const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            const int q = q0 + 4 * tx + a, k = k0 + 4 * ty + b;
+            if (q < K && k < K && q <= k) prod[g.off + (int64_t)k * (k + 1) / 2 + q] = fabs(gu[a][b]) * fabs(gv[a][b]);
+        }
+}
+
+// decisions of the host on the judged blocks (one thread per block)
+__global__ void k_wide_judge(const AcaJob *__restrict__ jobs, int nb, WideState *__restrict__ state,
+                             const int32_t *__restrict__ final_rank, const double *__restrict__ normS_in,
+                             int64_t *__restrict__ tab_all, unsigned long long *ws_top, int64_t ws_cap) {
+    for (int b = blockIdx.x * blockDim.x + threadIdx.x; b < nb; b += gridDim.x * blockDim.x) {
+        const int fr = final_rank[b];
+        if (fr == -2) continue;
+        WideState st = state[b];
+        st.dots_rank = -1;  // (the fixed phase is over)
+        if (fr >= 0) {
+            st.rank = fr;
+            st.done = 1;
+            st.status = 0;
+        } else {
+            // the stopping rule did not hold within the fixed phase: continue adaptively
+            st.normS = normS_in[b];
+            st.dots_rank = st.rank;
+            st.done = 0;
+            st.status = 0;
+            st.kfix = 0;
+            st.rp = st.next_rp;
+            wide_reserve(st, jobs[b], tab_all + jobs[b].tab, ws_top, ws_cap);
+        }
+        state[b] = st;
+    }
+}
+
+// sampled entries of finished blocks against their cross approximation
+__global__ __launch_bounds__(64) void k_aca_sample(const AcaJob *__restrict__ jobs, const int32_t *__restrict__ rank,
+                                                   const int32_t *__restrict__ idx, const double *__restrict__ ws,
+                                                   const int64_t *__restrict__ tab_all, KParams kp, int ns,
+                                                   double *__restrict__ err2, double *__restrict__ ref2) {
+    const AcaJob jb = jobs[blockIdx.x];
+    const int r = rank[blockIdx.x];
+    const int64_t *tab = tab_all + jb.tab;
+    double e2 = 0.0, a2 = 0.0;
+    if (jb.m > 0 && jb.n > 0)
+        for (int q = threadIdx.x; q < ns; q += 64) {
+            // (i, j) from a 64-bit mix of (block, sample)
+            uint64_t h = ((uint64_t)blockIdx.x << 32) ^ (uint64_t)q ^ 0x9E3779B97F4A7C15ull;
+            h ^= h >> 33;
+            h *= 0xff51afd7ed558ccdull;
+            h ^= h >> 33;
+            h *= 0xc4ceb9fe1a85ec53ull;
+            h ^= h >> 33;
+            const int i = (int)(h % (uint64_t)jb.m), j = (int)((h >> 32) % (uint64_t)jb.n);
+            const double a = kentry2(kp, idx[jb.rows + i], idx[jb.cols + j]);
+            double v = 0.0;
+            for (int k = 0; k < r; ++k) v = fma(ws[tab[2 * k] + i], ws[tab[2 * k + 1] + j], v);
+            e2 += (a - v) * (a - v);
+            a2 += a * a;
+        }
+    e2 = wave_sum(e2);
+    a2 = wave_sum(a2);
+    if (threadIdx.x == 0) {
+        err2[blockIdx.x] = e2;
+        ref2[blockIdx.x] = a2;
+    }
 }
 
 // L (unit lower) and R (upper) of the cross: L(i,j) = U_j[rb_i], R(j,i) = V_j[cb_i]
@@ -1454,75 +1592,204 @@ struct Coords {
     const T *x, *y, *z;
     const float *xl, *yl, *zl;
 };
-template <typename T>
-__device__ __forceinline__ T cdiff(const T *h, const float *l, T th, float tl, int64_t s);
-template <>
-__device__ __forceinline__ float cdiff<float>(const float *h, const float *l, float th, float tl, int64_t s) {
-    return (th - h[s]) + (tl - l[s]);
+
+// FP32 entries from the squared distance R2 (1D: the distance R): the common case of each
+// kernel (otf_val32) and the case of points closer than a = 1e-4 (otf_near32, RBF only;
+// otf_rare32 says whether it applies).  log R = log2(R^2) ln2 / 2, so the common 3D and 2D
+// cases need no square root; hardware log2 / rsq / rcp (about 1 ulp; arguments are never
+// denormal: R >= 1e-4 in the common case).
+template <int DIM, int CH>
+__device__ __forceinline__ float otf_val32(float R2) {
+    constexpr float kLn2 = 0.69314718055994531f;
+    if constexpr (DIM == 3) {
+        if constexpr (CH == 0) return 0.0795774715459476679f * __builtin_amdgcn_rsqf(R2);  // 1 / (4 pi R)
+        return __builtin_amdgcn_logf(R2) * (float)(0.5 * 0.69314718055994531 / -9.2103403719761836);  // log R / log a
+    } else if constexpr (DIM == 2) {
+        if constexpr (CH == 0) return (-0.0795774715459476679f * kLn2) * __builtin_amdgcn_logf(R2);  // -(1/4pi) log R^2
+        return 1.0e-4f * __builtin_amdgcn_rsqf(R2);  // a / R
+    } else {
+        if constexpr (CH == 0) return kLn2 * __builtin_amdgcn_logf(R2);  // log R (R2 holds R)
+        return 1.0e-4f * __builtin_amdgcn_rcpf(R2);                     // a / R
+    }
 }
-template <>
-__device__ __forceinline__ double cdiff<double>(const double *h, const float *, double th, float, int64_t s) {
-    return th - h[s];
+template <int DIM, int CH>
+__device__ __forceinline__ bool otf_rare32(float R2) {
+    if constexpr (CH == 0) return false;
+    return DIM == 1 ? R2 < 1.0e-4f : R2 < 1.0e-8f;
+}
+template <int DIM>
+__device__ __forceinline__ float otf_near32(float R2) {
+    const float R = DIM == 1 ? R2 : sqrtf(R2);
+    if constexpr (DIM == 3) return (R * logf(R) - 1.0f) * -0.9990798135f;  // (R log R - 1) / (a log a - 1)
+    return R * 1.0e4f;                                                       // R / a
 }
 
+// One source point of a tile: coordinates (FP32: hi and lo parts), its value and point id.
 template <typename T>
-__device__ __forceinline__ T otf_entry(const Coords<T> &C, const int32_t *pid, int dim, int choice, T diag, T tx,
-                                       T ty, T tz, float txl, float tyl, float tzl, int tp, int64_t s) {
-    if (pid[s] == tp) return diag;
-    T dx = cdiff<T>(C.x, C.xl, tx, txl, s);
-    T dy = dim > 1 ? cdiff<T>(C.y, C.yl, ty, tyl, s) : T(0);
-    T dz = dim > 2 ? cdiff<T>(C.z, C.zl, tz, tzl, s) : T(0);
-    return koff_t<T>(dim, choice, dx, dy, dz);
-}
+struct OtfSrc;
+template <>
+struct OtfSrc<float> {
+    float x, y, z, xl, yl, zl, v;
+    int pid;
+};
+template <>
+struct OtfSrc<double> {
+    double x, y, z, v;
+    int pid, pad;
+};
 
 template <typename T>
-__global__ __launch_bounds__(256) void k_otf(const GItem *__restrict__ items, const GTerm *__restrict__ terms,
-                                             Coords<T> C, const int32_t *__restrict__ pid, int dim, int choice,
-                                             T diag, T *vec) {
-    const GItem it = items[blockIdx.x];
-    const int lane = threadIdx.x & 63;
-    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    __shared__ T part[4][kRowTile];
-    for (int half = 0; half * 64 < it.rows; ++half) {
-        const int row = half * 64 + lane;
-        T acc0 = 0, acc1 = 0;
-        if (row < it.rows) {
-            const int64_t ts = it.y + row;
-            const T tx = C.x[ts], ty = dim > 1 ? C.y[ts] : T(0), tz = dim > 2 ? C.z[ts] : T(0);
-            float txl = 0.f, tyl = 0.f, tzl = 0.f;
-            if (C.xl) {
-                txl = C.xl[ts];
-                tyl = dim > 1 ? C.yl[ts] : 0.f;
-                tzl = dim > 2 ? C.zl[ts] : 0.f;
-            }
-            const int tp = pid[ts];
-            for (int tt = it.t0; tt < it.t1; ++tt) {
-                const GTerm tm = terms[tt];
-                const int64_t x0 = tm.x;
-                int c = wv;
-                for (; c + 4 < tm.cols; c += 8) {
-                    const int64_t s0 = x0 + c, s1 = s0 + 4;
-                    T e0 = otf_entry<T>(C, pid, dim, choice, diag, tx, ty, tz, txl, tyl, tzl, tp, s0);
-                    T e1 = otf_entry<T>(C, pid, dim, choice, diag, tx, ty, tz, txl, tyl, tzl, tp, s1);
-                    acc0 = fma(e0, vec[s0], acc0);
-                    acc1 = fma(e1, vec[s1], acc1);
-                }
-                for (; c < tm.cols; c += 4) {
-                    const int64_t s0 = x0 + c;
-                    T e0 = otf_entry<T>(C, pid, dim, choice, diag, tx, ty, tz, txl, tyl, tzl, tp, s0);
-                    acc0 = fma(e0, vec[s0], acc0);
-                }
+struct OtfTgt {
+    T x, y, z;
+    float xl, yl, zl;
+    int pid;
+};
+
+template <typename T>
+__device__ __forceinline__ OtfTgt<T> otf_target(const Coords<T> &C, const int32_t *pid, int dim, int64_t ts) {
+    OtfTgt<T> t;
+    t.x = C.x[ts];
+    t.y = dim > 1 ? C.y[ts] : T(0);
+    t.z = dim > 2 ? C.z[ts] : T(0);
+    t.xl = t.yl = t.zl = 0.f;
+    if (C.xl) {
+        t.xl = C.xl[ts];
+        t.yl = dim > 1 ? C.yl[ts] : 0.f;
+        t.zl = dim > 2 ? C.zl[ts] : 0.f;
+    }
+    t.pid = pid[ts];
+    return t;
+}
+
+// Entries K(t0, q) and K(t1, q) of two target rows and one source, accumulated with the
+// source's value.  DIAG: the diagonal rule where the points are the same (a tile holding one
+// of the item's own points: the near-field self blocks).  The FP32 case for points closer
+// than a is a branch the wave takes only when one of its lanes needs it.
+template <typename T, int DIM, int CH, bool DIAG>
+__device__ __forceinline__ void otf_pair(const OtfTgt<T> &t0, const OtfTgt<T> &t1, const OtfSrc<T> &q, T diag,
+                                         T &acc0, T &acc1) {
+    T v0, v1;
+    if constexpr (sizeof(T) == 4) {
+        float r0, r1;
+        if constexpr (DIM == 1) {
+            r0 = fabsf((t0.x - q.x) + (t0.xl - q.xl));
+            r1 = fabsf((t1.x - q.x) + (t1.xl - q.xl));
+        } else {
+            const float ax = (t0.x - q.x) + (t0.xl - q.xl), bx = (t1.x - q.x) + (t1.xl - q.xl);
+            const float ay = (t0.y - q.y) + (t0.yl - q.yl), by = (t1.y - q.y) + (t1.yl - q.yl);
+            r0 = fmaf(ay, ay, ax * ax);
+            r1 = fmaf(by, by, bx * bx);
+            if constexpr (DIM == 3) {
+                const float az = (t0.z - q.z) + (t0.zl - q.zl), bz = (t1.z - q.z) + (t1.zl - q.zl);
+                r0 = fmaf(az, az, r0);
+                r1 = fmaf(bz, bz, r1);
             }
         }
-        part[threadIdx.x >> 6][half * 64 + lane] = acc0 + acc1;
+        v0 = otf_val32<DIM, CH>(r0);
+        v1 = otf_val32<DIM, CH>(r1);
+        if constexpr (CH != 0) {
+            const bool n0 = otf_rare32<DIM, CH>(r0), n1 = otf_rare32<DIM, CH>(r1);
+            if (__ballot(n0 || n1)) {
+                if (n0) v0 = otf_near32<DIM>(r0);
+                if (n1) v1 = otf_near32<DIM>(r1);
+            }
+        }
+    } else {
+        // the stored operator's own formula (koff), so the FP64 products agree to rounding
+        v0 = koff(DIM, CH, t0.x - q.x, t0.y - q.y, t0.z - q.z);
+        v1 = koff(DIM, CH, t1.x - q.x, t1.y - q.y, t1.z - q.z);
+    }
+    if constexpr (DIAG) {
+        v0 = q.pid == t0.pid ? diag : v0;
+        v1 = q.pid == t1.pid ? diag : v1;
+    }
+    acc0 = fma(v0, q.v, acc0);
+    acc1 = fma(v1, q.v, acc1);
+}
+
+// P2P / M2L on the fly, tiled.  One workgroup per item (<= 128 target rows).  Source
+// points of the item's terms are staged 256 at a time in LDS (coordinates, value, id: one
+// coalesced load per source instead of one broadcast load per source and target row).
+// Thread t owns target rows g and g + h (h = ceil(rows / 2), g = t mod h) in registers and
+// the sources k = sub, sub + nsub, ... of each tile (sub = t / h, nsub = 256 / h), so every
+// source read from LDS feeds two entries; the partial sums of the nsub source subsets are
+// added through LDS at the end.  A tile is checked once for points of the item itself (the
+// near-field self blocks); only such tiles test every entry for the diagonal.
+constexpr int kOtfTile = 256;
+template <typename T, int DIM, int CH>
+__global__ __launch_bounds__(256) void k_otf(const GItem *__restrict__ items, const GTerm *__restrict__ terms,
+                                             Coords<T> C, const int32_t *__restrict__ pid, T diag, T *vec) {
+    constexpr int dim = DIM;
+    __shared__ OtfSrc<T> tile[kOtfTile];
+    __shared__ T part[2 * 256];
+    __shared__ int tpmin, tpmax;
+    const GItem it = items[blockIdx.x];
+    const int R = it.rows;
+    const int h = (R + 1) >> 1;
+    const int nsub = 256 / h;
+    const int t = threadIdx.x;
+    const bool act = t < nsub * h;
+    const int g = act ? t % h : 0, sub = act ? t / h : 0;
+    const bool two = g + h < R;
+    const OtfTgt<T> t0 = otf_target<T>(C, pid, dim, it.y + g);
+    const OtfTgt<T> t1 = otf_target<T>(C, pid, dim, it.y + (two ? g + h : g));
+    if (t == 0) {
+        tpmin = INT_MAX;
+        tpmax = INT_MIN;
     }
     __syncthreads();
-    if (threadIdx.x < it.rows) {
-        const int t = threadIdx.x;
-        T s = part[0][t] + part[1][t] + part[2][t] + part[3][t];
+    if (t < R) {
+        const int p = pid[it.y + t];
+        atomicMin(&tpmin, p);
+        atomicMax(&tpmax, p);
+    }
+    __syncthreads();
+    const int pmin = tpmin, pmax = tpmax;
+    T acc0 = 0, acc1 = 0;
+    for (int tt = it.t0; tt < it.t1; ++tt) {
+        const GTerm tm = terms[tt];
+        for (int c0 = 0; c0 < tm.cols; c0 += kOtfTile) {
+            const int n = min(kOtfTile, tm.cols - c0);
+            __syncthreads();  // (the previous tile is no longer read)
+            bool own = false;
+            if (t < n) {
+                const int64_t sl = tm.x + c0 + t;
+                OtfSrc<T> q;
+                q.x = C.x[sl];
+                q.y = dim > 1 ? C.y[sl] : T(0);
+                q.z = dim > 2 ? C.z[sl] : T(0);
+                if constexpr (sizeof(T) == 4) {
+                    q.xl = C.xl ? C.xl[sl] : 0.f;
+                    q.yl = dim > 1 && C.yl ? C.yl[sl] : 0.f;
+                    q.zl = dim > 2 && C.zl ? C.zl[sl] : 0.f;
+                }
+                q.v = vec[sl];
+                q.pid = pid[sl];
+                tile[t] = q;
+                own = q.pid >= pmin && q.pid <= pmax;
+            }
+            const bool diag_tile = __syncthreads_or(own);
+            if (!act) continue;
+            if (!diag_tile) {
+                for (int k = sub; k < n; k += nsub) otf_pair<T, DIM, CH, false>(t0, t1, tile[k], diag, acc0, acc1);
+            } else {
+                for (int k = sub; k < n; k += nsub) otf_pair<T, DIM, CH, true>(t0, t1, tile[k], diag, acc0, acc1);
+            }
+        }
+    }
+    // partial sums of the source subsets: part[sub * R + row]
+    __syncthreads();
+    if (act) {
+        part[sub * R + g] = acc0;
+        if (two) part[sub * R + g + h] = acc1;
+    }
+    __syncthreads();
+    if (t < R) {
+        T sum = 0;
+        for (int q = 0; q < nsub; ++q) sum += part[q * R + t];
         T *y = vec + it.y + t;
-        if (it.flags & 1) atomicAdd(y, s);
-        else *y += s;
+        if (it.flags & 1) atomicAdd(y, sum);
+        else *y += sum;
     }
 }
 
@@ -1893,17 +2160,30 @@ void launch_bgemv32(const GItem *items, int nitems, const GTerm *terms, const fl
                     float *vec, int role, int unr, hipStream_t s) {
     launch_bgemv_t<float>(items, nitems, terms, A0, A1, vec, role, unr, s);
 }
+template <typename T>
+static void launch_otf_t(const GItem *items, int nitems, const GTerm *terms, Coords<T> C, const SlotPoints &sp, T *vec,
+                         hipStream_t s) {
+    if (nitems <= 0) return;
+    const T diag = (T)sp.diag;
+    const int key = sp.dim * 2 + (sp.choice != 0);
+#define H2W_OTF(D, CH) \
+    hipLaunchKernelGGL((k_otf<T, D, CH>), dim3(nitems), dim3(256), 0, s, items, terms, C, sp.pid, diag, vec)
+    switch (key) {
+        case 2: H2W_OTF(1, 0); break;
+        case 3: H2W_OTF(1, 1); break;
+        case 4: H2W_OTF(2, 0); break;
+        case 5: H2W_OTF(2, 1); break;
+        case 6: H2W_OTF(3, 0); break;
+        default: H2W_OTF(3, 1); break;
+    }
+#undef H2W_OTF
+}
 void launch_otf32(const GItem *items, int nitems, const GTerm *terms, SlotPoints sp, float *vec, hipStream_t s) {
-    if (nitems > 0)
-        hipLaunchKernelGGL(k_otf<float>, dim3(nitems), dim3(256), 0, s, items, terms,
-                           Coords<float>{sp.x, sp.y, sp.z, sp.xl, sp.yl, sp.zl}, sp.pid, sp.dim, sp.choice,
-                           (float)sp.diag, vec);
+    launch_otf_t<float>(items, nitems, terms, Coords<float>{sp.x, sp.y, sp.z, sp.xl, sp.yl, sp.zl}, sp, vec, s);
 }
 void launch_otf64(const GItem *items, int nitems, const GTerm *terms, SlotPoints sp, double *vec, hipStream_t s) {
-    if (nitems > 0)
-        hipLaunchKernelGGL(k_otf<double>, dim3(nitems), dim3(256), 0, s, items, terms,
-                           Coords<double>{sp.xd, sp.yd, sp.zd, nullptr, nullptr, nullptr}, sp.pid, sp.dim, sp.choice,
-                           sp.diag, vec);
+    launch_otf_t<double>(items, nitems, terms, Coords<double>{sp.xd, sp.yd, sp.zd, nullptr, nullptr, nullptr}, sp,
+                         vec, s);
 }
 void launch_to_fp32(const double *a, float *b, int64_t n, hipStream_t s) {
     if (n > 0) hipLaunchKernelGGL(k_to_fp32, dim3(grid_for(n)), dim3(256), 0, s, a, b, n);
@@ -2092,6 +2372,33 @@ void launch_wide_step(const AcaJob *jobs, int nb, int maxm, int maxn, int maxran
 void launch_wide_finish(const WideState *state, int nb, int32_t *rank_out, int32_t *status_out, int32_t *ndone,
                         hipStream_t s) {
     hipLaunchKernelGGL(k_wide_finish, dim3(1), dim3(64), 0, s, state, nb, rank_out, status_out, ndone);
+}
+void launch_wide_fixed_step(const AcaJob *jobs, int nb, int maxm, int maxn, const int32_t *idx, double *ws,
+                            int64_t *tab, uint8_t *flags, WideState *state, WidePart *part, int pstride,
+                            int32_t *rowpiv, int32_t *colpiv, double *hist, unsigned long long *ws_top, int64_t ws_cap,
+                            KParams kp, double tol, hipStream_t s) {
+    const int gr = std::max(1, (maxn + kWideCols - 1) / kWideCols), gc = std::max(1, (maxm + kWideCols - 1) / kWideCols);
+    hipLaunchKernelGGL(k_wide_row, dim3(gr, nb), dim3(256), 0, s, jobs, idx, ws, ws, tab, flags, state, part, pstride,
+                       kp);
+    hipLaunchKernelGGL(k_wide_row_reduce, dim3(nb), dim3(256), 0, s, jobs, state, part, pstride);
+    hipLaunchKernelGGL(k_wide_col, dim3(gc, nb), dim3(256), 0, s, jobs, idx, ws, ws, tab, flags, state, part, pstride,
+                       kp);
+    hipLaunchKernelGGL(k_wide_accept, dim3(nb), dim3(256), 0, s, jobs, state, part, pstride, flags, rowpiv, colpiv,
+                       hist, tab, ws_top, ws_cap, tol);
+}
+void launch_wide_gram(const GramJob *gj, int njobs, const AcaJob *jobs, const WideState *state, const double *ws,
+                      const int64_t *tab, double *prod, hipStream_t s) {
+    if (njobs > 0) hipLaunchKernelGGL(k_wide_gram, dim3(njobs), dim3(256), 0, s, gj, jobs, state, ws, tab, prod);
+}
+void launch_wide_judge(const AcaJob *jobs, int nb, WideState *state, const int32_t *final_rank, const double *normS_in,
+                       int64_t *tab, unsigned long long *ws_top, int64_t ws_cap, hipStream_t s) {
+    if (nb > 0)
+        hipLaunchKernelGGL(k_wide_judge, dim3((nb + 63) / 64), dim3(64), 0, s, jobs, nb, state, final_rank, normS_in,
+                           tab, ws_top, ws_cap);
+}
+void launch_aca_sample(const AcaJob *jobs, int nb, const int32_t *rank, const int32_t *idx, const double *ws,
+                       const int64_t *tab, KParams kp, int ns, double *err2, double *ref2, hipStream_t s) {
+    if (nb > 0) hipLaunchKernelGGL(k_aca_sample, dim3(nb), dim3(64), 0, s, jobs, rank, idx, ws, tab, kp, ns, err2, ref2);
 }
 template <typename T>
 static void launch_bgemm_tl(const GItem *items, int nitems, const GTerm *terms, const T *A0, const T *A1,

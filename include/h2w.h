@@ -101,6 +101,14 @@ typedef struct {
     int rank_pred_under;     /* of those, blocks whose rank exceeded the estimate (re-run)         */
     double rank_pred_mae_log2; /* mean |log2(1 + predicted) - log2(1 + rank)| over those blocks    */
     double inner_tol_exp;    /* tolerance exponent p (10^-p) of the inner operator, 0 if none          */
+    int rank_fixed_blocks;   /* wide-cross blocks compressed with a fixed-rank phase from the rank model
+                                (kfix = 1.03 x predicted + 2 steps, then the reference's stopping rule
+                                applied to all of them at once)                                       */
+    int rank_fixed_resumed;  /* of those, blocks whose rank the model under-predicted: the stopping rule
+                                held at every fixed step and the adaptive cross continued (fallback)  */
+    int64_t rank_fixed_extra_steps; /* pivot steps run beyond where the stopping rule ended a block    */
+    double rank_fixed_sampled_err;  /* largest relative error of those blocks' cross approximation on
+                                       256 sampled entries each                                       */
 } h2w_stats;
 
 typedef struct {
@@ -194,7 +202,9 @@ typedef struct {
     double ms_per_matvec;
     double launch_ms[H2W_MAX_LAUNCHES];
     int n_launch;
-    int64_t launch_bytes[H2W_MAX_LAUNCHES];
+    int64_t launch_bytes[H2W_MAX_LAUNCHES]; /* operator bytes the launch streams; for the on-the-fly
+                                               precisions' M2L + near-field launch: entries it evaluates
+                                               (every directed pair) times the element size        */
     int launch_role[H2W_MAX_LAUNCHES]; /* -1 zero fill, 0 transfer pass, 1 M2L + near field (one
                                           stored block per unordered pair, applied both ways),
                                           2 semi-nested column factors                       */

@@ -68,6 +68,8 @@ def lib():
         L.orc_rank.argtypes = [_p, _i, _i, _i]
         L.orc_pivots.restype = _i
         L.orc_pivots.argtypes = [_p, _i, _i, _i, _pi, _pi]
+        L.orc_threads.restype = _i
+        L.orc_threads.argtypes = []
         L.orc_aca.restype = _i
         L.orc_aca.argtypes = [_p, _pi, _i, _pi, _i, _i, _pi, _pi, _pd, _pd]
         _lib = L
@@ -80,6 +82,11 @@ def _dp(a):
 
 def _ip(a):
     return a.ctypes.data_as(_pi)
+
+
+def threads():
+    """OpenMP threads the oracle's parallel regions use."""
+    return lib().orc_threads()
 
 
 def gl16():

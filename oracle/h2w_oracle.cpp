@@ -30,6 +30,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <omp.h>
@@ -289,6 +290,15 @@ struct Kernel {
     }
 };
 
+// ---------------------------------------------------------------- progress
+// With H2W_ORACLE_PROGRESS set, long runs report what they finished (levels, large crosses)
+// on stderr as they go.
+bool progress_on() {
+    static const bool on = std::getenv("H2W_ORACLE_PROGRESS") != nullptr;
+    return on;
+}
+double wall() { return omp_get_wtime(); }
+
 // ---------------------------------------------------------------- ACA
 // First index, in ascending order over the remaining set, of max |v|.
 // (2D/codes/aca.hpp:23-38); returns -1 on an empty set.
@@ -393,6 +403,10 @@ void aca(const Kernel &K, const std::vector<int> &rows, const std::vector<int> &
         for (int q = 0; q < rank; ++q) normS += 2 * std::fabs(vdot(o.U[q], u) * vdot(o.V[q], v));
         normS = std::sqrt(normS);
         ri = max_abs(col, rrem);
+        if (progress_on() && (double)N1 * N2 > 1e8 && rank % 100 == 0) {
+            std::fprintf(stderr, "[oracle]   cross %d x %d: rank %d\n", N1, N2, rank);
+            std::fflush(stderr);
+        }
     }
     o.rank = rank;
     o.L = Mat(rank, rank);
@@ -717,7 +731,7 @@ struct Oracle {
         if (dim == 1) return;
         // boxes of one level are independent (the reference loops over them serially; the pivots do
         // not depend on the order, the row/column gathers inside the cross then run serially)
-        for (int j = nL; j >= 2; --j)
+        for (int j = nL; j >= 2; --j) {
 #pragma omp parallel for schedule(dynamic, 1)
             for (int k = 0; k < nBoxes[j]; ++k) {
                 Box &B = tree[j][k];
@@ -738,10 +752,15 @@ struct Oracle {
                     B.L2P = Mat((int)rows.size(), 0);
                 }
             }
-        for (int j = 2; j <= nL; ++j) assemble_m2l(j, false);
+            note("far cross", j);
+        }
+        for (int j = 2; j <= nL; ++j) {
+            assemble_m2l(j, false);
+            note("far M2L", j);
+        }
     }
     void assemble_vertex_nested() {
-        for (int j = 1; j <= nL; ++j)
+        for (int j = 1; j <= nL; ++j) {
 #pragma omp parallel for schedule(dynamic, 1)
             for (int k = 0; k < nBoxes[j]; ++k) {
                 Box &B = tree[j][k];
@@ -762,7 +781,12 @@ struct Oracle {
                 for (int r : a.rb) B.Irv.push_back(B.pts[r]);
                 for (int c : a.cb) B.Icv.push_back(cols[c]);
             }
-        for (int j = 1; j <= nL; ++j) assemble_m2l(j, true);
+            note("vertex cross", j);
+        }
+        for (int j = 1; j <= nL; ++j) {
+            assemble_m2l(j, true);
+            note("vertex M2L", j);
+        }
         for (int j = 1; j <= nL; ++j)
 #pragma omp parallel for schedule(dynamic, 1)
             for (int k = 0; k < nBoxes[j]; ++k) {
@@ -826,7 +850,15 @@ struct Oracle {
             B.dense[nslot] = K.block(B.pts, B.pts);
         }
     }
+    void note(const char *what, int j) const {
+        if (progress_on()) {
+            std::fprintf(stderr, "[oracle] %s level %d done (%.1f s)\n", what, j, wall() - t_start);
+            std::fflush(stderr);
+        }
+    }
+    double t_start = 0.0;
     void assemble() {
+        t_start = wall();
         if (variant == 0) {
             assemble_far();
             assemble_vertex_nested();
@@ -839,6 +871,7 @@ struct Oracle {
             assemble_vertex_semi();
         }
         assemble_near();
+        note("near field", nL);
         find_memory();
         assembled = true;
     }
@@ -1241,6 +1274,8 @@ int orc_gmres(void *h, const double *b, double *x, int max_iter, double tol, dou
     return ((Oracle *)h)->gmres(b, x, max_iter, tol, resid, matvecs);
 }
 void orc_gl16(double *nodes, double *weights) { gauss_legendre16(nodes, weights); }
+// OpenMP threads a parallel region of the oracle runs on (the CPU baseline's core count)
+int orc_threads() { return omp_get_max_threads(); }
 // tree order: leaves' point lists concatenated
 void orc_tree_order(void *h, int *perm) {
     Oracle *o = (Oracle *)h;

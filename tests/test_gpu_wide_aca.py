@@ -10,6 +10,7 @@ import pytest
 
 import h2weak_amd as h2w
 import oracle
+from h2weak_amd import rank_mlp
 
 pytestmark = pytest.mark.gpu
 
@@ -69,3 +70,38 @@ def test_wide_cross_with_empty_boxes(wide):
     x = np.random.default_rng(1).normal(size=N)
     yo, yg = o.mat_vec_prod(x), g.mat_vec_prod(x)
     assert np.linalg.norm(yg - yo) <= 1e-6 * np.linalg.norm(yo)
+
+
+@pytest.mark.parametrize("N", [4096, 12000])
+def test_fixed_rank_phase_matches_oracle(wide, N):
+    """With the rank model set, every wide block first takes a fixed number of pivot steps (1.03 x its
+    predicted rank + 2) without the stopping test, then the reference's stopping rule is applied to all
+    of them at once from their Gram products: the block ends where the adaptive cross would have, or
+    continues adaptively when the model under-predicted.  Either way the skeletons are the oracle's."""
+    P = np.random.default_rng(7).uniform(-1, 1, (3, N))
+    o = oracle.Oracle(3, N, 64, 1.0, 10, 1, 0, points=P)
+    o.tree_initialisation()
+    g = h2w.Initialisation(3, N, 64, 1.0, 10, h2w.CHOICE_RBF, h2w.NESTED, points=P)
+    g.set_rank_model(rank_mlp.load(rank_mlp.default_model_path(3, 1, 10))[0])
+    g.tree_initialisation()
+    st = g.stats()
+    assert st["rank_fixed_blocks"] > 0
+    tot = same = 0
+    for f in (0, 1):
+        for j in range(2 if f == 0 else 1, o.nlevels + 1):
+            for b in range(8 ** j):
+                tot += 1
+                same += g.skeleton(f, j, b) == o.pivots(f, j, b)
+    print(f"N={N}: {st['rank_fixed_blocks']} fixed-rank blocks, {st['rank_fixed_resumed']} resumed, "
+          f"{st['rank_fixed_extra_steps']} extra steps, sampled error {st['rank_fixed_sampled_err']:.2e}; "
+          f"{same}/{tot} skeletons equal")
+    assert same / tot >= 0.98
+    assert st["rank_fixed_sampled_err"] <= 1e-8
+    x = np.random.default_rng(1).normal(size=N)
+    yo, yg = o.mat_vec_prod(x), g.mat_vec_prod(x)
+    if same == tot:
+        assert st["memory"] == o.memory
+        assert np.linalg.norm(yg - yo) <= 1e-12 * np.linalg.norm(yo)
+    else:
+        assert st["memory"] == pytest.approx(o.memory, rel=2e-3)
+        assert np.linalg.norm(yg - yo) <= 1e-9 * np.linalg.norm(yo)
