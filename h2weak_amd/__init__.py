@@ -25,7 +25,7 @@ LIB_PATH = os.path.join(_HERE, "libh2w.so")
 
 H2W_OK, H2W_EINVAL, H2W_ESTATE, H2W_EHIP, H2W_ENOMEM, H2W_ENOCONV, H2W_ENOLIB = range(7)
 NESTED, SEMI_NESTED, NON_NESTED = 0, 1, 2
-CHOICE_IE, CHOICE_RBF = 0, 1
+CHOICE_IE, CHOICE_RBF, CHOICE_HELMHOLTZ, CHOICE_HANKEL = 0, 1, 2, 10
 HOST, DEVICE = 0, 1
 FP64, FP32, FP32_OTF, FP64_OTF, FP32_INNER = 0, 1, 2, 3, 4
 ORDER_INPUT, ORDER_TREE = 0, 1
@@ -67,7 +67,7 @@ class Stats(ctypes.Structure):
                 ("rank_pred_under", ctypes.c_int), ("rank_pred_mae_log2", ctypes.c_double),
                 ("inner_tol_exp", ctypes.c_double), ("rank_fixed_blocks", ctypes.c_int),
                 ("rank_fixed_resumed", ctypes.c_int), ("rank_fixed_extra_steps", ctypes.c_int64),
-                ("rank_fixed_sampled_err", ctypes.c_double)]
+                ("rank_fixed_sampled_err", ctypes.c_double), ("diag_im", ctypes.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -191,6 +191,8 @@ class Initialisation:
         _check(lib().h2w_create(dim, variant, int(N), pp, int(n_max), float(L), int(tol_pow), int(choice),
                                 ctypes.byref(o), ctypes.byref(h)))
         self._h = h
+        # complex kernels (choice 2 / 10): vectors are complex128 (interleaved re, im in the C ABI)
+        self.cplx = int(choice) in (CHOICE_HELMHOLTZ, CHOICE_HANKEL)
 
     def __del__(self):
         h = getattr(self, "_h", None)
@@ -208,29 +210,35 @@ class Initialisation:
             _check(lib().h2w_matvec_ex(self._h, ctypes.c_void_p(charge.data_ptr()),
                                        ctypes.c_void_p(output.data_ptr()), 1, self.N, precision, ORDER_INPUT, DEVICE))
             return output
+        if self.cplx:
+            x = np.ascontiguousarray(charge, dtype=np.complex128)
+            y = np.empty(self.N, dtype=np.complex128)
+            _check(lib().h2w_matvec_ex(self._h, _ptr(x), _ptr(y), 1, self.N, precision, ORDER_INPUT, HOST))
+            return y
         x = _f64(charge)
         y = np.empty(self.N) if output is None else output
         _check(lib().h2w_matvec_ex(self._h, _ptr(x), _ptr(y), 1, self.N, precision, ORDER_INPUT, HOST))
         return y
 
     def matvec_multi(self, X, precision=FP64):
-        X = _f64(X)
+        X = np.ascontiguousarray(X, dtype=np.complex128) if self.cplx else _f64(X)
         nrhs = X.shape[0]
         Y = np.empty_like(X)
         _check(lib().h2w_matvec_ex(self._h, _ptr(X), _ptr(Y), nrhs, self.N, precision, ORDER_INPUT, HOST))
         return Y
 
     def dense_rows(self, rows, x):
-        x = _f64(x)
+        x = np.ascontiguousarray(x, dtype=np.complex128) if self.cplx else _f64(x)
         rows = None if rows is None else np.ascontiguousarray(rows, dtype=np.int64)
         n = self.N if rows is None else len(rows)
-        y = np.empty(n)
+        y = np.empty(n, dtype=x.dtype)
         _check(lib().h2w_dense_matvec(self._h, _ptr(x), _ptr(y), None if rows is None else _ptr(rows), n, HOST))
         return y
 
     def gmres(self, b, x0=None, max_iter=500, tol=1e-12, mode=GMRES_FP64):
-        b = _f64(b)
-        x = np.ones(self.N) if x0 is None else np.array(x0, dtype=np.float64)
+        dt = np.complex128 if self.cplx else np.float64
+        b = np.ascontiguousarray(b, dtype=dt)
+        x = np.ones(self.N, dtype=dt) if x0 is None else np.array(x0, dtype=dt)
         st = GmresStats()
         rc = lib().h2w_gmres(self._h, _ptr(b), _ptr(x), int(max_iter), float(tol), mode, HOST, ctypes.byref(st))
         if rc not in (H2W_OK, H2W_ENOCONV):
@@ -313,6 +321,7 @@ class Initialisation:
         st = Stats()
         _check(lib().h2w_get_stats(h, ctypes.byref(st)))
         self.dim, self.N = st.dim, st.N
+        self.cplx = st.choice in (CHOICE_HELMHOLTZ, CHOICE_HANKEL)
         self.nLevels = st.nlevels
         return self
 

@@ -13,7 +13,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_obj")
 LIB = os.path.join(HERE, "libh2w.so")
-SOURCES = ["plan.cpp", "kernels.hip", "h2w_ops.hip", "multi.hip", "batch.hip", "serialize.hip", "noalloc.hip"]
+SOURCES = ["plan.cpp", "kernels.hip", "h2w_ops.hip", "multi.hip", "batch.hip", "serialize.hip", "noalloc.hip", "complex.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("H2W_OFFLOAD_ARCH", "gfx950")
 FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-ffp-contract=off",

@@ -276,6 +276,7 @@ int h2w_batch_create(h2w_handle *const *systems, int nsys, h2w_batch **out) {
     for (int s = 0; s < nsys; ++s) {
         h2w_handle *h = systems[s];
         if (!h || !h->assembled) throw Error{H2W_ESTATE, "every system of a batch must be assembled"};
+        if (h->cplx) throw Error{H2W_EINVAL, "complex kernels are not batched (one h2w_gmres per system)"};
         if (h->opts.device != B->device) throw Error{H2W_EINVAL, "the systems of a batch share one device"};
         B->sys.push_back(h);
         B->inner_gen.push_back(h->inner_gen);

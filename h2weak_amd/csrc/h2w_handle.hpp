@@ -256,6 +256,9 @@ struct h2w_handle {
     std::vector<int32_t> sample_rank;                                 // last assembly with its rank
     hipStream_t stream = nullptr;
     h2w::KParams kp;
+    bool cplx = false;                  // complex kernel (choice 2 / 10): complex pools, vectors, GMRES
+    double diag_im = 0.0;               // imaginary part of the diagonal value
+    h2w::KParamsC kpc;
     h2w::DevBuf<double> d_px, d_py, d_pz;
     h2w::DevBuf<int32_t> d_perm;
     h2w::DevBuf<int32_t> d_iperm;          // input index -> tree index

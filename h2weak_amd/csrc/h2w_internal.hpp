@@ -36,6 +36,15 @@ struct KParams {
     const double *px, *py, *pz;  // tree-order coordinates (device)
 };
 
+// The same for the complex kernels (choice 2: 3D Helmholtz, 10: 2D Hankel); the diagonal
+// value is complex (diag + i diag_im).
+struct KParamsC {
+    int dim;
+    int choice;
+    double diag, diag_im;
+    const double *px, *py, *pz;
+};
+
 // Coordinates and point id of the point each vector-pool slot belongs to
 // (x/y slots: the tree-order points; o/u slots: the skeleton points), so that
 // near-field and M2L blocks can be evaluated on the fly from the work lists.
@@ -278,5 +287,31 @@ void launch_bgemv16(const double *V, int64_t ldv, int k, const double *h, int S,
                     const int32_t *seg, hipStream_t s);
 void launch_bscale16(const double *src, const double *sc, double *dst, int64_t n, const int32_t *seg, hipStream_t s);
 int mdot_blocks(int64_t n);
+
+// complex kernels (complex.hip): operator and vector pools of double2 (re, im), element offsets as
+// in the real path
+void launch_aca_c(const AcaJob *jobs, int njobs, const int32_t *idx, double2 *ws, int64_t ws_cap,
+                  unsigned long long *ws_top, int64_t *tab, uint8_t *flags, int32_t *rank, int32_t *status,
+                  int32_t *rowpiv, int32_t *colpiv, KParamsC kp, double tol, double *hist, int big, hipStream_t s);
+void launch_lr_extract_c(const LrJob *jobs, int njobs, const double2 *ws, const int64_t *tab, const int32_t *rowpiv,
+                         const int32_t *colpiv, double2 *out, hipStream_t s);
+void launch_eval_c(const EvalJob *jobs, int njobs, const int32_t *idx, double2 *A, KParamsC kp, hipStream_t s);
+void launch_xfer_c(const XferJob *jobs, int njobs, const int32_t *idx, const double2 *lr, double2 *A, KParamsC kp,
+                   hipStream_t s);
+void launch_cur_c(const CurJob *jobs, int njobs, const int32_t *idx, const double2 *lr, double2 *A, KParamsC kp,
+                  hipStream_t s);
+void launch_cgemv(const GItem *items, int nitems, const GTerm *terms, const double2 *A, double2 *vec, hipStream_t s);
+void launch_cfill(double2 *a, int64_t n, hipStream_t s);
+void launch_cpermute_in(const double2 *x, double2 *xt, const int32_t *perm, int64_t N, hipStream_t s);
+void launch_cpermute_out(const double2 *yt, double2 *y, const int32_t *perm, int64_t N, hipStream_t s);
+void launch_cdense_rows(const int32_t *rows, int64_t nrows, const double2 *xt, double2 *y, int64_t N, KParamsC kp,
+                        hipStream_t s);
+int cmdot_blocks(int64_t n);
+void launch_cmdot(const double2 *V, int64_t ldv, int k, const double2 *w, int64_t n, double2 *partial, int nblk,
+                  hipStream_t s);
+void launch_cmdot_reduce(const double2 *partial, int nblk, int k, double2 *out, hipStream_t s);
+void launch_cmgemv(const double2 *V, int64_t ldv, int k, const double2 *h, double2 *w, int64_t n, double sign,
+                   hipStream_t s);
+void launch_caxpby(double2 a, const double2 *x, double2 b, double2 *y, int64_t n, hipStream_t s);
 
 }  // namespace h2w

@@ -17,6 +17,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <complex>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -83,6 +84,36 @@ void gl16(double *nodes, double *weights) {
         nodes[2 * i + 1] = pos[s];
         weights[2 * i] = weights[2 * i + 1] = wts[s];
     }
+}
+
+// Diagonal of the complex kernels.  3D Helmholtz (choice 2): 500 sqrt(N) (3D/codes/kernel.hpp:132-144).
+// 2D Hankel (choice 10): (1 + 15 log N + Kii) / h^2 with h = N^(-1/2) and the complex
+// Kii = 4 GL16^2 of (i/4) H0^(1)(R) = (i J0(R) - Y0(R)) / 4 on [0, h/2]^2 (2D/codes/kernel.hpp:126-135,
+// 2D/codes/integral2d.hpp:16-57,60-90).
+void diag_value_c(int dim, int choice, int64_t N, double &re, double &im) {
+    if (choice == 2) {
+        re = 500.0 * std::sqrt(double(N));
+        im = 0.0;
+        return;
+    }
+    double nd[16], w[16];
+    gl16(nd, w);
+    const double h = 1.0 / std::sqrt(double(N));
+    const double cx = 0.5 * (0.5 * h), Lx = cx;
+    double kr = 0.0, ki = 0.0;
+    for (int i = 0; i < 16; ++i)
+        for (int j = 0; j < 16; ++j) {
+            const double tx = cx * nd[i] + Lx, ty = cx * nd[j] + Lx;
+            const double R = std::sqrt(tx * tx + ty * ty);
+            kr += w[i] * w[j] * (-::y0(R) / 4.0);
+            ki += w[i] * w[j] * (::j0(R) / 4.0);
+        }
+    kr *= cx * cx * 4;
+    ki *= cx * cx * 4;
+    const double h2 = std::pow(h, 2.0);
+    re = (1.0 + 15.0 * std::log(double(N)) + kr) / h2;
+    im = ki / h2;
+    (void)dim;
 }
 
 // diagonal value of A (the i == j branch of getMatrixEntry):
@@ -412,9 +443,13 @@ void run_aca(Handle &h, std::vector<AcaBlock> &blocks, int rank_hint, AcaPool &p
         const double pr = pred ? std::max(0.0, std::exp2((double)plog[b]) - 1.0) : 0.0;
         return std::max(2, (int)std::lround(1.03 * pr) + 2);
     };
+    // complex kernels: workspace, L and R in complex elements (two doubles each), every block through
+    // the one-workgroup cross
+    const bool cx = h.cplx;
+    const int esz = cx ? 2 : 1;
     int64_t need = 0;
     for (size_t b = 0; b < blocks.size(); ++b) need += (int64_t)(blocks[b].rows.size() + blocks[b].cols.size()) * est[b];
-    ensure_pool(h, pool, need);
+    ensure_pool(h, pool, need * esz);
     h.aca_pool_bytes = std::max<int64_t>(h.aca_pool_bytes, pool.cap * 8);
     DevBuf<uint8_t> flags;
     DevBuf<int32_t> didx, drank, dstat, drp, dcp;
@@ -425,7 +460,8 @@ void run_aca(Handle &h, std::vector<AcaBlock> &blocks, int rank_hint, AcaPool &p
     DevBuf<double> dlrout;
     const size_t kBig = 16384;    // m + n above which a block gets a 1024-thread workgroup
     // m + n above which a block's cross is spread over many workgroups (H2W_WIDE_MIN overrides)
-    const size_t kHuge = std::getenv("H2W_WIDE_MIN") ? (size_t)std::atoll(std::getenv("H2W_WIDE_MIN")) : 8192;
+    const size_t kHuge = cx ? SIZE_MAX
+                            : std::getenv("H2W_WIDE_MIN") ? (size_t)std::atoll(std::getenv("H2W_WIDE_MIN")) : 8192;
     while (!todo.empty()) {
         std::vector<size_t> rerun;
         size_t pos = 0;
@@ -436,7 +472,7 @@ void run_aca(Handle &h, std::vector<AcaBlock> &blocks, int rank_hint, AcaPool &p
                 size_t b = todo[pos];
                 const AcaBlock &B = blocks[b];
                 int64_t nb = (int64_t)(B.rows.size() + B.cols.size()) * est[b];
-                if (!batch.empty() && need + nb > pool.cap) break;
+                if (!batch.empty() && need + nb > pool.cap / esz) break;
                 batch.push_back(b);
                 need += nb;
                 isz += (int64_t)(B.rows.size() + B.cols.size());
@@ -486,14 +522,25 @@ void run_aca(Handle &h, std::vector<AcaBlock> &blocks, int rank_hint, AcaPool &p
             H2W_HIP(hipMemsetAsync(pool.top.p, 0, sizeof(unsigned long long), s));
             ++h.aca_batches;
             double tb = now_s();
-            if (nhuge > 0) run_wide(h, jobs, nhuge, djobs.p, didx.p, pool, dtab.p, flags.p, drank.p, dstat.p, drp.p,
-                                    dcp.p, dhist.p, tol);
-            if (nbig > nhuge)
-                launch_aca(djobs.p + nhuge, (int)(nbig - nhuge), didx.p, pool.ws.p, pool.cap, pool.top.p, dtab.p,
-                           flags.p, drank.p + nhuge, dstat.p + nhuge, drp.p, dcp.p, h.kp, tol, dhist.p, 1, s);
-            if (batch.size() > nbig)
-                launch_aca(djobs.p + nbig, (int)(batch.size() - nbig), didx.p, pool.ws.p, pool.cap, pool.top.p,
-                           dtab.p, flags.p, drank.p + nbig, dstat.p + nbig, drp.p, dcp.p, h.kp, tol, dhist.p, 0, s);
+            if (cx) {
+                double2 *wsc = reinterpret_cast<double2 *>(pool.ws.p);
+                if (nbig > 0)
+                    launch_aca_c(djobs.p, (int)nbig, didx.p, wsc, pool.cap / 2, pool.top.p, dtab.p, flags.p, drank.p,
+                                 dstat.p, drp.p, dcp.p, h.kpc, tol, dhist.p, 1, s);
+                if (batch.size() > nbig)
+                    launch_aca_c(djobs.p + nbig, (int)(batch.size() - nbig), didx.p, wsc, pool.cap / 2, pool.top.p,
+                                 dtab.p, flags.p, drank.p + nbig, dstat.p + nbig, drp.p, dcp.p, h.kpc, tol, dhist.p, 0, s);
+            } else {
+                if (nhuge > 0)
+                    run_wide(h, jobs, nhuge, djobs.p, didx.p, pool, dtab.p, flags.p, drank.p, dstat.p, drp.p, dcp.p,
+                             dhist.p, tol);
+                if (nbig > nhuge)
+                    launch_aca(djobs.p + nhuge, (int)(nbig - nhuge), didx.p, pool.ws.p, pool.cap, pool.top.p, dtab.p,
+                               flags.p, drank.p + nhuge, dstat.p + nhuge, drp.p, dcp.p, h.kp, tol, dhist.p, 1, s);
+                if (batch.size() > nbig)
+                    launch_aca(djobs.p + nbig, (int)(batch.size() - nbig), didx.p, pool.ws.p, pool.cap, pool.top.p,
+                               dtab.p, flags.p, drank.p + nbig, dstat.p + nbig, drp.p, dcp.p, h.kp, tol, dhist.p, 0, s);
+            }
             H2W_HIP(hipGetLastError());
             std::vector<int32_t> rk(batch.size()), st(batch.size()), rp(po), cp(po);
             std::vector<double> hs(2 * po);
@@ -556,15 +603,20 @@ void run_aca(Handle &h, std::vector<AcaBlock> &blocks, int rank_hint, AcaPool &p
             }
             if (!lj.empty()) {
                 dlr.upload(lj, s);
-                dlrout.ensure(lo);
-                launch_lr_extract(dlr.p, (int)lj.size(), pool.ws.p, dtab.p, drp.p, dcp.p, dlrout.p, s);
+                dlrout.ensure(lo * esz);
+                if (cx)
+                    launch_lr_extract_c(dlr.p, (int)lj.size(), reinterpret_cast<const double2 *>(pool.ws.p), dtab.p,
+                                        drp.p, dcp.p, reinterpret_cast<double2 *>(dlrout.p), s);
+                else launch_lr_extract(dlr.p, (int)lj.size(), pool.ws.p, dtab.p, drp.p, dcp.p, dlrout.p, s);
                 H2W_HIP(hipGetLastError());
-                std::vector<double> hl(lo);
-                H2W_HIP(hipMemcpyAsync(hl.data(), dlrout.p, lo * 8, hipMemcpyDeviceToHost, s));
+                std::vector<double> hl(lo * esz);
+                H2W_HIP(hipMemcpyAsync(hl.data(), dlrout.p, lo * esz * 8, hipMemcpyDeviceToHost, s));
                 H2W_HIP(hipStreamSynchronize(s));
                 for (size_t q = 0; q < lj.size(); ++q) {
                     AcaBlock &B = blocks[lb[q]];
-                    B.lr.assign(hl.begin() + lj[q].out, hl.begin() + lj[q].out + 2LL * lj[q].rank * lj[q].rank);
+                    // (L then R, r x r each; complex entries as (re, im) pairs)
+                    B.lr.assign(hl.begin() + lj[q].out * esz,
+                                hl.begin() + (lj[q].out + 2LL * lj[q].rank * lj[q].rank) * esz);
                 }
             }
         }
@@ -573,7 +625,7 @@ void run_aca(Handle &h, std::vector<AcaBlock> &blocks, int rank_hint, AcaPool &p
             // re-runs go with larger estimates: let the workspace grow to hold them
             int64_t need2 = 0;
             for (size_t b : todo) need2 += (int64_t)(blocks[b].rows.size() + blocks[b].cols.size()) * est[b];
-            ensure_pool(h, pool, need2);
+            ensure_pool(h, pool, need2 * esz);
         }
     }
 }
@@ -1485,10 +1537,13 @@ void assemble_operators(Handle &h, double t0) {
     }
     h.mat_len = std::max<int64_t>(mo, 32);
     h.memory = memory;
-    h.mat64.alloc(h.mat_len);
-    H2W_HIP(hipMemsetAsync(h.mat64.p, 0, h.mat_len * 8, s));
-    h.vec64.alloc(h.vec_len);
-    h.vec32.alloc(h.vec_len);
+    // (complex kernels: pools of (re, im) pairs at the same element offsets)
+    const int esz = h.cplx ? 2 : 1;
+    h.mat64.alloc(h.mat_len * esz);
+    H2W_HIP(hipMemsetAsync(h.mat64.p, 0, h.mat_len * esz * 8, s));
+    h.vec64.alloc(h.vec_len * esz);
+    if (h.cplx) h.vec32.release();
+    else h.vec32.alloc(h.vec_len);
     {
         // the point of every vector slot, for on-the-fly near-field / M2L evaluation
         std::vector<int32_t> pid(h.vec_len, -1);
@@ -1561,7 +1616,7 @@ void assemble_operators(Handle &h, double t0) {
                     idx.insert(idx.end(), fl.S[b].begin(), fl.S[b].end());
                     J.ic = (int64_t)idx.size();
                     idx.insert(idx.end(), fl.Ic[b].begin(), fl.Ic[b].end());
-                    J.lr = (int64_t)lr.size();
+                    J.lr = (int64_t)lr.size() / esz;
                     lr.insert(lr.end(), fl.lr[b].begin(), fl.lr[b].end());
                     J.e = fl.e_off[b];
                     J.et = fl.et_off[b];
@@ -1583,7 +1638,7 @@ void assemble_operators(Handle &h, double t0) {
             J.v = (int64_t)idx.size();
             auto vr = range_list(P.bstart[p.level][p.v], J.n);
             idx.insert(idx.end(), vr.begin(), vr.end());
-            J.lr = (int64_t)lr.size();
+            J.lr = (int64_t)lr.size() / esz;
             lr.insert(lr.end(), p.lr.begin(), p.lr.end());
             J.a = p.ar_off;
             J.lda = (int32_t)pad_ld(J.r);
@@ -1600,9 +1655,15 @@ void assemble_operators(Handle &h, double t0) {
         dxj.upload(xj, s);
         cj = split_cur(cj);
         dcj.upload(cj, s);
-        launch_xfer(dxj.p, (int)xj.size(), didx.p, dlr.p, h.mat64.p, h.kp, s);
-        H2W_HIP(hipGetLastError());
-        launch_cur(dcj.p, (int)cj.size(), didx.p, dlr.p, h.mat64.p, h.kp, s);
+        if (h.cplx) {
+            launch_xfer_c(dxj.p, (int)xj.size(), didx.p, reinterpret_cast<const double2 *>(dlr.p),
+                          reinterpret_cast<double2 *>(h.mat64.p), h.kpc, s);
+            launch_cur_c(dcj.p, (int)cj.size(), didx.p, reinterpret_cast<const double2 *>(dlr.p),
+                         reinterpret_cast<double2 *>(h.mat64.p), h.kpc, s);
+        } else {
+            launch_xfer(dxj.p, (int)xj.size(), didx.p, dlr.p, h.mat64.p, h.kp, s);
+            launch_cur(dcj.p, (int)cj.size(), didx.p, dlr.p, h.mat64.p, h.kp, s);
+        }
         H2W_HIP(hipGetLastError());
         H2W_HIP(hipStreamSynchronize(s));
     }
@@ -1630,7 +1691,8 @@ void assemble_operators(Handle &h, double t0) {
             if (ej.empty()) return;
             didx.upload(idx, s);
             dej.upload(ej, s);
-            launch_eval(dej.p, (int)ej.size(), didx.p, h.mat64.p, h.kp, s);
+            if (h.cplx) launch_eval_c(dej.p, (int)ej.size(), didx.p, reinterpret_cast<double2 *>(h.mat64.p), h.kpc, s);
+            else launch_eval(dej.p, (int)ej.size(), didx.p, h.mat64.p, h.kp, s);
             H2W_HIP(hipGetLastError());
             H2W_HIP(hipStreamSynchronize(s));
             idx.clear();
@@ -1856,6 +1918,11 @@ bool is32(int precision) { return precision == H2W_FP32 || precision == H2W_FP32
 // precisions evaluate the blocks from the points instead, over the directed lists.
 void run_launch(Handle &h, const Launch &Lh, int precision) {
     hipStream_t s = h.stream;
+    if (h.cplx) {  // complex operators: FP64 only, every role through the complex GEMV
+        launch_cgemv(h.d_items.p + Lh.item0, (int)Lh.nitems, h.d_terms.p, reinterpret_cast<const double2 *>(h.mat64.p),
+                     reinterpret_cast<double2 *>(h.vec64.p), s);
+        return;
+    }
     const bool otf = (precision == H2W_FP32_OTF || precision == H2W_FP64_OTF) && Lh.role == 1;
     if (otf) {
         for (auto &Lo : h.otf.launches) {
@@ -1876,7 +1943,8 @@ void run_launch(Handle &h, const Launch &Lh, int precision) {
 }
 
 void zero_outputs(Handle &h, int precision) {
-    if (is32(precision)) launch_fill32(h.vec32.p + h.off_y, h.vec_len - h.off_y, 0.0f, h.stream);
+    if (h.cplx) launch_cfill(reinterpret_cast<double2 *>(h.vec64.p) + h.off_y, h.vec_len - h.off_y, h.stream);
+    else if (is32(precision)) launch_fill32(h.vec32.p + h.off_y, h.vec_len - h.off_y, 0.0f, h.stream);
     else launch_fill(h.vec64.p + h.off_y, h.vec_len - h.off_y, 0.0, h.stream);
 }
 
@@ -1944,6 +2012,20 @@ void drop_graphs(Handle &h) {
 void matvec_dev(Handle &h, const double *x, double *y, int precision, bool tree) {
     hipStream_t s = h.stream;
     const int64_t N = h.N;
+    if (h.cplx) {
+        // complex vectors: N (re, im) pairs
+        if (precision != H2W_FP64) throw Error{H2W_EINVAL, "complex kernels: precision H2W_FP64 only"};
+        double2 *pool = reinterpret_cast<double2 *>(h.vec64.p);
+        const double2 *xc = reinterpret_cast<const double2 *>(x);
+        double2 *yc = reinterpret_cast<double2 *>(y);
+        if (tree) H2W_HIP(hipMemcpyAsync(pool + h.off_x, xc, N * 16, hipMemcpyDeviceToDevice, s));
+        else launch_cpermute_in(xc, pool + h.off_x, h.d_perm.p, N, s);
+        run_launches(h, precision);
+        if (tree) H2W_HIP(hipMemcpyAsync(yc, pool + h.off_y, N * 16, hipMemcpyDeviceToDevice, s));
+        else launch_cpermute_out(pool + h.off_y, yc, h.d_perm.p, N, s);
+        H2W_HIP(hipGetLastError());
+        return;
+    }
     if (is32(precision)) {
         if (!h.mat32.p) throw Error{H2W_ESTATE, "FP32 matvec requested but the FP32 shadow was not built"};
         if (tree) launch_to_fp32(x, h.vec32.p + h.off_x, N, s);
@@ -2196,6 +2278,130 @@ bool keep_space(Handle &h, const GmresOut &o, Recycled &rc) {
 
 }  // namespace
 
+// Complex GMRES (the reference's -DUSE_Complex / USE_Helm / USE_Hankel branch of
+// iterative_solver::GMRES, 1D/codes/gmres.hpp:52-67 and :91-200) on the tree-ordered complex
+// right-hand side bt (N (re, im) pairs, device), starting from and updating h.g_x.  Unrestarted,
+// Arnoldi by classical Gram-Schmidt twice with the Hermitian inner product h_{k,i} = v_k^H w, complex
+// Givens rotations G = [c, s; -conj(s), c] with real c = |dx| / n and s = (dx / |dx|) conj(dy) / n,
+// resid = |s_{i+1}| / ||b||, back-substitution.  (The reference's branch forms h_{k,i} as
+// w.dot(v_k) = w^H v_k, the conjugate, and applies -s instead of -conj(s); for complex data those
+// leave the Krylov basis non-orthogonal and the subdiagonal unreduced, so this restatement keeps the
+// textbook forms, which coincide with the reference's for real data.)
+h2w_gmres_stats solve_tree_c(Handle &h, const double *bt, int max_iter, double tol) {
+    using cd = std::complex<double>;
+    const int64_t N = h.N;
+    hipStream_t s = h.stream;
+    const double t0 = now_s();
+    h2w_gmres_stats S;
+    std::memset(&S, 0, sizeof(S));
+    double2 *x = reinterpret_cast<double2 *>(h.g_x.p);
+    const double2 *b = reinterpret_cast<const double2 *>(bt);
+    h.g_w.ensure(2 * N);
+    h.g_r.ensure(2 * N);
+    double2 *w = reinterpret_cast<double2 *>(h.g_w.p), *r = reinterpret_cast<double2 *>(h.g_r.p);
+    const int nb = cmdot_blocks(N);
+    const int m = max_iter;
+    h.g_part.ensure((size_t)2 * nb * (m + 2));
+    h.g_h.ensure((size_t)2 * (2 * (m + 2) + 2));
+    double2 *part = reinterpret_cast<double2 *>(h.g_part.p), *hv = reinterpret_cast<double2 *>(h.g_h.p);
+    auto norm2 = [&](const double2 *v) {
+        launch_cmdot(v, N, 1, v, N, part, nb, s);
+        launch_cmdot_reduce(part, nb, 1, hv, s);
+        double2 q;
+        H2W_HIP(hipMemcpyAsync(&q, hv, 16, hipMemcpyDeviceToHost, s));
+        H2W_HIP(hipStreamSynchronize(s));
+        return std::sqrt(std::max(0.0, q.x));
+    };
+    // basis: (m + 1) complex vectors in g_V
+    const int64_t vcols = std::min<int64_t>(m + 1, 1024);
+    if ((int64_t)h.g_V.n < vcols * 2 * N) {
+        h.g_V.alloc((size_t)vcols * 2 * N);
+        h.g_Vcols = 0;  // (the real basis layout is rebuilt on its next use)
+    }
+    double2 *V = reinterpret_cast<double2 *>(h.g_V.p);
+    double normb = norm2(b);
+    if (normb == 0.0) normb = 1.0;
+    matvec_dev(h, h.g_x.p, h.g_w.p, H2W_FP64, true);
+    S.matvecs_fp64 = 1;
+    H2W_HIP(hipMemcpyAsync(r, b, N * 16, hipMemcpyDeviceToDevice, s));
+    launch_caxpby(make_double2(-1.0, 0.0), w, make_double2(1.0, 0.0), r, N, s);
+    const double beta = norm2(r);
+    S.resid = beta / normb;
+    if (S.resid <= tol) {
+        S.converged = 1;
+        S.seconds = now_s() - t0;
+        return S;
+    }
+    const int mm = (int)std::min<int64_t>(m, vcols - 1);
+    std::vector<std::vector<cd>> H;
+    std::vector<cd> sv(mm + 1, cd(0.0)), cs(mm + 1, cd(0.0)), sn(mm + 1, cd(0.0));
+    launch_caxpby(make_double2(1.0 / beta, 0.0), r, make_double2(0.0, 0.0), V, N, s);
+    sv[0] = beta;
+    std::vector<double2> hh(2 * (mm + 2) + 2);
+    auto rot = [](cd &dx, cd &dy, cd c, cd sg) {
+        const cd t = c * dx + sg * dy;
+        dy = -std::conj(sg) * dx + c * dy;
+        dx = t;
+    };
+    for (int i = 0; i < mm; ++i) {
+        matvec_dev(h, reinterpret_cast<double *>(V + (int64_t)i * N), h.g_w.p, H2W_FP64, true);
+        S.matvecs_fp64++;
+        const int k = i + 1;
+        launch_cmdot(V, N, k, w, N, part, nb, s);
+        launch_cmdot_reduce(part, nb, k, hv, s);
+        launch_cmgemv(V, N, k, hv, w, N, -1.0, s);
+        launch_cmdot(V, N, k, w, N, part, nb, s);
+        launch_cmdot_reduce(part, nb, k, hv + k, s);
+        launch_cmgemv(V, N, k, hv + k, w, N, -1.0, s);
+        launch_cmdot(w, N, 1, w, N, part, nb, s);
+        launch_cmdot_reduce(part, nb, 1, hv + 2 * k, s);
+        H2W_HIP(hipMemcpyAsync(hh.data(), hv, (2 * k + 1) * 16, hipMemcpyDeviceToHost, s));
+        H2W_HIP(hipStreamSynchronize(s));
+        H.emplace_back(i + 2, cd(0.0));
+        std::vector<cd> &Hc = H.back();
+        for (int q = 0; q <= i; ++q) Hc[q] = cd(hh[q].x + hh[k + q].x, hh[q].y + hh[k + q].y);
+        const double hn = std::sqrt(std::max(0.0, hh[2 * k].x));
+        Hc[i + 1] = hn;
+        const bool breakdown = !(hn > 0.0);
+        if (breakdown) launch_cfill(V + (int64_t)(i + 1) * N, N, s);
+        else launch_caxpby(make_double2(1.0 / hn, 0.0), w, make_double2(0.0, 0.0), V + (int64_t)(i + 1) * N, N, s);
+        for (int q = 0; q < i; ++q) rot(Hc[q], Hc[q + 1], cs[q], sn[q]);
+        {
+            const cd dx = Hc[i], dy = Hc[i + 1];
+            if (std::abs(dx) == 0.0) {
+                cs[i] = 0.0;
+                sn[i] = 1.0;
+            } else {
+                const double sc = std::abs(dx) + std::abs(dy);
+                const double nrm = sc * std::sqrt(std::norm(dx / sc) + std::norm(dy / sc));
+                const cd alpha = dx / std::abs(dx);
+                cs[i] = std::abs(dx) / nrm;
+                sn[i] = alpha * std::conj(dy) / nrm;
+            }
+        }
+        rot(Hc[i], Hc[i + 1], cs[i], sn[i]);
+        rot(sv[i], sv[i + 1], cs[i], sn[i]);
+        S.resid = std::abs(sv[i + 1]) / normb;
+        S.iterations = i + 1;
+        if (S.resid < tol || i == mm - 1 || breakdown) {
+            std::vector<cd> y(sv.begin(), sv.begin() + i + 1);
+            for (int q = i; q >= 0; --q) {
+                y[q] /= H[q][q];
+                for (int jj = q - 1; jj >= 0; --jj) y[jj] -= H[q][jj] * y[q];
+            }
+            std::vector<double2> yd(i + 1);
+            for (int q = 0; q <= i; ++q) yd[q] = make_double2(y[q].real(), y[q].imag());
+            H2W_HIP(hipMemcpyAsync(hv, yd.data(), (i + 1) * 16, hipMemcpyHostToDevice, s));
+            launch_cmgemv(V, N, i + 1, hv, x, N, 1.0, s);
+            S.converged = S.resid < tol || (breakdown && S.resid == 0.0);
+            break;
+        }
+    }
+    H2W_HIP(hipStreamSynchronize(s));
+    S.seconds = now_s() - t0;
+    return S;
+}
+
 // GMRES on the tree-ordered right-hand side bt (device), starting from and
 // updating h.g_x: plain FP64 GMRES, or FP64 iterative refinement whose
 // corrections come from GMRES on an FP32 operator.
@@ -2422,6 +2628,13 @@ void attach_device(Handle &h) {
     h.kp.px = h.d_px.p;
     h.kp.py = h.d_py.p;
     h.kp.pz = h.d_pz.p;
+    h.kpc.dim = h.dim;
+    h.kpc.choice = h.choice;
+    h.kpc.diag = h.diag;
+    h.kpc.diag_im = h.diag_im;
+    h.kpc.px = h.d_px.p;
+    h.kpc.py = h.d_py.p;
+    h.kpc.pz = h.d_pz.p;
 }
 }  // namespace h2w
 
@@ -2470,7 +2683,12 @@ int h2w_create(int dim, int variant, int64_t N, const double *points, int n_max,
     if (variant != H2W_NESTED && variant != H2W_SEMI_NESTED && variant != H2W_NON_NESTED)
         throw Error{H2W_EINVAL, "unknown variant"};
     if (variant == H2W_NON_NESTED && dim == 1) throw Error{H2W_EINVAL, "H* is defined for 2D and 3D"};
-    if (choice != 0 && choice != 1) throw Error{H2W_EINVAL, "Wrong choice! choice must be 0 (IE) or 1 (RBF)"};
+    if (choice == H2W_CHOICE_HELMHOLTZ && dim != 3)
+        throw Error{H2W_EINVAL, "choice 2 (Helmholtz, e^{iR}/(4 pi)) is the 3D complex kernel"};
+    if (choice == H2W_CHOICE_HANKEL && dim != 2)
+        throw Error{H2W_EINVAL, "choice 10 (Hankel, (i/4) H0^(1)(R)) is the 2D complex kernel"};
+    if (choice != 0 && choice != 1 && choice != H2W_CHOICE_HELMHOLTZ && choice != H2W_CHOICE_HANKEL)
+        throw Error{H2W_EINVAL, "Wrong choice! choice must be 0 (IE), 1 (RBF), 2 (3D Helmholtz) or 10 (2D Hankel)"};
     if (N < 2 || N > INT32_MAX) throw Error{H2W_EINVAL, "N out of range"};
     if (n_max < 1) throw Error{H2W_EINVAL, "n_max must be positive"};
     if (!(L > 0.0)) throw Error{H2W_EINVAL, "L must be positive"};
@@ -2531,7 +2749,15 @@ int h2w_create(int dim, int variant, int64_t N, const double *points, int n_max,
     h->opts = o;
     h->use_graph = std::getenv("H2W_NO_GRAPH") == nullptr;
     h->plan.build(dim, N, points, L, nl);
-    h->diag = h2w::diag_value(dim, choice, N, o.compat);
+    h->cplx = choice == H2W_CHOICE_HELMHOLTZ || choice == H2W_CHOICE_HANKEL;
+    if (h->cplx) {
+        // complex kernels: FP64 operators only (no FP32 copies, no refinement operator)
+        h->opts.fp32_shadow = 0;
+        h->opts.ir_tol_pow = 0;
+        h2w::diag_value_c(dim, choice, N, h->diag, h->diag_im);
+    } else {
+        h->diag = h2w::diag_value(dim, choice, N, o.compat);
+    }
     if (!o.plan_only) h2w::attach_device(*h);
     *out = h.release();
     return H2W_OK;
@@ -2557,6 +2783,25 @@ int h2w_matvec_ex(h2w_handle *h, const double *x, double *y, int nrhs, int64_t l
     if (precision < H2W_FP64 || precision > H2W_FP32_INNER) throw Error{H2W_EINVAL, "bad precision"};
     H2W_HIP(hipSetDevice(h->opts.device));
     const int64_t N = h->N;
+    if (h->cplx) {
+        // complex kernels: x, y hold N (re, im) pairs per right-hand side, ld counts complex entries
+        if (precision != H2W_FP64) throw Error{H2W_EINVAL, "complex kernels: precision H2W_FP64 only"};
+        for (int r = 0; r < nrhs; ++r) {
+            const double *xr = x + 2 * r * ld;
+            double *yr = y + 2 * r * ld;
+            if (where == H2W_HOST) {
+                h->d_stage_in.ensure(2 * N);
+                h->d_stage_out.ensure(2 * N);
+                H2W_HIP(hipMemcpyAsync(h->d_stage_in.p, xr, N * 16, hipMemcpyHostToDevice, h->stream));
+                h2w::matvec_dev(*h, h->d_stage_in.p, h->d_stage_out.p, H2W_FP64, order == H2W_ORDER_TREE);
+                H2W_HIP(hipMemcpyAsync(yr, h->d_stage_out.p, N * 16, hipMemcpyDeviceToHost, h->stream));
+            } else {
+                h2w::matvec_dev(*h, xr, yr, H2W_FP64, order == H2W_ORDER_TREE);
+            }
+        }
+        H2W_HIP(hipStreamSynchronize(h->stream));
+        return H2W_OK;
+    }
     if (nrhs > 1 && (precision == H2W_FP64 || precision == H2W_FP32 || precision == H2W_FP32_INNER)) {
         // kNR right-hand sides per pass on the matrix cores
         const double *xs = x;
@@ -2607,6 +2852,38 @@ int h2w_gmres(h2w_handle *h, const double *b, double *x, int max_iter, double to
     if (!h->assembled) throw Error{H2W_ESTATE, "h2w_gmres before h2w_assemble"};
     if (max_iter < 1) throw Error{H2W_EINVAL, "max_iter must be positive"};
     if (mode < H2W_GMRES_FP64 || mode > H2W_GMRES_FP32_INNER_IR) throw Error{H2W_EINVAL, "bad mode"};
+    if (h->cplx) {
+        // complex kernels: b, x hold N (re, im) pairs; FP64 GMRES with the complex branch's rotations
+        if (mode != H2W_GMRES_FP64) throw Error{H2W_EINVAL, "complex kernels: mode H2W_GMRES_FP64 only"};
+        H2W_HIP(hipSetDevice(h->opts.device));
+        const int64_t N = h->N;
+        hipStream_t s = h->stream;
+        h->g_x.ensure(2 * N);
+        h->d_stage_in.ensure(2 * N);
+        h->d_stage_out.ensure(2 * N);
+        const double *bsrc = b, *xsrc = x;
+        if (where == H2W_HOST) {
+            H2W_HIP(hipMemcpyAsync(h->d_stage_in.p, b, N * 16, hipMemcpyHostToDevice, s));
+            H2W_HIP(hipMemcpyAsync(h->d_stage_out.p, x, N * 16, hipMemcpyHostToDevice, s));
+            bsrc = h->d_stage_in.p;
+            xsrc = h->d_stage_out.p;
+        }
+        h2w::DevBuf<double> bt;
+        bt.alloc(2 * N);
+        h2w::launch_cpermute_in(reinterpret_cast<const double2 *>(bsrc), reinterpret_cast<double2 *>(bt.p), h->d_perm.p, N,
+                                s);
+        h2w::launch_cpermute_in(reinterpret_cast<const double2 *>(xsrc), reinterpret_cast<double2 *>(h->g_x.p),
+                                h->d_perm.p, N, s);
+        H2W_HIP(hipStreamSynchronize(s));
+        h2w_gmres_stats S = h2w::solve_tree_c(*h, bt.p, max_iter, tol);
+        double *xdst = where == H2W_HOST ? h->d_stage_out.p : x;
+        h2w::launch_cpermute_out(reinterpret_cast<const double2 *>(h->g_x.p), reinterpret_cast<double2 *>(xdst),
+                                 h->d_perm.p, N, s);
+        if (where == H2W_HOST) H2W_HIP(hipMemcpyAsync(x, h->d_stage_out.p, N * 16, hipMemcpyDeviceToHost, s));
+        H2W_HIP(hipStreamSynchronize(s));
+        if (st) *st = S;
+        return S.converged ? H2W_OK : H2W_ENOCONV;
+    }
     if (mode != H2W_GMRES_FP64 && !h->mat32.p) throw Error{H2W_ESTATE, "FP32 shadow not built"};
     if (mode == H2W_GMRES_FP32_INNER_IR && h->inner.launches.empty())
         throw Error{H2W_ESTATE, "the inner operator was not built (opts.ir_tol_pow)"};
@@ -2646,6 +2923,7 @@ int h2w_gmres_multi(h2w_handle *h, const double *B, int64_t ldb, double *X, int6
     H2W_TRY
     if (!h || !B || !X || !st) throw Error{H2W_EINVAL, "NULL argument"};
     if (!h->assembled) throw Error{H2W_ESTATE, "h2w_gmres_multi before h2w_assemble"};
+    if (h->cplx) throw Error{H2W_EINVAL, "complex kernels: several right-hand sides go through h2w_gmres"};
     if (max_iter < 1 || nrhs < 1 || ldb < h->N || ldx < h->N) throw Error{H2W_EINVAL, "bad max_iter / nrhs / ld"};
     if (mode != H2W_GMRES_FP64 && mode != H2W_GMRES_FP32_IR && mode != H2W_GMRES_FP32_INNER_IR)
         throw Error{H2W_EINVAL, "several right-hand sides: modes FP64, FP32_IR and FP32_INNER_IR"};
@@ -2709,6 +2987,26 @@ int h2w_dense_matvec(h2w_handle *h, const double *x, double *y, const int64_t *r
     h2w::DevBuf<int32_t> drows;
     drows.upload(tr, s);
     h2w::DevBuf<double> xin, xt, yo;
+    if (h->cplx) {
+        // complex kernels: x (N) and y (nrows) hold (re, im) pairs
+        xt.alloc(2 * N);
+        yo.alloc(2 * std::max<int64_t>(nrows, 1));
+        const double *xs = x;
+        if (where == H2W_HOST) {
+            xin.alloc(2 * N);
+            H2W_HIP(hipMemcpyAsync(xin.p, x, N * 16, hipMemcpyHostToDevice, s));
+            xs = xin.p;
+        }
+        h2w::launch_cpermute_in(reinterpret_cast<const double2 *>(xs), reinterpret_cast<double2 *>(xt.p), h->d_perm.p, N,
+                                s);
+        h2w::launch_cdense_rows(drows.p, nrows, reinterpret_cast<const double2 *>(xt.p), reinterpret_cast<double2 *>(yo.p),
+                                N, h->kpc, s);
+        H2W_HIP(hipGetLastError());
+        H2W_HIP(hipMemcpyAsync(y, yo.p, nrows * 16, where == H2W_HOST ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice,
+                               s));
+        H2W_HIP(hipStreamSynchronize(s));
+        return H2W_OK;
+    }
     xt.alloc(N);
     yo.alloc(std::max<int64_t>(nrows, 1));
     const double *xs = x;
@@ -2775,7 +3073,8 @@ int h2w_get_stats(const h2w_handle *h, h2w_stats *st) {
             st->mean_rank_vertex = cnt ? sum / cnt : 0.0;
         }
     }
-    st->stored_bytes_fp64 = h->stored_elems * 8;
+    st->stored_bytes_fp64 = h->stored_elems * (h->cplx ? 16 : 8);
+    st->diag_im = h->diag_im;
     st->stored_bytes_fp32 = h->mat32.p ? h->stored_elems * 4 : 0;
     st->n_launches_matvec = (int)h->launches.size() + 3;
     st->stored_bytes_inner_fp32 = h->inner_elems * 4;
@@ -2874,14 +3173,18 @@ int h2w_bench_matvec(h2w_handle *h, int nrep, int precision, h2w_bench *out) {
     if (!h->assembled) throw Error{H2W_ESTATE, "not assembled"};
     if (precision < H2W_FP64 || precision > H2W_FP32_INNER) throw Error{H2W_EINVAL, "bad precision"};
     if (h2w::is32(precision) && !h->mat32.p) throw Error{H2W_ESTATE, "FP32 shadow not built"};
+    if (h->cplx && precision != H2W_FP64) throw Error{H2W_EINVAL, "complex kernels: precision H2W_FP64 only"};
     const auto &LL = h2w::launches_for(*h, precision);
     H2W_HIP(hipSetDevice(h->opts.device));
     std::memset(out, 0, sizeof(*out));
     hipStream_t s = h->stream;
     const int64_t N = h->N;
     // deterministic input in the pool's x segment
-    h2w::launch_fill(h->vec64.p + h->off_x, N, 1.0, s);
-    h2w::launch_fill32(h->vec32.p + h->off_x, N, 1.0f, s);
+    if (h->cplx) h2w::launch_fill(h->vec64.p + 2 * h->off_x, 2 * N, 1.0, s);
+    else {
+        h2w::launch_fill(h->vec64.p + h->off_x, N, 1.0, s);
+        h2w::launch_fill32(h->vec32.p + h->off_x, N, 1.0f, s);
+    }
     const int nl = std::min<int>((int)h2w::launches_for(*h, precision).size(), H2W_MAX_LAUNCHES - 1);
     std::vector<h2w::Event> ev(nl + 2);
     std::vector<double> acc(nl + 1, 0.0);
@@ -2905,7 +3208,7 @@ int h2w_bench_matvec(h2w_handle *h, int nrep, int precision, h2w_bench *out) {
         for (int q = 0; q <= nl; ++q) acc[q] += h2w::elapsed_ms(ev[q], ev[q + 1]);
     }
     out->n_launch = nl + 1;
-    const int es = h2w::is32(precision) ? 4 : 8;
+    const int es = h->cplx ? 16 : h2w::is32(precision) ? 4 : 8;
     out->launch_ms[0] = acc[0] / nrep;
     out->launch_bytes[0] = (h->vec_len - h->off_y) * (int64_t)es;
     out->launch_role[0] = -1;
@@ -2931,6 +3234,7 @@ int h2w_set_inner_tolerance(h2w_handle *h, double ir_tol_pow) {
     H2W_TRY
     if (!h) throw Error{H2W_EINVAL, "NULL handle"};
     if (!h->assembled) throw Error{H2W_ESTATE, "not assembled"};
+    if (h->cplx) throw Error{H2W_EINVAL, "complex kernels: FP64 operators only (no inner operator)"};
     if (!h->mat32.p) throw Error{H2W_ESTATE, "the inner operator needs the FP32 shadow (opts.fp32_shadow)"};
     if (!(ir_tol_pow >= 0.0) || (ir_tol_pow > 0.0 && ir_tol_pow >= h->tol_pow))
         throw Error{H2W_EINVAL, "ir_tol_pow must be 0 or in (0, tol_pow)"};
@@ -2945,6 +3249,7 @@ int h2w_autotune(h2w_handle *h, const double *b, int where, double tol, int max_
     if (!h || !out) throw Error{H2W_EINVAL, "NULL argument"};
     if (!h->assembled) throw Error{H2W_ESTATE, "not assembled"};
     if (flags & ~(H2W_TUNE_UNROLL | H2W_TUNE_INNER)) throw Error{H2W_EINVAL, "unknown flags"};
+    if (h->cplx) throw Error{H2W_EINVAL, "complex kernels: no autotuner (one complex GEMV kernel)"};
     if ((flags & H2W_TUNE_INNER) && (!b || max_iter < 1 || !(tol > 0.0)))
         throw Error{H2W_EINVAL, "inner tuning needs b, tol > 0 and max_iter >= 1"};
     if ((flags & H2W_TUNE_INNER) && !h->mat32.p) throw Error{H2W_ESTATE, "FP32 shadow not built"};

@@ -568,6 +568,7 @@ extern "C" int h2w_bench_matvec_multi(h2w_handle *h, int nrep, int precision, h2
     try {
         if (!h || !out || nrep < 1) throw Error{H2W_EINVAL, "bad argument"};
         if (!h->assembled) throw Error{H2W_ESTATE, "not assembled"};
+        if (h->cplx) throw Error{H2W_EINVAL, "complex kernels: no 16-wide product"};
         h2w::check_precision_m(*h, precision);
         H2W_HIP(hipSetDevice(h->opts.device));
         h2w::ensure_pool_m(*h, precision);

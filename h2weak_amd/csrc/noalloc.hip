@@ -73,6 +73,7 @@ int h2w_dense_matvec_noalloc(h2w_handle *h, const double *x, double *y, const in
     try {
         if (!h || !x || !y) throw Error{H2W_EINVAL, "NULL argument"};
         if (h->opts.plan_only || !h->stream) throw Error{H2W_ESTATE, "handle was created with plan_only"};
+        if (h->cplx) throw Error{H2W_EINVAL, "complex kernels: use h2w_dense_matvec"};
         const int64_t N = h->N;
         if (!rows) nrows = N;
         if (nrows < 0) throw Error{H2W_EINVAL, "nrows < 0"};

@@ -111,6 +111,7 @@ int h2w_save(const h2w_handle *h, const char *path) {
     try {
         if (!h || !path) throw Error{H2W_EINVAL, "NULL argument"};
         if (!h->assembled) throw Error{H2W_ESTATE, "h2w_save before h2w_assemble"};
+        if (h->cplx) throw Error{H2W_EINVAL, "complex kernels: operator files hold real skeleton factors only"};
         std::unique_ptr<std::FILE, h2w::FileCloser> fp(std::fopen(path, "wb"));
         if (!fp) throw Error{H2W_EINVAL, std::string("cannot open ") + path};
         h2w::Writer w{fp.get()};

@@ -46,6 +46,13 @@ extern "C" {
 /* kernel choice, as argv[5] of the reference's main */
 #define H2W_CHOICE_IE 0   /* integral equation kernel with its diagonal rule */
 #define H2W_CHOICE_RBF 1  /* the repo's RBF kernel                            */
+/* complex kernels (the reference's compile-time -DUSE_Helm / -DUSE_Hankel builds, with its ker_choice
+   numbers).  A handle with one of them is complex: every vector it takes or returns holds N complex
+   values as interleaved (re, im) doubles, operators are complex FP64 (precision H2W_FP64 and GMRES mode
+   H2W_GMRES_FP64 only), and entry points without a complex form refuse it with H2W_EINVAL. */
+#define H2W_CHOICE_HELMHOLTZ 2 /* 3D: e^{iR} / (4 pi), diagonal 500 sqrt(N) (3D/codes/kernel.hpp:132-144) */
+#define H2W_CHOICE_HANKEL 10   /* 2D: (i/4) H0^(1)(R), diagonal (1 + 15 log N + Kii) / h^2
+                                  (2D/codes/kernel.hpp:126-145, 2D/codes/integral2d.hpp:16-57)     */
 
 #define H2W_HOST 0
 #define H2W_DEVICE 1
@@ -109,6 +116,7 @@ typedef struct {
     int64_t rank_fixed_extra_steps; /* pivot steps run beyond where the stopping rule ended a block    */
     double rank_fixed_sampled_err;  /* largest relative error of those blocks' cross approximation on
                                        256 sampled entries each                                       */
+    double diag_im;          /* imaginary part of the diagonal value (complex kernels), else 0        */
 } h2w_stats;
 
 typedef struct {
