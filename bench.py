@@ -506,6 +506,7 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the 32-system batch section")
     ap.add_argument("--no-rank-model", action="store_true", help="assemble with heuristic workspace estimates")
     ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (default launch shapes, inner 1e-6)")
+    ap.add_argument("--no-deflation", action="store_true", help="tune without the refinement's deflation space")
     ap.add_argument("--pmc-probe", action="store_true",
                     help="assemble, then run 3 FP64 and 3 FP32-inner matvecs and a 1 GiB device copy "
                          "(the calibration kernel) for rocprofv3 --pmc; prints what it ran")
@@ -610,13 +611,19 @@ def main():
         # the tuner solves a right-hand side of its own (x_tune ~ U(-1,1), seed 12), never the timed b
         b_tune = A.dense_rows(None, np.random.default_rng(12).uniform(-1.0, 1.0, N))
         t0 = time.time()
-        tune = A.autotune(b_tune, tol=gtol, max_iter=500)
+        flags = h2w.TUNE_UNROLL | h2w.TUNE_INNER | (0 if args.no_deflation else h2w.TUNE_DEFLATION)
+        tune = A.autotune(b_tune, tol=gtol, max_iter=500, flags=flags)
+        tune["seconds"] = round(time.time() - t0, 2)
         tune["rhs"] = "b_tune = A x_tune, x_tune ~ U(-1,1) seed 12 (the timed b uses seed 2)"
         log(f"autotune {time.time() - t0:.1f} s: inner tolerance 10^-{tune['ir_tol_exp']:g}; "
             + ", ".join(f"10^-{c['ir_tol_exp']:g}: {c['seconds'] * 1e3:.1f} ms ({c['iterations']}+{c['outer']})"
                         for c in tune["candidates"])
             + f"; matvec ms FP64/FP32/inner {['%.3f' % v for v in tune['matvec_ms_before']]} -> "
             f"{['%.3f' % v for v in tune['matvec_ms_after']]}")
+        d = tune["deflation"]
+        log(f"deflation space m={d['m']} k={d['k']} (build {d['build_seconds'] * 1e3:.0f} ms); "
+            + ", ".join(f"({c['m']},{c['k']}) 10^-{c['ir_tol_exp']:g}: {c['seconds'] * 1e3:.1f} ms "
+                        f"({c['iterations']}+{c['outer']})" for c in d["candidates"]))
 
     hip = Hip()
     db, dx = hip.malloc(8 * N), hip.malloc(8 * N)
@@ -639,8 +646,8 @@ def main():
         for m in modes.values():
             solve(m)
     res = {}
-    for name, m in modes.items():
-        k = args.steps if name == HEADLINE else max(1, min(args.steps, 3))
+
+    def timed(name, m, k):
         hip.sync()
         t0 = time.perf_counter()
         for _ in range(k):
@@ -651,6 +658,16 @@ def main():
         hip.d2h(xs, dx)
         res[name] = {"s": t, "stats": st_, "err": float(np.linalg.norm(xs - xt) / np.linalg.norm(xt)), "runs": k}
         log(f"{name}: {t * 1e3:.2f} ms/solve, error {res[name]['err']:.2e}, {st_}")
+
+    for name, m in modes.items():
+        timed(name, m, args.steps if name == HEADLINE else max(1, min(args.steps, 3)))
+    defl = tune["deflation"] if tune else {"m": 0, "k": 0}
+    if defl["m"] > 0:
+        # the same refinement without the deflation space, then the space rebuilt for the later sections
+        A.set_deflation(0)
+        solve(modes[HEADLINE])
+        timed(HEADLINE + "_no_deflation", modes[HEADLINE], max(1, min(args.steps, 3)))
+        A.set_deflation(defl["m"], defl["k"])
     t_step = res[HEADLINE]["s"]
     t64 = res["fp64"]["s"]
 
@@ -790,6 +807,13 @@ def main():
            "roofline": roof, "roofline_other": roof2, "roofline_otf": roof_otf,
            "inner_operator_bytes_fp32": inner_bytes, "cpu_baseline": None}
     # GPU time-to-tolerance over FP64 GPU at equal batching: one right-hand side, 16 in lockstep, 512 in a batch
+    if HEADLINE + "_no_deflation" in res:
+        out["deflation"] = {"m": defl["m"], "k": defl["k"], "build_s": round(defl["build_seconds"], 3),
+                            "note": "m Arnoldi steps of the inner operator from a fixed pseudo-random vector, "
+                                    "its k Ritz vectors farthest from the diagonal value kept (k = 0: all); "
+                                    "built once per operator, independent of b, outside the timed solves",
+                            "fp32_inner_ir_no_deflation_s": round(res[HEADLINE + "_no_deflation"]["s"], 6),
+                            "speedup_vs_fp64_no_deflation": round(t64 / res[HEADLINE + "_no_deflation"]["s"], 3)}
     out["speedup_vs_fp64_like_for_like"] = {
         "single_rhs": round(t64 / t_step, 3),
         "rhs16_lockstep": multi["gmres"]["fp32_inner_ir_vs_fp64"] if multi else None,

@@ -34,13 +34,13 @@ GMRES_FP64, GMRES_FP32_IR, GMRES_FP32_OTF_IR, GMRES_FP32_INNER_IR = 0, 1, 2, 3
 # every symbol include/h2w.h declares
 EXPORTS = ("h2w_default_opts", "h2w_create", "h2w_assemble", "h2w_matvec", "h2w_matvec_ex", "h2w_gmres",
            "h2w_dense_matvec", "h2w_get_stats", "h2w_tree_order", "h2w_box_skeleton", "h2w_interaction_list",
-           "h2w_near_list", "h2w_destroy", "h2w_last_error", "h2w_bench_matvec", "h2w_set_inner_tolerance",
+           "h2w_near_list", "h2w_destroy", "h2w_last_error", "h2w_bench_matvec", "h2w_set_inner_tolerance", "h2w_set_deflation",
            "h2w_autotune", "h2w_gmres_multi", "h2w_bench_matvec_multi", "h2w_batch_create", "h2w_batch_matvec",
            "h2w_batch_gmres", "h2w_batch_bench_matvec", "h2w_batch_destroy", "h2w_set_rank_model",
            "h2w_rank_samples", "h2w_rank_predict", "h2w_save", "h2w_load", "h2w_bench_stream_read",
            "h2w_dense_matvec_noalloc")
 RANK_FEATURES, RANK_PARAMS = 16, 9473
-TUNE_UNROLL, TUNE_INNER = 1, 2
+TUNE_UNROLL, TUNE_INNER, TUNE_DEFLATION = 1, 2, 4
 
 
 class H2WError(RuntimeError):
@@ -92,7 +92,12 @@ class Tune(ctypes.Structure):
                 ("cand_seconds", ctypes.c_double * 12), ("cand_iterations", ctypes.c_int * 12),
                 ("cand_outer", ctypes.c_int * 12), ("cand_converged", ctypes.c_int * 12),
                 ("cand_inner_bytes", ctypes.c_int64 * 12), ("matvec_ms_before", ctypes.c_double * 3),
-                ("matvec_ms_after", ctypes.c_double * 3)]
+                ("matvec_ms_after", ctypes.c_double * 3), ("defl_m", ctypes.c_int), ("defl_k", ctypes.c_int),
+                ("defl_build_seconds", ctypes.c_double), ("n_defl_cand", ctypes.c_int),
+                ("cand_defl_m", ctypes.c_int * 12), ("cand_defl_k", ctypes.c_int * 12),
+                ("cand_defl_ir_tol_exp", ctypes.c_double * 12), ("cand_defl_seconds", ctypes.c_double * 12),
+                ("cand_defl_iterations", ctypes.c_int * 12), ("cand_defl_outer", ctypes.c_int * 12),
+                ("cand_defl_converged", ctypes.c_int * 12)]
 
 
 _lib = None
@@ -124,6 +129,7 @@ def lib():
     L.h2w_last_error.restype = ctypes.c_char_p
     L.h2w_bench_matvec.argtypes = [p, i, i, ctypes.POINTER(BenchResult)]
     L.h2w_set_inner_tolerance.argtypes = [p, d]
+    L.h2w_set_deflation.argtypes = [p, ctypes.c_int, ctypes.c_int, ctypes.c_int]
     L.h2w_bench_matvec_multi.argtypes = [p, i, i, ctypes.POINTER(BenchResult)]
     L.h2w_batch_create.argtypes = [ctypes.POINTER(p), i, ctypes.POINTER(p)]
     L.h2w_batch_matvec.argtypes = [p, p, p, i, i, i]
@@ -356,8 +362,16 @@ class Initialisation:
         fractional exponents as the autotuner's half decades)."""
         _check(lib().h2w_set_inner_tolerance(self._h, float(ir_tol_pow)))
 
+    def set_deflation(self, m, k=0, precision=None):
+        """Deflation space of the refinement: m Arnoldi steps of the inner operator (default the truncated
+        inner operator, H2W_FP32_INNER) from a fixed pseudo-random vector, the whole space (k = 0) or its k
+        Ritz vectors farthest from the diagonal value; every later correction in that refinement mode starts
+        from it.  m = 0 drops it."""
+        _check(lib().h2w_set_deflation(self._h, int(m), int(k), FP32_INNER if precision is None else int(precision)))
+
     def autotune(self, b=None, tol=1e-10, max_iter=1000, flags=TUNE_UNROLL | TUNE_INNER):
-        """GEMV unroll depth per launch and inner tolerance of the refinement (b in input order)."""
+        """GEMV unroll depth per launch, inner tolerance of the refinement and (TUNE_DEFLATION) its deflation
+        space (b in input order: a right-hand side other than the ones to be timed)."""
         t = Tune()
         bb = None
         if b is not None:
@@ -372,7 +386,14 @@ class Initialisation:
                                 "iterations": t.cand_iterations[q], "outer": t.cand_outer[q],
                                 "converged": bool(t.cand_converged[q]), "inner_bytes": t.cand_inner_bytes[q]}
                                for q in range(n)],
-                "matvec_ms_before": list(t.matvec_ms_before), "matvec_ms_after": list(t.matvec_ms_after)}
+                "matvec_ms_before": list(t.matvec_ms_before), "matvec_ms_after": list(t.matvec_ms_after),
+                "deflation": {"m": t.defl_m, "k": t.defl_k, "build_seconds": t.defl_build_seconds,
+                              "candidates": [{"m": t.cand_defl_m[q], "k": t.cand_defl_k[q],
+                                              "ir_tol_exp": t.cand_defl_ir_tol_exp[q],
+                                              "seconds": t.cand_defl_seconds[q],
+                                              "iterations": t.cand_defl_iterations[q], "outer": t.cand_defl_outer[q],
+                                              "converged": bool(t.cand_defl_converged[q])}
+                                             for q in range(t.n_defl_cand)]}}
 
 
     def bench_matvec_multi(self, nrep=10, precision=FP64):

@@ -300,6 +300,10 @@ struct h2w_handle {
     h2w::DevBuf<double> d_stage_in, d_stage_out, d_dense;
     h2w::DevBuf<double> g_V, g_w, g_r, g_b, g_x, g_h, g_part, g_d;
     h2w::DevBuf<double> g_U, g_coef;  // recycled refinement space: U (k columns), combination coefficients
+    // deflation space built once for an inner precision (h2w_set_deflation): U (k columns), then
+    // C = A U (k orthonormal columns); every correction of the refinement starts from it
+    h2w::DevBuf<double> g_defl;
+    int defl_k = 0, defl_prec = -1, defl_gen = -1;
     h2w::PinBuf g_pin;                // GMRES read-backs (Hessenberg columns)
     int64_t g_Vcols = 0;
     // kNR-wide lockstep GMRES (interleaved N x kNR vectors)

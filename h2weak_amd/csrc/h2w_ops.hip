@@ -2217,21 +2217,31 @@ GmresOut gmres_core(Handle &h, const double *b, double *x, int max_iter, double 
 }
 
 // The space of a finished correction kept for the next ones: with the Arnoldi relation
-// A V_j = V_{j+1} Hbar and Hbar = Q R (thin QR, host), U = V_j R^-1 and C = V_{j+1} Q satisfy
-// A U = C with orthonormal C.  C replaces the first j basis columns, U goes to g_U.  Returns
-// false (nothing kept) for fewer than two iterations or a near-singular R.
-bool keep_space(Handle &h, const GmresOut &o, Recycled &rc) {
+// A V_j = V_{j+1} Hbar, a selection V_j S of its space (S: j x k, the whole space when S is null,
+// S = I) has A V_j S = V_{j+1} Z with Z = Hbar S; with Z = Q R (thin QR, host) U = V_j S R^-1 and
+// C = V_{j+1} Q satisfy A U = C with orthonormal C.  C replaces the first k basis columns, U goes
+// to g_U.  Returns false (nothing kept) for fewer than two iterations or a near-singular R.
+bool keep_space(Handle &h, const GmresOut &o, Recycled &rc, const std::vector<std::vector<double>> *S = nullptr) {
     rc.k = 0;
     const int j = (int)o.H.size();
     if (j < 2 || o.iters != j) return false;
+    const int kk = S ? (int)S->size() : j;
+    if (kk < 1) return false;
     const int64_t N = h.N;
     hipStream_t s = h.stream;
-    // Hbar columns (j+1 rows), Gram-Schmidt twice in double
-    std::vector<std::vector<double>> Q(j, std::vector<double>(j + 1, 0.0));
-    std::vector<std::vector<double>> R(j, std::vector<double>(j, 0.0));  // R[col][row]
-    for (int c = 0; c < j; ++c) {
+    // Z columns (j+1 rows), Gram-Schmidt twice in double
+    std::vector<std::vector<double>> Q(kk, std::vector<double>(j + 1, 0.0));
+    std::vector<std::vector<double>> R(kk, std::vector<double>(kk, 0.0));  // R[col][row]
+    for (int c = 0; c < kk; ++c) {
         std::vector<double> v(j + 1, 0.0);
-        for (int q = 0; q < (int)o.H[c].size() && q <= j; ++q) v[q] = o.H[c][q];
+        if (S) {
+            for (int t = 0; t < j; ++t) {
+                const double sc = (*S)[c][t];
+                for (int q = 0; q < (int)o.H[t].size() && q <= j; ++q) v[q] += o.H[t][q] * sc;
+            }
+        } else {
+            for (int q = 0; q < (int)o.H[c].size() && q <= j; ++q) v[q] = o.H[c][q];
+        }
         for (int pass = 0; pass < 2; ++pass)
             for (int p = 0; p < c; ++p) {
                 double d = 0.0;
@@ -2248,32 +2258,198 @@ bool keep_space(Handle &h, const GmresOut &o, Recycled &rc) {
         for (int q = 0; q <= j; ++q) Q[c][q] = v[q] / nv;
     }
     // Rinv (upper triangular), column by column: R Rinv[:, c] = e_c
-    std::vector<std::vector<double>> Ri(j, std::vector<double>(j, 0.0));
-    for (int c = 0; c < j; ++c)
+    std::vector<std::vector<double>> Ri(kk, std::vector<double>(kk, 0.0));
+    for (int c = 0; c < kk; ++c)
         for (int q = c; q >= 0; --q) {
             double v = q == c ? 1.0 : 0.0;
             for (int p = q + 1; p <= c; ++p) v -= R[p][q] * Ri[c][p];
             Ri[c][q] = v / R[q][q];
         }
-    // coefficients: C_c = V Q[:, c] (j+1 terms), U_c = V Ri[:, c] (j terms, padded to j+1)
-    std::vector<double> coef((size_t)2 * j * (j + 1), 0.0);
-    for (int c = 0; c < j; ++c)
-        for (int q = 0; q <= j; ++q) {
-            coef[(size_t)c * (j + 1) + q] = Q[c][q];
-            coef[(size_t)(j + c) * (j + 1) + q] = q < j ? Ri[c][q] : 0.0;
+    // coefficients: C_c = V Q[:, c] (j+1 terms), U_c = V S Ri[:, c] (j terms, padded to j+1)
+    std::vector<double> coef((size_t)2 * kk * (j + 1), 0.0);
+    for (int c = 0; c < kk; ++c) {
+        for (int q = 0; q <= j; ++q) coef[(size_t)c * (j + 1) + q] = Q[c][q];
+        double *u = coef.data() + (size_t)(kk + c) * (j + 1);
+        for (int p = 0; p <= c; ++p) {
+            if (S)
+                for (int q = 0; q < j; ++q) u[q] += (*S)[p][q] * Ri[c][p];
+            else
+                u[p] = Ri[c][p];
         }
+    }
     h.g_coef.ensure(coef.size());
     H2W_HIP(hipMemcpyAsync(h.g_coef.p, coef.data(), coef.size() * 8, hipMemcpyHostToDevice, s));
-    h.g_U.ensure((size_t)2 * j * N);  // U, then C before it moves into the basis
-    launch_fill(h.g_U.p, 2 * j * N, 0.0, s);
-    for (int c = 0; c < j; ++c) {
-        launch_vec_update(h.g_V.p, N, j + 1, h.g_coef.p + (size_t)c * (j + 1), h.g_U.p + (int64_t)(j + c) * N, N, s);
-        launch_vec_update(h.g_V.p, N, j + 1, h.g_coef.p + (size_t)(j + c) * (j + 1), h.g_U.p + (int64_t)c * N, N, s);
+    h.g_U.ensure((size_t)2 * kk * N);  // U, then C before it moves into the basis
+    launch_fill(h.g_U.p, 2 * kk * N, 0.0, s);
+    for (int c = 0; c < kk; ++c) {
+        launch_vec_update(h.g_V.p, N, j + 1, h.g_coef.p + (size_t)c * (j + 1), h.g_U.p + (int64_t)(kk + c) * N, N, s);
+        launch_vec_update(h.g_V.p, N, j + 1, h.g_coef.p + (size_t)(kk + c) * (j + 1), h.g_U.p + (int64_t)c * N, N, s);
     }
-    H2W_HIP(hipMemcpyAsync(h.g_V.p, h.g_U.p + (int64_t)j * N, (size_t)j * N * 8, hipMemcpyDeviceToDevice, s));
+    H2W_HIP(hipMemcpyAsync(h.g_V.p, h.g_U.p + (int64_t)kk * N, (size_t)kk * N * 8, hipMemcpyDeviceToDevice, s));
     H2W_HIP(hipGetLastError());
-    rc.k = j;
+    // (the coefficients must outlive the copy: wait before the host vector goes)
+    H2W_HIP(hipStreamSynchronize(s));
+    rc.k = kk;
     return true;
+}
+
+// Eigen-decomposition of the symmetric n x n matrix a (row-major, destroyed): Householder reduction
+// to tridiagonal form T = Q^T A Q, then implicit QL iterations with shifts on T, the rotations
+// accumulated into Q.  Eigenvalues in w, eigenvectors as the columns of z (row-major).
+void sym_eig(std::vector<double> &a, int n, std::vector<double> &w, std::vector<double> &z) {
+    auto A = [&](int i, int j) -> double & { return a[(size_t)i * n + j]; };
+    z.assign((size_t)n * n, 0.0);
+    for (int i = 0; i < n; ++i) z[(size_t)i * n + i] = 1.0;
+    std::vector<double> v(n), p(n), e(n, 0.0);
+    // Householder: column k below the subdiagonal is zeroed by H = I - 2 v v^T on rows/cols k+1..n-1
+    for (int k = 0; k + 2 < n; ++k) {
+        double nx = 0.0;
+        for (int i = k + 1; i < n; ++i) nx += A(i, k) * A(i, k);
+        nx = std::sqrt(nx);
+        if (nx == 0.0) continue;
+        const double alpha = A(k + 1, k) > 0 ? -nx : nx;
+        for (int i = k + 1; i < n; ++i) v[i] = A(i, k);
+        v[k + 1] -= alpha;
+        double nv = 0.0;
+        for (int i = k + 1; i < n; ++i) nv += v[i] * v[i];
+        nv = std::sqrt(nv);
+        if (nv == 0.0) continue;
+        for (int i = k + 1; i < n; ++i) v[i] /= nv;
+        // p = A_sub v, K = v^T p, q = p - K v; A_sub -= 2 (v q^T + q v^T)
+        double K = 0.0;
+        for (int i = k + 1; i < n; ++i) {
+            double s = 0.0;
+            for (int j = k + 1; j < n; ++j) s += A(i, j) * v[j];
+            p[i] = s;
+            K += v[i] * s;
+        }
+        for (int i = k + 1; i < n; ++i) p[i] -= K * v[i];
+        for (int i = k + 1; i < n; ++i)
+            for (int j = k + 1; j < n; ++j) A(i, j) -= 2.0 * (v[i] * p[j] + p[i] * v[j]);
+        A(k + 1, k) = A(k, k + 1) = alpha;
+        for (int i = k + 2; i < n; ++i) A(i, k) = A(k, i) = 0.0;
+        // Q <- Q H: Q[:, k+1:] -= 2 (Q[:, k+1:] v) v^T
+        for (int r = 0; r < n; ++r) {
+            double s = 0.0;
+            for (int j = k + 1; j < n; ++j) s += z[(size_t)r * n + j] * v[j];
+            s *= 2.0;
+            for (int j = k + 1; j < n; ++j) z[(size_t)r * n + j] -= s * v[j];
+        }
+    }
+    w.resize(n);
+    for (int i = 0; i < n; ++i) w[i] = A(i, i);
+    for (int i = 0; i + 1 < n; ++i) e[i] = A(i + 1, i);
+    e[n - 1] = 0.0;
+    // implicit QL with Wilkinson-type shifts: eigenvalue l converges when e[l] vanishes
+    const double eps = 2.220446049250313e-16;
+    for (int l = 0; l < n; ++l) {
+        for (int iter = 0; iter < 60; ++iter) {
+            int m = l;
+            for (; m + 1 < n; ++m) {
+                const double dd = std::fabs(w[m]) + std::fabs(w[m + 1]);
+                if (std::fabs(e[m]) <= eps * dd) break;
+            }
+            if (m == l) break;
+            // shift from the leading 2 x 2 block at l
+            double g = (w[l + 1] - w[l]) / (2.0 * e[l]);
+            double r = std::hypot(g, 1.0);
+            g = w[m] - w[l] + e[l] / (g + (g >= 0 ? r : -r));
+            double s = 1.0, c = 1.0, pp = 0.0;
+            int i = m - 1;
+            bool under = false;
+            for (; i >= l; --i) {
+                double f = s * e[i], b = c * e[i];
+                r = std::hypot(f, g);
+                e[i + 1] = r;
+                if (r == 0.0) {  // (underflow: split here)
+                    w[i + 1] -= pp;
+                    e[m] = 0.0;
+                    under = true;
+                    break;
+                }
+                s = f / r;
+                c = g / r;
+                g = w[i + 1] - pp;
+                r = (w[i] - g) * s + 2.0 * c * b;
+                pp = s * r;
+                w[i + 1] = g + pp;
+                g = c * r - b;
+                for (int k = 0; k < n; ++k) {
+                    const double zi1 = z[(size_t)k * n + i + 1], zi = z[(size_t)k * n + i];
+                    z[(size_t)k * n + i + 1] = s * zi + c * zi1;
+                    z[(size_t)k * n + i] = c * zi - s * zi1;
+                }
+            }
+            if (under) continue;
+            w[l] -= pp;
+            e[l] = g;
+            e[m] = 0.0;
+        }
+    }
+}
+
+// The deflation space of the refinement's inner operator: m Arnoldi steps of that operator from a
+// fixed pseudo-random vector (independent of any right-hand side).  A Krylov space of a generic
+// vector holds good approximations of the operator's outlying eigenvectors, so a correction that
+// starts from it (x = U C^T r, every Arnoldi vector orthogonalised against C) needs fewer steps.
+// k = 0 keeps the whole space (U = V_m R^-1, C = V_{m+1} Q, as keep_space keeps a correction's
+// space); 0 < k < m keeps the k Ritz vectors (eigenvectors of the symmetric part of H_m) whose
+// Ritz values lie farthest from the diagonal value d: the operators here are d I plus a kernel
+// matrix, their spectrum clusters at d and GMRES spends its steps on the eigenvalues away from it,
+// so those k vectors give nearly the reduction of the whole space at k/m of the orthogonalisation
+// cost per step.  m = 0 drops the space.
+void build_deflation(Handle &h, int m, int k, int prec) {
+    const int64_t N = h.N;
+    hipStream_t s = h.stream;
+    H2W_HIP(hipStreamSynchronize(s));
+    h.defl_k = 0;
+    h.defl_prec = -1;
+    h.g_defl.release();
+    if (m <= 0) return;
+    std::vector<double> v(N);
+    uint64_t st = 0x9e3779b97f4a7c15ull;
+    for (int64_t i = 0; i < N; ++i) {  // splitmix64, uniform in [-1, 1)
+        uint64_t z = (st += 0x9e3779b97f4a7c15ull);
+        z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+        z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+        z ^= z >> 31;
+        v[i] = (double)(z >> 11) * (2.0 / 9007199254740992.0) - 1.0;
+    }
+    h.g_r.ensure(N);
+    h.g_d.ensure(N);
+    H2W_HIP(hipMemcpyAsync(h.g_r.p, v.data(), N * 8, hipMemcpyHostToDevice, s));
+    launch_fill(h.g_d.p, N, 0.0, s);
+    // tol 0: exactly m steps unless the space becomes invariant first
+    GmresOut o = gmres_core(h, h.g_r.p, h.g_d.p, m, 0.0, prec, true, -1.0, nullptr, true);
+    Recycled rc;
+    const int j = (int)o.H.size();
+    std::vector<std::vector<double>> S;
+    if (k > 0 && k < j) {
+        // symmetric part of H_j (the operator is symmetric; its FP32 products are so up to rounding)
+        std::vector<double> T((size_t)j * j, 0.0), w, Z;
+        for (int c = 0; c < j; ++c)
+            for (int q = 0; q < (int)o.H[c].size() && q < j; ++q) {
+                T[(size_t)q * j + c] += 0.5 * o.H[c][q];
+                T[(size_t)c * j + q] += 0.5 * o.H[c][q];
+            }
+        sym_eig(T, j, w, Z);
+        std::vector<int> idx(j);
+        for (int i = 0; i < j; ++i) idx[i] = i;
+        const double d = h.diag;
+        std::stable_sort(idx.begin(), idx.end(),
+                         [&](int a, int b) { return std::fabs(w[a] - d) > std::fabs(w[b] - d); });
+        S.assign(k, std::vector<double>(j));
+        for (int c = 0; c < k; ++c)
+            for (int t = 0; t < j; ++t) S[c][t] = Z[(size_t)t * j + idx[c]];
+    }
+    if (!keep_space(h, o, rc, S.empty() ? nullptr : &S))
+        throw Error{H2W_EINVAL, "deflation space: fewer than 2 steps, or a singular Arnoldi relation"};
+    h.g_defl.alloc((size_t)2 * rc.k * N);
+    H2W_HIP(hipMemcpyAsync(h.g_defl.p, h.g_U.p, (size_t)2 * rc.k * N * 8, hipMemcpyDeviceToDevice, s));
+    H2W_HIP(hipStreamSynchronize(s));
+    h.defl_k = rc.k;
+    h.defl_prec = prec;
+    h.defl_gen = h.inner_gen;
 }
 
 }  // namespace
@@ -2433,6 +2609,19 @@ h2w_gmres_stats solve_tree(Handle &h, const double *bt, int max_iter, double tol
         double rn = 0.0;
         bool fp64_resid = max_iter < 1;  // (no correction allowed: report x0's FP64 residual)
         Recycled rc;
+        // a deflation space built for this inner operator serves every correction (it goes where
+        // keep_space puts a kept space: U in g_U, C in the first basis columns)
+        const bool defl = h.defl_k > 0 && h.defl_prec == inner_prec &&
+                          (inner_prec != H2W_FP32_INNER || h.defl_gen == h.inner_gen);
+        if (defl) {
+            const int k = h.defl_k;
+            ensure_V(h, k + 2);
+            h.g_U.ensure((size_t)k * N);
+            H2W_HIP(hipMemcpyAsync(h.g_U.p, h.g_defl.p, (size_t)k * N * 8, hipMemcpyDeviceToDevice, s));
+            H2W_HIP(hipMemcpyAsync(h.g_V.p, h.g_defl.p + (int64_t)k * N, (size_t)k * N * 8,
+                                   hipMemcpyDeviceToDevice, s));
+            rc.k = k;
+        }
         while (true) {
             if (fp64_resid) {
                 matvec_dev(h, h.g_x.p, h.g_d.p, H2W_FP64, true);
@@ -2465,8 +2654,8 @@ h2w_gmres_stats solve_tree(Handle &h, const double *bt, int max_iter, double tol
             launch_fill(h.g_d.p, N, 0.0, s);
             // later corrections (same FP32 operator) recycle the first one's Krylov space
             auto o = gmres_core(h, h.g_r.p, h.g_d.p, max_iter - total, eta, inner_prec, true, nr,
-                                rc.k > 0 ? &rc : nullptr, outer == 0);
-            if (outer == 0) keep_space(h, o, rc);
+                                rc.k > 0 ? &rc : nullptr, outer == 0 && !defl);
+            if (outer == 0 && !defl) keep_space(h, o, rc);
             total += o.iters;
             S.matvecs_fp32 += o.matvecs;
             launch_axpby(1.0, h.g_d.p, 1.0, h.g_x.p, N, s);
@@ -2598,6 +2787,78 @@ void tune_inner(Handle &h, const double *bt, double tol, int max_iter, bool unro
         build_inner(h, best);
         if (unroll) tune_unroll(h, H2W_FP32_INNER);
     }
+    out->ir_tol_exp = h.ir_exp;
+}
+
+// Deflation spaces for the chosen inner operator, scored as tune_inner scores tolerances (the
+// slower of the solves from x0 = 0 and x0 = 1, a fifth more when a final residual exceeds tol / 2);
+// then, with retune_inner, the inner tolerances half a decade either side under the chosen
+// space.  Leaves the best (inner tolerance, space) built.
+void tune_deflation(Handle &h, const double *bt, double tol, int max_iter, bool unroll, bool retune_inner,
+                    h2w_tune *out) {
+    static const int cand[][2] = {{0, 0}, {64, 32}, {128, 48}, {256, 64}, {256, 96}, {384, 128}};
+    out->n_defl_cand = 0;
+    double sbest = 1e300, pbest = h.ir_exp;
+    int bm = 0, bk = 0;
+    auto score = [&](int m, int k) {
+        build_deflation(h, m, k, H2W_FP32_INNER);
+        double worst = 0.0, rmax = 0.0;
+        bool conv = true;
+        h2w_gmres_stats S{};
+        for (int start = 0; start < 2; ++start) {
+            launch_fill(h.g_x.p, h.N, start == 0 ? 0.0 : 1.0, h.stream);
+            h2w_gmres_stats S2 = solve_tree(h, bt, max_iter, tol, H2W_GMRES_FP32_INNER_IR);
+            conv = conv && S2.converged;
+            rmax = std::max(rmax, S2.resid);
+            if (S2.seconds > worst) {
+                worst = S2.seconds;
+                S = S2;
+            }
+        }
+        const double sc = worst * (rmax > tol / 2.0 ? 1.2 : 1.0);
+        if (out->n_defl_cand < H2W_TUNE_MAX_CAND) {
+            const int c = out->n_defl_cand++;
+            out->cand_defl_m[c] = m;
+            out->cand_defl_k[c] = k;
+            out->cand_defl_ir_tol_exp[c] = h.ir_exp;
+            out->cand_defl_seconds[c] = worst;
+            out->cand_defl_iterations[c] = S.iterations;
+            out->cand_defl_outer[c] = S.outer_iterations;
+            out->cand_defl_converged[c] = conv;
+        }
+        if (conv && sc < sbest) {
+            sbest = sc;
+            bm = m;
+            bk = k;
+            pbest = h.ir_exp;
+        }
+    };
+    time_matvec(h, H2W_FP32_INNER, 1);  // (graphs captured outside the timed solves)
+    time_matvec(h, H2W_FP64, 1);
+    for (auto &c : cand)
+        if ((int64_t)c[0] < h.N) score(c[0], c[1]);
+    if (retune_inner && bm > 0) {
+        const double p0 = h.ir_exp;
+        const int m = bm, k = bk;
+        for (double p : {p0 - 0.5, p0 + 0.5}) {
+            if (!(p >= 4.0) || !(p < h.tol_pow)) continue;
+            build_inner(h, p);
+            if (h.inner.launches.empty()) continue;
+            if (unroll) tune_unroll(h, H2W_FP32_INNER);
+            time_matvec(h, H2W_FP32_INNER, 1);
+            score(m, k);
+        }
+        if (pbest != h.ir_exp) {
+            build_inner(h, pbest);
+            if (unroll) tune_unroll(h, H2W_FP32_INNER);
+            time_matvec(h, H2W_FP32_INNER, 1);
+        }
+    }
+    const double t0 = now_s();
+    build_deflation(h, bm, bk, H2W_FP32_INNER);
+    out->defl_build_seconds = now_s() - t0;
+    out->defl_m = bm;
+    out->defl_k = bk;
     out->ir_tol_exp = h.ir_exp;
 }
 }  // namespace h2w
@@ -3244,15 +3505,37 @@ int h2w_set_inner_tolerance(h2w_handle *h, double ir_tol_pow) {
     H2W_CATCH
 }
 
+int h2w_set_deflation(h2w_handle *h, int m, int k, int precision) {
+    H2W_TRY
+    if (!h) throw Error{H2W_EINVAL, "NULL handle"};
+    if (!h->assembled) throw Error{H2W_ESTATE, "not assembled"};
+    if (h->cplx) throw Error{H2W_EINVAL, "complex kernels: no refinement, no deflation space"};
+    if (m < 0 || m > 512 || m == 1) throw Error{H2W_EINVAL, "m must be 0 or in [2, 512]"};
+    if (k < 0 || k > m) throw Error{H2W_EINVAL, "k must be in [0, m]"};
+    if (precision != H2W_FP32 && precision != H2W_FP32_INNER && precision != H2W_FP32_OTF)
+        throw Error{H2W_EINVAL, "precision must be an inner precision (H2W_FP32, H2W_FP32_INNER, H2W_FP32_OTF)"};
+    if ((precision == H2W_FP32 || precision == H2W_FP32_INNER) && !h->mat32.p)
+        throw Error{H2W_ESTATE, "the FP32 operators need the FP32 shadow (opts.fp32_shadow)"};
+    if (precision == H2W_FP32_INNER && !(h->ir_exp > 0.0)) throw Error{H2W_ESTATE, "no inner operator built"};
+    if ((int64_t)m >= h->N) throw Error{H2W_EINVAL, "m must be below N"};
+    H2W_HIP(hipSetDevice(h->opts.device));
+    h2w::build_deflation(*h, m, k, precision);
+    return H2W_OK;
+    H2W_CATCH
+}
+
 int h2w_autotune(h2w_handle *h, const double *b, int where, double tol, int max_iter, int flags, h2w_tune *out) {
     H2W_TRY
     if (!h || !out) throw Error{H2W_EINVAL, "NULL argument"};
     if (!h->assembled) throw Error{H2W_ESTATE, "not assembled"};
-    if (flags & ~(H2W_TUNE_UNROLL | H2W_TUNE_INNER)) throw Error{H2W_EINVAL, "unknown flags"};
+    if (flags & ~(H2W_TUNE_UNROLL | H2W_TUNE_INNER | H2W_TUNE_DEFLATION)) throw Error{H2W_EINVAL, "unknown flags"};
     if (h->cplx) throw Error{H2W_EINVAL, "complex kernels: no autotuner (one complex GEMV kernel)"};
-    if ((flags & H2W_TUNE_INNER) && (!b || max_iter < 1 || !(tol > 0.0)))
-        throw Error{H2W_EINVAL, "inner tuning needs b, tol > 0 and max_iter >= 1"};
-    if ((flags & H2W_TUNE_INNER) && !h->mat32.p) throw Error{H2W_ESTATE, "FP32 shadow not built"};
+    const bool solves = (flags & (H2W_TUNE_INNER | H2W_TUNE_DEFLATION)) != 0;
+    if (solves && (!b || max_iter < 1 || !(tol > 0.0)))
+        throw Error{H2W_EINVAL, "inner and deflation tuning need b, tol > 0 and max_iter >= 1"};
+    if (solves && !h->mat32.p) throw Error{H2W_ESTATE, "FP32 shadow not built"};
+    if ((flags & H2W_TUNE_DEFLATION) && !(flags & H2W_TUNE_INNER) && h->inner.launches.empty())
+        throw Error{H2W_ESTATE, "deflation tuning needs an inner operator"};
     H2W_HIP(hipSetDevice(h->opts.device));
     std::memset(out, 0, sizeof(*out));
     const int64_t N = h->N;
@@ -3265,11 +3548,11 @@ int h2w_autotune(h2w_handle *h, const double *b, int where, double tol, int max_
     if (flags & H2W_TUNE_UNROLL)
         for (int q = 0; q < 2; ++q)
             if (have(precs[q])) h2w::tune_unroll(*h, precs[q]);
-    if (flags & H2W_TUNE_INNER) {
+    h2w::DevBuf<double> bin, bt;
+    if (solves) {
         h->g_x.ensure(N);
         h->g_r.ensure(N);
         h->g_d.ensure(N);
-        h2w::DevBuf<double> bin, bt;
         bt.alloc(N);
         const double *bsrc = b;
         if (where == H2W_HOST) {
@@ -3279,11 +3562,16 @@ int h2w_autotune(h2w_handle *h, const double *b, int where, double tol, int max_
         }
         h2w::launch_permute_in(bsrc, N, bt.p, N, h->d_perm.p, N, 1, s);
         H2W_HIP(hipStreamSynchronize(s));
+    }
+    if (flags & H2W_TUNE_INNER) {
         h2w::tune_inner(*h, bt.p, tol, max_iter, (flags & H2W_TUNE_UNROLL) != 0, out);
     } else {
         if ((flags & H2W_TUNE_UNROLL) && have(H2W_FP32_INNER)) h2w::tune_unroll(*h, H2W_FP32_INNER);
         out->ir_tol_exp = h->inner.launches.empty() ? 0.0 : h->ir_exp;
     }
+    if ((flags & H2W_TUNE_DEFLATION) && !h->inner.launches.empty())
+        h2w::tune_deflation(*h, bt.p, tol, max_iter, (flags & H2W_TUNE_UNROLL) != 0, (flags & H2W_TUNE_INNER) != 0,
+                            out);
     for (int q = 0; q < 3; ++q) out->matvec_ms_after[q] = have(precs[q]) ? h2w::time_matvec(*h, precs[q], 5) : 0.0;
     H2W_HIP(hipStreamSynchronize(s));
     return H2W_OK;

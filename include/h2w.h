@@ -236,15 +236,35 @@ int h2w_bench_stream_read(int device, int64_t bytes, int nrep, double *gbps);
  * (which has one FP64 operator); the stored FP64 / FP32 operators are untouched. */
 int h2w_set_inner_tolerance(h2w_handle *h, double ir_tol_pow);
 
+/* Deflation space of the refinement (no counterpart in the reference, whose GMRES is
+ * unpreconditioned): m Arnoldi steps (2 <= m <= 512, m < N) of the inner operator `precision`
+ * (H2W_FP32, H2W_FP32_INNER or H2W_FP32_OTF) from a fixed pseudo-random vector.  k = 0 keeps that
+ * whole Krylov space, 0 < k < m the k Ritz vectors whose Ritz values lie farthest from the
+ * diagonal value (the operator is that value times I plus a kernel matrix; its spectrum clusters
+ * there).  Kept as U and C = A U with orthonormal C (2 k N doubles of device memory).  Every
+ * correction of a later h2w_gmres in the matching refinement mode starts from x = U C^T r and
+ * orthogonalises its Arnoldi vectors against C as well (GCRO), which removes the operator's
+ * outlying eigendirections from every solve.  It depends on the operator only, never on a
+ * right-hand side; m = 0 drops it, and rebuilding the inner operator (h2w_set_inner_tolerance,
+ * h2w_autotune) invalidates it for H2W_FP32_INNER.  Plain FP64 GMRES and the lockstep solves do
+ * not use it. */
+int h2w_set_deflation(h2w_handle *h, int m, int k, int precision);
+
 /* Autotuner (no counterpart in the reference).  With H2W_TUNE_UNROLL it times every launch of
  * the FP64, FP32 and FP32-inner matvecs with 2, 4 and 8 column loads in flight per lane and
  * keeps the fastest per launch.  With H2W_TUNE_INNER it rebuilds the inner operator for
  * tolerances 10^-p, p = 4, 4.5, ..., 8 below tol_pow, solves A x = b with
  * H2W_GMRES_FP32_INNER_IR from x0 = 0 and from x0 = 1, scores each p by the slower solve
- * (raised by a fifth when a final residual sits within 3x of tol) and keeps the best.
- * b has N entries in input order, on the host or the device (`where`). */
+ * (raised by a fifth when a final residual exceeds tol / 2) and keeps the best.  With
+ * H2W_TUNE_DEFLATION it then scores deflation spaces (h2w_set_deflation on the inner operator;
+ * (m, k) = (0, 0) i.e. none, (64, 32), (128, 48), (256, 64), (256, 96), (384, 128)) the same way,
+ * keeps the best, and with H2W_TUNE_INNER also re-scores the inner tolerances half a decade either
+ * side of the chosen one under that space.
+ * b has N entries in input order, on the host or the device (`where`); choose a b other than the
+ * right-hand sides to be timed. */
 #define H2W_TUNE_UNROLL 1
 #define H2W_TUNE_INNER 2
+#define H2W_TUNE_DEFLATION 4
 #define H2W_TUNE_MAX_CAND 12
 typedef struct {
     double ir_tol_exp;                       /* chosen inner exponent p (0: no inner operator) */
@@ -257,6 +277,15 @@ typedef struct {
     int64_t cand_inner_bytes[H2W_TUNE_MAX_CAND]; /* FP32 bytes one inner matvec streams */
     double matvec_ms_before[3];              /* FP64, FP32, FP32 inner matvec before the unroll tuning */
     double matvec_ms_after[3];
+    int defl_m, defl_k;                      /* chosen deflation space (0, 0: none) */
+    double defl_build_seconds;               /* time to build it */
+    int n_defl_cand;
+    int cand_defl_m[H2W_TUNE_MAX_CAND], cand_defl_k[H2W_TUNE_MAX_CAND];
+    double cand_defl_ir_tol_exp[H2W_TUNE_MAX_CAND];
+    double cand_defl_seconds[H2W_TUNE_MAX_CAND];  /* the slower of the two solves */
+    int cand_defl_iterations[H2W_TUNE_MAX_CAND];
+    int cand_defl_outer[H2W_TUNE_MAX_CAND];
+    int cand_defl_converged[H2W_TUNE_MAX_CAND];
 } h2w_tune;
 int h2w_autotune(h2w_handle *h, const double *b, int where, double tol, int max_iter, int flags, h2w_tune *out);
 
