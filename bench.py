@@ -206,18 +206,21 @@ def pmc_reduce(d, side, tp):
             for line in open(os.path.join(d, fn)):
                 if line.startswith('{"pmc_probe"'):
                     probe = json.loads(line)
-    for prec, key in (("double", "fp64_m2l_near"), ("float", "fp32inner_m2l_near")):
-        pat = re.compile(r"k_bgemv<%s, \d+, 1[,>]" % prec)
+    for prec, key in (("double", "fp64_m2l_near"), ("float", "fp32inner_m2l_near"), ("sym", "fp64_16wide_m2l_near")):
+        pat = re.compile(r"k_bgemm_sym<double>" if prec == "sym" else r"k_bgemv<%s, \d+, 1[,>]" % prec)
         ks = sorted(k for k, (n, v) in per["fetch"].items() if pat.search(n) and k in per["write"])
+        if not ks:
+            continue
         rd = [per["fetch"][k][1] * f_fetch for k in ks]
         wr = [per["write"][k][1] * f_write for k in ks]
         tot = sorted(a + b for a, b in zip(rd, wr))
         rec = {"dispatches": len(ks), "fetch_bytes": int(sorted(rd)[len(rd) // 2]),
                "write_bytes": int(sorted(wr)[len(wr) // 2]), "hbm_bytes_per_launch": int(tot[len(tot) // 2])}
         if probe:
-            tag, es = ("fp64", 8) if prec == "double" else ("fp32_inner", 4)
+            tag, es = ("fp32_inner", 4) if prec == "float" else ("fp64", 8)
             q = probe[f"launch_role_{tag}"].index(1)
-            alg = probe[f"launch_bytes_{tag}"][q] + 2 * es * probe["N"]  # the bench's algorithmic bytes
+            # the bench's algorithmic bytes: the stored blocks once, x and y once (16 columns each when 16-wide)
+            alg = probe[f"launch_bytes_{tag}"][q] + 2 * es * probe["N"] * (16 if prec == "sym" else 1)
             rec["algorithmic_bytes"] = int(alg)
             rec["traffic_over_algorithmic"] = round(rec["hbm_bytes_per_launch"] / alg, 4)
         out[f"3d_rbf_{side}_{tp}_{key}"] = rec
@@ -274,9 +277,10 @@ def bench_batch(S=32, tol_pow=8, seed=5):
     for prec, name, key, s_op in ((h2w.FP64, "fp64", "stored_bytes_fp64", 8), (h2w.FP32, "fp32", "stored_bytes_fp32", 4),
                                   (h2w.FP32_INNER, "fp32_inner", "stored_bytes_inner_fp32", 4)):
         r = B.bench_matvec(5, prec)
-        nbytes = sum(r["launch_bytes"]) + 2 * 8 * 16 * sum(Ns)  # symmetric blocks multiplied both ways
+        # every launch's operator bytes (a separate transposed launch reads the symmetric blocks again)
+        nbytes = sum(r["launch_bytes"]) + 2 * s_op * 16 * sum(Ns)
         out["matvec"][name] = {"ms_per_16": round(r["ms_per_pass"], 4), "launches": len(r["launch_role"]),
-                               "alg_GBps": round(nbytes / (r["ms_per_pass"] * 1e-3) * 1e-9, 1)}
+                               "streamed_GBps": round(nbytes / (r["ms_per_pass"] * 1e-3) * 1e-9, 1)}
         log(f"batch 16-wide product {name}: {r['ms_per_pass']:.3f} ms, {len(r['launch_role'])} launches")
     Xt = [np.random.default_rng(200 + q).normal(size=(16, g.N)) for q, g in enumerate(systems)]
     Bs = B.matvec(Xt)
@@ -592,6 +596,7 @@ def main():
         A.set_inner_tolerance(args.inner_tol)
         A.bench_matvec(3, h2w.FP64)
         A.bench_matvec(3, h2w.FP32_INNER)
+        A.bench_matvec_multi(2, h2w.FP64)  # the 16-wide fused launch (k_bgemm_sym)
         nb = 1 << 30
         src, dst = hip.malloc(nb), hip.malloc(nb)
         hip.d2d(dst, src, nb)
@@ -745,10 +750,12 @@ def main():
             r = A.bench_matvec_multi(5, prec)
             ms = r["ms_per_pass"]
             elems = r["launch_bytes"][0] // s_op  # operator elements multiplied (symmetric blocks twice)
-            nbytes = s_op * elems + 2 * 8 * N * 16
+            # streamed: every launch's operator bytes (each stored block once when the fused launch applies
+            # it both ways) and the 16-wide input and output vectors
+            nbytes = sum(r["launch_bytes"][1:]) + 2 * s_op * N * 16
             tflops = 2.0 * elems * 16 / (ms * 1e-3) * 1e-12
             multi["matvec"][name] = {"ms_per_16": round(ms, 4), "ms_per_rhs": round(ms / 16, 4),
-                                     "alg_GBps": round(nbytes / (ms * 1e-3) * 1e-9, 1),
+                                     "streamed_GBps": round(nbytes / (ms * 1e-3) * 1e-9, 1),
                                      "TFLOPs": round(tflops, 2), "mfma_frac": round(tflops / peak[name], 4),
                                      "speedup_vs_16_single": round(16 * mv[name]["ms"] / ms, 2),
                                      "launch_ms": [round(v, 4) for v in r["launch_ms"][1:]],

@@ -47,6 +47,7 @@ struct h2w_batch {
     std::vector<int32_t> perm;           // lockstep slot -> row of its system's caller block
     hipGraphExec_t graph[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
     h2w::LockBufs buf;
+    bool sym16 = true;                   // fused launch reads each block once (k_bgemm_sym)
 };
 
 namespace h2w {
@@ -73,7 +74,7 @@ void merge_lists(Batch &B, int which) {
         const LaunchSet &tr = which == 0 ? h.trans : h.inner_trans;
         for (auto &L : Ls) {
             groups[key_of(L)].push_back(Src{s, &L, which == 0 ? nullptr : &h.inner, L.role});
-            if (L.role == 1)  // the transposed products of the symmetric blocks, after the fused launch
+            if (L.role == 1 && !B.sym16)  // the transposed products, after the direct ones
                 for (auto &Lt : tr.launches) groups[key_of(Lt)].push_back(Src{s, &Lt, &tr, 3});
         }
     }
@@ -101,6 +102,7 @@ void merge_lists(Batch &B, int which) {
                 for (int32_t t = it.t0; t < it.t1; ++t) {
                     GTerm tm = ht[t];
                     tm.x += vb;
+                    if (tm.t >= 0) tm.t += vb;  // (the partner's rows of a symmetric term)
                     e += (int64_t)it.rows * tm.cols;
                     terms.push_back(tm);
                 }
@@ -140,7 +142,14 @@ void check_batch(Batch &B, int precision) {
 void run_batch_launch(Batch &B, const Launch &L, int precision) {
     hipStream_t s = B.stream;
     const LaunchSet &M = B.lists[precision == H2W_FP32_INNER ? 1 : 0];
-    if (is32(precision)) {
+    if (L.role == 1 && B.sym16) {
+        if (is32(precision))
+            launch_bgemm_sym32(M.d_items.p + L.item0, (int)L.nitems, M.d_terms.p, nullptr, nullptr, B.tab32.p,
+                               B.vec32m.p, s);
+        else
+            launch_bgemm_sym64(M.d_items.p + L.item0, (int)L.nitems, M.d_terms.p, nullptr, nullptr, B.tab64.p,
+                               B.vec64m.p, s);
+    } else if (is32(precision)) {
         if (L.role == 3)
             launch_bgemm_t32(M.d_items.p + L.item0, (int)L.nitems, M.d_terms.p, nullptr, nullptr, B.tab32.p,
                              B.vec32m.p, trans_narrow(L, true), s);
@@ -273,6 +282,7 @@ int h2w_batch_create(h2w_handle *const *systems, int nsys, h2w_batch **out) {
     B->S = nsys;
     B->device = systems[0] ? systems[0]->opts.device : 0;
     B->inner = true;
+    B->sym16 = std::getenv("H2W_NO_SYM16") == nullptr;
     for (int s = 0; s < nsys; ++s) {
         h2w_handle *h = systems[s];
         if (!h || !h->assembled) throw Error{H2W_ESTATE, "every system of a batch must be assembled"};

@@ -288,6 +288,7 @@ struct h2w_handle {
     hipGraphExec_t graph[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};  // one per precision
     hipGraphExec_t graph_m[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};  // the same, kNR right-hand sides
     bool use_graph = true;
+    bool sym16 = true;                  // 16-wide fused launch reads each block once (k_bgemm_sym)
     int64_t inner_elems = 0;
     int inner_gen = 0;                  // incremented whenever the inner operator is rebuilt
     double ir_exp = 0.0;                // the inner operator's tolerance exponent p (10^-p), 0: none

@@ -241,6 +241,12 @@ void launch_axpby(double a, const double *x, double b, double *y, int64_t n, hip
 void launch_scal(double a, double *y, int64_t n, hipStream_t s);  // y *= a
 void launch_vec_update(const double *V, int64_t ldv, int k, const double *c, double *x, int64_t n, hipStream_t s);
 
+// kNR right-hand sides, the fused M2L + near-field launch: every stored block applied directly and, for
+// symmetric terms, transposed (K^T X_b into the partner's rows, atomics) in one read of the block
+void launch_bgemm_sym64(const GItem *items, int nitems, const GTerm *terms, const double *A0, const double *A1,
+                        const double *const *tab, double *vec, hipStream_t s);
+void launch_bgemm_sym32(const GItem *items, int nitems, const GTerm *terms, const float *A0, const float *A1,
+                        const float *const *tab, float *vec, hipStream_t s);
 // kNR right-hand sides, transposed products of the stored symmetric blocks (role 3 lists)
 // (narrow: slabs of 4 VEC inner indices, 16 columns per load, four slabs per step -- for
 // blocks of few rows; else slabs of 16 VEC inner indices, 4 columns per load)

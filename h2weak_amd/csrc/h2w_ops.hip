@@ -3009,6 +3009,7 @@ int h2w_create(int dim, int variant, int64_t N, const double *points, int n_max,
     h->L = L;
     h->opts = o;
     h->use_graph = std::getenv("H2W_NO_GRAPH") == nullptr;
+    h->sym16 = std::getenv("H2W_NO_SYM16") == nullptr;
     h->plan.build(dim, N, points, L, nl);
     h->cplx = choice == H2W_CHOICE_HELMHOLTZ || choice == H2W_CHOICE_HANKEL;
     if (h->cplx) {

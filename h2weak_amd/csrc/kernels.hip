@@ -1372,6 +1372,204 @@ __global__ __launch_bounds__(256) void k_bgemm(const GItem *__restrict__ items, 
     }
 }
 
+// Direct and transposed products of the stored blocks in one read, kNR right-hand sides
+// (the fused M2L + near-field launch of the multi-vector pass).  One item (box b, rows <= 128)
+// per workgroup.  The item's terms K(b, p) are cut into 16-column groups (never across a term);
+// wave wv takes groups wv, wv + 4, ... and walks each group's row groups of RG = 16 VEC rows as
+// slabs, two slabs in flight.  A slab is loaded in the direct product's layout (lane (m, q):
+// VEC consecutive rows of column 4 j + q, j < 4: 16-byte loads, 256-byte runs per column) and
+// multiplied at once into the item's rows (as k_bgemm: A(m, q) = K(VEC m + e, 4 j + q),
+// B(q, m) = X_p(4 j + q, m)).  For a symmetric term (t >= 0) the wave also writes the slab to its
+// own LDS region and reads it back with a column per lane % 16 and a row per lane / 16, the A
+// operand of K^T: with B(q, m) = X_b(4 s + q, m) from the item's input rows (staged in LDS once)
+// it accumulates the 16 x kNR block Y_p(c0 .. c0 + 15, :) += K(rows, c0 ..)^T X_b over the row
+// groups, and adds it to the partner's output with atomics when the group ends.  The item's own
+// rows are added atomically too (other items' transposes land in them).  Each stored element is
+// read once per pass, where k_bgemm + k_bgemm_tl read it twice.
+template <typename T>
+// (at most 256 registers a lane: two waves per SIMD for FP64)
+__global__ __launch_bounds__(256, 2) void k_bgemm_sym(const GItem *__restrict__ items, const GTerm *__restrict__ terms,
+                                                   const T *__restrict__ A0, const T *__restrict__ A1,
+                                                   const T *const *__restrict__ tab, T *vec) {
+    typedef typename Mfma16<T>::v4 V4;
+    constexpr int VEC = 16 / sizeof(T);
+    constexpr int RG = 16 * VEC;           // rows of a slab
+    constexpr int NG = kRowTile / RG;      // row groups of a full item
+    constexpr int NA = NG * VEC;           // direct accumulators
+    constexpr int LDR = RG + VEC;          // LDS column of a slab, padded by 16 bytes
+    constexpr int XS = kRowTile * kNR;     // staged input rows
+    constexpr int SLB = 16 * LDR;          // one wave's slab
+    constexpr int RED = 2 * kRowTile * kNR;
+    constexpr int SM = XS + 4 * SLB > RED ? XS + 4 * SLB : RED;
+    typedef T VL __attribute__((ext_vector_type(VEC)));
+    __shared__ __attribute__((aligned(16))) T smem[SM];
+    T *const xs = smem;                    // [kRowTile][kNR]
+    const GItem it = items[blockIdx.x];
+    const T *__restrict__ A = tab ? tab[2 * (it.flags >> 8) + ((it.flags >> 1) & 1)] : ((it.flags & 2) ? A1 : A0);
+    const CGTerm *const tms = (const CGTerm *)terms;
+    const int lane = threadIdx.x & 63;
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    T *const slab = smem + XS + wv * SLB;  // [16][LDR], this wave's
+    const int rows = it.rows;
+    const int ng = (rows + RG - 1) / RG;
+    const int m = lane & 15, q = lane >> 4;
+    const int64_t lda = it.lda;
+    // the item's input rows X_b (zero beyond rows, up to the last row group)
+    for (int e = threadIdx.x; e < ng * RG * kNR; e += 256) {
+        const int r = e / kNR;
+        xs[e] = r < rows ? vec[(it.xb + r) * kNR + (e % kNR)] : T(0);
+    }
+    __syncthreads();
+    V4 acc[NA];
+#pragma unroll
+    for (int a = 0; a < NA; ++a) acc[a] = (V4)0;
+    V4 acct[2] = {(V4)0, (V4)0};  // (two chains of the transposed MFMAs)
+    // this lane's rows of every row group (out of range: past the end of any resource)
+    uint32_t roff[NG];
+#pragma unroll
+    for (int g = 0; g < NG; ++g)
+        roff[g] = (g < ng && RG * g + VEC * m < rows) ? (uint32_t)((it.row0 + VEC * m + RG * g) * sizeof(T)) : kOutOfRange;
+    // cursor: term tt, 16-column group cg of that term, row group rg
+    int tt = it.t0, cg = wv, rg = 0;
+    auto settle = [&]() {
+        while (tt < it.t1) {
+            const int ngr = (tms[tt].cols + 15) >> 4;
+            if (cg < ngr) break;
+            cg -= ngr;
+            ++tt;
+        }
+    };
+    struct Slab {
+        VL av[4];
+        T bx[4];
+        int64_t t;
+        int cols, c0, g;
+        bool has;
+    };
+    auto load = [&](Slab &S) {
+        S.has = tt < it.t1;
+        const GTerm tm = term_at(tms, S.has ? tt : it.t1 - 1);
+        S.t = tm.t;
+        S.cols = tm.cols;
+        S.c0 = 16 * cg;
+        S.g = rg;
+        const __amdgpu_buffer_rsrc_t ra = buf_rsrc(A + tm.a, (uint32_t)((int64_t)tm.cols * lda * sizeof(T)));
+        const __amdgpu_buffer_rsrc_t rx = buf_rsrc(vec + tm.x * kNR, (uint32_t)((int64_t)tm.cols * kNR * sizeof(T)));
+        uint32_t ro = kOutOfRange;
+#pragma unroll
+        for (int g = 0; g < NG; ++g) ro = g == rg ? roff[g] : ro;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int c = S.c0 + 4 * j + q;
+            const bool ok = S.has && c < tm.cols;
+            // (the group's X_p rows only with its first row group; later slabs keep them)
+            S.bx[j] = buf_ld<T>(rx, opaque_off(ok && rg == 0 ? (uint32_t)((c * kNR + m) * sizeof(T)) : kOutOfRange));
+            S.av[j] = buf_ld16_nt<VL>(
+                ra, opaque_off(ok && ro != kOutOfRange ? (uint32_t)(c * lda * sizeof(T)) + ro : kOutOfRange));
+        }
+        if (S.has) {
+            if (++rg == ng) {
+                rg = 0;
+                cg += 4;
+                settle();
+            }
+        }
+    };
+    auto wave_sync = [] {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    };
+    T bxc[4] = {T(0), T(0), T(0), T(0)};
+    auto compute = [&](const Slab &S) {
+        if (!S.has) return;
+        if (S.g == 0) {
+#pragma unroll
+            for (int j = 0; j < 4; ++j) bxc[j] = S.bx[j];
+        }
+        // direct: rows of row group S.g
+#pragma unroll
+        for (int g = 0; g < NG; ++g)
+            if (g == S.g)
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+#pragma unroll
+                    for (int e = 0; e < VEC; ++e) acc[g * VEC + e] = Mfma16<T>::op(S.av[j][e], bxc[j], acc[g * VEC + e]);
+        if (S.t >= 0) {
+            // transposed: the slab through LDS, a column per lane % 16
+#pragma unroll
+            for (int j = 0; j < 4; ++j) *(VL *)&slab[(4 * j + q) * LDR + VEC * m] = S.av[j];
+            wave_sync();
+            const T *xr = xs + (RG * S.g) * kNR;
+#pragma unroll
+            for (int st = 0; st < RG / 4; ++st)
+                acct[st & 1] = Mfma16<T>::op(slab[m * LDR + 4 * st + q], xr[(4 * st + q) * kNR + m], acct[st & 1]);
+            wave_sync();
+            if (S.g == ng - 1) {
+                T *yp = vec + S.t * kNR + m;
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int c = S.c0 + Mfma16<T>::row(q, v);
+                    if (c < S.cols) atomicAdd(yp + (int64_t)c * kNR, acct[0][v] + acct[1][v]);
+                }
+                acct[0] = acct[1] = (V4)0;
+            }
+        }
+    };
+    settle();
+    // (a slab loaded past the last one has has = false and adds nothing)
+    Slab S0, S1;
+    load(S0);
+    load(S1);
+    while (S0.has) {
+        compute(S0);
+        load(S0);
+        compute(S1);
+        load(S1);
+    }
+    // the four waves' direct products: 3 and 2 into 1 and 0, then 1 into 0 (LDS reused)
+    T(*red)[kRowTile][kNR] = (T(*)[kRowTile][kNR])smem;
+    auto row_of = [&](int g, int e, int v) { return RG * g + VEC * Mfma16<T>::row(q, v) + e; };
+    auto put = [&](int slot) {
+#pragma unroll
+        for (int g = 0; g < NG; ++g)
+            if (g < ng)
+#pragma unroll
+                for (int e = 0; e < VEC; ++e)
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) red[slot][row_of(g, e, v)][m] = acc[g * VEC + e][v];
+    };
+    auto add = [&](int slot) {
+#pragma unroll
+        for (int g = 0; g < NG; ++g)
+            if (g < ng)
+#pragma unroll
+                for (int e = 0; e < VEC; ++e)
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) acc[g * VEC + e][v] += red[slot][row_of(g, e, v)][m];
+    };
+    __syncthreads();  // (xs and the slabs are no longer read)
+    if (wv >= 2) put(wv - 2);
+    __syncthreads();
+    if (wv < 2) add(wv);
+    __syncthreads();
+    if (wv == 1) put(0);
+    __syncthreads();
+    if (wv == 0) {
+        T *y = vec + it.y * kNR + m;
+#pragma unroll
+        for (int g = 0; g < NG; ++g)
+            if (g < ng)
+#pragma unroll
+                for (int e = 0; e < VEC; ++e)
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) {
+                        const int r = row_of(g, e, v);
+                        if (r < rows) atomicAdd(y + (int64_t)r * kNR, acc[g * VEC + e][v] + red[0][r][m]);
+                    }
+    }
+}
+
 // Transposed products of the symmetric blocks on the 16-wide path:
 // Y[j, n] += sum_i K[i, row0 + j] X[i, n] for an item's output rows j (box p) and
 // terms K(b, p) (leading dimension tm.ld, inner dimension tm.cols = r_b), through LDS.
@@ -2408,6 +2606,16 @@ static void launch_bgemm_tl(const GItem *items, int nitems, const GTerm *terms, 
         hipLaunchKernelGGL((k_bgemm_tl<T, 4>), dim3(nitems), dim3(256), 0, s, items, terms, A0, A1 ? A1 : A0, tab, vec);
     else
         hipLaunchKernelGGL((k_bgemm_tl<T, 16>), dim3(nitems), dim3(256), 0, s, items, terms, A0, A1 ? A1 : A0, tab, vec);
+}
+void launch_bgemm_sym64(const GItem *items, int nitems, const GTerm *terms, const double *A0, const double *A1,
+                        const double *const *tab, double *vec, hipStream_t s) {
+    if (nitems > 0)
+        hipLaunchKernelGGL(k_bgemm_sym<double>, dim3(nitems), dim3(256), 0, s, items, terms, A0, A1 ? A1 : A0, tab, vec);
+}
+void launch_bgemm_sym32(const GItem *items, int nitems, const GTerm *terms, const float *A0, const float *A1,
+                        const float *const *tab, float *vec, hipStream_t s) {
+    if (nitems > 0)
+        hipLaunchKernelGGL(k_bgemm_sym<float>, dim3(nitems), dim3(256), 0, s, items, terms, A0, A1 ? A1 : A0, tab, vec);
 }
 void launch_bgemm_t64(const GItem *items, int nitems, const GTerm *terms, const double *A0, const double *A1,
                       const double *const *tab, double *vec, int narrow, hipStream_t s) {

@@ -25,9 +25,10 @@ void ensure_pool_m(Handle &h, int precision) {
     else h.vec64m.ensure((size_t)h.vec_len * kNR);
 }
 
-// The fused stage-1 launch applies the stored blocks K(b, p), p >= b, directly; the
-// transposed products K(b, p)^T X_b follow as the launches of the role-3 lists
-// (h.trans, h.inner_trans) on the matrix cores.
+// The fused stage-1 launch applies every stored block K(b, p), p >= b, directly and its
+// transpose K(b, p)^T X_b in the same read (k_bgemm_sym); with H2W_NO_SYM16 set it applies
+// them directly only and the transposed products follow as the launches of the role-3 lists
+// (h.trans, h.inner_trans).
 // One step of the sequence: a launch of the precision's lists, or (trans) launch Lh of
 // its role-3 lists.
 void run_launch_m(Handle &h, const Launch &Lh, int precision, bool trans) {
@@ -43,6 +44,17 @@ void run_launch_m(Handle &h, const Launch &Lh, int precision, bool trans) {
         else
             launch_bgemm_t64(T.d_items.p + Lh.item0, (int)Lh.nitems, T.d_terms.p, h.mat64.p, nullptr, nullptr,
                              h.vec64m.p, trans_narrow(Lh, false), s);
+    } else if (Lh.role == 1 && h.sym16) {
+        // direct and transposed products in one read of every block
+        if (precision == H2W_FP32_INNER)
+            launch_bgemm_sym32(h.inner.d_items.p + Lh.item0, (int)Lh.nitems, h.inner.d_terms.p, h.mat32.p,
+                               h.mat32in.p, nullptr, h.vec32m.p, s);
+        else if (precision == H2W_FP32)
+            launch_bgemm_sym32(h.d_items.p + Lh.item0, (int)Lh.nitems, h.d_terms.p, h.mat32.p, nullptr, nullptr,
+                               h.vec32m.p, s);
+        else
+            launch_bgemm_sym64(h.d_items.p + Lh.item0, (int)Lh.nitems, h.d_terms.p, h.mat64.p, nullptr, nullptr,
+                               h.vec64m.p, s);
     } else if (precision == H2W_FP32_INNER) {
         launch_bgemm32(h.inner.d_items.p + Lh.item0, (int)Lh.nitems, h.inner.d_terms.p, h.mat32.p, h.mat32in.p,
                        h.vec32m.p, Lh.role, per_wave_items(Lh), s);
@@ -61,7 +73,7 @@ std::vector<std::pair<const Launch *, bool>> steps_m(Handle &h, int precision) {
     const LaunchSet &T = precision == H2W_FP32_INNER ? h.inner_trans : h.trans;
     for (auto &Lh : launches_for(h, precision)) {
         st.push_back({&Lh, false});
-        if (Lh.role == 1)
+        if (Lh.role == 1 && !h.sym16)
             for (auto &Lt : T.launches) st.push_back({&Lt, true});
     }
     return st;

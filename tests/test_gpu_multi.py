@@ -86,3 +86,23 @@ def test_multi_rhs_gmres(built, case):
             assert em <= 10 * e1 + 1e-13, (mode, c, em, e1)
             if mode == h2w.GMRES_FP64:
                 assert abs(st[c]["iterations"] - s1["iterations"]) <= 1
+
+
+@pytest.mark.parametrize("case", CASES, ids=_ids)
+def test_single_read_launch_matches_two_launches(built, case, monkeypatch):
+    """The fused launch that applies every stored block directly and transposed in one read equals the
+    direct launch followed by the separate transposed launches (a handle built with H2W_NO_SYM16 set),
+    in every stored precision, to the rounding of the atomics' summation order."""
+    g, P = built(case)
+    dim, N, nmax, tp, ch, var, kind = case
+    monkeypatch.setenv("H2W_NO_SYM16", "1")
+    g2 = h2w.Initialisation(dim, N, nmax, 1.0, tp, ch, var, points=P)
+    g2.tree_initialisation()
+    monkeypatch.delenv("H2W_NO_SYM16")
+    X = np.random.default_rng(5).normal(size=(16, N))
+    for prec, tol in ((h2w.FP64, 1e-14), (h2w.FP32, 2e-6), (h2w.FP32_INNER, 2e-6)):
+        if prec == h2w.FP32_INNER and case[3] <= 6:
+            continue
+        Y1, Y2 = g.matvec_multi(X, precision=prec), g2.matvec_multi(X, precision=prec)
+        for c in range(16):
+            assert _rel(Y1[c], Y2[c]) <= tol, (prec, c)
