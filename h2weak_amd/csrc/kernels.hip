@@ -1376,31 +1376,37 @@ __global__ __launch_bounds__(256) void k_bgemm(const GItem *__restrict__ items, 
 // (the fused M2L + near-field launch of the multi-vector pass).  One item (box b, rows <= 128)
 // per workgroup.  The item's terms K(b, p) are cut into 16-column groups (never across a term);
 // wave wv takes groups wv, wv + 4, ... and walks each group's row groups of RG = 16 VEC rows as
-// slabs, two slabs in flight.  A slab is loaded in the direct product's layout (lane (m, q):
-// VEC consecutive rows of column 4 j + q, j < 4: 16-byte loads, 256-byte runs per column) and
-// multiplied at once into the item's rows (as k_bgemm: A(m, q) = K(VEC m + e, 4 j + q),
-// B(q, m) = X_p(4 j + q, m)).  For a symmetric term (t >= 0) the wave also writes the slab to its
-// own LDS region and reads it back with a column per lane % 16 and a row per lane / 16, the A
-// operand of K^T: with B(q, m) = X_b(4 s + q, m) from the item's input rows (staged in LDS once)
-// it accumulates the 16 x kNR block Y_p(c0 .. c0 + 15, :) += K(rows, c0 ..)^T X_b over the row
-// groups, and adds it to the partner's output with atomics when the group ends.  The item's own
-// rows are added atomically too (other items' transposes land in them).  Each stored element is
-// read once per pass, where k_bgemm + k_bgemm_tl read it twice.
+// slabs, the next slab in flight while one is multiplied.  A slab is loaded in the direct
+// product's layout (lane (m, q): VEC consecutive rows of column 4 j + q, j < 4: 16-byte loads,
+// 256-byte runs per column) and multiplied at once into the item's rows (as k_bgemm: A(m, q) =
+// K(VEC m + e, 4 j + q),
+// B(q, m) = X_p(4 j + q, m)).  For a symmetric term (t >= 0) the wave also writes the slab to one
+// of its two LDS buffers; one slab later it reads it back with a column per lane % 16 and a row
+// per lane / 16, the A operand of K^T: with B(q, m) = X_b(4 s + q, m) from the item's input rows
+// (staged in LDS once) it accumulates the 16 x kNR block Y_p(c0 .. c0 + 15, :) += K(rows, c0 ..)^T
+// X_b over the row groups, and adds it to the partner's output with atomics when the group ends.
+// (Reading a slab one step after writing it overlaps the LDS round trip with the next slab's
+// direct products.)  The item's own rows are added atomically too (other items' transposes land
+// in them).  Each stored element is read once per pass, where k_bgemm + k_bgemm_tl read it twice.
+// FP64: 64 MFMA flops per stored element (direct and transposed, 16 right-hand sides) against
+// 8 bytes, so the launch needs both HBM and the FP64 matrix cores (78.6 TFLOP/s) at once;
+// launch bounds cap it at 256 registers a lane (two waves per SIMD).
 template <typename T>
-// (at most 256 registers a lane: two waves per SIMD for FP64)
 __global__ __launch_bounds__(256, 2) void k_bgemm_sym(const GItem *__restrict__ items, const GTerm *__restrict__ terms,
-                                                   const T *__restrict__ A0, const T *__restrict__ A1,
-                                                   const T *const *__restrict__ tab, T *vec) {
+                                                      const T *__restrict__ A0, const T *__restrict__ A1,
+                                                      const T *const *__restrict__ tab, T *vec) {
     typedef typename Mfma16<T>::v4 V4;
     constexpr int VEC = 16 / sizeof(T);
     constexpr int RG = 16 * VEC;           // rows of a slab
     constexpr int NG = kRowTile / RG;      // row groups of a full item
     constexpr int NA = NG * VEC;           // direct accumulators
+    constexpr int NS = RG / 4;             // MFMA k-steps of a slab's transposed product
+    constexpr int kPipe = 2;               // slabs in the load rotation (3 measured no faster)
     constexpr int LDR = RG + VEC;          // LDS column of a slab, padded by 16 bytes
     constexpr int XS = kRowTile * kNR;     // staged input rows
-    constexpr int SLB = 16 * LDR;          // one wave's slab
+    constexpr int SLB = 16 * LDR;          // one slab buffer
     constexpr int RED = 2 * kRowTile * kNR;
-    constexpr int SM = XS + 4 * SLB > RED ? XS + 4 * SLB : RED;
+    constexpr int SM = XS + 8 * SLB > RED ? XS + 8 * SLB : RED;
     typedef T VL __attribute__((ext_vector_type(VEC)));
     __shared__ __attribute__((aligned(16))) T smem[SM];
     T *const xs = smem;                    // [kRowTile][kNR]
@@ -1409,7 +1415,7 @@ __global__ __launch_bounds__(256, 2) void k_bgemm_sym(const GItem *__restrict__ 
     const CGTerm *const tms = (const CGTerm *)terms;
     const int lane = threadIdx.x & 63;
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    T *const slab = smem + XS + wv * SLB;  // [16][LDR], this wave's
+    T *const slab0 = smem + XS + 2 * wv * SLB;  // this wave's two buffers, [16][LDR] each
     const int rows = it.rows;
     const int ng = (rows + RG - 1) / RG;
     const int m = lane & 15, q = lane >> 4;
@@ -1480,53 +1486,80 @@ __global__ __launch_bounds__(256, 2) void k_bgemm_sym(const GItem *__restrict__ 
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     };
+    // the slab waiting in LDS for its transposed product
+    bool pend = false;
+    int64_t pt = 0;
+    int pcols = 0, pc0 = 0, pg = 0, pbuf = 0, cur = 0;
     T bxc[4] = {T(0), T(0), T(0), T(0)};
     auto compute = [&](const Slab &S) {
-        if (!S.has) return;
-        if (S.g == 0) {
+        // read back the pending slab (its LDS round trip overlaps the direct products below)
+        T ta[NS], tb[NS];
+        if (pend) {
+            const T *sl = slab0 + pbuf * SLB;
+            const T *xr = xs + (RG * pg) * kNR;
 #pragma unroll
-            for (int j = 0; j < 4; ++j) bxc[j] = S.bx[j];
+            for (int st = 0; st < NS; ++st) {
+                ta[st] = sl[m * LDR + 4 * st + q];
+                tb[st] = xr[(4 * st + q) * kNR + m];
+            }
         }
-        // direct: rows of row group S.g
+        if (S.has) {
+            if (S.g == 0) {
 #pragma unroll
-        for (int g = 0; g < NG; ++g)
-            if (g == S.g)
+                for (int j = 0; j < 4; ++j) bxc[j] = S.bx[j];
+            }
+            // direct: rows of row group S.g
 #pragma unroll
-                for (int j = 0; j < 4; ++j)
+            for (int g = 0; g < NG; ++g)
+                if (g == S.g)
 #pragma unroll
-                    for (int e = 0; e < VEC; ++e) acc[g * VEC + e] = Mfma16<T>::op(S.av[j][e], bxc[j], acc[g * VEC + e]);
-        if (S.t >= 0) {
-            // transposed: the slab through LDS, a column per lane % 16
+                    for (int j = 0; j < 4; ++j)
 #pragma unroll
-            for (int j = 0; j < 4; ++j) *(VL *)&slab[(4 * j + q) * LDR + VEC * m] = S.av[j];
-            wave_sync();
-            const T *xr = xs + (RG * S.g) * kNR;
+                        for (int e = 0; e < VEC; ++e)
+                            acc[g * VEC + e] = Mfma16<T>::op(S.av[j][e], bxc[j], acc[g * VEC + e]);
+        }
+        if (pend) {
 #pragma unroll
-            for (int st = 0; st < RG / 4; ++st)
-                acct[st & 1] = Mfma16<T>::op(slab[m * LDR + 4 * st + q], xr[(4 * st + q) * kNR + m], acct[st & 1]);
-            wave_sync();
-            if (S.g == ng - 1) {
-                T *yp = vec + S.t * kNR + m;
+            for (int st = 0; st < NS; ++st) acct[st & 1] = Mfma16<T>::op(ta[st], tb[st], acct[st & 1]);
+            if (pg == ng - 1) {
+                T *yp = vec + pt * kNR + m;
 #pragma unroll
                 for (int v = 0; v < 4; ++v) {
-                    const int c = S.c0 + Mfma16<T>::row(q, v);
-                    if (c < S.cols) atomicAdd(yp + (int64_t)c * kNR, acct[0][v] + acct[1][v]);
+                    const int c = pc0 + Mfma16<T>::row(q, v);
+                    if (c < pcols) atomicAdd(yp + (int64_t)c * kNR, acct[0][v] + acct[1][v]);
                 }
                 acct[0] = acct[1] = (V4)0;
             }
         }
+        pend = S.has && S.t >= 0;
+        if (pend) {
+            // (the buffer written here was last read two slabs ago, and those reads are complete)
+            T *sl = slab0 + cur * SLB;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) *(VL *)&sl[(4 * j + q) * LDR + VEC * m] = S.av[j];
+            pt = S.t;
+            pcols = S.cols;
+            pc0 = S.c0;
+            pg = S.g;
+            pbuf = cur;
+            cur ^= 1;
+        }
+        wave_sync();
     };
     settle();
-    // (a slab loaded past the last one has has = false and adds nothing)
-    Slab S0, S1;
-    load(S0);
-    load(S1);
-    while (S0.has) {
-        compute(S0);
-        load(S0);
-        compute(S1);
-        load(S1);
+    // kPipe slabs rotate: while one is multiplied the next kPipe - 1 are in flight (a slab
+    // loaded past the last one has has = false and adds nothing)
+    Slab S[kPipe];
+#pragma unroll
+    for (int i = 0; i < kPipe; ++i) load(S[i]);
+    while (S[0].has) {
+#pragma unroll
+        for (int i = 0; i < kPipe; ++i) {
+            compute(S[i]);
+            load(S[i]);
+        }
     }
+    if (pend) compute(S[0]);  // (S[0].has is false: the last pending transposed product only)
     // the four waves' direct products: 3 and 2 into 1 and 0, then 1 into 0 (LDS reused)
     T(*red)[kRowTile][kNR] = (T(*)[kRowTile][kNR])smem;
     auto row_of = [&](int g, int e, int v) { return RG * g + VEC * Mfma16<T>::row(q, v) + e; };
