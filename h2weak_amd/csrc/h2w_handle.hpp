@@ -224,6 +224,7 @@ inline void app_rot(double &dx, double &dy, double cs, double sn) {
 // device work buffers of a lockstep solve
 struct LockBufs {
     DevBuf<double> V, w, r, d, b, part, h, sc;
+    DevBuf<double> spart, sh;  // shared-space dot products: per-block partials, reduced (2 passes)
     PinBuf pin;  // read-backs of the Hessenberg columns
     int64_t Vcols = 0;
     DevBuf<double> U, coef;  // recycled refinement space (U; C sits in the first rk basis columns)
@@ -304,6 +305,7 @@ struct h2w_handle {
     // deflation space built once for an inner precision (h2w_set_deflation): U (k columns), then
     // C = A U (k orthonormal columns); every correction of the refinement starts from it
     h2w::DevBuf<double> g_defl;
+    h2w::DevBuf<double> g_defl_rm;      // the same row-major (N x k each: U then C), for the lockstep solves
     int defl_k = 0, defl_prec = -1, defl_gen = -1;
     h2w::PinBuf g_pin;                // GMRES read-backs (Hessenberg columns)
     int64_t g_Vcols = 0;
@@ -355,6 +357,10 @@ struct Lockstep {
     std::vector<double> ir_floor;     // per system: 10^-p of its inner operator
     std::function<void(const double *, double *, int)> matvec;  // Y = A X, interleaved, a precision
     LockBufs *buf = nullptr;
+    // a deflation space shared by every column (one system only): row-major U and C (n x dk), built for
+    // the inner precision dprec; every correction of the refinement starts from it
+    const double *dU = nullptr, *dC = nullptr;
+    int dk = 0, dprec = -1;
 };
 // Split S consecutive slot ranges (system s: [start[s], start[s+1])) into dot-product blocks.
 void lock_blocks(const std::vector<int64_t> &start, std::vector<int2> &blk, std::vector<int32_t> &sblk);

@@ -2405,6 +2405,7 @@ void build_deflation(Handle &h, int m, int k, int prec) {
     h.defl_k = 0;
     h.defl_prec = -1;
     h.g_defl.release();
+    h.g_defl_rm.release();
     if (m <= 0) return;
     std::vector<double> v(N);
     uint64_t st = 0x9e3779b97f4a7c15ull;
@@ -2446,6 +2447,9 @@ void build_deflation(Handle &h, int m, int k, int prec) {
         throw Error{H2W_EINVAL, "deflation space: fewer than 2 steps, or a singular Arnoldi relation"};
     h.g_defl.alloc((size_t)2 * rc.k * N);
     H2W_HIP(hipMemcpyAsync(h.g_defl.p, h.g_U.p, (size_t)2 * rc.k * N * 8, hipMemcpyDeviceToDevice, s));
+    h.g_defl_rm.alloc((size_t)2 * rc.k * N);
+    launch_to_rowmajor(h.g_defl.p, N, rc.k, h.g_defl_rm.p, s);
+    launch_to_rowmajor(h.g_defl.p + (int64_t)rc.k * N, N, rc.k, h.g_defl_rm.p + (int64_t)rc.k * N, s);
     H2W_HIP(hipStreamSynchronize(s));
     h.defl_k = rc.k;
     h.defl_prec = prec;

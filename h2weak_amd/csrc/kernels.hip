@@ -2236,6 +2236,57 @@ __global__ __launch_bounds__(256) void k_bgemv16(const double *__restrict__ V, i
         w[e] += sign * acc;
     }
 }
+// A space shared by every column of a one-system lockstep vector (the deflation space): k vectors
+// of n slots, stored row-major (Cs[t * k + q]), so that the k values of a slot are contiguous.
+// partial[(q * nblk + b) * kNR + r] = sum over block b of Cs(t, q) w(t, r), q = 16 z .. 16 z + 15:
+// one pass over w serves 16 of the space's vectors (the reduction is k_bdot16_reduce's).
+__global__ __launch_bounds__(256) void k_sdot16(const double *__restrict__ Cs, int k, const double *__restrict__ w,
+                                                const int2 *__restrict__ blk, double *__restrict__ partial) {
+    const int b = blockIdx.x, z = blockIdx.y, nblk = gridDim.x;
+    const int q0 = 16 * z, nq = min(16, k - q0);
+    const int2 rg = blk[b];
+    const int r = threadIdx.x & 15;
+    double acc[16];
+#pragma unroll
+    for (int j = 0; j < 16; ++j) acc[j] = 0.0;
+    for (int64_t e = (int64_t)rg.x * kNR + threadIdx.x; e < (int64_t)rg.y * kNR; e += 256) {
+        const double we = w[e];
+        const double *c = Cs + (e / kNR) * k + q0;
+#pragma unroll
+        for (int j = 0; j < 16; ++j)
+            if (j < nq) acc[j] = fma(c[j], we, acc[j]);
+    }
+    __shared__ double sh[4][16][kNR];
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+        acc[j] += __shfl_xor(acc[j], 16, 64);
+        acc[j] += __shfl_xor(acc[j], 32, 64);
+        if ((threadIdx.x & 63) < 16) sh[threadIdx.x >> 6][j][r] = acc[j];
+    }
+    __syncthreads();
+    if (threadIdx.x < 16 * kNR) {
+        const int j = threadIdx.x / kNR, rr = threadIdx.x % kNR;
+        if (j < nq)
+            partial[((int64_t)(q0 + j) * nblk + b) * kNR + rr] = sh[0][j][rr] + sh[1][j][rr] + sh[2][j][rr] + sh[3][j][rr];
+    }
+}
+// w(t, r) += sign * sum_q Cs(t, q) h(q, r)   (the shared space, row-major as above)
+__global__ __launch_bounds__(256) void k_supd16(const double *__restrict__ Cs, int k, const double *__restrict__ h,
+                                                double sign, double *__restrict__ w, int64_t n) {
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n * kNR; e += (int64_t)gridDim.x * blockDim.x) {
+        const double *c = Cs + (e / kNR) * k;
+        const int r = (int)(e % kNR);
+        double acc = 0.0;
+        for (int q = 0; q < k; ++q) acc = fma(c[q], h[q * kNR + r], acc);
+        w[e] += sign * acc;
+    }
+}
+// dst[t * k + q] = src[q * n + t]: k column-major vectors of n entries to row-major
+__global__ void k_to_rowmajor(const double *__restrict__ src, int64_t n, int k, double *__restrict__ dst) {
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n * k; e += (int64_t)gridDim.x * blockDim.x)
+        dst[e] = src[(e % k) * n + e / k];
+}
+
 // dst(t, r) = sc(seg(t), r) * src(t, r)
 __global__ void k_bscale16(const double *__restrict__ src, const double *__restrict__ sc, double *dst, int64_t n,
                            const int32_t *__restrict__ seg) {
@@ -2549,6 +2600,15 @@ void launch_bgemv16(const double *V, int64_t ldv, int k, const double *h, int S,
     if (k > 0)
         hipLaunchKernelGGL(k_bgemv16, dim3(grid_for(n * kNR, 256, 4096)), dim3(256), 0, s, V, ldv, k, h, S, sign, w, n,
                            seg);
+}
+void launch_sdot16(const double *Cs, int k, const double *w, const int2 *blk, int nblk, double *partial, hipStream_t s) {
+    if (k > 0) hipLaunchKernelGGL(k_sdot16, dim3(nblk, (k + 15) / 16), dim3(256), 0, s, Cs, k, w, blk, partial);
+}
+void launch_supd16(const double *Cs, int k, const double *h, double sign, double *w, int64_t n, hipStream_t s) {
+    if (k > 0) hipLaunchKernelGGL(k_supd16, dim3(grid_for(n * kNR, 256, 4096)), dim3(256), 0, s, Cs, k, h, sign, w, n);
+}
+void launch_to_rowmajor(const double *src, int64_t n, int k, double *dst, hipStream_t s) {
+    if (k > 0) hipLaunchKernelGGL(k_to_rowmajor, dim3(grid_for(n * k, 256, 4096)), dim3(256), 0, s, src, n, k, dst);
 }
 void launch_bscale16(const double *src, const double *sc, double *dst, int64_t n, const int32_t *seg, hipStream_t s) {
     hipLaunchKernelGGL(k_bscale16, dim3(grid_for(n * kNR)), dim3(256), 0, s, src, sc, dst, n, seg);

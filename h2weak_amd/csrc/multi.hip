@@ -208,10 +208,12 @@ struct LockOut {
 // overrides |b_c|.  kc > 0 (with x_zero): recycle the space kept by lock_keep_space
 // (C in the first kc basis columns, U in buf->U), as the single-vector core does.
 // hkeep: receives the Hessenberg columns before the rotations, and every running
-// column's last basis vector is kept, for lock_keep_space.
+// column's last basis vector is kept, for lock_keep_space.  shared (with x_zero, kc = 0,
+// one system): start from the deflation space L.dU, L.dC that all columns share instead
+// (x = U C^T r, r -= C C^T r, every Arnoldi vector orthogonalised against C as well).
 LockOut lock_gmres(Lockstep &L, const double *b, double *x, const std::vector<char> &active, int max_iter,
                    const std::vector<double> &tol, int prec, bool x_zero, const double *normb_in, int kc = 0,
-                   std::vector<std::vector<double>> *hkeep = nullptr) {
+                   std::vector<std::vector<double>> *hkeep = nullptr, bool shared = false) {
     const int C = L.S * kNR;
     const int64_t len = L.n * kNR;
     hipStream_t s = L.s;
@@ -247,6 +249,16 @@ LockOut lock_gmres(Lockstep &L, const double *b, double *x, const std::vector<ch
         launch_bgemv16(B.V.p, len, kc, B.h.p, L.S, -1.0, r, L.n, L.seg, s);
         launch_bgemv16(B.U.p, len, kc, B.h.p, L.S, 1.0, x, L.n, L.seg, s);
     }
+    const int ks = shared && x_zero && kc == 0 && L.S == 1 ? L.dk : 0;
+    if (ks > 0) {
+        // the same with the shared space: x = U C^T r, r -= C C^T r
+        B.spart.ensure((size_t)L.nblk * ks * kNR);
+        B.sh.ensure((size_t)2 * ks * kNR);
+        launch_sdot16(L.dC, ks, r, L.blk, L.nblk, B.spart.p, s);
+        launch_bdot16_reduce(B.spart.p, L.nblk, ks, L.sblk, 1, B.sh.p, s);
+        launch_supd16(L.dC, ks, B.sh.p, -1.0, r, L.n, s);
+        launch_supd16(L.dU, ks, B.sh.p, 1.0, x, L.n, s);
+    }
     ldots(L, r, r, tmp.data());
     std::vector<char> run(C, 0);
     int nrun = 0;
@@ -273,7 +285,7 @@ LockOut lock_gmres(Lockstep &L, const double *b, double *x, const std::vector<ch
         sv[c * ld] = beta[c];
     }
     lscale(L, tmp, r, B.V.p + (int64_t)kc * len);
-    double *hh = B.pin.ensure((size_t)(2 * (kc + m + 2) + 2) * C);
+    double *hh = B.pin.ensure((size_t)(2 * (kc + m + 2) + 2) * C + (size_t)2 * ks * kNR);
     int last = 0;
     for (int i = 0; i < m && nrun > 0; ++i) {
         ensure_V(L, kc + i + 2);
@@ -281,24 +293,33 @@ LockOut lock_gmres(Lockstep &L, const double *b, double *x, const std::vector<ch
         L.matvec(V + (int64_t)i * len, w, prec);
         out.matvecs++;
         const int k = kc + i + 1;  // vectors orthogonalised against: C, v_0 .. v_i (contiguous)
-        launch_bdot16(Cb, len, k, w, L.blk, L.nblk, B.part.p, s);
-        launch_bdot16_reduce(B.part.p, L.nblk, k, L.sblk, L.S, B.h.p, s);
-        launch_bgemv16(Cb, len, k, B.h.p, L.S, -1.0, w, L.n, L.seg, s);
-        launch_bdot16(Cb, len, k, w, L.blk, L.nblk, B.part.p, s);
-        launch_bdot16_reduce(B.part.p, L.nblk, k, L.sblk, L.S, B.h.p + (size_t)k * C, s);
-        launch_bgemv16(Cb, len, k, B.h.p + (size_t)k * C, L.S, -1.0, w, L.n, L.seg, s);
+        // classical Gram-Schmidt twice; the shared space's dot products come from the same w
+        for (int pass = 0; pass < 2; ++pass) {
+            launch_bdot16(Cb, len, k, w, L.blk, L.nblk, B.part.p, s);
+            launch_bdot16_reduce(B.part.p, L.nblk, k, L.sblk, L.S, B.h.p + (size_t)pass * k * C, s);
+            if (ks > 0) {
+                launch_sdot16(L.dC, ks, w, L.blk, L.nblk, B.spart.p, s);
+                launch_bdot16_reduce(B.spart.p, L.nblk, ks, L.sblk, 1, B.sh.p + (size_t)pass * ks * kNR, s);
+                launch_supd16(L.dC, ks, B.sh.p + (size_t)pass * ks * kNR, -1.0, w, L.n, s);
+            }
+            launch_bgemv16(Cb, len, k, B.h.p + (size_t)pass * k * C, L.S, -1.0, w, L.n, L.seg, s);
+        }
         launch_bdot16(w, 0, 1, w, L.blk, L.nblk, B.part.p, s);
         launch_bdot16_reduce(B.part.p, L.nblk, 1, L.sblk, L.S, B.h.p + (size_t)2 * k * C, s);
         H2W_HIP(hipMemcpyAsync(hh, B.h.p, (size_t)(2 * k + 1) * C * 8, hipMemcpyDeviceToHost, s));
+        double *shh = hh + (size_t)(2 * k + 1) * C;
+        if (ks > 0) H2W_HIP(hipMemcpyAsync(shh, B.sh.p, (size_t)2 * ks * kNR * 8, hipMemcpyDeviceToHost, s));
         H2W_HIP(hipStreamSynchronize(s));
         last = i;
         H.emplace_back((size_t)(i + 2) * C, 0.0);
         Y.resize((size_t)(i + 1) * C, 0.0);
-        if (kc > 0) Bst.emplace_back((size_t)kc * C, 0.0);
+        if (kc > 0 || ks > 0) Bst.emplace_back((size_t)(kc + ks) * C, 0.0);
         for (int c = 0; c < C; ++c) {
             tmp[c] = 0.0;
             if (!run[c]) continue;
             for (int q = 0; q < kc; ++q) Bst.back()[(size_t)q * C + c] = hh[(size_t)q * C + c] + hh[(size_t)(k + q) * C + c];
+            for (int q = 0; q < ks; ++q)
+                Bst.back()[(size_t)q * C + c] = shh[(size_t)q * kNR + c] + shh[(size_t)(ks + q) * kNR + c];
             for (int q = 0; q <= i; ++q)
                 Hh(c, q, i) = hh[(size_t)(kc + q) * C + c] + hh[(size_t)(k + kc + q) * C + c];
             Hh(c, i + 1, i) = std::sqrt(hh[(size_t)2 * k * C + c]);
@@ -332,14 +353,16 @@ LockOut lock_gmres(Lockstep &L, const double *b, double *x, const std::vector<ch
     // x += V Y over the basis vectors the columns used, - U (B Y) with a recycled space
     H2W_HIP(hipMemcpyAsync(B.h.p, Y.data(), (size_t)(last + 1) * C * 8, hipMemcpyHostToDevice, s));
     launch_bgemv16(B.V.p + (int64_t)kc * len, len, last + 1, B.h.p, L.S, 1.0, x, L.n, L.seg, s);
-    if (kc > 0) {
-        std::vector<double> by((size_t)kc * C, 0.0);
+    if (kc > 0 || ks > 0) {
+        const int kk = kc + ks;
+        std::vector<double> by((size_t)kk * C, 0.0);
         for (int c = 0; c < C; ++c)
             for (int jj = 0; jj < out.iters[c] && jj < (int)Bst.size(); ++jj)
-                for (int q = 0; q < kc; ++q) by[(size_t)q * C + c] -= Bst[jj][(size_t)q * C + c] * Y[(size_t)jj * C + c];
+                for (int q = 0; q < kk; ++q) by[(size_t)q * C + c] -= Bst[jj][(size_t)q * C + c] * Y[(size_t)jj * C + c];
         B.coef.ensure(by.size());
         H2W_HIP(hipMemcpyAsync(B.coef.p, by.data(), by.size() * 8, hipMemcpyHostToDevice, s));
-        launch_bgemv16(B.U.p, len, kc, B.coef.p, L.S, 1.0, x, L.n, L.seg, s);
+        if (kc > 0) launch_bgemv16(B.U.p, len, kc, B.coef.p, L.S, 1.0, x, L.n, L.seg, s);
+        else launch_supd16(L.dU, ks, B.coef.p, 1.0, x, L.n, s);
     }
     H2W_HIP(hipStreamSynchronize(s));
     return out;
@@ -477,6 +500,7 @@ void lock_solve(Lockstep &L, const double *B, double *X, int nrhs, int max_iter,
         for (int c = 0; c < C; ++c) normb[c] = nb2[c] > 0.0 ? std::sqrt(nb2[c]) : 1.0;
         int outer = 0, total = 0, mv64 = 0, mv32 = 0;
         std::vector<char> run(C, 0);
+        const bool defl = L.dk > 0 && L.dprec == inner_prec && S == 1;
         bool fp64_resid = max_iter < 1;  // (no correction allowed: report x0's FP64 residual)
         while (true) {
             L.matvec(X, d, fp64_resid ? H2W_FP64 : inner_prec);
@@ -515,11 +539,14 @@ void lock_solve(Lockstep &L, const double *B, double *X, int nrhs, int max_iter,
             }
             lscale(L, tmp, r, r);  // converged columns get a zero right-hand side
             launch_fill(d, len, 0.0, s);
-            // later corrections recycle the first one's Krylov space (per column)
+            // every correction starts from the shared deflation space when one was built for this
+            // inner operator; otherwise later corrections recycle the first one's Krylov space (per column)
             std::vector<std::vector<double>> hk;
-            LockOut o = lock_gmres(L, r, d, run, max_iter - total, eta, inner_prec, true, nr.data(),
-                                   outer == 0 ? 0 : L.buf->rk, outer == 0 ? &hk : nullptr);
-            if (outer == 0) lock_keep_space(L, hk, o);
+            LockOut o = defl ? lock_gmres(L, r, d, run, max_iter - total, eta, inner_prec, true, nr.data(), 0,
+                                          nullptr, true)
+                             : lock_gmres(L, r, d, run, max_iter - total, eta, inner_prec, true, nr.data(),
+                                          outer == 0 ? 0 : L.buf->rk, outer == 0 ? &hk : nullptr);
+            if (outer == 0 && !defl) lock_keep_space(L, hk, o);
             int most = 0;
             for (int c = 0; c < C; ++c)
                 if (run[c]) {
@@ -531,8 +558,8 @@ void lock_solve(Lockstep &L, const double *B, double *X, int nrhs, int max_iter,
             mv32 += o.matvecs;
             launch_axpby(1.0, d, 1.0, X, len, s);
             ++outer;
-            // (no iteration and no recycled projection: the corrections were zero)
-            if (most == 0 && fp64_resid && L.buf->rk == 0) break;
+            // (no iteration and no recycled or deflation projection: the corrections were zero)
+            if (most == 0 && fp64_resid && !defl && L.buf->rk == 0) break;
             fp64_resid = true;
         }
         for (int c = 0; c < C; ++c)
@@ -568,6 +595,12 @@ void solve_tree_m(Handle &h, const double *B, double *X, int nrhs, int max_iter,
     L.ir_floor = {std::pow(10.0, -std::max(1.0, h.ir_exp))};
     L.matvec = [&h](const double *x, double *y, int prec) { matvec_m_tree(h, x, y, prec); };
     L.buf = &h.mbuf;
+    if (h.defl_k > 0 && (h.defl_prec != H2W_FP32_INNER || h.defl_gen == h.inner_gen)) {
+        L.dU = h.g_defl_rm.p;
+        L.dC = h.g_defl_rm.p + (int64_t)h.defl_k * h.N;
+        L.dk = h.defl_k;
+        L.dprec = h.defl_prec;
+    }
     lock_solve(L, B, X, nrhs, max_iter, tol, mode, st);
 }
 

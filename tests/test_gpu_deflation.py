@@ -142,3 +142,32 @@ def test_autotune_with_deflation(sys3d):
     assert s["converged"] and _rel(g.mat_vec_prod(x), b) < TOL and _rel(x, xt) <= 1e-8
     g.set_deflation(0)
     g.set_inner_tolerance(6.5)
+
+
+def test_lockstep_solves_share_the_space(sys3d):
+    """16 right-hand sides refined in lockstep start every correction from the same deflation space (one
+    row-major copy shared by the columns): every column converges to the single solve's accuracy, in no more
+    steps than without the space."""
+    g, xt, b = sys3d
+    rng = np.random.default_rng(8)
+    XT = rng.normal(size=(16, N))
+    B = np.stack([g.dense_rows(None, x) for x in XT])
+    g.set_deflation(0)
+    _, st0 = g.gmres_multi(B, None, 500, TOL, h2w.GMRES_FP32_INNER_IR)
+    g.set_deflation(64, 24)
+    X, st = g.gmres_multi(B, None, 500, TOL, h2w.GMRES_FP32_INNER_IR)
+    its0, its = max(s["iterations"] for s in st0), max(s["iterations"] for s in st)
+    print(f"lockstep: {its} inner steps with the space, {its0} without")
+    for c in range(16):
+        assert st[c]["converged"] and st[c]["resid"] < TOL, (c, st[c])
+        assert _rel(g.mat_vec_prod(X[c]), B[c]) < TOL
+        assert _rel(X[c], XT[c]) <= 1e-8
+    assert its <= its0
+    # 5 columns (a partial pass) and the stored FP32 operator's space
+    X5, st5 = g.gmres_multi(B[:5], None, 500, TOL, h2w.GMRES_FP32_INNER_IR)
+    assert all(s["converged"] for s in st5) and max(_rel(X5[c], XT[c]) for c in range(5)) <= 1e-8
+    g.set_deflation(64, 0, h2w.FP32)
+    X, st = g.gmres_multi(B, None, 500, TOL, h2w.GMRES_FP32_IR)
+    assert all(s["converged"] for s in st) and max(_rel(X[c], XT[c]) for c in range(16)) <= 1e-8
+    g.set_deflation(0, 0, h2w.FP32)
+    g.set_deflation(0)
