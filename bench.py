@@ -252,7 +252,7 @@ def kernel_table(path, top=25):
         print(f"| {tot * 1e-6:.1f} | {calls} | {avg * 1e-3:.1f} | {pct:.1f} | `{name}` |")
 
 
-def bench_batch(S=32, tol_pow=8, seed=5):
+def bench_batch(S=32, tol_pow=8, seed=5, defl=(128, 48)):
     """32 independent 3D H2* RBF systems, N log-uniform in [4096, 65536] (seed 5), points U([-1,1]^3),
     n_max 64, ACA and GMRES tolerance 1e-8, 16 right-hand sides each (X_true ~ N(0,1), B = A X_true):
     batched products (one launch per phase for all systems, 16 right-hand sides per pass) and lockstep
@@ -284,8 +284,7 @@ def bench_batch(S=32, tol_pow=8, seed=5):
         log(f"batch 16-wide product {name}: {r['ms_per_pass']:.3f} ms, {len(r['launch_role'])} launches")
     Xt = [np.random.default_rng(200 + q).normal(size=(16, g.N)) for q, g in enumerate(systems)]
     Bs = B.matvec(Xt)
-    for name, mode in (("fp64", h2w.GMRES_FP64), ("fp32_inner_ir", h2w.GMRES_FP32_INNER_IR),
-                       ("fp32_ir", h2w.GMRES_FP32_IR)):
+    def solves(name, mode):
         B.gmres([b[:1] for b in Bs], None, 300, gtol, mode)  # warm-up (graph capture)
         Xs, st = B.gmres(Bs, None, 300, gtol, mode)
         cols = [c for row in st for c in row]
@@ -295,6 +294,17 @@ def bench_batch(S=32, tol_pow=8, seed=5):
                               "all_converged": all(c["converged"] for c in cols),
                               "max_iterations": max(c["iterations"] for c in cols), "max_rel_error": err}
         log(f"batch solves {name}: {cols[0]['seconds'] * 1e3:.1f} ms for 512 columns, max error {err:.2e}")
+
+    solves("fp64", h2w.GMRES_FP64)
+    solves("fp32_ir", h2w.GMRES_FP32_IR)
+    if defl:
+        solves("fp32_inner_ir_no_deflation", h2w.GMRES_FP32_INNER_IR)
+        # every system's deflation space for its inner operator (operator only, outside the timed solves)
+        t0 = time.time()
+        for g in systems:
+            g.set_deflation(*defl)
+        out["deflation"] = {"m": defl[0], "k": defl[1], "build_s_all_systems": round(time.time() - t0, 3)}
+    solves("fp32_inner_ir", h2w.GMRES_FP32_INNER_IR)
     # plain FP64, one right-hand side at a time: every 4th system's first column, scaled to 512 solves
     ts = []
     for q in range(0, S, 4):

@@ -48,6 +48,12 @@ struct h2w_batch {
     hipGraphExec_t graph[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
     h2w::LockBufs buf;
     bool sym16 = true;                   // fused launch reads each block once (k_bgemm_sym)
+    // the systems' deflation spaces (h2w_set_deflation on each inner operator), row-major over the
+    // lockstep slots and zero-padded to the largest k: U (n x k) then C (n x k); rebuilt when a
+    // system's space changes
+    h2w::DevBuf<double> defl;
+    int defl_k = 0;
+    std::vector<int> defl_src;           // the systems' space serials it was built from
 };
 
 namespace h2w {
@@ -214,6 +220,37 @@ void batch_matvec_lock(Batch &B, const double *X, double *Y, int precision) {
     H2W_HIP(hipGetLastError());
 }
 
+// The systems' deflation spaces as one row-major array over the lockstep slots, when every system has one
+// for its current inner operator (else none is used).
+void batch_deflation(Batch &B, Lockstep &L) {
+    int k = 0;
+    std::vector<int> src;
+    for (auto *h : B.sys) {
+        if (!(h->defl_k > 0 && h->defl_prec == H2W_FP32_INNER && h->defl_gen == h->inner_gen)) return;
+        k = std::max(k, h->defl_k);
+        src.push_back(h->defl_serial);
+    }
+    hipStream_t s = B.stream;
+    if (k != B.defl_k || src != B.defl_src) {
+        B.defl.alloc((size_t)2 * k * B.n);
+        launch_fill(B.defl.p, 2 * k * B.n, 0.0, s);
+        for (int sy = 0; sy < B.S; ++sy) {
+            const h2w_handle &h = *B.sys[sy];
+            const int64_t Ns = B.nstart[sy + 1] - B.nstart[sy];
+            for (int part = 0; part < 2; ++part)  // U, then C
+                H2W_HIP(hipMemcpy2DAsync(B.defl.p + (int64_t)part * k * B.n + B.nstart[sy] * k, (size_t)k * 8,
+                                         h.g_defl_rm.p + (int64_t)part * h.defl_k * Ns, (size_t)h.defl_k * 8,
+                                         (size_t)h.defl_k * 8, (size_t)Ns, hipMemcpyDeviceToDevice, s));
+        }
+        B.defl_k = k;
+        B.defl_src = src;
+    }
+    L.dU = B.defl.p;
+    L.dC = B.defl.p + (int64_t)k * B.n;
+    L.dk = k;
+    L.dprec = H2W_FP32_INNER;
+}
+
 Lockstep lockstep_of(Batch &B) {
     Lockstep L;
     L.s = B.stream;
@@ -226,6 +263,7 @@ Lockstep lockstep_of(Batch &B) {
     for (auto *h : B.sys) L.ir_floor.push_back(std::pow(10.0, -std::max(1.0, h->ir_exp)));
     L.matvec = [&B](const double *x, double *y, int prec) { batch_matvec_lock(B, x, y, prec); };
     L.buf = &B.buf;
+    batch_deflation(B, L);
     return L;
 }
 

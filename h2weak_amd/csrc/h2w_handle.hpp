@@ -307,6 +307,7 @@ struct h2w_handle {
     h2w::DevBuf<double> g_defl;
     h2w::DevBuf<double> g_defl_rm;      // the same row-major (N x k each: U then C), for the lockstep solves
     int defl_k = 0, defl_prec = -1, defl_gen = -1;
+    int defl_serial = 0;                // incremented whenever the space is rebuilt or dropped
     h2w::PinBuf g_pin;                // GMRES read-backs (Hessenberg columns)
     int64_t g_Vcols = 0;
     // kNR-wide lockstep GMRES (interleaved N x kNR vectors)
@@ -357,8 +358,9 @@ struct Lockstep {
     std::vector<double> ir_floor;     // per system: 10^-p of its inner operator
     std::function<void(const double *, double *, int)> matvec;  // Y = A X, interleaved, a precision
     LockBufs *buf = nullptr;
-    // a deflation space shared by every column (one system only): row-major U and C (n x dk), built for
-    // the inner precision dprec; every correction of the refinement starts from it
+    // deflation spaces shared by the columns of each system: row-major U and C (n x dk, every system's
+    // slots holding its own space, zero-padded to dk), built for the inner precision dprec; every
+    // correction of the refinement starts from them
     const double *dU = nullptr, *dC = nullptr;
     int dk = 0, dprec = -1;
 };

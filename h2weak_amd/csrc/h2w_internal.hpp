@@ -285,10 +285,12 @@ void launch_scatter_b(const double *v, const int64_t *src, const int32_t *ld, in
 // rank predictor: out[i] = log2(1 + predicted rank) of block i (16 features each, BF16 parameters)
 void launch_rank_mlp(const float *feat, int n, const uint16_t *w, float *out, hipStream_t s);
 // lockstep solves over S systems x kNR columns (segmented interleaved vectors, seg == NULL: S == 1)
-// a space of k vectors shared by all kNR columns (row-major, n x k): per-block partial dot products with
-// a lockstep vector (reduced by launch_bdot16_reduce), w += sign Cs h (h: k x kNR), and the row-major copy
+// a space of k vectors shared by all kNR columns of a system (row-major, n x k; several systems: each
+// system's slots hold its own space): per-block partial dot products with a lockstep vector (reduced per
+// system by launch_bdot16_reduce), w += sign Cs h (h: k x S x kNR), and the row-major copy
 void launch_sdot16(const double *Cs, int k, const double *w, const int2 *blk, int nblk, double *partial, hipStream_t s);
-void launch_supd16(const double *Cs, int k, const double *h, double sign, double *w, int64_t n, hipStream_t s);
+void launch_supd16(const double *Cs, int k, const double *h, int S, double sign, double *w, int64_t n,
+                   const int32_t *seg, hipStream_t s);
 void launch_to_rowmajor(const double *src, int64_t n, int k, double *dst, hipStream_t s);
 void launch_bdot16(const double *V, int64_t ldv, int k, const double *w, const int2 *blk, int nblk, double *partial,
                    hipStream_t s);

@@ -2404,6 +2404,7 @@ void build_deflation(Handle &h, int m, int k, int prec) {
     H2W_HIP(hipStreamSynchronize(s));
     h.defl_k = 0;
     h.defl_prec = -1;
+    ++h.defl_serial;
     h.g_defl.release();
     h.g_defl_rm.release();
     if (m <= 0) return;

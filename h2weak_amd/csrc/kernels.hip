@@ -2270,14 +2270,17 @@ __global__ __launch_bounds__(256) void k_sdot16(const double *__restrict__ Cs, i
             partial[((int64_t)(q0 + j) * nblk + b) * kNR + rr] = sh[0][j][rr] + sh[1][j][rr] + sh[2][j][rr] + sh[3][j][rr];
     }
 }
-// w(t, r) += sign * sum_q Cs(t, q) h(q, r)   (the shared space, row-major as above)
+// w(t, r) += sign * sum_q Cs(t, q) h(q, seg(t), r)   (the shared space, row-major as above; with several
+// systems every system's slots carry its own space, h holds the coefficients of every system)
 __global__ __launch_bounds__(256) void k_supd16(const double *__restrict__ Cs, int k, const double *__restrict__ h,
-                                                double sign, double *__restrict__ w, int64_t n) {
+                                                int S, double sign, double *__restrict__ w, int64_t n,
+                                                const int32_t *__restrict__ seg) {
     for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n * kNR; e += (int64_t)gridDim.x * blockDim.x) {
-        const double *c = Cs + (e / kNR) * k;
-        const int r = (int)(e % kNR);
+        const int64_t t = e / kNR;
+        const double *c = Cs + t * k;
+        const int col = (seg ? seg[t] * kNR : 0) + (int)(e % kNR);
         double acc = 0.0;
-        for (int q = 0; q < k; ++q) acc = fma(c[q], h[q * kNR + r], acc);
+        for (int q = 0; q < k; ++q) acc = fma(c[q], h[(int64_t)q * S * kNR + col], acc);
         w[e] += sign * acc;
     }
 }
@@ -2604,8 +2607,10 @@ void launch_bgemv16(const double *V, int64_t ldv, int k, const double *h, int S,
 void launch_sdot16(const double *Cs, int k, const double *w, const int2 *blk, int nblk, double *partial, hipStream_t s) {
     if (k > 0) hipLaunchKernelGGL(k_sdot16, dim3(nblk, (k + 15) / 16), dim3(256), 0, s, Cs, k, w, blk, partial);
 }
-void launch_supd16(const double *Cs, int k, const double *h, double sign, double *w, int64_t n, hipStream_t s) {
-    if (k > 0) hipLaunchKernelGGL(k_supd16, dim3(grid_for(n * kNR, 256, 4096)), dim3(256), 0, s, Cs, k, h, sign, w, n);
+void launch_supd16(const double *Cs, int k, const double *h, int S, double sign, double *w, int64_t n,
+                   const int32_t *seg, hipStream_t s) {
+    if (k > 0)
+        hipLaunchKernelGGL(k_supd16, dim3(grid_for(n * kNR, 256, 4096)), dim3(256), 0, s, Cs, k, h, S, sign, w, n, seg);
 }
 void launch_to_rowmajor(const double *src, int64_t n, int k, double *dst, hipStream_t s) {
     if (k > 0) hipLaunchKernelGGL(k_to_rowmajor, dim3(grid_for(n * k, 256, 4096)), dim3(256), 0, s, src, n, k, dst);

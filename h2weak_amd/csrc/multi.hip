@@ -208,8 +208,8 @@ struct LockOut {
 // overrides |b_c|.  kc > 0 (with x_zero): recycle the space kept by lock_keep_space
 // (C in the first kc basis columns, U in buf->U), as the single-vector core does.
 // hkeep: receives the Hessenberg columns before the rotations, and every running
-// column's last basis vector is kept, for lock_keep_space.  shared (with x_zero, kc = 0,
-// one system): start from the deflation space L.dU, L.dC that all columns share instead
+// column's last basis vector is kept, for lock_keep_space.  shared (with x_zero, kc = 0):
+// start from the deflation space L.dU, L.dC that all columns of a system share instead
 // (x = U C^T r, r -= C C^T r, every Arnoldi vector orthogonalised against C as well).
 LockOut lock_gmres(Lockstep &L, const double *b, double *x, const std::vector<char> &active, int max_iter,
                    const std::vector<double> &tol, int prec, bool x_zero, const double *normb_in, int kc = 0,
@@ -249,15 +249,15 @@ LockOut lock_gmres(Lockstep &L, const double *b, double *x, const std::vector<ch
         launch_bgemv16(B.V.p, len, kc, B.h.p, L.S, -1.0, r, L.n, L.seg, s);
         launch_bgemv16(B.U.p, len, kc, B.h.p, L.S, 1.0, x, L.n, L.seg, s);
     }
-    const int ks = shared && x_zero && kc == 0 && L.S == 1 ? L.dk : 0;
+    const int ks = shared && x_zero && kc == 0 ? L.dk : 0;
     if (ks > 0) {
         // the same with the shared space: x = U C^T r, r -= C C^T r
         B.spart.ensure((size_t)L.nblk * ks * kNR);
-        B.sh.ensure((size_t)2 * ks * kNR);
+        B.sh.ensure((size_t)2 * ks * C);
         launch_sdot16(L.dC, ks, r, L.blk, L.nblk, B.spart.p, s);
-        launch_bdot16_reduce(B.spart.p, L.nblk, ks, L.sblk, 1, B.sh.p, s);
-        launch_supd16(L.dC, ks, B.sh.p, -1.0, r, L.n, s);
-        launch_supd16(L.dU, ks, B.sh.p, 1.0, x, L.n, s);
+        launch_bdot16_reduce(B.spart.p, L.nblk, ks, L.sblk, L.S, B.sh.p, s);
+        launch_supd16(L.dC, ks, B.sh.p, L.S, -1.0, r, L.n, L.seg, s);
+        launch_supd16(L.dU, ks, B.sh.p, L.S, 1.0, x, L.n, L.seg, s);
     }
     ldots(L, r, r, tmp.data());
     std::vector<char> run(C, 0);
@@ -285,7 +285,7 @@ LockOut lock_gmres(Lockstep &L, const double *b, double *x, const std::vector<ch
         sv[c * ld] = beta[c];
     }
     lscale(L, tmp, r, B.V.p + (int64_t)kc * len);
-    double *hh = B.pin.ensure((size_t)(2 * (kc + m + 2) + 2) * C + (size_t)2 * ks * kNR);
+    double *hh = B.pin.ensure((size_t)(2 * (kc + m + 2) + 2) * C + (size_t)2 * ks * C);
     int last = 0;
     for (int i = 0; i < m && nrun > 0; ++i) {
         ensure_V(L, kc + i + 2);
@@ -299,8 +299,8 @@ LockOut lock_gmres(Lockstep &L, const double *b, double *x, const std::vector<ch
             launch_bdot16_reduce(B.part.p, L.nblk, k, L.sblk, L.S, B.h.p + (size_t)pass * k * C, s);
             if (ks > 0) {
                 launch_sdot16(L.dC, ks, w, L.blk, L.nblk, B.spart.p, s);
-                launch_bdot16_reduce(B.spart.p, L.nblk, ks, L.sblk, 1, B.sh.p + (size_t)pass * ks * kNR, s);
-                launch_supd16(L.dC, ks, B.sh.p + (size_t)pass * ks * kNR, -1.0, w, L.n, s);
+                launch_bdot16_reduce(B.spart.p, L.nblk, ks, L.sblk, L.S, B.sh.p + (size_t)pass * ks * C, s);
+                launch_supd16(L.dC, ks, B.sh.p + (size_t)pass * ks * C, L.S, -1.0, w, L.n, L.seg, s);
             }
             launch_bgemv16(Cb, len, k, B.h.p + (size_t)pass * k * C, L.S, -1.0, w, L.n, L.seg, s);
         }
@@ -308,7 +308,7 @@ LockOut lock_gmres(Lockstep &L, const double *b, double *x, const std::vector<ch
         launch_bdot16_reduce(B.part.p, L.nblk, 1, L.sblk, L.S, B.h.p + (size_t)2 * k * C, s);
         H2W_HIP(hipMemcpyAsync(hh, B.h.p, (size_t)(2 * k + 1) * C * 8, hipMemcpyDeviceToHost, s));
         double *shh = hh + (size_t)(2 * k + 1) * C;
-        if (ks > 0) H2W_HIP(hipMemcpyAsync(shh, B.sh.p, (size_t)2 * ks * kNR * 8, hipMemcpyDeviceToHost, s));
+        if (ks > 0) H2W_HIP(hipMemcpyAsync(shh, B.sh.p, (size_t)2 * ks * C * 8, hipMemcpyDeviceToHost, s));
         H2W_HIP(hipStreamSynchronize(s));
         last = i;
         H.emplace_back((size_t)(i + 2) * C, 0.0);
@@ -319,7 +319,7 @@ LockOut lock_gmres(Lockstep &L, const double *b, double *x, const std::vector<ch
             if (!run[c]) continue;
             for (int q = 0; q < kc; ++q) Bst.back()[(size_t)q * C + c] = hh[(size_t)q * C + c] + hh[(size_t)(k + q) * C + c];
             for (int q = 0; q < ks; ++q)
-                Bst.back()[(size_t)q * C + c] = shh[(size_t)q * kNR + c] + shh[(size_t)(ks + q) * kNR + c];
+                Bst.back()[(size_t)q * C + c] = shh[(size_t)q * C + c] + shh[(size_t)(ks + q) * C + c];
             for (int q = 0; q <= i; ++q)
                 Hh(c, q, i) = hh[(size_t)(kc + q) * C + c] + hh[(size_t)(k + kc + q) * C + c];
             Hh(c, i + 1, i) = std::sqrt(hh[(size_t)2 * k * C + c]);
@@ -362,7 +362,7 @@ LockOut lock_gmres(Lockstep &L, const double *b, double *x, const std::vector<ch
         B.coef.ensure(by.size());
         H2W_HIP(hipMemcpyAsync(B.coef.p, by.data(), by.size() * 8, hipMemcpyHostToDevice, s));
         if (kc > 0) launch_bgemv16(B.U.p, len, kc, B.coef.p, L.S, 1.0, x, L.n, L.seg, s);
-        else launch_supd16(L.dU, ks, B.coef.p, 1.0, x, L.n, s);
+        else launch_supd16(L.dU, ks, B.coef.p, L.S, 1.0, x, L.n, L.seg, s);
     }
     H2W_HIP(hipStreamSynchronize(s));
     return out;
@@ -500,7 +500,7 @@ void lock_solve(Lockstep &L, const double *B, double *X, int nrhs, int max_iter,
         for (int c = 0; c < C; ++c) normb[c] = nb2[c] > 0.0 ? std::sqrt(nb2[c]) : 1.0;
         int outer = 0, total = 0, mv64 = 0, mv32 = 0;
         std::vector<char> run(C, 0);
-        const bool defl = L.dk > 0 && L.dprec == inner_prec && S == 1;
+        const bool defl = L.dk > 0 && L.dprec == inner_prec;
         bool fp64_resid = max_iter < 1;  // (no correction allowed: report x0's FP64 residual)
         while (true) {
             L.matvec(X, d, fp64_resid ? H2W_FP64 : inner_prec);

@@ -171,3 +171,38 @@ def test_lockstep_solves_share_the_space(sys3d):
     assert all(s["converged"] for s in st) and max(_rel(X[c], XT[c]) for c in range(16)) <= 1e-8
     g.set_deflation(0, 0, h2w.FP32)
     g.set_deflation(0)
+
+
+def test_batched_solves_use_each_systems_space():
+    """A batch whose systems each carry a deflation space: every system's lockstep columns start from its
+    own space (zero-padded to the largest k), every column converges to the single FP64 solve's accuracy, in
+    no more steps than without the spaces; a system without a space turns them off for the batch."""
+    rng = np.random.default_rng(21)
+    systems = []
+    for n in (3000, 5200, 4100):
+        P = rng.uniform(-1, 1, (3, n))
+        g = h2w.Initialisation(3, n, 64, 1.0, 8, h2w.CHOICE_RBF, h2w.NESTED, points=P)
+        g.tree_initialisation()
+        g.set_inner_tolerance(5)
+        systems.append(g)
+    XT = [np.random.default_rng(30 + q).normal(size=(16, g.N)) for q, g in enumerate(systems)]
+    Bs = [np.stack([g.dense_rows(None, x) for x in X]) for g, X in zip(systems, XT)]
+    bt = h2w.Batch(systems)
+    _, st0 = bt.gmres(Bs, None, 300, 1e-8, h2w.GMRES_FP32_INNER_IR)
+    for g, (m, k) in zip(systems, ((48, 16), (64, 24), (32, 0))):
+        g.set_deflation(m, k)
+    Xs, st = bt.gmres(Bs, None, 300, 1e-8, h2w.GMRES_FP32_INNER_IR)
+    its0 = max(c["iterations"] for row in st0 for c in row)
+    its = max(c["iterations"] for row in st for c in row)
+    print(f"batch: {its} inner steps with the spaces, {its0} without")
+    for q, g in enumerate(systems):
+        for c in range(16):
+            assert st[q][c]["converged"], (q, c, st[q][c])
+            e1 = _rel(g.gmres(Bs[q][c], None, 300, 1e-8, h2w.GMRES_FP64)[0], XT[q][c])
+            assert _rel(Xs[q][c], XT[q][c]) <= 10 * e1 + 1e-12
+    assert its <= its0
+    # one system without a space: none is used, the batch still converges
+    systems[1].set_deflation(0)
+    _, st2 = bt.gmres(Bs, None, 300, 1e-8, h2w.GMRES_FP32_INNER_IR)
+    assert all(c["converged"] for row in st2 for c in row)
+    assert abs(max(c["iterations"] for row in st2 for c in row) - its0) <= 1
