@@ -366,7 +366,7 @@ class Initialisation:
         """Deflation space of the refinement: m Arnoldi steps of the inner operator (default the truncated
         inner operator, H2W_FP32_INNER) from a fixed pseudo-random vector, the whole space (k = 0) or its k
         Ritz vectors farthest from the diagonal value; every later correction in that refinement mode starts
-        from it.  m = 0 drops it."""
+        from it (single solves, gmres_multi, and batches whose every system has one).  m = 0 drops it."""
         _check(lib().h2w_set_deflation(self._h, int(m), int(k), FP32_INNER if precision is None else int(precision)))
 
     def autotune(self, b=None, tol=1e-10, max_iter=1000, flags=TUNE_UNROLL | TUNE_INNER):

@@ -241,13 +241,14 @@ int h2w_set_inner_tolerance(h2w_handle *h, double ir_tol_pow);
  * (H2W_FP32, H2W_FP32_INNER or H2W_FP32_OTF) from a fixed pseudo-random vector.  k = 0 keeps that
  * whole Krylov space, 0 < k < m the k Ritz vectors whose Ritz values lie farthest from the
  * diagonal value (the operator is that value times I plus a kernel matrix; its spectrum clusters
- * there).  Kept as U and C = A U with orthonormal C (2 k N doubles of device memory).  Every
- * correction of a later h2w_gmres in the matching refinement mode starts from x = U C^T r and
- * orthogonalises its Arnoldi vectors against C as well (GCRO), which removes the operator's
- * outlying eigendirections from every solve.  It depends on the operator only, never on a
- * right-hand side; m = 0 drops it, and rebuilding the inner operator (h2w_set_inner_tolerance,
- * h2w_autotune) invalidates it for H2W_FP32_INNER.  Plain FP64 GMRES and the lockstep solves do
- * not use it. */
+ * there).  Kept as U and C = A U with orthonormal C (and a row-major copy of both for the
+ * 16-column solves: 4 k N doubles of device memory).  Every correction of a later h2w_gmres,
+ * h2w_gmres_multi (one copy shared by the columns) or h2w_batch_gmres (when every system of the
+ * batch has one) in the matching refinement mode starts from x = U C^T r and orthogonalises its
+ * Arnoldi vectors against C as well (GCRO), which removes the operator's outlying eigendirections
+ * from every solve.  It depends on the operator only, never on a right-hand side; m = 0 drops it,
+ * and rebuilding the inner operator (h2w_set_inner_tolerance, h2w_autotune) invalidates it for
+ * H2W_FP32_INNER.  Plain FP64 GMRES does not use it. */
 int h2w_set_deflation(h2w_handle *h, int m, int k, int precision);
 
 /* Autotuner (no counterpart in the reference).  With H2W_TUNE_UNROLL it times every launch of
