@@ -265,13 +265,12 @@ def bench_batch(S=32, tol_pow=8, seed=5, defl=(128, 48)):
     systems = []
     for q, N in enumerate(Ns):
         P = np.random.default_rng(100 + q).uniform(-1.0, 1.0, (3, N))
-        g = h2w.Initialisation(3, N, 64, 1.0, tol_pow, h2w.CHOICE_RBF, h2w.NESTED, points=P)
-        g.tree_initialisation()
-        systems.append(g)
+        systems.append(h2w.Initialisation(3, N, 64, 1.0, tol_pow, h2w.CHOICE_RBF, h2w.NESTED, points=P))
+    h2w.assemble_many(systems, 4)  # four host threads, one stream per system
     t_asm = time.time() - t0
-    log(f"batch: {S} systems, N {min(Ns)}..{max(Ns)} (sum {sum(Ns)}), assembly {t_asm:.1f} s")
+    log(f"batch: {S} systems, N {min(Ns)}..{max(Ns)} (sum {sum(Ns)}), assembly {t_asm:.1f} s (4 threads)")
     B = h2w.Batch(systems)
-    out = {"systems": S, "N": Ns, "nrhs": 16, "tol": gtol, "assembly_s": round(t_asm, 2),
+    out = {"systems": S, "N": Ns, "nrhs": 16, "tol": gtol, "assembly_s": round(t_asm, 2), "assembly_threads": 4,
            "nlevels": sorted({g.stats()["nlevels"] for g in systems}), "matvec": {}, "gmres": {}}
     sts = [g.stats() for g in systems]
     for prec, name, key, s_op in ((h2w.FP64, "fp64", "stored_bytes_fp64", 8), (h2w.FP32, "fp32", "stored_bytes_fp32", 4),

@@ -34,7 +34,7 @@ GMRES_FP64, GMRES_FP32_IR, GMRES_FP32_OTF_IR, GMRES_FP32_INNER_IR = 0, 1, 2, 3
 # every symbol include/h2w.h declares
 EXPORTS = ("h2w_default_opts", "h2w_create", "h2w_assemble", "h2w_matvec", "h2w_matvec_ex", "h2w_gmres",
            "h2w_dense_matvec", "h2w_get_stats", "h2w_tree_order", "h2w_box_skeleton", "h2w_interaction_list",
-           "h2w_near_list", "h2w_destroy", "h2w_last_error", "h2w_bench_matvec", "h2w_set_inner_tolerance", "h2w_set_deflation",
+           "h2w_near_list", "h2w_destroy", "h2w_last_error", "h2w_bench_matvec", "h2w_set_inner_tolerance", "h2w_set_deflation", "h2w_assemble_many",
            "h2w_autotune", "h2w_gmres_multi", "h2w_bench_matvec_multi", "h2w_batch_create", "h2w_batch_matvec",
            "h2w_batch_gmres", "h2w_batch_bench_matvec", "h2w_batch_destroy", "h2w_set_rank_model",
            "h2w_rank_samples", "h2w_rank_predict", "h2w_save", "h2w_load", "h2w_bench_stream_read",
@@ -130,6 +130,7 @@ def lib():
     L.h2w_bench_matvec.argtypes = [p, i, i, ctypes.POINTER(BenchResult)]
     L.h2w_set_inner_tolerance.argtypes = [p, d]
     L.h2w_set_deflation.argtypes = [p, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+    L.h2w_assemble_many.argtypes = [p, ctypes.c_int, ctypes.c_int]
     L.h2w_bench_matvec_multi.argtypes = [p, i, i, ctypes.POINTER(BenchResult)]
     L.h2w_batch_create.argtypes = [ctypes.POINTER(p), i, ctypes.POINTER(p)]
     L.h2w_batch_matvec.argtypes = [p, p, p, i, i, i]
@@ -462,6 +463,14 @@ class Batch:
         n = r.n_launch
         return {"ms_per_pass": r.ms_per_matvec, "nrhs": 16, "launch_bytes": list(r.launch_bytes[:n]),
                 "launch_role": list(r.launch_role[:n]), "launch_ms": list(r.launch_ms[:n])}
+
+
+def assemble_many(systems, nthreads=4):
+    """tree_initialisation() of several Initialisation objects at once: up to nthreads host threads, each
+    system on its own stream (h2w_assemble_many); the operators equal one-by-one assembly's."""
+    systems = list(systems)
+    arr = (ctypes.c_void_p * len(systems))(*[g._h for g in systems])
+    _check(lib().h2w_assemble_many(arr, len(systems), int(nthreads)))
 
 
 class IterativeSolver:

@@ -14,6 +14,10 @@
 // per level, leaf down, semi-nested CUR down).
 #include <hip/hip_runtime.h>
 
+#include <atomic>
+#include <mutex>
+#include <thread>
+
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -3037,6 +3041,43 @@ int h2w_assemble(h2w_handle *h) {
     if (h->opts.plan_only) throw Error{H2W_ESTATE, "handle was created with plan_only"};
     H2W_HIP(hipSetDevice(h->opts.device));
     h2w::assemble(*h);
+    return H2W_OK;
+    H2W_CATCH
+}
+
+int h2w_assemble_many(h2w_handle *const *hs, int n, int nthreads) {
+    H2W_TRY
+    if (!hs || n < 0 || nthreads < 1) throw Error{H2W_EINVAL, "bad arguments"};
+    for (int i = 0; i < n; ++i) {
+        if (!hs[i]) throw Error{H2W_EINVAL, "NULL handle"};
+        if (hs[i]->opts.plan_only) throw Error{H2W_ESTATE, "handle was created with plan_only"};
+        for (int j = 0; j < i; ++j)
+            if (hs[j] == hs[i]) throw Error{H2W_EINVAL, "a handle appears twice"};
+    }
+    // host threads take the handles in turn; each assembles on its handle's own stream, so the
+    // small systems' level-by-level launches overlap on the device
+    std::atomic<int> next{0};
+    std::mutex mu;
+    int code = H2W_OK;
+    std::string msg;
+    auto work = [&]() {
+        for (int i = next++; i < n; i = next++) {
+            const int rc = h2w_assemble(hs[i]);
+            if (rc != H2W_OK) {
+                std::lock_guard<std::mutex> lk(mu);
+                if (code == H2W_OK) {
+                    code = rc;
+                    msg = std::string("system ") + std::to_string(i) + ": " + h2w_last_error();
+                }
+            }
+        }
+    };
+    const int nt = std::min(nthreads, n);
+    std::vector<std::thread> th;
+    for (int t = 1; t < nt; ++t) th.emplace_back(work);
+    if (nt >= 1) work();
+    for (auto &t : th) t.join();
+    if (code != H2W_OK) throw Error{code, msg};
     return H2W_OK;
     H2W_CATCH
 }

@@ -154,6 +154,12 @@ int h2w_create(int dim, int variant, int64_t N, const double *points, int n_max,
 /* Assembly.  Replaces `initialisation::tree_initialisation()`
  * (3D/codes/matvec.hpp:33-61 nested, :86-113 semi-nested; 1D/codes/matvec.hpp:28-49,67-87). */
 int h2w_assemble(h2w_handle *h);
+/* No counterpart in the reference (one system per process): h2w_assemble on n handles from up to
+ * nthreads host threads, each system's launches on its own stream, so that the latency-bound level
+ * loops of small systems overlap on the device.  The operators are those of h2w_assemble one by one.
+ * Every handle is assembled or the first failure is returned (with the system's index in
+ * h2w_last_error()). */
+int h2w_assemble_many(h2w_handle *const *hs, int n, int nthreads);
 
 /* y = A x (overwrites y).  Replaces `initialisation::mat_vec_prod(Vec& charge, Vec& output)`
  * (3D/codes/matvec.hpp:63-83, :115-136; 1D/codes/matvec.hpp:51-65,89-103). */

@@ -87,3 +87,29 @@ def test_batched_solves(batch):
                 assert st[s][c]["converged"], (mode, s, c, st[s][c])
                 assert _rel(g.mat_vec_prod(Xs[s][c]), Bs[s][c]) <= 2 * gtol
                 assert _rel(Xs[s][c], Xt[s][c]) <= 10 * e64[s][c] + 1e-12, (mode, s, c)
+
+
+def test_assemble_many_equals_one_by_one():
+    """Systems assembled together from several host threads (one stream each) have the operators of
+    one-by-one assembly: the same skeletons and the same products."""
+    rng = np.random.default_rng(44)
+    pts = [rng.uniform(-1, 1, (3, n)) for n in (2500, 4096, 3100, 5000, 2000, 3600)]
+    one = []
+    for P in pts:
+        g = h2w.Initialisation(3, P.shape[1], 40, 1.0, 8, h2w.CHOICE_RBF, h2w.NESTED, points=P)
+        g.tree_initialisation()
+        one.append(g)
+    many = [h2w.Initialisation(3, P.shape[1], 40, 1.0, 8, h2w.CHOICE_RBF, h2w.NESTED, points=P) for P in pts]
+    h2w.assemble_many(many, 3)
+    for a, b in zip(one, many):
+        assert a.stats()["memory"] == b.stats()["memory"]
+        nl = a.stats()["nlevels"]
+        for fam, lev in ((0, nl), (1, 1)):
+            for box in range(0, 8 ** lev, 5):
+                assert a.skeleton(fam, lev, box) == b.skeleton(fam, lev, box)
+        x = np.random.default_rng(2).normal(size=a.N)
+        ya, yb = a.mat_vec_prod(x), b.mat_vec_prod(x)
+        assert np.linalg.norm(ya - yb) <= 1e-13 * np.linalg.norm(ya)
+    with pytest.raises(h2w.H2WError) as e:
+        h2w.assemble_many([many[0], many[0]], 2)
+    assert e.value.code == h2w.H2W_EINVAL
