@@ -308,6 +308,7 @@ struct h2w_handle {
     h2w::DevBuf<double> g_defl_rm;      // the same row-major (N x k each: U then C), for the lockstep solves
     int defl_k = 0, defl_prec = -1, defl_gen = -1;
     int defl_serial = 0;                // incremented whenever the space is rebuilt or dropped
+    int gv_defl = -1;                   // serial of the space whose U, C sit in g_U / g_V's first columns
     h2w::PinBuf g_pin;                // GMRES read-backs (Hessenberg columns)
     int64_t g_Vcols = 0;
     // kNR-wide lockstep GMRES (interleaved N x kNR vectors)

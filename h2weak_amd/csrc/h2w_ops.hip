@@ -2071,6 +2071,9 @@ struct GmresOut {
 // buffer, so that one fused pass orthogonalises against C and the new Arnoldi vectors.
 struct Recycled {
     int k = 0;
+    // C spans approximate eigenvectors (a deflation space): C^T A v is small for v orthogonal to C,
+    // so the second Gram-Schmidt pass is taken against the new Arnoldi vectors only
+    bool invariant = false;
 };
 
 void ensure_V(Handle &h, int64_t cols) {
@@ -2126,6 +2129,8 @@ GmresOut gmres_core(Handle &h, const double *b, double *x, int max_iter, double 
     // recycled space (x starts at zero): x = U C^T r, r -= C C^T r, and every Arnoldi vector is
     // orthogonalised against C as well (its coefficients B = C^T A v_j enter the update of x)
     const int kc = rc && x_zero ? rc->k : 0;
+    const bool once = kc > 0 && rc->invariant;
+    if (!once) h.gv_defl = -1;  // (its basis overwrites the first columns of g_V)
     const int m = max_iter;
     const int nb = mdot_blocks(N);
     h.g_part.ensure((size_t)nb * (kc + m + 2));
@@ -2168,9 +2173,12 @@ GmresOut gmres_core(Handle &h, const double *b, double *x, int max_iter, double 
         launch_mdot(C, N, k, w, N, h.g_part.p, nb, s);
         launch_mdot_reduce(h.g_part.p, nb, k, h.g_h.p, s);
         launch_mgemv_sub(C, N, k, h.g_h.p, w, N, s);
-        launch_mdot(C, N, k, w, N, h.g_part.p, nb, s);
-        launch_mdot_reduce(h.g_part.p, nb, k, h.g_h.p + k, s);
-        launch_mgemv_sub(C, N, k, h.g_h.p + k, w, N, s);
+        // second pass: against C as well, or (a deflation space) against v_0 .. v_i only
+        const int k0 = once ? kc : 0;
+        if (once) launch_fill(h.g_h.p + k, kc, 0.0, s);
+        launch_mdot(C + (int64_t)k0 * N, N, k - k0, w, N, h.g_part.p, nb, s);
+        launch_mdot_reduce(h.g_part.p, nb, k - k0, h.g_h.p + k + k0, s);
+        launch_mgemv_sub(C + (int64_t)k0 * N, N, k - k0, h.g_h.p + k + k0, w, N, s);
         launch_mdot(w, N, 1, w, N, h.g_part.p, nb, s);
         launch_mdot_reduce(h.g_part.p, nb, 1, h.g_h.p + 2 * k, s);
         H2W_HIP(hipMemcpyAsync(hh, h.g_h.p, (2 * k + 1) * 8, hipMemcpyDeviceToHost, s));
@@ -2227,6 +2235,7 @@ GmresOut gmres_core(Handle &h, const double *b, double *x, int max_iter, double 
 // to g_U.  Returns false (nothing kept) for fewer than two iterations or a near-singular R.
 bool keep_space(Handle &h, const GmresOut &o, Recycled &rc, const std::vector<std::vector<double>> *S = nullptr) {
     rc.k = 0;
+    h.gv_defl = -1;
     const int j = (int)o.H.size();
     if (j < 2 || o.iters != j) return false;
     const int kk = S ? (int)S->size() : j;
@@ -2502,6 +2511,7 @@ h2w_gmres_stats solve_tree_c(Handle &h, const double *bt, int max_iter, double t
     if ((int64_t)h.g_V.n < vcols * 2 * N) {
         h.g_V.alloc((size_t)vcols * 2 * N);
         h.g_Vcols = 0;  // (the real basis layout is rebuilt on its next use)
+        h.gv_defl = -1;
     }
     double2 *V = reinterpret_cast<double2 *>(h.g_V.p);
     double normb = norm2(b);
@@ -2624,12 +2634,17 @@ h2w_gmres_stats solve_tree(Handle &h, const double *bt, int max_iter, double tol
                           (inner_prec != H2W_FP32_INNER || h.defl_gen == h.inner_gen);
         if (defl) {
             const int k = h.defl_k;
-            ensure_V(h, k + 2);
+            ensure_V(h, k + 2);  // (a larger basis buffer keeps its columns)
+            const bool fresh = h.g_U.n < (size_t)k * N;  // (ensure reallocates: nothing kept)
             h.g_U.ensure((size_t)k * N);
-            H2W_HIP(hipMemcpyAsync(h.g_U.p, h.g_defl.p, (size_t)k * N * 8, hipMemcpyDeviceToDevice, s));
-            H2W_HIP(hipMemcpyAsync(h.g_V.p, h.g_defl.p + (int64_t)k * N, (size_t)k * N * 8,
-                                   hipMemcpyDeviceToDevice, s));
+            if (fresh || h.gv_defl != h.defl_serial) {  // (still there after an earlier deflated solve)
+                H2W_HIP(hipMemcpyAsync(h.g_U.p, h.g_defl.p, (size_t)k * N * 8, hipMemcpyDeviceToDevice, s));
+                H2W_HIP(hipMemcpyAsync(h.g_V.p, h.g_defl.p + (int64_t)k * N, (size_t)k * N * 8,
+                                       hipMemcpyDeviceToDevice, s));
+                h.gv_defl = h.defl_serial;
+            }
             rc.k = k;
+            rc.invariant = true;
         }
         while (true) {
             if (fp64_resid) {

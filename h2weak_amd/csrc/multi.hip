@@ -297,10 +297,12 @@ LockOut lock_gmres(Lockstep &L, const double *b, double *x, const std::vector<ch
         for (int pass = 0; pass < 2; ++pass) {
             launch_bdot16(Cb, len, k, w, L.blk, L.nblk, B.part.p, s);
             launch_bdot16_reduce(B.part.p, L.nblk, k, L.sblk, L.S, B.h.p + (size_t)pass * k * C, s);
-            if (ks > 0) {
+            if (ks > 0 && pass == 0) {
+                // (the shared space spans approximate eigenvectors: one pass against it, as the
+                // single-vector solve does)
                 launch_sdot16(L.dC, ks, w, L.blk, L.nblk, B.spart.p, s);
-                launch_bdot16_reduce(B.spart.p, L.nblk, ks, L.sblk, L.S, B.sh.p + (size_t)pass * ks * C, s);
-                launch_supd16(L.dC, ks, B.sh.p + (size_t)pass * ks * C, L.S, -1.0, w, L.n, L.seg, s);
+                launch_bdot16_reduce(B.spart.p, L.nblk, ks, L.sblk, L.S, B.sh.p, s);
+                launch_supd16(L.dC, ks, B.sh.p, L.S, -1.0, w, L.n, L.seg, s);
             }
             launch_bgemv16(Cb, len, k, B.h.p + (size_t)pass * k * C, L.S, -1.0, w, L.n, L.seg, s);
         }
@@ -308,7 +310,7 @@ LockOut lock_gmres(Lockstep &L, const double *b, double *x, const std::vector<ch
         launch_bdot16_reduce(B.part.p, L.nblk, 1, L.sblk, L.S, B.h.p + (size_t)2 * k * C, s);
         H2W_HIP(hipMemcpyAsync(hh, B.h.p, (size_t)(2 * k + 1) * C * 8, hipMemcpyDeviceToHost, s));
         double *shh = hh + (size_t)(2 * k + 1) * C;
-        if (ks > 0) H2W_HIP(hipMemcpyAsync(shh, B.sh.p, (size_t)2 * ks * C * 8, hipMemcpyDeviceToHost, s));
+        if (ks > 0) H2W_HIP(hipMemcpyAsync(shh, B.sh.p, (size_t)ks * C * 8, hipMemcpyDeviceToHost, s));
         H2W_HIP(hipStreamSynchronize(s));
         last = i;
         H.emplace_back((size_t)(i + 2) * C, 0.0);
@@ -318,8 +320,7 @@ LockOut lock_gmres(Lockstep &L, const double *b, double *x, const std::vector<ch
             tmp[c] = 0.0;
             if (!run[c]) continue;
             for (int q = 0; q < kc; ++q) Bst.back()[(size_t)q * C + c] = hh[(size_t)q * C + c] + hh[(size_t)(k + q) * C + c];
-            for (int q = 0; q < ks; ++q)
-                Bst.back()[(size_t)q * C + c] = shh[(size_t)q * C + c] + shh[(size_t)(ks + q) * C + c];
+            for (int q = 0; q < ks; ++q) Bst.back()[(size_t)q * C + c] = shh[(size_t)q * C + c];
             for (int q = 0; q <= i; ++q)
                 Hh(c, q, i) = hh[(size_t)(kc + q) * C + c] + hh[(size_t)(k + kc + q) * C + c];
             Hh(c, i + 1, i) = std::sqrt(hh[(size_t)2 * k * C + c]);
