@@ -406,6 +406,50 @@ def bench_configs(names=None, cpu=True):
     print(json.dumps({"configs": out}), flush=True)
 
 
+def bench_complex():
+    """The complex kernels (the reference's -DUSE_Helm / -DUSE_Hankel builds): 3D Helmholtz e^{iR}/(4 pi) on
+    the 32^3 grid and 2D Hankel (i/4) H0^(1) on the 256^2 grid, H2*, n_max 64, ACA tol 1e-8, FP64 complex GMRES
+    to 1e-8 from x0 = 1 with b = A x_true from the GPU's dense complex product (x_true complex N(0,1), seed 3):
+    assembly, the product's streamed bandwidth (16 bytes per stored complex element), GMRES time and error,
+    and 1024 sampled rows of the product against the dense rows."""
+    import h2weak_amd as h2w
+    out = {}
+    for name, dim, side, choice in (("helmholtz_3d_grid32", 3, 32, h2w.CHOICE_HELMHOLTZ),
+                                    ("hankel_2d_grid256", 2, 256, h2w.CHOICE_HANKEL)):
+        N = side ** dim
+        t0 = time.time()
+        A = h2w.Initialisation(dim, N, 64, 1.0, 8, choice, h2w.NESTED)
+        A.tree_initialisation()
+        t_asm = time.time() - t0
+        st = A.stats()
+        rng = np.random.default_rng(3)
+        xt = rng.normal(size=N) + 1j * rng.normal(size=N)
+        b = A.dense_rows(None, xt)
+        rows = np.sort(np.random.default_rng(4).choice(N, 1024, replace=False))
+        y = A.mat_vec_prod(xt)
+        row_err = float(np.linalg.norm(y[rows] - b[rows]) / np.linalg.norm(b[rows]))
+        r = A.bench_matvec(10, h2w.FP64)
+        streamed = sum(r["launch_bytes"]) + 2 * 16 * N
+        q = r["launch_role"].index(1)
+        A.gmres(b, None, 500, 1e-8, h2w.GMRES_FP64)  # warm-up
+        runs = [A.gmres(b, None, 500, 1e-8, h2w.GMRES_FP64) for _ in range(3)]
+        x, gs = runs[-1]
+        out[name] = {"N": N, "dim": dim, "choice": choice, "n_max": 64, "tol": 1e-8, "nlevels": st["nlevels"],
+                     "assembly_s": round(t_asm, 2), "memory": st["memory"],
+                     "stored_bytes_complex": st["stored_bytes_fp64"],
+                     "sampled_rows_rel_error": row_err,
+                     "matvec": {"ms": round(r["ms_per_matvec"], 4),
+                                "streamed_GBps": round(streamed / (r["ms_per_matvec"] * 1e-3) * 1e-9, 1),
+                                "m2l_near_launch_ms": round(r["launch_ms"][q], 4),
+                                "m2l_near_launch_GBps": round(r["launch_bytes"][q] / (r["launch_ms"][q] * 1e-3) * 1e-9, 1)},
+                     "gmres_fp64": {"s": float(np.median([g[1]["seconds"] for g in runs])),
+                                    "iterations": gs["iterations"], "resid": gs["resid"],
+                                    "rel_error": float(np.linalg.norm(x - xt) / np.linalg.norm(xt))}}
+        log(f"{name}: {out[name]}")
+        del A
+    print(json.dumps({"complex": out}), flush=True)
+
+
 # The reference's eight published runs (uniform grids, x0 = 1): parameters, and the printed iteration count,
 # storage (GB) and wall times of assembly and GMRES on its CPU (hardware unstated).
 TRANSCRIPTS = {
@@ -532,6 +576,7 @@ def main():
                     help="assemble the workload once (heuristic rank estimates) and print the time")
     ap.add_argument("--c5", action="store_true",
                     help="rank-predictor measurement (N = 262144 uniform and Gaussian, heuristic vs model)")
+    ap.add_argument("--complex", action="store_true", help="the complex kernels' measurement lines only")
     ap.add_argument("--configs", action="store_true",
                     help="the 1D / 2D benchmark configurations at full size, GPU and CPU oracle side by side")
     ap.add_argument("--transcripts", action="store_true",
@@ -560,6 +605,9 @@ def main():
         return
     if args.configs:
         bench_configs(cpu=not args.no_cpu)
+        return
+    if args.complex:
+        bench_complex()
         return
     if args.c5:
         bench_c5()

@@ -461,56 +461,84 @@ __global__ void k_cur_c(const CurJob *__restrict__ jobs, const int32_t *__restri
 // ------------------------------------------------------------------ complex batched GEMV
 // The real path's work lists over complex pools: item rows y (+)= sum_t A_t x_t; symmetric
 // terms (t >= 0) also add A_t^T x_item to vec[t ..] (plain transpose, atomics per real part).
-// One workgroup per item; thread = row for the direct product, a wave per column group for the
-// transposed one.  Role 2 items and atomic items (flags bit 0 or 2) add atomically.
+// One workgroup per item; lane = row (rows > 64: two passes), 16-byte loads of whole complex
+// entries.  Wave wv takes groups of kCU consecutive columns (wv, wv + 4, ...), all kCU columns'
+// loads issued before they are used.  The transposed sums of a group are reduced together: a
+// butterfly halves the kCU values per exchange (lane bit o keeps one half, sends the other), so
+// log2(64) exchanges serve kCU columns and lanes 0 .. kCU-1 end with one column sum each.  Role 2
+// items and atomic items (flags bit 0 or 2) add their rows atomically.
 __device__ __forceinline__ void catomic_add(double2 *p, double2 v) {
     double *d = reinterpret_cast<double *>(p);
     atomicAdd(d, v.x);
     atomicAdd(d + 1, v.y);
 }
+__device__ __forceinline__ double2 cshfl_xor(double2 v, int o) {
+    return cmk(__shfl_xor(v.x, o, 64), __shfl_xor(v.y, o, 64));
+}
+constexpr int kCU = 8;  // columns in flight per wave
 __global__ __launch_bounds__(256) void k_cgemv(const GItem *__restrict__ items, const GTerm *__restrict__ terms,
                                                const double2 *__restrict__ A, double2 *__restrict__ vec) {
     const GItem it = items[blockIdx.x];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     __shared__ double2 part[4][kRowTile];
     __shared__ double2 xb[kRowTile];
-    for (int r = threadIdx.x; r < it.rows; r += 256) xb[r] = vec[it.xb + r];
+    const int rows = it.rows;
+    // (zero beyond the item's rows: the transposed sums then need no row test)
+    for (int r = threadIdx.x; r < kRowTile; r += 256) xb[r] = r < rows ? vec[it.xb + r] : cmk(0.0, 0.0);
     __syncthreads();
-    // direct products: wave wv takes every fourth column of every term; lane = row (two passes)
-    double2 acc[2] = {cmk(0.0, 0.0), cmk(0.0, 0.0)};
+    const bool two = rows > 64;
+    const bool in0 = lane < rows, in1 = two && 64 + lane < rows;
+    const double2 xb0 = xb[lane], xb1 = xb[64 + lane];
+    double2 acc0 = cmk(0.0, 0.0), acc1 = cmk(0.0, 0.0);
     for (int tt = it.t0; tt < it.t1; ++tt) {
         const GTerm tm = terms[tt];
         const double2 *Ab = A + tm.a + it.row0;
-        for (int c = wv; c < tm.cols; c += 4) {
-            const double2 xv = vec[tm.x + c];
-            const double2 *col = Ab + (int64_t)c * it.lda;
+        for (int c0 = kCU * wv; c0 < tm.cols; c0 += 4 * kCU) {
+            double2 v0[kCU], v1[kCU], xv[kCU];
 #pragma unroll
-            for (int h = 0; h < 2; ++h) {
-                const int r = h * 64 + lane;
-                if (r < it.rows) acc[h] = cadd(acc[h], cmul(col[r], xv));
+            for (int u = 0; u < kCU; ++u) {
+                const int c = c0 + u;
+                const bool ok = c < tm.cols;
+                xv[u] = ok ? vec[tm.x + c] : cmk(0.0, 0.0);
+                const double2 *col = Ab + (int64_t)(ok ? c : c0) * it.lda;
+                v0[u] = ok && in0 ? col[lane] : cmk(0.0, 0.0);
+                v1[u] = ok && in1 ? col[64 + lane] : cmk(0.0, 0.0);
+            }
+#pragma unroll
+            for (int u = 0; u < kCU; ++u) {
+                acc0 = cadd(acc0, cmul(v0[u], xv[u]));
+                if (two) acc1 = cadd(acc1, cmul(v1[u], xv[u]));
             }
             if (tm.t >= 0) {
-                // transposed product: column c of A against this item's rows of x
-                double2 d = cmk(0.0, 0.0);
+                double2 d[kCU];
 #pragma unroll
-                for (int h = 0; h < 2; ++h) {
-                    const int r = h * 64 + lane;
-                    if (r < it.rows) d = cadd(d, cmul(col[r], xb[r]));
+                for (int u = 0; u < kCU; ++u) d[u] = cadd(cmul(v0[u], xb0), cmul(v1[u], xb1));
+                int ui = 0;
+#pragma unroll
+                for (int k = kCU, o = 1; k > 1; k >>= 1, o <<= 1) {
+                    const bool hi = lane & o;
+#pragma unroll
+                    for (int i = 0; i < k / 2; ++i) {
+                        const double2 snd = hi ? d[i] : d[i + k / 2];
+                        const double2 keep = hi ? d[i + k / 2] : d[i];
+                        d[i] = cadd(keep, cshfl_xor(snd, o));
+                    }
+                    ui += hi ? k / 2 : 0;
                 }
-                d.x = wsum(d.x);
-                d.y = wsum(d.y);
-                if (lane == 0) catomic_add(vec + tm.t + c, d);
+#pragma unroll
+                for (int o = kCU; o < 64; o <<= 1) d[0] = cadd(d[0], cshfl_xor(d[0], o));
+                if (lane < kCU && c0 + ui < tm.cols) catomic_add(vec + tm.t + c0 + ui, d[0]);
             }
         }
     }
-    part[wv][lane] = acc[0];
-    part[wv][64 + lane] = acc[1];
+    part[wv][lane] = acc0;
+    part[wv][64 + lane] = acc1;
     __syncthreads();
-    for (int r = threadIdx.x; r < it.rows; r += 256) {
-        const double2 s = cadd(cadd(part[0][r], part[1][r]), cadd(part[2][r], part[3][r]));
+    for (int r = threadIdx.x; r < rows; r += 256) {
+        const double2 sum = cadd(cadd(part[0][r], part[1][r]), cadd(part[2][r], part[3][r]));
         double2 *y = vec + it.y + r;
-        if (it.flags & 5) catomic_add(y, s);
-        else *y = cadd(*y, s);
+        if (it.flags & 5) catomic_add(y, sum);
+        else *y = cadd(*y, sum);
     }
 }
 
