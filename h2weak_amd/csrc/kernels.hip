@@ -2217,6 +2217,7 @@ __global__ __launch_bounds__(256) void k_bdot16_reduce(const double *__restrict_
     const int j = blockIdx.x, sy = blockIdx.y, S = gridDim.y;
     const int r = threadIdx.x & 15, q = threadIdx.x >> 4;
     double acc = 0.0;
+#pragma unroll 8
     for (int b = sblk[sy] + q; b < sblk[sy + 1]; b += 16) acc += partial[((int64_t)j * nblk + b) * kNR + r];
     acc += __shfl_xor(acc, 16, 64);
     acc += __shfl_xor(acc, 32, 64);
@@ -2238,50 +2239,134 @@ __global__ __launch_bounds__(256) void k_bgemv16(const double *__restrict__ V, i
 }
 // A space shared by every column of a one-system lockstep vector (the deflation space): k vectors
 // of n slots, stored row-major (Cs[t * k + q]), so that the k values of a slot are contiguous.
-// partial[(q * nblk + b) * kNR + r] = sum over block b of Cs(t, q) w(t, r), q = 16 z .. 16 z + 15:
-// one pass over w serves 16 of the space's vectors (the reduction is k_bdot16_reduce's).
+// Both products with it are tall-skinny GEMMs and run on the FP64 matrix cores (16x16x4 MFMA:
+// lane l supplies A(l % 16, l / 16) and B(l / 16, l % 16) and holds D(l / 16 + 4 v, l % 16), v < 4).
+//
+// partial[(q * nblk + b) * kNR + r] = sum over block b's slots of Cs(t, q) w(t, r), for the
+// kSdTiles 16-vector tiles q = qg .. qg + 16 kSdTiles - 1 of this workgroup (grid.y covers k):
+// D(16 q x 16 r) += A(16 q x 4 t) B(4 t x 16 r) with A(m, j) = Cs(t0 + j, q0 + m), B(j, n) = w(t0 + j, n);
+// the four waves take every fourth group of 4 slots and are summed through LDS (the reduction over
+// blocks is k_bdot16_reduce's).
+constexpr int kSdTiles = 8;
 __global__ __launch_bounds__(256) void k_sdot16(const double *__restrict__ Cs, int k, const double *__restrict__ w,
                                                 const int2 *__restrict__ blk, double *__restrict__ partial) {
-    const int b = blockIdx.x, z = blockIdx.y, nblk = gridDim.x;
-    const int q0 = 16 * z, nq = min(16, k - q0);
+    typedef double v4 __attribute__((ext_vector_type(4)));
+    const int b = blockIdx.x, nblk = gridDim.x;
+    const int qg = blockIdx.y * 16 * kSdTiles;
+    const int nz = min(kSdTiles, (k - qg + 15) / 16);
     const int2 rg = blk[b];
-    const int r = threadIdx.x & 15;
-    double acc[16];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int m = lane & 15, j = lane >> 4;
+    v4 acc[kSdTiles];
 #pragma unroll
-    for (int j = 0; j < 16; ++j) acc[j] = 0.0;
-    for (int64_t e = (int64_t)rg.x * kNR + threadIdx.x; e < (int64_t)rg.y * kNR; e += 256) {
-        const double we = w[e];
-        const double *c = Cs + (e / kNR) * k + q0;
+    for (int z = 0; z < kSdTiles; ++z) acc[z] = (v4)0.0;
+#pragma unroll 2
+    for (int t0 = rg.x + 4 * wv; t0 < rg.y; t0 += 16) {
+        const int t = t0 + j;
+        const bool ok = t < rg.y;
+        const double bv = ok ? w[(int64_t)t * kNR + m] : 0.0;
+        const double *c = Cs + (int64_t)t * k + qg + m;
+        double av[kSdTiles];
 #pragma unroll
-        for (int j = 0; j < 16; ++j)
-            if (j < nq) acc[j] = fma(c[j], we, acc[j]);
+        for (int z = 0; z < kSdTiles; ++z) av[z] = (z < nz && ok && qg + 16 * z + m < k) ? c[16 * z] : 0.0;
+#pragma unroll
+        for (int z = 0; z < kSdTiles; ++z)
+            if (z < nz) acc[z] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[z], bv, acc[z], 0, 0, 0);
     }
-    __shared__ double sh[4][16][kNR];
+    // waves 2, 3 -> waves 0, 1 -> wave 0
+    __shared__ double sh[2][kSdTiles][4][64];
+    if (wv >= 2)
 #pragma unroll
-    for (int j = 0; j < 16; ++j) {
-        acc[j] += __shfl_xor(acc[j], 16, 64);
-        acc[j] += __shfl_xor(acc[j], 32, 64);
-        if ((threadIdx.x & 63) < 16) sh[threadIdx.x >> 6][j][r] = acc[j];
-    }
+        for (int z = 0; z < kSdTiles; ++z)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) sh[wv - 2][z][v][lane] = acc[z][v];
     __syncthreads();
-    if (threadIdx.x < 16 * kNR) {
-        const int j = threadIdx.x / kNR, rr = threadIdx.x % kNR;
-        if (j < nq)
-            partial[((int64_t)(q0 + j) * nblk + b) * kNR + rr] = sh[0][j][rr] + sh[1][j][rr] + sh[2][j][rr] + sh[3][j][rr];
-    }
+    if (wv < 2)
+#pragma unroll
+        for (int z = 0; z < kSdTiles; ++z)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) acc[z][v] += sh[wv][z][v][lane];
+    __syncthreads();
+    if (wv == 1)
+#pragma unroll
+        for (int z = 0; z < kSdTiles; ++z)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) sh[0][z][v][lane] = acc[z][v];
+    __syncthreads();
+    if (wv == 0)
+#pragma unroll
+        for (int z = 0; z < kSdTiles; ++z)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int q = qg + 16 * z + j + 4 * v;
+                if (z < nz && q < k) partial[((int64_t)q * nblk + b) * kNR + m] = acc[z][v] + sh[0][z][v][lane];
+            }
 }
 // w(t, r) += sign * sum_q Cs(t, q) h(q, seg(t), r)   (the shared space, row-major as above; with several
-// systems every system's slots carry its own space, h holds the coefficients of every system)
+// systems every system's slots carry its own space, h holds the coefficients of every system).
+// A wave takes 64 slots, four tiles of 16: D_i(16 t x 16 r) += A_i(16 t x 4 q) B(4 q x 16 r) with
+// A_i(m, j) = Cs(t0 + 16 i + m, q), B(j, n) = h(q, s, n), q = q0 + j, one B load serving the four tiles;
+// 16 vectors per trip, their 20 loads issued before the 16 MFMAs.  Slots of several systems in one
+// group of 64 are taken once per system with the other systems' rows zeroed.
 __global__ __launch_bounds__(256) void k_supd16(const double *__restrict__ Cs, int k, const double *__restrict__ h,
                                                 int S, double sign, double *__restrict__ w, int64_t n,
                                                 const int32_t *__restrict__ seg) {
-    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n * kNR; e += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t t = e / kNR;
-        const double *c = Cs + t * k;
-        const int col = (seg ? seg[t] * kNR : 0) + (int)(e % kNR);
-        double acc = 0.0;
-        for (int q = 0; q < k; ++q) acc = fma(c[q], h[(int64_t)q * S * kNR + col], acc);
-        w[e] += sign * acc;
+    typedef double v4 __attribute__((ext_vector_type(4)));
+    const int lane = threadIdx.x & 63;
+    const int m = lane & 15, j = lane >> 4;
+    const int64_t ngrp = (n + 63) / 64;
+    for (int64_t g = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6); g < ngrp; g += (int64_t)gridDim.x * 4) {
+        const int64_t t0 = g * 64;
+        const int s_lo = seg ? seg[t0] : 0;
+        const int s_hi = seg ? seg[min(t0 + 63, n - 1)] : 0;
+        const double *c[4];
+        bool rok[4];
+        int s_row[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int64_t ta = t0 + 16 * i + m;  // this lane's A row of tile i
+            rok[i] = ta < n;
+            s_row[i] = seg && rok[i] ? seg[ta] : 0;
+            c[i] = Cs + (rok[i] ? ta : 0) * k;
+        }
+        v4 acc[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) acc[i] = (v4)0.0;
+        for (int sy = s_lo; sy <= s_hi; ++sy) {
+            const double *hq = h + (int64_t)sy * kNR + m;
+            const int64_t hs = (int64_t)S * kNR;
+            bool mine[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) mine[i] = rok[i] && s_row[i] == sy;
+            for (int q0 = 0; q0 < k; q0 += 16) {
+                double av[4][4], bv[4];
+                // (a tile's four loads read the same 16 lines: issued one after another)
+#pragma unroll
+                for (int i = 0; i < 4; ++i)
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        const int q = q0 + 4 * u + j;
+                        av[i][u] = (mine[i] && q < k) ? c[i][q] : 0.0;
+                    }
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int q = q0 + 4 * u + j;
+                    bv[u] = q < k ? hq[q * hs] : 0.0;
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+#pragma unroll
+                    for (int i = 0; i < 4; ++i)
+                        acc[i] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[i][u], bv[u], acc[i], 0, 0, 0);
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int64_t t = t0 + 16 * i + j + 4 * v;
+                if (t < n) w[t * kNR + m] += sign * acc[i][v];
+            }
     }
 }
 // dst[t * k + q] = src[q * n + t]: k column-major vectors of n entries to row-major
@@ -2605,12 +2690,15 @@ void launch_bgemv16(const double *V, int64_t ldv, int k, const double *h, int S,
                            seg);
 }
 void launch_sdot16(const double *Cs, int k, const double *w, const int2 *blk, int nblk, double *partial, hipStream_t s) {
-    if (k > 0) hipLaunchKernelGGL(k_sdot16, dim3(nblk, (k + 15) / 16), dim3(256), 0, s, Cs, k, w, blk, partial);
+    if (k > 0)
+        hipLaunchKernelGGL(k_sdot16, dim3(nblk, (k + 16 * kSdTiles - 1) / (16 * kSdTiles)), dim3(256), 0, s, Cs, k, w, blk,
+                           partial);
 }
 void launch_supd16(const double *Cs, int k, const double *h, int S, double sign, double *w, int64_t n,
                    const int32_t *seg, hipStream_t s) {
     if (k > 0)
-        hipLaunchKernelGGL(k_supd16, dim3(grid_for(n * kNR, 256, 4096)), dim3(256), 0, s, Cs, k, h, S, sign, w, n, seg);
+        hipLaunchKernelGGL(k_supd16, dim3((unsigned)std::min<int64_t>((n + 255) / 256, 8192)), dim3(256), 0, s, Cs, k, h, S,
+                           sign, w, n, seg);
 }
 void launch_to_rowmajor(const double *src, int64_t n, int k, double *dst, hipStream_t s) {
     if (k > 0) hipLaunchKernelGGL(k_to_rowmajor, dim3(grid_for(n * k, 256, 4096)), dim3(256), 0, s, src, n, k, dst);
