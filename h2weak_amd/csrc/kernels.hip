@@ -997,6 +997,13 @@ template <typename V>
 __device__ __forceinline__ V buf_ld16_nt(__amdgpu_buffer_rsrc_t r, uint32_t off) {
     return __builtin_bit_cast(V, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 2));
 }
+// one 4-, 8- or 16-byte vector, streamed (not kept in the caches)
+template <typename V>
+__device__ __forceinline__ V buf_ldv_nt(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+    if constexpr (sizeof(V) == 16) return __builtin_bit_cast(V, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 2));
+    else if constexpr (sizeof(V) == 8) return __builtin_bit_cast(V, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 2));
+    else return __builtin_bit_cast(V, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 2));
+}
 // N consecutive elements (N sizeof(T) = 8, 16, 32 or 64 bytes) in 8- or 16-byte loads
 template <typename T, int N>
 __device__ __forceinline__ void buf_ld_n(T (&out)[N], __amdgpu_buffer_rsrc_t r, uint32_t off) {
@@ -1391,12 +1398,11 @@ __global__ __launch_bounds__(256) void k_bgemm(const GItem *__restrict__ items, 
 // FP64: 64 MFMA flops per stored element (direct and transposed, 16 right-hand sides) against
 // 8 bytes, so the launch needs both HBM and the FP64 matrix cores (78.6 TFLOP/s) at once;
 // launch bounds cap it at 256 registers a lane (two waves per SIMD).
-template <typename T>
-__global__ __launch_bounds__(256, 2) void k_bgemm_sym(const GItem *__restrict__ items, const GTerm *__restrict__ terms,
-                                                      const T *__restrict__ A0, const T *__restrict__ A1,
-                                                      const T *const *__restrict__ tab, T *vec) {
+template <typename T, int VEC>
+__device__ __forceinline__ void bgemm_sym_item(const GItem it, const GTerm *__restrict__ terms,
+                                               const T *__restrict__ A0, const T *__restrict__ A1,
+                                               const T *const *__restrict__ tab, T *vec, T *smem) {
     typedef typename Mfma16<T>::v4 V4;
-    constexpr int VEC = 16 / sizeof(T);
     constexpr int RG = 16 * VEC;           // rows of a slab
     constexpr int NG = kRowTile / RG;      // row groups of a full item
     constexpr int NA = NG * VEC;           // direct accumulators
@@ -1405,12 +1411,8 @@ __global__ __launch_bounds__(256, 2) void k_bgemm_sym(const GItem *__restrict__ 
     constexpr int LDR = RG + VEC;          // LDS column of a slab, padded by 16 bytes
     constexpr int XS = kRowTile * kNR;     // staged input rows
     constexpr int SLB = 16 * LDR;          // one slab buffer
-    constexpr int RED = 2 * kRowTile * kNR;
-    constexpr int SM = XS + 8 * SLB > RED ? XS + 8 * SLB : RED;
     typedef T VL __attribute__((ext_vector_type(VEC)));
-    __shared__ __attribute__((aligned(16))) T smem[SM];
     T *const xs = smem;                    // [kRowTile][kNR]
-    const GItem it = items[blockIdx.x];
     const T *__restrict__ A = tab ? tab[2 * (it.flags >> 8) + ((it.flags >> 1) & 1)] : ((it.flags & 2) ? A1 : A0);
     const CGTerm *const tms = (const CGTerm *)terms;
     const int lane = threadIdx.x & 63;
@@ -1470,7 +1472,7 @@ __global__ __launch_bounds__(256, 2) void k_bgemm_sym(const GItem *__restrict__ 
             const bool ok = S.has && c < tm.cols;
             // (the group's X_p rows only with its first row group; later slabs keep them)
             S.bx[j] = buf_ld<T>(rx, opaque_off(ok && rg == 0 ? (uint32_t)((c * kNR + m) * sizeof(T)) : kOutOfRange));
-            S.av[j] = buf_ld16_nt<VL>(
+            S.av[j] = buf_ldv_nt<VL>(
                 ra, opaque_off(ok && ro != kOutOfRange ? (uint32_t)(c * lda * sizeof(T)) + ro : kOutOfRange));
         }
         if (S.has) {
@@ -1601,6 +1603,27 @@ __global__ __launch_bounds__(256, 2) void k_bgemm_sym(const GItem *__restrict__ 
                         if (r < rows) atomicAdd(y + (int64_t)r * kNR, acc[g * VEC + e][v] + red[0][r][m]);
                     }
     }
+}
+template <typename T, int VEC>
+constexpr int sym_smem() {
+    constexpr int RG = 16 * VEC, SLB = 16 * (RG + VEC), XS = kRowTile * kNR, RED = 2 * kRowTile * kNR;
+    return XS + 8 * SLB > RED ? XS + 8 * SLB : RED;
+}
+// One item per workgroup.  An item that fits in one narrower row group (16 VEC rows: 8- or 4-byte
+// loads per lane) runs in it: a box of 20 rows takes one 32-row group where a 64-row group would
+// leave two thirds of every MFMA on zero rows.  Taller items keep the 16-byte loads (measured: more
+// row groups of narrower loads are slower even when they pad less).
+template <typename T>
+__global__ __launch_bounds__(256, 2) void k_bgemm_sym(const GItem *__restrict__ items, const GTerm *__restrict__ terms,
+                                                      const T *__restrict__ A0, const T *__restrict__ A1,
+                                                      const T *const *__restrict__ tab, T *vec) {
+    constexpr int VMAX = 16 / sizeof(T);
+    __shared__ __attribute__((aligned(16))) T smem[sym_smem<T, VMAX>()];
+    const GItem it = items[blockIdx.x];
+    const int best = it.rows <= 16 ? 1 : it.rows <= 32 ? 2 : VMAX;
+    if (best == VMAX) bgemm_sym_item<T, VMAX>(it, terms, A0, A1, tab, vec, smem);
+    else if (best == 1) bgemm_sym_item<T, 1>(it, terms, A0, A1, tab, vec, smem);
+    else if constexpr (VMAX == 4) bgemm_sym_item<T, 2>(it, terms, A0, A1, tab, vec, smem);
 }
 
 // Transposed products of the symmetric blocks on the 16-wide path:
