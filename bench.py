@@ -252,11 +252,12 @@ def kernel_table(path, top=25):
         print(f"| {tot * 1e-6:.1f} | {calls} | {avg * 1e-3:.1f} | {pct:.1f} | `{name}` |")
 
 
-def bench_batch(S=32, tol_pow=8, seed=5, defl=(128, 48)):
+def bench_batch(S=32, tol_pow=8, seed=5, defl=((128, 48), (128, 64)), tune_inner=(6.0, 5.5, 5.0, 4.5)):
     """32 independent 3D H2* RBF systems, N log-uniform in [4096, 65536] (seed 5), points U([-1,1]^3),
     n_max 64, ACA and GMRES tolerance 1e-8, 16 right-hand sides each (X_true ~ N(0,1), B = A X_true):
     batched products (one launch per phase for all systems, 16 right-hand sides per pass) and lockstep
-    solves of all 512 columns, against plain FP64 solves one right-hand side at a time."""
+    solves of all 512 columns, against plain FP64 solves one right-hand side at a time.  The refinement's
+    inner tolerance and deflation spaces are chosen on held-out right-hand sides before the timed solves."""
     import h2weak_amd as h2w
     rng = np.random.default_rng(seed)
     Ns = [int(v) for v in np.rint(np.exp(rng.uniform(np.log(4096), np.log(65536), S)))]
@@ -298,11 +299,37 @@ def bench_batch(S=32, tol_pow=8, seed=5, defl=(128, 48)):
     solves("fp32_ir", h2w.GMRES_FP32_IR)
     if defl:
         solves("fp32_inner_ir_no_deflation", h2w.GMRES_FP32_INNER_IR)
-        # every system's deflation space for its inner operator (operator only, outside the timed solves)
+        # the inner tolerance and every system's deflation space for its inner operator, chosen by timing
+        # lockstep solves of held-out right-hand sides (X ~ N(0,1), seeds 300 + q), never the timed ones
         t0 = time.time()
+        Bh = B.matvec([np.random.default_rng(300 + q).normal(size=(16, g.N)) for q, g in enumerate(systems)])
+        cands, best = [], None
+        for itol in tune_inner:
+            for g in systems:
+                g.set_inner_tolerance(itol)
+            for d in defl:
+                for g in systems:
+                    g.set_deflation(*d)
+                Bt = h2w.Batch(systems)
+                Bt.gmres([b[:1] for b in Bh], None, 300, gtol, h2w.GMRES_FP32_INNER_IR)  # warm-up
+                _, st = Bt.gmres(Bh, None, 300, gtol, h2w.GMRES_FP32_INNER_IR)
+                cols = [c for row in st for c in row]
+                c = {"inner_tol_exp": itol, "m": d[0], "k": d[1], "s_512": round(cols[0]["seconds"], 4),
+                     "max_iterations": max(c["iterations"] for c in cols), "converged": all(c["converged"] for c in cols)}
+                cands.append(c)
+                if c["converged"] and (best is None or c["s_512"] < best["s_512"]):
+                    best = c
+                del Bt
         for g in systems:
-            g.set_deflation(*defl)
-        out["deflation"] = {"m": defl[0], "k": defl[1], "build_s_all_systems": round(time.time() - t0, 3)}
+            g.set_inner_tolerance(best["inner_tol_exp"])
+            g.set_deflation(best["m"], best["k"])
+        B = h2w.Batch(systems)
+        out["deflation"] = {"m": best["m"], "k": best["k"], "inner_tol_exp": best["inner_tol_exp"],
+                            "tune_s": round(time.time() - t0, 2), "tuned_on": "held-out right-hand sides (seeds 300 + q)",
+                            "candidates": cands}
+        log(f"batch: inner 10^-{best['inner_tol_exp']}, spaces ({best['m']}, {best['k']}) chosen in "
+            f"{out['deflation']['tune_s']} s from " + ", ".join(f"10^-{c['inner_tol_exp']} ({c['m']},{c['k']}): "
+                                                                f"{c['s_512'] * 1e3:.1f} ms" for c in cands))
     solves("fp32_inner_ir", h2w.GMRES_FP32_INNER_IR)
     # plain FP64, one right-hand side at a time: every 4th system's first column, scaled to 512 solves
     ts = []
