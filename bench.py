@@ -303,31 +303,13 @@ def bench_batch(S=32, tol_pow=8, seed=5, defl=((128, 48), (128, 64)), tune_inner
         # lockstep solves of held-out right-hand sides (X ~ N(0,1), seeds 300 + q), never the timed ones
         t0 = time.time()
         Bh = B.matvec([np.random.default_rng(300 + q).normal(size=(16, g.N)) for q, g in enumerate(systems)])
-        cands, best = [], None
-        for itol in tune_inner:
-            for g in systems:
-                g.set_inner_tolerance(itol)
-            for d in defl:
-                for g in systems:
-                    g.set_deflation(*d)
-                Bt = h2w.Batch(systems)
-                Bt.gmres([b[:1] for b in Bh], None, 300, gtol, h2w.GMRES_FP32_INNER_IR)  # warm-up
-                _, st = Bt.gmres(Bh, None, 300, gtol, h2w.GMRES_FP32_INNER_IR)
-                cols = [c for row in st for c in row]
-                c = {"inner_tol_exp": itol, "m": d[0], "k": d[1], "s_512": round(cols[0]["seconds"], 4),
-                     "max_iterations": max(c["iterations"] for c in cols), "converged": all(c["converged"] for c in cols)}
-                cands.append(c)
-                if c["converged"] and (best is None or c["s_512"] < best["s_512"]):
-                    best = c
-                del Bt
-        for g in systems:
-            g.set_inner_tolerance(best["inner_tol_exp"])
-            g.set_deflation(best["m"], best["k"])
-        B = h2w.Batch(systems)
-        out["deflation"] = {"m": best["m"], "k": best["k"], "inner_tol_exp": best["inner_tol_exp"],
+        tune = B.autotune(Bh, gtol, 300, tune_inner, defl)
+        cands = [{"inner_tol_exp": c["inner_tol_exp"], "m": c["m"], "k": c["k"], "s_512": round(c["seconds"], 4),
+                  "max_iterations": c["max_iterations"], "converged": c["converged"]} for c in tune["candidates"]]
+        out["deflation"] = {"m": tune["m"], "k": tune["k"], "inner_tol_exp": tune["inner_tol_exp"],
                             "tune_s": round(time.time() - t0, 2), "tuned_on": "held-out right-hand sides (seeds 300 + q)",
                             "candidates": cands}
-        log(f"batch: inner 10^-{best['inner_tol_exp']}, spaces ({best['m']}, {best['k']}) chosen in "
+        log(f"batch: inner 10^-{tune['inner_tol_exp']}, spaces ({tune['m']}, {tune['k']}) chosen in "
             f"{out['deflation']['tune_s']} s from " + ", ".join(f"10^-{c['inner_tol_exp']} ({c['m']},{c['k']}): "
                                                                 f"{c['s_512'] * 1e3:.1f} ms" for c in cands))
     solves("fp32_inner_ir", h2w.GMRES_FP32_INNER_IR)

@@ -457,6 +457,46 @@ class Batch:
         stats = [[st[s * nrhs + r].as_dict() for r in range(nrhs)] for s in range(len(self.systems))]
         return self._unpack(X, nrhs), stats
 
+    def _recreate(self):
+        """A new native batch over the same systems (after their inner operators were rebuilt)."""
+        lib().h2w_batch_destroy(self._h)
+        self._h = None
+        arr = (ctypes.c_void_p * len(self.systems))(*[s._h for s in self.systems])
+        h = ctypes.c_void_p()
+        _check(lib().h2w_batch_create(arr, len(self.systems), ctypes.byref(h)))
+        self._h = h
+
+    def autotune(self, Bs, tol=1e-8, max_iter=300, inner=(6.0, 5.5, 5.0, 4.5), spaces=((128, 48), (128, 64))):
+        """Choose the refinement's inner tolerance (10^-inner, the same for every system) and every system's
+        deflation space (m Arnoldi steps, k Ritz vectors) by timing lockstep FP32-inner refined solves of Bs,
+        right-hand sides that should not be the ones timed afterwards; keeps the fastest candidate whose
+        columns all converged.  Returns the choice and every candidate's time and step count."""
+        cands, best = [], None
+        for itol in inner:
+            for g in self.systems:
+                g.set_inner_tolerance(itol)
+            self._recreate()
+            for m, k in spaces:
+                for g in self.systems:
+                    g.set_deflation(m, k)
+                self.gmres([b[:1] for b in Bs], None, max_iter, tol, GMRES_FP32_INNER_IR)  # warm-up
+                _, st = self.gmres(Bs, None, max_iter, tol, GMRES_FP32_INNER_IR)
+                cols = [c for row in st for c in row]
+                c = {"inner_tol_exp": float(itol), "m": int(m), "k": int(k), "seconds": cols[0]["seconds"],
+                     "max_iterations": max(c["iterations"] for c in cols),
+                     "converged": all(c["converged"] for c in cols)}
+                cands.append(c)
+                if c["converged"] and (best is None or c["seconds"] < best["seconds"]):
+                    best = c
+        if best is None:
+            raise H2WError(H2W_ENOCONV, "no candidate converged")
+        for g in self.systems:
+            g.set_inner_tolerance(best["inner_tol_exp"])
+        self._recreate()
+        for g in self.systems:
+            g.set_deflation(best["m"], best["k"])
+        return {"inner_tol_exp": best["inner_tol_exp"], "m": best["m"], "k": best["k"], "candidates": cands}
+
     def bench_matvec(self, nrep=10, precision=FP64):
         r = BenchResult()
         _check(lib().h2w_batch_bench_matvec(self._h, int(nrep), precision, ctypes.byref(r)))

@@ -206,3 +206,32 @@ def test_batched_solves_use_each_systems_space():
     _, st2 = bt.gmres(Bs, None, 300, 1e-8, h2w.GMRES_FP32_INNER_IR)
     assert all(c["converged"] for row in st2 for c in row)
     assert abs(max(c["iterations"] for row in st2 for c in row) - its0) <= 1
+
+
+def test_batch_autotune_on_held_out_right_hand_sides():
+    """Batch.autotune times lockstep refined solves of held-out right-hand sides over inner tolerances and
+    deflation spaces, keeps the fastest converged candidate (applied to every system) and rebuilds the native
+    batch: later solves of other right-hand sides converge to the single FP64 solve's accuracy."""
+    rng = np.random.default_rng(23)
+    systems = []
+    for n in (2600, 4700, 3900):
+        P = rng.uniform(-1, 1, (3, n))
+        g = h2w.Initialisation(3, n, 64, 1.0, 8, h2w.CHOICE_RBF, h2w.NESTED, points=P)
+        g.tree_initialisation()
+        systems.append(g)
+    bt = h2w.Batch(systems)
+    Bh = bt.matvec([np.random.default_rng(40 + q).normal(size=(16, g.N)) for q, g in enumerate(systems)])
+    t = bt.autotune(Bh, 1e-8, 300, inner=(6.0, 5.0), spaces=((32, 8), (64, 16)))
+    conv = [c for c in t["candidates"] if c["converged"]]
+    assert len(t["candidates"]) == 4 and conv
+    best = min(conv, key=lambda c: c["seconds"])
+    assert (t["inner_tol_exp"], t["m"], t["k"]) == (best["inner_tol_exp"], best["m"], best["k"])
+    assert all(g.stats()["inner_tol_exp"] == t["inner_tol_exp"] for g in systems)
+    XT = [np.random.default_rng(50 + q).normal(size=(16, g.N)) for q, g in enumerate(systems)]
+    Bs = [np.stack([g.dense_rows(None, x) for x in X]) for g, X in zip(systems, XT)]
+    Xs, st = bt.gmres(Bs, None, 300, 1e-8, h2w.GMRES_FP32_INNER_IR)
+    for q, g in enumerate(systems):
+        for c in range(16):
+            assert st[q][c]["converged"], (q, c, st[q][c])
+            e1 = _rel(g.gmres(Bs[q][c], None, 300, 1e-8, h2w.GMRES_FP64)[0], XT[q][c])
+            assert _rel(Xs[q][c], XT[q][c]) <= 10 * e1 + 1e-12
