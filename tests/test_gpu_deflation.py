@@ -235,3 +235,22 @@ def test_batch_autotune_on_held_out_right_hand_sides():
             assert st[q][c]["converged"], (q, c, st[q][c])
             e1 = _rel(g.gmres(Bs[q][c], None, 300, 1e-8, h2w.GMRES_FP64)[0], XT[q][c])
             assert _rel(Xs[q][c], XT[q][c]) <= 10 * e1 + 1e-12
+
+
+@pytest.mark.parametrize("m,k", [(64, 24), (300, 200)])
+def test_lockstep_columns_take_the_single_solves_steps(sys3d, m, k):
+    """The lockstep solves project on the same space as the single solve (its row-major copy, on the matrix
+    cores; k = 200 spans two workgroup tiles of the dot-product kernel): every column takes the single solve's
+    inner steps on the same right-hand side, up to one step of summation-order noise."""
+    g, xt, b = sys3d
+    g.set_deflation(m, k)
+    rng = np.random.default_rng(31)
+    XT = rng.normal(size=(5, N))
+    B = np.stack([g.dense_rows(None, x) for x in XT])
+    X, st = g.gmres_multi(B, None, 500, TOL, h2w.GMRES_FP32_INNER_IR)
+    for c in range(5):
+        x1, s1 = g.gmres(B[c], None, 500, TOL, h2w.GMRES_FP32_INNER_IR)
+        assert st[c]["converged"] and abs(st[c]["iterations"] - s1["iterations"]) <= 1, (c, st[c], s1)
+        assert st[c]["outer_iterations"] == s1["outer_iterations"]
+        assert _rel(X[c], x1) <= 1e-9
+    g.set_deflation(0)
