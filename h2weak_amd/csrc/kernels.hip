@@ -2719,7 +2719,7 @@ void launch_sdot16(const double *Cs, int k, const double *w, const int2 *blk, in
 }
 void launch_supd16(const double *Cs, int k, const double *h, int S, double sign, double *w, int64_t n,
                    const int32_t *seg, hipStream_t s) {
-    if (k > 0)
+    if (k > 0 && n > 0)
         hipLaunchKernelGGL(k_supd16, dim3((unsigned)std::min<int64_t>((n + 255) / 256, 8192)), dim3(256), 0, s, Cs, k, h, S,
                            sign, w, n, seg);
 }
