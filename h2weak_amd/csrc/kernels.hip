@@ -396,6 +396,7 @@ __global__ __launch_bounds__(NT) void k_aca(const AcaJob *__restrict__ jobs, con
 // in WideState; a finished block (converged, exhausted, or out of workspace:
 // status 1) is skipped by every later launch.
 constexpr int kWideCols = 2048;  // row / column entries one workgroup of the wide passes covers
+constexpr int kWidePT = kWideCols / 256;  // of them per thread
 
 __device__ __forceinline__ void wide_reserve(WideState &st, const AcaJob &jb, int64_t *tab,
                                              unsigned long long *ws_top, int64_t ws_cap) {
@@ -456,7 +457,15 @@ __global__ __launch_bounds__(256) void k_wide_row(const AcaJob *__restrict__ job
     PointI pr = load_point(kp, rows[rp]);
     AMax am{-1.0, INT_MAX, 0.0};
     double vn2 = 0.0;
-    const int nch = max(1, (rank + kChunk - 1) / kChunk);
+    // a thread's kWidePT columns s0 + threadIdx.x + 256 i stay in registers over the residual's terms,
+    // so every term issues kWidePT independent loads (the arithmetic per column is unchanged)
+    double rv[kWidePT];
+#pragma unroll
+    for (int i = 0; i < kWidePT; ++i) {
+        const int s = s0 + threadIdx.x + 256 * i;
+        rv[i] = s < s1 ? kentry(kp, pr, cols[s]) : 0.0;
+    }
+    const int nch = (rank + kChunk - 1) / kChunk;
     for (int ch = 0; ch < nch; ++ch) {
         const int l0 = ch * kChunk, l1 = min(rank, l0 + kChunk);
         for (int q = l0 + threadIdx.x; q < l1; q += 256) {
@@ -464,16 +473,25 @@ __global__ __launch_bounds__(256) void k_wide_row(const AcaJob *__restrict__ job
             cofs[q - l0] = tab[2 * q + 1];
         }
         __syncthreads();
-        for (int s = s0 + threadIdx.x; s < s1; s += 256) {
-            double r = ch == 0 ? kentry(kp, pr, cols[s]) : Vr[s];
-            for (int q = l0; q < l1; ++q) r -= coef[q - l0] * wsr[cofs[q - l0] + s];
-            Vr[s] = r;
-            if (ch == nch - 1) {
-                vn2 += r * r;
-                if (!cu[s]) amax_update(am, r, s);
+        for (int q = l0; q < l1; ++q) {
+            const double c = coef[q - l0];
+            const double *v = wsr + cofs[q - l0];
+#pragma unroll
+            for (int i = 0; i < kWidePT; ++i) {
+                const int s = s0 + threadIdx.x + 256 * i;
+                if (s < s1) rv[i] -= c * v[s];
             }
         }
         __syncthreads();
+    }
+#pragma unroll
+    for (int i = 0; i < kWidePT; ++i) {
+        const int s = s0 + threadIdx.x + 256 * i;
+        if (s < s1) {
+            Vr[s] = rv[i];
+            vn2 += rv[i] * rv[i];
+            if (!cu[s]) amax_update(am, rv[i], s);
+        }
     }
     double dummy = 0.0;
     block_reduce<256>(red, vn2, dummy, am);
@@ -550,7 +568,13 @@ __global__ __launch_bounds__(256) void k_wide_col(const AcaJob *__restrict__ job
     PointI pc = load_point(kp, cols[cp]);
     AMax an{-1.0, INT_MAX, 0.0};
     double un2 = 0.0;
-    const int nch = max(1, (rank + kChunk - 1) / kChunk);
+    double cv[kWidePT];  // (as in k_wide_row)
+#pragma unroll
+    for (int i = 0; i < kWidePT; ++i) {
+        const int t = t0 + threadIdx.x + 256 * i;
+        cv[i] = t < t1 ? kentry(kp, pc, rows[t]) : 0.0;
+    }
+    const int nch = (rank + kChunk - 1) / kChunk;
     for (int ch = 0; ch < nch; ++ch) {
         const int l0 = ch * kChunk, l1 = min(rank, l0 + kChunk);
         for (int q = l0 + threadIdx.x; q < l1; q += 256) {
@@ -558,19 +582,26 @@ __global__ __launch_bounds__(256) void k_wide_col(const AcaJob *__restrict__ job
             cofs[q - l0] = tab[2 * q];
         }
         __syncthreads();
-        for (int t = t0 + threadIdx.x; t < t1; t += 256) {
-            double c = ch == 0 ? kentry(kp, pc, rows[t]) : Ur[t];
-            for (int q = l0; q < l1; ++q) c -= coef[q - l0] * wsr[cofs[q - l0] + t];
-            if (ch == nch - 1) {
-                double u = c / piv;
-                Ur[t] = u;
-                un2 += u * u;
-                if (t != rp && !ru[t]) amax_update(an, c, t);
-            } else {
-                Ur[t] = c;
+        for (int q = l0; q < l1; ++q) {
+            const double c = coef[q - l0];
+            const double *v = wsr + cofs[q - l0];
+#pragma unroll
+            for (int i = 0; i < kWidePT; ++i) {
+                const int t = t0 + threadIdx.x + 256 * i;
+                if (t < t1) cv[i] -= c * v[t];
             }
         }
         __syncthreads();
+    }
+#pragma unroll
+    for (int i = 0; i < kWidePT; ++i) {
+        const int t = t0 + threadIdx.x + 256 * i;
+        if (t < t1) {
+            const double u = cv[i] / piv;
+            Ur[t] = u;
+            un2 += u * u;
+            if (t != rp && !ru[t]) amax_update(an, cv[i], t);
+        }
     }
     double dummy = 0.0;
     block_reduce<256>(red, un2, dummy, an);
@@ -655,9 +686,20 @@ __global__ __launch_bounds__(256) void k_wide_dots(const AcaJob *__restrict__ jo
     const int64_t *tab = tab_all + jb.tab;
     const double *Uq = ws + tab[2 * q], *Vq = ws + tab[2 * q + 1];
     const double *Uk = ws + tab[2 * (st.rank - 1)], *Vk = ws + tab[2 * (st.rank - 1) + 1];
-    double du = 0.0, dv = 0.0;
-    for (int t = threadIdx.x; t < jb.m; t += 256) du += Uq[t] * Uk[t];
-    for (int s = threadIdx.x; s < jb.n; s += 256) dv += Vq[s] * Vk[s];
+    // four partial sums per thread: four independent load pairs in flight
+    auto dot = [](const double *a, const double *b, int len) {
+        double p0 = 0.0, p1 = 0.0, p2 = 0.0, p3 = 0.0;
+        int t = threadIdx.x;
+        for (; t + 768 < len; t += 1024) {
+            p0 += a[t] * b[t];
+            p1 += a[t + 256] * b[t + 256];
+            p2 += a[t + 512] * b[t + 512];
+            p3 += a[t + 768] * b[t + 768];
+        }
+        for (; t < len; t += 256) p0 += a[t] * b[t];
+        return (p0 + p1) + (p2 + p3);
+    };
+    double du = dot(Uq, Uk, jb.m), dv = dot(Vq, Vk, jb.n);
     __shared__ BlockRed<256> red;
     AMax dummy{-1.0, INT_MAX, 0.0};
     block_reduce<256>(red, du, dv, dummy);
